@@ -1,0 +1,238 @@
+/*
+ * nemo_hip.h -- C ABI of libnemo_hip.so, the MI355X (gfx950) HIP simulation
+ * backend for the NeMo spiking-network simulator.
+ *
+ * The functions mirror NeMo's C API (src/nemo.h, implemented in
+ * src/nemo/nemo_c.cpp) with the same names, argument meaning and status
+ * codes, so a C or FFI caller of NeMo links against this library instead.
+ * The backend factory entry points at the end mirror the dlopen'ed CUDA
+ * backend ABI (src/nemo/cuda/create_simulation.hpp:14-22,
+ * src/nemo/cuda/devices.hpp:21-56).  Citations are relative to the NeMo
+ * v0.7.2 source tree.
+ *
+ * Error handling: every nemo_status_t function returns NEMO_OK (0) on
+ * success; otherwise nemo_strerror() describes the most recent error
+ * (src/nemo/nemo_c.cpp:20-60).  Handles are not thread-safe; one host thread
+ * drives one simulation (as in the reference).
+ *
+ * Entry points marked "extension" have no counterpart in NeMo's C API; each
+ * says which reference behaviour it generalises.
+ */
+#ifndef NEMO_HIP_H
+#define NEMO_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes: src/nemo/types.h:31-61 */
+typedef int nemo_status_t;
+#define NEMO_OK 0
+#define NEMO_CUDA_INVOCATION_ERROR 1
+#define NEMO_CUDA_ASSERTION_FAILURE 2
+#define NEMO_CUDA_MEMORY_ERROR 3
+#define NEMO_CUDA_ERROR 4
+#define NEMO_API_UNSUPPORTED 5
+#define NEMO_INVALID_INPUT 6
+#define NEMO_BUFFER_OVERFLOW 7
+#define NEMO_BUFFER_UNDERFLOW 8
+#define NEMO_LOGIC_ERROR 9
+#define NEMO_ALLOCATION_ERROR 10
+#define NEMO_MPI_ERROR 11
+#define NEMO_DL_ERROR 12
+#define NEMO_IO_ERROR 13
+#define NEMO_UNKNOWN_ERROR 14
+
+/* Backends: src/nemo/types.h:63-66, plus the HIP backend this library provides */
+typedef unsigned backend_t;
+#define NEMO_BACKEND_CUDA 0
+#define NEMO_BACKEND_CPU 1
+#define NEMO_BACKEND_HIP 2
+
+typedef uint64_t synapse_id; /* src/nemo/types.h:70: (source << 32) | index */
+
+typedef struct nemo_network_class* nemo_network_t;
+typedef struct nemo_configuration_class* nemo_configuration_t;
+typedef struct nemo_simulation_class* nemo_simulation_t;
+
+/* ---------------------------------------------------------------- library */
+
+/* src/nemo.h:56-57 */
+const char* nemo_version(void);
+/* src/nemo.h:770-790 (nemo_c.cpp:20-60) */
+const char* nemo_strerror(void);
+/* src/nemo.h:72-74: directory searched for <Model>.ini and hip/lib<Model>.so */
+nemo_status_t nemo_add_plugin_path(const char* dir);
+/* src/nemo.h:88-95 (nemo_cuda_device_count / _description), for HIP devices */
+nemo_status_t nemo_hip_device_count(unsigned* count);
+nemo_status_t nemo_hip_device_description(unsigned device, const char** descr);
+
+/* ---------------------------------------------------------- configuration */
+
+/* src/nemo.h:105-110 */
+nemo_configuration_t nemo_new_configuration(void);
+void nemo_delete_configuration(nemo_configuration_t);
+/* src/nemo.h:114-115 */
+nemo_status_t nemo_log_stdout(nemo_configuration_t);
+/* src/nemo.h:148-157 */
+nemo_status_t nemo_set_stdp_function(nemo_configuration_t,
+		float prefire_fn[], size_t prefire_len,
+		float postfire_fn[], size_t postfire_len,
+		float min_excitatory_weight, float max_excitatory_weight,
+		float min_inhibitory_weight, float max_inhibitory_weight);
+/* src/nemo.h:160-162.  Accepted, but this library only builds HIP simulations:
+ * nemo_new_simulation then fails with NEMO_API_UNSUPPORTED. */
+nemo_status_t nemo_set_cpu_backend(nemo_configuration_t);
+/* src/nemo.h:166-168 (nemo_set_cuda_backend), for the HIP backend; dev -1 = default */
+nemo_status_t nemo_set_hip_backend(nemo_configuration_t, int dev);
+/* src/nemo.h:171-173 (nemo_cuda_device), for the HIP backend */
+nemo_status_t nemo_hip_device(nemo_configuration_t, int* dev);
+/* src/nemo.h:177-185 */
+nemo_status_t nemo_backend(nemo_configuration_t, backend_t* backend);
+nemo_status_t nemo_backend_description(nemo_configuration_t, const char** descr);
+/* src/nemo.h:189-191 */
+nemo_status_t nemo_set_write_only_synapses(nemo_configuration_t);
+
+/* ------------------------------------------------------------ construction */
+
+/* src/nemo.h:215-221 */
+nemo_network_t nemo_new_network(void);
+void nemo_delete_network(nemo_network_t);
+/* src/nemo.h:235-239 */
+nemo_status_t nemo_add_neuron_type(nemo_network_t, const char* name, unsigned* neuron_type);
+/* src/nemo.h:245-251 (deprecated in the reference; kept for drop-in) */
+nemo_status_t nemo_add_neuron_iz(nemo_network_t, unsigned idx,
+		float a, float b, float c, float d, float u, float v, float sigma);
+/* src/nemo.h:265-268: args = parameters followed by state variables */
+nemo_status_t nemo_add_neuron(nemo_network_t, unsigned type, unsigned idx,
+		unsigned nargs, float args[]);
+/* src/nemo.h:278-286 */
+nemo_status_t nemo_add_synapse(nemo_network_t, unsigned source, unsigned target,
+		unsigned delay, float weight, unsigned char is_plastic, synapse_id* id);
+/* src/nemo.h:290-292 */
+nemo_status_t nemo_neuron_count(nemo_network_t, unsigned* ncount);
+
+/* Extension: n neurons at once, args is n x nargs row-major.  Same effect as
+ * n calls of nemo_add_neuron (the Python binding's vector form,
+ * src/api/python/nemo_py.cpp add_neuron overloads). */
+nemo_status_t nemo_add_neurons(nemo_network_t, unsigned type, size_t n,
+		const unsigned idx[], unsigned nargs, const float args[]);
+/* Extension: n synapses at once; plastic and ids may be NULL.  Same effect as
+ * n calls of nemo_add_synapse (src/api/python/nemo_py.cpp add_synapse vector form). */
+nemo_status_t nemo_add_synapses(nemo_network_t, size_t n,
+		const unsigned source[], const unsigned target[], const unsigned delay[],
+		const float weight[], const unsigned char plastic[], synapse_id ids[]);
+
+/* src/nemo.h:399-420 (network getters) and 613-680 (network setters) */
+nemo_status_t nemo_get_neuron_state_n(nemo_network_t, unsigned neuron, unsigned var, float* val);
+nemo_status_t nemo_get_neuron_parameter_n(nemo_network_t, unsigned neuron, unsigned param, float* val);
+nemo_status_t nemo_set_neuron_n(nemo_network_t, unsigned idx, unsigned nargs, float args[]);
+nemo_status_t nemo_set_neuron_state_n(nemo_network_t, unsigned neuron, unsigned var, float val);
+nemo_status_t nemo_set_neuron_parameter_n(nemo_network_t, unsigned neuron, unsigned param, float val);
+/* src/nemo.h:467-567 (network synapse getters) */
+nemo_status_t nemo_get_synapse_source_n(nemo_network_t, synapse_id, unsigned* source);
+nemo_status_t nemo_get_synapse_target_n(nemo_network_t, synapse_id, unsigned* target);
+nemo_status_t nemo_get_synapse_delay_n(nemo_network_t, synapse_id, unsigned* delay);
+nemo_status_t nemo_get_synapse_weight_n(nemo_network_t, synapse_id, float* weight);
+nemo_status_t nemo_get_synapse_plastic_n(nemo_network_t, synapse_id, unsigned char* plastic);
+nemo_status_t nemo_get_synapses_from_n(nemo_network_t, unsigned source, synapse_id* synapses[], size_t* len);
+
+/* -------------------------------------------------------------- simulation */
+
+/* src/nemo.h:311-317; the simulation is built by the backend the
+ * configuration selects (src/nemo/nemo.cpp:109-147) */
+nemo_simulation_t nemo_new_simulation(nemo_network_t, nemo_configuration_t);
+void nemo_delete_simulation(nemo_simulation_t);
+/* src/nemo.h:347-352.  fired is borrowed until the next step. */
+nemo_status_t nemo_step(nemo_simulation_t,
+		unsigned fstim_nidx[], size_t fstim_count,
+		unsigned istim_nidx[], float istim_current[], size_t istim_count,
+		unsigned* fired[], size_t* fired_count);
+/* src/nemo.h:356-358 */
+nemo_status_t nemo_apply_stdp(nemo_simulation_t, float reward);
+/* src/nemo.h:380-382 and 437-458 */
+nemo_status_t nemo_get_membrane_potential(nemo_simulation_t, unsigned neuron, float* v);
+nemo_status_t nemo_get_neuron_state_s(nemo_simulation_t, unsigned neuron, unsigned var, float* val);
+nemo_status_t nemo_get_neuron_parameter_s(nemo_simulation_t, unsigned neuron, unsigned param, float* val);
+/* src/nemo.h:625-732 (simulation setters) */
+nemo_status_t nemo_set_neuron_s(nemo_simulation_t, unsigned idx, unsigned nargs, float args[]);
+nemo_status_t nemo_set_neuron_state_s(nemo_simulation_t, unsigned neuron, unsigned var, float val);
+nemo_status_t nemo_set_neuron_parameter_s(nemo_simulation_t, unsigned neuron, unsigned param, float val);
+/* src/nemo.h:478-581 (simulation synapse getters) */
+nemo_status_t nemo_get_synapse_source_s(nemo_simulation_t, synapse_id, unsigned* source);
+nemo_status_t nemo_get_synapse_target_s(nemo_simulation_t, synapse_id, unsigned* target);
+nemo_status_t nemo_get_synapse_delay_s(nemo_simulation_t, synapse_id, unsigned* delay);
+nemo_status_t nemo_get_synapse_weight_s(nemo_simulation_t, synapse_id, float* weight);
+nemo_status_t nemo_get_synapse_plastic_s(nemo_simulation_t, synapse_id, unsigned char* plastic);
+nemo_status_t nemo_get_synapses_from_s(nemo_simulation_t, unsigned source, synapse_id* synapses[], size_t* len);
+/* src/nemo.h:753-763 */
+nemo_status_t nemo_elapsed_wallclock(nemo_simulation_t, unsigned long* ms);
+nemo_status_t nemo_elapsed_simulation(nemo_simulation_t, unsigned long* ms);
+nemo_status_t nemo_reset_timer(nemo_simulation_t);
+
+/* Extension: run nsteps steps without stimulus and without a host round trip
+ * per step (steps are captured into HIP graphs).  Equivalent to nsteps calls
+ * of nemo_step with no stimulus (src/nemo/SimulationBackend.cpp:22-30) plus
+ * nemo_apply_stdp(reward) after every stdp_period steps when stdp_period > 0
+ * (the reference benchmark loop, src/examples/common.cpp:72-90).  The firing
+ * of every step is returned: counts[s] neurons fired in step s, and their
+ * indices, each step in ascending simulation order, are concatenated in
+ * *fired (borrowed until the next call).  counts may be NULL to skip the
+ * raster readout; *events receives the synaptic events delivered. */
+nemo_status_t nemo_hip_run(nemo_simulation_t, unsigned nsteps,
+		unsigned stdp_period, float reward,
+		unsigned counts[], unsigned* fired[], size_t* fired_len, uint64_t* events);
+
+/* Extension (parity inspection): float currents (E then I) delivered to each
+ * neuron in the most recent step, in simulation order.  The reference CPU
+ * backend holds these in m_currentE / m_currentI (cpu/Simulation.cpp:279-288).
+ * Requires nemo_hip_set_debug(sim, 1) before the step. */
+nemo_status_t nemo_hip_set_debug(nemo_simulation_t, int enable);
+nemo_status_t nemo_hip_get_currents(nemo_simulation_t, float E[], float I[], size_t n);
+/* Extension (parity inspection): simulation-order -> user index map */
+nemo_status_t nemo_hip_local_to_global(nemo_simulation_t, unsigned gidx[], size_t n);
+/* Extension (parity inspection): 64-bit firing history, simulation order
+ * (cpu/Simulation.hpp m_recentFiring) */
+nemo_status_t nemo_hip_get_recent_firing(nemo_simulation_t, uint64_t hist[], size_t n);
+/* Extension (parity inspection): RNG state, 4 words per neuron, simulation order */
+nemo_status_t nemo_hip_get_rng_state(nemo_simulation_t, uint32_t state[], size_t n);
+/* Extension (parity inspection): Q11.20 weight and STDP accumulator of every
+ * synapse in network order (source ascending, then insertion order) */
+nemo_status_t nemo_hip_get_fixed_weights(nemo_simulation_t, int32_t w[], size_t n);
+nemo_status_t nemo_hip_get_stdp_accumulators(nemo_simulation_t, int32_t acc[], size_t n);
+/* Extension: synapse and neuron counts of a built simulation */
+nemo_status_t nemo_hip_counts(nemo_simulation_t, size_t* neurons, size_t* synapses, size_t* plastic);
+
+/* Extension (measurement): run nsteps steps (no stimulus) with HIP events at
+ * every phase boundary on the simulation's stream.  avg_us receives the mean
+ * duration of delivery, neuron update (all models), STDP accumulation and the
+ * clock advance; counters the events delivered, neurons fired, plastic
+ * synapses examined and steps run.  The reference's Timer
+ * (src/nemo/Timer.hpp:23-79) only measures whole steps. */
+nemo_status_t nemo_hip_profile(nemo_simulation_t, unsigned nsteps, double avg_us[4], uint64_t counters[4]);
+
+/* Extension (diagnostics): evaluate the device's glibc-exact logf (fn 0),
+ * sinf (fn 1) or expf (fn 2) on n host inputs, results to host out.  The
+ * reference calls the host C library for these (RNG.cpp:28-32, HH.cpp:92-98). */
+nemo_status_t nemo_hip_eval_math(int fn, const float in[], float out[], size_t n);
+
+/* ------------------------------------------------- backend factory (dlopen) */
+
+/* src/nemo/cuda/create_simulation.hpp:14-22 (cuda_simulation): build a HIP
+ * simulation of net under conf.  Returns NULL and sets nemo_strerror() on failure. */
+nemo_simulation_t hip_simulation(nemo_network_t net, nemo_configuration_t conf);
+/* src/nemo/cuda/devices.hpp:21-56 (cuda_set_configuration, cuda_device_count,
+ * cuda_device_description, cuda_test_device) */
+nemo_status_t hip_set_configuration(nemo_configuration_t conf, int dev);
+unsigned hip_device_count(void);
+const char* hip_device_description(unsigned device);
+nemo_status_t hip_test_device(unsigned device);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,382 @@
+"""nemo_amd -- Python interface of the MI355X HIP backend for NeMo.
+
+Mirrors NeMo's Python module (src/api/python/wrappers.py:27-331 over
+nemo_py.cpp): Configuration, Network and Simulation with the same method
+names and argument meaning, plus set_hip_backend (the device selection the
+HIP backend adds) and Simulation.run (many steps per host round trip).
+Calls go through the C ABI of libnemo_hip.so (include/nemo_hip.h).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from ._lib import NemoError, check, lib
+
+__all__ = ["Configuration", "Network", "Simulation", "NemoError", "version", "hip_device_count",
+           "hip_device_description", "add_plugin_path", "plugin_dir"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def plugin_dir():
+    """Directory holding <Model>.ini and hip/lib<Model>.so."""
+    return os.path.join(_HERE, "plugins")
+
+
+def version():
+    return lib().nemo_version().decode()
+
+
+def add_plugin_path(path):
+    check(lib().nemo_add_plugin_path(path.encode()))
+
+
+def hip_device_count():
+    n = ctypes.c_uint(0)
+    check(lib().nemo_hip_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def hip_device_description(dev):
+    s = ctypes.c_char_p()
+    check(lib().nemo_hip_device_description(dev, ctypes.byref(s)))
+    return s.value.decode()
+
+
+def _f32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+
+
+def _u32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.uint32))
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(ctypes.POINTER(t))
+
+
+def _ini_names(model):
+    """Parameter and state names of a model from its .ini file."""
+    path = os.path.join(plugin_dir(), model + ".ini")
+    params, state, section = {}, {}, ""
+    if not os.path.exists(path):
+        return params, state
+    with open(path) as f:
+        for line in f:
+            line = line.split("#", 1)[0].strip()
+            if not line:
+                continue
+            if line.startswith("[") and line.endswith("]"):
+                section = line[1:-1]
+                continue
+            if "=" in line and section in ("param_names", "state_names"):
+                k, v = (x.strip() for x in line.split("=", 1))
+                (params if section == "param_names" else state)[k] = int(v)
+    return params, state
+
+
+class Configuration:
+    """Simulation configuration (src/nemo/Configuration.hpp:70-195)."""
+
+    def __init__(self):
+        self._conf = lib().nemo_new_configuration()
+
+    def __del__(self):
+        if getattr(self, "_conf", None):
+            lib().nemo_delete_configuration(self._conf)
+            self._conf = None
+
+    def set_stdp_function(self, prefire, postfire, minWeight, maxWeight, minInhibitory=None, maxInhibitory=None):
+        """STDP function sampled per ms.  With four weights the arguments are
+        (minE, maxE, minI, maxI); with two they are (minWeight, maxWeight) as in
+        Configuration::setStdpFunction (Configuration.cpp:148-151): excitatory
+        [0, maxWeight], inhibitory [minWeight, 0]."""
+        if minInhibitory is None:
+            minE, maxE, minI, maxI = 0.0, maxWeight, 0.0, minWeight
+        else:
+            minE, maxE, minI, maxI = minWeight, maxWeight, minInhibitory, maxInhibitory
+        pre, post = _f32(prefire), _f32(postfire)
+        check(lib().nemo_set_stdp_function(self._conf, _ptr(pre, ctypes.c_float), pre.size,
+                                           _ptr(post, ctypes.c_float), post.size, minE, maxE, minI, maxI))
+
+    def set_hip_backend(self, device=-1):
+        check(lib().nemo_set_hip_backend(self._conf, device))
+
+    def set_cuda_backend(self, device=-1):
+        """The GPU backend choice of the reference; selects the HIP device."""
+        self.set_hip_backend(device)
+
+    def set_cpu_backend(self):
+        check(lib().nemo_set_cpu_backend(self._conf))
+
+    def backend_description(self):
+        s = ctypes.c_char_p()
+        check(lib().nemo_backend_description(self._conf, ctypes.byref(s)))
+        return s.value.decode()
+
+    def set_write_only_synapses(self):
+        check(lib().nemo_set_write_only_synapses(self._conf))
+
+    def log_stdout(self):
+        check(lib().nemo_log_stdout(self._conf))
+
+
+class _SynapseQueries:
+    """Synapse getters shared by Network (_n) and Simulation (_s)."""
+
+    def get_synapses_from(self, source):
+        ptr = ctypes.POINTER(ctypes.c_uint64)()
+        n = ctypes.c_size_t(0)
+        check(getattr(lib(), "nemo_get_synapses_from_" + self._sfx)(self._h, source, ctypes.byref(ptr), ctypes.byref(n)))
+        return [ptr[i] for i in range(n.value)]
+
+    def _get(self, what, ctype, ids):
+        f = getattr(lib(), "nemo_get_synapse_%s_%s" % (what, self._sfx))
+        scalar = np.isscalar(ids)
+        out = []
+        for i in ([ids] if scalar else ids):
+            v = ctype()
+            check(f(self._h, int(i), ctypes.byref(v)))
+            out.append(v.value)
+        return out[0] if scalar else out
+
+    def get_synapse_source(self, ids):
+        return self._get("source", ctypes.c_uint, ids)
+
+    def get_synapse_target(self, ids):
+        return self._get("target", ctypes.c_uint, ids)
+
+    def get_synapse_delay(self, ids):
+        return self._get("delay", ctypes.c_uint, ids)
+
+    def get_synapse_weight(self, ids):
+        return self._get("weight", ctypes.c_float, ids)
+
+    def get_synapse_plastic(self, ids):
+        return self._get("plastic", ctypes.c_ubyte, ids)
+
+
+class Network(_SynapseQueries):
+    """Network under construction (src/nemo/Network.hpp:45-231)."""
+
+    _sfx = "n"
+
+    def __init__(self):
+        self._h = lib().nemo_new_network()
+        self._types = {}
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().nemo_delete_network(self._h)
+            self._h = None
+
+    def add_neuron_type(self, name):
+        t = ctypes.c_uint(0)
+        check(lib().nemo_add_neuron_type(self._h, name.encode(), ctypes.byref(t)))
+        self._types[t.value] = name
+        return t.value
+
+    def add_neuron(self, neuron_type, neuron_idx, *args, paramDict=None, stateDict=None):
+        """Add one neuron or a vector of neurons.  args are the model's
+        parameters followed by its state variables, each a scalar or a sequence
+        of the same length as neuron_idx; alternatively name them in paramDict
+        and stateDict (wrappers.py:72-155)."""
+        if paramDict is not None or stateDict is not None:
+            pnames, snames = _ini_names(self._types[neuron_type])
+            paramDict, stateDict = paramDict or {}, stateDict or {}
+            ordered = [None] * (len(pnames) + len(snames))
+            for k, i in pnames.items():
+                ordered[i] = paramDict[k]
+            for k, i in snames.items():
+                ordered[len(pnames) + i] = stateDict[k]
+            args = tuple(ordered)
+        idx = np.atleast_1d(_u32(neuron_idx))
+        n = idx.size
+        cols = [np.broadcast_to(np.asarray(a, dtype=np.float32), (n,)) for a in args]
+        mat = np.ascontiguousarray(np.stack(cols, axis=1) if cols else np.zeros((n, 0), np.float32))
+        check(lib().nemo_add_neurons(self._h, neuron_type, n, _ptr(idx, ctypes.c_uint), len(cols),
+                                     _ptr(mat, ctypes.c_float)))
+
+    def add_synapse(self, source, target, delay, weight, plastic):
+        """Add one synapse (returns its id) or vectors of synapses (returns a
+        numpy array of ids); scalars broadcast against vectors."""
+        scalar = all(np.isscalar(x) for x in (source, target, delay, weight, plastic))
+        n = max(np.size(x) for x in (source, target, delay, weight, plastic))
+        s = np.ascontiguousarray(np.broadcast_to(_u32(source), (n,)))
+        t = np.ascontiguousarray(np.broadcast_to(_u32(target), (n,)))
+        d = np.ascontiguousarray(np.broadcast_to(_u32(delay), (n,)))
+        w = np.ascontiguousarray(np.broadcast_to(_f32(weight), (n,)))
+        p = np.ascontiguousarray(np.broadcast_to(np.asarray(plastic, dtype=np.uint8), (n,)))
+        ids = np.empty(n, dtype=np.uint64)
+        check(lib().nemo_add_synapses(self._h, n, _ptr(s, ctypes.c_uint), _ptr(t, ctypes.c_uint),
+                                      _ptr(d, ctypes.c_uint), _ptr(w, ctypes.c_float),
+                                      _ptr(p, ctypes.c_ubyte), _ptr(ids, ctypes.c_uint64)))
+        return int(ids[0]) if scalar else ids
+
+    def neuron_count(self):
+        n = ctypes.c_uint(0)
+        check(lib().nemo_neuron_count(self._h, ctypes.byref(n)))
+        return n.value
+
+    def get_neuron_state(self, idx, var):
+        v = ctypes.c_float()
+        check(lib().nemo_get_neuron_state_n(self._h, idx, var, ctypes.byref(v)))
+        return v.value
+
+    def get_neuron_parameter(self, idx, param):
+        v = ctypes.c_float()
+        check(lib().nemo_get_neuron_parameter_n(self._h, idx, param, ctypes.byref(v)))
+        return v.value
+
+    def set_neuron(self, idx, *args):
+        a = _f32(args)
+        check(lib().nemo_set_neuron_n(self._h, idx, a.size, _ptr(a, ctypes.c_float)))
+
+    def set_neuron_state(self, idx, var, val):
+        check(lib().nemo_set_neuron_state_n(self._h, idx, var, val))
+
+    def set_neuron_parameter(self, idx, param, val):
+        check(lib().nemo_set_neuron_parameter_n(self._h, idx, param, val))
+
+
+class Simulation(_SynapseQueries):
+    """A HIP simulation of a network (src/nemo/Simulation.hpp:38-218)."""
+
+    _sfx = "s"
+
+    def __init__(self, net, conf):
+        h = lib().nemo_new_simulation(net._h, conf._conf)
+        if not h:
+            raise NemoError(-1, lib().nemo_strerror().decode())
+        self._h = h
+        n = ctypes.c_size_t()
+        s = ctypes.c_size_t()
+        p = ctypes.c_size_t()
+        check(lib().nemo_hip_counts(self._h, ctypes.byref(n), ctypes.byref(s), ctypes.byref(p)))
+        self.neuron_count, self.synapse_count, self.plastic_count = n.value, s.value, p.value
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().nemo_delete_simulation(self._h)
+            self._h = None
+
+    def step(self, fstim=None, istim=None):
+        """One 1 ms step; fstim: neurons forced to fire, istim: list of
+        (neuron, current) pairs.  Returns the fired neurons in simulation order
+        (wrappers.py:252-276)."""
+        f = _u32(fstim if fstim is not None else [])
+        if istim:
+            ii = _u32([x[0] for x in istim])
+            iv = _f32([x[1] for x in istim])
+        else:
+            ii, iv = _u32([]), _f32([])
+        out = ctypes.POINTER(ctypes.c_uint)()
+        n = ctypes.c_size_t(0)
+        check(lib().nemo_step(self._h, _ptr(f, ctypes.c_uint), f.size, _ptr(ii, ctypes.c_uint),
+                              _ptr(iv, ctypes.c_float), ii.size, ctypes.byref(out), ctypes.byref(n)))
+        return [out[i] for i in range(n.value)]
+
+    def run(self, nsteps, stdp_period=0, reward=1.0, raster=True):
+        """Run nsteps steps on the device (no stimulus), applying STDP with the
+        given reward every stdp_period steps.  Returns (counts, fired, events):
+        per-step fired counts, the concatenated fired neurons (each step in
+        simulation order) and the synaptic events delivered."""
+        counts = np.zeros(nsteps, dtype=np.uint32) if raster else None
+        out = ctypes.POINTER(ctypes.c_uint)()
+        n = ctypes.c_size_t(0)
+        ev = ctypes.c_uint64(0)
+        check(lib().nemo_hip_run(self._h, nsteps, stdp_period, reward,
+                                 _ptr(counts, ctypes.c_uint) if raster else None,
+                                 ctypes.byref(out), ctypes.byref(n), ctypes.byref(ev)))
+        if not raster:
+            return None, None, ev.value
+        fired = np.ctypeslib.as_array(out, shape=(n.value,)).copy() if n.value else np.zeros(0, np.uint32)
+        return counts, fired, ev.value
+
+    def apply_stdp(self, reward):
+        check(lib().nemo_apply_stdp(self._h, reward))
+
+    def get_membrane_potential(self, idx):
+        v = ctypes.c_float()
+        check(lib().nemo_get_membrane_potential(self._h, idx, ctypes.byref(v)))
+        return v.value
+
+    def get_neuron_state(self, idx, var):
+        v = ctypes.c_float()
+        check(lib().nemo_get_neuron_state_s(self._h, idx, var, ctypes.byref(v)))
+        return v.value
+
+    def get_neuron_parameter(self, idx, param):
+        v = ctypes.c_float()
+        check(lib().nemo_get_neuron_parameter_s(self._h, idx, param, ctypes.byref(v)))
+        return v.value
+
+    def set_neuron(self, idx, *args):
+        a = _f32(args)
+        check(lib().nemo_set_neuron_s(self._h, idx, a.size, _ptr(a, ctypes.c_float)))
+
+    def set_neuron_state(self, idx, var, val):
+        check(lib().nemo_set_neuron_state_s(self._h, idx, var, val))
+
+    def set_neuron_parameter(self, idx, param, val):
+        check(lib().nemo_set_neuron_parameter_s(self._h, idx, param, val))
+
+    def elapsed_wallclock(self):
+        v = ctypes.c_ulong()
+        check(lib().nemo_elapsed_wallclock(self._h, ctypes.byref(v)))
+        return v.value
+
+    def elapsed_simulation(self):
+        v = ctypes.c_ulong()
+        check(lib().nemo_elapsed_simulation(self._h, ctypes.byref(v)))
+        return v.value
+
+    def reset_timer(self):
+        check(lib().nemo_reset_timer(self._h))
+
+    def profile(self, nsteps):
+        """Per-phase mean kernel time (us) over nsteps eager steps, measured
+        with HIP events on the simulation's stream, and the work done."""
+        avg = (ctypes.c_double * 4)()
+        cnt = (ctypes.c_uint64 * 4)()
+        check(lib().nemo_hip_profile(self._h, nsteps, avg, cnt))
+        keys = ("deliver", "update", "stdp", "advance")
+        return ({k: avg[i] for i, k in enumerate(keys)},
+                {"events": cnt[0], "fired": cnt[1], "stdp_synapses": cnt[2], "steps": cnt[3]})
+
+    # ---- parity inspection (extensions) ----
+    def set_debug(self, on=True):
+        check(lib().nemo_hip_set_debug(self._h, 1 if on else 0))
+
+    def currents(self):
+        n = self.neuron_count
+        e = np.zeros(n, np.float32)
+        i = np.zeros(n, np.float32)
+        check(lib().nemo_hip_get_currents(self._h, _ptr(e, ctypes.c_float), _ptr(i, ctypes.c_float), n))
+        return e, i
+
+    def local_to_global(self):
+        g = np.zeros(self.neuron_count, np.uint32)
+        check(lib().nemo_hip_local_to_global(self._h, _ptr(g, ctypes.c_uint), g.size))
+        return g
+
+    def recent_firing(self):
+        h = np.zeros(self.neuron_count, np.uint64)
+        check(lib().nemo_hip_get_recent_firing(self._h, _ptr(h, ctypes.c_uint64), h.size))
+        return h
+
+    def rng_state(self):
+        s = np.zeros(4 * self.neuron_count, np.uint32)
+        check(lib().nemo_hip_get_rng_state(self._h, _ptr(s, ctypes.c_uint32), self.neuron_count))
+        return s.reshape(-1, 4)
+
+    def fixed_weights(self):
+        w = np.zeros(self.synapse_count, np.int32)
+        check(lib().nemo_hip_get_fixed_weights(self._h, _ptr(w, ctypes.c_int32), w.size))
+        return w
+
+    def stdp_accumulators(self):
+        a = np.zeros(self.synapse_count, np.int32)
+        check(lib().nemo_hip_get_stdp_accumulators(self._h, _ptr(a, ctypes.c_int32), a.size))
+        return a

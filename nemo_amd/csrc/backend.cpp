@@ -1,0 +1,1339 @@
+/*
+ * Host side of the HIP backend; see backend.hpp for the interfaces mirrored.
+ */
+#include "backend.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <random>
+#include <sstream>
+
+#include <dlfcn.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace nemo_hip {
+
+void hipCheck(hipError_t err, const char* what)
+{
+	if(err != hipSuccess) {
+		int code = (err == hipErrorOutOfMemory) ? NEMO_CUDA_MEMORY_ERROR : NEMO_CUDA_INVOCATION_ERROR;
+		throw Error(code, std::string("HIP error in ") + what + ": " + hipGetErrorString(err));
+	}
+}
+
+template<typename T>
+void DeviceArray<T>::allocate(size_t count)
+{
+	release();
+	n = count;
+	if(count) {
+		HIP_CHECK(hipMalloc(reinterpret_cast<void**>(&ptr), count * sizeof(T)));
+	}
+}
+
+template<typename T>
+void DeviceArray<T>::release()
+{
+	if(ptr) {
+		(void) hipFree(ptr);
+		ptr = nullptr;
+	}
+	n = 0;
+}
+
+template struct DeviceArray<float>;
+template struct DeviceArray<uint32_t>;
+template struct DeviceArray<int32_t>;
+template struct DeviceArray<int64_t>;
+template struct DeviceArray<uint64_t>;
+template struct DeviceArray<int2>;
+template struct DeviceArray<nemo_hip_clock>;
+
+/* ----------------------------------------------------- fixed point (Q11.20) */
+
+/* fx_toFix, src/nemo/fixedpoint.cpp:17-28 */
+static int32_t fxToFix(float f, unsigned fbits)
+{
+	if(std::abs(int(f)) >= (1 << (31 - fbits))) {
+		std::ostringstream msg;
+		msg << "Fixed-point overflow. Value " << f << " does not fit into fixed-point format Q"
+			<< 31 - fbits << "." << fbits;
+		throw Error(NEMO_INVALID_INPUT, msg.str());
+	}
+	return static_cast<int32_t>(f * float(1 << fbits));
+}
+
+static float fxToFloat(int32_t v, unsigned fbits)
+{
+	return float(v) / float(1 << fbits);
+}
+
+/* ------------------------------------------------------------ plugin paths */
+
+static std::vector<std::string>& userPluginPaths()
+{
+	static std::vector<std::string> paths;
+	return paths;
+}
+
+void addPluginPath(const std::string& dir)
+{
+	userPluginPaths().push_back(dir);
+}
+
+static std::string libraryDirectory()
+{
+	Dl_info info;
+	if(dladdr(reinterpret_cast<void*>(&libraryDirectory), &info) && info.dli_fname) {
+		std::string p(info.dli_fname);
+		size_t slash = p.rfind('/');
+		return slash == std::string::npos ? std::string(".") : p.substr(0, slash);
+	}
+	return ".";
+}
+
+/* Search order: directories added at run time, then NEMO_PLUGIN_PATH, then the
+ * plugins directory installed beside the library (NeuronType.cpp:47-72). */
+std::vector<std::string> pluginPaths()
+{
+	std::vector<std::string> paths = userPluginPaths();
+	if(const char* env = std::getenv("NEMO_PLUGIN_PATH")) {
+		std::stringstream ss(env);
+		std::string item;
+		while(std::getline(ss, item, ':')) {
+			if(!item.empty()) paths.push_back(item);
+		}
+	}
+	paths.push_back(libraryDirectory() + "/../plugins");
+	return paths;
+}
+
+static bool fileExists(const std::string& p)
+{
+	std::ifstream f(p);
+	return f.good();
+}
+
+static std::string trim(const std::string& s)
+{
+	size_t a = s.find_first_not_of(" \t\r\n");
+	if(a == std::string::npos) return "";
+	size_t b = s.find_last_not_of(" \t\r\n");
+	return s.substr(a, b - a + 1);
+}
+
+static bool parseBool(const std::string& v)
+{
+	return v == "true" || v == "1" || v == "yes";
+}
+
+/* <Model>.ini: key=value lines, [section] headers, '#' comments
+ * (NeuronType.cpp:93-158) */
+NeuronType loadNeuronType(const std::string& name)
+{
+	for(const std::string& dir : pluginPaths()) {
+		std::string path = dir + "/" + name + ".ini";
+		if(!fileExists(path)) continue;
+		std::ifstream in(path);
+		NeuronType t;
+		t.name = name;
+		t.pluginDir = dir;
+		std::string section, line;
+		bool hip = false;
+		bool haveParams = false, haveState = false;
+		while(std::getline(in, line)) {
+			size_t hash = line.find('#');
+			if(hash != std::string::npos) line = line.substr(0, hash);
+			line = trim(line);
+			if(line.empty()) continue;
+			if(line.front() == '[' && line.back() == ']') {
+				section = line.substr(1, line.size() - 2);
+				continue;
+			}
+			size_t eq = line.find('=');
+			if(eq == std::string::npos) continue;
+			std::string key = trim(line.substr(0, eq));
+			std::string val = trim(line.substr(eq + 1));
+			if(section.empty()) {
+				if(key == "parameters") { t.nParam = unsigned(std::stoul(val)); haveParams = true; }
+				else if(key == "state-variables") { t.nState = unsigned(std::stoul(val)); haveState = true; }
+				else if(key == "membrane-potential") t.membranePotential = unsigned(std::stoul(val));
+				else if(key == "history") t.history = unsigned(std::stoul(val));
+			} else if(section == "rng") {
+				if(key == "normal") t.rngNormal = parseBool(val);
+			} else if(section == "rcm") {
+				if(parseBool(val)) t.usesRcm = true;
+			} else if(section == "backends") {
+				if(key == "hip") hip = parseBool(val);
+			} else if(section == "param_names") {
+				t.paramNames[key] = unsigned(std::stoul(val));
+			} else if(section == "state_names") {
+				t.stateNames[key] = unsigned(std::stoul(val));
+			}
+		}
+		if(!haveParams || !haveState) {
+			throw Error(NEMO_INVALID_INPUT, "neuron type description " + path + " lacks parameter or state counts");
+		}
+		if(!hip) {
+			throw Error(NEMO_API_UNSUPPORTED, "neuron type " + name + " has no HIP backend");
+		}
+		if(t.history != 1) {
+			throw Error(NEMO_API_UNSUPPORTED, "neuron type " + name + ": only state history length 1 is supported");
+		}
+		return t;
+	}
+	throw Error(NEMO_INVALID_INPUT, "could not find neuron type description " + name + ".ini in the plugin path");
+}
+
+Plugin::Plugin(const NeuronType& type)
+{
+	std::string path = type.pluginDir + "/hip/lib" + type.name + ".so";
+	handle = dlopen(path.c_str(), RTLD_NOW | RTLD_LOCAL);
+	if(!handle) {
+		throw Error(NEMO_DL_ERROR, std::string("failed to load neuron model plugin ") + path + ": " + dlerror());
+	}
+	update = reinterpret_cast<hip_update_neurons_t*>(dlsym(handle, "hip_update_neurons"));
+	init = reinterpret_cast<hip_init_neurons_t*>(dlsym(handle, "hip_init_neurons"));
+	if(!update || !init) {
+		dlclose(handle);
+		handle = nullptr;
+		throw Error(NEMO_DL_ERROR, "plugin " + path + " does not export hip_update_neurons/hip_init_neurons");
+	}
+}
+
+Plugin::~Plugin()
+{
+	/* Plugins stay mapped: their kernels' code objects are registered with the
+	 * HIP runtime for the life of the process. */
+}
+
+/* ---------------------------------------------------------------- network */
+
+unsigned Network::addNeuronType(const std::string& name)
+{
+	auto it = typeIds.find(name);
+	if(it != typeIds.end()) return it->second;
+	NeuronType t = loadNeuronType(name);
+	unsigned id = unsigned(types.size());
+	types.push_back(t);
+	Group g;
+	g.param.resize(t.nParam);
+	g.state.resize(t.nState);
+	groups.push_back(std::move(g));
+	typeIds[name] = id;
+	return id;
+}
+
+void Network::addNeurons(unsigned type, size_t n, const unsigned* idx, unsigned nargs, const float* args)
+{
+	if(type >= types.size()) {
+		throw Error(NEMO_INVALID_INPUT, "Invalid neuron type id " + std::to_string(type));
+	}
+	const NeuronType& t = types[type];
+	if(nargs != t.nParam + t.nState) {
+		std::ostringstream msg;
+		msg << "Unexpected number of parameters/state variables when adding neuron. Expected "
+			<< (t.nParam + t.nState) << ", found " << nargs;
+		throw Error(NEMO_INVALID_INPUT, msg.str());
+	}
+	Group& g = groups[type];
+	for(size_t i = 0; i < n; ++i) {
+		if(!mapper.emplace(idx[i], std::make_pair(type, unsigned(g.gidx.size()))).second) {
+			throw Error(NEMO_INVALID_INPUT, "Duplicate neuron index " + std::to_string(idx[i]));
+		}
+		const float* a = args + i * nargs;
+		for(unsigned p = 0; p < t.nParam; ++p) g.param[p].push_back(*a++);
+		for(unsigned s = 0; s < t.nState; ++s) g.state[s].push_back(*a++);
+		g.gidx.push_back(idx[i]);
+	}
+}
+
+void Network::addSynapses(size_t n, const unsigned* src, const unsigned* tgt, const unsigned* delay,
+		const float* weight, const unsigned char* plastic, synapse_id* ids)
+{
+	for(size_t i = 0; i < n; ++i) {
+		if(delay[i] < 1) {
+			std::ostringstream msg;
+			msg << "Invalid delay (" << delay[i] << ") for synapse between " << src[i] << " and " << tgt[i];
+			throw Error(NEMO_INVALID_INPUT, msg.str());
+		}
+		if(delay[i] > 255) {
+			throw Error(NEMO_INVALID_INPUT, "Delay " + std::to_string(delay[i]) + " exceeds 255 ms");
+		}
+	}
+	Axon* last = nullptr;
+	unsigned lastSrc = 0;
+	for(size_t i = 0; i < n; ++i) {
+		if(!last || src[i] != lastSrc) {
+			last = &fcm[src[i]];
+			lastSrc = src[i];
+		}
+		Axon& ax = *last;
+		if(ids) ids[i] = (synapse_id(src[i]) << 32) | synapse_id(ax.target.size());
+		ax.target.push_back(tgt[i]);
+		ax.delay.push_back(uint8_t(delay[i]));
+		ax.weight.push_back(weight[i]);
+		ax.plastic.push_back(plastic ? (plastic[i] != 0) : 0);
+		maxDelay = std::max(maxDelay, delay[i]);
+	}
+	synapseCount += n;
+}
+
+unsigned Network::neuronCount() const
+{
+	size_t c = 0;
+	for(const Group& g : groups) c += g.gidx.size();
+	return unsigned(c);
+}
+
+std::pair<unsigned, unsigned> Network::address(unsigned idx) const
+{
+	auto it = mapper.find(idx);
+	if(it == mapper.end()) {
+		throw Error(NEMO_INVALID_INPUT, "Non-existing neuron index " + std::to_string(idx));
+	}
+	return it->second;
+}
+
+float Network::getNeuronState(unsigned idx, unsigned var) const
+{
+	auto a = address(idx);
+	if(var >= types[a.first].nState) throw Error(NEMO_INVALID_INPUT, "Invalid state variable index " + std::to_string(var));
+	return groups[a.first].state[var][a.second];
+}
+
+float Network::getNeuronParameter(unsigned idx, unsigned param) const
+{
+	auto a = address(idx);
+	if(param >= types[a.first].nParam) throw Error(NEMO_INVALID_INPUT, "Invalid parameter index " + std::to_string(param));
+	return groups[a.first].param[param][a.second];
+}
+
+void Network::setNeuron(unsigned idx, unsigned nargs, const float* args)
+{
+	auto a = address(idx);
+	const NeuronType& t = types[a.first];
+	if(nargs != t.nParam + t.nState) {
+		throw Error(NEMO_INVALID_INPUT, "Unexpected number of parameters/state variables when modifying neuron");
+	}
+	for(unsigned p = 0; p < t.nParam; ++p) groups[a.first].param[p][a.second] = *args++;
+	for(unsigned s = 0; s < t.nState; ++s) groups[a.first].state[s][a.second] = *args++;
+}
+
+void Network::setNeuronState(unsigned idx, unsigned var, float val)
+{
+	auto a = address(idx);
+	if(var >= types[a.first].nState) throw Error(NEMO_INVALID_INPUT, "Invalid state variable index " + std::to_string(var));
+	groups[a.first].state[var][a.second] = val;
+}
+
+void Network::setNeuronParameter(unsigned idx, unsigned param, float val)
+{
+	auto a = address(idx);
+	if(param >= types[a.first].nParam) throw Error(NEMO_INVALID_INPUT, "Invalid parameter index " + std::to_string(param));
+	groups[a.first].param[param][a.second] = val;
+}
+
+const Network::Axon& Network::axon(unsigned source) const
+{
+	auto it = fcm.find(source);
+	if(it == fcm.end()) {
+		throw Error(NEMO_INVALID_INPUT, "synapses of non-existing neuron (" + std::to_string(source) + ") requested");
+	}
+	return it->second;
+}
+
+const std::vector<synapse_id>& Network::getSynapsesFrom(unsigned source)
+{
+	m_queried.clear();
+	auto it = fcm.find(source);
+	if(it != fcm.end()) {
+		for(size_t k = 0; k < it->second.target.size(); ++k) {
+			m_queried.push_back((synapse_id(source) << 32) | synapse_id(k));
+		}
+	}
+	return m_queried;
+}
+
+/* ---------------------------------------------------------- configuration */
+
+void Configuration::setStdpFunction(const std::vector<float>& pre, const std::vector<float>& post,
+		float minE_, float maxE_, float minI_, float maxI_)
+{
+	/* StdpFunction.cpp:12-48 */
+	if(maxI_ > 0.0f || minI_ > 0.0f) {
+		throw Error(NEMO_INVALID_INPUT, "STDP function should not have positive limits for inhibitory synapses");
+	}
+	if(maxE_ < 0.0f || minE_ < 0.0f) {
+		throw Error(NEMO_INVALID_INPUT, "STDP function should not have negative limits for excitatory synapses");
+	}
+	if(maxE_ < minE_) {
+		throw Error(NEMO_INVALID_INPUT, "STDP should have maximum excitatory weight >= minimum excitatory weight");
+	}
+	if(maxI_ > minI_) {
+		throw Error(NEMO_INVALID_INPUT, "STDP should have abs(maximum inhibitory weight) >= abs(minimum inhibitory weight)");
+	}
+	if(pre.size() + post.size() > 64) {
+		throw Error(NEMO_INVALID_INPUT, "size of STDP window too large");
+	}
+	stdp = true;
+	prefire = pre;
+	postfire = post;
+	minE = minE_;
+	maxE = maxE_;
+	minI = minI_;
+	maxI = maxI_;
+}
+
+void Configuration::setHipBackend(int dev)
+{
+	int count = 0;
+	if(hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+		throw Error(NEMO_CUDA_ERROR, "no HIP device available");
+	}
+	if(dev >= count) {
+		throw Error(NEMO_INVALID_INPUT, "invalid HIP device " + std::to_string(dev));
+	}
+	backend = NEMO_BACKEND_HIP;
+	device = dev < 0 ? 0 : dev;
+	hipDeviceProp_t prop;
+	HIP_CHECK(hipGetDeviceProperties(&prop, device));
+	description = std::string("HIP backend (") + prop.name + ", " + prop.gcnArchName + ")";
+}
+
+/* ------------------------------------------------------------- simulation */
+
+Simulation::Simulation(const Network& net, const Configuration& conf)
+{
+	if(conf.backend != NEMO_BACKEND_HIP) {
+		throw Error(NEMO_API_UNSUPPORTED, "this library provides the HIP backend only; select it with nemo_set_hip_backend");
+	}
+	/* nemo.cpp:111-117 */
+	if(net.neuronCount() == 0) {
+		throw Error(NEMO_INVALID_INPUT, "Cannot create simulation from empty network");
+	}
+	if(conf.stdp && net.maxDelay + conf.postfire.size() > 64) {
+		throw Error(NEMO_INVALID_INPUT, "Max delay + postfire part of STDP function must be <= 64");
+	}
+	if(net.maxDelay > NEMO_MAX_DELAY) {
+		throw Error(NEMO_INVALID_INPUT, "The network has synapses with delay " + std::to_string(net.maxDelay)
+				+ "ms. The HIP backend supports a maximum of 64 ms");
+	}
+	m_device = conf.device < 0 ? 0 : conf.device;
+	m_fbits = conf.fbits;
+	m_writeOnly = conf.writeOnlySynapses;
+	m_logging = conf.logging;
+	activate();
+	HIP_CHECK(hipStreamCreateWithFlags(&m_stream, hipStreamNonBlocking));
+	HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&m_pinnedClock), sizeof(nemo_hip_clock), hipHostMallocDefault));
+	HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&m_pinnedStepStart), NEMO_STEP_RING * sizeof(uint64_t),
+				hipHostMallocDefault));
+
+	buildNeurons(net);
+	buildConnectivity(net, conf);
+	buildStdp(conf);
+	allocateStepState();
+	chooseLaunchShapes();
+
+	nemo_hip_step_buffers b = stepBuffers(false, false);
+	for(Group& g : m_groups) {
+		HIP_CHECK(g.plugin->init(m_stream, &g.dev, &b));
+	}
+	HIP_CHECK(hipStreamSynchronize(m_stream));
+	if(m_logging) {
+		std::printf("nemo-hip: %zu neurons, %zu synapses (%zu plastic), max delay %u, Q%u.%u\n",
+				m_ncount, m_nsynapses, m_nplastic, m_D, 31 - m_fbits, m_fbits);
+	}
+	resetTimer();
+}
+
+Simulation::~Simulation()
+{
+	(void) hipSetDevice(m_device);
+	for(auto& kv : m_graphs) (void) hipGraphExecDestroy(kv.second);
+	if(m_stream) {
+		(void) hipStreamSynchronize(m_stream);
+		(void) hipStreamDestroy(m_stream);
+	}
+	if(m_pinnedClock) (void) hipHostFree(m_pinnedClock);
+	if(m_pinnedStepStart) (void) hipHostFree(m_pinnedStepStart);
+	if(m_pinnedLog) (void) hipHostFree(m_pinnedLog);
+}
+
+void Simulation::activate() const
+{
+	HIP_CHECK(hipSetDevice(m_device));
+}
+
+unsigned Simulation::localIdx(unsigned g) const
+{
+	auto it = m_g2l.find(g);
+	if(it == m_g2l.end()) {
+		throw Error(NEMO_INVALID_INPUT, "Non-existing neuron index " + std::to_string(g));
+	}
+	return it->second;
+}
+
+Simulation::Group& Simulation::groupOf(unsigned l)
+{
+	return m_groups.at(m_typeOfLocal.at(l));
+}
+
+/* Simulation indices: type by type (addNeuronType order), insertion order
+ * within a type (cpu/Simulation.cpp:60-77, cpu/Neurons.cpp:27-40).  RNG
+ * seeds: mt19937 default seed, four words >> 1 per simulation index
+ * (RNG.cpp:38-63, seeded by local index as cpu/Neurons.cpp:41 does). */
+void Simulation::buildNeurons(const Network& net)
+{
+	m_ncount = net.neuronCount();
+	m_l2g.resize(m_ncount);
+	m_typeOfLocal.resize(m_ncount);
+	m_g2l.reserve(m_ncount * 2);
+
+	std::vector<uint32_t> seeds(4 * m_ncount);
+	{
+		std::mt19937 mt;
+		for(size_t i = 0; i < seeds.size(); ++i) seeds[i] = uint32_t(mt()) >> 1;
+	}
+
+	unsigned base = 0;
+	for(unsigned t = 0; t < net.types.size(); ++t) {
+		const Network::Group& ng = net.groups[t];
+		if(ng.gidx.empty()) continue;
+		m_groups.emplace_back();
+		Group& g = m_groups.back();
+		g.type = net.types[t];
+		g.plugin.reset(new Plugin(g.type));
+		g.start = base;
+		g.count = unsigned(ng.gidx.size());
+		for(unsigned i = 0; i < g.count; ++i) {
+			m_l2g[base + i] = ng.gidx[i];
+			m_g2l[ng.gidx[i]] = base + i;
+			m_typeOfLocal[base + i] = unsigned(m_groups.size() - 1);
+		}
+		const size_t pitch = g.count;
+		std::vector<float> param(std::max<size_t>(1, g.type.nParam * pitch));
+		std::vector<float> state(std::max<size_t>(1, g.type.nState * pitch));
+		for(unsigned p = 0; p < g.type.nParam; ++p) {
+			std::copy(ng.param[p].begin(), ng.param[p].end(), param.begin() + p * pitch);
+		}
+		for(unsigned s = 0; s < g.type.nState; ++s) {
+			std::copy(ng.state[s].begin(), ng.state[s].end(), state.begin() + s * pitch);
+		}
+		g.param.allocate(param.size());
+		g.state.allocate(state.size());
+		g.rng.allocate(4 * size_t(g.count));
+		HIP_CHECK(hipMemcpy(g.param.ptr, param.data(), g.param.bytes(), hipMemcpyHostToDevice));
+		HIP_CHECK(hipMemcpy(g.state.ptr, state.data(), g.state.bytes(), hipMemcpyHostToDevice));
+		HIP_CHECK(hipMemcpy(g.rng.ptr, seeds.data() + 4 * size_t(base), g.rng.bytes(), hipMemcpyHostToDevice));
+		g.dev.start = g.start;
+		g.dev.count = g.count;
+		g.dev.pitch = pitch;
+		g.dev.param = g.param.ptr;
+		g.dev.state = g.state.ptr;
+		g.dev.rng = g.rng.ptr;
+		base += g.count;
+	}
+}
+
+/* The forward matrix: for every (source, delay) a contiguous row of
+ * {target, Q11.20 weight}, rows in (source, delay) order, synapses within a
+ * row in insertion order (ConnectivityMatrix.cpp:65-176).  The reverse matrix
+ * holds, per target, its plastic incoming synapses (construction/RCM.cpp:
+ * 112-178): packed source/delay and the forward index of the weight. */
+void Simulation::buildConnectivity(const Network& net, const Configuration& conf)
+{
+	m_D = std::max(1u, net.maxDelay);
+	const unsigned D = m_D;
+	const size_t N = m_ncount;
+	if(N >= (size_t(1) << 26)) {
+		throw Error(NEMO_API_UNSUPPORTED, "the HIP backend supports at most 2^26 neurons per simulation");
+	}
+	if(net.synapseCount >= (size_t(1) << 32)) {
+		throw Error(NEMO_API_UNSUPPORTED, "the HIP backend supports fewer than 2^32 synapses per simulation");
+	}
+
+	struct Src { unsigned g; unsigned l; const Network::Axon* ax; size_t seq; };
+	std::vector<Src> srcs;
+	srcs.reserve(net.fcm.size());
+	size_t seq = 0;
+	for(auto& kv : net.fcm) {
+		srcs.push_back(Src{kv.first, localIdx(kv.first), &kv.second, seq});
+		seq += kv.second.target.size();
+	}
+	m_nsynapses = seq;
+	const long ns = long(srcs.size());
+
+	/* local target index of every synapse, checked (verifySynapseTerminals) */
+	std::vector<uint32_t> ltgt(m_nsynapses);
+	int bad = 0;
+	unsigned badIdx = 0;
+#pragma omp parallel for schedule(dynamic, 64)
+	for(long i = 0; i < ns; ++i) {
+		const Src& s = srcs[i];
+		for(size_t k = 0; k < s.ax->target.size(); ++k) {
+			auto it = m_g2l.find(s.ax->target[k]);
+			if(it == m_g2l.end()) {
+#pragma omp critical
+				{ bad = 1; badIdx = s.ax->target[k]; }
+				ltgt[s.seq + k] = 0;
+			} else {
+				ltgt[s.seq + k] = it->second;
+			}
+		}
+	}
+	if(bad) {
+		throw Error(NEMO_INVALID_INPUT, "Invalid synapse target neuron " + std::to_string(badIdx));
+	}
+
+	/* row lengths and offsets */
+	std::vector<uint32_t> rowptr(N * D + 1, 0);
+#pragma omp parallel for schedule(dynamic, 64)
+	for(long i = 0; i < ns; ++i) {
+		const Src& s = srcs[i];
+		uint32_t* cnt = &rowptr[size_t(s.l) * D + 1];
+		for(uint8_t d : s.ax->delay) cnt[d - 1] += 1;
+	}
+	for(size_t r = 0; r < N * D; ++r) rowptr[r + 1] += rowptr[r];
+
+	/* fill rows (stable within a source) */
+	std::vector<int2> fcm(std::max<size_t>(1, m_nsynapses));
+	if(!m_writeOnly) {
+		m_synAddr.resize(m_nsynapses);
+		m_synTarget.resize(m_nsynapses);
+		m_synDelay.resize(m_nsynapses);
+		m_synPlastic.resize(m_nsynapses);
+	}
+	int overflow = 0;
+	float overflowValue = 0;
+#pragma omp parallel for schedule(dynamic, 64)
+	for(long i = 0; i < ns; ++i) {
+		const Src& s = srcs[i];
+		uint32_t cursor[NEMO_MAX_DELAY];
+		std::copy(&rowptr[size_t(s.l) * D], &rowptr[size_t(s.l) * D] + D, cursor);
+		for(size_t k = 0; k < s.ax->target.size(); ++k) {
+			const unsigned d = s.ax->delay[k];
+			const uint32_t pos = cursor[d - 1]++;
+			const float w = s.ax->weight[k];
+			int32_t wf = 0;
+			if(std::abs(int(w)) >= (1 << (31 - m_fbits))) {
+#pragma omp critical
+				{ overflow = 1; overflowValue = w; }
+			} else {
+				wf = static_cast<int32_t>(w * float(1 << m_fbits));
+			}
+			fcm[pos].x = int(ltgt[s.seq + k]);
+			fcm[pos].y = wf;
+			if(!m_writeOnly) {
+				m_synAddr[s.seq + k] = pos;
+				m_synTarget[s.seq + k] = s.ax->target[k];
+				m_synDelay[s.seq + k] = uint8_t(d);
+				m_synPlastic[s.seq + k] = s.ax->plastic[k];
+			}
+		}
+	}
+	if(overflow) {
+		fxToFix(overflowValue, m_fbits); /* throws with the reference message */
+	}
+	for(const Src& s : srcs) {
+		m_synRange[s.g] = std::make_pair(s.seq, s.seq + s.ax->target.size());
+	}
+
+	m_rowptr.allocate(rowptr.size());
+	HIP_CHECK(hipMemcpy(m_rowptr.ptr, rowptr.data(), m_rowptr.bytes(), hipMemcpyHostToDevice));
+	m_fcm.allocate(fcm.size());
+	HIP_CHECK(hipMemcpy(m_fcm.ptr, fcm.data(), m_fcm.bytes(), hipMemcpyHostToDevice));
+	if(!m_writeOnly) {
+		m_fcmHost.swap(fcm);
+		m_fcmHostValid = true;
+	}
+
+	/* reverse matrix (plastic synapses only, STDP enabled) */
+	m_stdp = conf.stdp;
+	std::vector<uint32_t> rcmPtr(N + 1, 0);
+	std::vector<uint32_t> rcmSd, rcmFwd;
+	if(m_stdp && m_writeOnly) {
+		throw Error(NEMO_API_UNSUPPORTED, "STDP with write-only synapses is not supported by the HIP backend");
+	}
+	if(m_stdp) {
+		/* per-thread counts over contiguous source ranges give each thread a
+		 * private, deterministic slot range in every target's row */
+		int T = 1;
+#ifdef _OPENMP
+		T = std::max(1, omp_get_max_threads());
+#endif
+		const long chunk = (ns + T - 1) / T;
+		std::vector<std::vector<uint32_t> > cnt(T, std::vector<uint32_t>(N, 0));
+#pragma omp parallel for num_threads(T) schedule(static, 1)
+		for(int t = 0; t < T; ++t) {
+			for(long i = t * chunk; i < std::min(ns, (t + 1) * chunk); ++i) {
+				const Src& s = srcs[i];
+				for(size_t k = 0; k < s.ax->target.size(); ++k) {
+					if(s.ax->plastic[k]) cnt[t][ltgt[s.seq + k]] += 1;
+				}
+			}
+		}
+		for(size_t n = 0; n < N; ++n) {
+			uint32_t run = rcmPtr[n];
+			for(int t = 0; t < T; ++t) {
+				uint32_t c = cnt[t][n];
+				cnt[t][n] = run;
+				run += c;
+			}
+			rcmPtr[n + 1] = run;
+		}
+		m_nplastic = rcmPtr[N];
+		rcmSd.resize(std::max<size_t>(1, m_nplastic));
+		rcmFwd.resize(std::max<size_t>(1, m_nplastic));
+#pragma omp parallel for num_threads(T) schedule(static, 1)
+		for(int t = 0; t < T; ++t) {
+			for(long i = t * chunk; i < std::min(ns, (t + 1) * chunk); ++i) {
+				const Src& s = srcs[i];
+				for(size_t k = 0; k < s.ax->target.size(); ++k) {
+					if(!s.ax->plastic[k]) continue;
+					const uint32_t tg = ltgt[s.seq + k];
+					const uint32_t pos = cnt[t][tg]++;
+					rcmSd[pos] = uint32_t(s.l) | (uint32_t(s.ax->delay[k] - 1) << 26);
+					rcmFwd[pos] = m_synAddr[s.seq + k];
+				}
+			}
+		}
+	}
+	m_rcmPtr.allocate(rcmPtr.size());
+	HIP_CHECK(hipMemcpy(m_rcmPtr.ptr, rcmPtr.data(), m_rcmPtr.bytes(), hipMemcpyHostToDevice));
+	if(m_stdp) {
+		m_rcmSd.allocate(rcmSd.size());
+		m_rcmFwd.allocate(rcmFwd.size());
+		m_rcmAcc.allocate(rcmSd.size());
+		HIP_CHECK(hipMemcpy(m_rcmSd.ptr, rcmSd.data(), m_rcmSd.bytes(), hipMemcpyHostToDevice));
+		HIP_CHECK(hipMemcpy(m_rcmFwd.ptr, rcmFwd.data(), m_rcmFwd.bytes(), hipMemcpyHostToDevice));
+		HIP_CHECK(hipMemset(m_rcmAcc.ptr, 0, m_rcmAcc.bytes()));
+	}
+}
+
+/* StdpProcess constructor (StdpProcess.cpp:20-52) and StdpFunction::getBits
+ * (StdpFunction.cpp:72-108): fixed-point tables and window bit masks. */
+void Simulation::buildStdp(const Configuration& conf)
+{
+	std::memset(&m_stdpArgs, 0, sizeof(m_stdpArgs));
+	if(!m_stdp) return;
+	const unsigned fbits = m_fbits;
+	m_minE = fxToFix(conf.minE, fbits);
+	m_maxE = fxToFix(conf.maxE, fbits);
+	m_minI = fxToFix(conf.minI, fbits);
+	m_maxI = fxToFix(conf.maxI, fbits);
+	for(size_t i = 0; i < conf.prefire.size(); ++i) m_stdpArgs.fnPre[i] = fxToFix(conf.prefire[i], fbits);
+	for(size_t i = 0; i < conf.postfire.size(); ++i) m_stdpArgs.fnPost[i] = fxToFix(conf.postfire[i], fbits);
+	uint64_t pot = 0, dep = 0;
+	unsigned n = 0;
+	for(auto f = conf.postfire.rbegin(); f != conf.postfire.rend(); ++f, ++n) {
+		if(*f > 0.0f) pot |= uint64_t(1) << n;
+		if(*f < 0.0f) dep |= uint64_t(1) << n;
+	}
+	for(auto f = conf.prefire.begin(); f != conf.prefire.end(); ++f, ++n) {
+		if(*f > 0.0f) pot |= uint64_t(1) << n;
+		if(*f < 0.0f) dep |= uint64_t(1) << n;
+	}
+	const unsigned pre = unsigned(conf.prefire.size());
+	const unsigned post = unsigned(conf.postfire.size());
+	auto lowMask = [](unsigned k) { return k >= 64 ? ~uint64_t(0) : ((uint64_t(1) << k) - 1); };
+	m_stdpArgs.postPreWindow = post;
+	m_stdpArgs.potentiationBits = pot;
+	m_stdpArgs.depressionBits = dep;
+	m_stdpArgs.prePostBits = post >= 64 ? 0 : (lowMask(pre) << post);
+	m_stdpArgs.postPreBits = lowMask(post);
+}
+
+void Simulation::allocateStepState()
+{
+	const size_t N = m_ncount;
+	m_clock.allocate(1);
+	HIP_CHECK(hipMemset(m_clock.ptr, 0, m_clock.bytes()));
+	m_stepStart.allocate(NEMO_STEP_RING);
+	HIP_CHECK(hipMemset(m_stepStart.ptr, 0, m_stepStart.bytes()));
+
+	/* The log must hold every firing that delivery (steps t-1..t-D), STDP
+	 * (step t-P) and an undrained run chunk can still need, even if every
+	 * neuron fires every step. */
+	const size_t window = std::max<size_t>(m_D, m_stdpArgs.postPreWindow + 1) + 2 + 64;
+	size_t cap = 1;
+	while(cap < window * N) cap <<= 1;
+	m_firedLog.allocate(cap);
+	m_logMask = cap - 1;
+
+	m_accPitch = (N + 63) & ~size_t(63);
+	m_acc.allocate(2 * m_accPitch);
+	HIP_CHECK(hipMemset(m_acc.ptr, 0, m_acc.bytes()));
+	m_hist.allocate(N);
+	HIP_CHECK(hipMemset(m_hist.ptr, 0, m_hist.bytes()));
+	m_ext.allocate(N);
+	HIP_CHECK(hipMemset(m_ext.ptr, 0, m_ext.bytes()));
+	m_fstim.allocate((N + 31) / 32);
+	HIP_CHECK(hipMemset(m_fstim.ptr, 0, m_fstim.bytes()));
+	m_debugCurrent.allocate(2 * m_accPitch);
+	HIP_CHECK(hipMemset(m_debugCurrent.ptr, 0, m_debugCurrent.bytes()));
+
+	m_stdpArgs.clock = m_clock.ptr;
+	m_stdpArgs.step_start = m_stepStart.ptr;
+	m_stdpArgs.fired_log = m_firedLog.ptr;
+	m_stdpArgs.log_mask = m_logMask;
+	m_stdpArgs.rcm_ptr = m_rcmPtr.ptr;
+	m_stdpArgs.rcm_sd = m_rcmSd.ptr;
+	m_stdpArgs.rcm_acc = m_rcmAcc.ptr;
+	m_stdpArgs.hist = m_hist.ptr;
+}
+
+static unsigned pow2AtLeast(double x, unsigned lo, unsigned hi)
+{
+	unsigned v = lo;
+	while(v < hi && double(v) < x) v <<= 1;
+	return v;
+}
+
+/* Lanes per synapse row: about a quarter of the mean row length (each lane
+ * keeps four loads in flight); enough blocks for a few percent of the
+ * neurons firing per step. */
+void Simulation::chooseLaunchShapes()
+{
+	size_t rows = 0;
+	if(m_nsynapses) {
+		std::vector<uint32_t> rp(m_rowptr.n);
+		HIP_CHECK(hipMemcpy(rp.data(), m_rowptr.ptr, m_rowptr.bytes(), hipMemcpyDeviceToHost));
+		for(size_t r = 0; r + 1 < rp.size(); ++r) rows += rp[r + 1] != rp[r];
+	}
+	const double meanRow = rows ? double(m_nsynapses) / double(rows) : 1.0;
+	m_deliverLanes = pow2AtLeast(meanRow / 4.0, 1, 64);
+	const double expectedRows = std::max(1.0, 0.05 * double(m_ncount) * double(rows) / double(std::max<size_t>(1, m_ncount)));
+	const double groupsPerBlock = 256.0 / m_deliverLanes;
+	m_deliverBlocks = unsigned(std::min(4096.0, std::max(1.0, std::ceil(expectedRows / groupsPerBlock))));
+
+	const double meanIn = m_ncount ? double(m_nplastic) / double(m_ncount) : 1.0;
+	m_stdpLanes = pow2AtLeast(meanIn / 4.0, 1, 64);
+	const double expectedTargets = std::max(1.0, 0.05 * double(m_ncount));
+	m_stdpBlocks = unsigned(std::min(4096.0, std::max(1.0, std::ceil(expectedTargets / (256.0 / m_stdpLanes)))));
+	if(const char* e = std::getenv("NEMO_HIP_DELIVER_LANES")) m_deliverLanes = unsigned(std::atoi(e));
+	if(const char* e = std::getenv("NEMO_HIP_DELIVER_BLOCKS")) m_deliverBlocks = unsigned(std::atoi(e));
+	if(const char* e = std::getenv("NEMO_HIP_STDP_LANES")) m_stdpLanes = unsigned(std::atoi(e));
+	if(const char* e = std::getenv("NEMO_HIP_STDP_BLOCKS")) m_stdpBlocks = unsigned(std::atoi(e));
+}
+
+nemo_hip_step_buffers Simulation::stepBuffers(bool ext, bool fstim) const
+{
+	nemo_hip_step_buffers b;
+	b.acc = m_acc.ptr;
+	b.acc_pitch = m_accPitch;
+	b.ext_current = ext ? m_ext.ptr : nullptr;
+	b.fstim = fstim ? m_fstim.ptr : nullptr;
+	b.hist = m_hist.ptr;
+	b.fired_log = m_firedLog.ptr;
+	b.log_mask = m_logMask;
+	b.clock = m_clock.ptr;
+	b.fbits = m_fbits;
+	b.debug_current = m_debug ? m_debugCurrent.ptr : nullptr;
+	return b;
+}
+
+/* One step on the stream: deliver (prefire), neuron updates (fire), STDP
+ * (postfire), advance the clock (SimulationBackend.cpp:22-77). */
+void Simulation::enqueueStep(bool ext, bool fstim, hipEvent_t* ev)
+{
+	if(ev) HIP_CHECK(hipEventRecord(ev[0], m_stream));
+	nemo_deliver_args d;
+	d.clock = m_clock.ptr;
+	d.step_start = m_stepStart.ptr;
+	d.fired_log = m_firedLog.ptr;
+	d.log_mask = m_logMask;
+	d.rowptr = m_rowptr.ptr;
+	d.fcm = m_fcm.ptr;
+	d.acc = m_acc.ptr;
+	d.acc_pitch = m_accPitch;
+	d.D = m_D;
+	if(m_nsynapses) {
+		HIP_CHECK(nemo_launch_deliver(m_stream, &d, m_deliverLanes, m_deliverBlocks));
+	}
+	if(ev) HIP_CHECK(hipEventRecord(ev[1], m_stream));
+	nemo_hip_step_buffers b = stepBuffers(ext, fstim);
+	for(Group& g : m_groups) {
+		HIP_CHECK(g.plugin->update(m_stream, &g.dev, &b));
+	}
+	if(ev) HIP_CHECK(hipEventRecord(ev[2], m_stream));
+	if(m_stdp && m_nplastic) {
+		HIP_CHECK(nemo_launch_stdp(m_stream, &m_stdpArgs, m_stdpLanes, m_stdpBlocks));
+	}
+	if(ev) HIP_CHECK(hipEventRecord(ev[3], m_stream));
+	HIP_CHECK(nemo_launch_advance(m_stream, m_clock.ptr, m_stepStart.ptr));
+	if(ev) HIP_CHECK(hipEventRecord(ev[4], m_stream));
+	m_hostStep += 1;
+}
+
+/* Eager steps with HIP events at every phase boundary on the simulation's
+ * stream: average microseconds of delivery, neuron update (all groups),
+ * STDP accumulation and the clock advance, and the work counters of the
+ * profiled steps (events delivered, neurons fired, plastic synapses examined,
+ * steps).  The steps are real simulation steps; their firing is not returned. */
+void Simulation::profile(unsigned nsteps, double avgUs[4], uint64_t counters[4])
+{
+	activate();
+	HIP_CHECK(hipStreamSynchronize(m_stream));
+	HIP_CHECK(hipMemcpy(m_pinnedClock, m_clock.ptr, sizeof(nemo_hip_clock), hipMemcpyDeviceToHost));
+	const nemo_hip_clock c0 = *m_pinnedClock;
+	std::vector<hipEvent_t> ev(5 * size_t(nsteps));
+	for(hipEvent_t& e : ev) HIP_CHECK(hipEventCreate(&e));
+	for(unsigned s = 0; s < nsteps; ++s) enqueueStep(false, false, &ev[5 * size_t(s)]);
+	HIP_CHECK(hipStreamSynchronize(m_stream));
+	double sum[4] = {0, 0, 0, 0};
+	for(unsigned s = 0; s < nsteps; ++s) {
+		for(int k = 0; k < 4; ++k) {
+			float ms = 0;
+			HIP_CHECK(hipEventElapsedTime(&ms, ev[5 * size_t(s) + k], ev[5 * size_t(s) + k + 1]));
+			sum[k] += ms;
+		}
+	}
+	for(hipEvent_t& e : ev) (void) hipEventDestroy(e);
+	HIP_CHECK(hipMemcpy(m_pinnedClock, m_clock.ptr, sizeof(nemo_hip_clock), hipMemcpyDeviceToHost));
+	const nemo_hip_clock c1 = *m_pinnedClock;
+	for(int k = 0; k < 4; ++k) avgUs[k] = nsteps ? 1000.0 * sum[k] / nsteps : 0.0;
+	counters[0] = c1.events - c0.events;
+	counters[1] = c1.log_head - c0.log_head;
+	counters[2] = c1.stdp_synapses - c0.stdp_synapses;
+	counters[3] = nsteps;
+	m_drainedHead = c1.log_head;
+}
+
+void Simulation::ensurePinnedLog(size_t n)
+{
+	if(n <= m_pinnedLogCap) return;
+	if(m_pinnedLog) HIP_CHECK(hipHostFree(m_pinnedLog));
+	m_pinnedLogCap = std::max<size_t>(n, 4096);
+	HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&m_pinnedLog), m_pinnedLogCap * sizeof(uint32_t),
+				hipHostMallocDefault));
+}
+
+/* Copy log positions [from, to) to out (wrapping around the circular log). */
+void Simulation::readLogRange(uint64_t from, uint64_t to, uint32_t* out)
+{
+	const uint64_t cap = m_logMask + 1;
+	if(to - from > cap) {
+		throw Error(NEMO_BUFFER_OVERFLOW, "fired log overrun");
+	}
+	uint64_t pos = from;
+	while(pos < to) {
+		const uint64_t off = pos & m_logMask;
+		const uint64_t len = std::min<uint64_t>(to - pos, cap - off);
+		HIP_CHECK(hipMemcpyAsync(out + (pos - from), m_firedLog.ptr + off, len * sizeof(uint32_t),
+					hipMemcpyDeviceToHost, m_stream));
+		pos += len;
+	}
+}
+
+const std::vector<unsigned>& Simulation::step(const unsigned* fstim, size_t nf,
+		const unsigned* istimIdx, const float* istimVal, size_t ni)
+{
+	activate();
+	/* current stimulus, then firing stimulus (SimulationBackend.cpp:61-77);
+	 * for repeated neurons the last current wins (cpu/Simulation.cpp:140-145) */
+	std::vector<uint32_t> iIdx;
+	std::vector<float> iVal;
+	if(ni) {
+		std::unordered_map<unsigned, size_t> last;
+		for(size_t i = 0; i < ni; ++i) last[localIdx(istimIdx[i])] = i;
+		for(auto& kv : last) {
+			iIdx.push_back(kv.first);
+			iVal.push_back(istimVal[kv.second]);
+		}
+	}
+	std::vector<uint32_t> fIdx(nf);
+	for(size_t i = 0; i < nf; ++i) fIdx[i] = localIdx(fstim[i]);
+
+	const size_t nstim = std::max(iIdx.size(), fIdx.size());
+	if(nstim > m_stimIdx.n) {
+		HIP_CHECK(hipStreamSynchronize(m_stream));
+		m_stimIdx.allocate(nstim);
+		m_stimVal.allocate(nstim);
+	}
+	if(!iIdx.empty()) {
+		HIP_CHECK(hipMemcpyAsync(m_stimIdx.ptr, iIdx.data(), iIdx.size() * 4, hipMemcpyHostToDevice, m_stream));
+		HIP_CHECK(hipMemcpyAsync(m_stimVal.ptr, iVal.data(), iVal.size() * 4, hipMemcpyHostToDevice, m_stream));
+		HIP_CHECK(nemo_launch_istim(m_stream, m_ext.ptr, m_stimIdx.ptr, m_stimVal.ptr, iIdx.size()));
+		/* the index buffer is reused for the firing stimulus below */
+		HIP_CHECK(hipStreamSynchronize(m_stream));
+	}
+	if(!fIdx.empty()) {
+		HIP_CHECK(hipMemcpyAsync(m_stimIdx.ptr, fIdx.data(), fIdx.size() * 4, hipMemcpyHostToDevice, m_stream));
+		HIP_CHECK(nemo_launch_fstim(m_stream, m_fstim.ptr, m_stimIdx.ptr, fIdx.size(), 1));
+	}
+	enqueueStep(!iIdx.empty(), !fIdx.empty());
+	if(!fIdx.empty()) {
+		HIP_CHECK(nemo_launch_fstim(m_stream, m_fstim.ptr, m_stimIdx.ptr, fIdx.size(), 0));
+	}
+
+	HIP_CHECK(hipMemcpyAsync(m_pinnedClock, m_clock.ptr, sizeof(nemo_hip_clock), hipMemcpyDeviceToHost, m_stream));
+	HIP_CHECK(hipStreamSynchronize(m_stream));
+	const uint64_t head = m_pinnedClock->log_head;
+	const size_t nfired = size_t(head - m_drainedHead);
+	ensurePinnedLog(nfired);
+	readLogRange(m_drainedHead, head, m_pinnedLog);
+	HIP_CHECK(hipStreamSynchronize(m_stream));
+	m_drainedHead = head;
+	/* output in ascending simulation order (cpu/Simulation.cpp:200-209) */
+	std::sort(m_pinnedLog, m_pinnedLog + nfired);
+	m_firedOut.resize(nfired);
+	for(size_t i = 0; i < nfired; ++i) m_firedOut[i] = m_l2g[m_pinnedLog[i]];
+	return m_firedOut;
+}
+
+hipGraphExec_t Simulation::graphFor(unsigned k)
+{
+	auto it = m_graphs.find(k);
+	if(it != m_graphs.end()) return it->second;
+	hipGraph_t graph;
+	HIP_CHECK(hipStreamBeginCapture(m_stream, hipStreamCaptureModeThreadLocal));
+	const bool debug = m_debug;
+	m_debug = false;
+	try {
+		for(unsigned i = 0; i < k; ++i) enqueueStep(false, false);
+	} catch(...) {
+		m_debug = debug;
+		(void) hipStreamEndCapture(m_stream, &graph);
+		throw;
+	}
+	m_debug = debug;
+	m_hostStep -= k; /* capture does not run anything */
+	HIP_CHECK(hipStreamEndCapture(m_stream, &graph));
+	hipGraphExec_t exec;
+	HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+	HIP_CHECK(hipGraphDestroy(graph));
+	m_graphs[k] = exec;
+	return exec;
+}
+
+/* Steps without stimulus, captured in graphs of up to 64 steps; the fired log
+ * is drained after every graph (it holds at least 64 steps of worst-case
+ * firing beyond the delivery window). */
+void Simulation::run(unsigned nsteps, unsigned stdpPeriod, float reward, bool readRaster,
+		std::vector<unsigned>* counts, std::vector<unsigned>* fired, uint64_t* events)
+{
+	activate();
+	if(stdpPeriod && !m_stdp) {
+		throw Error(NEMO_LOGIC_ERROR, "applyStdp called, but no STDP model specified");
+	}
+	HIP_CHECK(hipMemcpyAsync(m_pinnedClock, m_clock.ptr, sizeof(nemo_hip_clock), hipMemcpyDeviceToHost, m_stream));
+	HIP_CHECK(hipStreamSynchronize(m_stream));
+	const uint64_t events0 = m_pinnedClock->events;
+	uint64_t stepsSinceApply = 0;
+	const unsigned CHUNK = 64;
+	unsigned done = 0;
+	std::vector<uint32_t> buf;
+	while(done < nsteps) {
+		unsigned k = std::min(CHUNK, nsteps - done);
+		if(stdpPeriod) k = unsigned(std::min<uint64_t>(k, stdpPeriod - stepsSinceApply));
+		const uint64_t t0 = m_hostStep;
+		HIP_CHECK(hipGraphLaunch(graphFor(k), m_stream));
+		m_hostStep += k;
+		HIP_CHECK(hipMemcpyAsync(m_pinnedClock, m_clock.ptr, sizeof(nemo_hip_clock), hipMemcpyDeviceToHost, m_stream));
+		if(readRaster) {
+			HIP_CHECK(hipMemcpyAsync(m_pinnedStepStart, m_stepStart.ptr, NEMO_STEP_RING * sizeof(uint64_t),
+						hipMemcpyDeviceToHost, m_stream));
+		}
+		HIP_CHECK(hipStreamSynchronize(m_stream));
+		const uint64_t head = m_pinnedClock->log_head;
+		if(readRaster) {
+			const size_t nf = size_t(head - m_drainedHead);
+			ensurePinnedLog(nf);
+			readLogRange(m_drainedHead, head, m_pinnedLog);
+			HIP_CHECK(hipStreamSynchronize(m_stream));
+			for(unsigned s = 0; s < k; ++s) {
+				const uint64_t step = t0 + s;
+				const uint64_t b = m_pinnedStepStart[step & (NEMO_STEP_RING - 1)];
+				const uint64_t e = m_pinnedStepStart[(step + 1) & (NEMO_STEP_RING - 1)];
+				uint32_t* first = m_pinnedLog + (b - m_drainedHead);
+				uint32_t* last = m_pinnedLog + (e - m_drainedHead);
+				std::sort(first, last);
+				counts->push_back(unsigned(e - b));
+				for(uint32_t* p = first; p != last; ++p) fired->push_back(m_l2g[*p]);
+			}
+		}
+		m_drainedHead = head;
+		done += k;
+		stepsSinceApply += k;
+		if(stdpPeriod && stepsSinceApply == stdpPeriod) {
+			applyStdp(reward);
+			stepsSinceApply = 0;
+		}
+	}
+	HIP_CHECK(hipStreamSynchronize(m_stream));
+	if(events) {
+		HIP_CHECK(hipMemcpy(m_pinnedClock, m_clock.ptr, sizeof(nemo_hip_clock), hipMemcpyDeviceToHost));
+		*events = m_pinnedClock->events - events0;
+	}
+	if(!readRaster && !m_firedOut.empty()) m_firedOut.clear();
+}
+
+void Simulation::applyStdp(float reward)
+{
+	activate();
+	if(!m_stdp) {
+		throw Error(NEMO_LOGIC_ERROR, "applyStdp called, but no STDP model specified");
+	}
+	/* ConnectivityMatrix.cpp:279-291 */
+	int32_t fx_reward = fxToFix(reward, m_fbits);
+	if(fx_reward == 0 && reward != 0) {
+		std::ostringstream msg;
+		msg << "STDP reward rounded down to zero. The smallest valid reward is " << fxToFloat(1, m_fbits);
+		throw Error(NEMO_INVALID_INPUT, msg.str());
+	}
+	nemo_apply_args a;
+	a.rcm_fwd = m_rcmFwd.ptr;
+	a.rcm_acc = m_rcmAcc.ptr;
+	a.fcm = m_fcm.ptr;
+	a.count = m_nplastic;
+	a.reward = fx_reward;
+	a.minE = m_minE;
+	a.maxE = m_maxE;
+	a.minI = m_minI;
+	a.maxI = m_maxI;
+	a.fbits = m_fbits;
+	HIP_CHECK(nemo_launch_apply_stdp(m_stream, &a));
+	m_fcmHostValid = false;
+}
+
+/* --------------------------------------------------------- neuron access */
+
+float Simulation::getNeuronState(unsigned idx, unsigned var)
+{
+	activate();
+	unsigned l = localIdx(idx);
+	Group& g = groupOf(l);
+	if(var >= g.type.nState) throw Error(NEMO_INVALID_INPUT, "Invalid state variable index " + std::to_string(var));
+	float v;
+	HIP_CHECK(hipStreamSynchronize(m_stream));
+	HIP_CHECK(hipMemcpy(&v, g.state.ptr + size_t(var) * g.count + (l - g.start), 4, hipMemcpyDeviceToHost));
+	return v;
+}
+
+float Simulation::getNeuronParameter(unsigned idx, unsigned param)
+{
+	activate();
+	unsigned l = localIdx(idx);
+	Group& g = groupOf(l);
+	if(param >= g.type.nParam) throw Error(NEMO_INVALID_INPUT, "Invalid parameter index " + std::to_string(param));
+	float v;
+	HIP_CHECK(hipStreamSynchronize(m_stream));
+	HIP_CHECK(hipMemcpy(&v, g.param.ptr + size_t(param) * g.count + (l - g.start), 4, hipMemcpyDeviceToHost));
+	return v;
+}
+
+float Simulation::getMembranePotential(unsigned idx)
+{
+	unsigned l = localIdx(idx);
+	Group& g = groupOf(l);
+	if(g.type.membranePotential >= g.type.nState) {
+		throw Error(NEMO_INVALID_INPUT, "neuron type " + g.type.name + " has no membrane potential");
+	}
+	return getNeuronState(idx, g.type.membranePotential);
+}
+
+void Simulation::setNeuron(unsigned idx, unsigned nargs, const float* args)
+{
+	activate();
+	unsigned l = localIdx(idx);
+	Group& g = groupOf(l);
+	if(nargs != g.type.nParam + g.type.nState) {
+		std::ostringstream msg;
+		msg << "Unexpected number of parameters/state variables when modifying neuron. Expected "
+			<< (g.type.nParam + g.type.nState) << ", found " << nargs;
+		throw Error(NEMO_INVALID_INPUT, msg.str());
+	}
+	HIP_CHECK(hipStreamSynchronize(m_stream));
+	for(unsigned p = 0; p < g.type.nParam; ++p) {
+		HIP_CHECK(hipMemcpy(g.param.ptr + size_t(p) * g.count + (l - g.start), args + p, 4, hipMemcpyHostToDevice));
+	}
+	for(unsigned s = 0; s < g.type.nState; ++s) {
+		HIP_CHECK(hipMemcpy(g.state.ptr + size_t(s) * g.count + (l - g.start), args + g.type.nParam + s, 4,
+					hipMemcpyHostToDevice));
+	}
+}
+
+void Simulation::setNeuronState(unsigned idx, unsigned var, float val)
+{
+	activate();
+	unsigned l = localIdx(idx);
+	Group& g = groupOf(l);
+	if(var >= g.type.nState) throw Error(NEMO_INVALID_INPUT, "Invalid state variable index " + std::to_string(var));
+	HIP_CHECK(hipStreamSynchronize(m_stream));
+	HIP_CHECK(hipMemcpy(g.state.ptr + size_t(var) * g.count + (l - g.start), &val, 4, hipMemcpyHostToDevice));
+}
+
+void Simulation::setNeuronParameter(unsigned idx, unsigned param, float val)
+{
+	activate();
+	unsigned l = localIdx(idx);
+	Group& g = groupOf(l);
+	if(param >= g.type.nParam) throw Error(NEMO_INVALID_INPUT, "Invalid parameter index " + std::to_string(param));
+	HIP_CHECK(hipStreamSynchronize(m_stream));
+	HIP_CHECK(hipMemcpy(g.param.ptr + size_t(param) * g.count + (l - g.start), &val, 4, hipMemcpyHostToDevice));
+}
+
+/* -------------------------------------------------------- synapse access */
+
+void Simulation::syncFcmHost()
+{
+	if(m_fcmHostValid) return;
+	activate();
+	HIP_CHECK(hipStreamSynchronize(m_stream));
+	m_fcmHost.resize(m_fcm.n);
+	HIP_CHECK(hipMemcpy(m_fcmHost.data(), m_fcm.ptr, m_fcm.bytes(), hipMemcpyDeviceToHost));
+	m_fcmHostValid = true;
+}
+
+size_t Simulation::synapseIndex(synapse_id id) const
+{
+	if(m_writeOnly) {
+		throw Error(NEMO_INVALID_INPUT, "Cannot read synapse state if simulation configured with write-only synapses");
+	}
+	unsigned src = unsigned(id >> 32);
+	size_t k = size_t(id & 0xffffffffu);
+	auto it = m_synRange.find(src);
+	if(it == m_synRange.end()) {
+		throw Error(NEMO_INVALID_INPUT, "Non-existing neuron id (" + std::to_string(src) + ") in synapse query");
+	}
+	if(k >= it->second.second - it->second.first) {
+		throw Error(NEMO_INVALID_INPUT, "synapse id out of range");
+	}
+	return it->second.first + k;
+}
+
+const std::vector<synapse_id>& Simulation::getSynapsesFrom(unsigned source)
+{
+	if(m_writeOnly) {
+		throw Error(NEMO_INVALID_INPUT, "Cannot read synapse state if simulation configured with write-only synapses");
+	}
+	m_queried.clear();
+	auto it = m_synRange.find(source);
+	if(it == m_synRange.end()) {
+		localIdx(source); /* throws for a non-existing neuron */
+	} else {
+		const size_t n = it->second.second - it->second.first;
+		m_queried.resize(n);
+		for(size_t k = 0; k < n; ++k) m_queried[k] = (synapse_id(source) << 32) | synapse_id(k);
+	}
+	return m_queried;
+}
+
+unsigned Simulation::getSynapseTarget(synapse_id id) { return m_synTarget[synapseIndex(id)]; }
+unsigned Simulation::getSynapseDelay(synapse_id id) { return m_synDelay[synapseIndex(id)]; }
+unsigned char Simulation::getSynapsePlastic(synapse_id id) { return m_synPlastic[synapseIndex(id)]; }
+
+float Simulation::getSynapseWeight(synapse_id id)
+{
+	size_t s = synapseIndex(id);
+	syncFcmHost();
+	return fxToFloat(m_fcmHost[m_synAddr[s]].y, m_fbits);
+}
+
+/* ----------------------------------------------------------------- timer */
+
+unsigned long Simulation::elapsedWallclock() const
+{
+	using namespace std::chrono;
+	(void) hipStreamSynchronize(m_stream);
+	return (unsigned long) duration_cast<milliseconds>(steady_clock::now() - m_wallStart).count();
+}
+
+unsigned long Simulation::elapsedSimulation()
+{
+	activate();
+	HIP_CHECK(hipStreamSynchronize(m_stream));
+	HIP_CHECK(hipMemcpy(m_pinnedClock, m_clock.ptr, sizeof(nemo_hip_clock), hipMemcpyDeviceToHost));
+	return (unsigned long) m_pinnedClock->cycle;
+}
+
+void Simulation::resetTimer()
+{
+	activate();
+	HIP_CHECK(hipStreamSynchronize(m_stream));
+	uint64_t zero = 0;
+	HIP_CHECK(hipMemcpy(&m_clock.ptr->cycle, &zero, sizeof(zero), hipMemcpyHostToDevice));
+	m_wallStart = std::chrono::steady_clock::now();
+}
+
+/* ------------------------------------------------------------- inspection */
+
+void Simulation::setDebug(bool on)
+{
+	m_debug = on;
+}
+
+void Simulation::getCurrents(float* E, float* I, size_t n)
+{
+	if(n != m_ncount) throw Error(NEMO_INVALID_INPUT, "size mismatch");
+	activate();
+	HIP_CHECK(hipStreamSynchronize(m_stream));
+	HIP_CHECK(hipMemcpy(E, m_debugCurrent.ptr, n * 4, hipMemcpyDeviceToHost));
+	HIP_CHECK(hipMemcpy(I, m_debugCurrent.ptr + m_accPitch, n * 4, hipMemcpyDeviceToHost));
+}
+
+void Simulation::localToGlobal(unsigned* g, size_t n) const
+{
+	if(n != m_ncount) throw Error(NEMO_INVALID_INPUT, "size mismatch");
+	std::copy(m_l2g.begin(), m_l2g.end(), g);
+}
+
+void Simulation::getRecentFiring(uint64_t* h, size_t n)
+{
+	if(n != m_ncount) throw Error(NEMO_INVALID_INPUT, "size mismatch");
+	activate();
+	HIP_CHECK(hipStreamSynchronize(m_stream));
+	HIP_CHECK(hipMemcpy(h, m_hist.ptr, n * 8, hipMemcpyDeviceToHost));
+}
+
+void Simulation::getRngState(uint32_t* s, size_t n)
+{
+	if(n != m_ncount) throw Error(NEMO_INVALID_INPUT, "size mismatch");
+	activate();
+	HIP_CHECK(hipStreamSynchronize(m_stream));
+	for(Group& g : m_groups) {
+		HIP_CHECK(hipMemcpy(s + 4 * size_t(g.start), g.rng.ptr, g.rng.bytes(), hipMemcpyDeviceToHost));
+	}
+}
+
+void Simulation::getFixedWeights(int32_t* w, size_t n)
+{
+	if(m_writeOnly) throw Error(NEMO_INVALID_INPUT, "write-only synapses");
+	if(n != m_nsynapses) throw Error(NEMO_INVALID_INPUT, "size mismatch");
+	syncFcmHost();
+	for(size_t k = 0; k < n; ++k) w[k] = m_fcmHost[m_synAddr[k]].y;
+}
+
+void Simulation::getStdpAccumulators(int32_t* acc, size_t n)
+{
+	if(m_writeOnly) throw Error(NEMO_INVALID_INPUT, "write-only synapses");
+	if(n != m_nsynapses) throw Error(NEMO_INVALID_INPUT, "size mismatch");
+	std::fill(acc, acc + n, 0);
+	if(!m_stdp || m_nplastic == 0) return;
+	activate();
+	HIP_CHECK(hipStreamSynchronize(m_stream));
+	std::vector<int32_t> racc(m_nplastic);
+	std::vector<uint32_t> fwd(m_nplastic);
+	HIP_CHECK(hipMemcpy(racc.data(), m_rcmAcc.ptr, m_nplastic * 4, hipMemcpyDeviceToHost));
+	HIP_CHECK(hipMemcpy(fwd.data(), m_rcmFwd.ptr, m_nplastic * 4, hipMemcpyDeviceToHost));
+	std::vector<int32_t> byAddr(m_fcm.n, 0);
+	for(size_t r = 0; r < m_nplastic; ++r) byAddr[fwd[r]] = racc[r];
+	for(size_t k = 0; k < n; ++k) acc[k] = byAddr[m_synAddr[k]];
+}
+
+uint64_t Simulation::eventsDelivered()
+{
+	activate();
+	HIP_CHECK(hipStreamSynchronize(m_stream));
+	HIP_CHECK(hipMemcpy(m_pinnedClock, m_clock.ptr, sizeof(nemo_hip_clock), hipMemcpyDeviceToHost));
+	return m_pinnedClock->events;
+}
+
+} // namespace nemo_hip

@@ -1,0 +1,289 @@
+/*
+ * Host side of the HIP backend: network construction, configuration, model
+ * plugin loading, the connectivity builder and the simulation object behind
+ * the C ABI in include/nemo_hip.h.
+ *
+ * Reference interfaces mirrored (NeMo v0.7.2):
+ *   Network        src/nemo/Network.hpp:45-231, NetworkImpl.cpp:27-128
+ *   Configuration  src/nemo/Configuration.hpp:70-195, ConfigurationImpl.cpp
+ *   Simulation     src/nemo/Simulation.hpp:38-218, SimulationBackend.hpp:39-163
+ *   NeuronType     src/nemo/NeuronType.cpp:26-158 (.ini metadata)
+ *   Plugin         src/nemo/Plugin.cpp:31-195 (model shared libraries)
+ */
+#ifndef NEMO_AMD_BACKEND_HPP
+#define NEMO_AMD_BACKEND_HPP
+
+#include <chrono>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include <hip/hip_runtime_api.h>
+
+#include "nemo_hip.h"
+#include "nemo_hip_plugin.h"
+#include "kernels.h"
+
+namespace nemo_hip {
+
+/* nemo::exception (src/nemo/exception.hpp:23-38): an error with a status code */
+struct Error : std::runtime_error {
+	int code;
+	Error(int c, const std::string& msg) : std::runtime_error(msg), code(c) {}
+};
+
+void hipCheck(hipError_t err, const char* what);
+#define HIP_CHECK(x) ::nemo_hip::hipCheck((x), #x)
+
+/* Neuron model metadata from <Model>.ini (NeuronType.cpp:93-158) */
+struct NeuronType {
+	std::string name;
+	unsigned nParam = 0;
+	unsigned nState = 0;
+	unsigned membranePotential = 0;
+	unsigned history = 1;
+	bool rngNormal = false;
+	bool usesRcm = false;
+	std::string pluginDir;
+	std::map<std::string, unsigned> paramNames;
+	std::map<std::string, unsigned> stateNames;
+};
+
+NeuronType loadNeuronType(const std::string& name);
+void addPluginPath(const std::string& dir);
+std::vector<std::string> pluginPaths();
+
+/* Network under construction (NetworkImpl) */
+class Network {
+public:
+	unsigned addNeuronType(const std::string& name);
+	void addNeurons(unsigned type, size_t n, const unsigned* idx, unsigned nargs, const float* args);
+	void addSynapses(size_t n, const unsigned* src, const unsigned* tgt, const unsigned* delay,
+			const float* weight, const unsigned char* plastic, synapse_id* ids);
+	unsigned neuronCount() const;
+
+	float getNeuronState(unsigned idx, unsigned var) const;
+	float getNeuronParameter(unsigned idx, unsigned param) const;
+	void setNeuron(unsigned idx, unsigned nargs, const float* args);
+	void setNeuronState(unsigned idx, unsigned var, float val);
+	void setNeuronParameter(unsigned idx, unsigned param, float val);
+
+	struct Axon {
+		std::vector<uint32_t> target;
+		std::vector<uint8_t> delay;
+		std::vector<float> weight;
+		std::vector<uint8_t> plastic;
+	};
+	const Axon& axon(unsigned source) const;
+	const std::vector<synapse_id>& getSynapsesFrom(unsigned source);
+
+	struct Group {
+		std::vector<std::vector<float> > param; /* [p][i] */
+		std::vector<std::vector<float> > state; /* [s][i] */
+		std::vector<unsigned> gidx;
+	};
+	std::vector<NeuronType> types;
+	std::map<std::string, unsigned> typeIds;
+	std::vector<Group> groups;
+	std::unordered_map<unsigned, std::pair<unsigned, unsigned> > mapper; /* gidx -> (type, i) */
+	std::map<unsigned, Axon> fcm; /* source -> synapses in insertion order */
+	unsigned maxDelay = 0;
+	size_t synapseCount = 0;
+
+private:
+	std::pair<unsigned, unsigned> address(unsigned idx) const;
+	std::vector<synapse_id> m_queried;
+};
+
+/* Configuration (ConfigurationImpl) */
+struct Configuration {
+	bool stdp = false;
+	std::vector<float> prefire, postfire;
+	float minE = 0, maxE = 0, minI = 0, maxI = 0;
+	unsigned fbits = 20; /* ConfigurationImpl.cpp:22 */
+	backend_t backend = NEMO_BACKEND_HIP;
+	int device = -1;
+	bool writeOnlySynapses = false;
+	bool logging = false;
+	std::string description;
+
+	void setStdpFunction(const std::vector<float>& pre, const std::vector<float>& post,
+			float minE, float maxE, float minI, float maxI);
+	void setHipBackend(int dev);
+};
+
+/* A loaded model plugin (Plugin.cpp:141-180) */
+struct Plugin {
+	void* handle = nullptr;
+	hip_update_neurons_t* update = nullptr;
+	hip_init_neurons_t* init = nullptr;
+	explicit Plugin(const NeuronType& type);
+	~Plugin();
+	Plugin(const Plugin&) = delete;
+	Plugin& operator=(const Plugin&) = delete;
+};
+
+/* Owning device allocation */
+template<typename T>
+struct DeviceArray {
+	T* ptr = nullptr;
+	size_t n = 0;
+	DeviceArray() {}
+	explicit DeviceArray(size_t count) { allocate(count); }
+	~DeviceArray() { release(); }
+	DeviceArray(const DeviceArray&) = delete;
+	DeviceArray& operator=(const DeviceArray&) = delete;
+	DeviceArray(DeviceArray&& o) noexcept : ptr(o.ptr), n(o.n) { o.ptr = nullptr; o.n = 0; }
+	DeviceArray& operator=(DeviceArray&& o) noexcept {
+		if(this != &o) { release(); ptr = o.ptr; n = o.n; o.ptr = nullptr; o.n = 0; }
+		return *this;
+	}
+	void allocate(size_t count);
+	void release();
+	size_t bytes() const { return n * sizeof(T); }
+};
+
+/* The HIP simulation (cuda::Simulation's role, cuda/Simulation.cpp:91-380) */
+class Simulation {
+public:
+	Simulation(const Network& net, const Configuration& conf);
+	~Simulation();
+
+	/* SimulationBackend::step (SimulationBackend.cpp:61-77) */
+	const std::vector<unsigned>& step(const unsigned* fstim, size_t nf,
+			const unsigned* istimIdx, const float* istimVal, size_t ni);
+	void applyStdp(float reward);
+	void run(unsigned nsteps, unsigned stdpPeriod, float reward, bool readRaster,
+			std::vector<unsigned>* counts, std::vector<unsigned>* fired, uint64_t* events);
+	void profile(unsigned nsteps, double avgUs[4], uint64_t counters[4]);
+
+	float getNeuronState(unsigned idx, unsigned var);
+	float getNeuronParameter(unsigned idx, unsigned param);
+	float getMembranePotential(unsigned idx);
+	void setNeuron(unsigned idx, unsigned nargs, const float* args);
+	void setNeuronState(unsigned idx, unsigned var, float val);
+	void setNeuronParameter(unsigned idx, unsigned param, float val);
+
+	const std::vector<synapse_id>& getSynapsesFrom(unsigned source);
+	unsigned getSynapseTarget(synapse_id id);
+	unsigned getSynapseDelay(synapse_id id);
+	float getSynapseWeight(synapse_id id);
+	unsigned char getSynapsePlastic(synapse_id id);
+
+	unsigned long elapsedWallclock() const;
+	unsigned long elapsedSimulation();
+	void resetTimer();
+
+	/* inspection */
+	void setDebug(bool on);
+	void getCurrents(float* E, float* I, size_t n);
+	void localToGlobal(unsigned* g, size_t n) const;
+	void getRecentFiring(uint64_t* h, size_t n);
+	void getRngState(uint32_t* s, size_t n);
+	void getFixedWeights(int32_t* w, size_t n);
+	void getStdpAccumulators(int32_t* acc, size_t n);
+	size_t neuronCount() const { return m_ncount; }
+	size_t synapseCount() const { return m_nsynapses; }
+	size_t plasticCount() const { return m_nplastic; }
+	uint64_t eventsDelivered();
+
+	const std::vector<unsigned>& firedOut() const { return m_firedOut; }
+
+private:
+	struct Group {
+		NeuronType type;
+		std::unique_ptr<Plugin> plugin;
+		unsigned start = 0, count = 0;
+		DeviceArray<float> param, state;
+		DeviceArray<uint32_t> rng;
+		nemo_hip_neuron_group dev;
+	};
+
+	int m_device = 0;
+	hipStream_t m_stream = nullptr;
+	size_t m_ncount = 0;
+	unsigned m_D = 1;
+	unsigned m_fbits = 20;
+	size_t m_nsynapses = 0, m_nplastic = 0;
+	bool m_writeOnly = false;
+	bool m_logging = false;
+
+	std::vector<unsigned> m_l2g;
+	std::unordered_map<unsigned, unsigned> m_g2l;
+	std::vector<unsigned> m_typeOfLocal;
+	std::vector<Group> m_groups;
+
+	/* device state */
+	DeviceArray<nemo_hip_clock> m_clock;
+	DeviceArray<uint64_t> m_stepStart;
+	DeviceArray<uint32_t> m_firedLog;
+	uint64_t m_logMask = 0;
+	DeviceArray<int64_t> m_acc;
+	size_t m_accPitch = 0;
+	DeviceArray<uint64_t> m_hist;
+	DeviceArray<float> m_ext;
+	DeviceArray<uint32_t> m_fstim;
+	DeviceArray<float> m_debugCurrent;
+	bool m_debug = false;
+	DeviceArray<uint32_t> m_rowptr;
+	DeviceArray<int2> m_fcm;
+	DeviceArray<uint32_t> m_rcmPtr, m_rcmSd, m_rcmFwd;
+	DeviceArray<int32_t> m_rcmAcc;
+	DeviceArray<uint32_t> m_stimIdx;
+	DeviceArray<float> m_stimVal;
+
+	/* STDP */
+	bool m_stdp = false;
+	nemo_stdp_args m_stdpArgs;
+	int32_t m_minE = 0, m_maxE = 0, m_minI = 0, m_maxI = 0;
+
+	/* launch shapes (tuned per network) */
+	unsigned m_deliverLanes = 16, m_deliverBlocks = 1024;
+	unsigned m_stdpLanes = 16, m_stdpBlocks = 512;
+
+	/* host side of synapse queries (network order = source ascending, then insertion) */
+	std::map<unsigned, std::pair<size_t, size_t> > m_synRange; /* gsrc -> [first,last) */
+	std::vector<uint32_t> m_synAddr;   /* synapse -> fcm index */
+	std::vector<uint32_t> m_synTarget; /* synapse -> user target index */
+	std::vector<uint8_t> m_synDelay;
+	std::vector<uint8_t> m_synPlastic;
+	std::vector<int2> m_fcmHost;       /* cached copy of the fcm for weight queries */
+	bool m_fcmHostValid = false;
+	std::vector<synapse_id> m_queried;
+
+	/* host staging */
+	uint64_t m_hostStep = 0;          /* steps launched */
+	uint64_t m_drainedHead = 0;       /* log position up to which firing was read */
+	nemo_hip_clock* m_pinnedClock = nullptr;
+	uint32_t* m_pinnedLog = nullptr;
+	size_t m_pinnedLogCap = 0;
+	uint64_t* m_pinnedStepStart = nullptr;
+	std::vector<unsigned> m_firedOut;
+	std::chrono::steady_clock::time_point m_wallStart;
+
+	/* captured multi-step graphs, keyed by step count */
+	std::map<unsigned, hipGraphExec_t> m_graphs;
+
+	void activate() const;
+	unsigned localIdx(unsigned g) const;
+	Group& groupOf(unsigned l);
+	void buildNeurons(const Network& net);
+	void buildConnectivity(const Network& net, const Configuration& conf);
+	void buildStdp(const Configuration& conf);
+	void allocateStepState();
+	void chooseLaunchShapes();
+	nemo_hip_step_buffers stepBuffers(bool ext, bool fstim) const;
+	void enqueueStep(bool ext, bool fstim, hipEvent_t* ev = nullptr);
+	void readLogRange(uint64_t from, uint64_t to, uint32_t* out);
+	void ensurePinnedLog(size_t n);
+	hipGraphExec_t graphFor(unsigned k);
+	void syncFcmHost();
+	size_t synapseIndex(synapse_id id) const;
+};
+
+} // namespace nemo_hip
+
+#endif

@@ -1,0 +1,541 @@
+/*
+ * C ABI of libnemo_hip.so (include/nemo_hip.h).  Mirrors NeMo's C API
+ * (src/nemo/nemo_c.cpp): exceptions are caught at the boundary and turned into
+ * a status code plus a last-error string (nemo_c.cpp:20-60).
+ */
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "backend.hpp"
+
+using nemo_hip::Error;
+
+struct nemo_network_class { nemo_hip::Network impl; };
+struct nemo_configuration_class { nemo_hip::Configuration impl; };
+struct nemo_simulation_class {
+	explicit nemo_simulation_class(const nemo_hip::Network& n, const nemo_hip::Configuration& c) : impl(n, c) {}
+	nemo_hip::Simulation impl;
+	std::vector<unsigned> runCounts, runFired;
+};
+
+namespace {
+
+thread_local std::string g_lastError = "No error";
+thread_local std::string g_descr;
+
+template<typename F>
+nemo_status_t guarded(F f)
+{
+	try {
+		f();
+		return NEMO_OK;
+	} catch(Error& e) {
+		g_lastError = e.what();
+		return e.code;
+	} catch(std::bad_alloc&) {
+		g_lastError = "memory allocation failed";
+		return NEMO_ALLOCATION_ERROR;
+	} catch(std::exception& e) {
+		g_lastError = e.what();
+		return NEMO_UNKNOWN_ERROR;
+	} catch(...) {
+		g_lastError = "unknown exception";
+		return NEMO_UNKNOWN_ERROR;
+	}
+}
+
+template<typename P>
+void check(P* p, const char* what)
+{
+	if(!p) throw Error(NEMO_INVALID_INPUT, std::string("NULL ") + what + " handle");
+}
+
+} // namespace
+
+extern "C" {
+
+const char* nemo_version(void) { return "0.7.2-hip"; }
+
+const char* nemo_strerror(void) { return g_lastError.c_str(); }
+
+nemo_status_t nemo_add_plugin_path(const char* dir)
+{
+	return guarded([&] { nemo_hip::addPluginPath(dir); });
+}
+
+nemo_status_t nemo_hip_device_count(unsigned* count)
+{
+	return guarded([&] {
+		int n = 0;
+		if(hipGetDeviceCount(&n) != hipSuccess) n = 0;
+		*count = unsigned(n);
+	});
+}
+
+nemo_status_t nemo_hip_device_description(unsigned device, const char** descr)
+{
+	return guarded([&] {
+		hipDeviceProp_t prop;
+		HIP_CHECK(hipGetDeviceProperties(&prop, int(device)));
+		g_descr = std::string(prop.name) + " (" + prop.gcnArchName + ", "
+			+ std::to_string(prop.multiProcessorCount) + " CUs, "
+			+ std::to_string(prop.totalGlobalMem >> 30) + " GiB)";
+		*descr = g_descr.c_str();
+	});
+}
+
+/* ---------------------------------------------------------- configuration */
+
+nemo_configuration_t nemo_new_configuration(void)
+{
+	nemo_configuration_t c = new nemo_configuration_class();
+	/* default hardware (nemo.cpp:174-186): the HIP device if there is one */
+	try {
+		c->impl.setHipBackend(-1);
+	} catch(...) {
+		c->impl.description = "HIP backend (no device)";
+	}
+	return c;
+}
+
+void nemo_delete_configuration(nemo_configuration_t c) { delete c; }
+
+nemo_status_t nemo_log_stdout(nemo_configuration_t c)
+{
+	return guarded([&] { check(c, "configuration"); c->impl.logging = true; });
+}
+
+nemo_status_t nemo_set_stdp_function(nemo_configuration_t c,
+		float prefire_fn[], size_t prefire_len, float postfire_fn[], size_t postfire_len,
+		float minE, float maxE, float minI, float maxI)
+{
+	return guarded([&] {
+		check(c, "configuration");
+		c->impl.setStdpFunction(std::vector<float>(prefire_fn, prefire_fn + prefire_len),
+				std::vector<float>(postfire_fn, postfire_fn + postfire_len), minE, maxE, minI, maxI);
+	});
+}
+
+nemo_status_t nemo_set_cpu_backend(nemo_configuration_t c)
+{
+	return guarded([&] {
+		check(c, "configuration");
+		c->impl.backend = NEMO_BACKEND_CPU;
+		c->impl.description = "CPU backend (not provided by this library)";
+	});
+}
+
+nemo_status_t nemo_set_hip_backend(nemo_configuration_t c, int dev)
+{
+	return guarded([&] { check(c, "configuration"); c->impl.setHipBackend(dev); });
+}
+
+nemo_status_t nemo_hip_device(nemo_configuration_t c, int* dev)
+{
+	return guarded([&] { check(c, "configuration"); *dev = c->impl.device; });
+}
+
+nemo_status_t nemo_backend(nemo_configuration_t c, backend_t* backend)
+{
+	return guarded([&] { check(c, "configuration"); *backend = c->impl.backend; });
+}
+
+nemo_status_t nemo_backend_description(nemo_configuration_t c, const char** descr)
+{
+	return guarded([&] { check(c, "configuration"); *descr = c->impl.description.c_str(); });
+}
+
+nemo_status_t nemo_set_write_only_synapses(nemo_configuration_t c)
+{
+	return guarded([&] { check(c, "configuration"); c->impl.writeOnlySynapses = true; });
+}
+
+/* ----------------------------------------------------------- construction */
+
+nemo_network_t nemo_new_network(void) { return new nemo_network_class(); }
+
+void nemo_delete_network(nemo_network_t n) { delete n; }
+
+nemo_status_t nemo_add_neuron_type(nemo_network_t n, const char* name, unsigned* type)
+{
+	return guarded([&] { check(n, "network"); *type = n->impl.addNeuronType(name); });
+}
+
+nemo_status_t nemo_add_neuron_iz(nemo_network_t n, unsigned idx,
+		float a, float b, float c, float d, float u, float v, float sigma)
+{
+	/* Network::addNeuron (src/nemo/Network.cpp:47-56): args {a,b,c,d,sigma,u,v} */
+	return guarded([&] {
+		check(n, "network");
+		unsigned type = n->impl.addNeuronType("Izhikevich");
+		float args[7] = {a, b, c, d, sigma, u, v};
+		n->impl.addNeurons(type, 1, &idx, 7, args);
+	});
+}
+
+nemo_status_t nemo_add_neuron(nemo_network_t n, unsigned type, unsigned idx, unsigned nargs, float args[])
+{
+	return guarded([&] { check(n, "network"); n->impl.addNeurons(type, 1, &idx, nargs, args); });
+}
+
+nemo_status_t nemo_add_neurons(nemo_network_t n, unsigned type, size_t count,
+		const unsigned idx[], unsigned nargs, const float args[])
+{
+	return guarded([&] { check(n, "network"); n->impl.addNeurons(type, count, idx, nargs, args); });
+}
+
+nemo_status_t nemo_add_synapse(nemo_network_t n, unsigned source, unsigned target,
+		unsigned delay, float weight, unsigned char is_plastic, synapse_id* id)
+{
+	return guarded([&] {
+		check(n, "network");
+		n->impl.addSynapses(1, &source, &target, &delay, &weight, &is_plastic, id);
+	});
+}
+
+nemo_status_t nemo_add_synapses(nemo_network_t n, size_t count,
+		const unsigned source[], const unsigned target[], const unsigned delay[],
+		const float weight[], const unsigned char plastic[], synapse_id ids[])
+{
+	return guarded([&] {
+		check(n, "network");
+		n->impl.addSynapses(count, source, target, delay, weight, plastic, ids);
+	});
+}
+
+nemo_status_t nemo_neuron_count(nemo_network_t n, unsigned* ncount)
+{
+	return guarded([&] { check(n, "network"); *ncount = n->impl.neuronCount(); });
+}
+
+nemo_status_t nemo_get_neuron_state_n(nemo_network_t n, unsigned neuron, unsigned var, float* val)
+{
+	return guarded([&] { check(n, "network"); *val = n->impl.getNeuronState(neuron, var); });
+}
+
+nemo_status_t nemo_get_neuron_parameter_n(nemo_network_t n, unsigned neuron, unsigned param, float* val)
+{
+	return guarded([&] { check(n, "network"); *val = n->impl.getNeuronParameter(neuron, param); });
+}
+
+nemo_status_t nemo_set_neuron_n(nemo_network_t n, unsigned idx, unsigned nargs, float args[])
+{
+	return guarded([&] { check(n, "network"); n->impl.setNeuron(idx, nargs, args); });
+}
+
+nemo_status_t nemo_set_neuron_state_n(nemo_network_t n, unsigned neuron, unsigned var, float val)
+{
+	return guarded([&] { check(n, "network"); n->impl.setNeuronState(neuron, var, val); });
+}
+
+nemo_status_t nemo_set_neuron_parameter_n(nemo_network_t n, unsigned neuron, unsigned param, float val)
+{
+	return guarded([&] { check(n, "network"); n->impl.setNeuronParameter(neuron, param, val); });
+}
+
+static size_t networkSynapse(nemo_network_t n, synapse_id id, const nemo_hip::Network::Axon** ax)
+{
+	*ax = &n->impl.axon(unsigned(id >> 32));
+	size_t k = size_t(id & 0xffffffffu);
+	if(k >= (*ax)->target.size()) throw Error(NEMO_INVALID_INPUT, "synapse id out or range");
+	return k;
+}
+
+nemo_status_t nemo_get_synapse_source_n(nemo_network_t n, synapse_id id, unsigned* source)
+{
+	return guarded([&] {
+		check(n, "network");
+		const nemo_hip::Network::Axon* ax;
+		networkSynapse(n, id, &ax);
+		*source = unsigned(id >> 32);
+	});
+}
+
+nemo_status_t nemo_get_synapse_target_n(nemo_network_t n, synapse_id id, unsigned* target)
+{
+	return guarded([&] {
+		check(n, "network");
+		const nemo_hip::Network::Axon* ax;
+		size_t k = networkSynapse(n, id, &ax);
+		*target = ax->target[k];
+	});
+}
+
+nemo_status_t nemo_get_synapse_delay_n(nemo_network_t n, synapse_id id, unsigned* delay)
+{
+	return guarded([&] {
+		check(n, "network");
+		const nemo_hip::Network::Axon* ax;
+		size_t k = networkSynapse(n, id, &ax);
+		*delay = ax->delay[k];
+	});
+}
+
+nemo_status_t nemo_get_synapse_weight_n(nemo_network_t n, synapse_id id, float* weight)
+{
+	return guarded([&] {
+		check(n, "network");
+		const nemo_hip::Network::Axon* ax;
+		size_t k = networkSynapse(n, id, &ax);
+		*weight = ax->weight[k];
+	});
+}
+
+nemo_status_t nemo_get_synapse_plastic_n(nemo_network_t n, synapse_id id, unsigned char* plastic)
+{
+	return guarded([&] {
+		check(n, "network");
+		const nemo_hip::Network::Axon* ax;
+		size_t k = networkSynapse(n, id, &ax);
+		*plastic = ax->plastic[k];
+	});
+}
+
+nemo_status_t nemo_get_synapses_from_n(nemo_network_t n, unsigned source, synapse_id* synapses[], size_t* len)
+{
+	return guarded([&] {
+		check(n, "network");
+		const std::vector<synapse_id>& ids = n->impl.getSynapsesFrom(source);
+		*synapses = const_cast<synapse_id*>(ids.data());
+		*len = ids.size();
+	});
+}
+
+/* ------------------------------------------------------------- simulation */
+
+nemo_simulation_t nemo_new_simulation(nemo_network_t n, nemo_configuration_t c)
+{
+	nemo_simulation_t s = nullptr;
+	guarded([&] {
+		check(n, "network");
+		check(c, "configuration");
+		s = new nemo_simulation_class(n->impl, c->impl);
+	});
+	return s;
+}
+
+nemo_simulation_t hip_simulation(nemo_network_t n, nemo_configuration_t c)
+{
+	return nemo_new_simulation(n, c);
+}
+
+void nemo_delete_simulation(nemo_simulation_t s) { delete s; }
+
+nemo_status_t nemo_step(nemo_simulation_t s,
+		unsigned fstim_nidx[], size_t fstim_count,
+		unsigned istim_nidx[], float istim_current[], size_t istim_count,
+		unsigned* fired[], size_t* fired_count)
+{
+	return guarded([&] {
+		check(s, "simulation");
+		const std::vector<unsigned>& f = s->impl.step(fstim_nidx, fstim_count, istim_nidx, istim_current, istim_count);
+		if(fired) *fired = const_cast<unsigned*>(f.data());
+		if(fired_count) *fired_count = f.size();
+	});
+}
+
+nemo_status_t nemo_apply_stdp(nemo_simulation_t s, float reward)
+{
+	return guarded([&] { check(s, "simulation"); s->impl.applyStdp(reward); });
+}
+
+nemo_status_t nemo_get_membrane_potential(nemo_simulation_t s, unsigned neuron, float* v)
+{
+	return guarded([&] { check(s, "simulation"); *v = s->impl.getMembranePotential(neuron); });
+}
+
+nemo_status_t nemo_get_neuron_state_s(nemo_simulation_t s, unsigned neuron, unsigned var, float* val)
+{
+	return guarded([&] { check(s, "simulation"); *val = s->impl.getNeuronState(neuron, var); });
+}
+
+nemo_status_t nemo_get_neuron_parameter_s(nemo_simulation_t s, unsigned neuron, unsigned param, float* val)
+{
+	return guarded([&] { check(s, "simulation"); *val = s->impl.getNeuronParameter(neuron, param); });
+}
+
+nemo_status_t nemo_set_neuron_s(nemo_simulation_t s, unsigned idx, unsigned nargs, float args[])
+{
+	return guarded([&] { check(s, "simulation"); s->impl.setNeuron(idx, nargs, args); });
+}
+
+nemo_status_t nemo_set_neuron_state_s(nemo_simulation_t s, unsigned neuron, unsigned var, float val)
+{
+	return guarded([&] { check(s, "simulation"); s->impl.setNeuronState(neuron, var, val); });
+}
+
+nemo_status_t nemo_set_neuron_parameter_s(nemo_simulation_t s, unsigned neuron, unsigned param, float val)
+{
+	return guarded([&] { check(s, "simulation"); s->impl.setNeuronParameter(neuron, param, val); });
+}
+
+nemo_status_t nemo_get_synapse_source_s(nemo_simulation_t s, synapse_id id, unsigned* source)
+{
+	return guarded([&] {
+		check(s, "simulation");
+		(void) s->impl.getSynapseDelay(id); /* validates the id */
+		*source = unsigned(id >> 32);
+	});
+}
+
+nemo_status_t nemo_get_synapse_target_s(nemo_simulation_t s, synapse_id id, unsigned* target)
+{
+	return guarded([&] { check(s, "simulation"); *target = s->impl.getSynapseTarget(id); });
+}
+
+nemo_status_t nemo_get_synapse_delay_s(nemo_simulation_t s, synapse_id id, unsigned* delay)
+{
+	return guarded([&] { check(s, "simulation"); *delay = s->impl.getSynapseDelay(id); });
+}
+
+nemo_status_t nemo_get_synapse_weight_s(nemo_simulation_t s, synapse_id id, float* weight)
+{
+	return guarded([&] { check(s, "simulation"); *weight = s->impl.getSynapseWeight(id); });
+}
+
+nemo_status_t nemo_get_synapse_plastic_s(nemo_simulation_t s, synapse_id id, unsigned char* plastic)
+{
+	return guarded([&] { check(s, "simulation"); *plastic = s->impl.getSynapsePlastic(id); });
+}
+
+nemo_status_t nemo_get_synapses_from_s(nemo_simulation_t s, unsigned source, synapse_id* synapses[], size_t* len)
+{
+	return guarded([&] {
+		check(s, "simulation");
+		const std::vector<synapse_id>& ids = s->impl.getSynapsesFrom(source);
+		*synapses = const_cast<synapse_id*>(ids.data());
+		*len = ids.size();
+	});
+}
+
+nemo_status_t nemo_elapsed_wallclock(nemo_simulation_t s, unsigned long* ms)
+{
+	return guarded([&] { check(s, "simulation"); *ms = s->impl.elapsedWallclock(); });
+}
+
+nemo_status_t nemo_elapsed_simulation(nemo_simulation_t s, unsigned long* ms)
+{
+	return guarded([&] { check(s, "simulation"); *ms = s->impl.elapsedSimulation(); });
+}
+
+nemo_status_t nemo_reset_timer(nemo_simulation_t s)
+{
+	return guarded([&] { check(s, "simulation"); s->impl.resetTimer(); });
+}
+
+nemo_status_t nemo_hip_run(nemo_simulation_t s, unsigned nsteps, unsigned stdp_period, float reward,
+		unsigned counts[], unsigned* fired[], size_t* fired_len, uint64_t* events)
+{
+	return guarded([&] {
+		check(s, "simulation");
+		s->runCounts.clear();
+		s->runFired.clear();
+		const bool raster = counts != nullptr;
+		s->impl.run(nsteps, stdp_period, reward, raster, &s->runCounts, &s->runFired, events);
+		if(raster) {
+			std::memcpy(counts, s->runCounts.data(), s->runCounts.size() * sizeof(unsigned));
+			if(fired) *fired = s->runFired.data();
+			if(fired_len) *fired_len = s->runFired.size();
+		}
+	});
+}
+
+nemo_status_t nemo_hip_set_debug(nemo_simulation_t s, int enable)
+{
+	return guarded([&] { check(s, "simulation"); s->impl.setDebug(enable != 0); });
+}
+
+nemo_status_t nemo_hip_get_currents(nemo_simulation_t s, float E[], float I[], size_t n)
+{
+	return guarded([&] { check(s, "simulation"); s->impl.getCurrents(E, I, n); });
+}
+
+nemo_status_t nemo_hip_local_to_global(nemo_simulation_t s, unsigned gidx[], size_t n)
+{
+	return guarded([&] { check(s, "simulation"); s->impl.localToGlobal(gidx, n); });
+}
+
+nemo_status_t nemo_hip_get_recent_firing(nemo_simulation_t s, uint64_t hist[], size_t n)
+{
+	return guarded([&] { check(s, "simulation"); s->impl.getRecentFiring(hist, n); });
+}
+
+nemo_status_t nemo_hip_get_rng_state(nemo_simulation_t s, uint32_t state[], size_t n)
+{
+	return guarded([&] { check(s, "simulation"); s->impl.getRngState(state, n); });
+}
+
+nemo_status_t nemo_hip_get_fixed_weights(nemo_simulation_t s, int32_t w[], size_t n)
+{
+	return guarded([&] { check(s, "simulation"); s->impl.getFixedWeights(w, n); });
+}
+
+nemo_status_t nemo_hip_get_stdp_accumulators(nemo_simulation_t s, int32_t acc[], size_t n)
+{
+	return guarded([&] { check(s, "simulation"); s->impl.getStdpAccumulators(acc, n); });
+}
+
+nemo_status_t nemo_hip_counts(nemo_simulation_t s, size_t* neurons, size_t* synapses, size_t* plastic)
+{
+	return guarded([&] {
+		check(s, "simulation");
+		if(neurons) *neurons = s->impl.neuronCount();
+		if(synapses) *synapses = s->impl.synapseCount();
+		if(plastic) *plastic = s->impl.plasticCount();
+	});
+}
+
+nemo_status_t nemo_hip_profile(nemo_simulation_t s, unsigned nsteps, double avg_us[4], uint64_t counters[4])
+{
+	return guarded([&] { check(s, "simulation"); s->impl.profile(nsteps, avg_us, counters); });
+}
+
+nemo_status_t nemo_hip_eval_math(int fn, const float in[], float out[], size_t n)
+{
+	return guarded([&] {
+		if(fn < 0 || fn > 2) throw Error(NEMO_INVALID_INPUT, "unknown function");
+		if(n == 0) return;
+		nemo_hip::DeviceArray<float> din(n), dout(n);
+		HIP_CHECK(hipMemcpy(din.ptr, in, n * sizeof(float), hipMemcpyHostToDevice));
+		HIP_CHECK(nemo_launch_eval_math(nullptr, fn, din.ptr, dout.ptr, n));
+		HIP_CHECK(hipDeviceSynchronize());
+		HIP_CHECK(hipMemcpy(out, dout.ptr, n * sizeof(float), hipMemcpyDeviceToHost));
+	});
+}
+
+/* ------------------------------------------------- backend factory (dlopen) */
+
+nemo_status_t hip_set_configuration(nemo_configuration_t c, int dev)
+{
+	return nemo_set_hip_backend(c, dev);
+}
+
+unsigned hip_device_count(void)
+{
+	int n = 0;
+	if(hipGetDeviceCount(&n) != hipSuccess) return 0;
+	return unsigned(n);
+}
+
+const char* hip_device_description(unsigned device)
+{
+	const char* d = nullptr;
+	if(nemo_hip_device_description(device, &d) != NEMO_OK) return nullptr;
+	return d;
+}
+
+nemo_status_t hip_test_device(unsigned device)
+{
+	return guarded([&] {
+		int n = 0;
+		HIP_CHECK(hipGetDeviceCount(&n));
+		if(int(device) >= n) throw Error(NEMO_INVALID_INPUT, "invalid HIP device");
+		HIP_CHECK(hipSetDevice(int(device)));
+		void* p = nullptr;
+		HIP_CHECK(hipMalloc(&p, 256));
+		HIP_CHECK(hipFree(p));
+	});
+}
+
+} // extern "C"

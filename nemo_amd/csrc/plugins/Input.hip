@@ -1,0 +1,40 @@
+/*
+ * Input neuron plugin (libInput.so): fires exactly when stimulated
+ * (src/nemo/cpu/plugins/Input.cpp:6-26).  Currents are ignored; the
+ * accumulators are read and cleared.
+ */
+#include "../plugin_device.h"
+
+namespace {
+
+__global__ void __launch_bounds__(NEMO_UPDATE_BLOCK)
+input_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
+{
+	const unsigned nl = blockIdx.x * NEMO_UPDATE_BLOCK + threadIdx.x;
+	const bool valid = nl < g.count;
+	const unsigned ng = g.start + nl;
+	bool fired = false;
+	if(valid) {
+		float E, I;
+		nemo_dev::take_current(b, ng, E, I); /* clears the accumulators */
+		fired = nemo_dev::stimulated(b, ng);
+	}
+	nemo_dev::record_firing(b, ng, valid, fired);
+}
+
+} // namespace
+
+extern "C" hipError_t
+hip_update_neurons(hipStream_t stream, const nemo_hip_neuron_group* g, const nemo_hip_step_buffers* b)
+{
+	if(g->count == 0) return hipSuccess;
+	const unsigned blocks = (g->count + NEMO_UPDATE_BLOCK - 1) / NEMO_UPDATE_BLOCK;
+	hipLaunchKernelGGL(input_update, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), 0, stream, *g, *b);
+	return hipGetLastError();
+}
+
+extern "C" hipError_t
+hip_init_neurons(hipStream_t, const nemo_hip_neuron_group*, const nemo_hip_step_buffers*)
+{
+	return hipSuccess;
+}

@@ -1,0 +1,87 @@
+/*
+ * Izhikevich neuron model plugin (libIzhikevich.so).
+ *
+ * Bit-exact with the CPU model src/nemo/cpu/plugins/Izhikevich.cpp:14-98:
+ *   I = (E + I) + ext;  I += sigma * nrand  (when sigma != 0)
+ *   4 substeps of 0.25 ms, each skipped once the neuron has fired:
+ *     v = float(double(v) + 0.25 * (((0.04 v + 5) v + 140) - u + I))   [FP64]
+ *     u = u + 0.25f * (a * (b v - u))                                  [FP32]
+ *     fired = v >= 30
+ *   fired |= fstim; on firing v = c, u += d.
+ * The reference writes 0.04, 5.0 and 140.0 as double literals, so its v update
+ * is evaluated in double; this kernel does the same, with FP contraction off
+ * (the build passes -ffp-contract=off).
+ * Parameters a, b, c, d, sigma and state u, v use the indices of
+ * src/nemo/plugins/Izhikevich.h:6-15.
+ */
+#include "../plugin_device.h"
+
+namespace {
+
+enum { PARAM_A = 0, PARAM_B = 1, PARAM_C = 2, PARAM_D = 3, PARAM_SIGMA = 4 };
+enum { STATE_U = 0, STATE_V = 1 };
+
+__global__ void __launch_bounds__(NEMO_UPDATE_BLOCK)
+izhikevich_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
+{
+	const unsigned nl = blockIdx.x * NEMO_UPDATE_BLOCK + threadIdx.x;
+	const bool valid = nl < g.count;
+	const unsigned ng = g.start + nl;
+	bool fired = false;
+
+	if(valid) {
+		const size_t P = g.pitch;
+		float E, In;
+		nemo_dev::take_current(b, ng, E, In);
+		float I = E + In + nemo_dev::take_external(b, ng);
+
+		const float sigma = g.param[PARAM_SIGMA * P + nl];
+		if(sigma != 0.0f) {
+			uint4* rng = reinterpret_cast<uint4*>(g.rng);
+			uint4 s = rng[nl];
+			I += sigma * nemo_dev::nrand(s);
+			rng[nl] = s;
+		}
+
+		const float a = g.param[PARAM_A * P + nl];
+		const float bb = g.param[PARAM_B * P + nl];
+		float u = g.state[STATE_U * P + nl];
+		float v = g.state[STATE_V * P + nl];
+
+#pragma unroll
+		for(int t = 0; t < 4; ++t) {
+			if(!fired) {
+				const double vd = double(v);
+				v = float(vd + double(0.25f) * ((((0.04 * vd + 5.0) * vd) + 140.0) - double(u) + double(I)));
+				u = u + 0.25f * (a * (bb * v - u));
+				fired = v >= 30.0f;
+			}
+		}
+
+		fired = fired || nemo_dev::stimulated(b, ng);
+		if(fired) {
+			v = g.param[PARAM_C * P + nl];
+			u += g.param[PARAM_D * P + nl];
+		}
+		g.state[STATE_U * P + nl] = u;
+		g.state[STATE_V * P + nl] = v;
+	}
+	nemo_dev::record_firing(b, ng, valid, fired);
+}
+
+} // namespace
+
+extern "C" hipError_t
+hip_update_neurons(hipStream_t stream, const nemo_hip_neuron_group* g, const nemo_hip_step_buffers* b)
+{
+	if(g->count == 0) return hipSuccess;
+	const unsigned blocks = (g->count + NEMO_UPDATE_BLOCK - 1) / NEMO_UPDATE_BLOCK;
+	hipLaunchKernelGGL(izhikevich_update, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), 0, stream, *g, *b);
+	return hipGetLastError();
+}
+
+extern "C" hipError_t
+hip_init_neurons(hipStream_t, const nemo_hip_neuron_group*, const nemo_hip_step_buffers*)
+{
+	return hipSuccess; /* src/nemo/cpu/plugins/default_init.c:3-21: nothing to do */
+}

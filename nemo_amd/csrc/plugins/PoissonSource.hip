@@ -1,0 +1,49 @@
+/*
+ * Poisson spike source plugin (libPoissonSource.so).
+ *
+ * Bit-exact with src/nemo/cpu/plugins/PoissonSource.cpp:6-30: the neuron
+ * fires when (urand & 0xffff) < unsigned(rate * 2^16), or when stimulated.
+ * One xorshift draw per neuron per step.  Parameter 0 is the rate per ms.
+ * Synaptic and external currents are ignored (and, as in the reference, the
+ * external current is not cleared); the accumulators are still read and cleared.
+ */
+#include "../plugin_device.h"
+
+namespace {
+
+__global__ void __launch_bounds__(NEMO_UPDATE_BLOCK)
+poisson_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
+{
+	const unsigned nl = blockIdx.x * NEMO_UPDATE_BLOCK + threadIdx.x;
+	const bool valid = nl < g.count;
+	const unsigned ng = g.start + nl;
+	bool fired = false;
+	if(valid) {
+		float E, I;
+		nemo_dev::take_current(b, ng, E, I); /* clears the accumulators */
+		const unsigned p0 = unsigned(g.param[nl] * float(1 << 16));
+		uint4* rng = reinterpret_cast<uint4*>(g.rng);
+		uint4 s = rng[nl];
+		const unsigned p1 = nemo_dev::urand(s) & 0xffff;
+		rng[nl] = s;
+		fired = p1 < p0 || nemo_dev::stimulated(b, ng);
+	}
+	nemo_dev::record_firing(b, ng, valid, fired);
+}
+
+} // namespace
+
+extern "C" hipError_t
+hip_update_neurons(hipStream_t stream, const nemo_hip_neuron_group* g, const nemo_hip_step_buffers* b)
+{
+	if(g->count == 0) return hipSuccess;
+	const unsigned blocks = (g->count + NEMO_UPDATE_BLOCK - 1) / NEMO_UPDATE_BLOCK;
+	hipLaunchKernelGGL(poisson_update, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), 0, stream, *g, *b);
+	return hipGetLastError();
+}
+
+extern "C" hipError_t
+hip_init_neurons(hipStream_t, const nemo_hip_neuron_group*, const nemo_hip_step_buffers*)
+{
+	return hipSuccess;
+}

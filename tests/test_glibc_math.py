@@ -1,0 +1,80 @@
+"""The glibc-exact logf/sinf/expf of nemo_amd/csrc/glibc_math.h against the
+host C library, over every float the callers can produce:
+
+  logf(1 - b), b = float(u32) * 2^-32       (Box-Muller radius, RNG.cpp:30)
+  sinf(a),     a = float(u32) * 1.4629181e-9 (Box-Muller angle, RNG.cpp:28)
+  expf(x),     x in the HH model's range    (HH.cpp:92-98)
+
+The same header is compiled for the host here; the GPU tests run it on the
+device against the same host results.
+"""
+import os
+import subprocess
+import tempfile
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+PROGRAM = r"""
+#include <math.h>
+#include <stdio.h>
+#include <stdint.h>
+#include <string.h>
+#include "glibc_math.h"
+using namespace nemo_math;
+static uint32_t bits(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+int main() {
+    long bad_log = 0, bad_sin = 0, bad_exp = 0, n_log = 0, n_sin = 0, n_exp = 0;
+    /* every distinct float(u32): exact below 2^24, then 2^23 values per binade */
+    for (uint64_t u = 0; u < (1ull << 32); ) {
+        float f = (float)(uint32_t)u;
+        float b = f * 0.00000000023283064365386962890625f;
+        float x = 1 - b;
+        if (x > 0) { n_log++; if (bits(logf(x)) != bits(logf_exact(x))) bad_log++; }
+        float a = f * 1.4629180792671596810513378043098e-9f;
+        n_sin++; if (bits(sinf(a)) != bits(sinf_exact(a))) bad_sin++;
+        uint64_t step = u < (1ull << 24) ? 1 : (1ull << (64 - __builtin_clzll(u) - 24));
+        u += step;
+    }
+    /* all floats in [-128, 16] */
+    for (uint32_t u = 0; u < 0x41800000u; u += 1) {
+        float x; memcpy(&x, &u, 4);
+        n_exp++; if (bits(expf(x)) != bits(expf_exact(x))) bad_exp++;
+        float y = -x;
+        if (y >= -128.0f) { n_exp++; if (bits(expf(y)) != bits(expf_exact(y))) bad_exp++; }
+    }
+    printf("%ld %ld %ld %ld %ld %ld\n", bad_log, n_log, bad_sin, n_sin, bad_exp, n_exp);
+    return 0;
+}
+"""
+
+
+@pytest.fixture(scope="module")
+def results():
+    with tempfile.TemporaryDirectory() as d:
+        src = os.path.join(d, "m.cpp")
+        exe = os.path.join(d, "m")
+        open(src, "w").write(PROGRAM)
+        subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-I" + os.path.join(ROOT, "nemo_amd", "csrc"),
+                        src, "-o", exe, "-lm"], check=True)
+        out = subprocess.run([exe], check=True, stdout=subprocess.PIPE, text=True, timeout=600).stdout
+    return [int(x) for x in out.split()]
+
+
+def test_logf_matches_host_over_box_muller_domain(results):
+    bad, n = results[0], results[1]
+    assert n > 1 << 24
+    assert bad == 0
+
+
+def test_sinf_matches_host_over_box_muller_domain(results):
+    bad, n = results[2], results[3]
+    assert n > 1 << 26
+    assert bad == 0
+
+
+def test_expf_matches_host_over_hh_range(results):
+    bad, n = results[4], results[5]
+    assert n > 1 << 30
+    assert bad == 0

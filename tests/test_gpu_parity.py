@@ -1,0 +1,286 @@
+"""Parity of the HIP backend with the CPU oracle on the same seeded networks.
+
+Bar: bit-exact rasters, fixed-point currents (as the floats the models see),
+membrane state, RNG state, firing history, STDP accumulators and weights.
+Every call goes through the C ABI of libnemo_hip.so.
+"""
+import ctypes
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import nemo_amd
+from nemo_amd import generators as gen
+import oracle_api as oa
+import scenarios as sc
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def _gpu(require_gpu):
+    pass
+
+
+def both(spec, stdp=None):
+    """The same network in the oracle and in the HIP backend."""
+    sims = []
+    for api in (oa, nemo_amd):
+        conf = api.Configuration()
+        if stdp is not None:
+            conf.set_stdp_function(*stdp)
+        sims.append(sc.make(api, spec, conf))
+    return sims
+
+
+def same_bits(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    if a.dtype.kind == "f":
+        # NaN payloads differ between x86 and the GPU; any NaN equals any NaN
+        an, bn = np.isnan(a), np.isnan(b)
+        return np.array_equal(an, bn) and np.array_equal(a.view(np.uint32)[~an], b.view(np.uint32)[~bn])
+    return np.array_equal(a, b)
+
+
+def state_planes(sim, idx, nstate):
+    return [np.array([sim.get_neuron_state(int(i), v) for i in idx], np.float32) for v in range(nstate)]
+
+
+# ------------------------------------------------------------ device math
+@pytest.fixture(scope="module")
+def host_libm():
+    d = tempfile.mkdtemp()
+    src = os.path.join(d, "ref.c")
+    so = os.path.join(d, "libref.so")
+    open(src, "w").write(
+        "#include <math.h>\n#include <stddef.h>\n"
+        "void eval(int fn, const float* in, float* out, size_t n) {\n"
+        "  for (size_t i = 0; i < n; ++i) out[i] = fn == 0 ? logf(in[i]) : fn == 1 ? sinf(in[i]) : expf(in[i]);\n}\n")
+    subprocess.run(["gcc", "-O2", "-shared", "-fPIC", src, "-o", so, "-lm"], check=True)
+    l = ctypes.CDLL(so)
+    l.eval.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), ctypes.c_size_t]
+    return l
+
+
+def _eval(fn, x, dev, host_libm):
+    out = np.empty_like(x)
+    if dev:
+        from nemo_amd._lib import check, lib
+        check(lib().nemo_hip_eval_math(fn, x.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                       out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), x.size))
+    else:
+        host_libm.eval(fn, x.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                       out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), x.size)
+    return out
+
+
+def float_u32_domain():
+    parts = [np.arange(1 << 24, dtype=np.uint64)]
+    for k in range(24, 32):
+        parts.append(np.arange(1 << k, 1 << (k + 1), 1 << (k - 23), dtype=np.uint64))
+    return np.concatenate(parts).astype(np.float32)
+
+
+def test_device_logf_sinf_match_host_over_box_muller_domain(host_libm):
+    f = float_u32_domain()
+    b = (f * np.float32(2.0 ** -32)).astype(np.float32)
+    x = np.unique((np.float32(1) - b).astype(np.float32))
+    x = x[x > 0]
+    assert same_bits(_eval(0, x, True, host_libm), _eval(0, x, False, host_libm))
+    a = (f * np.float32(1.4629180792671596810513378043098e-9)).astype(np.float32)
+    assert same_bits(_eval(1, a, True, host_libm), _eval(1, a, False, host_libm))
+
+
+def test_device_expf_matches_host_over_hh_range(host_libm):
+    u = np.arange(0, 0x41800000, 7, dtype=np.uint64).astype(np.uint32)
+    x = u.view(np.float32)
+    x = np.concatenate([x, -x])
+    assert same_bits(_eval(2, x, True, host_libm), _eval(2, x, False, host_libm))
+
+
+# -------------------------------------------------------- reference KATs
+@pytest.mark.parametrize("ncount,delay", [(1000, 1), (1024, 1), (2000, 1), (4000, 1), (2000, 20), (2000, 63)])
+def test_ring(ncount, delay):
+    sc.run_ring(nemo_amd, ncount, delay)
+
+
+def test_double_ring():
+    sc.run_double_ring(nemo_amd)
+
+
+def test_fixed_point_saturation():
+    hip = sc.fixed_point_saturation(nemo_amd)
+    ref = sc.fixed_point_saturation(oa)
+    assert hip == ref
+
+
+def test_firing_stimulus():
+    sc.firing_stimulus(nemo_amd)
+
+
+def test_current_stimulus():
+    sc.current_stimulus(nemo_amd)
+
+
+def test_invalid_current_stimulus():
+    sc.invalid_current_stimulus(nemo_amd, nemo_amd.NemoError)
+
+
+@pytest.mark.parametrize("n0,nstep", [(1, 1), (1000000, 4)])
+def test_non_contiguous_indices(n0, nstep):
+    sc.non_contiguous(nemo_amd, n0, nstep, seconds=1)
+
+
+@pytest.mark.parametrize("others,iz_first", [(0, True), (1000, True), (2000, False)])
+def test_neuron_type_mixture(others, iz_first):
+    sc.neuron_type_mixture(nemo_amd, others, iz_first)
+
+
+@pytest.mark.parametrize("noise,reward", [(False, 1.0), (True, 1.0), (True, 0.9)])
+def test_stdp_closed_form(noise, reward):
+    hip = sc.stdp_closed_form(nemo_amd, noise, reward)
+    ref = sc.stdp_closed_form(oa, noise, reward)
+    for h, r in zip(hip, ref):
+        assert np.array_equal(h, r)
+
+
+# ------------------------------------------------ random-network parity
+def test_random_network_step_parity():
+    """1000 Izhikevich neurons x 100 synapses, delays 1..20, Gaussian input
+    sigma 5/2, 1000 steps (the small benchmark network): raster, currents,
+    state, RNG and history bit-exact, step by step."""
+    spec = gen.random_izhikevich(1000, 100, dmax=20, seed=11)
+    ref, hip = both(spec)
+    hip.set_debug(True)
+    assert np.array_equal(ref.local_to_global(), hip.local_to_global())
+    for t in range(1000):
+        fr = ref.step()
+        fh = hip.step()
+        assert fr == fh, ("raster diverged", t)
+        if t % 50 == 0 or t < 5:
+            er, ir = ref.currents()
+            eh, ih = hip.currents()
+            assert same_bits(er, eh) and same_bits(ir, ih), ("currents diverged", t)
+    idx = np.arange(1000)
+    for pr, ph in zip(state_planes(ref, idx, 2), state_planes(hip, idx, 2)):
+        assert same_bits(pr, ph)
+    assert np.array_equal(ref.rng_state(), hip.rng_state())
+    assert np.array_equal(ref.recent_firing(), hip.recent_firing())
+
+
+def stdp_fn():
+    pre, post = gen.stdp_exp_function(0.1, -0.12, 20.0, 20)
+    return (pre, post, 0.0, 10.0, 0.0, -10.0)
+
+
+@pytest.mark.parametrize("n,m,steps,period", [(2000, 200, 1200, 300), (5000, 100, 600, 200)])
+def test_stdp_network_run_parity(n, m, steps, period):
+    """Izhikevich network with plastic excitatory synapses (the STDP benchmark
+    rule, reduced size): the run() graph path against the oracle stepped one
+    step at a time, applyStdp(1.0) every `period` steps.  Raster, weights and
+    accumulators bit-exact."""
+    spec = gen.random_izhikevich(n, m, dmax=20, seed=5, stdp=True)
+    ref, hip = both(spec, stdp_fn())
+    cr, fr, er = ref.run(steps, stdp_period=period, reward=1.0)
+    ch, fh, eh = hip.run(steps, stdp_period=period, reward=1.0)
+    assert np.array_equal(cr, ch)
+    assert np.array_equal(fr, fh)
+    assert er == eh
+    nsyn = spec.synapse_count
+    assert np.array_equal(ref.fixed_weights(nsyn), hip.fixed_weights())
+    assert np.array_equal(ref.stdp_accumulators(nsyn), hip.stdp_accumulators())
+    assert cr.sum() > 0
+
+
+def test_run_matches_step():
+    spec = gen.random_izhikevich(3000, 150, dmax=15, seed=9, stdp=True)
+    a = sc.make(nemo_amd, spec, _conf(nemo_amd, stdp_fn()))
+    b = sc.make(nemo_amd, spec, _conf(nemo_amd, stdp_fn()))
+    ca, fa, _ = a.run(300, stdp_period=100)
+    fb = []
+    cb = []
+    for t in range(300):
+        f = b.step()
+        cb.append(len(f))
+        fb.extend(f)
+        if (t + 1) % 100 == 0:
+            b.apply_stdp(1.0)
+    assert np.array_equal(ca, cb)
+    assert np.array_equal(fa, fb)
+    assert np.array_equal(a.fixed_weights(), b.fixed_weights())
+
+
+def _conf(api, stdp):
+    c = api.Configuration()
+    c.set_stdp_function(*stdp)
+    return c
+
+
+def test_mixed_models_parity():
+    """QIF first (so the reference's QIF indexing is in bounds), then
+    IF_curr_exp, HH, Poisson sources and Input neurons, randomly connected:
+    raster and all state variables bit-exact over 300 steps."""
+    rng = np.random.default_rng(4)
+    blocks = []
+    base = 0
+
+    def block(model, n, args):
+        nonlocal base
+        idx = np.arange(base, base + n, dtype=np.uint32)
+        base += n
+        blocks.append(gen.NeuronBlock(model, idx, [np.asarray(a, np.float32) * np.ones(n, np.float32) for a in args]))
+        return idx
+
+    q = block("QIF", 300, [2.0, 0.0])
+    lif = block("IF_curr_exp", 400, [-65.0, -65.0, 1.0, 20.0, 2.0, 5.0, 5.0, -50.0, 0.9, -65.0, 0.0, 0.0, 100.0])
+    hh = block("HH", 50, [-65.0, 0.3177, 0.0529, 0.5961, 0.0])
+    po = block("PoissonSource", 100, [0.01])
+    inp = block("Input", 20, [])
+    n = base
+    m = 40
+    src = np.repeat(np.arange(n, dtype=np.uint32), m)
+    tgt = rng.integers(0, n - 120, size=src.size, dtype=np.uint32)  # no synapses onto sources
+    dly = rng.integers(1, 11, size=src.size, dtype=np.uint32)
+    w = rng.random(src.size, dtype=np.float32) * np.where(rng.random(src.size) < 0.8, 1.0, -1.0).astype(np.float32)
+    spec = gen.NetworkSpec(blocks=blocks, source=src, target=tgt, delay=dly, weight=w.astype(np.float32),
+                           plastic=np.zeros(src.size, np.uint8))
+    ref, hip = both(spec)
+    hip.set_debug(True)
+    for t in range(300):
+        fs = list(inp[rng.random(inp.size) < 0.1])
+        istim = [(int(i), 2.0) for i in lif[rng.random(lif.size) < 0.05]]
+        fr = ref.step(fstim=fs, istim=istim)
+        fh = hip.step(fstim=fs, istim=istim)
+        assert fr == fh, ("raster diverged", t)
+        er, ir = ref.currents()
+        eh, ih = hip.currents()
+        assert same_bits(er, eh) and same_bits(ir, ih), ("currents diverged", t)
+    for idx, ns in ((q, 1), (lif, 4), (hh, 5)):
+        for pr, ph in zip(state_planes(ref, idx, ns), state_planes(hip, idx, ns)):
+            assert same_bits(pr, ph)
+    assert np.array_equal(ref.rng_state(), hip.rng_state())
+
+
+def test_events_match_raster_full_size():
+    """At the full 100k x 1000 STDP size, where the oracle is too slow to step
+    through a test, check a size-independent property: events delivered equal
+    the out-degrees implied by the raster (every firing of n at step s delivers
+    row (n, d) at step s + d), and firing stays in a physiological range."""
+    n, m, D = 100000, 1000, 20
+    spec = gen.random_izhikevich(n, m, dmax=D, seed=2, stdp=True)
+    sim = sc.make(nemo_amd, spec, _conf(nemo_amd, stdp_fn()))
+    steps = 200
+    counts, fired, events = sim.run(steps, stdp_period=100)
+    rowlen = np.zeros((n, D + 1), np.int64)
+    np.add.at(rowlen, (spec.source.astype(np.int64), spec.delay.astype(np.int64)), 1)
+    cum = np.cumsum(rowlen, axis=1)  # cum[n, d] = synapses of n with delay <= d
+    step_of = np.repeat(np.arange(steps), counts)
+    horizon = np.minimum(steps - 1 - step_of, D)
+    expected = int(cum[fired.astype(np.int64), horizon].sum())
+    assert events == expected
+    rate_hz = counts.sum() / n / (steps / 1000.0)
+    assert 0.5 < rate_hz < 200
