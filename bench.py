@@ -116,9 +116,8 @@ def main():
     kern, work = sim.profile(args.profile_steps)
     steps_p = work["steps"]
     per_launch = {
-        "deliver": BYTES_PER_EVENT * work["events"] / steps_p,
+        "propagate": (BYTES_PER_EVENT * work["events"] + BYTES_PER_STDP_SYNAPSE * work["stdp_synapses"]) / steps_p,
         "update": BYTES_PER_IZH_NEURON * n,
-        "stdp": BYTES_PER_STDP_SYNAPSE * work["stdp_synapses"] / steps_p,
     }
     dominant = max(per_launch, key=lambda k: kern[k])
     achieved = per_launch[dominant] / (kern[dominant] * 1e-6) / 1e9

@@ -10,20 +10,29 @@
  * neuron_model.h:26-62); the model's .ini file gives its parameter and state
  * counts as in NeMo (src/nemo/NeuronType.cpp:93-158).
  *
+ * Neurons live in the backend's device index space: each neuron type occupies
+ * a range starting at a multiple of NEMO_TILE, and neuron i of the type has
+ * device index start + i.  One update block of NEMO_TILE threads owns one
+ * tile of NEMO_TILE consecutive device indices.
+ *
  * Contract of hip_update_neurons, for neurons [start, start+count) of one type
- * in the current step (the semantics of cpu_update_neurons,
- * src/nemo/cpu/plugins/neuron_model.h:25-51):
- *   - read and clear the accumulated Q(31-fbits).fbits currents acc[E plane]
- *     and acc[I plane], saturating each to the int32 range before converting
- *     to float (src/nemo/fixedpoint.cpp:40-49);
+ * in step s = clock->base_step + step_offset (the semantics of
+ * cpu_update_neurons, src/nemo/cpu/plugins/neuron_model.h:25-51):
+ *   - sum the step's synaptic input of the tile: the binned events of every
+ *     delivery block (bin_table / bins) plus the overflow accumulators acc,
+ *     excitatory (w >= 0) and inhibitory separately, saturate each sum to the
+ *     int32 range and convert to float (src/nemo/fixedpoint.cpp:40-49); clear
+ *     the overflow accumulators;
  *   - read and clear ext_current when it is not NULL;
  *   - apply one 1 ms step of the model; a neuron fires if the model says so or
  *     if its bit in fstim is set (fstim may be NULL);
  *   - shift the neuron's 64-bit firing history left by one, OR-ing in firing;
- *   - append each fired neuron's simulation index to the fired log at
- *     position (clock->log_head++ & log_mask), in any order.
- * All pointers are device pointers; the launch goes on the given stream and
- * must not synchronise it, so that the backend can capture it in a graph.
+ *   - write the tile's words of the step's firing bitmap,
+ *     fired_bits[(s % NEMO_STEP_RING) * fired_words + index / 64].
+ * The helpers in nemo_amd/csrc/plugin_device.h implement all of this but the
+ * model step.  All pointers are device pointers; the launch goes on the given
+ * stream and must not synchronise it, so that the backend can capture it in a
+ * graph.
  */
 #ifndef NEMO_HIP_PLUGIN_H
 #define NEMO_HIP_PLUGIN_H
@@ -36,19 +45,25 @@
 extern "C" {
 #endif
 
-/* Device-resident step clock, advanced by the backend at the end of a step. */
+/* Neurons per tile (and threads per update block). */
+#define NEMO_TILE 256
+/* Steps of firing bitmaps kept (power of two, > maximum delay + STDP window). */
+#define NEMO_STEP_RING 128
+/* Most delivery blocks (bin regions) a step's input can come from. */
+#define NEMO_MAX_BIN_BLOCKS 1024
+
+/* Device-resident step base, set by the host before launches. */
 typedef struct nemo_hip_clock {
-	uint64_t step;     /* steps completed since construction (monotonic) */
-	uint64_t cycle;    /* simulated ms since the last timer reset (Timer, src/nemo/Timer.hpp:23-79) */
-	uint64_t log_head; /* fired-log entries appended since construction */
-	uint64_t events;   /* synaptic events delivered since construction */
-	uint64_t stdp_synapses; /* plastic synapses examined by STDP accumulation */
+	uint64_t base_step;    /* added to each launch's step_offset */
+	uint64_t cycle_origin; /* step of the last timer reset: cycle = step - cycle_origin
+	                          (Timer, src/nemo/Timer.hpp:23-79) */
+	uint64_t stdp_flushed; /* step whose STDP accumulation already ran on its own (~0: none) */
 } nemo_hip_clock;
 
 /* One neuron type's state: structure of arrays, plane p of a quantity starts
  * at element p*pitch. */
 typedef struct nemo_hip_neuron_group {
-	unsigned start;    /* simulation index of the first neuron */
+	unsigned start;    /* device index of the first neuron (multiple of NEMO_TILE) */
 	unsigned count;    /* neurons in the group */
 	size_t pitch;      /* elements between planes */
 	float* param;      /* nParam planes */
@@ -58,16 +73,21 @@ typedef struct nemo_hip_neuron_group {
 
 /* Per-simulation buffers shared by all groups in a step. */
 typedef struct nemo_hip_step_buffers {
-	int64_t* acc;          /* accumulated currents: E plane [0, acc_pitch), I plane after it */
+	uint64_t step_offset;      /* step = clock->base_step + step_offset */
+	const nemo_hip_clock* clock;
+	const uint32_t* bin_table; /* [tile][bin_blocks]: count << 16 | offset */
+	const int2* bins;          /* [bin_blocks][bin_capacity] {target % NEMO_TILE, Q weight} */
+	unsigned bin_blocks;
+	unsigned bin_capacity;
+	int64_t* acc;              /* overflow accumulators: E plane [0, acc_pitch), then I plane */
 	size_t acc_pitch;
-	float* ext_current;    /* per simulation index, or NULL */
-	const uint32_t* fstim; /* firing-stimulus bit-vector, or NULL */
-	uint64_t* hist;        /* 64-bit firing history per simulation index */
-	uint32_t* fired_log;   /* circular log of fired simulation indices */
-	uint64_t log_mask;     /* log capacity - 1 (capacity is a power of two) */
-	nemo_hip_clock* clock;
-	unsigned fbits;        /* fractional bits of the fixed-point currents */
-	float* debug_current;  /* NULL, or 2 planes of acc_pitch floats receiving E and I */
+	float* ext_current;        /* per device index, or NULL */
+	const uint32_t* fstim;     /* firing-stimulus bit-vector, or NULL */
+	uint64_t* hist;            /* 64-bit firing history per device index */
+	uint64_t* fired_bits;      /* NEMO_STEP_RING bitmaps of fired_words words */
+	size_t fired_words;
+	unsigned fbits;            /* fractional bits of the fixed-point currents */
+	float* debug_current;      /* NULL, or 2 planes of acc_pitch floats receiving E and I */
 } nemo_hip_step_buffers;
 
 typedef hipError_t hip_update_neurons_t(hipStream_t stream,

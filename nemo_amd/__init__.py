@@ -336,12 +336,14 @@ class Simulation(_SynapseQueries):
         check(lib().nemo_reset_timer(self._h))
 
     def profile(self, nsteps):
-        """Per-phase mean kernel time (us) over nsteps eager steps, measured
-        with HIP events on the simulation's stream, and the work done."""
+        """Mean time (us) of the two launches of a step -- "propagate"
+        (delivery of the step and STDP accumulation of the previous one) and
+        "update" (all neuron models) -- over nsteps eager steps, measured with
+        HIP events on the simulation's stream, and the work done."""
         avg = (ctypes.c_double * 4)()
         cnt = (ctypes.c_uint64 * 4)()
         check(lib().nemo_hip_profile(self._h, nsteps, avg, cnt))
-        keys = ("deliver", "update", "stdp", "advance")
+        keys = ("propagate", "update")
         return ({k: avg[i] for i, k in enumerate(keys)},
                 {"events": cnt[0], "fired": cnt[1], "stdp_synapses": cnt[2], "steps": cnt[3]})
 

@@ -54,6 +54,7 @@ template struct DeviceArray<int64_t>;
 template struct DeviceArray<uint64_t>;
 template struct DeviceArray<int2>;
 template struct DeviceArray<nemo_hip_clock>;
+template struct DeviceArray<unsigned long long>;
 
 /* ----------------------------------------------------- fixed point (Q11.20) */
 
@@ -431,8 +432,7 @@ Simulation::Simulation(const Network& net, const Configuration& conf)
 	m_logging = conf.logging;
 	activate();
 	HIP_CHECK(hipStreamCreateWithFlags(&m_stream, hipStreamNonBlocking));
-	HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&m_pinnedClock), sizeof(nemo_hip_clock), hipHostMallocDefault));
-	HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&m_pinnedStepStart), NEMO_STEP_RING * sizeof(uint64_t),
+	HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&m_pinnedScalars), SCALAR_SLOTS * sizeof(uint64_t),
 				hipHostMallocDefault));
 
 	buildNeurons(net);
@@ -441,14 +441,16 @@ Simulation::Simulation(const Network& net, const Configuration& conf)
 	allocateStepState();
 	chooseLaunchShapes();
 
-	nemo_hip_step_buffers b = stepBuffers(false, false);
+	nemo_hip_step_buffers b = stepBuffers(0, false, false);
 	for(Group& g : m_groups) {
 		HIP_CHECK(g.plugin->init(m_stream, &g.dev, &b));
 	}
 	HIP_CHECK(hipStreamSynchronize(m_stream));
 	if(m_logging) {
-		std::printf("nemo-hip: %zu neurons, %zu synapses (%zu plastic), max delay %u, Q%u.%u\n",
-				m_ncount, m_nsynapses, m_nplastic, m_D, 31 - m_fbits, m_fbits);
+		std::printf("nemo-hip: %zu neurons (%u tiles), %zu synapses (%zu plastic), max delay %u, Q%u.%u, "
+				"delivery %u blocks x %u events, STDP %u blocks\n",
+				m_ncount, m_tiles, m_nsynapses, m_nplastic, m_D, 31 - m_fbits, m_fbits,
+				m_binBlocks, m_binCap, m_stdpBlocks);
 	}
 	resetTimer();
 }
@@ -461,9 +463,8 @@ Simulation::~Simulation()
 		(void) hipStreamSynchronize(m_stream);
 		(void) hipStreamDestroy(m_stream);
 	}
-	if(m_pinnedClock) (void) hipHostFree(m_pinnedClock);
-	if(m_pinnedStepStart) (void) hipHostFree(m_pinnedStepStart);
-	if(m_pinnedLog) (void) hipHostFree(m_pinnedLog);
+	if(m_pinnedScalars) (void) hipHostFree(m_pinnedScalars);
+	if(m_pinnedBits) (void) hipHostFree(m_pinnedBits);
 }
 
 void Simulation::activate() const
@@ -485,15 +486,25 @@ Simulation::Group& Simulation::groupOf(unsigned l)
 	return m_groups.at(m_typeOfLocal.at(l));
 }
 
-/* Simulation indices: type by type (addNeuronType order), insertion order
- * within a type (cpu/Simulation.cpp:60-77, cpu/Neurons.cpp:27-40).  RNG
- * seeds: mt19937 default seed, four words >> 1 per simulation index
- * (RNG.cpp:38-63, seeded by local index as cpu/Neurons.cpp:41 does). */
+/* Simulation order: type by type (addNeuronType order), insertion order within
+ * a type (cpu/Simulation.cpp:60-77, cpu/Neurons.cpp:27-40).  On the device
+ * each type starts at a multiple of NEMO_TILE, so that one update block owns
+ * one tile of one type.  RNG seeds: mt19937 default seed, four words >> 1 per
+ * simulation index (RNG.cpp:38-63, seeded by the CPU backend's local index,
+ * cpu/Neurons.cpp:41). */
 void Simulation::buildNeurons(const Network& net)
 {
 	m_ncount = net.neuronCount();
-	m_l2g.resize(m_ncount);
-	m_typeOfLocal.resize(m_ncount);
+	size_t npad = 0;
+	for(unsigned t = 0; t < net.types.size(); ++t) {
+		const size_t c = net.groups[t].gidx.size();
+		if(c) npad += (c + NEMO_TILE - 1) / NEMO_TILE * NEMO_TILE;
+	}
+	m_npad = npad;
+	m_tiles = unsigned(npad / NEMO_TILE);
+	m_l2g.assign(m_npad, ~0u);
+	m_typeOfLocal.assign(m_npad, -1);
+	m_order.resize(m_ncount);
 	m_g2l.reserve(m_ncount * 2);
 
 	std::vector<uint32_t> seeds(4 * m_ncount);
@@ -502,7 +513,8 @@ void Simulation::buildNeurons(const Network& net)
 		for(size_t i = 0; i < seeds.size(); ++i) seeds[i] = uint32_t(mt()) >> 1;
 	}
 
-	unsigned base = 0;
+	unsigned start = 0; /* device index */
+	size_t sim = 0;     /* simulation index */
 	for(unsigned t = 0; t < net.types.size(); ++t) {
 		const Network::Group& ng = net.groups[t];
 		if(ng.gidx.empty()) continue;
@@ -510,12 +522,15 @@ void Simulation::buildNeurons(const Network& net)
 		Group& g = m_groups.back();
 		g.type = net.types[t];
 		g.plugin.reset(new Plugin(g.type));
-		g.start = base;
+		g.start = start;
 		g.count = unsigned(ng.gidx.size());
 		for(unsigned i = 0; i < g.count; ++i) {
-			m_l2g[base + i] = ng.gidx[i];
-			m_g2l[ng.gidx[i]] = base + i;
-			m_typeOfLocal[base + i] = unsigned(m_groups.size() - 1);
+			if(!m_g2l.emplace(ng.gidx[i], start + i).second) {
+				throw Error(NEMO_INVALID_INPUT, "Duplicate neuron index " + std::to_string(ng.gidx[i]));
+			}
+			m_l2g[start + i] = ng.gidx[i];
+			m_typeOfLocal[start + i] = int(m_groups.size() - 1);
+			m_order[sim + i] = start + i;
 		}
 		const size_t pitch = g.count;
 		std::vector<float> param(std::max<size_t>(1, g.type.nParam * pitch));
@@ -531,14 +546,15 @@ void Simulation::buildNeurons(const Network& net)
 		g.rng.allocate(4 * size_t(g.count));
 		HIP_CHECK(hipMemcpy(g.param.ptr, param.data(), g.param.bytes(), hipMemcpyHostToDevice));
 		HIP_CHECK(hipMemcpy(g.state.ptr, state.data(), g.state.bytes(), hipMemcpyHostToDevice));
-		HIP_CHECK(hipMemcpy(g.rng.ptr, seeds.data() + 4 * size_t(base), g.rng.bytes(), hipMemcpyHostToDevice));
+		HIP_CHECK(hipMemcpy(g.rng.ptr, seeds.data() + 4 * sim, g.rng.bytes(), hipMemcpyHostToDevice));
 		g.dev.start = g.start;
 		g.dev.count = g.count;
 		g.dev.pitch = pitch;
 		g.dev.param = g.param.ptr;
 		g.dev.state = g.state.ptr;
 		g.dev.rng = g.rng.ptr;
-		base += g.count;
+		start += (g.count + NEMO_TILE - 1) / NEMO_TILE * NEMO_TILE;
+		sim += g.count;
 	}
 }
 
@@ -551,7 +567,7 @@ void Simulation::buildConnectivity(const Network& net, const Configuration& conf
 {
 	m_D = std::max(1u, net.maxDelay);
 	const unsigned D = m_D;
-	const size_t N = m_ncount;
+	const size_t N = m_npad;
 	if(N >= (size_t(1) << 26)) {
 		throw Error(NEMO_API_UNSUPPORTED, "the HIP backend supports at most 2^26 neurons per simulation");
 	}
@@ -570,7 +586,7 @@ void Simulation::buildConnectivity(const Network& net, const Configuration& conf
 	m_nsynapses = seq;
 	const long ns = long(srcs.size());
 
-	/* local target index of every synapse, checked (verifySynapseTerminals) */
+	/* device target index of every synapse, checked (verifySynapseTerminals) */
 	std::vector<uint32_t> ltgt(m_nsynapses);
 	int bad = 0;
 	unsigned badIdx = 0;
@@ -604,8 +620,8 @@ void Simulation::buildConnectivity(const Network& net, const Configuration& conf
 
 	/* fill rows (stable within a source) */
 	std::vector<int2> fcm(std::max<size_t>(1, m_nsynapses));
+	m_synAddr.resize(m_nsynapses);
 	if(!m_writeOnly) {
-		m_synAddr.resize(m_nsynapses);
 		m_synTarget.resize(m_nsynapses);
 		m_synDelay.resize(m_nsynapses);
 		m_synPlastic.resize(m_nsynapses);
@@ -630,8 +646,8 @@ void Simulation::buildConnectivity(const Network& net, const Configuration& conf
 			}
 			fcm[pos].x = int(ltgt[s.seq + k]);
 			fcm[pos].y = wf;
+			m_synAddr[s.seq + k] = pos;
 			if(!m_writeOnly) {
-				m_synAddr[s.seq + k] = pos;
 				m_synTarget[s.seq + k] = s.ax->target[k];
 				m_synDelay[s.seq + k] = uint8_t(d);
 				m_synPlastic[s.seq + k] = s.ax->plastic[k];
@@ -649,6 +665,9 @@ void Simulation::buildConnectivity(const Network& net, const Configuration& conf
 	HIP_CHECK(hipMemcpy(m_rowptr.ptr, rowptr.data(), m_rowptr.bytes(), hipMemcpyHostToDevice));
 	m_fcm.allocate(fcm.size());
 	HIP_CHECK(hipMemcpy(m_fcm.ptr, fcm.data(), m_fcm.bytes(), hipMemcpyHostToDevice));
+	size_t rows = 0;
+	for(size_t r = 0; r < N * D; ++r) rows += rowptr[r + 1] != rowptr[r];
+	m_nonEmptyRows = rows;
 	if(!m_writeOnly) {
 		m_fcmHost.swap(fcm);
 		m_fcmHostValid = true;
@@ -705,6 +724,9 @@ void Simulation::buildConnectivity(const Network& net, const Configuration& conf
 			}
 		}
 	}
+	if(m_writeOnly) {
+		std::vector<uint32_t>().swap(m_synAddr);
+	}
 	m_rcmPtr.allocate(rcmPtr.size());
 	HIP_CHECK(hipMemcpy(m_rcmPtr.ptr, rcmPtr.data(), m_rcmPtr.bytes(), hipMemcpyHostToDevice));
 	if(m_stdp) {
@@ -721,15 +743,15 @@ void Simulation::buildConnectivity(const Network& net, const Configuration& conf
  * (StdpFunction.cpp:72-108): fixed-point tables and window bit masks. */
 void Simulation::buildStdp(const Configuration& conf)
 {
-	std::memset(&m_stdpArgs, 0, sizeof(m_stdpArgs));
+	std::memset(&m_propagate, 0, sizeof(m_propagate));
 	if(!m_stdp) return;
 	const unsigned fbits = m_fbits;
 	m_minE = fxToFix(conf.minE, fbits);
 	m_maxE = fxToFix(conf.maxE, fbits);
 	m_minI = fxToFix(conf.minI, fbits);
 	m_maxI = fxToFix(conf.maxI, fbits);
-	for(size_t i = 0; i < conf.prefire.size(); ++i) m_stdpArgs.fnPre[i] = fxToFix(conf.prefire[i], fbits);
-	for(size_t i = 0; i < conf.postfire.size(); ++i) m_stdpArgs.fnPost[i] = fxToFix(conf.postfire[i], fbits);
+	for(size_t i = 0; i < conf.prefire.size(); ++i) m_propagate.fnPre[i] = fxToFix(conf.prefire[i], fbits);
+	for(size_t i = 0; i < conf.postfire.size(); ++i) m_propagate.fnPost[i] = fxToFix(conf.postfire[i], fbits);
 	uint64_t pot = 0, dep = 0;
 	unsigned n = 0;
 	for(auto f = conf.postfire.rbegin(); f != conf.postfire.rend(); ++f, ++n) {
@@ -743,31 +765,95 @@ void Simulation::buildStdp(const Configuration& conf)
 	const unsigned pre = unsigned(conf.prefire.size());
 	const unsigned post = unsigned(conf.postfire.size());
 	auto lowMask = [](unsigned k) { return k >= 64 ? ~uint64_t(0) : ((uint64_t(1) << k) - 1); };
-	m_stdpArgs.postPreWindow = post;
-	m_stdpArgs.potentiationBits = pot;
-	m_stdpArgs.depressionBits = dep;
-	m_stdpArgs.prePostBits = post >= 64 ? 0 : (lowMask(pre) << post);
-	m_stdpArgs.postPreBits = lowMask(post);
+	m_propagate.postPreWindow = post;
+	m_propagate.potentiationBits = pot;
+	m_propagate.depressionBits = dep;
+	m_propagate.prePostBits = post >= 64 ? 0 : (lowMask(pre) << post);
+	m_propagate.postPreBits = lowMask(post);
+}
+
+static unsigned pow2AtLeast(double x, unsigned lo, unsigned hi)
+{
+	unsigned v = lo;
+	while(v < hi && double(v) < x) v <<= 1;
+	return v;
+}
+
+/* Delivery shape: bin capacity from the LDS the device offers, enough blocks
+ * for about 1% of the neurons firing per step at half capacity, lanes per
+ * row about a quarter of the mean row length (each lane keeps four loads in
+ * flight).  Environment variables override each choice (for tuning). */
+void Simulation::chooseLaunchShapes()
+{
+	int maxLds = 65536;
+	(void) hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, m_device);
+	/* static LDS of the propagate kernel is about 14 KB */
+	const size_t budget = size_t(maxLds) - 16384;
+	unsigned cap = 4096;
+	while(cap > 256 && nemo_propagate_lds_bytes(m_tiles, cap) > budget) cap >>= 1;
+	if(nemo_propagate_lds_bytes(m_tiles, cap) > budget) {
+		throw Error(NEMO_API_UNSUPPORTED, "network too large for the delivery kernel's LDS");
+	}
+	m_binCap = cap;
+	const double eventsGuess = 0.01 * double(m_nsynapses);
+	const uint64_t pairs = uint64_t(m_D) * m_tiles;
+	double blocks = std::ceil(eventsGuess / (0.5 * cap));
+	blocks = std::max(1.0, std::min({blocks, double(NEMO_MAX_BIN_BLOCKS), double(pairs)}));
+	m_binBlocks = unsigned(blocks);
+	m_stdpWords = 2;
+	if(const char* e = std::getenv("NEMO_HIP_DELIVER_BLOCKS")) {
+		m_binBlocks = unsigned(std::max(1, std::min(std::atoi(e), int(NEMO_MAX_BIN_BLOCKS))));
+	}
+	if(const char* e = std::getenv("NEMO_HIP_BIN_CAPACITY")) {
+		unsigned c = unsigned(std::atoi(e));
+		if(c >= 64 && c <= 32768 && nemo_propagate_lds_bytes(m_tiles, c) <= budget) m_binCap = c;
+	}
+	if(const char* e = std::getenv("NEMO_HIP_STDP_WORDS")) {
+		m_stdpWords = unsigned(std::max(1, std::min(std::atoi(e), 4)));
+	}
+	m_stdpBlocks = (m_stdp && m_nplastic) ? unsigned((m_firedWords + m_stdpWords - 1) / m_stdpWords) : 0;
+
+	m_bins.allocate(size_t(m_binBlocks) * m_binCap);
+	m_binTable.allocate(size_t(m_tiles) * m_binBlocks);
+	HIP_CHECK(hipMemset(m_binTable.ptr, 0, m_binTable.bytes()));
+	m_blockEvents.allocate(m_binBlocks);
+	HIP_CHECK(hipMemset(m_blockEvents.ptr, 0, m_blockEvents.bytes()));
+	m_blockExamined.allocate(std::max(1u, m_stdpBlocks));
+	HIP_CHECK(hipMemset(m_blockExamined.ptr, 0, m_blockExamined.bytes()));
+
+	nemo_propagate_args& a = m_propagate;
+	a.clock = m_clock.ptr;
+	a.step_offset = 0;
+	a.fired_bits = m_firedBits.ptr;
+	a.fired_words = m_firedWords;
+	a.deliver_blocks = m_nsynapses ? m_binBlocks : 0;
+	a.rowptr = m_rowptr.ptr;
+	a.fcm = m_fcm.ptr;
+	a.D = m_D;
+	a.tiles = m_tiles;
+	a.bin_capacity = m_binCap;
+	a.bin_table = m_binTable.ptr;
+	a.bins = m_bins.ptr;
+	a.acc = m_acc.ptr;
+	a.acc_pitch = m_accPitch;
+	a.block_events = m_blockEvents.ptr;
+	a.stdp_words = m_stdpBlocks ? m_stdpWords : 0;
+	a.block_examined = m_blockExamined.ptr;
 }
 
 void Simulation::allocateStepState()
 {
-	const size_t N = m_ncount;
+	const size_t N = m_npad;
 	m_clock.allocate(1);
-	HIP_CHECK(hipMemset(m_clock.ptr, 0, m_clock.bytes()));
-	m_stepStart.allocate(NEMO_STEP_RING);
-	HIP_CHECK(hipMemset(m_stepStart.ptr, 0, m_stepStart.bytes()));
-
-	/* The log must hold every firing that delivery (steps t-1..t-D), STDP
-	 * (step t-P) and an undrained run chunk can still need, even if every
-	 * neuron fires every step. */
-	const size_t window = std::max<size_t>(m_D, m_stdpArgs.postPreWindow + 1) + 2 + 64;
-	size_t cap = 1;
-	while(cap < window * N) cap <<= 1;
-	m_firedLog.allocate(cap);
-	m_logMask = cap - 1;
-
-	m_accPitch = (N + 63) & ~size_t(63);
+	nemo_hip_clock c0;
+	c0.base_step = 0;
+	c0.cycle_origin = 0;
+	c0.stdp_flushed = ~uint64_t(0);
+	HIP_CHECK(hipMemcpy(m_clock.ptr, &c0, sizeof(c0), hipMemcpyHostToDevice));
+	m_firedWords = N / 64;
+	m_firedBits.allocate(size_t(NEMO_STEP_RING) * m_firedWords);
+	HIP_CHECK(hipMemset(m_firedBits.ptr, 0, m_firedBits.bytes()));
+	m_accPitch = N;
 	m_acc.allocate(2 * m_accPitch);
 	HIP_CHECK(hipMemset(m_acc.ptr, 0, m_acc.bytes()));
 	m_hist.allocate(N);
@@ -778,157 +864,114 @@ void Simulation::allocateStepState()
 	HIP_CHECK(hipMemset(m_fstim.ptr, 0, m_fstim.bytes()));
 	m_debugCurrent.allocate(2 * m_accPitch);
 	HIP_CHECK(hipMemset(m_debugCurrent.ptr, 0, m_debugCurrent.bytes()));
+	HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&m_pinnedBits),
+				size_t(NEMO_STEP_RING) * m_firedWords * sizeof(uint64_t), hipHostMallocDefault));
 
-	m_stdpArgs.clock = m_clock.ptr;
-	m_stdpArgs.step_start = m_stepStart.ptr;
-	m_stdpArgs.fired_log = m_firedLog.ptr;
-	m_stdpArgs.log_mask = m_logMask;
-	m_stdpArgs.rcm_ptr = m_rcmPtr.ptr;
-	m_stdpArgs.rcm_sd = m_rcmSd.ptr;
-	m_stdpArgs.rcm_acc = m_rcmAcc.ptr;
-	m_stdpArgs.hist = m_hist.ptr;
+	m_propagate.rcm_ptr = m_rcmPtr.ptr;
+	m_propagate.rcm_sd = m_rcmSd.ptr;
+	m_propagate.rcm_acc = m_rcmAcc.ptr;
+	m_propagate.hist = m_hist.ptr;
 }
 
-static unsigned pow2AtLeast(double x, unsigned lo, unsigned hi)
-{
-	unsigned v = lo;
-	while(v < hi && double(v) < x) v <<= 1;
-	return v;
-}
-
-/* Lanes per synapse row: about a quarter of the mean row length (each lane
- * keeps four loads in flight); enough blocks for a few percent of the
- * neurons firing per step. */
-void Simulation::chooseLaunchShapes()
-{
-	size_t rows = 0;
-	if(m_nsynapses) {
-		std::vector<uint32_t> rp(m_rowptr.n);
-		HIP_CHECK(hipMemcpy(rp.data(), m_rowptr.ptr, m_rowptr.bytes(), hipMemcpyDeviceToHost));
-		for(size_t r = 0; r + 1 < rp.size(); ++r) rows += rp[r + 1] != rp[r];
-	}
-	const double meanRow = rows ? double(m_nsynapses) / double(rows) : 1.0;
-	m_deliverLanes = pow2AtLeast(meanRow / 4.0, 1, 64);
-	const double expectedRows = std::max(1.0, 0.05 * double(m_ncount) * double(rows) / double(std::max<size_t>(1, m_ncount)));
-	const double groupsPerBlock = 256.0 / m_deliverLanes;
-	m_deliverBlocks = unsigned(std::min(4096.0, std::max(1.0, std::ceil(expectedRows / groupsPerBlock))));
-
-	const double meanIn = m_ncount ? double(m_nplastic) / double(m_ncount) : 1.0;
-	m_stdpLanes = pow2AtLeast(meanIn / 4.0, 1, 64);
-	const double expectedTargets = std::max(1.0, 0.05 * double(m_ncount));
-	m_stdpBlocks = unsigned(std::min(4096.0, std::max(1.0, std::ceil(expectedTargets / (256.0 / m_stdpLanes)))));
-	if(const char* e = std::getenv("NEMO_HIP_DELIVER_LANES")) m_deliverLanes = unsigned(std::atoi(e));
-	if(const char* e = std::getenv("NEMO_HIP_DELIVER_BLOCKS")) m_deliverBlocks = unsigned(std::atoi(e));
-	if(const char* e = std::getenv("NEMO_HIP_STDP_LANES")) m_stdpLanes = unsigned(std::atoi(e));
-	if(const char* e = std::getenv("NEMO_HIP_STDP_BLOCKS")) m_stdpBlocks = unsigned(std::atoi(e));
-}
-
-nemo_hip_step_buffers Simulation::stepBuffers(bool ext, bool fstim) const
+nemo_hip_step_buffers Simulation::stepBuffers(uint64_t offset, bool ext, bool fstim) const
 {
 	nemo_hip_step_buffers b;
+	b.step_offset = offset;
+	b.clock = m_clock.ptr;
+	b.bin_table = m_binTable.ptr;
+	b.bins = m_bins.ptr;
+	b.bin_blocks = m_nsynapses ? m_binBlocks : 0;
+	b.bin_capacity = m_binCap;
 	b.acc = m_acc.ptr;
 	b.acc_pitch = m_accPitch;
 	b.ext_current = ext ? m_ext.ptr : nullptr;
 	b.fstim = fstim ? m_fstim.ptr : nullptr;
 	b.hist = m_hist.ptr;
-	b.fired_log = m_firedLog.ptr;
-	b.log_mask = m_logMask;
-	b.clock = m_clock.ptr;
+	b.fired_bits = m_firedBits.ptr;
+	b.fired_words = m_firedWords;
 	b.fbits = m_fbits;
 	b.debug_current = m_debug ? m_debugCurrent.ptr : nullptr;
 	return b;
 }
 
-/* One step on the stream: deliver (prefire), neuron updates (fire), STDP
- * (postfire), advance the clock (SimulationBackend.cpp:22-77). */
-void Simulation::enqueueStep(bool ext, bool fstim, hipEvent_t* ev)
+/* Copy a host scalar to the device on the stream.  The source is a fresh
+ * pinned slot of a ring, so it stays untouched until the stream has passed
+ * the copy (the stream is synchronised at least once per SCALAR_SLOTS copies:
+ * every step, run chunk and flush synchronises). */
+void Simulation::copyScalar(uint64_t* dst, uint64_t value)
+{
+	uint64_t* slot = &m_pinnedScalars[m_scalarNext++ % SCALAR_SLOTS];
+	*slot = value;
+	HIP_CHECK(hipMemcpyAsync(dst, slot, sizeof(uint64_t), hipMemcpyHostToDevice, m_stream));
+}
+
+/* The step base used by the next launches: step = base + launch offset. */
+void Simulation::setBase(uint64_t base)
+{
+	copyScalar(&m_clock.ptr->base_step, base);
+}
+
+/* One step on the stream (SimulationBackend.cpp:22-77): the propagate kernel
+ * (delivery of this step; STDP accumulation of the previous step, which only
+ * needs the firing history its update left), then the neuron updates.  The
+ * step's own STDP accumulation runs with the next step, or on its own when
+ * flushStdp() needs it earlier.  With ev, HIP events bracket the two
+ * launches. */
+void Simulation::enqueueStep(uint64_t offset, bool ext, bool fstim, hipEvent_t* ev)
 {
 	if(ev) HIP_CHECK(hipEventRecord(ev[0], m_stream));
-	nemo_deliver_args d;
-	d.clock = m_clock.ptr;
-	d.step_start = m_stepStart.ptr;
-	d.fired_log = m_firedLog.ptr;
-	d.log_mask = m_logMask;
-	d.rowptr = m_rowptr.ptr;
-	d.fcm = m_fcm.ptr;
-	d.acc = m_acc.ptr;
-	d.acc_pitch = m_accPitch;
-	d.D = m_D;
-	if(m_nsynapses) {
-		HIP_CHECK(nemo_launch_deliver(m_stream, &d, m_deliverLanes, m_deliverBlocks));
-	}
+	nemo_propagate_args a = m_propagate;
+	a.step_offset = offset;
+	HIP_CHECK(nemo_launch_propagate(m_stream, &a, m_stdpBlocks));
 	if(ev) HIP_CHECK(hipEventRecord(ev[1], m_stream));
-	nemo_hip_step_buffers b = stepBuffers(ext, fstim);
+	nemo_hip_step_buffers b = stepBuffers(offset, ext, fstim);
 	for(Group& g : m_groups) {
 		HIP_CHECK(g.plugin->update(m_stream, &g.dev, &b));
 	}
 	if(ev) HIP_CHECK(hipEventRecord(ev[2], m_stream));
-	if(m_stdp && m_nplastic) {
-		HIP_CHECK(nemo_launch_stdp(m_stream, &m_stdpArgs, m_stdpLanes, m_stdpBlocks));
-	}
-	if(ev) HIP_CHECK(hipEventRecord(ev[3], m_stream));
-	HIP_CHECK(nemo_launch_advance(m_stream, m_clock.ptr, m_stepStart.ptr));
-	if(ev) HIP_CHECK(hipEventRecord(ev[4], m_stream));
-	m_hostStep += 1;
 }
 
-/* Eager steps with HIP events at every phase boundary on the simulation's
- * stream: average microseconds of delivery, neuron update (all groups),
- * STDP accumulation and the clock advance, and the work counters of the
- * profiled steps (events delivered, neurons fired, plastic synapses examined,
- * steps).  The steps are real simulation steps; their firing is not returned. */
-void Simulation::profile(unsigned nsteps, double avgUs[4], uint64_t counters[4])
+/* Run the pending STDP accumulation of the last step on its own, and tell the
+ * next propagate launch to skip it. */
+void Simulation::flushStdp()
 {
-	activate();
+	if(!m_stdpBlocks || m_hostStep == 0 || m_flushed == m_hostStep - 1) return;
+	const uint64_t s = m_hostStep - 1;
+	nemo_propagate_args a = m_propagate;
+	a.deliver_blocks = 0;
+	a.step_offset = 0;
+	setBase(s + 1);
+	HIP_CHECK(nemo_launch_propagate(m_stream, &a, m_stdpBlocks));
+	copyScalar(&m_clock.ptr->stdp_flushed, s);
+	m_flushed = s;
 	HIP_CHECK(hipStreamSynchronize(m_stream));
-	HIP_CHECK(hipMemcpy(m_pinnedClock, m_clock.ptr, sizeof(nemo_hip_clock), hipMemcpyDeviceToHost));
-	const nemo_hip_clock c0 = *m_pinnedClock;
-	std::vector<hipEvent_t> ev(5 * size_t(nsteps));
-	for(hipEvent_t& e : ev) HIP_CHECK(hipEventCreate(&e));
-	for(unsigned s = 0; s < nsteps; ++s) enqueueStep(false, false, &ev[5 * size_t(s)]);
-	HIP_CHECK(hipStreamSynchronize(m_stream));
-	double sum[4] = {0, 0, 0, 0};
-	for(unsigned s = 0; s < nsteps; ++s) {
-		for(int k = 0; k < 4; ++k) {
-			float ms = 0;
-			HIP_CHECK(hipEventElapsedTime(&ms, ev[5 * size_t(s) + k], ev[5 * size_t(s) + k + 1]));
-			sum[k] += ms;
+}
+
+/* Neurons (user indices) set in the firing bitmap of one step, in simulation
+ * order (cpu/Simulation.cpp:200-209). */
+void Simulation::decodeFiring(const uint64_t* bits, std::vector<unsigned>& out) const
+{
+	for(size_t w = 0; w < m_firedWords; ++w) {
+		uint64_t word = bits[w];
+		while(word) {
+			const unsigned bit = unsigned(__builtin_ctzll(word));
+			word &= word - 1;
+			out.push_back(m_l2g[w * 64 + bit]);
 		}
 	}
-	for(hipEvent_t& e : ev) (void) hipEventDestroy(e);
-	HIP_CHECK(hipMemcpy(m_pinnedClock, m_clock.ptr, sizeof(nemo_hip_clock), hipMemcpyDeviceToHost));
-	const nemo_hip_clock c1 = *m_pinnedClock;
-	for(int k = 0; k < 4; ++k) avgUs[k] = nsteps ? 1000.0 * sum[k] / nsteps : 0.0;
-	counters[0] = c1.events - c0.events;
-	counters[1] = c1.log_head - c0.log_head;
-	counters[2] = c1.stdp_synapses - c0.stdp_synapses;
-	counters[3] = nsteps;
-	m_drainedHead = c1.log_head;
 }
 
-void Simulation::ensurePinnedLog(size_t n)
+/* Copy the bitmaps of steps [t0, t0+k) (k <= NEMO_STEP_RING) to m_pinnedBits. */
+void Simulation::readBitmaps(uint64_t t0, unsigned k)
 {
-	if(n <= m_pinnedLogCap) return;
-	if(m_pinnedLog) HIP_CHECK(hipHostFree(m_pinnedLog));
-	m_pinnedLogCap = std::max<size_t>(n, 4096);
-	HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&m_pinnedLog), m_pinnedLogCap * sizeof(uint32_t),
-				hipHostMallocDefault));
-}
-
-/* Copy log positions [from, to) to out (wrapping around the circular log). */
-void Simulation::readLogRange(uint64_t from, uint64_t to, uint32_t* out)
-{
-	const uint64_t cap = m_logMask + 1;
-	if(to - from > cap) {
-		throw Error(NEMO_BUFFER_OVERFLOW, "fired log overrun");
-	}
-	uint64_t pos = from;
-	while(pos < to) {
-		const uint64_t off = pos & m_logMask;
-		const uint64_t len = std::min<uint64_t>(to - pos, cap - off);
-		HIP_CHECK(hipMemcpyAsync(out + (pos - from), m_firedLog.ptr + off, len * sizeof(uint32_t),
-					hipMemcpyDeviceToHost, m_stream));
-		pos += len;
+	const size_t W = m_firedWords;
+	unsigned done = 0;
+	while(done < k) {
+		const uint64_t slot = (t0 + done) & (NEMO_STEP_RING - 1);
+		const unsigned len = unsigned(std::min<uint64_t>(k - done, NEMO_STEP_RING - slot));
+		HIP_CHECK(hipMemcpyAsync(m_pinnedBits + size_t(done) * W, m_firedBits.ptr + slot * W,
+					size_t(len) * W * sizeof(uint64_t), hipMemcpyDeviceToHost, m_stream));
+		done += len;
 	}
 }
 
@@ -968,23 +1011,17 @@ const std::vector<unsigned>& Simulation::step(const unsigned* fstim, size_t nf,
 		HIP_CHECK(hipMemcpyAsync(m_stimIdx.ptr, fIdx.data(), fIdx.size() * 4, hipMemcpyHostToDevice, m_stream));
 		HIP_CHECK(nemo_launch_fstim(m_stream, m_fstim.ptr, m_stimIdx.ptr, fIdx.size(), 1));
 	}
-	enqueueStep(!iIdx.empty(), !fIdx.empty());
+	const uint64_t t = m_hostStep;
+	setBase(t);
+	enqueueStep(0, !iIdx.empty(), !fIdx.empty());
+	m_hostStep += 1;
 	if(!fIdx.empty()) {
 		HIP_CHECK(nemo_launch_fstim(m_stream, m_fstim.ptr, m_stimIdx.ptr, fIdx.size(), 0));
 	}
-
-	HIP_CHECK(hipMemcpyAsync(m_pinnedClock, m_clock.ptr, sizeof(nemo_hip_clock), hipMemcpyDeviceToHost, m_stream));
+	readBitmaps(t, 1);
 	HIP_CHECK(hipStreamSynchronize(m_stream));
-	const uint64_t head = m_pinnedClock->log_head;
-	const size_t nfired = size_t(head - m_drainedHead);
-	ensurePinnedLog(nfired);
-	readLogRange(m_drainedHead, head, m_pinnedLog);
-	HIP_CHECK(hipStreamSynchronize(m_stream));
-	m_drainedHead = head;
-	/* output in ascending simulation order (cpu/Simulation.cpp:200-209) */
-	std::sort(m_pinnedLog, m_pinnedLog + nfired);
-	m_firedOut.resize(nfired);
-	for(size_t i = 0; i < nfired; ++i) m_firedOut[i] = m_l2g[m_pinnedLog[i]];
+	m_firedOut.clear();
+	decodeFiring(m_pinnedBits, m_firedOut);
 	return m_firedOut;
 }
 
@@ -997,14 +1034,13 @@ hipGraphExec_t Simulation::graphFor(unsigned k)
 	const bool debug = m_debug;
 	m_debug = false;
 	try {
-		for(unsigned i = 0; i < k; ++i) enqueueStep(false, false);
+		for(unsigned i = 0; i < k; ++i) enqueueStep(i, false, false);
 	} catch(...) {
 		m_debug = debug;
 		(void) hipStreamEndCapture(m_stream, &graph);
 		throw;
 	}
 	m_debug = debug;
-	m_hostStep -= k; /* capture does not run anything */
 	HIP_CHECK(hipStreamEndCapture(m_stream, &graph));
 	hipGraphExec_t exec;
 	HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
@@ -1013,9 +1049,17 @@ hipGraphExec_t Simulation::graphFor(unsigned k)
 	return exec;
 }
 
-/* Steps without stimulus, captured in graphs of up to 64 steps; the fired log
- * is drained after every graph (it holds at least 64 steps of worst-case
- * firing beyond the delivery window). */
+uint64_t Simulation::sumCounters(const DeviceArray<unsigned long long>& a)
+{
+	std::vector<unsigned long long> h(a.n);
+	HIP_CHECK(hipMemcpy(h.data(), a.ptr, a.bytes(), hipMemcpyDeviceToHost));
+	uint64_t s = 0;
+	for(unsigned long long v : h) s += v;
+	return s;
+}
+
+/* Steps without stimulus, captured in graphs of up to 64 steps; the firing
+ * bitmaps of each graph's steps are read back after it. */
 void Simulation::run(unsigned nsteps, unsigned stdpPeriod, float reward, bool readRaster,
 		std::vector<unsigned>* counts, std::vector<unsigned>* fired, uint64_t* events)
 {
@@ -1023,43 +1067,30 @@ void Simulation::run(unsigned nsteps, unsigned stdpPeriod, float reward, bool re
 	if(stdpPeriod && !m_stdp) {
 		throw Error(NEMO_LOGIC_ERROR, "applyStdp called, but no STDP model specified");
 	}
-	HIP_CHECK(hipMemcpyAsync(m_pinnedClock, m_clock.ptr, sizeof(nemo_hip_clock), hipMemcpyDeviceToHost, m_stream));
 	HIP_CHECK(hipStreamSynchronize(m_stream));
-	const uint64_t events0 = m_pinnedClock->events;
+	const uint64_t events0 = events ? sumCounters(m_blockEvents) : 0;
 	uint64_t stepsSinceApply = 0;
 	const unsigned CHUNK = 64;
 	unsigned done = 0;
-	std::vector<uint32_t> buf;
 	while(done < nsteps) {
 		unsigned k = std::min(CHUNK, nsteps - done);
 		if(stdpPeriod) k = unsigned(std::min<uint64_t>(k, stdpPeriod - stepsSinceApply));
 		const uint64_t t0 = m_hostStep;
-		HIP_CHECK(hipGraphLaunch(graphFor(k), m_stream));
+		hipGraphExec_t g = graphFor(k);
+		setBase(t0);
+		HIP_CHECK(hipGraphLaunch(g, m_stream));
 		m_hostStep += k;
-		HIP_CHECK(hipMemcpyAsync(m_pinnedClock, m_clock.ptr, sizeof(nemo_hip_clock), hipMemcpyDeviceToHost, m_stream));
 		if(readRaster) {
-			HIP_CHECK(hipMemcpyAsync(m_pinnedStepStart, m_stepStart.ptr, NEMO_STEP_RING * sizeof(uint64_t),
-						hipMemcpyDeviceToHost, m_stream));
+			readBitmaps(t0, k);
 		}
 		HIP_CHECK(hipStreamSynchronize(m_stream));
-		const uint64_t head = m_pinnedClock->log_head;
 		if(readRaster) {
-			const size_t nf = size_t(head - m_drainedHead);
-			ensurePinnedLog(nf);
-			readLogRange(m_drainedHead, head, m_pinnedLog);
-			HIP_CHECK(hipStreamSynchronize(m_stream));
 			for(unsigned s = 0; s < k; ++s) {
-				const uint64_t step = t0 + s;
-				const uint64_t b = m_pinnedStepStart[step & (NEMO_STEP_RING - 1)];
-				const uint64_t e = m_pinnedStepStart[(step + 1) & (NEMO_STEP_RING - 1)];
-				uint32_t* first = m_pinnedLog + (b - m_drainedHead);
-				uint32_t* last = m_pinnedLog + (e - m_drainedHead);
-				std::sort(first, last);
-				counts->push_back(unsigned(e - b));
-				for(uint32_t* p = first; p != last; ++p) fired->push_back(m_l2g[*p]);
+				const size_t before = fired->size();
+				decodeFiring(m_pinnedBits + size_t(s) * m_firedWords, *fired);
+				counts->push_back(unsigned(fired->size() - before));
 			}
 		}
-		m_drainedHead = head;
 		done += k;
 		stepsSinceApply += k;
 		if(stdpPeriod && stepsSinceApply == stdpPeriod) {
@@ -1068,11 +1099,51 @@ void Simulation::run(unsigned nsteps, unsigned stdpPeriod, float reward, bool re
 		}
 	}
 	HIP_CHECK(hipStreamSynchronize(m_stream));
-	if(events) {
-		HIP_CHECK(hipMemcpy(m_pinnedClock, m_clock.ptr, sizeof(nemo_hip_clock), hipMemcpyDeviceToHost));
-		*events = m_pinnedClock->events - events0;
+	if(events) *events = sumCounters(m_blockEvents) - events0;
+	m_firedOut.clear();
+}
+
+/* Eager steps with HIP events around each launch on the simulation's stream:
+ * average microseconds of the propagate kernel (delivery and STDP
+ * accumulation) and of the neuron updates (all groups), and the work counters
+ * of the profiled steps (events delivered, neurons fired, plastic synapses
+ * examined, steps).  The steps are real simulation steps; their firing is not
+ * returned. */
+void Simulation::profile(unsigned nsteps, double avgUs[4], uint64_t counters[4])
+{
+	activate();
+	HIP_CHECK(hipStreamSynchronize(m_stream));
+	const uint64_t e0 = sumCounters(m_blockEvents);
+	const uint64_t x0 = sumCounters(m_blockExamined);
+	std::vector<hipEvent_t> ev(3 * size_t(nsteps));
+	for(hipEvent_t& e : ev) HIP_CHECK(hipEventCreate(&e));
+	const uint64_t t0 = m_hostStep;
+	setBase(t0);
+	for(unsigned s = 0; s < nsteps; ++s) enqueueStep(s, false, false, &ev[3 * size_t(s)]);
+	m_hostStep += nsteps;
+	HIP_CHECK(hipStreamSynchronize(m_stream));
+	double sum[3] = {0, 0, 0};
+	for(unsigned s = 0; s < nsteps; ++s) {
+		for(int k = 0; k < 2; ++k) {
+			float ms = 0;
+			HIP_CHECK(hipEventElapsedTime(&ms, ev[3 * size_t(s) + k], ev[3 * size_t(s) + k + 1]));
+			sum[k] += ms;
+		}
 	}
-	if(!readRaster && !m_firedOut.empty()) m_firedOut.clear();
+	for(hipEvent_t& e : ev) (void) hipEventDestroy(e);
+	/* neurons fired in the last min(nsteps, ring) profiled steps, scaled */
+	const unsigned kb = std::min<unsigned>(nsteps, NEMO_STEP_RING / 2);
+	readBitmaps(m_hostStep - kb, kb);
+	HIP_CHECK(hipStreamSynchronize(m_stream));
+	uint64_t firedCount = 0;
+	for(size_t i = 0; i < size_t(kb) * m_firedWords; ++i) firedCount += __builtin_popcountll(m_pinnedBits[i]);
+	for(int k = 0; k < 2; ++k) avgUs[k] = nsteps ? 1000.0 * sum[k] / nsteps : 0.0;
+	avgUs[2] = 0.0;
+	avgUs[3] = 0.0;
+	counters[0] = sumCounters(m_blockEvents) - e0;
+	counters[1] = kb ? firedCount * nsteps / kb : 0;
+	counters[2] = sumCounters(m_blockExamined) - x0;
+	counters[3] = nsteps;
 }
 
 void Simulation::applyStdp(float reward)
@@ -1088,6 +1159,7 @@ void Simulation::applyStdp(float reward)
 		msg << "STDP reward rounded down to zero. The smallest valid reward is " << fxToFloat(1, m_fbits);
 		throw Error(NEMO_INVALID_INPUT, msg.str());
 	}
+	flushStdp();
 	nemo_apply_args a;
 	a.rcm_fwd = m_rcmFwd.ptr;
 	a.rcm_acc = m_rcmAcc.ptr;
@@ -1248,18 +1320,15 @@ unsigned long Simulation::elapsedWallclock() const
 
 unsigned long Simulation::elapsedSimulation()
 {
-	activate();
-	HIP_CHECK(hipStreamSynchronize(m_stream));
-	HIP_CHECK(hipMemcpy(m_pinnedClock, m_clock.ptr, sizeof(nemo_hip_clock), hipMemcpyDeviceToHost));
-	return (unsigned long) m_pinnedClock->cycle;
+	return (unsigned long) (m_hostStep - m_cycleOrigin);
 }
 
 void Simulation::resetTimer()
 {
 	activate();
+	m_cycleOrigin = m_hostStep;
+	copyScalar(&m_clock.ptr->cycle_origin, m_cycleOrigin);
 	HIP_CHECK(hipStreamSynchronize(m_stream));
-	uint64_t zero = 0;
-	HIP_CHECK(hipMemcpy(&m_clock.ptr->cycle, &zero, sizeof(zero), hipMemcpyHostToDevice));
 	m_wallStart = std::chrono::steady_clock::now();
 }
 
@@ -1275,14 +1344,18 @@ void Simulation::getCurrents(float* E, float* I, size_t n)
 	if(n != m_ncount) throw Error(NEMO_INVALID_INPUT, "size mismatch");
 	activate();
 	HIP_CHECK(hipStreamSynchronize(m_stream));
-	HIP_CHECK(hipMemcpy(E, m_debugCurrent.ptr, n * 4, hipMemcpyDeviceToHost));
-	HIP_CHECK(hipMemcpy(I, m_debugCurrent.ptr + m_accPitch, n * 4, hipMemcpyDeviceToHost));
+	std::vector<float> h(2 * m_accPitch);
+	HIP_CHECK(hipMemcpy(h.data(), m_debugCurrent.ptr, h.size() * 4, hipMemcpyDeviceToHost));
+	for(size_t i = 0; i < n; ++i) {
+		E[i] = h[m_order[i]];
+		I[i] = h[m_accPitch + m_order[i]];
+	}
 }
 
 void Simulation::localToGlobal(unsigned* g, size_t n) const
 {
 	if(n != m_ncount) throw Error(NEMO_INVALID_INPUT, "size mismatch");
-	std::copy(m_l2g.begin(), m_l2g.end(), g);
+	for(size_t i = 0; i < n; ++i) g[i] = m_l2g[m_order[i]];
 }
 
 void Simulation::getRecentFiring(uint64_t* h, size_t n)
@@ -1290,7 +1363,9 @@ void Simulation::getRecentFiring(uint64_t* h, size_t n)
 	if(n != m_ncount) throw Error(NEMO_INVALID_INPUT, "size mismatch");
 	activate();
 	HIP_CHECK(hipStreamSynchronize(m_stream));
-	HIP_CHECK(hipMemcpy(h, m_hist.ptr, n * 8, hipMemcpyDeviceToHost));
+	std::vector<uint64_t> d(m_npad);
+	HIP_CHECK(hipMemcpy(d.data(), m_hist.ptr, m_npad * 8, hipMemcpyDeviceToHost));
+	for(size_t i = 0; i < n; ++i) h[i] = d[m_order[i]];
 }
 
 void Simulation::getRngState(uint32_t* s, size_t n)
@@ -1298,8 +1373,10 @@ void Simulation::getRngState(uint32_t* s, size_t n)
 	if(n != m_ncount) throw Error(NEMO_INVALID_INPUT, "size mismatch");
 	activate();
 	HIP_CHECK(hipStreamSynchronize(m_stream));
+	size_t sim = 0;
 	for(Group& g : m_groups) {
-		HIP_CHECK(hipMemcpy(s + 4 * size_t(g.start), g.rng.ptr, g.rng.bytes(), hipMemcpyDeviceToHost));
+		HIP_CHECK(hipMemcpy(s + 4 * sim, g.rng.ptr, g.rng.bytes(), hipMemcpyDeviceToHost));
+		sim += g.count;
 	}
 }
 
@@ -1318,6 +1395,7 @@ void Simulation::getStdpAccumulators(int32_t* acc, size_t n)
 	std::fill(acc, acc + n, 0);
 	if(!m_stdp || m_nplastic == 0) return;
 	activate();
+	flushStdp();
 	HIP_CHECK(hipStreamSynchronize(m_stream));
 	std::vector<int32_t> racc(m_nplastic);
 	std::vector<uint32_t> fwd(m_nplastic);
@@ -1332,8 +1410,7 @@ uint64_t Simulation::eventsDelivered()
 {
 	activate();
 	HIP_CHECK(hipStreamSynchronize(m_stream));
-	HIP_CHECK(hipMemcpy(m_pinnedClock, m_clock.ptr, sizeof(nemo_hip_clock), hipMemcpyDeviceToHost));
-	return m_pinnedClock->events;
+	return sumCounters(m_blockEvents);
 }
 
 } // namespace nemo_hip

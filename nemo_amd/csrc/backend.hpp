@@ -204,24 +204,26 @@ private:
 
 	int m_device = 0;
 	hipStream_t m_stream = nullptr;
-	size_t m_ncount = 0;
+	size_t m_ncount = 0;   /* neurons */
+	size_t m_npad = 0;     /* device index space: each type padded to whole tiles */
+	unsigned m_tiles = 0;
 	unsigned m_D = 1;
 	unsigned m_fbits = 20;
-	size_t m_nsynapses = 0, m_nplastic = 0;
+	size_t m_nsynapses = 0, m_nplastic = 0, m_nonEmptyRows = 0;
 	bool m_writeOnly = false;
 	bool m_logging = false;
 
-	std::vector<unsigned> m_l2g;
-	std::unordered_map<unsigned, unsigned> m_g2l;
-	std::vector<unsigned> m_typeOfLocal;
+	std::vector<unsigned> m_l2g;          /* device index -> user index (~0 for padding) */
+	std::vector<uint32_t> m_order;        /* simulation index -> device index */
+	std::unordered_map<unsigned, unsigned> m_g2l; /* user index -> device index */
+	std::vector<int> m_typeOfLocal;       /* device index -> group (-1 for padding) */
 	std::vector<Group> m_groups;
 
 	/* device state */
 	DeviceArray<nemo_hip_clock> m_clock;
-	DeviceArray<uint64_t> m_stepStart;
-	DeviceArray<uint32_t> m_firedLog;
-	uint64_t m_logMask = 0;
-	DeviceArray<int64_t> m_acc;
+	DeviceArray<uint64_t> m_firedBits;    /* NEMO_STEP_RING firing bitmaps */
+	size_t m_firedWords = 0;
+	DeviceArray<int64_t> m_acc;           /* overflow accumulators */
 	size_t m_accPitch = 0;
 	DeviceArray<uint64_t> m_hist;
 	DeviceArray<float> m_ext;
@@ -234,15 +236,19 @@ private:
 	DeviceArray<int32_t> m_rcmAcc;
 	DeviceArray<uint32_t> m_stimIdx;
 	DeviceArray<float> m_stimVal;
+	DeviceArray<int2> m_bins;
+	DeviceArray<uint32_t> m_binTable;
+	DeviceArray<unsigned long long> m_blockEvents, m_blockExamined;
 
 	/* STDP */
 	bool m_stdp = false;
-	nemo_stdp_args m_stdpArgs;
 	int32_t m_minE = 0, m_maxE = 0, m_minI = 0, m_maxI = 0;
+	uint64_t m_flushed = ~uint64_t(0); /* step whose STDP accumulation ran on its own */
 
-	/* launch shapes (tuned per network) */
-	unsigned m_deliverLanes = 16, m_deliverBlocks = 1024;
-	unsigned m_stdpLanes = 16, m_stdpBlocks = 512;
+	/* propagate-kernel arguments and launch shapes (chosen per network) */
+	nemo_propagate_args m_propagate;
+	unsigned m_binBlocks = 1, m_binCap = 4096;
+	unsigned m_stdpBlocks = 0, m_stdpWords = 2;
 
 	/* host side of synapse queries (network order = source ascending, then insertion) */
 	std::map<unsigned, std::pair<size_t, size_t> > m_synRange; /* gsrc -> [first,last) */
@@ -255,12 +261,12 @@ private:
 	std::vector<synapse_id> m_queried;
 
 	/* host staging */
-	uint64_t m_hostStep = 0;          /* steps launched */
-	uint64_t m_drainedHead = 0;       /* log position up to which firing was read */
-	nemo_hip_clock* m_pinnedClock = nullptr;
-	uint32_t* m_pinnedLog = nullptr;
-	size_t m_pinnedLogCap = 0;
-	uint64_t* m_pinnedStepStart = nullptr;
+	uint64_t m_hostStep = 0;           /* steps launched */
+	uint64_t m_cycleOrigin = 0;        /* step of the last timer reset */
+	static const unsigned SCALAR_SLOTS = 256;
+	uint64_t* m_pinnedScalars = nullptr; /* ring of sources for host-to-device scalars */
+	unsigned m_scalarNext = 0;
+	uint64_t* m_pinnedBits = nullptr;  /* firing bitmaps read back */
 	std::vector<unsigned> m_firedOut;
 	std::chrono::steady_clock::time_point m_wallStart;
 
@@ -275,10 +281,14 @@ private:
 	void buildStdp(const Configuration& conf);
 	void allocateStepState();
 	void chooseLaunchShapes();
-	nemo_hip_step_buffers stepBuffers(bool ext, bool fstim) const;
-	void enqueueStep(bool ext, bool fstim, hipEvent_t* ev = nullptr);
-	void readLogRange(uint64_t from, uint64_t to, uint32_t* out);
-	void ensurePinnedLog(size_t n);
+	nemo_hip_step_buffers stepBuffers(uint64_t offset, bool ext, bool fstim) const;
+	void copyScalar(uint64_t* dst, uint64_t value);
+	void setBase(uint64_t base);
+	void enqueueStep(uint64_t offset, bool ext, bool fstim, hipEvent_t* ev = nullptr);
+	void flushStdp();
+	void decodeFiring(const uint64_t* bits, std::vector<unsigned>& out) const;
+	void readBitmaps(uint64_t t0, unsigned k);
+	uint64_t sumCounters(const DeviceArray<unsigned long long>& a);
 	hipGraphExec_t graphFor(unsigned k);
 	void syncFcmHost();
 	size_t synapseIndex(synapse_id id) const;

@@ -18,46 +18,61 @@ extern "C" {
 
 /* Longest supported synaptic delay (ms); the firing-history word is 64 bits. */
 #define NEMO_MAX_DELAY 64
-/* Steps remembered in the step-start ring (power of two, > max delay + 1). */
-#define NEMO_STEP_RING 128
+/* Threads per block of the backend's kernels. */
+#define NEMO_BLOCK 256
 
-/* Spike delivery: for every delay d in 1..D and every neuron n that fired at
- * step t-d, add the weights of the synapse row (n, d) into the target
- * accumulators, excitatory (w >= 0) and inhibitory separately.
- * Semantics of cpu::Simulation::deliverSpikes (cpu/Simulation.cpp:259-308). */
-typedef struct nemo_deliver_args {
-	nemo_hip_clock* clock;           /* reads step, adds delivered events */
-	const uint64_t* step_start;      /* ring of NEMO_STEP_RING log positions */
-	const uint32_t* fired_log;
-	uint64_t log_mask;
-	const uint32_t* rowptr;          /* N*D+1 offsets into fcm, row (n,d) = n*D + d-1 */
-	const int2* fcm;                 /* {target simulation index, Q11.20 weight} */
-	int64_t* acc;                    /* E plane, then I plane at acc_pitch */
-	size_t acc_pitch;
+/* One launch per step, before the neuron updates, doing two independent jobs:
+ *
+ * Spike delivery of step t (blocks [0, deliver_blocks)), binned by target
+ * tile (the update kernels gather the bins).  For every delay d in 1..D and
+ * every neuron n that fired at step t-d, the synapse row (n, d) is delivered:
+ * each event {target, w} is counting-sorted in LDS by target tile and written
+ * to the block's bin region; bin_table[tile][block] = count << 16 | offset.
+ * Events beyond the block's bin capacity are added with global atomics into
+ * acc (E if w >= 0, else I).  Block b delivers the (delay, tile) pairs
+ * [b*Q/B, (b+1)*Q/B), Q = D*tiles.  Semantics of
+ * cpu::Simulation::deliverSpikes (cpu/Simulation.cpp:259-308).
+ *
+ * STDP accumulation of step t-1 (blocks [deliver_blocks, gridDim)), for the
+ * neurons that fired at step t-1-P, P = |postfire| (ConnectivityMatrix::
+ * accumulateStdp, ConnectivityMatrix.cpp:224-258, StdpProcess::weightChange,
+ * StdpProcess.cpp:68-124).  It reads the firing history as left by the update
+ * of step t-1, which is why it may run beside delivery of step t.  Skipped
+ * when t == 0 or when step t-1's accumulation was already run on its own
+ * (clock->stdp_flushed == t-1). */
+typedef struct nemo_propagate_args {
+	const nemo_hip_clock* clock;
+	uint64_t step_offset;            /* t = clock->base_step + step_offset */
+	const uint64_t* fired_bits;      /* NEMO_STEP_RING bitmaps of fired_words words */
+	size_t fired_words;
+	/* delivery */
+	unsigned deliver_blocks;
+	const uint32_t* rowptr;          /* (neurons*D)+1 offsets, row (n,d) = n*D + d-1 */
+	const int2* fcm;                 /* {target device index, Q11.20 weight} */
 	unsigned D;                      /* rows per source (network max delay) */
-} nemo_deliver_args;
-
-hipError_t nemo_launch_deliver(hipStream_t, const nemo_deliver_args*, unsigned lanes_per_row, unsigned blocks);
-
-/* STDP accumulation for the neurons that fired P = |postfire| steps ago
- * (ConnectivityMatrix::accumulateStdp, ConnectivityMatrix.cpp:224-258,
- * StdpProcess::weightChange, StdpProcess.cpp:68-124). */
-typedef struct nemo_stdp_args {
-	nemo_hip_clock* clock;           /* reads step, adds examined synapses */
-	const uint64_t* step_start;
-	const uint32_t* fired_log;
-	uint64_t log_mask;
-	const uint32_t* rcm_ptr;         /* N+1 offsets of each target's plastic incoming synapses */
-	const uint32_t* rcm_sd;          /* source simulation index | (delay-1) << 26 */
+	unsigned tiles;
+	unsigned bin_capacity;           /* events per block region */
+	uint32_t* bin_table;             /* [tiles][deliver_blocks] */
+	int2* bins;                      /* [deliver_blocks][bin_capacity] {target % NEMO_TILE, w} */
+	int64_t* acc;                    /* overflow accumulators, E plane then I plane */
+	size_t acc_pitch;
+	unsigned long long* block_events; /* [deliver_blocks] events delivered, accumulated */
+	/* STDP */
+	unsigned stdp_words;             /* bitmap words per STDP block (1..4); 0: no STDP */
+	const uint32_t* rcm_ptr;         /* neurons+1 offsets of each target's plastic incoming synapses */
+	const uint32_t* rcm_sd;          /* source device index | (delay-1) << 26 */
 	int32_t* rcm_acc;                /* accumulated weight change */
 	const uint64_t* hist;
 	unsigned postPreWindow;          /* P */
 	uint64_t potentiationBits, depressionBits, prePostBits, postPreBits;
 	int32_t fnPre[64];
 	int32_t fnPost[64];
-} nemo_stdp_args;
+	unsigned long long* block_examined; /* [STDP blocks] plastic synapses examined, accumulated */
+} nemo_propagate_args;
 
-hipError_t nemo_launch_stdp(hipStream_t, const nemo_stdp_args*, unsigned lanes_per_row, unsigned blocks);
+/* dynamic LDS bytes the propagate kernel needs */
+size_t nemo_propagate_lds_bytes(unsigned tiles, unsigned bin_capacity);
+hipError_t nemo_launch_propagate(hipStream_t, const nemo_propagate_args*, unsigned stdp_blocks);
 
 /* applyStdp (ConnectivityMatrix::applyStdp, ConnectivityMatrix.cpp:272-325 and
  * StdpProcess::updatedWeight, StdpProcess.cpp:128-138). */
@@ -73,10 +88,7 @@ typedef struct nemo_apply_args {
 
 hipError_t nemo_launch_apply_stdp(hipStream_t, const nemo_apply_args*);
 
-/* End of step: step += 1, cycle += 1, step_start[step] = log_head. */
-hipError_t nemo_launch_advance(hipStream_t, nemo_hip_clock*, uint64_t* step_start);
-
-/* Firing stimulus: set (or clear) the bits of the n given simulation indices. */
+/* Firing stimulus: set (or clear) the bits of the n given device indices. */
 hipError_t nemo_launch_fstim(hipStream_t, uint32_t* bits, const uint32_t* idx, size_t n, int set);
 
 /* Current stimulus: ext[idx[i]] = val[i] (indices must be distinct). */

@@ -1,110 +1,188 @@
 /*
- * The backend's own gfx950 kernels: spike delivery, STDP accumulation,
- * applyStdp, the step clock and stimulus helpers.  Neuron updates live in the
- * model plugins (plugins/*.hip).
+ * The backend's own gfx950 kernels: the per-step propagate kernel (binned
+ * spike delivery plus STDP accumulation), applyStdp and stimulus helpers.
+ * Neuron updates, which also gather the binned input, live in the model
+ * plugins (plugins/*.hip).
  *
- * Data layout in HBM (all indices are simulation indices: neurons numbered
- * type by type, in insertion order within a type, as NeMo's CPU backend does,
- * src/nemo/cpu/Simulation.cpp:60-77):
- *   fcm      int2[S]       synapse rows (source, delay) back to back; row
- *                          (n, d) = [rowptr[n*D+d-1], rowptr[n*D+d]); each
- *                          entry {target, Q11.20 weight}: 8 bytes, so a wave
- *                          reads 512 contiguous bytes per row chunk
- *   acc      int64[2][N]   excitatory / inhibitory current accumulators
- *   hist     uint64[N]     firing history (bit k: fired k steps ago)
- *   fired_log uint32[2^k]  circular log of fired neurons; step s's neurons are
- *                          log[step_start[s] .. step_start[s+1])
- *   rcm_*    per target, its plastic incoming synapses (STDP only)
+ * Data layout in HBM.  Device indices number neurons type by type, in
+ * insertion order within a type (NeMo's simulation order,
+ * src/nemo/cpu/Simulation.cpp:60-77), each type starting at a multiple of
+ * NEMO_TILE (256):
+ *   fcm        int2[S]     synapse rows (source, delay) back to back; row
+ *                          (n, d) = [rowptr[n*D+d-1], rowptr[n*D+d]); entries
+ *                          {target, Q11.20 weight}, 8 bytes, so consecutive
+ *                          lanes read consecutive 8-byte words of a row
+ *   fired_bits uint64[128][N/64]  one firing bitmap per step (ring)
+ *   hist       uint64[N]   firing history (bit k: fired k steps ago), STDP
+ *   bins       int2[B][CAP] each delivery block's events, sorted by tile
+ *   bin_table  uint32[T][B] count << 16 | offset of tile t in block b's bins
+ *   acc        int64[2][N] overflow accumulators (events beyond CAP)
+ *   rcm_*      per target, its plastic incoming synapses (STDP only)
  */
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
 #include "glibc_math.h"
-
-#define NEMO_BLOCK 256
+#include "plugin_device.h"
 
 namespace {
 
 constexpr uint64_t RING_MASK = NEMO_STEP_RING - 1;
+/* fired rows a delivery block lists in LDS at a time */
+constexpr unsigned ROWS = 1024;
+constexpr unsigned ROWS_PER_THREAD = ROWS / NEMO_BLOCK;
+
+__device__ inline unsigned kth_set_bit(uint64_t w, unsigned k)
+{
+	for(unsigned i = 0; i < k; ++i) w &= w - 1;
+	return unsigned(__builtin_ctzll(w));
+}
 
 /* ------------------------------------------------------------ delivery */
 
-template<int G>
-__global__ void __launch_bounds__(NEMO_BLOCK)
-deliver_kernel(nemo_deliver_args a)
+/* A delivery block: list the fired rows of its (delay, tile) pairs, read
+ * their row pointers in parallel, then spread all their events evenly over
+ * the block (four loads in flight per thread), staging them in LDS; finally
+ * counting-sort the staged events by target tile into the block's bins. */
+__device__ void deliver_block(const nemo_propagate_args& a, uint64_t t, int2* stage, uint32_t* cnt, uint32_t* cur)
 {
-	__shared__ uint64_t s_begin[NEMO_MAX_DELAY];
-	__shared__ uint32_t s_prefix[NEMO_MAX_DELAY + 1];
-	__shared__ unsigned long long s_events;
+	__shared__ uint64_t s_word[NEMO_BLOCK];
+	__shared__ uint32_t s_wpre[NEMO_BLOCK + 1];
+	__shared__ uint32_t s_rb[ROWS];
+	__shared__ uint32_t s_rpre[ROWS + 1];
 
-	const uint64_t t = a.clock->step;
+	const unsigned CAP = a.bin_capacity;
+	const unsigned T = a.tiles;
 	const unsigned D = a.D;
+	const unsigned tid = threadIdx.x;
+	const unsigned b = blockIdx.x;
+	for(unsigned p = tid; p < T; p += NEMO_BLOCK) cnt[p] = 0;
 
-	/* wave 0: the fired lists of steps t-1 .. t-D and an exclusive scan of
-	 * their lengths, so that work item j maps to (delay bin, list position) */
-	if(threadIdx.x < 64) {
-		const unsigned d = threadIdx.x + 1;
-		uint64_t beg = 0;
-		uint32_t cnt = 0;
-		if(d <= D && t >= d) {
-			const uint64_t s = t - d;
-			beg = a.step_start[s & RING_MASK];
-			cnt = uint32_t(a.step_start[(s + 1) & RING_MASK] - beg);
-		}
-		s_begin[threadIdx.x] = beg;
-		uint32_t x = cnt;
-#pragma unroll
-		for(int o = 1; o < 64; o <<= 1) {
-			const uint32_t y = __shfl_up(x, o);
-			if(int(threadIdx.x) >= o) x += y;
-		}
-		s_prefix[threadIdx.x + 1] = x;
-		if(threadIdx.x == 0) {
-			s_prefix[0] = 0;
-			s_events = 0;
-		}
-	}
-	__syncthreads();
-
-	const uint32_t total = s_prefix[D];
-	constexpr unsigned GROUPS = NEMO_BLOCK / G;
-	const unsigned lane = threadIdx.x % G;
+	const uint64_t Q = uint64_t(D) * T;
+	const uint64_t q0 = Q * b / a.deliver_blocks;
+	const uint64_t q1 = Q * (b + 1) / a.deliver_blocks;
+	const uint64_t nwords = (q1 - q0) * 4;
+	uint32_t fill = 0;
 	unsigned long long events = 0;
 
-	for(uint32_t j = blockIdx.x * GROUPS + threadIdx.x / G; j < total; j += gridDim.x * GROUPS) {
-		unsigned lo = 0, hi = D; /* s_prefix[lo] <= j < s_prefix[hi] */
-		while(hi - lo > 1) {
-			const unsigned mid = (lo + hi) >> 1;
-			if(s_prefix[mid] <= j) lo = mid; else hi = mid;
+	for(uint64_t wb = 0; wb < nwords; wb += NEMO_BLOCK) {
+		/* the bitmap words of this batch, one per thread */
+		uint64_t word = 0;
+		const uint64_t w = wb + tid;
+		if(w < nwords) {
+			const uint64_t q = q0 + w / 4;
+			const unsigned d = unsigned(q / T) + 1;
+			const unsigned p = unsigned(q % T);
+			if(t >= d) word = a.fired_bits[((t - d) & RING_MASK) * a.fired_words + size_t(p) * 4 + (w & 3)];
 		}
-		const uint32_t n = a.fired_log[(s_begin[lo] + (j - s_prefix[lo])) & a.log_mask];
-		const size_t row = size_t(n) * D + lo;
-		const uint32_t b = a.rowptr[row];
-		const uint32_t e = a.rowptr[row + 1];
-		if(lane == 0) events += e - b;
-		/* four independent loads in flight per lane before the atomics */
-		for(uint32_t k0 = b + lane; k0 < e; k0 += 4 * G) {
-			int2 s[4];
+		__syncthreads(); /* previous batch done with s_word */
+		s_word[tid] = word;
+		uint32_t R;
+		const uint32_t wpre = nemo_dev::block_exclusive_scan(uint32_t(__popcll(word)), &R);
+		s_wpre[tid] = wpre;
+		if(tid == 0) s_wpre[NEMO_BLOCK] = R;
+		__syncthreads();
+
+		for(uint32_t r0 = 0; r0 < R; r0 += ROWS) {
+			const uint32_t nr = min(ROWS, R - r0);
+			/* row pointers of up to ROWS fired rows, ROWS_PER_THREAD per thread */
+			uint32_t lens[ROWS_PER_THREAD];
+			uint32_t mine = 0;
 #pragma unroll
-			for(int u = 0; u < 4; ++u) {
-				const uint32_t k = k0 + u * G;
-				s[u] = k < e ? a.fcm[k] : make_int2(0, 0);
+			for(unsigned k = 0; k < ROWS_PER_THREAD; ++k) {
+				const uint32_t ri = tid * ROWS_PER_THREAD + k;
+				lens[k] = 0;
+				if(ri < nr) {
+					const uint32_t r = r0 + ri;
+					unsigned lo = 0, hi = NEMO_BLOCK; /* s_wpre[lo] <= r < s_wpre[hi] */
+					while(hi - lo > 1) {
+						const unsigned mid = (lo + hi) >> 1;
+						if(s_wpre[mid] <= r) lo = mid; else hi = mid;
+					}
+					const uint64_t wg = wb + lo;
+					const uint64_t q = q0 + wg / 4;
+					const unsigned d = unsigned(q / T) + 1;
+					const unsigned p = unsigned(q % T);
+					const unsigned bit = kth_set_bit(s_word[lo], r - s_wpre[lo]);
+					const uint32_t n = p * NEMO_TILE + unsigned(wg & 3) * 64 + bit;
+					const size_t row = size_t(n) * D + (d - 1);
+					const uint32_t rb = a.rowptr[row];
+					lens[k] = a.rowptr[row + 1] - rb;
+					s_rb[ri] = rb;
+				}
+				mine += lens[k];
 			}
+			uint32_t M;
+			uint32_t pre = nemo_dev::block_exclusive_scan(mine, &M);
 #pragma unroll
-			for(int u = 0; u < 4; ++u) {
-				if(k0 + u * G < e) {
-					int64_t* p = a.acc + (s[u].y >= 0 ? size_t(0) : a.acc_pitch) + uint32_t(s[u].x);
-					atomicAdd(reinterpret_cast<unsigned long long*>(p),
-							(unsigned long long)(int64_t)s[u].y);
+			for(unsigned k = 0; k < ROWS_PER_THREAD; ++k) {
+				const uint32_t ri = tid * ROWS_PER_THREAD + k;
+				if(ri < nr) s_rpre[ri] = pre;
+				pre += lens[k];
+			}
+			if(tid == 0) s_rpre[nr] = M;
+			__syncthreads();
+
+			/* the events of these rows, evenly over the block */
+			for(uint32_t e0 = tid; e0 < M; e0 += 4 * NEMO_BLOCK) {
+				int2 s[4];
+#pragma unroll
+				for(int u = 0; u < 4; ++u) {
+					const uint32_t e = e0 + u * NEMO_BLOCK;
+					s[u] = make_int2(0, 0);
+					if(e < M) {
+						unsigned lo = 0, hi = nr; /* s_rpre[lo] <= e < s_rpre[hi] */
+						while(hi - lo > 1) {
+							const unsigned mid = (lo + hi) >> 1;
+							if(s_rpre[mid] <= e) lo = mid; else hi = mid;
+						}
+						s[u] = a.fcm[s_rb[lo] + (e - s_rpre[lo])];
+					}
+				}
+#pragma unroll
+				for(int u = 0; u < 4; ++u) {
+					const uint32_t e = e0 + u * NEMO_BLOCK;
+					if(e < M) {
+						const uint32_t pos = fill + e;
+						if(pos < CAP) {
+							stage[pos] = s[u];
+							atomicAdd(&cnt[uint32_t(s[u].x) / NEMO_TILE], 1u);
+						} else {
+							int64_t* dst = a.acc + (s[u].y >= 0 ? size_t(0) : a.acc_pitch) + uint32_t(s[u].x);
+							atomicAdd(reinterpret_cast<unsigned long long*>(dst),
+									(unsigned long long)(int64_t)s[u].y);
+						}
+					}
 				}
 			}
+			fill += M;
+			events += M;
+			__syncthreads(); /* s_rb / s_rpre are rewritten by the next chunk */
 		}
 	}
-	if(events) atomicAdd(&s_events, events);
-	__syncthreads();
-	if(threadIdx.x == 0 && s_events) {
-		atomicAdd(reinterpret_cast<unsigned long long*>(&a.clock->events), s_events);
+
+	/* counting sort of the staged events by target tile */
+	const uint32_t staged = min(fill, CAP);
+	uint32_t carry = 0;
+	for(uint32_t base = 0; base < T; base += NEMO_BLOCK) {
+		const uint32_t p = base + tid;
+		const uint32_t c = p < T ? cnt[p] : 0u;
+		uint32_t total;
+		const uint32_t pre = nemo_dev::block_exclusive_scan(c, &total);
+		if(p < T) {
+			cur[p] = carry + pre;
+			a.bin_table[size_t(p) * a.deliver_blocks + b] = (c << 16) | (carry + pre);
+		}
+		carry += total;
 	}
+	__syncthreads();
+	int2* out = a.bins + size_t(b) * CAP;
+	for(uint32_t i = tid; i < staged; i += NEMO_BLOCK) {
+		const int2 e = stage[i];
+		const uint32_t pos = atomicAdd(&cur[uint32_t(e.x) / NEMO_TILE], 1u);
+		out[pos] = make_int2(int(uint32_t(e.x) % NEMO_TILE), e.y);
+	}
+	if(tid == 0 && events) a.block_events[b] += events;
 }
 
 /* ---------------------------------------------------------------- STDP */
@@ -114,7 +192,7 @@ __device__ inline int32_t wrap_add(int32_t x, int32_t y)
 	return int32_t(uint32_t(x) + uint32_t(y));
 }
 
-__device__ inline int32_t update_region(uint64_t arrivals, const nemo_stdp_args& a,
+__device__ inline int32_t update_region(uint64_t arrivals, const nemo_propagate_args& a,
 		const int32_t* fnPre, const int32_t* fnPost)
 {
 	/* StdpProcess::updateRegion, StdpProcess.cpp:86-111: the nearest arrival
@@ -130,51 +208,106 @@ __device__ inline int32_t update_region(uint64_t arrivals, const nemo_stdp_args&
 	return 0;
 }
 
-template<int G>
-__global__ void __launch_bounds__(NEMO_BLOCK)
-stdp_kernel(nemo_stdp_args a)
+/* An STDP block: the targets among its bitmap words that fired P steps before
+ * step s, and all their plastic incoming synapses spread evenly over its
+ * threads (four in flight per thread). */
+__device__ void stdp_block(const nemo_propagate_args& a, uint64_t s, unsigned sb)
 {
 	__shared__ int32_t s_pre[64];
 	__shared__ int32_t s_post[64];
-	__shared__ unsigned long long s_examined;
-	const uint64_t t = a.clock->step;
+	__shared__ uint32_t s_rb[NEMO_BLOCK];
+	__shared__ uint32_t s_pos[NEMO_BLOCK + 1];
 	const unsigned P = a.postPreWindow;
-	if(t < P) return;
-	if(threadIdx.x < 64) {
-		s_pre[threadIdx.x] = a.fnPre[threadIdx.x];
-		s_post[threadIdx.x] = a.fnPost[threadIdx.x];
+	const unsigned tid = threadIdx.x;
+	if(tid < 64) {
+		s_pre[tid] = a.fnPre[tid];
+		s_post[tid] = a.fnPost[tid];
 	}
-	if(threadIdx.x == 0) s_examined = 0;
+
+	const uint64_t slot = (s - P) & RING_MASK;
+	const size_t w0 = size_t(sb) * a.stdp_words;
+	const size_t word = w0 + tid / 64;
+	bool fired = false;
+	if(tid / 64 < a.stdp_words && word < a.fired_words) {
+		fired = (a.fired_bits[slot * a.fired_words + word] >> (tid & 63)) & 1u;
+	}
+	uint32_t nfired;
+	const uint32_t idx = nemo_dev::block_exclusive_scan(fired ? 1u : 0u, &nfired);
+	uint32_t len = 0, rb = 0;
+	if(fired) {
+		const uint32_t n = uint32_t(w0 * 64 + tid);
+		rb = a.rcm_ptr[n];
+		len = a.rcm_ptr[n + 1] - rb;
+	}
+	uint32_t total;
+	const uint32_t pos = nemo_dev::block_exclusive_scan(len, &total);
+	if(fired) {
+		s_rb[idx] = rb;
+		s_pos[idx] = pos;
+	}
+	if(tid == 0) s_pos[nfired] = total;
 	__syncthreads();
-	unsigned long long examined = 0;
+	if(total == 0) return;
 
-	/* the targets whose firing history has bit P set: those that fired at t-P */
-	const uint64_t s = t - P;
-	const uint64_t beg = a.step_start[s & RING_MASK];
-	const uint64_t end = P == 0 ? a.clock->log_head : a.step_start[(s + 1) & RING_MASK];
-	const uint32_t total = uint32_t(end - beg);
-	constexpr unsigned GROUPS = NEMO_BLOCK / G;
-	const unsigned lane = threadIdx.x % G;
-
-	for(uint32_t j = blockIdx.x * GROUPS + threadIdx.x / G; j < total; j += gridDim.x * GROUPS) {
-		const uint32_t n = a.fired_log[(beg + j) & a.log_mask];
-		const uint32_t rb = a.rcm_ptr[n];
-		const uint32_t re = a.rcm_ptr[n + 1];
-		if(lane == 0) examined += re - rb;
-		for(uint32_t k = rb + lane; k < re; k += G) {
-			const uint32_t sd = a.rcm_sd[k];
-			const uint64_t preFiring = a.hist[sd & 0x3ffffffu] >> ((sd >> 26) + 1);
-			const int32_t dw = wrap_add(update_region(preFiring & a.potentiationBits, a, s_pre, s_post),
-					update_region(preFiring & a.depressionBits, a, s_pre, s_post));
-			if(dw != 0) {
-				a.rcm_acc[k] = wrap_add(a.rcm_acc[k], dw);
+	for(uint32_t e0 = tid; e0 < total; e0 += 4 * NEMO_BLOCK) {
+		uint32_t k[4];
+		uint32_t sd[4];
+#pragma unroll
+		for(int u = 0; u < 4; ++u) {
+			const uint32_t e = e0 + u * NEMO_BLOCK;
+			k[u] = 0;
+			sd[u] = 0;
+			if(e < total) {
+				unsigned lo = 0, hi = nfired; /* s_pos[lo] <= e < s_pos[hi] */
+				while(hi - lo > 1) {
+					const unsigned mid = (lo + hi) >> 1;
+					if(s_pos[mid] <= e) lo = mid; else hi = mid;
+				}
+				k[u] = s_rb[lo] + (e - s_pos[lo]);
+				sd[u] = a.rcm_sd[k[u]];
+			}
+		}
+		uint64_t h[4];
+		int32_t acc[4];
+#pragma unroll
+		for(int u = 0; u < 4; ++u) {
+			h[u] = 0;
+			acc[u] = 0;
+			if(e0 + u * NEMO_BLOCK < total) {
+				h[u] = a.hist[sd[u] & 0x3ffffffu];
+				acc[u] = a.rcm_acc[k[u]];
+			}
+		}
+#pragma unroll
+		for(int u = 0; u < 4; ++u) {
+			if(e0 + u * NEMO_BLOCK < total) {
+				const unsigned d = (sd[u] >> 26) + 1;
+				const uint64_t preFiring = d >= 64 ? 0 : (h[u] >> d);
+				const int32_t dw = wrap_add(update_region(preFiring & a.potentiationBits, a, s_pre, s_post),
+						update_region(preFiring & a.depressionBits, a, s_pre, s_post));
+				if(dw != 0) {
+					a.rcm_acc[k[u]] = wrap_add(acc[u], dw);
+				}
 			}
 		}
 	}
-	if(examined) atomicAdd(&s_examined, examined);
-	__syncthreads();
-	if(threadIdx.x == 0 && s_examined) {
-		atomicAdd(reinterpret_cast<unsigned long long*>(&a.clock->stdp_synapses), s_examined);
+	if(tid == 0) a.block_examined[sb] += total;
+}
+
+__global__ void __launch_bounds__(NEMO_BLOCK)
+propagate_kernel(nemo_propagate_args a)
+{
+	extern __shared__ int2 lds_dyn[];
+	const uint64_t t = a.clock->base_step + a.step_offset;
+	if(blockIdx.x < a.deliver_blocks) {
+		int2* stage = lds_dyn;
+		uint32_t* cnt = reinterpret_cast<uint32_t*>(lds_dyn + a.bin_capacity);
+		deliver_block(a, t, stage, cnt, cnt + a.tiles);
+	} else {
+		if(t == 0) return;
+		const uint64_t s = t - 1;
+		if(s == a.clock->stdp_flushed || s < a.postPreWindow) return;
+		stdp_block(a, s, blockIdx.x - a.deliver_blocks);
 	}
 }
 
@@ -200,17 +333,7 @@ apply_stdp_kernel(nemo_apply_args a)
 	}
 }
 
-/* ------------------------------------------------------------ bookkeeping */
-
-__global__ void advance_kernel(nemo_hip_clock* c, uint64_t* step_start)
-{
-	if(threadIdx.x == 0 && blockIdx.x == 0) {
-		const uint64_t next = c->step + 1;
-		c->step = next;
-		c->cycle = c->cycle + 1;
-		step_start[next & RING_MASK] = c->log_head;
-	}
-}
+/* ------------------------------------------------------------ stimulus */
 
 __global__ void fstim_kernel(uint32_t* bits, const uint32_t* idx, size_t n, int set)
 {
@@ -237,52 +360,31 @@ __global__ void eval_math_kernel(int fn, const float* in, float* out, size_t n)
 	}
 }
 
-template<int G>
-hipError_t launchDeliverG(hipStream_t st, const nemo_deliver_args* a, unsigned blocks)
-{
-	hipLaunchKernelGGL(deliver_kernel<G>, dim3(blocks), dim3(NEMO_BLOCK), 0, st, *a);
-	return hipGetLastError();
-}
-
-template<int G>
-hipError_t launchStdpG(hipStream_t st, const nemo_stdp_args* a, unsigned blocks)
-{
-	hipLaunchKernelGGL(stdp_kernel<G>, dim3(blocks), dim3(NEMO_BLOCK), 0, st, *a);
-	return hipGetLastError();
-}
-
 } // namespace
 
 extern "C" {
 
-hipError_t nemo_launch_deliver(hipStream_t st, const nemo_deliver_args* a, unsigned G, unsigned blocks)
+size_t nemo_propagate_lds_bytes(unsigned tiles, unsigned bin_capacity)
 {
-	if(a->D == 0 || a->D > NEMO_MAX_DELAY || blocks == 0) return hipErrorInvalidValue;
-	switch(G) {
-		case 1: return launchDeliverG<1>(st, a, blocks);
-		case 2: return launchDeliverG<2>(st, a, blocks);
-		case 4: return launchDeliverG<4>(st, a, blocks);
-		case 8: return launchDeliverG<8>(st, a, blocks);
-		case 16: return launchDeliverG<16>(st, a, blocks);
-		case 32: return launchDeliverG<32>(st, a, blocks);
-		case 64: return launchDeliverG<64>(st, a, blocks);
-		default: return hipErrorInvalidValue;
-	}
+	return size_t(bin_capacity) * sizeof(int2) + 2 * size_t(tiles) * sizeof(uint32_t);
 }
 
-hipError_t nemo_launch_stdp(hipStream_t st, const nemo_stdp_args* a, unsigned G, unsigned blocks)
+hipError_t nemo_launch_propagate(hipStream_t st, const nemo_propagate_args* a, unsigned stdp_blocks)
 {
-	if(a->postPreWindow > 63 || blocks == 0) return hipErrorInvalidValue;
-	switch(G) {
-		case 1: return launchStdpG<1>(st, a, blocks);
-		case 2: return launchStdpG<2>(st, a, blocks);
-		case 4: return launchStdpG<4>(st, a, blocks);
-		case 8: return launchStdpG<8>(st, a, blocks);
-		case 16: return launchStdpG<16>(st, a, blocks);
-		case 32: return launchStdpG<32>(st, a, blocks);
-		case 64: return launchStdpG<64>(st, a, blocks);
-		default: return hipErrorInvalidValue;
+	if(a->deliver_blocks > NEMO_MAX_BIN_BLOCKS) return hipErrorInvalidValue;
+	if(a->deliver_blocks && (a->D == 0 || a->D > NEMO_MAX_DELAY || a->bin_capacity == 0
+				|| a->bin_capacity > 32768)) {
+		return hipErrorInvalidValue;
 	}
+	if(stdp_blocks && (a->postPreWindow > 63 || a->stdp_words == 0 || a->stdp_words > 4
+				|| size_t(stdp_blocks) * a->stdp_words < a->fired_words)) {
+		return hipErrorInvalidValue;
+	}
+	const unsigned blocks = a->deliver_blocks + stdp_blocks;
+	if(blocks == 0) return hipSuccess;
+	const size_t lds = a->deliver_blocks ? nemo_propagate_lds_bytes(a->tiles, a->bin_capacity) : 0;
+	hipLaunchKernelGGL(propagate_kernel, dim3(blocks), dim3(NEMO_BLOCK), lds, st, *a);
+	return hipGetLastError();
 }
 
 hipError_t nemo_launch_apply_stdp(hipStream_t st, const nemo_apply_args* a)
@@ -291,12 +393,6 @@ hipError_t nemo_launch_apply_stdp(hipStream_t st, const nemo_apply_args* a)
 	size_t blocks = (a->count + NEMO_BLOCK - 1) / NEMO_BLOCK;
 	if(blocks > 8192) blocks = 8192;
 	hipLaunchKernelGGL(apply_stdp_kernel, dim3(unsigned(blocks)), dim3(NEMO_BLOCK), 0, st, *a);
-	return hipGetLastError();
-}
-
-hipError_t nemo_launch_advance(hipStream_t st, nemo_hip_clock* c, uint64_t* step_start)
-{
-	hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(64), 0, st, c, step_start);
 	return hipGetLastError();
 }
 

@@ -1,9 +1,11 @@
 /*
  * Device helpers shared by the neuron-model plugin kernels.
  *
- * Every plugin kernel is launched with blocks of NEMO_UPDATE_BLOCK threads,
- * one neuron per thread, and every thread of a block must reach
- * nemo_record_firing() (it contains block barriers).
+ * An update kernel runs one block of NEMO_TILE threads per tile of its neuron
+ * type, one neuron per thread, and every thread of a block calls
+ *   TileInput in; gather_input(b, g, in);       (block barriers inside)
+ *   take_current(b, in, ng, E, I);              (valid threads)
+ *   record_firing(b, ng, valid, fired);         (every thread)
  */
 #ifndef NEMO_AMD_PLUGIN_DEVICE_H
 #define NEMO_AMD_PLUGIN_DEVICE_H
@@ -14,9 +16,112 @@
 #include "nemo_hip_plugin.h"
 #include "glibc_math.h"
 
-#define NEMO_UPDATE_BLOCK 256
+#define NEMO_UPDATE_BLOCK NEMO_TILE
 
 namespace nemo_dev {
+
+__device__ inline uint64_t current_step(const nemo_hip_step_buffers& b)
+{
+	return b.clock->base_step + b.step_offset;
+}
+
+/* Exclusive scan of one value per thread over a block of NEMO_TILE threads;
+ * returns the thread's prefix and sets *total. */
+__device__ inline uint32_t block_exclusive_scan(uint32_t x, uint32_t* total)
+{
+	__shared__ uint32_t s_wave[NEMO_TILE / 64];
+	const unsigned lane = threadIdx.x & 63;
+	const unsigned wave = threadIdx.x >> 6;
+	uint32_t v = x;
+#pragma unroll
+	for(int o = 1; o < 64; o <<= 1) {
+		const uint32_t y = __shfl_up(v, o);
+		if(int(lane) >= o) v += y;
+	}
+	if(lane == 63) s_wave[wave] = v;
+	__syncthreads();
+	uint32_t before = 0, all = 0;
+#pragma unroll
+	for(unsigned w = 0; w < NEMO_TILE / 64; ++w) {
+		const uint32_t t = s_wave[w];
+		before += w < wave ? t : 0;
+		all += t;
+	}
+	__syncthreads();
+	*total = all;
+	return before + v - x;
+}
+
+/* Synaptic input of one tile for the current step. */
+struct TileInput {
+	long long E[NEMO_TILE];
+	long long I[NEMO_TILE];
+};
+
+/* Sum the tile's binned events from every delivery block into LDS (integer
+ * sums, so the order of the 64-bit LDS atomics does not matter). */
+__device__ inline void gather_input(const nemo_hip_step_buffers& b, const nemo_hip_neuron_group& g, TileInput& in)
+{
+	__shared__ uint32_t s_pre[NEMO_MAX_BIN_BLOCKS + 1];
+	__shared__ uint32_t s_off[NEMO_MAX_BIN_BLOCKS];
+	const unsigned tid = threadIdx.x;
+	in.E[tid] = 0;
+	in.I[tid] = 0;
+	const unsigned B = b.bin_blocks;
+	if(B == 0) {
+		__syncthreads();
+		return;
+	}
+	const unsigned tile = g.start / NEMO_TILE + blockIdx.x;
+	const uint32_t* row = b.bin_table + size_t(tile) * B;
+	constexpr unsigned PER = NEMO_MAX_BIN_BLOCKS / NEMO_TILE;
+	uint32_t cnt[PER];
+	uint32_t mine = 0;
+#pragma unroll
+	for(unsigned k = 0; k < PER; ++k) {
+		const unsigned i = tid * PER + k;
+		const uint32_t e = i < B ? row[i] : 0u;
+		cnt[k] = e >> 16;
+		if(i < B) s_off[i] = e & 0xffffu;
+		mine += cnt[k];
+	}
+	uint32_t total;
+	uint32_t pre = block_exclusive_scan(mine, &total);
+#pragma unroll
+	for(unsigned k = 0; k < PER; ++k) {
+		const unsigned i = tid * PER + k;
+		if(i < B) s_pre[i] = pre;
+		pre += cnt[k];
+	}
+	if(tid == 0) s_pre[B] = total;
+	__syncthreads();
+	const size_t cap = b.bin_capacity;
+	/* four independent loads in flight per thread before the LDS atomics */
+	for(uint32_t j0 = tid; j0 < total; j0 += 4 * NEMO_TILE) {
+		int2 ev[4];
+#pragma unroll
+		for(int u = 0; u < 4; ++u) {
+			const uint32_t j = j0 + u * NEMO_TILE;
+			ev[u] = make_int2(0, 0);
+			if(j < total) {
+				unsigned lo = 0, hi = B; /* s_pre[lo] <= j < s_pre[hi] */
+				while(hi - lo > 1) {
+					const unsigned mid = (lo + hi) >> 1;
+					if(s_pre[mid] <= j) lo = mid; else hi = mid;
+				}
+				ev[u] = b.bins[size_t(lo) * cap + s_off[lo] + (j - s_pre[lo])];
+			}
+		}
+#pragma unroll
+		for(int u = 0; u < 4; ++u) {
+			if(j0 + u * NEMO_TILE < total) {
+				long long* dst = ev[u].y >= 0 ? &in.E[ev[u].x] : &in.I[ev[u].x];
+				atomicAdd(reinterpret_cast<unsigned long long*>(dst), (unsigned long long)(long long)ev[u].y);
+			}
+		}
+	}
+	__syncthreads();
+}
 
 /* Saturate a 64-bit fixed-point sum to the int32 range and convert to float,
  * as NeMo's wfx_toFloat (src/nemo/fixedpoint.cpp:40-49). */
@@ -27,14 +132,17 @@ __device__ inline float wfx_to_float(int64_t v, unsigned fbits)
 	return float(int32_t(v)) / float(1u << fbits);
 }
 
-/* Read the excitatory and inhibitory current of neuron ng and clear the
- * accumulators for the next step (cpu/Simulation.cpp:279-288). */
-__device__ inline void take_current(const nemo_hip_step_buffers& b, unsigned ng, float& E, float& I)
+/* Excitatory and inhibitory current of neuron ng (thread threadIdx.x of the
+ * tile): binned input plus overflow accumulators, which are cleared
+ * (cpu/Simulation.cpp:279-288). */
+__device__ inline void take_current(const nemo_hip_step_buffers& b, const TileInput& in, unsigned ng, float& E, float& I)
 {
-	int64_t e = b.acc[ng];
-	int64_t i = b.acc[b.acc_pitch + ng];
-	b.acc[ng] = 0;
-	b.acc[b.acc_pitch + ng] = 0;
+	int64_t e = in.E[threadIdx.x];
+	int64_t i = in.I[threadIdx.x];
+	const int64_t oe = b.acc[ng];
+	const int64_t oi = b.acc[b.acc_pitch + ng];
+	if(oe) { e += oe; b.acc[ng] = 0; }
+	if(oi) { i += oi; b.acc[b.acc_pitch + ng] = 0; }
 	E = wfx_to_float(e, b.fbits);
 	I = wfx_to_float(i, b.fbits);
 	if(b.debug_current) {
@@ -77,36 +185,18 @@ __device__ inline float nrand(uint4& s)
 	return nemo_math::sinf_exact(a) * r;
 }
 
-/* Shift the firing history and append fired neurons to the fired log.  One
- * atomic on the log head per block. */
+/* Shift the firing history and write the tile's words of the step's firing
+ * bitmap (one ballot per wave; device indices of a wave are 64-aligned). */
 __device__ inline void record_firing(const nemo_hip_step_buffers& b, unsigned ng, bool valid, bool fired)
 {
-	__shared__ unsigned s_count;
-	__shared__ unsigned long long s_base;
 	if(valid) {
-		uint64_t h = b.hist[ng];
+		const uint64_t h = b.hist[ng];
 		b.hist[ng] = (h << 1) | uint64_t(fired ? 1 : 0);
 	}
-	if(threadIdx.x == 0) s_count = 0;
-	__syncthreads();
-	const bool f = valid && fired;
-	const uint64_t mask = __ballot(f);
-	const unsigned lane = threadIdx.x & 63;
-	unsigned waveBase = 0;
-	if(mask) {
-		if(lane == 0) {
-			waveBase = atomicAdd(&s_count, unsigned(__popcll(mask)));
-		}
-		waveBase = __shfl(waveBase, 0);
-	}
-	__syncthreads();
-	if(threadIdx.x == 0 && s_count) {
-		s_base = atomicAdd((unsigned long long*)&b.clock->log_head, (unsigned long long)s_count);
-	}
-	__syncthreads();
-	if(f) {
-		const unsigned rank = unsigned(__popcll(mask & ((uint64_t(1) << lane) - 1)));
-		b.fired_log[(s_base + waveBase + rank) & b.log_mask] = ng;
+	const uint64_t mask = __ballot(valid && fired);
+	if((threadIdx.x & 63) == 0) {
+		const uint64_t slot = current_step(b) & (NEMO_STEP_RING - 1);
+		b.fired_bits[slot * b.fired_words + (ng >> 6)] = mask;
 	}
 }
 

@@ -15,6 +15,8 @@ __global__ void __launch_bounds__(NEMO_UPDATE_BLOCK)
 hh_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 {
 	using nemo_math::expf_exact;
+	__shared__ nemo_dev::TileInput input;
+	nemo_dev::gather_input(b, g, input);
 	const unsigned nl = blockIdx.x * NEMO_UPDATE_BLOCK + threadIdx.x;
 	const bool valid = nl < g.count;
 	const unsigned nn = g.start + nl;
@@ -29,9 +31,9 @@ hh_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 		float h = s[3 * P + nl];
 		float dir = s[4 * P + nl];
 		float Excit, Inhib;
-		nemo_dev::take_current(b, nn, Excit, Inhib);
+		nemo_dev::take_current(b, input, nn, Excit, Inhib);
 		const float Exter = nemo_dev::take_external(b, nn);
-		const uint64_t cycle = b.clock->cycle;
+		const uint64_t cycle = nemo_dev::current_step(b) - b.clock->cycle_origin;
 
 		const float dt = 0.001f;
 		const float gNa = 120.0f;

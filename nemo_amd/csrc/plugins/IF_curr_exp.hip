@@ -23,6 +23,8 @@ enum { S_V = 0, S_IE = 1, S_II = 2, S_LASTFIRED = 3 };
 __global__ void __launch_bounds__(NEMO_UPDATE_BLOCK)
 if_curr_exp_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 {
+	__shared__ nemo_dev::TileInput input;
+	nemo_dev::gather_input(b, g, input);
 	const unsigned nl = blockIdx.x * NEMO_UPDATE_BLOCK + threadIdx.x;
 	const bool valid = nl < g.count;
 	const unsigned ng = g.start + nl;
@@ -33,7 +35,7 @@ if_curr_exp_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 		const float* p = g.param;
 		float* s = g.state;
 		float E, In;
-		nemo_dev::take_current(b, ng, E, In);
+		nemo_dev::take_current(b, input, ng, E, In);
 		float Ie = ((1.0f - 1.0f / p[TAU_SYN_E * P + nl]) * s[S_IE * P + nl]) + E;
 		float Ii = ((1.0f - 1.0f / p[TAU_SYN_I * P + nl]) * s[S_II * P + nl]) + In;
 		float I = Ie + Ii + nemo_dev::take_external(b, ng) + p[I_OFFSET * P + nl];

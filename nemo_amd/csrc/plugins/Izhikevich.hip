@@ -24,6 +24,8 @@ enum { STATE_U = 0, STATE_V = 1 };
 __global__ void __launch_bounds__(NEMO_UPDATE_BLOCK)
 izhikevich_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 {
+	__shared__ nemo_dev::TileInput input;
+	nemo_dev::gather_input(b, g, input);
 	const unsigned nl = blockIdx.x * NEMO_UPDATE_BLOCK + threadIdx.x;
 	const bool valid = nl < g.count;
 	const unsigned ng = g.start + nl;
@@ -32,7 +34,7 @@ izhikevich_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 	if(valid) {
 		const size_t P = g.pitch;
 		float E, In;
-		nemo_dev::take_current(b, ng, E, In);
+		nemo_dev::take_current(b, input, ng, E, In);
 		float I = E + In + nemo_dev::take_external(b, ng);
 
 		const float sigma = g.param[PARAM_SIGMA * P + nl];

@@ -14,13 +14,15 @@ namespace {
 __global__ void __launch_bounds__(NEMO_UPDATE_BLOCK)
 poisson_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 {
+	__shared__ nemo_dev::TileInput input;
+	nemo_dev::gather_input(b, g, input);
 	const unsigned nl = blockIdx.x * NEMO_UPDATE_BLOCK + threadIdx.x;
 	const bool valid = nl < g.count;
 	const unsigned ng = g.start + nl;
 	bool fired = false;
 	if(valid) {
 		float E, I;
-		nemo_dev::take_current(b, ng, E, I); /* clears the accumulators */
+		nemo_dev::take_current(b, input, ng, E, I); /* clears the overflow accumulators */
 		const unsigned p0 = unsigned(g.param[nl] * float(1 << 16));
 		uint4* rng = reinterpret_cast<uint4*>(g.rng);
 		uint4 s = rng[nl];

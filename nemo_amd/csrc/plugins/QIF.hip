@@ -15,6 +15,8 @@ namespace {
 __global__ void __launch_bounds__(NEMO_UPDATE_BLOCK)
 qif_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 {
+	__shared__ nemo_dev::TileInput input;
+	nemo_dev::gather_input(b, g, input);
 	const unsigned nl = blockIdx.x * NEMO_UPDATE_BLOCK + threadIdx.x;
 	const bool valid = nl < g.count;
 	const unsigned ng = g.start + nl;
@@ -22,7 +24,7 @@ qif_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 
 	if(valid) {
 		float Excit, Inhib;
-		nemo_dev::take_current(b, ng, Excit, Inhib);
+		nemo_dev::take_current(b, input, ng, Excit, Inhib);
 		const float Exter = nemo_dev::take_external(b, ng);
 		const float dt = 0.1f;
 		const float a1 = 0.5f * dt;
