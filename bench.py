@@ -87,14 +87,24 @@ def main():
     ap.add_argument("--synapses", type=int, default=1000)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run the plain copy kernel on BYTES (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the HIP backend runs one simulation per device and this "
               "build has no multi-GPU path (see DESIGN.md on sharding)", file=sys.stderr)
         sys.exit(2)
 
+    import ctypes
     import nemo_amd
     from nemo_amd import generators as gen
+    from nemo_amd._lib import check, lib
+
+    if args.copy_probe:
+        gbps = ctypes.c_double()
+        check(lib().nemo_hip_copy_probe(args.copy_probe, 10, ctypes.byref(gbps)))
+        print(json.dumps({"copy_probe_bytes": args.copy_probe, "launches": 11, "achieved_GBps": gbps.value}))
+        return
 
     n, m = args.neurons, args.synapses
     t_gen = time.perf_counter()

@@ -214,6 +214,13 @@ nemo_status_t nemo_hip_counts(nemo_simulation_t, size_t* neurons, size_t* synaps
  * (src/nemo/Timer.hpp:23-79) only measures whole steps. */
 nemo_status_t nemo_hip_profile(nemo_simulation_t, unsigned nsteps, double avg_us[4], uint64_t counters[4]);
 
+/* Extension (measurement): copy `bytes` (rounded down to 16) between two
+ * device buffers reps times with a plain streaming kernel; *gbps receives the
+ * mean achieved bandwidth (read + write bytes per second / 1e9).  Used to
+ * calibrate hardware byte counters and as the achievable-bandwidth
+ * reference; no counterpart in the reference. */
+nemo_status_t nemo_hip_copy_probe(size_t bytes, unsigned reps, double* gbps);
+
 /* Extension (diagnostics): evaluate the device's glibc-exact logf (fn 0),
  * sinf (fn 1) or expf (fn 2) on n host inputs, results to host out.  The
  * reference calls the host C library for these (RNG.cpp:28-32, HH.cpp:92-98). */

@@ -91,6 +91,7 @@ _PROTOTYPES = [
     ("nemo_hip_get_stdp_accumulators", ctypes.c_int, [vp, c_i32_p, ctypes.c_size_t]),
     ("nemo_hip_counts", ctypes.c_int, [vp, c_size_p, c_size_p, c_size_p]),
     ("nemo_hip_profile", ctypes.c_int, [vp, ctypes.c_uint, ctypes.POINTER(ctypes.c_double), c_u64_p]),
+    ("nemo_hip_copy_probe", ctypes.c_int, [ctypes.c_size_t, ctypes.c_uint, ctypes.POINTER(ctypes.c_double)]),
     ("nemo_hip_eval_math", ctypes.c_int, [ctypes.c_int, c_float_p, c_float_p, ctypes.c_size_t]),
     ("hip_simulation", vp, [vp, vp]),
     ("hip_set_configuration", ctypes.c_int, [vp, ctypes.c_int]),

@@ -55,6 +55,7 @@ template struct DeviceArray<uint64_t>;
 template struct DeviceArray<int2>;
 template struct DeviceArray<nemo_hip_clock>;
 template struct DeviceArray<unsigned long long>;
+template struct DeviceArray<int4>;
 
 /* ----------------------------------------------------- fixed point (Q11.20) */
 

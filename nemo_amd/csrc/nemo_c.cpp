@@ -491,6 +491,29 @@ nemo_status_t nemo_hip_profile(nemo_simulation_t s, unsigned nsteps, double avg_
 	return guarded([&] { check(s, "simulation"); s->impl.profile(nsteps, avg_us, counters); });
 }
 
+nemo_status_t nemo_hip_copy_probe(size_t bytes, unsigned reps, double* gbps)
+{
+	return guarded([&] {
+		bytes &= ~size_t(15);
+		if(bytes == 0 || reps == 0) throw Error(NEMO_INVALID_INPUT, "empty copy probe");
+		nemo_hip::DeviceArray<int4> src(bytes / 16), dst(bytes / 16);
+		HIP_CHECK(hipMemset(src.ptr, 1, bytes));
+		hipEvent_t e0, e1;
+		HIP_CHECK(hipEventCreate(&e0));
+		HIP_CHECK(hipEventCreate(&e1));
+		HIP_CHECK(nemo_launch_copy(nullptr, src.ptr, dst.ptr, bytes)); /* warm */
+		HIP_CHECK(hipEventRecord(e0, nullptr));
+		for(unsigned r = 0; r < reps; ++r) HIP_CHECK(nemo_launch_copy(nullptr, src.ptr, dst.ptr, bytes));
+		HIP_CHECK(hipEventRecord(e1, nullptr));
+		HIP_CHECK(hipEventSynchronize(e1));
+		float ms = 0;
+		HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+		(void) hipEventDestroy(e0);
+		(void) hipEventDestroy(e1);
+		*gbps = 2.0 * double(bytes) * reps / (double(ms) * 1e-3) / 1e9;
+	});
+}
+
 nemo_status_t nemo_hip_eval_math(int fn, const float in[], float out[], size_t n)
 {
 	return guarded([&] {
