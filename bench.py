@@ -77,6 +77,130 @@ def cpu_baseline(spec, seconds, max_steps):
     }
 
 
+def batched(api, gen, count, steps, autotune_steps):
+    """The batched workload: `count` independent networks (log-uniform sizes
+    1k..50k, U{100..1000} synapses per neuron, STDP on every other one) in one
+    simulation, against the same networks run one after another as plain
+    simulations.  Both from construction, `steps` steps each, firing read
+    back every step, graphs captured before the timed region; each network's
+    per-step firing counts are compared with its solo run."""
+    t = time.perf_counter()
+    specs = gen.batch_izhikevich(count=count)
+    nets = []
+    for spec, _ in specs:
+        net = api.Network()
+        spec.load(net)
+        nets.append(net)
+    nsyn = sum(spec.synapse_count for spec, _ in specs)
+    nneu = sum(spec.neuron_count for spec, _ in specs)
+    del specs[:]
+    conf = api.Configuration()
+    conf.set_stdp_function(*stdp_function())
+    batch = api.Batch(nets, conf)
+    print("batch of %d networks: %d neurons, %d synapses, built in %.1f s"
+          % (count, nneu, nsyn, time.perf_counter() - t), file=sys.stderr, flush=True)
+    default_shape = batch.launch_shape
+    shape = batch.autotune(autotune_steps) if autotune_steps else default_shape
+    batch.prepare(steps, stdp_period=1000)
+    t0 = time.perf_counter()
+    bcounts, _, bevents = batch.run(steps, stdp_period=1000, reward=1.0)
+    bwall = time.perf_counter() - t0
+    del batch
+    print("batched run %.3f s" % bwall, file=sys.stderr, flush=True)
+
+    swall = 0.0
+    sevents = 0
+    exact = True
+    stdp_conf = api.Configuration()
+    stdp_conf.set_stdp_function(*stdp_function())
+    plain_conf = api.Configuration()
+    for k, net in enumerate(nets):
+        stdp = k % 2 == 0
+        sim = api.Simulation(net, stdp_conf if stdp else plain_conf)
+        sim.prepare(steps, stdp_period=1000 if stdp else 0)
+        t0 = time.perf_counter()
+        c, _, e = sim.run(steps, stdp_period=1000 if stdp else 0, reward=1.0)
+        swall += time.perf_counter() - t0
+        sevents += e
+        exact = exact and np.array_equal(c, bcounts[:, k])
+        del sim
+    print("sequential runs %.3f s" % swall, file=sys.stderr, flush=True)
+    return {
+        "workload": "izhikevich_80_20_batch_%d_nets" % count,
+        "networks": count, "neurons": nneu, "synapses": nsyn, "steps": steps,
+        "events_per_s": bevents / bwall,
+        "ms_per_step": 1000.0 * bwall / steps,
+        "sequential_events_per_s": sevents / swall,
+        "sequential_ms_per_step": 1000.0 * swall / steps,
+        "speedup_vs_sequential": (bevents / bwall) / (sevents / swall),
+        "events_equal": bevents == sevents,
+        "firing_counts_equal_solo": bool(exact),
+        "launch_shape": list(shape),
+        "default_launch_shape": list(default_shape),
+        "note": "sequential = the same networks as separate plain simulations (default launch shapes), "
+                "one after another; construction and graph capture excluded from both",
+    }
+
+
+def differentiable(api, gen, steps=500, checkpoint_every=50, reps=3):
+    """The differentiable workload: 10k Izhikevich neurons x 200 synapses
+    (delays U{1..10}), FP32 weights and per-neuron a, b, d trainable, Gaussian
+    input sigma 5/2, 500 steps; loss = MSE between the spike rates of the
+    last 250 steps and target rates U[2,20] Hz.  Forward and adjoint timed
+    separately with events on the stream both run on (median of reps)."""
+    import torch
+    from nemo_amd.differentiable import FloatNetwork
+    n = 10000
+    spec = gen.random_izhikevich(n, 200, dmax=10, seed=5)
+    net = api.Network()
+    spec.load(net)
+    fnet = FloatNetwork(net, 0)
+    dev = torch.device("cuda", 0)
+    a, b, c, d, sg, u, v = [torch.tensor(np.asarray(x), device=dev) for x in spec.blocks[0].args]
+    w = torch.tensor(spec.weight, device=dev).requires_grad_(True)
+    a.requires_grad_(True)
+    b.requires_grad_(True)
+    d.requires_grad_(True)
+    g = torch.Generator(device=dev).manual_seed(5)
+    x = sg * torch.randn(steps, n, device=dev, generator=g)
+    target = 2.0 + 18.0 * torch.rand(n, device=dev, generator=g)
+    last = steps // 2
+
+    def loss_of(S):
+        rates = S[-last:].sum(0) / (last * 1e-3)
+        return ((rates - target) ** 2).mean()
+
+    fwd, bwd = [], []
+    for r in range(reps + 1):
+        for p in (w, a, b, d):
+            p.grad = None
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        torch.cuda.synchronize()
+        e0.record()
+        S, V = fnet(w, a, b, d, x, c=c, v0=v, u0=u, checkpoint_every=checkpoint_every)
+        loss = loss_of(S)
+        e1.record()
+        loss.backward()
+        e2.record()
+        torch.cuda.synchronize()
+        if r:  # the first iteration warms up
+            fwd.append(e0.elapsed_time(e1))
+            bwd.append(e1.elapsed_time(e2))
+    f, bk = float(np.median(fwd)), float(np.median(bwd))
+    return {
+        "workload": "izhikevich_float_10k_x_200_500_steps",
+        "neurons": n, "synapses": int(spec.synapse_count), "steps": steps,
+        "checkpoint_every": checkpoint_every,
+        "forward_ms_per_step": f / steps,
+        "adjoint_ms_per_step": bk / steps,
+        "adjoint_over_forward": bk / f,
+        "firing_rate_hz": float(S.sum()) / n / (steps * 1e-3),
+        "loss": float(loss),
+        "grad_norms": {k: float(p.grad.norm()) for k, p in (("w", w), ("a", a), ("b", b), ("d", d))},
+        "dtype": "f32 (neuron substeps in f64)",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +211,10 @@ def main():
     ap.add_argument("--synapses", type=int, default=1000)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-nets", type=int, default=64, help="networks in the batched run (0: skip it)")
+    ap.add_argument("--batch-steps", type=int, default=1000)
+    ap.add_argument("--autotune-steps", type=int, default=128, help="steps per autotuner trial (0: default shape)")
+    ap.add_argument("--diff-steps", type=int, default=500, help="steps of the differentiable run (0: skip it)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel on BYTES (counter calibration)")
     args = ap.parse_args()
@@ -116,7 +244,10 @@ def main():
           % (n, sim.synapse_count, sim.plastic_count, t_ready - t_build, t_build - t_gen), file=sys.stderr, flush=True)
 
     sim.run(args.warmup, stdp_period=1000, reward=1.0)
-    print("warmup done", file=sys.stderr, flush=True)
+    default_shape = sim.launch_shape
+    shape = sim.autotune(args.autotune_steps) if args.autotune_steps else default_shape
+    sim.prepare(args.steps, stdp_period=1000)
+    print("warmup done; launch shape %s (default %s)" % (shape, default_shape), file=sys.stderr, flush=True)
 
     t0 = time.perf_counter()
     counts, fired, events = sim.run(args.steps, stdp_period=1000, reward=1.0)
@@ -151,6 +282,7 @@ def main():
             "neurons": n, "synapses_per_neuron": m, "max_delay_ms": 20, "seed": 2,
             "stdp": "A+=0.1exp(-dt/20) A-=-0.12exp(-dt/20) 20ms windows, reward 1.0 every 1000 steps, w in [0,10]",
             "method": "plain (fixed-point Q11.20, bit-exact with the CPU backend)",
+            "launch_shape": list(shape), "default_launch_shape": list(default_shape),
         },
         "sim_to_wall": args.steps * 1e-3 / wall,
         "firing_rate_hz": float(counts.sum()) / n / (args.steps * 1e-3),
@@ -168,6 +300,12 @@ def main():
         },
         "step_roofline_frac": (step_bytes / (wall / args.steps)) / 1e9 / HBM_PEAK_GBS,
     }
+    del sim
+    if args.batch_nets:
+        result["batched"] = batched(nemo_amd, gen, args.batch_nets, args.batch_steps, args.autotune_steps)
+    if args.diff_steps:
+        result["differentiable"] = differentiable(nemo_amd, gen, steps=args.diff_steps)
+        print("differentiable done", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         print("cpu baseline ...", file=sys.stderr, flush=True)
         cb = cpu_baseline(spec, args.cpu_seconds, 100000)

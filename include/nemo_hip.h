@@ -181,10 +181,87 @@ nemo_status_t nemo_reset_timer(nemo_simulation_t);
  * of every step is returned: counts[s] neurons fired in step s, and their
  * indices, each step in ascending simulation order, are concatenated in
  * *fired (borrowed until the next call).  counts may be NULL to skip the
- * raster readout; *events receives the synaptic events delivered. */
+ * raster readout; *events receives the synaptic events delivered.  For a
+ * batch (nemo_hip_new_batch) counts holds one entry per step and network. */
 nemo_status_t nemo_hip_run(nemo_simulation_t, unsigned nsteps,
 		unsigned stdp_period, float reward,
 		unsigned counts[], unsigned* fired[], size_t* fired_len, uint64_t* events);
+
+/* Extension: a batch of independent networks (trials, parameter sweeps)
+ * advanced together in the same kernel launches.  Each network keeps its own
+ * neuron indices, its own RNG seeds (as in a simulation of it alone) and its
+ * own firing output; neuron types are shared by name, and the configuration
+ * (STDP function, fixed-point format, device) is common.  The handle is a
+ * simulation: nemo_hip_run advances it (counts[s*count + k] neurons of network
+ * k fired in step s; within a step the fired lists follow network order),
+ * nemo_apply_stdp applies to every network, nemo_delete_simulation frees it.
+ * Single-network queries address network 0, and nemo_step is refused for
+ * more than one network.  No counterpart in the reference. */
+nemo_status_t nemo_hip_new_batch(const nemo_network_t nets[], size_t count, nemo_configuration_t,
+		nemo_simulation_t* sim);
+nemo_status_t nemo_hip_batch_size(nemo_simulation_t, unsigned* count);
+nemo_status_t nemo_hip_batch_get_neuron_state(nemo_simulation_t, unsigned net, unsigned neuron, unsigned var,
+		float* val);
+nemo_status_t nemo_hip_batch_get_neuron_parameter(nemo_simulation_t, unsigned net, unsigned neuron,
+		unsigned param, float* val);
+nemo_status_t nemo_hip_batch_get_membrane_potential(nemo_simulation_t, unsigned net, unsigned neuron, float* val);
+
+/* Extension: capture the step graphs nemo_hip_run(nsteps, stdp_period) uses,
+ * so that a timed run excludes graph construction. */
+nemo_status_t nemo_hip_prepare(nemo_simulation_t, unsigned nsteps, unsigned stdp_period);
+
+/* Extension: launch shape of the per-step kernels: shape[0] delivery blocks
+ * (1..1024), shape[1] events a delivery block stages in LDS (64..32768),
+ * shape[2] 1 if STDP accumulation shares the delivery launch, shape[3] firing
+ * bitmap words per STDP block (1..4).  Every shape gives identical results. */
+nemo_status_t nemo_hip_get_launch_shape(nemo_simulation_t, unsigned shape[4]);
+nemo_status_t nemo_hip_set_launch_shape(nemo_simulation_t, const unsigned shape[4]);
+
+/* Extension: autotuner.  Times candidate launch shapes (coordinate descent
+ * over the four shape parameters), each over steps_per_trial steps from the
+ * same snapshot of the simulation state, then restores that state and keeps
+ * the fastest shape, returned in shape.  Results of the simulation are
+ * unaffected. */
+nemo_status_t nemo_hip_autotune(nemo_simulation_t, unsigned steps_per_trial, unsigned shape[4]);
+
+/* Extension: float-weight mode with a hand-written adjoint (no counterpart in
+ * the reference, whose only mode is the fixed-point one).  A network of one
+ * Izhikevich type with neuron indices 0..N-1, simulated with FP32 weights and
+ * FP32 current sums; where spikes feed synapses and the loss, the threshold
+ * is differentiated through the fast-sigmoid surrogate
+ * dS/dv = 1 / (1 + 10 |v - 30|)^2 (the reset and the substep stops are held
+ * constant).  Per step t:
+ *   I = x[t] + sum of w over synapses whose source spiked at t - delay;
+ *   four 0.25 ms substeps of the Izhikevich step, each only while v < 30
+ *   (v <- v + h (0.04 v^2 + 5 v + 140 - u + I), u <- u + h a (b v - u));
+ *   S[t] = [v >= 30]; v <- v + S (c - v); u <- u + S d; V[t] = v.
+ * All arrays are device pointers owned by the caller: weights w in network
+ * order (source ascending, then insertion order, as synapse ids), per-neuron
+ * a, b, c, d, v0, u0 by index, x, spikes and v_out [steps][N] row-major;
+ * checkpoints [ceil(steps / checkpoint_every)][2][N] holds (v, u) before every
+ * checkpoint_every-th step.  Work is queued on `stream` (a hipStream_t; NULL
+ * for the default stream); calls on one handle must not overlap. */
+typedef struct nemo_hip_float_network_class* nemo_hip_float_network_t;
+nemo_status_t nemo_hip_new_float_network(nemo_network_t, int device, nemo_hip_float_network_t* out);
+void nemo_hip_delete_float_network(nemo_hip_float_network_t);
+nemo_status_t nemo_hip_float_network_sizes(nemo_hip_float_network_t, size_t* neurons, size_t* synapses,
+		unsigned* max_delay);
+nemo_status_t nemo_hip_float_forward(nemo_hip_float_network_t, void* stream, unsigned steps,
+		unsigned checkpoint_every, const float* w, const float* a, const float* b, const float* c,
+		const float* d, const float* v0, const float* u0, const float* x,
+		float* spikes, float* v_out, float* checkpoints);
+/* The adjoint of nemo_hip_float_forward: from dL/dspikes and dL/dv_out (either
+ * may be NULL) and the forward's spikes and checkpoints, the gradients of the
+ * loss with respect to w, a, b, d and x (grad_x may be NULL) and the initial
+ * state v0, u0 (either may be NULL).  Each checkpoint segment is recomputed
+ * forward, then stepped backward in time: the transposed delivery gathers
+ * each source's spike adjoint from the current adjoints of its targets at
+ * step + delay. */
+nemo_status_t nemo_hip_float_backward(nemo_hip_float_network_t, void* stream, unsigned steps,
+		unsigned checkpoint_every, const float* w, const float* a, const float* b, const float* c,
+		const float* d, const float* x, const float* spikes, const float* checkpoints,
+		const float* grad_spikes, const float* grad_v, float* grad_w, float* grad_a, float* grad_b,
+		float* grad_d, float* grad_x, float* grad_v0, float* grad_u0);
 
 /* Extension (parity inspection): float currents (E then I) delivered to each
  * neuron in the most recent step, in simulation order.  The reference CPU
@@ -206,12 +283,12 @@ nemo_status_t nemo_hip_get_stdp_accumulators(nemo_simulation_t, int32_t acc[], s
 /* Extension: synapse and neuron counts of a built simulation */
 nemo_status_t nemo_hip_counts(nemo_simulation_t, size_t* neurons, size_t* synapses, size_t* plastic);
 
-/* Extension (measurement): run nsteps steps (no stimulus) with HIP events at
- * every phase boundary on the simulation's stream.  avg_us receives the mean
- * duration of delivery, neuron update (all models), STDP accumulation and the
- * clock advance; counters the events delivered, neurons fired, plastic
- * synapses examined and steps run.  The reference's Timer
- * (src/nemo/Timer.hpp:23-79) only measures whole steps. */
+/* Extension (measurement): run nsteps steps (no stimulus) with HIP events
+ * around each phase on the simulation's stream.  avg_us receives the mean
+ * duration of the propagate phase (delivery and STDP accumulation), of the
+ * neuron updates (all models), and two zeros (reserved); counters the events
+ * delivered, neurons fired, plastic synapses examined and steps run.  The
+ * reference's Timer (src/nemo/Timer.hpp:23-79) only measures whole steps. */
 nemo_status_t nemo_hip_profile(nemo_simulation_t, unsigned nsteps, double avg_us[4], uint64_t counters[4]);
 
 /* Extension (measurement): copy `bytes` (rounded down to 16) between two

@@ -18,8 +18,9 @@
  * Contract of hip_update_neurons, for neurons [start, start+count) of one type
  * in step s = clock->base_step + step_offset (the semantics of
  * cpu_update_neurons, src/nemo/cpu/plugins/neuron_model.h:25-51):
- *   - sum the step's synaptic input of the tile: the binned events of every
- *     delivery block (bin_table / bins) plus the overflow accumulators acc,
+ *   - sum the step's synaptic input of the tile: the binned events of the
+ *     delivery blocks in the tile's bin_range (bin_table / bins) plus the
+ *     overflow accumulators acc,
  *     excitatory (w >= 0) and inhibitory separately, saturate each sum to the
  *     int32 range and convert to float (src/nemo/fixedpoint.cpp:40-49); clear
  *     the overflow accumulators;
@@ -76,6 +77,8 @@ typedef struct nemo_hip_step_buffers {
 	uint64_t step_offset;      /* step = clock->base_step + step_offset */
 	const nemo_hip_clock* clock;
 	const uint32_t* bin_table; /* [tile][bin_blocks]: count << 16 | offset */
+	const uint32_t* bin_range; /* per tile: first | end << 16 of the delivery blocks
+	                              holding its events (valid table entries) */
 	const int2* bins;          /* [bin_blocks][bin_capacity] {target % NEMO_TILE, Q weight} */
 	unsigned bin_blocks;
 	unsigned bin_capacity;

@@ -13,7 +13,7 @@ import numpy as np
 
 from ._lib import NemoError, check, lib
 
-__all__ = ["Configuration", "Network", "Simulation", "NemoError", "version", "hip_device_count",
+__all__ = ["Configuration", "Network", "Simulation", "Batch", "NemoError", "version", "hip_device_count",
            "hip_device_description", "add_plugin_path", "plugin_dir"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -249,6 +249,9 @@ class Simulation(_SynapseQueries):
         h = lib().nemo_new_simulation(net._h, conf._conf)
         if not h:
             raise NemoError(-1, lib().nemo_strerror().decode())
+        self._adopt(h)
+
+    def _adopt(self, h):
         self._h = h
         n = ctypes.c_size_t()
         s = ctypes.c_size_t()
@@ -296,6 +299,30 @@ class Simulation(_SynapseQueries):
 
     def apply_stdp(self, reward):
         check(lib().nemo_apply_stdp(self._h, reward))
+
+    def prepare(self, nsteps, stdp_period=0):
+        """Capture the step graphs run(nsteps, stdp_period) uses (so that a timed
+        run excludes graph construction)."""
+        check(lib().nemo_hip_prepare(self._h, nsteps, stdp_period))
+
+    @property
+    def launch_shape(self):
+        """(delivery blocks, bin capacity, STDP fused, STDP bitmap words)."""
+        a = (ctypes.c_uint * 4)()
+        check(lib().nemo_hip_get_launch_shape(self._h, a))
+        return tuple(a)
+
+    @launch_shape.setter
+    def launch_shape(self, shape):
+        a = (ctypes.c_uint * 4)(*shape)
+        check(lib().nemo_hip_set_launch_shape(self._h, a))
+
+    def autotune(self, steps_per_trial=128):
+        """Pick the fastest launch shape by timing candidates from a snapshot of
+        the current state (restored afterwards); returns the chosen shape."""
+        a = (ctypes.c_uint * 4)()
+        check(lib().nemo_hip_autotune(self._h, steps_per_trial, a))
+        return tuple(a)
 
     def get_membrane_potential(self, idx):
         v = ctypes.c_float()
@@ -382,3 +409,62 @@ class Simulation(_SynapseQueries):
         a = np.zeros(self.synapse_count, np.int32)
         check(lib().nemo_hip_get_stdp_accumulators(self._h, _ptr(a, ctypes.c_int32), a.size))
         return a
+
+
+class Batch(Simulation):
+    """Independent networks advanced together in the same kernel launches
+    (nemo_hip_new_batch).  Every network behaves exactly as in a Simulation of
+    its own: same RNG streams, same firing."""
+
+    def __init__(self, nets, conf):
+        arr = (ctypes.c_void_p * len(nets))(*[n._h for n in nets])
+        h = ctypes.c_void_p()
+        check(lib().nemo_hip_new_batch(arr, len(nets), conf._conf, ctypes.byref(h)))
+        self._adopt(h.value)
+        k = ctypes.c_uint()
+        check(lib().nemo_hip_batch_size(self._h, ctypes.byref(k)))
+        self.size = k.value
+
+    def run(self, nsteps, stdp_period=0, reward=1.0, raster=True):
+        """As Simulation.run; counts has shape (nsteps, size) and fired lists,
+        step by step, each network's fired neurons in turn."""
+        counts = np.zeros((nsteps, self.size), dtype=np.uint32) if raster else None
+        out = ctypes.POINTER(ctypes.c_uint)()
+        n = ctypes.c_size_t(0)
+        ev = ctypes.c_uint64(0)
+        check(lib().nemo_hip_run(self._h, nsteps, stdp_period, reward,
+                                 _ptr(counts, ctypes.c_uint) if raster else None,
+                                 ctypes.byref(out), ctypes.byref(n), ctypes.byref(ev)))
+        if not raster:
+            return None, None, ev.value
+        fired = np.ctypeslib.as_array(out, shape=(n.value,)).copy() if n.value else np.zeros(0, np.uint32)
+        return counts, fired, ev.value
+
+    @staticmethod
+    def split(counts, fired):
+        """Per network (counts, fired) in the format of Simulation.run."""
+        flat = counts.ravel()
+        ends = np.cumsum(flat.astype(np.int64))
+        starts = ends - flat
+        k = counts.shape[1]
+        out = []
+        for j in range(k):
+            idx = np.arange(j, flat.size, k)
+            parts = [fired[starts[i]:ends[i]] for i in idx]
+            out.append((counts[:, j].copy(), np.concatenate(parts) if parts else np.zeros(0, np.uint32)))
+        return out
+
+    def get_membrane_potential(self, idx, net=0):
+        v = ctypes.c_float()
+        check(lib().nemo_hip_batch_get_membrane_potential(self._h, net, idx, ctypes.byref(v)))
+        return v.value
+
+    def get_neuron_state(self, idx, var, net=0):
+        v = ctypes.c_float()
+        check(lib().nemo_hip_batch_get_neuron_state(self._h, net, idx, var, ctypes.byref(v)))
+        return v.value
+
+    def get_neuron_parameter(self, idx, param, net=0):
+        v = ctypes.c_float()
+        check(lib().nemo_hip_batch_get_neuron_parameter(self._h, net, idx, param, ctypes.byref(v)))
+        return v.value

@@ -82,6 +82,21 @@ _PROTOTYPES = [
     ("nemo_reset_timer", ctypes.c_int, [vp]),
     ("nemo_hip_run", ctypes.c_int, [vp, ctypes.c_uint, ctypes.c_uint, ctypes.c_float, c_uint_p,
                                     ctypes.POINTER(c_uint_p), c_size_p, c_u64_p]),
+    ("nemo_hip_new_batch", ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_size_t, vp, ctypes.POINTER(vp)]),
+    ("nemo_hip_batch_size", ctypes.c_int, [vp, c_uint_p]),
+    ("nemo_hip_batch_get_neuron_state", ctypes.c_int, [vp, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, c_float_p]),
+    ("nemo_hip_batch_get_neuron_parameter", ctypes.c_int, [vp, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint,
+                                                           c_float_p]),
+    ("nemo_hip_batch_get_membrane_potential", ctypes.c_int, [vp, ctypes.c_uint, ctypes.c_uint, c_float_p]),
+    ("nemo_hip_prepare", ctypes.c_int, [vp, ctypes.c_uint, ctypes.c_uint]),
+    ("nemo_hip_get_launch_shape", ctypes.c_int, [vp, c_uint_p]),
+    ("nemo_hip_set_launch_shape", ctypes.c_int, [vp, c_uint_p]),
+    ("nemo_hip_autotune", ctypes.c_int, [vp, ctypes.c_uint, c_uint_p]),
+    ("nemo_hip_new_float_network", ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(vp)]),
+    ("nemo_hip_delete_float_network", None, [vp]),
+    ("nemo_hip_float_network_sizes", ctypes.c_int, [vp, c_size_p, c_size_p, c_uint_p]),
+    ("nemo_hip_float_forward", ctypes.c_int, [vp, vp, ctypes.c_uint, ctypes.c_uint] + [vp] * 11),
+    ("nemo_hip_float_backward", ctypes.c_int, [vp, vp, ctypes.c_uint, ctypes.c_uint] + [vp] * 17),
     ("nemo_hip_set_debug", ctypes.c_int, [vp, ctypes.c_int]),
     ("nemo_hip_get_currents", ctypes.c_int, [vp, c_float_p, c_float_p, ctypes.c_size_t]),
     ("nemo_hip_local_to_global", ctypes.c_int, [vp, c_uint_p, ctypes.c_size_t]),
@@ -115,6 +130,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError("libnemo_hip.so is not built (%s); run `python -m nemo_amd.build`" % LIB_PATH)
+        # One HIP runtime per process: PyTorch-ROCm ships its own
+        # libamdhip64.so.7, and loading it first makes this library bind to
+        # that copy (same soname), so that torch tensors and this library's
+        # kernels share one runtime.  Without torch, the system runtime is used.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         l = ctypes.CDLL(LIB_PATH)
         for name, res, args in _PROTOTYPES:
             f = getattr(l, name)

@@ -60,9 +60,10 @@ def build(verbose=False):
     os.makedirs(PLUGIN_OUT, exist_ok=True)
     jobs = []
     # backend objects
-    jobs.append([_hipcc()] + HIP_FLAGS + ["-c", os.path.join(CSRC, "kernels.hip"),
-                                          "-o", os.path.join(BUILD, "kernels.o")])
-    for src in ("backend.cpp", "nemo_c.cpp"):
+    for src in ("kernels.hip", "float_mode.hip"):
+        jobs.append([_hipcc()] + HIP_FLAGS + ["-c", os.path.join(CSRC, src),
+                                              "-o", os.path.join(BUILD, src.replace(".hip", ".o"))])
+    for src in ("backend.cpp", "float_network.cpp", "nemo_c.cpp"):
         jobs.append(["g++"] + CXX_FLAGS + ["-c", os.path.join(CSRC, src),
                                            "-o", os.path.join(BUILD, src.replace(".cpp", ".o"))])
     # plugins: one shared library per model
@@ -75,7 +76,8 @@ def build(verbose=False):
             if verbose and out.strip():
                 print(out)
     link = ["g++", "-shared", "-o", os.path.join(LIBDIR, "libnemo_hip.so"),
-            os.path.join(BUILD, "kernels.o"), os.path.join(BUILD, "backend.o"),
+            os.path.join(BUILD, "kernels.o"), os.path.join(BUILD, "float_mode.o"),
+            os.path.join(BUILD, "backend.o"), os.path.join(BUILD, "float_network.o"),
             os.path.join(BUILD, "nemo_c.o"),
             "-L" + os.path.join(ROCM, "lib"), "-lamdhip64", "-fopenmp", "-ldl",
             "-Wl,-rpath," + os.path.join(ROCM, "lib")]

@@ -56,6 +56,8 @@ template struct DeviceArray<int2>;
 template struct DeviceArray<nemo_hip_clock>;
 template struct DeviceArray<unsigned long long>;
 template struct DeviceArray<int4>;
+template struct DeviceArray<uint2>;
+template struct DeviceArray<unsigned char>;
 
 /* ----------------------------------------------------- fixed point (Q11.20) */
 
@@ -413,20 +415,36 @@ void Configuration::setHipBackend(int dev)
 
 Simulation::Simulation(const Network& net, const Configuration& conf)
 {
+	construct(std::vector<const Network*>(1, &net), conf);
+}
+
+Simulation::Simulation(const std::vector<const Network*>& nets, const Configuration& conf)
+{
+	construct(nets, conf);
+}
+
+void Simulation::construct(const std::vector<const Network*>& nets, const Configuration& conf)
+{
 	if(conf.backend != NEMO_BACKEND_HIP) {
 		throw Error(NEMO_API_UNSUPPORTED, "this library provides the HIP backend only; select it with nemo_set_hip_backend");
 	}
-	/* nemo.cpp:111-117 */
-	if(net.neuronCount() == 0) {
-		throw Error(NEMO_INVALID_INPUT, "Cannot create simulation from empty network");
+	if(nets.empty() || nets.size() > 65535) {
+		throw Error(NEMO_INVALID_INPUT, "a batch holds 1 to 65535 networks");
 	}
-	if(conf.stdp && net.maxDelay + conf.postfire.size() > 64) {
-		throw Error(NEMO_INVALID_INPUT, "Max delay + postfire part of STDP function must be <= 64");
+	for(const Network* net : nets) {
+		/* nemo.cpp:111-117 */
+		if(!net || net->neuronCount() == 0) {
+			throw Error(NEMO_INVALID_INPUT, "Cannot create simulation from empty network");
+		}
+		if(conf.stdp && net->maxDelay + conf.postfire.size() > 64) {
+			throw Error(NEMO_INVALID_INPUT, "Max delay + postfire part of STDP function must be <= 64");
+		}
+		if(net->maxDelay > NEMO_MAX_DELAY) {
+			throw Error(NEMO_INVALID_INPUT, "The network has synapses with delay " + std::to_string(net->maxDelay)
+					+ "ms. The HIP backend supports a maximum of 64 ms");
+		}
 	}
-	if(net.maxDelay > NEMO_MAX_DELAY) {
-		throw Error(NEMO_INVALID_INPUT, "The network has synapses with delay " + std::to_string(net.maxDelay)
-				+ "ms. The HIP backend supports a maximum of 64 ms");
-	}
+	m_nets = unsigned(nets.size());
 	m_device = conf.device < 0 ? 0 : conf.device;
 	m_fbits = conf.fbits;
 	m_writeOnly = conf.writeOnlySynapses;
@@ -436,11 +454,11 @@ Simulation::Simulation(const Network& net, const Configuration& conf)
 	HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&m_pinnedScalars), SCALAR_SLOTS * sizeof(uint64_t),
 				hipHostMallocDefault));
 
-	buildNeurons(net);
-	buildConnectivity(net, conf);
+	buildNeurons(nets);
+	buildConnectivity(nets, conf);
 	buildStdp(conf);
 	allocateStepState();
-	chooseLaunchShapes();
+	applyShape(defaultShape());
 
 	nemo_hip_step_buffers b = stepBuffers(0, false, false);
 	for(Group& g : m_groups) {
@@ -448,10 +466,10 @@ Simulation::Simulation(const Network& net, const Configuration& conf)
 	}
 	HIP_CHECK(hipStreamSynchronize(m_stream));
 	if(m_logging) {
-		std::printf("nemo-hip: %zu neurons (%u tiles), %zu synapses (%zu plastic), max delay %u, Q%u.%u, "
-				"delivery %u blocks x %u events, STDP %u blocks\n",
-				m_ncount, m_tiles, m_nsynapses, m_nplastic, m_D, 31 - m_fbits, m_fbits,
-				m_binBlocks, m_binCap, m_stdpBlocks);
+		std::printf("nemo-hip: %u network(s), %zu neurons (%u tiles), %zu synapses (%zu plastic), max delay %u, "
+				"Q%u.%u, delivery %u blocks x %u events, STDP %u blocks (%s)\n",
+				m_nets, m_ncount, m_tiles, m_nsynapses, m_nplastic, m_D, 31 - m_fbits, m_fbits,
+				m_shape.deliverBlocks, m_shape.binCapacity, m_stdpBlocks, m_shape.fuseStdp ? "fused" : "own launch");
 	}
 	resetTimer();
 }
@@ -459,7 +477,7 @@ Simulation::Simulation(const Network& net, const Configuration& conf)
 Simulation::~Simulation()
 {
 	(void) hipSetDevice(m_device);
-	for(auto& kv : m_graphs) (void) hipGraphExecDestroy(kv.second);
+	clearGraphs();
 	if(m_stream) {
 		(void) hipStreamSynchronize(m_stream);
 		(void) hipStreamDestroy(m_stream);
@@ -473,9 +491,17 @@ void Simulation::activate() const
 	HIP_CHECK(hipSetDevice(m_device));
 }
 
-unsigned Simulation::localIdx(unsigned g) const
+static inline uint64_t netKey(unsigned net, unsigned g)
 {
-	auto it = m_g2l.find(g);
+	return (uint64_t(net) << 32) | g;
+}
+
+unsigned Simulation::localIdx(unsigned g, unsigned net) const
+{
+	if(net >= m_nets) {
+		throw Error(NEMO_INVALID_INPUT, "Non-existing network " + std::to_string(net) + " in batch");
+	}
+	auto it = m_g2l.find(netKey(net, g));
 	if(it == m_g2l.end()) {
 		throw Error(NEMO_INVALID_INPUT, "Non-existing neuron index " + std::to_string(g));
 	}
@@ -487,67 +513,107 @@ Simulation::Group& Simulation::groupOf(unsigned l)
 	return m_groups.at(m_typeOfLocal.at(l));
 }
 
-/* Simulation order: type by type (addNeuronType order), insertion order within
- * a type (cpu/Simulation.cpp:60-77, cpu/Neurons.cpp:27-40).  On the device
- * each type starts at a multiple of NEMO_TILE, so that one update block owns
- * one tile of one type.  RNG seeds: mt19937 default seed, four words >> 1 per
- * simulation index (RNG.cpp:38-63, seeded by the CPU backend's local index,
- * cpu/Neurons.cpp:41). */
-void Simulation::buildNeurons(const Network& net)
+/* Simulation order of a network: type by type (addNeuronType order),
+ * insertion order within a type (cpu/Simulation.cpp:60-77, cpu/Neurons.cpp:
+ * 27-40).  On the device the types follow each other (by name, in order of
+ * first appearance in the batch), each starting at a multiple of NEMO_TILE so
+ * that one update block owns one tile of one type; within a type the
+ * networks of a batch follow each other without padding.  RNG seeds: mt19937
+ * default seed, four words >> 1 per simulation index of the neuron in its own
+ * network (RNG.cpp:38-63, seeded by the CPU backend's local index,
+ * cpu/Neurons.cpp:41), so that a network's random streams do not depend on
+ * the batch it is in. */
+void Simulation::buildNeurons(const std::vector<const Network*>& nets)
 {
-	m_ncount = net.neuronCount();
+	std::vector<NeuronType> types;
+	std::map<std::string, unsigned> slotOf;
+	size_t maxNet = 0;
+	m_ncount = 0;
+	for(const Network* net : nets) {
+		for(unsigned t = 0; t < net->types.size(); ++t) {
+			if(net->groups[t].gidx.empty()) continue;
+			if(slotOf.emplace(net->types[t].name, unsigned(types.size())).second) {
+				types.push_back(net->types[t]);
+			}
+		}
+		m_ncount += net->neuronCount();
+		maxNet = std::max<size_t>(maxNet, net->neuronCount());
+	}
+	std::vector<size_t> typeCount(types.size(), 0);
+	for(const Network* net : nets) {
+		for(unsigned t = 0; t < net->types.size(); ++t) {
+			if(!net->groups[t].gidx.empty()) typeCount[slotOf[net->types[t].name]] += net->groups[t].gidx.size();
+		}
+	}
 	size_t npad = 0;
-	for(unsigned t = 0; t < net.types.size(); ++t) {
-		const size_t c = net.groups[t].gidx.size();
-		if(c) npad += (c + NEMO_TILE - 1) / NEMO_TILE * NEMO_TILE;
+	for(size_t c : typeCount) npad += (c + NEMO_TILE - 1) / NEMO_TILE * NEMO_TILE;
+	if(npad >= (size_t(1) << 26)) {
+		throw Error(NEMO_API_UNSUPPORTED, "the HIP backend supports at most 2^26 neurons per simulation");
 	}
 	m_npad = npad;
 	m_tiles = unsigned(npad / NEMO_TILE);
 	m_l2g.assign(m_npad, ~0u);
+	if(m_nets > 1) m_netOf.assign(m_npad, uint16_t(0xffff));
 	m_typeOfLocal.assign(m_npad, -1);
 	m_order.resize(m_ncount);
 	m_g2l.reserve(m_ncount * 2);
 
-	std::vector<uint32_t> seeds(4 * m_ncount);
+	std::vector<uint32_t> seeds(4 * maxNet);
 	{
 		std::mt19937 mt;
 		for(size_t i = 0; i < seeds.size(); ++i) seeds[i] = uint32_t(mt()) >> 1;
 	}
+	std::vector<size_t> netSimBase(nets.size() + 1, 0);
+	for(size_t k = 0; k < nets.size(); ++k) netSimBase[k + 1] = netSimBase[k] + nets[k]->neuronCount();
 
 	unsigned start = 0; /* device index */
-	size_t sim = 0;     /* simulation index */
-	for(unsigned t = 0; t < net.types.size(); ++t) {
-		const Network::Group& ng = net.groups[t];
-		if(ng.gidx.empty()) continue;
+	for(unsigned ds = 0; ds < types.size(); ++ds) {
 		m_groups.emplace_back();
 		Group& g = m_groups.back();
-		g.type = net.types[t];
+		g.type = types[ds];
 		g.plugin.reset(new Plugin(g.type));
 		g.start = start;
-		g.count = unsigned(ng.gidx.size());
-		for(unsigned i = 0; i < g.count; ++i) {
-			if(!m_g2l.emplace(ng.gidx[i], start + i).second) {
-				throw Error(NEMO_INVALID_INPUT, "Duplicate neuron index " + std::to_string(ng.gidx[i]));
-			}
-			m_l2g[start + i] = ng.gidx[i];
-			m_typeOfLocal[start + i] = int(m_groups.size() - 1);
-			m_order[sim + i] = start + i;
-		}
+		g.count = unsigned(typeCount[ds]);
 		const size_t pitch = g.count;
 		std::vector<float> param(std::max<size_t>(1, g.type.nParam * pitch));
 		std::vector<float> state(std::max<size_t>(1, g.type.nState * pitch));
-		for(unsigned p = 0; p < g.type.nParam; ++p) {
-			std::copy(ng.param[p].begin(), ng.param[p].end(), param.begin() + p * pitch);
-		}
-		for(unsigned s = 0; s < g.type.nState; ++s) {
-			std::copy(ng.state[s].begin(), ng.state[s].end(), state.begin() + s * pitch);
+		std::vector<uint32_t> rng(4 * size_t(g.count));
+		size_t pos = 0;
+		for(unsigned k = 0; k < nets.size(); ++k) {
+			const Network& net = *nets[k];
+			auto tid = net.typeIds.find(g.type.name);
+			if(tid == net.typeIds.end()) continue;
+			const unsigned t = tid->second;
+			const Network::Group& ng = net.groups[t];
+			const size_t n = ng.gidx.size();
+			if(n == 0) continue;
+			size_t simBase = 0; /* simulation index of the type's first neuron in its own network */
+			for(unsigned u = 0; u < t; ++u) simBase += net.groups[u].gidx.size();
+			for(size_t i = 0; i < n; ++i) {
+				const unsigned l = unsigned(start + pos + i);
+				if(!m_g2l.emplace(netKey(k, ng.gidx[i]), l).second) {
+					throw Error(NEMO_INVALID_INPUT, "Duplicate neuron index " + std::to_string(ng.gidx[i]));
+				}
+				m_l2g[l] = ng.gidx[i];
+				if(m_nets > 1) m_netOf[l] = uint16_t(k);
+				m_typeOfLocal[l] = int(m_groups.size() - 1);
+				m_order[netSimBase[k] + simBase + i] = l;
+			}
+			std::copy(seeds.begin() + 4 * simBase, seeds.begin() + 4 * (simBase + n), rng.begin() + 4 * pos);
+			for(unsigned p = 0; p < g.type.nParam; ++p) {
+				std::copy(ng.param[p].begin(), ng.param[p].end(), param.begin() + p * pitch + pos);
+			}
+			for(unsigned v = 0; v < g.type.nState; ++v) {
+				std::copy(ng.state[v].begin(), ng.state[v].end(), state.begin() + v * pitch + pos);
+			}
+			pos += n;
 		}
 		g.param.allocate(param.size());
 		g.state.allocate(state.size());
-		g.rng.allocate(4 * size_t(g.count));
+		g.rng.allocate(rng.size());
 		HIP_CHECK(hipMemcpy(g.param.ptr, param.data(), g.param.bytes(), hipMemcpyHostToDevice));
 		HIP_CHECK(hipMemcpy(g.state.ptr, state.data(), g.state.bytes(), hipMemcpyHostToDevice));
-		HIP_CHECK(hipMemcpy(g.rng.ptr, seeds.data() + 4 * sim, g.rng.bytes(), hipMemcpyHostToDevice));
+		HIP_CHECK(hipMemcpy(g.rng.ptr, rng.data(), g.rng.bytes(), hipMemcpyHostToDevice));
 		g.dev.start = g.start;
 		g.dev.count = g.count;
 		g.dev.pitch = pitch;
@@ -555,7 +621,6 @@ void Simulation::buildNeurons(const Network& net)
 		g.dev.state = g.state.ptr;
 		g.dev.rng = g.rng.ptr;
 		start += (g.count + NEMO_TILE - 1) / NEMO_TILE * NEMO_TILE;
-		sim += g.count;
 	}
 }
 
@@ -564,49 +629,66 @@ void Simulation::buildNeurons(const Network& net)
  * row in insertion order (ConnectivityMatrix.cpp:65-176).  The reverse matrix
  * holds, per target, its plastic incoming synapses (construction/RCM.cpp:
  * 112-178): packed source/delay and the forward index of the weight. */
-void Simulation::buildConnectivity(const Network& net, const Configuration& conf)
+void Simulation::buildConnectivity(const std::vector<const Network*>& nets, const Configuration& conf)
 {
-	m_D = std::max(1u, net.maxDelay);
+	m_D = 1;
+	size_t total = 0;
+	for(const Network* net : nets) {
+		m_D = std::max(m_D, net->maxDelay);
+		total += net->synapseCount;
+	}
 	const unsigned D = m_D;
 	const size_t N = m_npad;
-	if(N >= (size_t(1) << 26)) {
-		throw Error(NEMO_API_UNSUPPORTED, "the HIP backend supports at most 2^26 neurons per simulation");
-	}
-	if(net.synapseCount >= (size_t(1) << 32)) {
+	if(total >= (size_t(1) << 32)) {
 		throw Error(NEMO_API_UNSUPPORTED, "the HIP backend supports fewer than 2^32 synapses per simulation");
 	}
 
-	struct Src { unsigned g; unsigned l; const Network::Axon* ax; size_t seq; };
+	/* sources of all networks, each network's in ascending user index */
+	struct Src { unsigned net; unsigned g; unsigned l; const Network::Axon* ax; size_t seq; };
 	std::vector<Src> srcs;
-	srcs.reserve(net.fcm.size());
 	size_t seq = 0;
-	for(auto& kv : net.fcm) {
-		srcs.push_back(Src{kv.first, localIdx(kv.first), &kv.second, seq});
-		seq += kv.second.target.size();
+	for(unsigned k = 0; k < nets.size(); ++k) {
+		for(auto& kv : nets[k]->fcm) {
+			srcs.push_back(Src{k, kv.first, localIdx(kv.first, k), &kv.second, seq});
+			seq += kv.second.target.size();
+		}
 	}
 	m_nsynapses = seq;
 	const long ns = long(srcs.size());
 
 	/* device target index of every synapse, checked (verifySynapseTerminals) */
 	std::vector<uint32_t> ltgt(m_nsynapses);
+	std::vector<uint2> srcWin(srcs.size());
 	int bad = 0;
 	unsigned badIdx = 0;
 #pragma omp parallel for schedule(dynamic, 64)
 	for(long i = 0; i < ns; ++i) {
 		const Src& s = srcs[i];
+		uint32_t lo = ~0u, hi = 0;
 		for(size_t k = 0; k < s.ax->target.size(); ++k) {
-			auto it = m_g2l.find(s.ax->target[k]);
+			auto it = m_g2l.find(netKey(s.net, s.ax->target[k]));
 			if(it == m_g2l.end()) {
 #pragma omp critical
 				{ bad = 1; badIdx = s.ax->target[k]; }
 				ltgt[s.seq + k] = 0;
 			} else {
 				ltgt[s.seq + k] = it->second;
+				lo = std::min(lo, it->second / NEMO_TILE);
+				hi = std::max(hi, it->second / NEMO_TILE + 1);
 			}
 		}
+		srcWin[i] = make_uint2(lo, hi);
 	}
 	if(bad) {
 		throw Error(NEMO_INVALID_INPUT, "Invalid synapse target neuron " + std::to_string(badIdx));
+	}
+	/* target tiles reached from each source tile */
+	m_srcWindow.assign(m_tiles, make_uint2(~0u, 0));
+	for(long i = 0; i < ns; ++i) {
+		if(srcWin[i].y == 0) continue;
+		uint2& w = m_srcWindow[srcs[i].l / NEMO_TILE];
+		w.x = std::min(w.x, srcWin[i].x);
+		w.y = std::max(w.y, srcWin[i].y);
 	}
 
 	/* row lengths and offsets */
@@ -659,7 +741,13 @@ void Simulation::buildConnectivity(const Network& net, const Configuration& conf
 		fxToFix(overflowValue, m_fbits); /* throws with the reference message */
 	}
 	for(const Src& s : srcs) {
-		m_synRange[s.g] = std::make_pair(s.seq, s.seq + s.ax->target.size());
+		m_synRange[netKey(s.net, s.g)] = std::make_pair(s.seq, s.seq + s.ax->target.size());
+	}
+	/* synapses per (source tile, delay) pair: the delivery work the blocks share */
+	m_pairWeight.assign(size_t(m_tiles) * D, 0);
+	for(size_t n = 0; n < N; ++n) {
+		uint64_t* w = &m_pairWeight[(n / NEMO_TILE) * D];
+		for(unsigned d = 0; d < D; ++d) w[d] += rowptr[n * D + d + 1] - rowptr[n * D + d];
 	}
 
 	m_rowptr.allocate(rowptr.size());
@@ -773,78 +861,174 @@ void Simulation::buildStdp(const Configuration& conf)
 	m_propagate.postPreBits = lowMask(post);
 }
 
-static unsigned pow2AtLeast(double x, unsigned lo, unsigned hi)
+/* Split the (source tile, delay) pairs over B delivery blocks with about
+ * equal synapse counts, and give each block its target-tile window.  The raw
+ * window of a block (the targets of its source tiles) is widened so that the
+ * lower and upper ends never decrease from one block to the next; then the
+ * blocks whose windows hold a given tile form one contiguous range, which the
+ * update kernels read. */
+struct Partition {
+	std::vector<uint32_t> bounds;  /* B+1 */
+	std::vector<uint2> windows;    /* B */
+	std::vector<uint32_t> range;   /* per tile: first | end << 16 */
+	unsigned maxWindow = 0;
+};
+
+static Partition partitionPairs(const std::vector<uint64_t>& weight, const std::vector<uint2>& srcWindow,
+		unsigned D, unsigned tiles, unsigned B)
 {
-	unsigned v = lo;
-	while(v < hi && double(v) < x) v <<= 1;
-	return v;
+	Partition P;
+	const size_t Q = weight.size();
+	uint64_t total = 0;
+	for(uint64_t w : weight) total += w;
+	P.bounds.assign(B + 1, 0);
+	size_t q = 0;
+	uint64_t cum = 0;
+	for(unsigned b = 1; b < B; ++b) {
+		const uint64_t target = (total * b + B - 1) / B;
+		while(q < Q && cum < target) cum += weight[q++];
+		P.bounds[b] = uint32_t(q);
+	}
+	P.bounds[B] = uint32_t(Q);
+
+	std::vector<uint32_t> lo(B, tiles), hi(B, 0);
+	for(unsigned b = 0; b < B; ++b) {
+		if(P.bounds[b] == P.bounds[b + 1]) continue;
+		const unsigned p0 = P.bounds[b] / D, p1 = (P.bounds[b + 1] - 1) / D;
+		for(unsigned p = p0; p <= p1; ++p) {
+			if(srcWindow[p].y == 0) continue;
+			lo[b] = std::min(lo[b], srcWindow[p].x);
+			hi[b] = std::max(hi[b], srcWindow[p].y);
+		}
+	}
+	for(unsigned b = 1; b < B; ++b) hi[b] = std::max(hi[b], hi[b - 1]);
+	for(unsigned b = B - 1; b-- > 0;) lo[b] = std::min(lo[b], lo[b + 1]);
+	P.windows.resize(B);
+	for(unsigned b = 0; b < B; ++b) {
+		if(lo[b] > hi[b]) lo[b] = hi[b];
+		P.windows[b] = make_uint2(lo[b], hi[b]);
+		P.maxWindow = std::max(P.maxWindow, hi[b] - lo[b]);
+	}
+	P.range.assign(tiles, 0);
+	unsigned first = 0, end = 0;
+	for(unsigned t = 0; t < tiles; ++t) {
+		while(first < B && hi[first] <= t) ++first;
+		while(end < B && lo[end] <= t) ++end;
+		P.range[t] = first < end ? (first | (end << 16)) : 0u;
+	}
+	return P;
 }
 
-/* Delivery shape: bin capacity from the LDS the device offers, enough blocks
- * for about 1% of the neurons firing per step at half capacity, lanes per
- * row about a quarter of the mean row length (each lane keeps four loads in
- * flight).  Environment variables override each choice (for tuning). */
-void Simulation::chooseLaunchShapes()
+/* Default shape: bin capacity 4096 events (halved while the LDS does not
+ * allow it), enough delivery blocks for about 1% of the synapses per step at
+ * half capacity, STDP fused into the delivery launch, two bitmap words per
+ * STDP block.  NEMO_HIP_DELIVER_BLOCKS, NEMO_HIP_BIN_CAPACITY,
+ * NEMO_HIP_FUSE_STDP and NEMO_HIP_STDP_WORDS override it (tuning). */
+LaunchShape Simulation::defaultShape() const
 {
-	int maxLds = 65536;
-	(void) hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, m_device);
-	/* static LDS of the propagate kernel is about 14 KB */
-	const size_t budget = size_t(maxLds) - 16384;
-	unsigned cap = 4096;
-	while(cap > 256 && nemo_propagate_lds_bytes(m_tiles, cap) > budget) cap >>= 1;
-	if(nemo_propagate_lds_bytes(m_tiles, cap) > budget) {
-		throw Error(NEMO_API_UNSUPPORTED, "network too large for the delivery kernel's LDS");
-	}
-	m_binCap = cap;
-	const double eventsGuess = 0.01 * double(m_nsynapses);
+	LaunchShape s;
 	const uint64_t pairs = uint64_t(m_D) * m_tiles;
-	double blocks = std::ceil(eventsGuess / (0.5 * cap));
-	blocks = std::max(1.0, std::min({blocks, double(NEMO_MAX_BIN_BLOCKS), double(pairs)}));
-	m_binBlocks = unsigned(blocks);
-	m_stdpWords = 2;
+	for(unsigned cap = 4096; cap >= 256; cap >>= 1) {
+		double blocks = std::ceil(0.01 * double(m_nsynapses) / (0.5 * cap));
+		blocks = std::max(1.0, std::min({blocks, double(NEMO_MAX_BIN_BLOCKS), double(pairs)}));
+		s.deliverBlocks = unsigned(blocks);
+		s.binCapacity = cap;
+		const Partition P = partitionPairs(m_pairWeight, m_srcWindow, m_D, m_tiles, s.deliverBlocks);
+		if(nemo_propagate_lds_bytes(std::max(1u, P.maxWindow), cap) <= m_ldsBudget) break;
+	}
 	if(const char* e = std::getenv("NEMO_HIP_DELIVER_BLOCKS")) {
-		m_binBlocks = unsigned(std::max(1, std::min(std::atoi(e), int(NEMO_MAX_BIN_BLOCKS))));
+		s.deliverBlocks = unsigned(std::max(1, std::min(std::atoi(e), int(NEMO_MAX_BIN_BLOCKS))));
 	}
 	if(const char* e = std::getenv("NEMO_HIP_BIN_CAPACITY")) {
-		unsigned c = unsigned(std::atoi(e));
-		if(c >= 64 && c <= 32768 && nemo_propagate_lds_bytes(m_tiles, c) <= budget) m_binCap = c;
+		const int c = std::atoi(e);
+		if(c >= 64 && c <= 32768) s.binCapacity = unsigned(c);
 	}
+	if(const char* e = std::getenv("NEMO_HIP_FUSE_STDP")) s.fuseStdp = std::atoi(e) ? 1 : 0;
 	if(const char* e = std::getenv("NEMO_HIP_STDP_WORDS")) {
-		m_stdpWords = unsigned(std::max(1, std::min(std::atoi(e), 4)));
+		s.stdpWords = unsigned(std::max(1, std::min(std::atoi(e), 4)));
 	}
-	m_stdpBlocks = (m_stdp && m_nplastic) ? unsigned((m_firedWords + m_stdpWords - 1) / m_stdpWords) : 0;
+	return s;
+}
 
-	m_bins.allocate(size_t(m_binBlocks) * m_binCap);
-	m_binTable.allocate(size_t(m_tiles) * m_binBlocks);
+void Simulation::checkShape(const LaunchShape& s) const
+{
+	if(s.deliverBlocks < 1 || s.deliverBlocks > NEMO_MAX_BIN_BLOCKS) {
+		throw Error(NEMO_INVALID_INPUT, "delivery blocks must be in [1, " + std::to_string(NEMO_MAX_BIN_BLOCKS) + "]");
+	}
+	if(s.binCapacity < 64 || s.binCapacity > 32768) {
+		throw Error(NEMO_INVALID_INPUT, "bin capacity must be in [64, 32768]");
+	}
+	if(s.stdpWords < 1 || s.stdpWords > 4) {
+		throw Error(NEMO_INVALID_INPUT, "STDP words per block must be in [1, 4]");
+	}
+	if(s.fuseStdp > 1) {
+		throw Error(NEMO_INVALID_INPUT, "fuse flag must be 0 or 1");
+	}
+}
+
+/* Set up the buffers and kernel arguments of a launch shape.  Bins and table
+ * only carry data from a step's delivery to the same step's update, so the
+ * shape may change between any two steps. */
+void Simulation::applyShape(const LaunchShape& s)
+{
+	checkShape(s);
+	const unsigned B = s.deliverBlocks;
+	const Partition P = partitionPairs(m_pairWeight, m_srcWindow, m_D, m_tiles, B);
+	const unsigned maxWindow = std::max(1u, P.maxWindow);
+	if(nemo_propagate_lds_bytes(maxWindow, s.binCapacity) > m_ldsBudget) {
+		throw Error(NEMO_INVALID_INPUT, "launch shape needs more LDS than a block can have");
+	}
+	activate();
+	HIP_CHECK(hipStreamSynchronize(m_stream));
+	clearGraphs();
+	m_shape = s;
+	m_pairBounds.allocate(P.bounds.size());
+	HIP_CHECK(hipMemcpy(m_pairBounds.ptr, P.bounds.data(), m_pairBounds.bytes(), hipMemcpyHostToDevice));
+	m_windows.allocate(P.windows.size());
+	HIP_CHECK(hipMemcpy(m_windows.ptr, P.windows.data(), m_windows.bytes(), hipMemcpyHostToDevice));
+	m_binRange.allocate(P.range.size());
+	HIP_CHECK(hipMemcpy(m_binRange.ptr, P.range.data(), m_binRange.bytes(), hipMemcpyHostToDevice));
+	m_bins.allocate(size_t(B) * s.binCapacity);
+	m_binTable.allocate(size_t(m_tiles) * B);
 	HIP_CHECK(hipMemset(m_binTable.ptr, 0, m_binTable.bytes()));
-	m_blockEvents.allocate(m_binBlocks);
-	HIP_CHECK(hipMemset(m_blockEvents.ptr, 0, m_blockEvents.bytes()));
-	m_blockExamined.allocate(std::max(1u, m_stdpBlocks));
-	HIP_CHECK(hipMemset(m_blockExamined.ptr, 0, m_blockExamined.bytes()));
+	m_stdpBlocks = (m_stdp && m_nplastic) ? unsigned((m_firedWords + s.stdpWords - 1) / s.stdpWords) : 0;
 
 	nemo_propagate_args& a = m_propagate;
 	a.clock = m_clock.ptr;
 	a.step_offset = 0;
 	a.fired_bits = m_firedBits.ptr;
 	a.fired_words = m_firedWords;
-	a.deliver_blocks = m_nsynapses ? m_binBlocks : 0;
+	a.deliver_blocks = m_nsynapses ? B : 0;
+	a.pair_bounds = m_pairBounds.ptr;
+	a.windows = m_windows.ptr;
+	a.max_window = maxWindow;
 	a.rowptr = m_rowptr.ptr;
 	a.fcm = m_fcm.ptr;
 	a.D = m_D;
 	a.tiles = m_tiles;
-	a.bin_capacity = m_binCap;
+	a.bin_capacity = s.binCapacity;
 	a.bin_table = m_binTable.ptr;
 	a.bins = m_bins.ptr;
 	a.acc = m_acc.ptr;
 	a.acc_pitch = m_accPitch;
 	a.block_events = m_blockEvents.ptr;
-	a.stdp_words = m_stdpBlocks ? m_stdpWords : 0;
+	a.stdp_words = m_stdpBlocks ? s.stdpWords : 0;
 	a.block_examined = m_blockExamined.ptr;
+}
+
+void Simulation::setLaunchShape(const LaunchShape& s)
+{
+	flushStdp();
+	applyShape(s);
 }
 
 void Simulation::allocateStepState()
 {
 	const size_t N = m_npad;
+	int maxLds = 65536;
+	(void) hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, m_device);
+	/* the propagate kernel's static LDS is about 14 KB */
+	m_ldsBudget = size_t(maxLds) - 16384;
 	m_clock.allocate(1);
 	nemo_hip_clock c0;
 	c0.base_step = 0;
@@ -868,6 +1052,12 @@ void Simulation::allocateStepState()
 	HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&m_pinnedBits),
 				size_t(NEMO_STEP_RING) * m_firedWords * sizeof(uint64_t), hipHostMallocDefault));
 
+	/* work counters, sized for any launch shape */
+	m_blockEvents.allocate(NEMO_MAX_BIN_BLOCKS);
+	HIP_CHECK(hipMemset(m_blockEvents.ptr, 0, m_blockEvents.bytes()));
+	m_blockExamined.allocate(std::max<size_t>(1, m_firedWords));
+	HIP_CHECK(hipMemset(m_blockExamined.ptr, 0, m_blockExamined.bytes()));
+
 	m_propagate.rcm_ptr = m_rcmPtr.ptr;
 	m_propagate.rcm_sd = m_rcmSd.ptr;
 	m_propagate.rcm_acc = m_rcmAcc.ptr;
@@ -880,9 +1070,10 @@ nemo_hip_step_buffers Simulation::stepBuffers(uint64_t offset, bool ext, bool fs
 	b.step_offset = offset;
 	b.clock = m_clock.ptr;
 	b.bin_table = m_binTable.ptr;
+	b.bin_range = m_binRange.ptr;
 	b.bins = m_bins.ptr;
-	b.bin_blocks = m_nsynapses ? m_binBlocks : 0;
-	b.bin_capacity = m_binCap;
+	b.bin_blocks = m_nsynapses ? m_shape.deliverBlocks : 0;
+	b.bin_capacity = m_shape.binCapacity;
 	b.acc = m_acc.ptr;
 	b.acc_pitch = m_accPitch;
 	b.ext_current = ext ? m_ext.ptr : nullptr;
@@ -923,7 +1114,7 @@ void Simulation::enqueueStep(uint64_t offset, bool ext, bool fstim, hipEvent_t* 
 	if(ev) HIP_CHECK(hipEventRecord(ev[0], m_stream));
 	nemo_propagate_args a = m_propagate;
 	a.step_offset = offset;
-	HIP_CHECK(nemo_launch_propagate(m_stream, &a, m_stdpBlocks));
+	HIP_CHECK(nemo_launch_propagate(m_stream, &a, m_stdpBlocks, int(m_shape.fuseStdp)));
 	if(ev) HIP_CHECK(hipEventRecord(ev[1], m_stream));
 	nemo_hip_step_buffers b = stepBuffers(offset, ext, fstim);
 	for(Group& g : m_groups) {
@@ -942,7 +1133,7 @@ void Simulation::flushStdp()
 	a.deliver_blocks = 0;
 	a.step_offset = 0;
 	setBase(s + 1);
-	HIP_CHECK(nemo_launch_propagate(m_stream, &a, m_stdpBlocks));
+	HIP_CHECK(nemo_launch_propagate(m_stream, &a, m_stdpBlocks, 1));
 	copyScalar(&m_clock.ptr->stdp_flushed, s);
 	m_flushed = s;
 	HIP_CHECK(hipStreamSynchronize(m_stream));
@@ -959,6 +1150,35 @@ void Simulation::decodeFiring(const uint64_t* bits, std::vector<unsigned>& out) 
 			word &= word - 1;
 			out.push_back(m_l2g[w * 64 + bit]);
 		}
+	}
+}
+
+/* Fired neurons of every network of a batch for one step: out gets each
+ * network's user indices in turn (ascending device order within a network,
+ * which is its simulation order within each type) and counts one entry per
+ * network. */
+void Simulation::decodeFiringBatch(const uint64_t* bits, std::vector<unsigned>& out,
+		std::vector<unsigned>& counts) const
+{
+	if(m_nets == 1) {
+		const size_t before = out.size();
+		decodeFiring(bits, out);
+		counts.push_back(unsigned(out.size() - before));
+		return;
+	}
+	std::vector<std::vector<unsigned> > per(m_nets);
+	for(size_t w = 0; w < m_firedWords; ++w) {
+		uint64_t word = bits[w];
+		while(word) {
+			const unsigned bit = unsigned(__builtin_ctzll(word));
+			word &= word - 1;
+			const size_t l = w * 64 + bit;
+			per[m_netOf[l]].push_back(m_l2g[l]);
+		}
+	}
+	for(const std::vector<unsigned>& v : per) {
+		out.insert(out.end(), v.begin(), v.end());
+		counts.push_back(unsigned(v.size()));
 	}
 }
 
@@ -980,6 +1200,9 @@ const std::vector<unsigned>& Simulation::step(const unsigned* fstim, size_t nf,
 		const unsigned* istimIdx, const float* istimVal, size_t ni)
 {
 	activate();
+	if(m_nets > 1) {
+		throw Error(NEMO_API_UNSUPPORTED, "a batch of networks is advanced with nemo_hip_run");
+	}
 	/* current stimulus, then firing stimulus (SimulationBackend.cpp:61-77);
 	 * for repeated neurons the last current wins (cpu/Simulation.cpp:140-145) */
 	std::vector<uint32_t> iIdx;
@@ -1050,6 +1273,12 @@ hipGraphExec_t Simulation::graphFor(unsigned k)
 	return exec;
 }
 
+void Simulation::clearGraphs()
+{
+	for(auto& kv : m_graphs) (void) hipGraphExecDestroy(kv.second);
+	m_graphs.clear();
+}
+
 uint64_t Simulation::sumCounters(const DeviceArray<unsigned long long>& a)
 {
 	std::vector<unsigned long long> h(a.n);
@@ -1087,9 +1316,7 @@ void Simulation::run(unsigned nsteps, unsigned stdpPeriod, float reward, bool re
 		HIP_CHECK(hipStreamSynchronize(m_stream));
 		if(readRaster) {
 			for(unsigned s = 0; s < k; ++s) {
-				const size_t before = fired->size();
-				decodeFiring(m_pinnedBits + size_t(s) * m_firedWords, *fired);
-				counts->push_back(unsigned(fired->size() - before));
+				decodeFiringBatch(m_pinnedBits + size_t(s) * m_firedWords, *fired, *counts);
 			}
 		}
 		done += k;
@@ -1147,6 +1374,169 @@ void Simulation::profile(unsigned nsteps, double avgUs[4], uint64_t counters[4])
 	counters[3] = nsteps;
 }
 
+void Simulation::prepare(unsigned nsteps, unsigned stdpPeriod)
+{
+	activate();
+	uint64_t since = 0;
+	unsigned done = 0;
+	while(done < nsteps) {
+		unsigned k = std::min(64u, nsteps - done);
+		if(stdpPeriod) k = unsigned(std::min<uint64_t>(k, stdpPeriod - since));
+		graphFor(k);
+		done += k;
+		since += k;
+		if(stdpPeriod && since == stdpPeriod) since = 0;
+	}
+}
+
+/* Device state a step changes (the weights change only in applyStdp). */
+struct Simulation::Snapshot {
+	std::vector<std::pair<void*, size_t> > regions;
+	std::vector<DeviceArray<unsigned char> > copies;
+	uint64_t hostStep = 0;
+	uint64_t flushed = 0;
+};
+
+void Simulation::takeSnapshot(Snapshot& s)
+{
+	activate();
+	HIP_CHECK(hipStreamSynchronize(m_stream));
+	s.regions.clear();
+	auto add = [&](void* p, size_t bytes) {
+		if(p && bytes) s.regions.emplace_back(p, bytes);
+	};
+	for(Group& g : m_groups) {
+		add(g.state.ptr, g.state.bytes());
+		add(g.rng.ptr, g.rng.bytes());
+	}
+	add(m_clock.ptr, m_clock.bytes());
+	add(m_firedBits.ptr, m_firedBits.bytes());
+	add(m_acc.ptr, m_acc.bytes());
+	add(m_hist.ptr, m_hist.bytes());
+	add(m_ext.ptr, m_ext.bytes());
+	add(m_rcmAcc.ptr, m_rcmAcc.bytes());
+	add(m_blockEvents.ptr, m_blockEvents.bytes());
+	add(m_blockExamined.ptr, m_blockExamined.bytes());
+	s.copies.clear();
+	s.copies.resize(s.regions.size());
+	for(size_t i = 0; i < s.regions.size(); ++i) {
+		s.copies[i].allocate(s.regions[i].second);
+		HIP_CHECK(hipMemcpyAsync(s.copies[i].ptr, s.regions[i].first, s.regions[i].second,
+					hipMemcpyDeviceToDevice, m_stream));
+	}
+	HIP_CHECK(hipStreamSynchronize(m_stream));
+	s.hostStep = m_hostStep;
+	s.flushed = m_flushed;
+}
+
+void Simulation::restoreSnapshot(const Snapshot& s)
+{
+	activate();
+	HIP_CHECK(hipStreamSynchronize(m_stream));
+	for(size_t i = 0; i < s.regions.size(); ++i) {
+		HIP_CHECK(hipMemcpyAsync(s.regions[i].first, s.copies[i].ptr, s.regions[i].second,
+					hipMemcpyDeviceToDevice, m_stream));
+	}
+	HIP_CHECK(hipStreamSynchronize(m_stream));
+	m_hostStep = s.hostStep;
+	m_flushed = s.flushed;
+}
+
+/* Advance nsteps steps in graphs of up to 64 steps; seconds on the stream. */
+double Simulation::timeSteps(unsigned nsteps)
+{
+	hipEvent_t e0, e1;
+	HIP_CHECK(hipEventCreate(&e0));
+	HIP_CHECK(hipEventCreate(&e1));
+	HIP_CHECK(hipEventRecord(e0, m_stream));
+	unsigned done = 0;
+	while(done < nsteps) {
+		const unsigned k = std::min(64u, nsteps - done);
+		hipGraphExec_t g = graphFor(k);
+		setBase(m_hostStep);
+		HIP_CHECK(hipGraphLaunch(g, m_stream));
+		m_hostStep += k;
+		done += k;
+	}
+	HIP_CHECK(hipEventRecord(e1, m_stream));
+	HIP_CHECK(hipStreamSynchronize(m_stream));
+	float ms = 0;
+	HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+	(void) hipEventDestroy(e0);
+	(void) hipEventDestroy(e1);
+	return 1e-3 * ms;
+}
+
+/* Coordinate descent over delivery blocks, bin capacity, STDP fusion and
+ * STDP block width; every trial runs the same steps from the same state. */
+LaunchShape Simulation::autotune(unsigned stepsPerTrial)
+{
+	activate();
+	if(m_nsynapses == 0 || stepsPerTrial == 0) return m_shape;
+	stepsPerTrial = std::min(stepsPerTrial, 1024u);
+	Snapshot snap;
+	takeSnapshot(snap);
+	const LaunchShape start = m_shape;
+	auto trial = [&](const LaunchShape& c) -> double {
+		try {
+			applyShape(c);
+		} catch(const Error&) {
+			return 1e30;
+		}
+		restoreSnapshot(snap);
+		timeSteps(std::min(64u, stepsPerTrial)); /* captures the graph, warms caches */
+		restoreSnapshot(snap);
+		return timeSteps(stepsPerTrial);
+	};
+	LaunchShape best = start;
+	double tBest = trial(best);
+	auto consider = [&](const LaunchShape& c) {
+		const double t = trial(c);
+		if(t < tBest) {
+			tBest = t;
+			best = c;
+		}
+	};
+	const unsigned maxBlocks = unsigned(std::min<uint64_t>(NEMO_MAX_BIN_BLOCKS, uint64_t(m_D) * m_tiles));
+	std::vector<unsigned> blocks;
+	for(double f : {0.25, 0.5, 2.0, 4.0}) {
+		const unsigned b = unsigned(std::max(1.0, std::min(double(maxBlocks), std::round(start.deliverBlocks * f))));
+		if(b != start.deliverBlocks && std::find(blocks.begin(), blocks.end(), b) == blocks.end()) blocks.push_back(b);
+	}
+	for(unsigned b : blocks) {
+		LaunchShape c = best;
+		c.deliverBlocks = b;
+		consider(c);
+	}
+	for(unsigned cap : {1024u, 2048u, 4096u, 8192u, 16384u}) {
+		if(cap == best.binCapacity) continue;
+		LaunchShape c = best;
+		c.binCapacity = cap;
+		consider(c);
+	}
+	if(m_stdp && m_nplastic) {
+		{
+			LaunchShape c = best;
+			c.fuseStdp = 1 - best.fuseStdp;
+			consider(c);
+		}
+		for(unsigned w : {1u, 2u, 4u}) {
+			if(w == best.stdpWords) continue;
+			LaunchShape c = best;
+			c.stdpWords = w;
+			consider(c);
+		}
+	}
+	applyShape(best);
+	restoreSnapshot(snap);
+	if(m_logging) {
+		std::printf("nemo-hip: autotuned launch shape: %u delivery blocks x %u events, STDP %s, %u words "
+				"(%.1f us/step)\n", best.deliverBlocks, best.binCapacity, best.fuseStdp ? "fused" : "own launch",
+				best.stdpWords, 1e6 * tBest / stepsPerTrial);
+	}
+	return best;
+}
+
 void Simulation::applyStdp(float reward)
 {
 	activate();
@@ -1178,10 +1568,10 @@ void Simulation::applyStdp(float reward)
 
 /* --------------------------------------------------------- neuron access */
 
-float Simulation::getNeuronState(unsigned idx, unsigned var)
+float Simulation::getNeuronState(unsigned idx, unsigned var, unsigned net)
 {
 	activate();
-	unsigned l = localIdx(idx);
+	unsigned l = localIdx(idx, net);
 	Group& g = groupOf(l);
 	if(var >= g.type.nState) throw Error(NEMO_INVALID_INPUT, "Invalid state variable index " + std::to_string(var));
 	float v;
@@ -1190,10 +1580,10 @@ float Simulation::getNeuronState(unsigned idx, unsigned var)
 	return v;
 }
 
-float Simulation::getNeuronParameter(unsigned idx, unsigned param)
+float Simulation::getNeuronParameter(unsigned idx, unsigned param, unsigned net)
 {
 	activate();
-	unsigned l = localIdx(idx);
+	unsigned l = localIdx(idx, net);
 	Group& g = groupOf(l);
 	if(param >= g.type.nParam) throw Error(NEMO_INVALID_INPUT, "Invalid parameter index " + std::to_string(param));
 	float v;
@@ -1202,14 +1592,14 @@ float Simulation::getNeuronParameter(unsigned idx, unsigned param)
 	return v;
 }
 
-float Simulation::getMembranePotential(unsigned idx)
+float Simulation::getMembranePotential(unsigned idx, unsigned net)
 {
-	unsigned l = localIdx(idx);
+	unsigned l = localIdx(idx, net);
 	Group& g = groupOf(l);
 	if(g.type.membranePotential >= g.type.nState) {
 		throw Error(NEMO_INVALID_INPUT, "neuron type " + g.type.name + " has no membrane potential");
 	}
-	return getNeuronState(idx, g.type.membranePotential);
+	return getNeuronState(idx, g.type.membranePotential, net);
 }
 
 void Simulation::setNeuron(unsigned idx, unsigned nargs, const float* args)
@@ -1272,7 +1662,7 @@ size_t Simulation::synapseIndex(synapse_id id) const
 	}
 	unsigned src = unsigned(id >> 32);
 	size_t k = size_t(id & 0xffffffffu);
-	auto it = m_synRange.find(src);
+	auto it = m_synRange.find(netKey(0, src));
 	if(it == m_synRange.end()) {
 		throw Error(NEMO_INVALID_INPUT, "Non-existing neuron id (" + std::to_string(src) + ") in synapse query");
 	}
@@ -1288,7 +1678,7 @@ const std::vector<synapse_id>& Simulation::getSynapsesFrom(unsigned source)
 		throw Error(NEMO_INVALID_INPUT, "Cannot read synapse state if simulation configured with write-only synapses");
 	}
 	m_queried.clear();
-	auto it = m_synRange.find(source);
+	auto it = m_synRange.find(netKey(0, source));
 	if(it == m_synRange.end()) {
 		localIdx(source); /* throws for a non-existing neuron */
 	} else {
@@ -1374,10 +1764,15 @@ void Simulation::getRngState(uint32_t* s, size_t n)
 	if(n != m_ncount) throw Error(NEMO_INVALID_INPUT, "size mismatch");
 	activate();
 	HIP_CHECK(hipStreamSynchronize(m_stream));
-	size_t sim = 0;
 	for(Group& g : m_groups) {
-		HIP_CHECK(hipMemcpy(s + 4 * sim, g.rng.ptr, g.rng.bytes(), hipMemcpyDeviceToHost));
-		sim += g.count;
+		std::vector<uint32_t> h(g.rng.n);
+		HIP_CHECK(hipMemcpy(h.data(), g.rng.ptr, g.rng.bytes(), hipMemcpyDeviceToHost));
+		for(size_t i = 0; i < n; ++i) {
+			const unsigned l = m_order[i];
+			if(l >= g.start && l < g.start + g.count) {
+				std::copy(h.begin() + 4 * size_t(l - g.start), h.begin() + 4 * size_t(l - g.start + 1), s + 4 * i);
+			}
+		}
 	}
 }
 

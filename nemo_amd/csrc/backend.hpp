@@ -146,23 +146,51 @@ struct DeviceArray {
 	size_t bytes() const { return n * sizeof(T); }
 };
 
-/* The HIP simulation (cuda::Simulation's role, cuda/Simulation.cpp:91-380) */
+/* Launch shape of the per-step kernels; any shape gives the same results. */
+struct LaunchShape {
+	unsigned deliverBlocks = 1;   /* delivery blocks (bin regions), <= NEMO_MAX_BIN_BLOCKS */
+	unsigned binCapacity = 4096;  /* events a delivery block stages in LDS */
+	unsigned fuseStdp = 1;        /* STDP blocks in the delivery launch (1) or their own (0) */
+	unsigned stdpWords = 2;       /* firing-bitmap words (64 targets each) per STDP block */
+};
+
+/* The HIP simulation (cuda::Simulation's role, cuda/Simulation.cpp:91-380).
+ *
+ * A simulation advances one network, or a batch of independent networks in
+ * the same launches: each network keeps its own neuron indices, RNG seeds
+ * (as in its own simulation) and firing output; neuron types are shared by
+ * name.  Single-network calls address network 0. */
 class Simulation {
 public:
 	Simulation(const Network& net, const Configuration& conf);
+	Simulation(const std::vector<const Network*>& nets, const Configuration& conf);
 	~Simulation();
+
+	unsigned networkCount() const { return m_nets; }
 
 	/* SimulationBackend::step (SimulationBackend.cpp:61-77) */
 	const std::vector<unsigned>& step(const unsigned* fstim, size_t nf,
 			const unsigned* istimIdx, const float* istimVal, size_t ni);
 	void applyStdp(float reward);
+	/* nsteps steps without stimulus; with readRaster, counts gets [step][network]
+	 * firing counts and fired the fired neurons (by step, then network, then
+	 * ascending simulation order) */
 	void run(unsigned nsteps, unsigned stdpPeriod, float reward, bool readRaster,
 			std::vector<unsigned>* counts, std::vector<unsigned>* fired, uint64_t* events);
+	/* capture the step graphs a run(nsteps, stdpPeriod) uses */
+	void prepare(unsigned nsteps, unsigned stdpPeriod);
 	void profile(unsigned nsteps, double avgUs[4], uint64_t counters[4]);
 
-	float getNeuronState(unsigned idx, unsigned var);
-	float getNeuronParameter(unsigned idx, unsigned param);
-	float getMembranePotential(unsigned idx);
+	LaunchShape launchShape() const { return m_shape; }
+	void setLaunchShape(const LaunchShape& s);
+	/* time candidate launch shapes over stepsPerTrial steps each, from a
+	 * snapshot of the simulation state that is restored afterwards; keep the
+	 * fastest */
+	LaunchShape autotune(unsigned stepsPerTrial);
+
+	float getNeuronState(unsigned idx, unsigned var, unsigned net = 0);
+	float getNeuronParameter(unsigned idx, unsigned param, unsigned net = 0);
+	float getMembranePotential(unsigned idx, unsigned net = 0);
 	void setNeuron(unsigned idx, unsigned nargs, const float* args);
 	void setNeuronState(unsigned idx, unsigned var, float val);
 	void setNeuronParameter(unsigned idx, unsigned param, float val);
@@ -204,6 +232,7 @@ private:
 
 	int m_device = 0;
 	hipStream_t m_stream = nullptr;
+	unsigned m_nets = 1;   /* networks in the batch */
 	size_t m_ncount = 0;   /* neurons */
 	size_t m_npad = 0;     /* device index space: each type padded to whole tiles */
 	unsigned m_tiles = 0;
@@ -214,8 +243,9 @@ private:
 	bool m_logging = false;
 
 	std::vector<unsigned> m_l2g;          /* device index -> user index (~0 for padding) */
-	std::vector<uint32_t> m_order;        /* simulation index -> device index */
-	std::unordered_map<unsigned, unsigned> m_g2l; /* user index -> device index */
+	std::vector<uint16_t> m_netOf;        /* device index -> network (batches only) */
+	std::vector<uint32_t> m_order;        /* simulation index (networks back to back) -> device index */
+	std::unordered_map<uint64_t, unsigned> m_g2l; /* network << 32 | user index -> device index */
 	std::vector<int> m_typeOfLocal;       /* device index -> group (-1 for padding) */
 	std::vector<Group> m_groups;
 
@@ -238,20 +268,25 @@ private:
 	DeviceArray<float> m_stimVal;
 	DeviceArray<int2> m_bins;
 	DeviceArray<uint32_t> m_binTable;
+	DeviceArray<uint32_t> m_pairBounds, m_binRange;
+	DeviceArray<uint2> m_windows;
 	DeviceArray<unsigned long long> m_blockEvents, m_blockExamined;
+	std::vector<uint2> m_srcWindow;       /* per source tile: target tiles [x, y) of its synapses */
+	std::vector<uint64_t> m_pairWeight;   /* synapses per (source tile, delay) pair */
 
 	/* STDP */
 	bool m_stdp = false;
 	int32_t m_minE = 0, m_maxE = 0, m_minI = 0, m_maxI = 0;
 	uint64_t m_flushed = ~uint64_t(0); /* step whose STDP accumulation ran on its own */
 
-	/* propagate-kernel arguments and launch shapes (chosen per network) */
+	/* propagate-kernel arguments and launch shape */
 	nemo_propagate_args m_propagate;
-	unsigned m_binBlocks = 1, m_binCap = 4096;
-	unsigned m_stdpBlocks = 0, m_stdpWords = 2;
+	LaunchShape m_shape;
+	unsigned m_stdpBlocks = 0;
+	size_t m_ldsBudget = 0;
 
 	/* host side of synapse queries (network order = source ascending, then insertion) */
-	std::map<unsigned, std::pair<size_t, size_t> > m_synRange; /* gsrc -> [first,last) */
+	std::map<uint64_t, std::pair<size_t, size_t> > m_synRange; /* network << 32 | gsrc -> [first,last) */
 	std::vector<uint32_t> m_synAddr;   /* synapse -> fcm index */
 	std::vector<uint32_t> m_synTarget; /* synapse -> user target index */
 	std::vector<uint8_t> m_synDelay;
@@ -274,24 +309,66 @@ private:
 	std::map<unsigned, hipGraphExec_t> m_graphs;
 
 	void activate() const;
-	unsigned localIdx(unsigned g) const;
+	unsigned localIdx(unsigned g, unsigned net = 0) const;
 	Group& groupOf(unsigned l);
-	void buildNeurons(const Network& net);
-	void buildConnectivity(const Network& net, const Configuration& conf);
+	void construct(const std::vector<const Network*>& nets, const Configuration& conf);
+	void buildNeurons(const std::vector<const Network*>& nets);
+	void buildConnectivity(const std::vector<const Network*>& nets, const Configuration& conf);
 	void buildStdp(const Configuration& conf);
 	void allocateStepState();
-	void chooseLaunchShapes();
+	LaunchShape defaultShape() const;
+	void applyShape(const LaunchShape& s);
+	void checkShape(const LaunchShape& s) const;
 	nemo_hip_step_buffers stepBuffers(uint64_t offset, bool ext, bool fstim) const;
 	void copyScalar(uint64_t* dst, uint64_t value);
 	void setBase(uint64_t base);
 	void enqueueStep(uint64_t offset, bool ext, bool fstim, hipEvent_t* ev = nullptr);
 	void flushStdp();
 	void decodeFiring(const uint64_t* bits, std::vector<unsigned>& out) const;
+	void decodeFiringBatch(const uint64_t* bits, std::vector<unsigned>& out, std::vector<unsigned>& counts) const;
 	void readBitmaps(uint64_t t0, unsigned k);
 	uint64_t sumCounters(const DeviceArray<unsigned long long>& a);
 	hipGraphExec_t graphFor(unsigned k);
+	void clearGraphs();
+	struct Snapshot;
+	void takeSnapshot(Snapshot& s);
+	void restoreSnapshot(const Snapshot& s);
+	double timeSteps(unsigned nsteps);
 	void syncFcmHost();
 	size_t synapseIndex(synapse_id id) const;
+};
+
+/* A network prepared for the float-weight mode and its adjoint
+ * (float_mode.hip).  One Izhikevich neuron type with indices 0..N-1;
+ * weights, parameters, inputs and outputs are caller-owned device arrays
+ * (neurons by index, synapses in network order).  Calls on one object must
+ * not overlap (they share its workspace). */
+class FloatNetwork {
+public:
+	FloatNetwork(const Network& net, int device);
+
+	size_t neurons() const { return m_n; }
+	size_t synapses() const { return m_s; }
+	unsigned maxDelay() const { return m_D; }
+
+	void forward(hipStream_t st, unsigned T, unsigned K, const float* w, const float* a, const float* b,
+			const float* c, const float* d, const float* v0, const float* u0, const float* x,
+			float* S, float* V, float* ckpt);
+	void backward(hipStream_t st, unsigned T, unsigned K, const float* w, const float* a, const float* b,
+			const float* c, const float* d, const float* x, const float* S, const float* ckpt,
+			const float* gS, const float* gV, float* gw, float* ga, float* gb, float* gd, float* gx,
+			float* gv0, float* gu0);
+
+private:
+	int m_device = 0;
+	size_t m_n = 0, m_s = 0;
+	unsigned m_D = 1;
+	DeviceArray<uint32_t> m_fcmPtr, m_fcmTd;          /* network order is forward order */
+	DeviceArray<uint32_t> m_rcmPtr, m_rcmSd, m_rcmUser;
+	DeviceArray<float> m_wr, m_v, m_u, m_LV, m_LU, m_LI, m_seg;
+	unsigned m_segK = 0;
+
+	void prepare(unsigned T, unsigned K);
 };
 
 } // namespace nemo_hip

@@ -1,0 +1,84 @@
+/*
+ * Launch interface of the float-weight mode kernels (float_mode.hip): plain
+ * C ABI, device pointers, sizes and a stream; each returns the launch status.
+ */
+#ifndef NEMO_AMD_FLOAT_MODE_H
+#define NEMO_AMD_FLOAT_MODE_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One forward step t of the float-weight network (all N neurons). */
+typedef struct nemo_float_step_args {
+	unsigned n;                /* neurons */
+	unsigned t;                /* step */
+	const uint32_t* rcm_ptr;   /* n+1 offsets of each target's incoming synapses */
+	const uint32_t* rcm_sd;    /* source | (delay-1) << 26 */
+	const float* wr;           /* weights in reverse-matrix order */
+	const float* pa;           /* Izhikevich a, b, c, d per neuron */
+	const float* pb;
+	const float* pc;
+	const float* pd;
+	const float* x;            /* [T][n] external current */
+	float* S;                  /* [T][n] spikes (read for t-d, written at t when store) */
+	float* V;                  /* [T][n] membrane potential after the step (store) */
+	float* v;                  /* [n] state, updated */
+	float* u;
+	int store;                 /* write S[t], V[t] */
+	int record;                /* write the step's entry state and input to seg_* */
+	unsigned t0;               /* first step of the recorded segment */
+	float* seg_v;              /* [K][n] */
+	float* seg_u;
+	float* seg_I;
+	float* ckpt;               /* NULL, or [ceil(T/K)][2][n]: (v, u) before every K-th step */
+	unsigned K;
+} nemo_float_step_args;
+
+/* One backward step t (in decreasing t) of the adjoint. */
+typedef struct nemo_float_back_args {
+	unsigned n;
+	unsigned t;
+	unsigned T;                /* steps of the forward run */
+	unsigned ring;             /* slots of LI (max delay + 1) */
+	const uint32_t* fcm_ptr;   /* n+1 offsets of each source's synapses */
+	const uint32_t* fcm_td;    /* target | (delay-1) << 26 */
+	const float* wf;           /* weights in forward-matrix order */
+	const float* pa;
+	const float* pb;
+	const float* pc;
+	const float* pd;
+	const float* S;            /* [T][n] forward spikes */
+	const float* gS;           /* [T][n] dL/dS, or NULL */
+	const float* gV;           /* [T][n] dL/dV, or NULL */
+	unsigned t0;               /* first step of the recorded segment */
+	const float* seg_v;        /* recorded entry state and input of the segment */
+	const float* seg_u;
+	const float* seg_I;
+	float* LV;                 /* [n] dL/dv of the state after step t; becomes the one before */
+	float* LU;
+	float* LI;                 /* [ring][n] dL/dI of recent steps */
+	float* gw;                 /* [synapses] dL/dw in forward-matrix order, accumulated */
+	float* ga;                 /* [n] accumulated */
+	float* gb;
+	float* gd;
+	float* gx;                 /* [T][n] dL/dx, or NULL */
+} nemo_float_back_args;
+
+hipError_t nemo_launch_float_forward_step(hipStream_t, const nemo_float_step_args*);
+hipError_t nemo_launch_float_backward_step(hipStream_t, const nemo_float_back_args*);
+/* out[k] = in[idx[k]] */
+hipError_t nemo_launch_gather(hipStream_t, float* out, const float* in, const uint32_t* idx, size_t n);
+/* out[idx[k]] = in[k] (idx a permutation) */
+hipError_t nemo_launch_scatter(hipStream_t, float* out, const float* in, const uint32_t* idx, size_t n);
+hipError_t nemo_launch_fill(hipStream_t, float* p, float v, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,228 @@
+/*
+ * Host side of the float-weight mode: connectivity in forward (network)
+ * order and reverse (by target) order, and the forward / checkpointed
+ * adjoint step loops over the kernels of float_mode.hip.
+ */
+#include "backend.hpp"
+#include "float_mode.h"
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+namespace nemo_hip {
+
+FloatNetwork::FloatNetwork(const Network& net, int device) : m_device(device < 0 ? 0 : device)
+{
+	HIP_CHECK(hipSetDevice(m_device));
+	unsigned types = 0;
+	const Network::Group* grp = nullptr;
+	for(unsigned t = 0; t < net.types.size(); ++t) {
+		if(net.groups[t].gidx.empty()) continue;
+		++types;
+		grp = &net.groups[t];
+		if(net.types[t].name != "Izhikevich") {
+			throw Error(NEMO_API_UNSUPPORTED, "the float-weight mode supports Izhikevich neurons only");
+		}
+	}
+	if(types != 1) {
+		throw Error(NEMO_INVALID_INPUT, "the float-weight mode needs one neuron type");
+	}
+	m_n = grp->gidx.size();
+	if(m_n >= (size_t(1) << 26)) throw Error(NEMO_API_UNSUPPORTED, "too many neurons for the float-weight mode");
+	std::vector<char> seen(m_n, 0);
+	for(unsigned g : grp->gidx) {
+		if(g >= m_n || seen[g]) {
+			throw Error(NEMO_INVALID_INPUT, "the float-weight mode needs neuron indices 0..N-1");
+		}
+		seen[g] = 1;
+	}
+	m_s = net.synapseCount;
+	if(m_s >= (size_t(1) << 32)) throw Error(NEMO_API_UNSUPPORTED, "too many synapses for the float-weight mode");
+	m_D = std::max(1u, net.maxDelay);
+
+	/* forward order = network order: sources ascending, insertion order */
+	std::vector<uint32_t> fptr(m_n + 1, 0), ftd(std::max<size_t>(1, m_s)), tgtOf(std::max<size_t>(1, m_s));
+	std::vector<uint32_t> incoming(m_n + 1, 0);
+	size_t k = 0;
+	for(auto& kv : net.fcm) {
+		const Network::Axon& ax = kv.second;
+		if(kv.first >= m_n) {
+			throw Error(NEMO_INVALID_INPUT, "Invalid synapse source neuron " + std::to_string(kv.first));
+		}
+		fptr[kv.first + 1] = uint32_t(ax.target.size());
+		for(size_t i = 0; i < ax.target.size(); ++i, ++k) {
+			const uint32_t tgt = ax.target[i];
+			if(tgt >= m_n) throw Error(NEMO_INVALID_INPUT, "Invalid synapse target neuron " + std::to_string(tgt));
+			ftd[k] = tgt | (uint32_t(ax.delay[i] - 1) << 26);
+			tgtOf[k] = tgt;
+			incoming[tgt + 1] += 1;
+		}
+	}
+	for(size_t i = 0; i < m_n; ++i) fptr[i + 1] += fptr[i];
+	/* reverse order: by target, then network order */
+	for(size_t i = 0; i < m_n; ++i) incoming[i + 1] += incoming[i];
+	std::vector<uint32_t> cursor(incoming.begin(), incoming.end() - 1);
+	std::vector<uint32_t> rsd(std::max<size_t>(1, m_s)), ruser(std::max<size_t>(1, m_s));
+	k = 0;
+	for(auto& kv : net.fcm) {
+		const Network::Axon& ax = kv.second;
+		for(size_t i = 0; i < ax.target.size(); ++i, ++k) {
+			const uint32_t pos = cursor[tgtOf[k]]++;
+			rsd[pos] = kv.first | (uint32_t(ax.delay[i] - 1) << 26);
+			ruser[pos] = uint32_t(k);
+		}
+	}
+	auto upload = [](DeviceArray<uint32_t>& d, const std::vector<uint32_t>& h) {
+		d.allocate(h.size());
+		HIP_CHECK(hipMemcpy(d.ptr, h.data(), d.bytes(), hipMemcpyHostToDevice));
+	};
+	upload(m_fcmPtr, fptr);
+	upload(m_fcmTd, ftd);
+	upload(m_rcmPtr, incoming);
+	upload(m_rcmSd, rsd);
+	upload(m_rcmUser, ruser);
+	m_wr.allocate(std::max<size_t>(1, m_s));
+	m_v.allocate(m_n);
+	m_u.allocate(m_n);
+	m_LV.allocate(m_n);
+	m_LU.allocate(m_n);
+	m_LI.allocate(size_t(m_D + 1) * m_n);
+}
+
+void FloatNetwork::prepare(unsigned T, unsigned K)
+{
+	if(T == 0) throw Error(NEMO_INVALID_INPUT, "the float-weight mode needs at least one step");
+	if(K == 0) throw Error(NEMO_INVALID_INPUT, "checkpoint interval must be positive");
+	HIP_CHECK(hipSetDevice(m_device));
+	const unsigned k = std::min(K, T);
+	if(k > m_segK) {
+		m_seg.allocate(3 * size_t(k) * m_n);
+		m_segK = k;
+	}
+}
+
+void FloatNetwork::forward(hipStream_t st, unsigned T, unsigned K, const float* w, const float* a,
+		const float* b, const float* c, const float* d, const float* v0, const float* u0, const float* x,
+		float* S, float* V, float* ckpt)
+{
+	prepare(T, K);
+	HIP_CHECK(nemo_launch_gather(st, m_wr.ptr, w, m_rcmUser.ptr, m_s));
+	HIP_CHECK(hipMemcpyAsync(m_v.ptr, v0, m_n * sizeof(float), hipMemcpyDeviceToDevice, st));
+	HIP_CHECK(hipMemcpyAsync(m_u.ptr, u0, m_n * sizeof(float), hipMemcpyDeviceToDevice, st));
+	nemo_float_step_args f = {};
+	f.n = unsigned(m_n);
+	f.rcm_ptr = m_rcmPtr.ptr;
+	f.rcm_sd = m_rcmSd.ptr;
+	f.wr = m_wr.ptr;
+	f.pa = a;
+	f.pb = b;
+	f.pc = c;
+	f.pd = d;
+	f.x = x;
+	f.S = S;
+	f.V = V;
+	f.v = m_v.ptr;
+	f.u = m_u.ptr;
+	f.store = 1;
+	f.record = 0;
+	f.ckpt = ckpt;
+	f.K = K;
+	for(unsigned t = 0; t < T; ++t) {
+		f.t = t;
+		HIP_CHECK(nemo_launch_float_forward_step(st, &f));
+	}
+}
+
+/* Segments of K steps in reverse: restore the segment's checkpoint, replay
+ * its forward steps recording entry state and input, then run its adjoint
+ * steps backwards. */
+void FloatNetwork::backward(hipStream_t st, unsigned T, unsigned K, const float* w, const float* a,
+		const float* b, const float* c, const float* d, const float* x, const float* S, const float* ckpt,
+		const float* gS, const float* gV, float* gw, float* ga, float* gb, float* gd, float* gx,
+		float* gv0, float* gu0)
+{
+	prepare(T, K);
+	HIP_CHECK(nemo_launch_gather(st, m_wr.ptr, w, m_rcmUser.ptr, m_s));
+	HIP_CHECK(nemo_launch_fill(st, m_LV.ptr, 0.0f, m_n));
+	HIP_CHECK(nemo_launch_fill(st, m_LU.ptr, 0.0f, m_n));
+	HIP_CHECK(nemo_launch_fill(st, m_LI.ptr, 0.0f, m_LI.n));
+	HIP_CHECK(nemo_launch_fill(st, gw, 0.0f, m_s));
+	HIP_CHECK(nemo_launch_fill(st, ga, 0.0f, m_n));
+	HIP_CHECK(nemo_launch_fill(st, gb, 0.0f, m_n));
+	HIP_CHECK(nemo_launch_fill(st, gd, 0.0f, m_n));
+	const unsigned Kseg = std::min(K, T);
+	const size_t segPlane = size_t(Kseg) * m_n;
+
+	nemo_float_step_args f = {};
+	f.n = unsigned(m_n);
+	f.rcm_ptr = m_rcmPtr.ptr;
+	f.rcm_sd = m_rcmSd.ptr;
+	f.wr = m_wr.ptr;
+	f.pa = a;
+	f.pb = b;
+	f.pc = c;
+	f.pd = d;
+	f.x = x;
+	f.S = const_cast<float*>(S); /* read only: store == 0 */
+	f.V = nullptr;
+	f.v = m_v.ptr;
+	f.u = m_u.ptr;
+	f.store = 0;
+	f.record = 1;
+	f.seg_v = m_seg.ptr;
+	f.seg_u = m_seg.ptr + segPlane;
+	f.seg_I = m_seg.ptr + 2 * segPlane;
+	f.ckpt = nullptr;
+	f.K = K;
+
+	nemo_float_back_args g = {};
+	g.n = unsigned(m_n);
+	g.T = T;
+	g.ring = m_D + 1;
+	g.fcm_ptr = m_fcmPtr.ptr;
+	g.fcm_td = m_fcmTd.ptr;
+	g.wf = w;
+	g.pa = a;
+	g.pb = b;
+	g.pc = c;
+	g.pd = d;
+	g.S = S;
+	g.gS = gS;
+	g.gV = gV;
+	g.seg_v = f.seg_v;
+	g.seg_u = f.seg_u;
+	g.seg_I = f.seg_I;
+	g.LV = m_LV.ptr;
+	g.LU = m_LU.ptr;
+	g.LI = m_LI.ptr;
+	g.gw = gw;
+	g.ga = ga;
+	g.gb = gb;
+	g.gd = gd;
+	g.gx = gx;
+
+	const unsigned segments = (T + K - 1) / K;
+	for(unsigned s = segments; s-- > 0;) {
+		const unsigned t0 = s * K;
+		const unsigned t1 = std::min(T, t0 + K);
+		HIP_CHECK(hipMemcpyAsync(m_v.ptr, ckpt + size_t(s) * 2 * m_n, m_n * sizeof(float),
+					hipMemcpyDeviceToDevice, st));
+		HIP_CHECK(hipMemcpyAsync(m_u.ptr, ckpt + size_t(s) * 2 * m_n + m_n, m_n * sizeof(float),
+					hipMemcpyDeviceToDevice, st));
+		f.t0 = t0;
+		for(unsigned t = t0; t < t1; ++t) {
+			f.t = t;
+			HIP_CHECK(nemo_launch_float_forward_step(st, &f));
+		}
+		g.t0 = t0;
+		for(unsigned t = t1; t-- > t0;) {
+			g.t = t;
+			HIP_CHECK(nemo_launch_float_backward_step(st, &g));
+		}
+	}
+	if(gv0) HIP_CHECK(hipMemcpyAsync(gv0, m_LV.ptr, m_n * sizeof(float), hipMemcpyDeviceToDevice, st));
+	if(gu0) HIP_CHECK(hipMemcpyAsync(gu0, m_LU.ptr, m_n * sizeof(float), hipMemcpyDeviceToDevice, st));
+}
+
+} // namespace nemo_hip

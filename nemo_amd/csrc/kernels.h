@@ -21,25 +21,29 @@ extern "C" {
 /* Threads per block of the backend's kernels. */
 #define NEMO_BLOCK 256
 
-/* One launch per step, before the neuron updates, doing two independent jobs:
+/* Per step, before the neuron updates, two independent jobs:
  *
- * Spike delivery of step t (blocks [0, deliver_blocks)), binned by target
- * tile (the update kernels gather the bins).  For every delay d in 1..D and
- * every neuron n that fired at step t-d, the synapse row (n, d) is delivered:
- * each event {target, w} is counting-sorted in LDS by target tile and written
- * to the block's bin region; bin_table[tile][block] = count << 16 | offset.
- * Events beyond the block's bin capacity are added with global atomics into
- * acc (E if w >= 0, else I).  Block b delivers the (delay, tile) pairs
- * [b*Q/B, (b+1)*Q/B), Q = D*tiles.  Semantics of
+ * Spike delivery of step t, binned by target tile (the update kernels gather
+ * the bins).  For every delay d in 1..D and every neuron n that fired at step
+ * t-d, the synapse row (n, d) is delivered: each event {target, w} is
+ * counting-sorted in LDS by target tile and written to the block's bin
+ * region; bin_table[tile][block] = count << 16 | offset.  Events beyond the
+ * block's bin capacity are added with global atomics into acc (E if w >= 0,
+ * else I).  Delivery block b covers the (source tile, delay) pairs
+ * q in [pair_bounds[b], pair_bounds[b+1]), q = tile*D + d-1, and every target
+ * of those rows lies in its tile window [windows[b].x, windows[b].y); it
+ * writes the table entries of exactly the tiles of its window.  Semantics of
  * cpu::Simulation::deliverSpikes (cpu/Simulation.cpp:259-308).
  *
- * STDP accumulation of step t-1 (blocks [deliver_blocks, gridDim)), for the
- * neurons that fired at step t-1-P, P = |postfire| (ConnectivityMatrix::
- * accumulateStdp, ConnectivityMatrix.cpp:224-258, StdpProcess::weightChange,
- * StdpProcess.cpp:68-124).  It reads the firing history as left by the update
- * of step t-1, which is why it may run beside delivery of step t.  Skipped
- * when t == 0 or when step t-1's accumulation was already run on its own
- * (clock->stdp_flushed == t-1). */
+ * STDP accumulation of step t-1, for the neurons that fired at step t-1-P,
+ * P = |postfire| (ConnectivityMatrix::accumulateStdp, ConnectivityMatrix.cpp:
+ * 224-258, StdpProcess::weightChange, StdpProcess.cpp:68-124).  It reads the
+ * firing history as left by the update of step t-1, which is why it may run
+ * beside delivery of step t.  Skipped when t == 0 or when step t-1's
+ * accumulation was already run on its own (clock->stdp_flushed == t-1).
+ *
+ * With fuse_stdp the STDP blocks follow the delivery blocks in one launch;
+ * otherwise STDP is a launch of its own, before delivery. */
 typedef struct nemo_propagate_args {
 	const nemo_hip_clock* clock;
 	uint64_t step_offset;            /* t = clock->base_step + step_offset */
@@ -47,6 +51,9 @@ typedef struct nemo_propagate_args {
 	size_t fired_words;
 	/* delivery */
 	unsigned deliver_blocks;
+	const uint32_t* pair_bounds;     /* [deliver_blocks+1] first pair of each block */
+	const uint2* windows;            /* [deliver_blocks] target tiles [x, y) of each block */
+	unsigned max_window;             /* largest window (tiles) */
 	const uint32_t* rowptr;          /* (neurons*D)+1 offsets, row (n,d) = n*D + d-1 */
 	const int2* fcm;                 /* {target device index, Q11.20 weight} */
 	unsigned D;                      /* rows per source (network max delay) */
@@ -70,9 +77,11 @@ typedef struct nemo_propagate_args {
 	unsigned long long* block_examined; /* [STDP blocks] plastic synapses examined, accumulated */
 } nemo_propagate_args;
 
-/* dynamic LDS bytes the propagate kernel needs */
-size_t nemo_propagate_lds_bytes(unsigned tiles, unsigned bin_capacity);
-hipError_t nemo_launch_propagate(hipStream_t, const nemo_propagate_args*, unsigned stdp_blocks);
+/* dynamic LDS bytes a delivery block needs */
+size_t nemo_propagate_lds_bytes(unsigned max_window, unsigned bin_capacity);
+/* Delivery (a->deliver_blocks blocks, none if 0) and STDP accumulation
+ * (stdp_blocks blocks, none if 0), fused into one launch or not. */
+hipError_t nemo_launch_propagate(hipStream_t, const nemo_propagate_args*, unsigned stdp_blocks, int fuse_stdp);
 
 /* applyStdp (ConnectivityMatrix::applyStdp, ConnectivityMatrix.cpp:272-325 and
  * StdpProcess::updatedWeight, StdpProcess.cpp:128-138). */

@@ -16,6 +16,7 @@
  *   hist       uint64[N]   firing history (bit k: fired k steps ago), STDP
  *   bins       int2[B][CAP] each delivery block's events, sorted by tile
  *   bin_table  uint32[T][B] count << 16 | offset of tile t in block b's bins
+ *                          (entries of the blocks whose window holds t)
  *   acc        int64[2][N] overflow accumulators (events beyond CAP)
  *   rcm_*      per target, its plastic incoming synapses (STDP only)
  */
@@ -40,10 +41,11 @@ __device__ inline unsigned kth_set_bit(uint64_t w, unsigned k)
 
 /* ------------------------------------------------------------ delivery */
 
-/* A delivery block: list the fired rows of its (delay, tile) pairs, read
- * their row pointers in parallel, then spread all their events evenly over
- * the block (four loads in flight per thread), staging them in LDS; finally
- * counting-sort the staged events by target tile into the block's bins. */
+/* A delivery block: list the fired rows of its (source tile, delay) pairs,
+ * read their row pointers in parallel, then spread all their events evenly
+ * over the block (four loads in flight per thread), staging them in LDS;
+ * finally counting-sort the staged events by target tile (relative to the
+ * block's window) into the block's bins. */
 __device__ void deliver_block(const nemo_propagate_args& a, uint64_t t, int2* stage, uint32_t* cnt, uint32_t* cur)
 {
 	__shared__ uint64_t s_word[NEMO_BLOCK];
@@ -52,15 +54,16 @@ __device__ void deliver_block(const nemo_propagate_args& a, uint64_t t, int2* st
 	__shared__ uint32_t s_rpre[ROWS + 1];
 
 	const unsigned CAP = a.bin_capacity;
-	const unsigned T = a.tiles;
 	const unsigned D = a.D;
 	const unsigned tid = threadIdx.x;
 	const unsigned b = blockIdx.x;
-	for(unsigned p = tid; p < T; p += NEMO_BLOCK) cnt[p] = 0;
+	const uint2 win = a.windows[b];
+	const unsigned tlo = win.x;
+	const unsigned W = win.y - win.x; /* tiles in the window */
+	for(unsigned p = tid; p < W; p += NEMO_BLOCK) cnt[p] = 0;
 
-	const uint64_t Q = uint64_t(D) * T;
-	const uint64_t q0 = Q * b / a.deliver_blocks;
-	const uint64_t q1 = Q * (b + 1) / a.deliver_blocks;
+	const uint64_t q0 = a.pair_bounds[b];
+	const uint64_t q1 = a.pair_bounds[b + 1];
 	const uint64_t nwords = (q1 - q0) * 4;
 	uint32_t fill = 0;
 	unsigned long long events = 0;
@@ -71,8 +74,8 @@ __device__ void deliver_block(const nemo_propagate_args& a, uint64_t t, int2* st
 		const uint64_t w = wb + tid;
 		if(w < nwords) {
 			const uint64_t q = q0 + w / 4;
-			const unsigned d = unsigned(q / T) + 1;
-			const unsigned p = unsigned(q % T);
+			const unsigned p = unsigned(q / D);
+			const unsigned d = unsigned(q % D) + 1;
 			if(t >= d) word = a.fired_bits[((t - d) & RING_MASK) * a.fired_words + size_t(p) * 4 + (w & 3)];
 		}
 		__syncthreads(); /* previous batch done with s_word */
@@ -101,8 +104,8 @@ __device__ void deliver_block(const nemo_propagate_args& a, uint64_t t, int2* st
 					}
 					const uint64_t wg = wb + lo;
 					const uint64_t q = q0 + wg / 4;
-					const unsigned d = unsigned(q / T) + 1;
-					const unsigned p = unsigned(q % T);
+					const unsigned p = unsigned(q / D);
+					const unsigned d = unsigned(q % D) + 1;
 					const unsigned bit = kth_set_bit(s_word[lo], r - s_wpre[lo]);
 					const uint32_t n = p * NEMO_TILE + unsigned(wg & 3) * 64 + bit;
 					const size_t row = size_t(n) * D + (d - 1);
@@ -144,10 +147,14 @@ __device__ void deliver_block(const nemo_propagate_args& a, uint64_t t, int2* st
 					const uint32_t e = e0 + u * NEMO_BLOCK;
 					if(e < M) {
 						const uint32_t pos = fill + e;
-						if(pos < CAP) {
+						const uint32_t rel = uint32_t(s[u].x) / NEMO_TILE - tlo;
+						if(pos < CAP && rel < W) {
 							stage[pos] = s[u];
-							atomicAdd(&cnt[uint32_t(s[u].x) / NEMO_TILE], 1u);
+							atomicAdd(&cnt[rel], 1u);
 						} else {
+							/* beyond capacity (or, defensively, outside the
+							 * window): straight into the accumulators */
+							if(pos < CAP) stage[pos] = make_int2(-1, 0);
 							int64_t* dst = a.acc + (s[u].y >= 0 ? size_t(0) : a.acc_pitch) + uint32_t(s[u].x);
 							atomicAdd(reinterpret_cast<unsigned long long*>(dst),
 									(unsigned long long)(int64_t)s[u].y);
@@ -164,14 +171,14 @@ __device__ void deliver_block(const nemo_propagate_args& a, uint64_t t, int2* st
 	/* counting sort of the staged events by target tile */
 	const uint32_t staged = min(fill, CAP);
 	uint32_t carry = 0;
-	for(uint32_t base = 0; base < T; base += NEMO_BLOCK) {
+	for(uint32_t base = 0; base < W; base += NEMO_BLOCK) {
 		const uint32_t p = base + tid;
-		const uint32_t c = p < T ? cnt[p] : 0u;
+		const uint32_t c = p < W ? cnt[p] : 0u;
 		uint32_t total;
 		const uint32_t pre = nemo_dev::block_exclusive_scan(c, &total);
-		if(p < T) {
+		if(p < W) {
 			cur[p] = carry + pre;
-			a.bin_table[size_t(p) * a.deliver_blocks + b] = (c << 16) | (carry + pre);
+			a.bin_table[size_t(tlo + p) * a.deliver_blocks + b] = (c << 16) | (carry + pre);
 		}
 		carry += total;
 	}
@@ -179,7 +186,8 @@ __device__ void deliver_block(const nemo_propagate_args& a, uint64_t t, int2* st
 	int2* out = a.bins + size_t(b) * CAP;
 	for(uint32_t i = tid; i < staged; i += NEMO_BLOCK) {
 		const int2 e = stage[i];
-		const uint32_t pos = atomicAdd(&cur[uint32_t(e.x) / NEMO_TILE], 1u);
+		if(e.x < 0) continue;
+		const uint32_t pos = atomicAdd(&cur[uint32_t(e.x) / NEMO_TILE - tlo], 1u);
 		out[pos] = make_int2(int(uint32_t(e.x) % NEMO_TILE), e.y);
 	}
 	if(tid == 0 && events) a.block_events[b] += events;
@@ -294,6 +302,16 @@ __device__ void stdp_block(const nemo_propagate_args& a, uint64_t s, unsigned sb
 	if(tid == 0) a.block_examined[sb] += total;
 }
 
+/* STDP accumulation of step t-1 by STDP block sb, unless it is not due */
+__device__ inline void stdp_step(const nemo_propagate_args& a, uint64_t t, unsigned sb)
+{
+	if(t == 0) return;
+	const uint64_t s = t - 1;
+	if(s == a.clock->stdp_flushed || s < a.postPreWindow) return;
+	stdp_block(a, s, sb);
+}
+
+/* delivery blocks, then (fused) STDP blocks */
 __global__ void __launch_bounds__(NEMO_BLOCK)
 propagate_kernel(nemo_propagate_args a)
 {
@@ -302,13 +320,17 @@ propagate_kernel(nemo_propagate_args a)
 	if(blockIdx.x < a.deliver_blocks) {
 		int2* stage = lds_dyn;
 		uint32_t* cnt = reinterpret_cast<uint32_t*>(lds_dyn + a.bin_capacity);
-		deliver_block(a, t, stage, cnt, cnt + a.tiles);
+		deliver_block(a, t, stage, cnt, cnt + a.max_window);
 	} else {
-		if(t == 0) return;
-		const uint64_t s = t - 1;
-		if(s == a.clock->stdp_flushed || s < a.postPreWindow) return;
-		stdp_block(a, s, blockIdx.x - a.deliver_blocks);
+		stdp_step(a, t, blockIdx.x - a.deliver_blocks);
 	}
+}
+
+/* STDP blocks only */
+__global__ void __launch_bounds__(NEMO_BLOCK)
+stdp_kernel(nemo_propagate_args a)
+{
+	stdp_step(a, a.clock->base_step + a.step_offset, blockIdx.x);
 }
 
 __global__ void __launch_bounds__(NEMO_BLOCK)
@@ -373,27 +395,40 @@ __global__ void eval_math_kernel(int fn, const float* in, float* out, size_t n)
 
 extern "C" {
 
-size_t nemo_propagate_lds_bytes(unsigned tiles, unsigned bin_capacity)
+size_t nemo_propagate_lds_bytes(unsigned max_window, unsigned bin_capacity)
 {
-	return size_t(bin_capacity) * sizeof(int2) + 2 * size_t(tiles) * sizeof(uint32_t);
+	return size_t(bin_capacity) * sizeof(int2) + 2 * size_t(max_window) * sizeof(uint32_t);
 }
 
-hipError_t nemo_launch_propagate(hipStream_t st, const nemo_propagate_args* a, unsigned stdp_blocks)
+hipError_t nemo_launch_propagate(hipStream_t st, const nemo_propagate_args* a, unsigned stdp_blocks, int fuse_stdp)
 {
 	if(a->deliver_blocks > NEMO_MAX_BIN_BLOCKS) return hipErrorInvalidValue;
 	if(a->deliver_blocks && (a->D == 0 || a->D > NEMO_MAX_DELAY || a->bin_capacity == 0
-				|| a->bin_capacity > 32768)) {
+				|| a->bin_capacity > 32768 || a->max_window == 0 || a->max_window > a->tiles
+				|| !a->pair_bounds || !a->windows)) {
 		return hipErrorInvalidValue;
 	}
 	if(stdp_blocks && (a->postPreWindow > 63 || a->stdp_words == 0 || a->stdp_words > 4
 				|| size_t(stdp_blocks) * a->stdp_words < a->fired_words)) {
 		return hipErrorInvalidValue;
 	}
-	const unsigned blocks = a->deliver_blocks + stdp_blocks;
-	if(blocks == 0) return hipSuccess;
-	const size_t lds = a->deliver_blocks ? nemo_propagate_lds_bytes(a->tiles, a->bin_capacity) : 0;
-	hipLaunchKernelGGL(propagate_kernel, dim3(blocks), dim3(NEMO_BLOCK), lds, st, *a);
-	return hipGetLastError();
+	const size_t lds = a->deliver_blocks ? nemo_propagate_lds_bytes(a->max_window, a->bin_capacity) : 0;
+	if(fuse_stdp) {
+		const unsigned blocks = a->deliver_blocks + stdp_blocks;
+		if(blocks == 0) return hipSuccess;
+		hipLaunchKernelGGL(propagate_kernel, dim3(blocks), dim3(NEMO_BLOCK), lds, st, *a);
+		return hipGetLastError();
+	}
+	if(stdp_blocks) {
+		hipLaunchKernelGGL(stdp_kernel, dim3(stdp_blocks), dim3(NEMO_BLOCK), 0, st, *a);
+		const hipError_t e = hipGetLastError();
+		if(e != hipSuccess) return e;
+	}
+	if(a->deliver_blocks) {
+		hipLaunchKernelGGL(propagate_kernel, dim3(a->deliver_blocks), dim3(NEMO_BLOCK), lds, st, *a);
+		return hipGetLastError();
+	}
+	return hipSuccess;
 }
 
 hipError_t nemo_launch_apply_stdp(hipStream_t st, const nemo_apply_args* a)

@@ -15,6 +15,8 @@ struct nemo_network_class { nemo_hip::Network impl; };
 struct nemo_configuration_class { nemo_hip::Configuration impl; };
 struct nemo_simulation_class {
 	explicit nemo_simulation_class(const nemo_hip::Network& n, const nemo_hip::Configuration& c) : impl(n, c) {}
+	nemo_simulation_class(const std::vector<const nemo_hip::Network*>& n, const nemo_hip::Configuration& c)
+		: impl(n, c) {}
 	nemo_hip::Simulation impl;
 	std::vector<unsigned> runCounts, runFired;
 };
@@ -441,6 +443,83 @@ nemo_status_t nemo_hip_run(nemo_simulation_t s, unsigned nsteps, unsigned stdp_p
 	});
 }
 
+nemo_status_t nemo_hip_new_batch(const nemo_network_t nets[], size_t count, nemo_configuration_t c,
+		nemo_simulation_t* sim)
+{
+	return guarded([&] {
+		check(c, "configuration");
+		if(!sim || (!nets && count)) throw nemo_hip::Error(NEMO_INVALID_INPUT, "null argument");
+		std::vector<const nemo_hip::Network*> v(count);
+		for(size_t k = 0; k < count; ++k) {
+			check(nets[k], "network");
+			v[k] = &nets[k]->impl;
+		}
+		*sim = new nemo_simulation_class(v, c->impl);
+	});
+}
+
+nemo_status_t nemo_hip_batch_size(nemo_simulation_t s, unsigned* count)
+{
+	return guarded([&] { check(s, "simulation"); *count = s->impl.networkCount(); });
+}
+
+nemo_status_t nemo_hip_batch_get_neuron_state(nemo_simulation_t s, unsigned net, unsigned neuron, unsigned var,
+		float* val)
+{
+	return guarded([&] { check(s, "simulation"); *val = s->impl.getNeuronState(neuron, var, net); });
+}
+
+nemo_status_t nemo_hip_batch_get_neuron_parameter(nemo_simulation_t s, unsigned net, unsigned neuron,
+		unsigned param, float* val)
+{
+	return guarded([&] { check(s, "simulation"); *val = s->impl.getNeuronParameter(neuron, param, net); });
+}
+
+nemo_status_t nemo_hip_batch_get_membrane_potential(nemo_simulation_t s, unsigned net, unsigned neuron, float* val)
+{
+	return guarded([&] { check(s, "simulation"); *val = s->impl.getMembranePotential(neuron, net); });
+}
+
+nemo_status_t nemo_hip_prepare(nemo_simulation_t s, unsigned nsteps, unsigned stdp_period)
+{
+	return guarded([&] { check(s, "simulation"); s->impl.prepare(nsteps, stdp_period); });
+}
+
+static void shapeOut(const nemo_hip::LaunchShape& l, unsigned shape[4])
+{
+	shape[0] = l.deliverBlocks;
+	shape[1] = l.binCapacity;
+	shape[2] = l.fuseStdp;
+	shape[3] = l.stdpWords;
+}
+
+nemo_status_t nemo_hip_get_launch_shape(nemo_simulation_t s, unsigned shape[4])
+{
+	return guarded([&] { check(s, "simulation"); shapeOut(s->impl.launchShape(), shape); });
+}
+
+nemo_status_t nemo_hip_set_launch_shape(nemo_simulation_t s, const unsigned shape[4])
+{
+	return guarded([&] {
+		check(s, "simulation");
+		nemo_hip::LaunchShape l;
+		l.deliverBlocks = shape[0];
+		l.binCapacity = shape[1];
+		l.fuseStdp = shape[2];
+		l.stdpWords = shape[3];
+		s->impl.setLaunchShape(l);
+	});
+}
+
+nemo_status_t nemo_hip_autotune(nemo_simulation_t s, unsigned steps_per_trial, unsigned shape[4])
+{
+	return guarded([&] {
+		check(s, "simulation");
+		const nemo_hip::LaunchShape l = s->impl.autotune(steps_per_trial);
+		if(shape) shapeOut(l, shape);
+	});
+}
+
 nemo_status_t nemo_hip_set_debug(nemo_simulation_t s, int enable)
 {
 	return guarded([&] { check(s, "simulation"); s->impl.setDebug(enable != 0); });
@@ -558,6 +637,74 @@ nemo_status_t hip_test_device(unsigned device)
 		void* p = nullptr;
 		HIP_CHECK(hipMalloc(&p, 256));
 		HIP_CHECK(hipFree(p));
+	});
+}
+
+} // extern "C"
+
+/* ---------------------------------------------------- float-weight mode */
+
+struct nemo_hip_float_network_class {
+	nemo_hip_float_network_class(const nemo_hip::Network& n, int dev) : impl(n, dev) {}
+	nemo_hip::FloatNetwork impl;
+};
+
+extern "C" {
+
+nemo_status_t nemo_hip_new_float_network(nemo_network_t n, int device, nemo_hip_float_network_t* out)
+{
+	return guarded([&] {
+		check(n, "network");
+		if(!out) throw nemo_hip::Error(NEMO_INVALID_INPUT, "null argument");
+		*out = new nemo_hip_float_network_class(n->impl, device);
+	});
+}
+
+void nemo_hip_delete_float_network(nemo_hip_float_network_t f)
+{
+	delete f;
+}
+
+nemo_status_t nemo_hip_float_network_sizes(nemo_hip_float_network_t f, size_t* neurons, size_t* synapses,
+		unsigned* max_delay)
+{
+	return guarded([&] {
+		check(f, "float network");
+		if(neurons) *neurons = f->impl.neurons();
+		if(synapses) *synapses = f->impl.synapses();
+		if(max_delay) *max_delay = f->impl.maxDelay();
+	});
+}
+
+nemo_status_t nemo_hip_float_forward(nemo_hip_float_network_t f, void* stream, unsigned steps,
+		unsigned checkpoint_every, const float* w, const float* a, const float* b, const float* c,
+		const float* d, const float* v0, const float* u0, const float* x,
+		float* spikes, float* v_out, float* checkpoints)
+{
+	return guarded([&] {
+		check(f, "float network");
+		if(!w || !a || !b || !c || !d || !v0 || !u0 || !x || !spikes || !v_out || !checkpoints) {
+			throw nemo_hip::Error(NEMO_INVALID_INPUT, "null argument");
+		}
+		f->impl.forward(static_cast<hipStream_t>(stream), steps, checkpoint_every, w, a, b, c, d, v0, u0, x,
+				spikes, v_out, checkpoints);
+	});
+}
+
+nemo_status_t nemo_hip_float_backward(nemo_hip_float_network_t f, void* stream, unsigned steps,
+		unsigned checkpoint_every, const float* w, const float* a, const float* b, const float* c,
+		const float* d, const float* x, const float* spikes, const float* checkpoints,
+		const float* grad_spikes, const float* grad_v, float* grad_w, float* grad_a, float* grad_b,
+		float* grad_d, float* grad_x, float* grad_v0, float* grad_u0)
+{
+	return guarded([&] {
+		check(f, "float network");
+		if(!w || !a || !b || !c || !d || !x || !spikes || !checkpoints || !grad_w || !grad_a || !grad_b
+				|| !grad_d) {
+			throw nemo_hip::Error(NEMO_INVALID_INPUT, "null argument");
+		}
+		f->impl.backward(static_cast<hipStream_t>(stream), steps, checkpoint_every, w, a, b, c, d, x, spikes,
+				checkpoints, grad_spikes, grad_v, grad_w, grad_a, grad_b, grad_d, grad_x, grad_v0, grad_u0);
 	});
 }
 

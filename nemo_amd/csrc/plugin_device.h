@@ -67,13 +67,16 @@ __device__ inline void gather_input(const nemo_hip_step_buffers& b, const nemo_h
 	const unsigned tid = threadIdx.x;
 	in.E[tid] = 0;
 	in.I[tid] = 0;
-	const unsigned B = b.bin_blocks;
-	if(B == 0) {
+	if(b.bin_blocks == 0) {
 		__syncthreads();
 		return;
 	}
+	/* the delivery blocks [blo, blo + B) whose windows hold this tile */
 	const unsigned tile = g.start / NEMO_TILE + blockIdx.x;
-	const uint32_t* row = b.bin_table + size_t(tile) * B;
+	const uint32_t range = b.bin_range[tile];
+	const unsigned blo = range & 0xffffu;
+	const unsigned B = (range >> 16) - blo;
+	const uint32_t* row = b.bin_table + size_t(tile) * b.bin_blocks + blo;
 	constexpr unsigned PER = NEMO_MAX_BIN_BLOCKS / NEMO_TILE;
 	uint32_t cnt[PER];
 	uint32_t mine = 0;
@@ -109,7 +112,7 @@ __device__ inline void gather_input(const nemo_hip_step_buffers& b, const nemo_h
 					const unsigned mid = (lo + hi) >> 1;
 					if(s_pre[mid] <= j) lo = mid; else hi = mid;
 				}
-				ev[u] = b.bins[size_t(lo) * cap + s_off[lo] + (j - s_pre[lo])];
+				ev[u] = b.bins[size_t(blo + lo) * cap + s_off[lo] + (j - s_pre[lo])];
 			}
 		}
 #pragma unroll
@@ -185,8 +188,27 @@ __device__ inline float nrand(uint4& s)
 	return nemo_math::sinf_exact(a) * r;
 }
 
+/* Write the tile's words of the step's firing bitmap (one ballot per wave;
+ * device indices of a wave are 64-aligned). */
+__device__ inline void store_firing(const nemo_hip_step_buffers& b, unsigned ng, bool valid, bool fired)
+{
+	const uint64_t mask = __ballot(valid && fired);
+	if((threadIdx.x & 63) == 0) {
+		const uint64_t slot = current_step(b) & (NEMO_STEP_RING - 1);
+		b.fired_bits[slot * b.fired_words + (ng >> 6)] = mask;
+	}
+}
+
+/* Shift the firing history (whose old value h the caller loaded) and write
+ * the step's firing bitmap. */
+__device__ inline void record_firing(const nemo_hip_step_buffers& b, unsigned ng, bool valid, bool fired, uint64_t h)
+{
+	if(valid) b.hist[ng] = (h << 1) | uint64_t(fired ? 1 : 0);
+	store_firing(b, ng, valid, fired);
+}
+
 /* Shift the firing history and write the tile's words of the step's firing
- * bitmap (one ballot per wave; device indices of a wave are 64-aligned). */
+ * bitmap. */
 __device__ inline void record_firing(const nemo_hip_step_buffers& b, unsigned ng, bool valid, bool fired)
 {
 	if(valid) {

@@ -25,30 +25,37 @@ __global__ void __launch_bounds__(NEMO_UPDATE_BLOCK)
 izhikevich_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 {
 	__shared__ nemo_dev::TileInput input;
-	nemo_dev::gather_input(b, g, input);
 	const unsigned nl = blockIdx.x * NEMO_UPDATE_BLOCK + threadIdx.x;
 	const bool valid = nl < g.count;
 	const unsigned ng = g.start + nl;
+	const size_t P = g.pitch;
+
+	/* the neuron's parameters, state, RNG and history are loaded before the
+	 * input gather, so that their latency overlaps it */
+	float sigma = 0.0f, a = 0.0f, bb = 0.0f, u = 0.0f, v = 0.0f;
+	uint4 s = make_uint4(0, 0, 0, 0);
+	uint64_t h = 0;
+	if(valid) {
+		sigma = g.param[PARAM_SIGMA * P + nl];
+		a = g.param[PARAM_A * P + nl];
+		bb = g.param[PARAM_B * P + nl];
+		u = g.state[STATE_U * P + nl];
+		v = g.state[STATE_V * P + nl];
+		if(g.rng) s = reinterpret_cast<const uint4*>(g.rng)[nl];
+		h = b.hist[ng];
+	}
+	nemo_dev::gather_input(b, g, input);
 	bool fired = false;
 
 	if(valid) {
-		const size_t P = g.pitch;
 		float E, In;
 		nemo_dev::take_current(b, input, ng, E, In);
 		float I = E + In + nemo_dev::take_external(b, ng);
 
-		const float sigma = g.param[PARAM_SIGMA * P + nl];
 		if(sigma != 0.0f) {
-			uint4* rng = reinterpret_cast<uint4*>(g.rng);
-			uint4 s = rng[nl];
 			I += sigma * nemo_dev::nrand(s);
-			rng[nl] = s;
+			reinterpret_cast<uint4*>(g.rng)[nl] = s;
 		}
-
-		const float a = g.param[PARAM_A * P + nl];
-		const float bb = g.param[PARAM_B * P + nl];
-		float u = g.state[STATE_U * P + nl];
-		float v = g.state[STATE_V * P + nl];
 
 #pragma unroll
 		for(int t = 0; t < 4; ++t) {
@@ -68,7 +75,7 @@ izhikevich_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 		g.state[STATE_U * P + nl] = u;
 		g.state[STATE_V * P + nl] = v;
 	}
-	nemo_dev::record_firing(b, ng, valid, fired);
+	nemo_dev::record_firing(b, ng, valid, fired, h);
 }
 
 } // namespace

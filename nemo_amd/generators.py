@@ -92,6 +92,87 @@ def random_izhikevich(n, m, dmax=20, seed=1, stdp=False, sigma=(5.0, 2.0), first
     return spec
 
 
+def batch_izhikevich(count=64, nmin=1000, nmax=50000, smin=100, smax=1000, dmax=20, seed=4, seed_base=1000,
+                     stdp_every=2):
+    """The batched workload: count independent 80/20 Izhikevich networks
+    (random_izhikevich), network i with N_i log-uniform in [nmin, nmax],
+    S_i uniform in {smin..smax}, its own seed seed_base + i, and plastic
+    excitatory synapses when i % stdp_every == 0.  Returns a list of
+    (spec, stdp) pairs."""
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(np.log(nmin), np.log(nmax), count)).astype(np.int64)
+    ss = rng.integers(smin, smax + 1, count)
+    out = []
+    for i in range(count):
+        stdp = stdp_every > 0 and i % stdp_every == 0
+        out.append((random_izhikevich(int(ns[i]), int(ss[i]), dmax=dmax, seed=seed_base + i, stdp=stdp), stdp))
+    return out
+
+
+def torus_mixed(n_lif=40000, n_qif=8000, n_hh=2000, n_poisson=1000, m=500, sigma=0.1, seed=3,
+                cm=1.0, i_offset=0.7, qif_a=0.1, rate=0.01, grid=50):
+    """Mixed-model network on the unit torus: IF_curr_exp (v_rest -65 mV,
+    v_reset -65, cm 1 nF, tau_m 20 ms, refractory 2 ms, tau_syn 5 ms, threshold
+    -50 mV, i_offset 0.7 nA: resting just below threshold), QIF (a = 0.1), HH
+    (resting gating state) and Poisson sources (rate per ms; 0.01 = 10 Hz),
+    added in that order, neuron i at a position U[0,1)^2.  Every neuron has m
+    outgoing synapses to non-source neurons with probability proportional to
+    exp(-d^2 / (2 sigma^2)) of the torus distance d, and delay ceil(20 d) + 1
+    ms.  Targets are drawn by adding a Gaussian offset (sigma) to the source
+    position and taking a uniformly chosen neuron of the grid cell (side
+    1/grid) holding that point: the cell discretises the kernel at 1/grid, far
+    below sigma.  Weights U[0,1) from excitatory and -U[0,1) from inhibitory
+    sources; 20% of the non-source neurons, chosen at random, are
+    inhibitory.  No synapse is plastic."""
+    rng = np.random.default_rng(seed)
+    n = n_lif + n_qif + n_hh
+    total = n + n_poisson
+    pos = rng.random((total, 2))
+    blocks = []
+    base = 0
+
+    def block(model, count, args):
+        nonlocal base
+        idx = np.arange(base, base + count, dtype=np.uint32)
+        base += count
+        blocks.append(NeuronBlock(model, idx, [np.broadcast_to(np.float32(a), (count,)).astype(np.float32)
+                                               if np.isscalar(a) else np.asarray(a, np.float32) for a in args]))
+
+    block("IF_curr_exp", n_lif, [-65.0, -65.0, cm, 20.0, 2.0, 5.0, 5.0, -50.0, i_offset,
+                                 -65.0, 0.0, 0.0, 100.0])
+    block("QIF", n_qif, [qif_a, 0.0])
+    block("HH", n_hh, [-65.0, 0.3177, 0.0529, 0.5961, 0.0])
+    block("PoissonSource", n_poisson, [rate])
+    spec = NetworkSpec(blocks=blocks)
+
+    # grid cells of the candidate targets (the non-source neurons)
+    cell = (np.floor(pos[:n, 0] * grid).astype(np.int64) % grid) * grid + \
+        (np.floor(pos[:n, 1] * grid).astype(np.int64) % grid)
+    order = np.argsort(cell, kind="stable")
+    starts = np.searchsorted(cell[order], np.arange(grid * grid))
+    ends = np.searchsorted(cell[order], np.arange(grid * grid), side="right")
+    if np.any(ends == starts):
+        raise ValueError("empty grid cell: use a coarser grid")
+
+    src = np.repeat(np.arange(total, dtype=np.uint32), m)
+    pt = (pos[src] + rng.normal(0.0, sigma, size=(src.size, 2))) % 1.0
+    c = (np.floor(pt[:, 0] * grid).astype(np.int64) % grid) * grid + (np.floor(pt[:, 1] * grid).astype(np.int64) % grid)
+    pick = starts[c] + (rng.random(src.size) * (ends[c] - starts[c])).astype(np.int64)
+    tgt = order[pick].astype(np.uint32)
+    dxy = np.abs(pos[tgt] - pos[src])
+    dxy = np.minimum(dxy, 1.0 - dxy)
+    dist = np.sqrt((dxy * dxy).sum(axis=1))
+    inh = np.zeros(total, bool)
+    inh[:n] = rng.random(n) < 0.2
+    u = rng.random(src.size, dtype=np.float32)
+    spec.source = src
+    spec.target = tgt
+    spec.delay = (np.ceil(20.0 * dist).astype(np.uint32) + 1).astype(np.uint32)
+    spec.weight = np.where(inh[src], -u, u).astype(np.float32)
+    spec.plastic = np.zeros(src.size, np.uint8)
+    return spec
+
+
 def ring(ncount, n0=0, plastic=False, nstep=1, delay=1, sigma=0.0):
     """NeMo's test ring (src/test/utils.cpp:117-140): neuron i excites neuron
     i+1 with weight 1000 at the given delay; RS-like Izhikevich neurons."""

@@ -284,3 +284,135 @@ def test_events_match_raster_full_size():
     assert events == expected
     rate_hz = counts.sum() / n / (steps / 1000.0)
     assert 0.5 < rate_hz < 200
+
+
+# ------------------------------------------------------------ batches
+def _mixed_spec(seed):
+    """Poisson sources (added first) driving Izhikevich neurons: a network
+    whose types come in a different order from the Izhikevich-only ones."""
+    rng = np.random.default_rng(seed)
+    npo, niz = 300, 700
+    po = gen.NetworkSpec(blocks=[gen.NeuronBlock("PoissonSource", np.arange(npo, dtype=np.uint32),
+                                                 [np.full(npo, 0.02, np.float32)])])
+    iz = gen.random_izhikevich(niz, 0, seed=seed, first_index=npo)
+    spec = gen.merge(po, iz)
+    n = npo + niz
+    src = np.repeat(np.arange(n, dtype=np.uint32), 30)
+    spec.source = src
+    spec.target = rng.integers(npo, n, size=src.size, dtype=np.uint32)
+    spec.delay = rng.integers(1, 9, size=src.size, dtype=np.uint32)
+    spec.weight = (rng.random(src.size, dtype=np.float32) * np.float32(3.0)).astype(np.float32)
+    spec.plastic = np.zeros(src.size, np.uint8)
+    return spec
+
+
+def test_batch_matches_solo_runs():
+    """Networks of different sizes, types and STDP settings advanced as one
+    batch: each network's raster is bit-exact with its own simulation, and the
+    weights after applyStdp equal the solo runs' (networks back to back)."""
+    specs = gen.batch_izhikevich(count=6, nmin=200, nmax=3000, smin=20, smax=200, seed=11)
+    specs.insert(2, (_mixed_spec(3), False))
+    conf = _conf(nemo_amd, stdp_fn())
+    nets = []
+    for spec, _ in specs:
+        net = nemo_amd.Network()
+        spec.load(net)
+        nets.append(net)
+    batch = nemo_amd.Batch(nets, conf)
+    assert batch.size == len(specs)
+    steps, period = 300, 100
+    counts, fired, events = batch.run(steps, stdp_period=period, reward=1.0)
+    per = nemo_amd.Batch.split(counts, fired)
+    solo_events = 0
+    weights = []
+    for k, ((spec, stdp), net) in enumerate(zip(specs, nets)):
+        solo = nemo_amd.Simulation(net, _conf(nemo_amd, stdp_fn()) if stdp else nemo_amd.Configuration())
+        c, f, e = solo.run(steps, stdp_period=period if stdp else 0, reward=1.0)
+        assert np.array_equal(per[k][0], c), ("counts", k)
+        assert np.array_equal(per[k][1], f), ("fired", k)
+        assert c.sum() > 0
+        solo_events += e
+        weights.append(solo.fixed_weights())
+        iz_idx = spec.blocks[-1].idx
+        for i in iz_idx[:: max(1, iz_idx.size // 7)]:
+            assert same_bits(np.float32(batch.get_membrane_potential(int(i), net=k)),
+                             np.float32(solo.get_membrane_potential(int(i))))
+    assert events == solo_events
+    assert np.array_equal(batch.fixed_weights(), np.concatenate(weights))
+
+
+def test_launch_shapes_give_identical_results():
+    """Delivery blocks, bin capacity (down to 64 events, so that most events
+    take the overflow path), STDP fusion and STDP block width change only
+    speed: rasters, weights and accumulators are identical."""
+    spec = gen.random_izhikevich(4000, 150, dmax=15, seed=21, stdp=True)
+    shapes = [None, (1, 64, 1, 1), (7, 256, 0, 4), (200, 1024, 0, 2), (1024, 4096, 1, 1), (33, 2048, 1, 3)]
+    results = []
+    for shape in shapes:
+        sim = sc.make(nemo_amd, spec, _conf(nemo_amd, stdp_fn()))
+        if shape is not None:
+            sim.launch_shape = shape
+            assert sim.launch_shape == shape
+        c, f, e = sim.run(250, stdp_period=100)
+        results.append((c, f, e, sim.fixed_weights(), sim.stdp_accumulators()))
+    for r in results[1:]:
+        for x, y in zip(results[0], r):
+            assert np.array_equal(x, y)
+    assert results[0][0].sum() > 0
+
+
+def test_autotune_keeps_the_simulation_state():
+    spec = gen.random_izhikevich(5000, 200, dmax=20, seed=8, stdp=True)
+    a = sc.make(nemo_amd, spec, _conf(nemo_amd, stdp_fn()))
+    b = sc.make(nemo_amd, spec, _conf(nemo_amd, stdp_fn()))
+    ca1, fa1, ea1 = a.run(150, stdp_period=100)
+    shape = a.autotune(32)
+    assert a.launch_shape == shape
+    ca2, fa2, ea2 = a.run(150, stdp_period=100)
+    cb1, fb1, eb1 = b.run(150, stdp_period=100)
+    cb2, fb2, eb2 = b.run(150, stdp_period=100)
+    assert np.array_equal(np.concatenate([ca1, ca2]), np.concatenate([cb1, cb2]))
+    assert np.array_equal(np.concatenate([fa1, fa2]), np.concatenate([fb1, fb2]))
+    assert (ea1, ea2) == (eb1, eb2)
+    assert np.array_equal(a.fixed_weights(), b.fixed_weights())
+    assert np.array_equal(a.stdp_accumulators(), b.stdp_accumulators())
+
+
+# ------------------------------------------------------- mixed-model torus
+def test_torus_mixed_parity():
+    """The mixed-model torus network (IF_curr_exp, QIF, HH, Poisson; distance
+    dependent delays) at a tenth of its size: rasters, currents and every
+    state variable bit-exact with the oracle over 300 steps."""
+    spec = gen.torus_mixed(n_lif=4000, n_qif=800, n_hh=200, n_poisson=100, m=50, grid=15)
+    ref, hip = both(spec)
+    hip.set_debug(True)
+    for t in range(300):
+        fr = ref.step()
+        fh = hip.step()
+        assert fr == fh, ("raster diverged", t)
+        if t % 50 == 49:
+            er, ir = ref.currents()
+            eh, ih = hip.currents()
+            assert same_bits(er, eh) and same_bits(ir, ih), ("currents diverged", t)
+    for blk, ns in zip(spec.blocks[:3], (4, 1, 5)):
+        for pr, ph in zip(state_planes(ref, blk.idx[::7], ns), state_planes(hip, blk.idx[::7], ns)):
+            assert same_bits(pr, ph)
+    assert np.array_equal(ref.rng_state(), hip.rng_state())
+
+
+def test_torus_mixed_full_size_events():
+    """The full 51k-neuron, 500-synapse torus (2.55e7 synapses) for 200 steps:
+    events delivered equal the out-degrees implied by the raster."""
+    spec = gen.torus_mixed()
+    sim = sc.make(nemo_amd, spec)
+    steps = 200
+    D = int(spec.delay.max())
+    n = spec.neuron_count
+    counts, fired, events = sim.run(steps)
+    rowlen = np.zeros((n, D + 1), np.int64)
+    np.add.at(rowlen, (spec.source.astype(np.int64), spec.delay.astype(np.int64)), 1)
+    cum = np.cumsum(rowlen, axis=1)
+    step_of = np.repeat(np.arange(steps), counts)
+    horizon = np.minimum(steps - 1 - step_of, D)
+    assert events == int(cum[fired.astype(np.int64), horizon].sum())
+    assert counts.sum() > 0
