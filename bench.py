@@ -219,8 +219,8 @@ def main():
                     help="only run the plain copy kernel on BYTES (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: the HIP backend runs one simulation per device and this "
-              "build has no multi-GPU path (see DESIGN.md on sharding)", file=sys.stderr)
+        print("bench.py measures one GPU: the HIP backend runs one simulation per device and has no "
+              "multi-GPU path", file=sys.stderr)
         sys.exit(2)
 
     import ctypes

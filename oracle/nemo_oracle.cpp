@@ -19,7 +19,7 @@
  *   - step protocol                    src/nemo/SimulationBackend.cpp:22-77, cpu/Simulation.cpp:94-111
  *   - neuron models                    src/nemo/cpu/plugins/{Izhikevich,QIF,IF_curr_exp,HH,PoissonSource,Input}.cpp
  *
- * Deliberate deviations (each also listed in DESIGN.md):
+ * Deliberate deviations:
  *   1. QIF and HH index their per-type parameter/state arrays with the local
  *      index nl = n - start.  The reference indexes them with the simulation
  *      index n (QIF.cpp:53-105, HH.cpp:68-135), which reads out of bounds
@@ -29,6 +29,10 @@
  *      undefined behaviour in C++.
  *   3. Signed 32-bit overflow in STDP accumulation and weight update wraps
  *      (two's complement), which is what the reference's x86 build does.
+ *   4. In STDP accumulation a presynaptic history shifted by a delay of 64
+ *      is empty.  The reference shifts a 64-bit word by the delay
+ *      (ConnectivityMatrix.cpp:248), undefined behaviour for 64; the
+ *      case needs a postfire window of length 0.
  *
  * Build: g++ -O2 -fopenmp -ffp-contract=off (no fast-math), see Makefile.
  */
@@ -549,7 +553,8 @@ void accumulateStdp(oracle_simulation& s)
 	for(size_t target = 0; target < s.ncount; ++target) {
 		if(!(s.recentFiring[target] & s.stdp.postFireMask)) continue;
 		for(size_t r = s.rcmStart[target]; r < s.rcmStart[target + 1]; ++r) {
-			uint64_t preFiring = s.recentFiring[s.rcmSource[r]] >> s.rcmDelay[r];
+			const unsigned dr = s.rcmDelay[r];
+			uint64_t preFiring = dr >= 64 ? 0 : s.recentFiring[s.rcmSource[r]] >> dr;
 			fix_t w_diff = s.stdp.weightChange(preFiring);
 			if(w_diff != 0) {
 				s.rcmAcc[r] = wrap_add(s.rcmAcc[r], w_diff);
