@@ -215,6 +215,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--autotune-steps", type=int, default=128, help="steps per autotuner trial (0: default shape)")
     ap.add_argument("--diff-steps", type=int, default=500, help="steps of the differentiable run (0: skip it)")
+    ap.add_argument("--only", choices=["batched", "differentiable"],
+                    help="run one of the secondary workloads alone and print its JSON (profiling)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel on BYTES (counter calibration)")
     args = ap.parse_args()
@@ -232,6 +234,13 @@ def main():
         gbps = ctypes.c_double()
         check(lib().nemo_hip_copy_probe(args.copy_probe, 10, ctypes.byref(gbps)))
         print(json.dumps({"copy_probe_bytes": args.copy_probe, "launches": 11, "achieved_GBps": gbps.value}))
+        return
+
+    if args.only == "batched":
+        print(json.dumps(batched(nemo_amd, gen, args.batch_nets or 64, args.batch_steps, args.autotune_steps)))
+        return
+    if args.only == "differentiable":
+        print(json.dumps(differentiable(nemo_amd, gen, steps=args.diff_steps or 500)))
         return
 
     n, m = args.neurons, args.synapses

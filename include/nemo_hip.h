@@ -238,9 +238,11 @@ nemo_status_t nemo_hip_autotune(nemo_simulation_t, unsigned steps_per_trial, uns
  * All arrays are device pointers owned by the caller: weights w in network
  * order (source ascending, then insertion order, as synapse ids), per-neuron
  * a, b, c, d, v0, u0 by index, x, spikes and v_out [steps][N] row-major;
- * checkpoints [ceil(steps / checkpoint_every)][2][N] holds (v, u) before every
- * checkpoint_every-th step.  Work is queued on `stream` (a hipStream_t; NULL
- * for the default stream); calls on one handle must not overlap. */
+ * checkpoints, double [ceil(steps / checkpoint_every)][2][N], holds (v, u)
+ * before every checkpoint_every-th step (the state is kept in double; the
+ * weights and their per-step sum are float).  Work is queued on `stream` (a
+ * hipStream_t; NULL for the default stream); calls on one handle must not
+ * overlap. */
 typedef struct nemo_hip_float_network_class* nemo_hip_float_network_t;
 nemo_status_t nemo_hip_new_float_network(nemo_network_t, int device, nemo_hip_float_network_t* out);
 void nemo_hip_delete_float_network(nemo_hip_float_network_t);
@@ -249,7 +251,7 @@ nemo_status_t nemo_hip_float_network_sizes(nemo_hip_float_network_t, size_t* neu
 nemo_status_t nemo_hip_float_forward(nemo_hip_float_network_t, void* stream, unsigned steps,
 		unsigned checkpoint_every, const float* w, const float* a, const float* b, const float* c,
 		const float* d, const float* v0, const float* u0, const float* x,
-		float* spikes, float* v_out, float* checkpoints);
+		float* spikes, float* v_out, double* checkpoints);
 /* The adjoint of nemo_hip_float_forward: from dL/dspikes and dL/dv_out (either
  * may be NULL) and the forward's spikes and checkpoints, the gradients of the
  * loss with respect to w, a, b, d and x (grad_x may be NULL) and the initial
@@ -259,7 +261,7 @@ nemo_status_t nemo_hip_float_forward(nemo_hip_float_network_t, void* stream, uns
  * step + delay. */
 nemo_status_t nemo_hip_float_backward(nemo_hip_float_network_t, void* stream, unsigned steps,
 		unsigned checkpoint_every, const float* w, const float* a, const float* b, const float* c,
-		const float* d, const float* x, const float* spikes, const float* checkpoints,
+		const float* d, const float* x, const float* spikes, const double* checkpoints,
 		const float* grad_spikes, const float* grad_v, float* grad_w, float* grad_a, float* grad_b,
 		float* grad_d, float* grad_x, float* grad_v0, float* grad_u0);
 

@@ -48,6 +48,7 @@ void DeviceArray<T>::release()
 }
 
 template struct DeviceArray<float>;
+template struct DeviceArray<double>;
 template struct DeviceArray<uint32_t>;
 template struct DeviceArray<int32_t>;
 template struct DeviceArray<int64_t>;
@@ -554,6 +555,19 @@ void Simulation::buildNeurons(const std::vector<const Network*>& nets)
 	m_tiles = unsigned(npad / NEMO_TILE);
 	m_l2g.assign(m_npad, ~0u);
 	if(m_nets > 1) m_netOf.assign(m_npad, uint16_t(0xffff));
+	/* a network whose types appear in another order than the device's must
+	 * have its fired neurons reordered to its own simulation order */
+	m_reorder.assign(nets.size(), 0);
+	for(size_t k = 0; k < nets.size(); ++k) {
+		unsigned last = 0;
+		for(unsigned t = 0; t < nets[k]->types.size(); ++t) {
+			if(nets[k]->groups[t].gidx.empty()) continue;
+			const unsigned slot = slotOf[nets[k]->types[t].name];
+			if(slot < last) m_reorder[k] = 1;
+			last = slot;
+		}
+	}
+	if(std::find(m_reorder.begin(), m_reorder.end(), 1) != m_reorder.end()) m_simOf.assign(m_npad, 0);
 	m_typeOfLocal.assign(m_npad, -1);
 	m_order.resize(m_ncount);
 	m_g2l.reserve(m_ncount * 2);
@@ -596,6 +610,7 @@ void Simulation::buildNeurons(const std::vector<const Network*>& nets)
 				}
 				m_l2g[l] = ng.gidx[i];
 				if(m_nets > 1) m_netOf[l] = uint16_t(k);
+				if(!m_simOf.empty()) m_simOf[l] = uint32_t(simBase + i);
 				m_typeOfLocal[l] = int(m_groups.size() - 1);
 				m_order[netSimBase[k] + simBase + i] = l;
 			}
@@ -1154,9 +1169,8 @@ void Simulation::decodeFiring(const uint64_t* bits, std::vector<unsigned>& out) 
 }
 
 /* Fired neurons of every network of a batch for one step: out gets each
- * network's user indices in turn (ascending device order within a network,
- * which is its simulation order within each type) and counts one entry per
- * network. */
+ * network's user indices in turn, in that network's simulation order, and
+ * counts one entry per network. */
 void Simulation::decodeFiringBatch(const uint64_t* bits, std::vector<unsigned>& out,
 		std::vector<unsigned>& counts) const
 {
@@ -1172,25 +1186,29 @@ void Simulation::decodeFiringBatch(const uint64_t* bits, std::vector<unsigned>& 
 		while(word) {
 			const unsigned bit = unsigned(__builtin_ctzll(word));
 			word &= word - 1;
-			const size_t l = w * 64 + bit;
-			per[m_netOf[l]].push_back(m_l2g[l]);
+			per[m_netOf[w * 64 + bit]].push_back(unsigned(w * 64 + bit));
 		}
 	}
-	for(const std::vector<unsigned>& v : per) {
-		out.insert(out.end(), v.begin(), v.end());
+	for(unsigned k = 0; k < m_nets; ++k) {
+		std::vector<unsigned>& v = per[k];
+		if(m_reorder[k]) {
+			std::sort(v.begin(), v.end(), [this](unsigned x, unsigned y) { return m_simOf[x] < m_simOf[y]; });
+		}
+		for(unsigned l : v) out.push_back(m_l2g[l]);
 		counts.push_back(unsigned(v.size()));
 	}
 }
 
 /* Copy the bitmaps of steps [t0, t0+k) (k <= NEMO_STEP_RING) to m_pinnedBits. */
-void Simulation::readBitmaps(uint64_t t0, unsigned k)
+void Simulation::readBitmaps(uint64_t t0, unsigned k, uint64_t* dst)
 {
 	const size_t W = m_firedWords;
+	if(!dst) dst = m_pinnedBits;
 	unsigned done = 0;
 	while(done < k) {
 		const uint64_t slot = (t0 + done) & (NEMO_STEP_RING - 1);
 		const unsigned len = unsigned(std::min<uint64_t>(k - done, NEMO_STEP_RING - slot));
-		HIP_CHECK(hipMemcpyAsync(m_pinnedBits + size_t(done) * W, m_firedBits.ptr + slot * W,
+		HIP_CHECK(hipMemcpyAsync(dst + size_t(done) * W, m_firedBits.ptr + slot * W,
 					size_t(len) * W * sizeof(uint64_t), hipMemcpyDeviceToHost, m_stream));
 		done += len;
 	}
@@ -1300,32 +1318,63 @@ void Simulation::run(unsigned nsteps, unsigned stdpPeriod, float reward, bool re
 	HIP_CHECK(hipStreamSynchronize(m_stream));
 	const uint64_t events0 = events ? sumCounters(m_blockEvents) : 0;
 	uint64_t stepsSinceApply = 0;
-	const unsigned CHUNK = 64;
-	unsigned done = 0;
-	while(done < nsteps) {
-		unsigned k = std::min(CHUNK, nsteps - done);
-		if(stdpPeriod) k = unsigned(std::min<uint64_t>(k, stdpPeriod - stepsSinceApply));
-		const uint64_t t0 = m_hostStep;
-		hipGraphExec_t g = graphFor(k);
-		setBase(t0);
-		HIP_CHECK(hipGraphLaunch(g, m_stream));
-		m_hostStep += k;
-		if(readRaster) {
-			readBitmaps(t0, k);
+	const unsigned CHUNK = NEMO_STEP_RING / 2;
+	/* Chunks alternate between the two halves of the pinned bitmap buffer:
+	 * chunk i+1 is queued before chunk i's bitmaps are decoded on the host,
+	 * so the device does not wait for the host between chunks.  The device
+	 * ring holds 128 steps, so the bitmaps of chunk i are still intact while
+	 * chunk i+1 runs. */
+	hipEvent_t copied[2];
+	HIP_CHECK(hipEventCreateWithFlags(&copied[0], hipEventDisableTiming));
+	HIP_CHECK(hipEventCreateWithFlags(&copied[1], hipEventDisableTiming));
+	unsigned pendingSteps = 0; /* steps of the chunk waiting to be decoded */
+	int pendingHalf = -1;
+	auto decodePending = [&]() {
+		if(pendingHalf < 0) return;
+		HIP_CHECK(hipEventSynchronize(copied[pendingHalf]));
+		const uint64_t* bits = m_pinnedBits + size_t(pendingHalf) * CHUNK * m_firedWords;
+		for(unsigned s = 0; s < pendingSteps; ++s) {
+			decodeFiringBatch(bits + size_t(s) * m_firedWords, *fired, *counts);
 		}
-		HIP_CHECK(hipStreamSynchronize(m_stream));
-		if(readRaster) {
-			for(unsigned s = 0; s < k; ++s) {
-				decodeFiringBatch(m_pinnedBits + size_t(s) * m_firedWords, *fired, *counts);
+		pendingHalf = -1;
+	};
+	try {
+		unsigned done = 0;
+		int half = 0;
+		while(done < nsteps) {
+			unsigned k = std::min(CHUNK, nsteps - done);
+			if(stdpPeriod) k = unsigned(std::min<uint64_t>(k, stdpPeriod - stepsSinceApply));
+			const uint64_t t0 = m_hostStep;
+			hipGraphExec_t g = graphFor(k);
+			setBase(t0);
+			HIP_CHECK(hipGraphLaunch(g, m_stream));
+			m_hostStep += k;
+			if(readRaster) {
+				readBitmaps(t0, k, m_pinnedBits + size_t(half) * CHUNK * m_firedWords);
+				HIP_CHECK(hipEventRecord(copied[half], m_stream));
+			}
+			decodePending();
+			if(readRaster) {
+				pendingHalf = half;
+				pendingSteps = k;
+				half ^= 1;
+			}
+			done += k;
+			stepsSinceApply += k;
+			if(stdpPeriod && stepsSinceApply == stdpPeriod) {
+				applyStdp(reward);
+				stepsSinceApply = 0;
 			}
 		}
-		done += k;
-		stepsSinceApply += k;
-		if(stdpPeriod && stepsSinceApply == stdpPeriod) {
-			applyStdp(reward);
-			stepsSinceApply = 0;
-		}
+		decodePending();
+	} catch(...) {
+		(void) hipStreamSynchronize(m_stream);
+		(void) hipEventDestroy(copied[0]);
+		(void) hipEventDestroy(copied[1]);
+		throw;
 	}
+	(void) hipEventDestroy(copied[0]);
+	(void) hipEventDestroy(copied[1]);
 	HIP_CHECK(hipStreamSynchronize(m_stream));
 	if(events) *events = sumCounters(m_blockEvents) - events0;
 	m_firedOut.clear();

@@ -244,6 +244,10 @@ private:
 
 	std::vector<unsigned> m_l2g;          /* device index -> user index (~0 for padding) */
 	std::vector<uint16_t> m_netOf;        /* device index -> network (batches only) */
+	std::vector<uint32_t> m_simOf;        /* device index -> simulation index in its network
+	                                         (batches whose device type order differs from a
+	                                         network's own type order) */
+	std::vector<char> m_reorder;          /* per network: device order != simulation order */
 	std::vector<uint32_t> m_order;        /* simulation index (networks back to back) -> device index */
 	std::unordered_map<uint64_t, unsigned> m_g2l; /* network << 32 | user index -> device index */
 	std::vector<int> m_typeOfLocal;       /* device index -> group (-1 for padding) */
@@ -326,7 +330,7 @@ private:
 	void flushStdp();
 	void decodeFiring(const uint64_t* bits, std::vector<unsigned>& out) const;
 	void decodeFiringBatch(const uint64_t* bits, std::vector<unsigned>& out, std::vector<unsigned>& counts) const;
-	void readBitmaps(uint64_t t0, unsigned k);
+	void readBitmaps(uint64_t t0, unsigned k, uint64_t* dst = nullptr);
 	uint64_t sumCounters(const DeviceArray<unsigned long long>& a);
 	hipGraphExec_t graphFor(unsigned k);
 	void clearGraphs();
@@ -353,9 +357,9 @@ public:
 
 	void forward(hipStream_t st, unsigned T, unsigned K, const float* w, const float* a, const float* b,
 			const float* c, const float* d, const float* v0, const float* u0, const float* x,
-			float* S, float* V, float* ckpt);
+			float* S, float* V, double* ckpt);
 	void backward(hipStream_t st, unsigned T, unsigned K, const float* w, const float* a, const float* b,
-			const float* c, const float* d, const float* x, const float* S, const float* ckpt,
+			const float* c, const float* d, const float* x, const float* S, const double* ckpt,
 			const float* gS, const float* gV, float* gw, float* ga, float* gb, float* gd, float* gx,
 			float* gv0, float* gu0);
 
@@ -365,7 +369,9 @@ private:
 	unsigned m_D = 1;
 	DeviceArray<uint32_t> m_fcmPtr, m_fcmTd;          /* network order is forward order */
 	DeviceArray<uint32_t> m_rcmPtr, m_rcmSd, m_rcmUser;
-	DeviceArray<float> m_wr, m_v, m_u, m_LV, m_LU, m_LI, m_seg;
+	DeviceArray<float> m_wr;
+	DeviceArray<double> m_v, m_u, m_seg;  /* state in double; segment: entry v, u and input I */
+	DeviceArray<double> m_LV, m_LU, m_LI; /* adjoint state and recent input adjoints */
 	unsigned m_segK = 0;
 
 	void prepare(unsigned T, unsigned K);

@@ -27,15 +27,15 @@ typedef struct nemo_float_step_args {
 	const float* x;            /* [T][n] external current */
 	float* S;                  /* [T][n] spikes (read for t-d, written at t when store) */
 	float* V;                  /* [T][n] membrane potential after the step (store) */
-	float* v;                  /* [n] state, updated */
-	float* u;
+	double* v;                 /* [n] state, updated */
+	double* u;
 	int store;                 /* write S[t], V[t] */
 	int record;                /* write the step's entry state and input to seg_* */
 	unsigned t0;               /* first step of the recorded segment */
-	float* seg_v;              /* [K][n] */
-	float* seg_u;
-	float* seg_I;
-	float* ckpt;               /* NULL, or [ceil(T/K)][2][n]: (v, u) before every K-th step */
+	double* seg_v;             /* [K][n] */
+	double* seg_u;
+	double* seg_I;
+	double* ckpt;              /* NULL, or [ceil(T/K)][2][n]: (v, u) before every K-th step */
 	unsigned K;
 } nemo_float_step_args;
 
@@ -56,12 +56,12 @@ typedef struct nemo_float_back_args {
 	const float* gS;           /* [T][n] dL/dS, or NULL */
 	const float* gV;           /* [T][n] dL/dV, or NULL */
 	unsigned t0;               /* first step of the recorded segment */
-	const float* seg_v;        /* recorded entry state and input of the segment */
-	const float* seg_u;
-	const float* seg_I;
-	float* LV;                 /* [n] dL/dv of the state after step t; becomes the one before */
-	float* LU;
-	float* LI;                 /* [ring][n] dL/dI of recent steps */
+	const double* seg_v;       /* recorded entry state and input of the segment */
+	const double* seg_u;
+	const double* seg_I;
+	double* LV;                /* [n] dL/dv of the state after step t; becomes the one before */
+	double* LU;
+	double* LI;                /* [ring][n] dL/dI of recent steps */
 	float* gw;                 /* [synapses] dL/dw in forward-matrix order, accumulated */
 	float* ga;                 /* [n] accumulated */
 	float* gb;
@@ -76,6 +76,11 @@ hipError_t nemo_launch_gather(hipStream_t, float* out, const float* in, const ui
 /* out[idx[k]] = in[k] (idx a permutation) */
 hipError_t nemo_launch_scatter(hipStream_t, float* out, const float* in, const uint32_t* idx, size_t n);
 hipError_t nemo_launch_fill(hipStream_t, float* p, float v, size_t n);
+hipError_t nemo_launch_fill_double(hipStream_t, double* p, double v, size_t n);
+/* out[k] = float(in[k]) */
+hipError_t nemo_launch_narrow(hipStream_t, float* out, const double* in, size_t n);
+/* out[k] = double(in[k]) */
+hipError_t nemo_launch_widen(hipStream_t, double* out, const float* in, size_t n);
 
 #ifdef __cplusplus
 }

@@ -6,8 +6,8 @@
  * src/nemo/cpu/plugins/Izhikevich.cpp:14-98, whose neuron step this mode
  * keeps).
  *
- * Model, per 1 ms step t and neuron j (v, u stored as float between steps,
- * evaluated in double within a step):
+ * Model, per 1 ms step t and neuron j (weights in float; their sum, the
+ * state v, u and all adjoint quantities in double):
  *   I    = x[t][j] + sum over synapses (i -> j, delay d, weight w) of w * S[t-d][i]
  *   4 substeps k (h = 0.25), each only while the neuron has not crossed 30 mV
  *   in an earlier substep (mask m_k):
@@ -41,7 +41,7 @@ constexpr unsigned WAVES = BLOCK / 64;
 constexpr uint32_t IDX_MASK = 0x3ffffffu;
 constexpr double H = 0.25;
 
-__device__ inline float wave_sum(float x)
+__device__ inline double wave_sum(double x)
 {
 #pragma unroll
 	for(int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
@@ -62,18 +62,18 @@ forward_step_kernel(nemo_float_step_args a)
 	if(j >= a.n) return;
 	const unsigned t = a.t;
 	const size_t n = a.n;
-	float acc = 0.0f;
+	double acc = 0.0;
 	for(uint32_t k = a.rcm_ptr[j] + lane; k < a.rcm_ptr[j + 1]; k += 64) {
 		const uint32_t sd = a.rcm_sd[k];
 		const unsigned d = (sd >> 26) + 1;
-		if(t >= d && a.S[size_t(t - d) * n + (sd & IDX_MASK)] != 0.0f) acc += a.wr[k];
+		if(t >= d && a.S[size_t(t - d) * n + (sd & IDX_MASK)] != 0.0f) acc += double(a.wr[k]);
 	}
 	acc = wave_sum(acc);
 	if(lane != 0) return;
 
-	const float I = acc + a.x[size_t(t) * n + j];
-	const float v0 = a.v[j];
-	const float u0 = a.u[j];
+	const double I = acc + double(a.x[size_t(t) * n + j]);
+	const double v0 = a.v[j];
+	const double u0 = a.u[j];
 	if(a.record) {
 		const size_t r = size_t(t - a.t0) * n + j;
 		a.seg_v[r] = v0;
@@ -90,7 +90,7 @@ forward_step_kernel(nemo_float_step_args a)
 #pragma unroll
 	for(int k = 0; k < 4; ++k) {
 		if(m) {
-			v = v + H * (0.04 * v * v + 5.0 * v + 140.0 - u + double(I));
+			v = v + H * (0.04 * v * v + 5.0 * v + 140.0 - u + I);
 			u = u + H * A * (B * v - u);
 			m = v < 30.0;
 		}
@@ -100,8 +100,8 @@ forward_step_kernel(nemo_float_step_args a)
 		v = double(a.pc[j]);
 		u = u + double(a.pd[j]);
 	}
-	a.v[j] = float(v);
-	a.u[j] = float(u);
+	a.v[j] = v;
+	a.u[j] = u;
 	if(a.store) {
 		a.S[size_t(t) * n + j] = s ? 1.0f : 0.0f;
 		a.V[size_t(t) * n + j] = float(v);
@@ -121,23 +121,23 @@ backward_step_kernel(nemo_float_back_args a)
 
 	/* adjoint of this step's spike from the steps it reaches (transposed
 	 * delivery), and the weight gradient of the synapses it drove */
-	float acc = 0.0f;
+	double acc = 0.0;
 	for(uint32_t k = a.fcm_ptr[i] + lane; k < a.fcm_ptr[i + 1]; k += 64) {
 		const uint32_t td = a.fcm_td[k];
 		const unsigned d = (td >> 26) + 1;
 		if(tau + d < a.T) {
-			const float l = a.LI[size_t((tau + d) % R) * n + (td & IDX_MASK)];
-			acc += a.wf[k] * l;
-			if(fired) a.gw[k] += l;
+			const double l = a.LI[size_t((tau + d) % R) * n + (td & IDX_MASK)];
+			acc += double(a.wf[k]) * l;
+			if(fired) a.gw[k] = float(double(a.gw[k]) + l);
 		}
 	}
 	acc = wave_sum(acc);
 	if(lane != 0) return;
 
 	const size_t r = size_t(tau) * n + i;
-	const double lS = (a.gS ? double(a.gS[r]) : 0.0) + double(acc);
-	double lv = double(a.LV[i]) + (a.gV ? double(a.gV[r]) : 0.0); /* d L / d v(tau+1) */
-	double lu = double(a.LU[i]);                                  /* d L / d u(tau+1) */
+	const double lS = (a.gS ? double(a.gS[r]) : 0.0) + acc;
+	double lv = a.LV[i] + (a.gV ? double(a.gV[r]) : 0.0); /* d L / d v(tau+1) */
+	double lu = a.LU[i];                                  /* d L / d u(tau+1) */
 
 	/* recompute the step's substeps from the recorded state and input */
 	const size_t q = size_t(tau - a.t0) * n + i;
@@ -182,9 +182,9 @@ backward_step_kernel(nemo_float_back_args a)
 		Lu = nLu;
 		lI += Lv_tot * H;
 	}
-	a.LV[i] = float(Lv);
-	a.LU[i] = float(Lu);
-	a.LI[size_t(tau % R) * n + i] = float(lI);
+	a.LV[i] = Lv;
+	a.LU[i] = Lu;
+	a.LI[size_t(tau % R) * n + i] = lI;
 	if(a.gx) a.gx[r] = float(lI);
 	a.ga[i] += float(ga);
 	a.gb[i] += float(gb);
@@ -203,7 +203,25 @@ __global__ void scatter_kernel(float* out, const float* in, const uint32_t* idx,
 	if(k < n) out[idx[k]] = in[k];
 }
 
+__global__ void widen_kernel(double* out, const float* in, size_t n)
+{
+	const size_t k = size_t(blockIdx.x) * BLOCK + threadIdx.x;
+	if(k < n) out[k] = double(in[k]);
+}
+
+__global__ void narrow_kernel(float* out, const double* in, size_t n)
+{
+	const size_t k = size_t(blockIdx.x) * BLOCK + threadIdx.x;
+	if(k < n) out[k] = float(in[k]);
+}
+
 __global__ void fill_kernel(float* p, float v, size_t n)
+{
+	const size_t k = size_t(blockIdx.x) * BLOCK + threadIdx.x;
+	if(k < n) p[k] = v;
+}
+
+__global__ void fill_double_kernel(double* p, double v, size_t n)
 {
 	const size_t k = size_t(blockIdx.x) * BLOCK + threadIdx.x;
 	if(k < n) p[k] = v;
@@ -246,6 +264,27 @@ hipError_t nemo_launch_scatter(hipStream_t st, float* out, const float* in, cons
 {
 	if(n == 0) return hipSuccess;
 	hipLaunchKernelGGL(scatter_kernel, dim3(blocks_for(n)), dim3(BLOCK), 0, st, out, in, idx, n);
+	return hipGetLastError();
+}
+
+hipError_t nemo_launch_fill_double(hipStream_t st, double* p, double v, size_t n)
+{
+	if(n == 0) return hipSuccess;
+	hipLaunchKernelGGL(fill_double_kernel, dim3(blocks_for(n)), dim3(BLOCK), 0, st, p, v, n);
+	return hipGetLastError();
+}
+
+hipError_t nemo_launch_narrow(hipStream_t st, float* out, const double* in, size_t n)
+{
+	if(n == 0) return hipSuccess;
+	hipLaunchKernelGGL(narrow_kernel, dim3(blocks_for(n)), dim3(BLOCK), 0, st, out, in, n);
+	return hipGetLastError();
+}
+
+hipError_t nemo_launch_widen(hipStream_t st, double* out, const float* in, size_t n)
+{
+	if(n == 0) return hipSuccess;
+	hipLaunchKernelGGL(widen_kernel, dim3(blocks_for(n)), dim3(BLOCK), 0, st, out, in, n);
 	return hipGetLastError();
 }
 

@@ -104,12 +104,12 @@ void FloatNetwork::prepare(unsigned T, unsigned K)
 
 void FloatNetwork::forward(hipStream_t st, unsigned T, unsigned K, const float* w, const float* a,
 		const float* b, const float* c, const float* d, const float* v0, const float* u0, const float* x,
-		float* S, float* V, float* ckpt)
+		float* S, float* V, double* ckpt)
 {
 	prepare(T, K);
 	HIP_CHECK(nemo_launch_gather(st, m_wr.ptr, w, m_rcmUser.ptr, m_s));
-	HIP_CHECK(hipMemcpyAsync(m_v.ptr, v0, m_n * sizeof(float), hipMemcpyDeviceToDevice, st));
-	HIP_CHECK(hipMemcpyAsync(m_u.ptr, u0, m_n * sizeof(float), hipMemcpyDeviceToDevice, st));
+	HIP_CHECK(nemo_launch_widen(st, m_v.ptr, v0, m_n));
+	HIP_CHECK(nemo_launch_widen(st, m_u.ptr, u0, m_n));
 	nemo_float_step_args f = {};
 	f.n = unsigned(m_n);
 	f.rcm_ptr = m_rcmPtr.ptr;
@@ -138,15 +138,15 @@ void FloatNetwork::forward(hipStream_t st, unsigned T, unsigned K, const float* 
  * its forward steps recording entry state and input, then run its adjoint
  * steps backwards. */
 void FloatNetwork::backward(hipStream_t st, unsigned T, unsigned K, const float* w, const float* a,
-		const float* b, const float* c, const float* d, const float* x, const float* S, const float* ckpt,
+		const float* b, const float* c, const float* d, const float* x, const float* S, const double* ckpt,
 		const float* gS, const float* gV, float* gw, float* ga, float* gb, float* gd, float* gx,
 		float* gv0, float* gu0)
 {
 	prepare(T, K);
 	HIP_CHECK(nemo_launch_gather(st, m_wr.ptr, w, m_rcmUser.ptr, m_s));
-	HIP_CHECK(nemo_launch_fill(st, m_LV.ptr, 0.0f, m_n));
-	HIP_CHECK(nemo_launch_fill(st, m_LU.ptr, 0.0f, m_n));
-	HIP_CHECK(nemo_launch_fill(st, m_LI.ptr, 0.0f, m_LI.n));
+	HIP_CHECK(nemo_launch_fill_double(st, m_LV.ptr, 0.0, m_n));
+	HIP_CHECK(nemo_launch_fill_double(st, m_LU.ptr, 0.0, m_n));
+	HIP_CHECK(nemo_launch_fill_double(st, m_LI.ptr, 0.0, m_LI.n));
 	HIP_CHECK(nemo_launch_fill(st, gw, 0.0f, m_s));
 	HIP_CHECK(nemo_launch_fill(st, ga, 0.0f, m_n));
 	HIP_CHECK(nemo_launch_fill(st, gb, 0.0f, m_n));
@@ -206,9 +206,9 @@ void FloatNetwork::backward(hipStream_t st, unsigned T, unsigned K, const float*
 	for(unsigned s = segments; s-- > 0;) {
 		const unsigned t0 = s * K;
 		const unsigned t1 = std::min(T, t0 + K);
-		HIP_CHECK(hipMemcpyAsync(m_v.ptr, ckpt + size_t(s) * 2 * m_n, m_n * sizeof(float),
+		HIP_CHECK(hipMemcpyAsync(m_v.ptr, ckpt + size_t(s) * 2 * m_n, m_n * sizeof(double),
 					hipMemcpyDeviceToDevice, st));
-		HIP_CHECK(hipMemcpyAsync(m_u.ptr, ckpt + size_t(s) * 2 * m_n + m_n, m_n * sizeof(float),
+		HIP_CHECK(hipMemcpyAsync(m_u.ptr, ckpt + size_t(s) * 2 * m_n + m_n, m_n * sizeof(double),
 					hipMemcpyDeviceToDevice, st));
 		f.t0 = t0;
 		for(unsigned t = t0; t < t1; ++t) {
@@ -221,8 +221,8 @@ void FloatNetwork::backward(hipStream_t st, unsigned T, unsigned K, const float*
 			HIP_CHECK(nemo_launch_float_backward_step(st, &g));
 		}
 	}
-	if(gv0) HIP_CHECK(hipMemcpyAsync(gv0, m_LV.ptr, m_n * sizeof(float), hipMemcpyDeviceToDevice, st));
-	if(gu0) HIP_CHECK(hipMemcpyAsync(gu0, m_LU.ptr, m_n * sizeof(float), hipMemcpyDeviceToDevice, st));
+	if(gv0) HIP_CHECK(nemo_launch_narrow(st, gv0, m_LV.ptr, m_n));
+	if(gu0) HIP_CHECK(nemo_launch_narrow(st, gu0, m_LU.ptr, m_n));
 }
 
 } // namespace nemo_hip

@@ -32,6 +32,9 @@ constexpr uint64_t RING_MASK = NEMO_STEP_RING - 1;
 /* fired rows a delivery block lists in LDS at a time */
 constexpr unsigned ROWS = 1024;
 constexpr unsigned ROWS_PER_THREAD = ROWS / NEMO_BLOCK;
+/* independent global loads each thread keeps in flight in the event and
+ * synapse loops */
+constexpr int ILP = 8;
 
 __device__ inline unsigned kth_set_bit(uint64_t w, unsigned k)
 {
@@ -127,10 +130,10 @@ __device__ void deliver_block(const nemo_propagate_args& a, uint64_t t, int2* st
 			__syncthreads();
 
 			/* the events of these rows, evenly over the block */
-			for(uint32_t e0 = tid; e0 < M; e0 += 4 * NEMO_BLOCK) {
-				int2 s[4];
+			for(uint32_t e0 = tid; e0 < M; e0 += ILP * NEMO_BLOCK) {
+				int2 s[ILP];
 #pragma unroll
-				for(int u = 0; u < 4; ++u) {
+				for(int u = 0; u < ILP; ++u) {
 					const uint32_t e = e0 + u * NEMO_BLOCK;
 					s[u] = make_int2(0, 0);
 					if(e < M) {
@@ -143,7 +146,7 @@ __device__ void deliver_block(const nemo_propagate_args& a, uint64_t t, int2* st
 					}
 				}
 #pragma unroll
-				for(int u = 0; u < 4; ++u) {
+				for(int u = 0; u < ILP; ++u) {
 					const uint32_t e = e0 + u * NEMO_BLOCK;
 					if(e < M) {
 						const uint32_t pos = fill + e;
@@ -257,14 +260,16 @@ __device__ void stdp_block(const nemo_propagate_args& a, uint64_t s, unsigned sb
 	__syncthreads();
 	if(total == 0) return;
 
-	for(uint32_t e0 = tid; e0 < total; e0 += 4 * NEMO_BLOCK) {
-		uint32_t k[4];
-		uint32_t sd[4];
+	for(uint32_t e0 = tid; e0 < total; e0 += ILP * NEMO_BLOCK) {
+		uint32_t k[ILP];
+		uint32_t sd[ILP];
+		int32_t acc[ILP];
 #pragma unroll
-		for(int u = 0; u < 4; ++u) {
+		for(int u = 0; u < ILP; ++u) {
 			const uint32_t e = e0 + u * NEMO_BLOCK;
 			k[u] = 0;
 			sd[u] = 0;
+			acc[u] = 0;
 			if(e < total) {
 				unsigned lo = 0, hi = nfired; /* s_pos[lo] <= e < s_pos[hi] */
 				while(hi - lo > 1) {
@@ -273,21 +278,17 @@ __device__ void stdp_block(const nemo_propagate_args& a, uint64_t s, unsigned sb
 				}
 				k[u] = s_rb[lo] + (e - s_pos[lo]);
 				sd[u] = a.rcm_sd[k[u]];
-			}
-		}
-		uint64_t h[4];
-		int32_t acc[4];
-#pragma unroll
-		for(int u = 0; u < 4; ++u) {
-			h[u] = 0;
-			acc[u] = 0;
-			if(e0 + u * NEMO_BLOCK < total) {
-				h[u] = a.hist[sd[u] & 0x3ffffffu];
 				acc[u] = a.rcm_acc[k[u]];
 			}
 		}
+		uint64_t h[ILP];
 #pragma unroll
-		for(int u = 0; u < 4; ++u) {
+		for(int u = 0; u < ILP; ++u) {
+			h[u] = 0;
+			if(e0 + u * NEMO_BLOCK < total) h[u] = a.hist[sd[u] & 0x3ffffffu];
+		}
+#pragma unroll
+		for(int u = 0; u < ILP; ++u) {
 			if(e0 + u * NEMO_BLOCK < total) {
 				const unsigned d = (sd[u] >> 26) + 1;
 				const uint64_t preFiring = d >= 64 ? 0 : (h[u] >> d);

@@ -679,7 +679,7 @@ nemo_status_t nemo_hip_float_network_sizes(nemo_hip_float_network_t f, size_t* n
 nemo_status_t nemo_hip_float_forward(nemo_hip_float_network_t f, void* stream, unsigned steps,
 		unsigned checkpoint_every, const float* w, const float* a, const float* b, const float* c,
 		const float* d, const float* v0, const float* u0, const float* x,
-		float* spikes, float* v_out, float* checkpoints)
+		float* spikes, float* v_out, double* checkpoints)
 {
 	return guarded([&] {
 		check(f, "float network");
@@ -693,7 +693,7 @@ nemo_status_t nemo_hip_float_forward(nemo_hip_float_network_t f, void* stream, u
 
 nemo_status_t nemo_hip_float_backward(nemo_hip_float_network_t f, void* stream, unsigned steps,
 		unsigned checkpoint_every, const float* w, const float* a, const float* b, const float* c,
-		const float* d, const float* x, const float* spikes, const float* checkpoints,
+		const float* d, const float* x, const float* spikes, const double* checkpoints,
 		const float* grad_spikes, const float* grad_v, float* grad_w, float* grad_a, float* grad_b,
 		float* grad_d, float* grad_x, float* grad_v0, float* grad_u0)
 {

@@ -99,11 +99,12 @@ __device__ inline void gather_input(const nemo_hip_step_buffers& b, const nemo_h
 	if(tid == 0) s_pre[B] = total;
 	__syncthreads();
 	const size_t cap = b.bin_capacity;
-	/* four independent loads in flight per thread before the LDS atomics */
-	for(uint32_t j0 = tid; j0 < total; j0 += 4 * NEMO_TILE) {
-		int2 ev[4];
+	/* GATHER_ILP independent loads in flight per thread before the LDS atomics */
+	constexpr int GATHER_ILP = 8;
+	for(uint32_t j0 = tid; j0 < total; j0 += GATHER_ILP * NEMO_TILE) {
+		int2 ev[GATHER_ILP];
 #pragma unroll
-		for(int u = 0; u < 4; ++u) {
+		for(int u = 0; u < GATHER_ILP; ++u) {
 			const uint32_t j = j0 + u * NEMO_TILE;
 			ev[u] = make_int2(0, 0);
 			if(j < total) {
@@ -116,7 +117,7 @@ __device__ inline void gather_input(const nemo_hip_step_buffers& b, const nemo_h
 			}
 		}
 #pragma unroll
-		for(int u = 0; u < 4; ++u) {
+		for(int u = 0; u < GATHER_ILP; ++u) {
 			if(j0 + u * NEMO_TILE < total) {
 				long long* dst = ev[u].y >= 0 ? &in.E[ev[u].x] : &in.I[ev[u].x];
 				atomicAdd(reinterpret_cast<unsigned long long*>(dst), (unsigned long long)(long long)ev[u].y);

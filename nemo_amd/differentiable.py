@@ -45,7 +45,7 @@ class _Spiking(torch.autograd.Function):
         dev = x.device
         S = torch.empty((T, N), dtype=torch.float32, device=dev)
         V = torch.empty((T, N), dtype=torch.float32, device=dev)
-        ck = torch.empty(((T + K - 1) // K, 2, N), dtype=torch.float32, device=dev)
+        ck = torch.empty(((T + K - 1) // K, 2, N), dtype=torch.float64, device=dev)
         check(lib().nemo_hip_float_forward(fnet._h, _stream(dev), T, K, _ptr(w), _ptr(a), _ptr(b), _ptr(c),
                                            _ptr(d), _ptr(v0), _ptr(u0), _ptr(x), _ptr(S), _ptr(V), _ptr(ck)))
         ctx.save_for_backward(w, a, b, d, x, c, S, ck)
