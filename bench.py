@@ -35,6 +35,22 @@ BYTES_PER_IZH_NEURON = 84      # 20 B params + 16 B u,v r/w + 32 B RNG r/w + 16 
 BYTES_PER_STDP_SYNAPSE = 20    # 4 B packed source/delay + 8 B presynaptic history + 8 B accumulator r/w
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprof counter
+    summary of this workload (FETCH_SIZE and WRITE_SIZE passes scaled by the
+    copy-kernel calibration), or None if there is none."""
+    path = os.path.join(ROOT, "profiles", "propagate_update_izh100k_stdp.md")
+    names = {"propagate": "propagate_kernel", "update": "izhikevich_update"}
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                fields = dict(f.split("=") for f in line.split(":", 1)[1].split())
+                return float(fields[names[kernel]])
+    except (OSError, KeyError, ValueError):
+        pass
+    return None
+
+
 def stdp_function():
     from nemo_amd import generators as gen
     pre, post = gen.stdp_exp_function(0.1, -0.12, 20.0, 20)
@@ -304,7 +320,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(dominant),
             "algorithmic_bytes_per_launch": per_launch[dominant],
         },
         "step_roofline_frac": (step_bytes / (wall / args.steps)) / 1e9 / HBM_PEAK_GBS,
