@@ -212,8 +212,9 @@ nemo_status_t nemo_hip_prepare(nemo_simulation_t, unsigned nsteps, unsigned stdp
 
 /* Extension: launch shape of the per-step kernels: shape[0] delivery blocks
  * (1..1024), shape[1] events a delivery block stages in LDS (64..32768),
- * shape[2] 1 if STDP accumulation shares the delivery launch, shape[3] firing
- * bitmap words per STDP block (1..4).  Every shape gives identical results. */
+ * shape[2] 1 if STDP accumulation runs inside the neuron-update kernels, 0 for
+ * a launch of its own, shape[3] firing-bitmap words per block of that launch
+ * (1..4).  Every shape gives identical results. */
 nemo_status_t nemo_hip_get_launch_shape(nemo_simulation_t, unsigned shape[4]);
 nemo_status_t nemo_hip_set_launch_shape(nemo_simulation_t, const unsigned shape[4]);
 

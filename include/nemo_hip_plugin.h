@@ -27,9 +27,14 @@
  *   - read and clear ext_current when it is not NULL;
  *   - apply one 1 ms step of the model; a neuron fires if the model says so or
  *     if its bit in fstim is set (fstim may be NULL);
- *   - shift the neuron's 64-bit firing history left by one, OR-ing in firing;
+ *   - shift the neuron's 64-bit firing history left by one, OR-ing in firing:
+ *     the history after step s is in plane s % 2 of hist (plane p starts at
+ *     p * hist_pitch), so the update reads plane (s-1) % 2 and writes plane
+ *     s % 2;
  *   - write the tile's words of the step's firing bitmap,
- *     fired_bits[(s % NEMO_STEP_RING) * fired_words + index / 64].
+ *     fired_bits[(s % NEMO_STEP_RING) * fired_words + index / 64];
+ *   - when stdp.rcm_ptr is not NULL, run the STDP accumulation of step s-1 for
+ *     the tile's neurons (see nemo_hip_stdp).
  * The helpers in nemo_amd/csrc/plugin_device.h implement all of this but the
  * model step.  All pointers are device pointers; the launch goes on the given
  * stream and must not synchronise it, so that the backend can capture it in a
@@ -72,6 +77,29 @@ typedef struct nemo_hip_neuron_group {
 	uint32_t* rng;     /* 4 xorshift words per neuron, interleaved; may be NULL */
 } nemo_hip_neuron_group;
 
+/* STDP accumulation of step r = s-1 for the neurons of an update block's tile
+ * (ConnectivityMatrix::accumulateStdp, ConnectivityMatrix.cpp:224-258;
+ * StdpProcess::weightChange, StdpProcess.cpp:68-124).  Skipped when s == 0,
+ * r < post_window or r == clock->stdp_flushed.  A neuron n whose history
+ * after step r has bit post_window set fired post_window steps before r; for
+ * each of its plastic incoming synapses k (rcm_ptr[n] <= k < rcm_ptr[n+1],
+ * rcm_sd[k] = source | (delay-1) << 26) the presynaptic firing is the
+ * source's history after step r shifted right by the delay (empty for 64);
+ * the nearest arrival before and after the postsynaptic spike, within the
+ * potentiation and depression masks, selects fn_pre[dt] or fn_post[dt] (a tie
+ * gives nothing), and the sum of the two regions is added to rcm_acc[k] with
+ * 32-bit wrap-around.  examined[tile] counts the synapses looked at. */
+typedef struct nemo_hip_stdp {
+	const uint32_t* rcm_ptr;   /* NULL: no STDP accumulation in the update */
+	const uint32_t* rcm_sd;
+	int32_t* rcm_acc;
+	unsigned long long* examined;
+	unsigned post_window;      /* P = |postfire| */
+	uint64_t potentiation, depression, pre_post, post_pre;
+	int32_t fn_pre[64];        /* fixed-point weight change at dt = index+1 */
+	int32_t fn_post[64];
+} nemo_hip_stdp;
+
 /* Per-simulation buffers shared by all groups in a step. */
 typedef struct nemo_hip_step_buffers {
 	uint64_t step_offset;      /* step = clock->base_step + step_offset */
@@ -86,11 +114,13 @@ typedef struct nemo_hip_step_buffers {
 	size_t acc_pitch;
 	float* ext_current;        /* per device index, or NULL */
 	const uint32_t* fstim;     /* firing-stimulus bit-vector, or NULL */
-	uint64_t* hist;            /* 64-bit firing history per device index */
+	uint64_t* hist;            /* 64-bit firing history: two planes of hist_pitch */
+	size_t hist_pitch;
 	uint64_t* fired_bits;      /* NEMO_STEP_RING bitmaps of fired_words words */
 	size_t fired_words;
 	unsigned fbits;            /* fractional bits of the fixed-point currents */
 	float* debug_current;      /* NULL, or 2 planes of acc_pitch floats receiving E and I */
+	nemo_hip_stdp stdp;
 } nemo_hip_step_buffers;
 
 typedef hipError_t hip_update_neurons_t(hipStream_t stream,

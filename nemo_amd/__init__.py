@@ -307,7 +307,8 @@ class Simulation(_SynapseQueries):
 
     @property
     def launch_shape(self):
-        """(delivery blocks, bin capacity, STDP fused, STDP bitmap words)."""
+        """(delivery blocks, bin capacity, STDP inside the updates (1) or in its
+        own launch (0), bitmap words per block of that launch)."""
         a = (ctypes.c_uint * 4)()
         check(lib().nemo_hip_get_launch_shape(self._h, a))
         return tuple(a)

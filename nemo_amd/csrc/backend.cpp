@@ -452,6 +452,9 @@ void Simulation::construct(const std::vector<const Network*>& nets, const Config
 	m_logging = conf.logging;
 	activate();
 	HIP_CHECK(hipStreamCreateWithFlags(&m_stream, hipStreamNonBlocking));
+	HIP_CHECK(hipStreamCreateWithFlags(&m_stdpStream, hipStreamNonBlocking));
+	HIP_CHECK(hipEventCreateWithFlags(&m_stdpFork, hipEventDisableTiming));
+	HIP_CHECK(hipEventCreateWithFlags(&m_stdpDone, hipEventDisableTiming));
 	HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&m_pinnedScalars), SCALAR_SLOTS * sizeof(uint64_t),
 				hipHostMallocDefault));
 
@@ -470,7 +473,7 @@ void Simulation::construct(const std::vector<const Network*>& nets, const Config
 		std::printf("nemo-hip: %u network(s), %zu neurons (%u tiles), %zu synapses (%zu plastic), max delay %u, "
 				"Q%u.%u, delivery %u blocks x %u events, STDP %u blocks (%s)\n",
 				m_nets, m_ncount, m_tiles, m_nsynapses, m_nplastic, m_D, 31 - m_fbits, m_fbits,
-				m_shape.deliverBlocks, m_shape.binCapacity, m_stdpBlocks, m_shape.fuseStdp ? "fused" : "own launch");
+				m_shape.deliverBlocks, m_shape.binCapacity, m_stdpBlocks, m_shape.fuseStdp ? "in the updates" : "own launch");
 	}
 	resetTimer();
 }
@@ -478,10 +481,14 @@ void Simulation::construct(const std::vector<const Network*>& nets, const Config
 Simulation::~Simulation()
 {
 	(void) hipSetDevice(m_device);
+	if(m_stream) (void) hipStreamSynchronize(m_stream);
+	if(m_stdpStream) (void) hipStreamSynchronize(m_stdpStream);
 	clearGraphs();
-	if(m_stream) {
-		(void) hipStreamSynchronize(m_stream);
-		(void) hipStreamDestroy(m_stream);
+	if(m_stream) (void) hipStreamDestroy(m_stream);
+	if(m_stdpStream) {
+		(void) hipStreamDestroy(m_stdpStream);
+		(void) hipEventDestroy(m_stdpFork);
+		(void) hipEventDestroy(m_stdpDone);
 	}
 	if(m_pinnedScalars) (void) hipHostFree(m_pinnedScalars);
 	if(m_pinnedBits) (void) hipHostFree(m_pinnedBits);
@@ -848,14 +855,16 @@ void Simulation::buildConnectivity(const std::vector<const Network*>& nets, cons
 void Simulation::buildStdp(const Configuration& conf)
 {
 	std::memset(&m_propagate, 0, sizeof(m_propagate));
+	std::memset(&m_stdpArgs, 0, sizeof(m_stdpArgs));
 	if(!m_stdp) return;
 	const unsigned fbits = m_fbits;
 	m_minE = fxToFix(conf.minE, fbits);
 	m_maxE = fxToFix(conf.maxE, fbits);
 	m_minI = fxToFix(conf.minI, fbits);
 	m_maxI = fxToFix(conf.maxI, fbits);
-	for(size_t i = 0; i < conf.prefire.size(); ++i) m_propagate.fnPre[i] = fxToFix(conf.prefire[i], fbits);
-	for(size_t i = 0; i < conf.postfire.size(); ++i) m_propagate.fnPost[i] = fxToFix(conf.postfire[i], fbits);
+	nemo_hip_stdp& st = m_stdpArgs;
+	for(size_t i = 0; i < conf.prefire.size(); ++i) st.fn_pre[i] = fxToFix(conf.prefire[i], fbits);
+	for(size_t i = 0; i < conf.postfire.size(); ++i) st.fn_post[i] = fxToFix(conf.postfire[i], fbits);
 	uint64_t pot = 0, dep = 0;
 	unsigned n = 0;
 	for(auto f = conf.postfire.rbegin(); f != conf.postfire.rend(); ++f, ++n) {
@@ -869,11 +878,11 @@ void Simulation::buildStdp(const Configuration& conf)
 	const unsigned pre = unsigned(conf.prefire.size());
 	const unsigned post = unsigned(conf.postfire.size());
 	auto lowMask = [](unsigned k) { return k >= 64 ? ~uint64_t(0) : ((uint64_t(1) << k) - 1); };
-	m_propagate.postPreWindow = post;
-	m_propagate.potentiationBits = pot;
-	m_propagate.depressionBits = dep;
-	m_propagate.prePostBits = post >= 64 ? 0 : (lowMask(pre) << post);
-	m_propagate.postPreBits = lowMask(post);
+	st.post_window = post;
+	st.potentiation = pot;
+	st.depression = dep;
+	st.pre_post = post >= 64 ? 0 : (lowMask(pre) << post);
+	st.post_pre = lowMask(post);
 }
 
 /* Split the (source tile, delay) pairs over B delivery blocks with about
@@ -936,7 +945,7 @@ static Partition partitionPairs(const std::vector<uint64_t>& weight, const std::
 
 /* Default shape: bin capacity 4096 events (halved while the LDS does not
  * allow it), enough delivery blocks for about 1% of the synapses per step at
- * half capacity, STDP fused into the delivery launch, two bitmap words per
+ * half capacity, STDP accumulation inside the update kernels, two bitmap words per
  * STDP block.  NEMO_HIP_DELIVER_BLOCKS, NEMO_HIP_BIN_CAPACITY,
  * NEMO_HIP_FUSE_STDP and NEMO_HIP_STDP_WORDS override it (tuning). */
 LaunchShape Simulation::defaultShape() const
@@ -1028,7 +1037,9 @@ void Simulation::applyShape(const LaunchShape& s)
 	a.acc_pitch = m_accPitch;
 	a.block_events = m_blockEvents.ptr;
 	a.stdp_words = m_stdpBlocks ? s.stdpWords : 0;
-	a.block_examined = m_blockExamined.ptr;
+	a.hist = m_hist.ptr;
+	a.hist_pitch = m_accPitch;
+	a.stdp = m_stdpArgs;
 }
 
 void Simulation::setLaunchShape(const LaunchShape& s)
@@ -1056,7 +1067,7 @@ void Simulation::allocateStepState()
 	m_accPitch = N;
 	m_acc.allocate(2 * m_accPitch);
 	HIP_CHECK(hipMemset(m_acc.ptr, 0, m_acc.bytes()));
-	m_hist.allocate(N);
+	m_hist.allocate(2 * N); /* plane s % 2 holds the history after step s */
 	HIP_CHECK(hipMemset(m_hist.ptr, 0, m_hist.bytes()));
 	m_ext.allocate(N);
 	HIP_CHECK(hipMemset(m_ext.ptr, 0, m_ext.bytes()));
@@ -1073,10 +1084,14 @@ void Simulation::allocateStepState()
 	m_blockExamined.allocate(std::max<size_t>(1, m_firedWords));
 	HIP_CHECK(hipMemset(m_blockExamined.ptr, 0, m_blockExamined.bytes()));
 
-	m_propagate.rcm_ptr = m_rcmPtr.ptr;
-	m_propagate.rcm_sd = m_rcmSd.ptr;
-	m_propagate.rcm_acc = m_rcmAcc.ptr;
-	m_propagate.hist = m_hist.ptr;
+	if(m_stdp && m_nplastic) {
+		m_stdpArgs.rcm_ptr = m_rcmPtr.ptr;
+		m_stdpArgs.rcm_sd = m_rcmSd.ptr;
+		m_stdpArgs.rcm_acc = m_rcmAcc.ptr;
+		m_stdpArgs.examined = m_blockExamined.ptr;
+	} else {
+		m_stdpArgs.rcm_ptr = nullptr;
+	}
 }
 
 nemo_hip_step_buffers Simulation::stepBuffers(uint64_t offset, bool ext, bool fstim) const
@@ -1094,10 +1109,13 @@ nemo_hip_step_buffers Simulation::stepBuffers(uint64_t offset, bool ext, bool fs
 	b.ext_current = ext ? m_ext.ptr : nullptr;
 	b.fstim = fstim ? m_fstim.ptr : nullptr;
 	b.hist = m_hist.ptr;
+	b.hist_pitch = m_accPitch;
 	b.fired_bits = m_firedBits.ptr;
 	b.fired_words = m_firedWords;
 	b.fbits = m_fbits;
 	b.debug_current = m_debug ? m_debugCurrent.ptr : nullptr;
+	b.stdp = m_stdpArgs;
+	if(!m_shape.fuseStdp) b.stdp.rcm_ptr = nullptr; /* STDP runs in its own launch */
 	return b;
 }
 
@@ -1126,10 +1144,24 @@ void Simulation::setBase(uint64_t base)
  * launches. */
 void Simulation::enqueueStep(uint64_t offset, bool ext, bool fstim, hipEvent_t* ev)
 {
+	if(!m_shape.fuseStdp && m_stdpBlocks) {
+		/* STDP of the previous step on a parallel branch: it needs the history
+		 * its update left (plane (t-1) % 2), which the update of step t+1
+		 * overwrites, so it joins back before this step's successor. */
+		joinStdp();
+		HIP_CHECK(hipEventRecord(m_stdpFork, m_stream));
+		HIP_CHECK(hipStreamWaitEvent(m_stdpStream, m_stdpFork, 0));
+		nemo_propagate_args sa = m_propagate;
+		sa.step_offset = offset;
+		sa.deliver_blocks = 0;
+		HIP_CHECK(nemo_launch_propagate(m_stdpStream, &sa, m_stdpBlocks));
+		HIP_CHECK(hipEventRecord(m_stdpDone, m_stdpStream));
+		m_stdpPending = true;
+	}
 	if(ev) HIP_CHECK(hipEventRecord(ev[0], m_stream));
 	nemo_propagate_args a = m_propagate;
 	a.step_offset = offset;
-	HIP_CHECK(nemo_launch_propagate(m_stream, &a, m_stdpBlocks, int(m_shape.fuseStdp)));
+	HIP_CHECK(nemo_launch_propagate(m_stream, &a, 0));
 	if(ev) HIP_CHECK(hipEventRecord(ev[1], m_stream));
 	nemo_hip_step_buffers b = stepBuffers(offset, ext, fstim);
 	for(Group& g : m_groups) {
@@ -1138,17 +1170,26 @@ void Simulation::enqueueStep(uint64_t offset, bool ext, bool fstim, hipEvent_t* 
 	if(ev) HIP_CHECK(hipEventRecord(ev[2], m_stream));
 }
 
+/* Make the main stream wait for the STDP branch of the last enqueued step. */
+void Simulation::joinStdp()
+{
+	if(!m_stdpPending) return;
+	HIP_CHECK(hipStreamWaitEvent(m_stream, m_stdpDone, 0));
+	m_stdpPending = false;
+}
+
 /* Run the pending STDP accumulation of the last step on its own, and tell the
  * next propagate launch to skip it. */
 void Simulation::flushStdp()
 {
+	joinStdp();
 	if(!m_stdpBlocks || m_hostStep == 0 || m_flushed == m_hostStep - 1) return;
 	const uint64_t s = m_hostStep - 1;
 	nemo_propagate_args a = m_propagate;
 	a.deliver_blocks = 0;
 	a.step_offset = 0;
 	setBase(s + 1);
-	HIP_CHECK(nemo_launch_propagate(m_stream, &a, m_stdpBlocks, 1));
+	HIP_CHECK(nemo_launch_propagate(m_stream, &a, m_stdpBlocks));
 	copyScalar(&m_clock.ptr->stdp_flushed, s);
 	m_flushed = s;
 	HIP_CHECK(hipStreamSynchronize(m_stream));
@@ -1256,6 +1297,7 @@ const std::vector<unsigned>& Simulation::step(const unsigned* fstim, size_t nf,
 	const uint64_t t = m_hostStep;
 	setBase(t);
 	enqueueStep(0, !iIdx.empty(), !fIdx.empty());
+	joinStdp();
 	m_hostStep += 1;
 	if(!fIdx.empty()) {
 		HIP_CHECK(nemo_launch_fstim(m_stream, m_fstim.ptr, m_stimIdx.ptr, fIdx.size(), 0));
@@ -1277,8 +1319,10 @@ hipGraphExec_t Simulation::graphFor(unsigned k)
 	m_debug = false;
 	try {
 		for(unsigned i = 0; i < k; ++i) enqueueStep(i, false, false);
+		joinStdp();
 	} catch(...) {
 		m_debug = debug;
+		m_stdpPending = false;
 		(void) hipStreamEndCapture(m_stream, &graph);
 		throw;
 	}
@@ -1397,6 +1441,7 @@ void Simulation::profile(unsigned nsteps, double avgUs[4], uint64_t counters[4])
 	const uint64_t t0 = m_hostStep;
 	setBase(t0);
 	for(unsigned s = 0; s < nsteps; ++s) enqueueStep(s, false, false, &ev[3 * size_t(s)]);
+	joinStdp();
 	m_hostStep += nsteps;
 	HIP_CHECK(hipStreamSynchronize(m_stream));
 	double sum[3] = {0, 0, 0};
@@ -1580,7 +1625,7 @@ LaunchShape Simulation::autotune(unsigned stepsPerTrial)
 	restoreSnapshot(snap);
 	if(m_logging) {
 		std::printf("nemo-hip: autotuned launch shape: %u delivery blocks x %u events, STDP %s, %u words "
-				"(%.1f us/step)\n", best.deliverBlocks, best.binCapacity, best.fuseStdp ? "fused" : "own launch",
+				"(%.1f us/step)\n", best.deliverBlocks, best.binCapacity, best.fuseStdp ? "in the updates" : "own launch",
 				best.stdpWords, 1e6 * tBest / stepsPerTrial);
 	}
 	return best;
@@ -1804,7 +1849,8 @@ void Simulation::getRecentFiring(uint64_t* h, size_t n)
 	activate();
 	HIP_CHECK(hipStreamSynchronize(m_stream));
 	std::vector<uint64_t> d(m_npad);
-	HIP_CHECK(hipMemcpy(d.data(), m_hist.ptr, m_npad * 8, hipMemcpyDeviceToHost));
+	const size_t plane = size_t((m_hostStep + 1) & 1); /* after the last step */
+	HIP_CHECK(hipMemcpy(d.data(), m_hist.ptr + plane * m_accPitch, m_npad * 8, hipMemcpyDeviceToHost));
 	for(size_t i = 0; i < n; ++i) h[i] = d[m_order[i]];
 }
 

@@ -150,7 +150,7 @@ struct DeviceArray {
 struct LaunchShape {
 	unsigned deliverBlocks = 1;   /* delivery blocks (bin regions), <= NEMO_MAX_BIN_BLOCKS */
 	unsigned binCapacity = 4096;  /* events a delivery block stages in LDS */
-	unsigned fuseStdp = 1;        /* STDP blocks in the delivery launch (1) or their own (0) */
+	unsigned fuseStdp = 1;        /* STDP accumulation in the update kernels (1) or its own launch (0) */
 	unsigned stdpWords = 2;       /* firing-bitmap words (64 targets each) per STDP block */
 };
 
@@ -232,6 +232,9 @@ private:
 
 	int m_device = 0;
 	hipStream_t m_stream = nullptr;
+	hipStream_t m_stdpStream = nullptr;  /* parallel branch for STDP in its own launch */
+	hipEvent_t m_stdpFork = nullptr, m_stdpDone = nullptr;
+	bool m_stdpPending = false;          /* the STDP branch has not been joined yet */
 	unsigned m_nets = 1;   /* networks in the batch */
 	size_t m_ncount = 0;   /* neurons */
 	size_t m_npad = 0;     /* device index space: each type padded to whole tiles */
@@ -285,6 +288,7 @@ private:
 
 	/* propagate-kernel arguments and launch shape */
 	nemo_propagate_args m_propagate;
+	nemo_hip_stdp m_stdpArgs;            /* STDP accumulation arguments (rcm_ptr NULL: none) */
 	LaunchShape m_shape;
 	unsigned m_stdpBlocks = 0;
 	size_t m_ldsBudget = 0;
@@ -328,6 +332,7 @@ private:
 	void setBase(uint64_t base);
 	void enqueueStep(uint64_t offset, bool ext, bool fstim, hipEvent_t* ev = nullptr);
 	void flushStdp();
+	void joinStdp();
 	void decodeFiring(const uint64_t* bits, std::vector<unsigned>& out) const;
 	void decodeFiringBatch(const uint64_t* bits, std::vector<unsigned>& out, std::vector<unsigned>& counts) const;
 	void readBitmaps(uint64_t t0, unsigned k, uint64_t* dst = nullptr);

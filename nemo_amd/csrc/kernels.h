@@ -21,29 +21,27 @@ extern "C" {
 /* Threads per block of the backend's kernels. */
 #define NEMO_BLOCK 256
 
-/* Per step, before the neuron updates, two independent jobs:
+/* Per step, before the neuron updates:
  *
- * Spike delivery of step t, binned by target tile (the update kernels gather
- * the bins).  For every delay d in 1..D and every neuron n that fired at step
- * t-d, the synapse row (n, d) is delivered: each event {target, w} is
- * counting-sorted in LDS by target tile and written to the block's bin
- * region; bin_table[tile][block] = count << 16 | offset.  Events beyond the
- * block's bin capacity are added with global atomics into acc (E if w >= 0,
- * else I).  Delivery block b covers the (source tile, delay) pairs
- * q in [pair_bounds[b], pair_bounds[b+1]), q = tile*D + d-1, and every target
- * of those rows lies in its tile window [windows[b].x, windows[b].y); it
- * writes the table entries of exactly the tiles of its window.  Semantics of
- * cpu::Simulation::deliverSpikes (cpu/Simulation.cpp:259-308).
+ * Spike delivery of step t (propagate_kernel), binned by target tile (the
+ * update kernels gather the bins).  For every delay d in 1..D and every
+ * neuron n that fired at step t-d, the synapse row (n, d) is delivered: each
+ * event {target, w} is counting-sorted in LDS by target tile and written to
+ * the block's bin region; bin_table[tile][block] = count << 16 | offset.
+ * Events beyond the block's bin capacity are added with global atomics into
+ * acc (E if w >= 0, else I).  Delivery block b covers the (source tile,
+ * delay) pairs q in [pair_bounds[b], pair_bounds[b+1]), q = tile*D + d-1, and
+ * every target of those rows lies in its tile window [windows[b].x,
+ * windows[b].y); it writes the table entries of exactly the tiles of its
+ * window.  Semantics of cpu::Simulation::deliverSpikes (cpu/Simulation.cpp:
+ * 259-308).
  *
- * STDP accumulation of step t-1, for the neurons that fired at step t-1-P,
- * P = |postfire| (ConnectivityMatrix::accumulateStdp, ConnectivityMatrix.cpp:
- * 224-258, StdpProcess::weightChange, StdpProcess.cpp:68-124).  It reads the
- * firing history as left by the update of step t-1, which is why it may run
- * beside delivery of step t.  Skipped when t == 0 or when step t-1's
- * accumulation was already run on its own (clock->stdp_flushed == t-1).
- *
- * With fuse_stdp the STDP blocks follow the delivery blocks in one launch;
- * otherwise STDP is a launch of its own, before delivery. */
+ * Optionally, before delivery, STDP accumulation of step t-1 in a launch of
+ * its own (stdp_kernel), for the neurons that fired at step t-1-P (the
+ * contract of nemo_hip_stdp); usually the update kernels do it instead.  It
+ * reads the firing history after step t-1, plane (t-1) % 2 of hist.  Skipped
+ * when t == 0 or when step t-1's accumulation already ran
+ * (clock->stdp_flushed == t-1). */
 typedef struct nemo_propagate_args {
 	const nemo_hip_clock* clock;
 	uint64_t step_offset;            /* t = clock->base_step + step_offset */
@@ -64,24 +62,18 @@ typedef struct nemo_propagate_args {
 	int64_t* acc;                    /* overflow accumulators, E plane then I plane */
 	size_t acc_pitch;
 	unsigned long long* block_events; /* [deliver_blocks] events delivered, accumulated */
-	/* STDP */
-	unsigned stdp_words;             /* bitmap words per STDP block (1..4); 0: no STDP */
-	const uint32_t* rcm_ptr;         /* neurons+1 offsets of each target's plastic incoming synapses */
-	const uint32_t* rcm_sd;          /* source device index | (delay-1) << 26 */
-	int32_t* rcm_acc;                /* accumulated weight change */
-	const uint64_t* hist;
-	unsigned postPreWindow;          /* P */
-	uint64_t potentiationBits, depressionBits, prePostBits, postPreBits;
-	int32_t fnPre[64];
-	int32_t fnPost[64];
-	unsigned long long* block_examined; /* [STDP blocks] plastic synapses examined, accumulated */
+	/* STDP in its own launch */
+	unsigned stdp_words;             /* bitmap words per STDP block (1..4) */
+	const uint64_t* hist;            /* two planes of hist_pitch */
+	size_t hist_pitch;
+	nemo_hip_stdp stdp;              /* examined: one counter per STDP block */
 } nemo_propagate_args;
 
 /* dynamic LDS bytes a delivery block needs */
 size_t nemo_propagate_lds_bytes(unsigned max_window, unsigned bin_capacity);
-/* Delivery (a->deliver_blocks blocks, none if 0) and STDP accumulation
- * (stdp_blocks blocks, none if 0), fused into one launch or not. */
-hipError_t nemo_launch_propagate(hipStream_t, const nemo_propagate_args*, unsigned stdp_blocks, int fuse_stdp);
+/* STDP accumulation (stdp_blocks blocks; none if 0), then delivery
+ * (a->deliver_blocks blocks; none if 0). */
+hipError_t nemo_launch_propagate(hipStream_t, const nemo_propagate_args*, unsigned stdp_blocks);
 
 /* applyStdp (ConnectivityMatrix::applyStdp, ConnectivityMatrix.cpp:272-325 and
  * StdpProcess::updatedWeight, StdpProcess.cpp:128-138). */

@@ -1,8 +1,8 @@
 /*
  * The backend's own gfx950 kernels: the per-step propagate kernel (binned
- * spike delivery plus STDP accumulation), applyStdp and stimulus helpers.
- * Neuron updates, which also gather the binned input, live in the model
- * plugins (plugins/*.hip).
+ * spike delivery), the STDP-only kernel, applyStdp and stimulus helpers.
+ * Neuron updates, which also gather the binned input and usually run the
+ * STDP accumulation, live in the model plugins (plugins/<Model>.hip).
  *
  * Data layout in HBM.  Device indices number neurons type by type, in
  * insertion order within a type (NeMo's simulation order,
@@ -13,7 +13,8 @@
  *                          {target, Q11.20 weight}, 8 bytes, so consecutive
  *                          lanes read consecutive 8-byte words of a row
  *   fired_bits uint64[128][N/64]  one firing bitmap per step (ring)
- *   hist       uint64[N]   firing history (bit k: fired k steps ago), STDP
+ *   hist       uint64[2][N] firing history (bit k: fired k steps ago); plane
+ *                          s % 2 holds the history after step s
  *   bins       int2[B][CAP] each delivery block's events, sorted by tile
  *   bin_table  uint32[T][B] count << 16 | offset of tile t in block b's bins
  *                          (entries of the blocks whose window holds t)
@@ -198,44 +199,26 @@ __device__ void deliver_block(const nemo_propagate_args& a, uint64_t t, int2* st
 
 /* ---------------------------------------------------------------- STDP */
 
-__device__ inline int32_t wrap_add(int32_t x, int32_t y)
-{
-	return int32_t(uint32_t(x) + uint32_t(y));
-}
+using nemo_dev::stdp_wrap_add;
 
-__device__ inline int32_t update_region(uint64_t arrivals, const nemo_propagate_args& a,
-		const int32_t* fnPre, const int32_t* fnPost)
+/* An STDP block of the STDP-only launch: the targets among its bitmap words
+ * that fired P steps before step r, and all their plastic incoming synapses
+ * spread evenly over its threads (ILP loads in flight per thread).  The
+ * update kernels do the same per tile when STDP runs inside them. */
+__device__ void stdp_block(const nemo_propagate_args& a, uint64_t r, unsigned sb)
 {
-	/* StdpProcess::updateRegion, StdpProcess.cpp:86-111: the nearest arrival
-	 * on either side of the postsynaptic spike wins; a tie gives nothing. */
-	if(!arrivals) return 0;
-	const unsigned NONE = ~0u;
-	const uint64_t pre = arrivals & a.prePostBits;
-	const uint64_t post = arrivals & a.postPreBits;
-	const unsigned dt_pre = pre ? unsigned(__builtin_ctzll(pre >> a.postPreWindow)) : NONE;
-	const unsigned dt_post = post ? unsigned(__builtin_clzll(post << (64 - a.postPreWindow))) : NONE;
-	if(dt_pre < dt_post) return fnPre[dt_pre];
-	if(dt_post < dt_pre) return fnPost[dt_post];
-	return 0;
-}
-
-/* An STDP block: the targets among its bitmap words that fired P steps before
- * step s, and all their plastic incoming synapses spread evenly over its
- * threads (four in flight per thread). */
-__device__ void stdp_block(const nemo_propagate_args& a, uint64_t s, unsigned sb)
-{
-	__shared__ int32_t s_pre[64];
-	__shared__ int32_t s_post[64];
+	const nemo_hip_stdp& st = a.stdp;
+	__shared__ int32_t s_fn[128];
 	__shared__ uint32_t s_rb[NEMO_BLOCK];
 	__shared__ uint32_t s_pos[NEMO_BLOCK + 1];
-	const unsigned P = a.postPreWindow;
+	const unsigned P = st.post_window;
 	const unsigned tid = threadIdx.x;
 	if(tid < 64) {
-		s_pre[tid] = a.fnPre[tid];
-		s_post[tid] = a.fnPost[tid];
+		s_fn[tid] = st.fn_pre[tid];
+		s_fn[64 + tid] = st.fn_post[tid];
 	}
 
-	const uint64_t slot = (s - P) & RING_MASK;
+	const uint64_t slot = (r - P) & RING_MASK;
 	const size_t w0 = size_t(sb) * a.stdp_words;
 	const size_t word = w0 + tid / 64;
 	bool fired = false;
@@ -247,8 +230,8 @@ __device__ void stdp_block(const nemo_propagate_args& a, uint64_t s, unsigned sb
 	uint32_t len = 0, rb = 0;
 	if(fired) {
 		const uint32_t n = uint32_t(w0 * 64 + tid);
-		rb = a.rcm_ptr[n];
-		len = a.rcm_ptr[n + 1] - rb;
+		rb = st.rcm_ptr[n];
+		len = st.rcm_ptr[n + 1] - rb;
 	}
 	uint32_t total;
 	const uint32_t pos = nemo_dev::block_exclusive_scan(len, &total);
@@ -260,6 +243,7 @@ __device__ void stdp_block(const nemo_propagate_args& a, uint64_t s, unsigned sb
 	__syncthreads();
 	if(total == 0) return;
 
+	const uint64_t* hist = a.hist + (r & 1) * a.hist_pitch;
 	for(uint32_t e0 = tid; e0 < total; e0 += ILP * NEMO_BLOCK) {
 		uint32_t k[ILP];
 		uint32_t sd[ILP];
@@ -277,61 +261,54 @@ __device__ void stdp_block(const nemo_propagate_args& a, uint64_t s, unsigned sb
 					if(s_pos[mid] <= e) lo = mid; else hi = mid;
 				}
 				k[u] = s_rb[lo] + (e - s_pos[lo]);
-				sd[u] = a.rcm_sd[k[u]];
-				acc[u] = a.rcm_acc[k[u]];
+				sd[u] = st.rcm_sd[k[u]];
+				acc[u] = st.rcm_acc[k[u]];
 			}
 		}
 		uint64_t h[ILP];
 #pragma unroll
 		for(int u = 0; u < ILP; ++u) {
 			h[u] = 0;
-			if(e0 + u * NEMO_BLOCK < total) h[u] = a.hist[sd[u] & 0x3ffffffu];
+			if(e0 + u * NEMO_BLOCK < total) h[u] = hist[sd[u] & 0x3ffffffu];
 		}
 #pragma unroll
 		for(int u = 0; u < ILP; ++u) {
 			if(e0 + u * NEMO_BLOCK < total) {
 				const unsigned d = (sd[u] >> 26) + 1;
 				const uint64_t preFiring = d >= 64 ? 0 : (h[u] >> d);
-				const int32_t dw = wrap_add(update_region(preFiring & a.potentiationBits, a, s_pre, s_post),
-						update_region(preFiring & a.depressionBits, a, s_pre, s_post));
+				const int32_t dw = stdp_wrap_add(
+						nemo_dev::stdp_update_region(preFiring & st.potentiation, P, st.pre_post, st.post_pre,
+							s_fn, s_fn + 64),
+						nemo_dev::stdp_update_region(preFiring & st.depression, P, st.pre_post, st.post_pre,
+							s_fn, s_fn + 64));
 				if(dw != 0) {
-					a.rcm_acc[k[u]] = wrap_add(acc[u], dw);
+					st.rcm_acc[k[u]] = stdp_wrap_add(acc[u], dw);
 				}
 			}
 		}
 	}
-	if(tid == 0) a.block_examined[sb] += total;
+	if(tid == 0) st.examined[sb] += total;
 }
 
-/* STDP accumulation of step t-1 by STDP block sb, unless it is not due */
-__device__ inline void stdp_step(const nemo_propagate_args& a, uint64_t t, unsigned sb)
-{
-	if(t == 0) return;
-	const uint64_t s = t - 1;
-	if(s == a.clock->stdp_flushed || s < a.postPreWindow) return;
-	stdp_block(a, s, sb);
-}
-
-/* delivery blocks, then (fused) STDP blocks */
 __global__ void __launch_bounds__(NEMO_BLOCK)
 propagate_kernel(nemo_propagate_args a)
 {
 	extern __shared__ int2 lds_dyn[];
 	const uint64_t t = a.clock->base_step + a.step_offset;
-	if(blockIdx.x < a.deliver_blocks) {
-		int2* stage = lds_dyn;
-		uint32_t* cnt = reinterpret_cast<uint32_t*>(lds_dyn + a.bin_capacity);
-		deliver_block(a, t, stage, cnt, cnt + a.max_window);
-	} else {
-		stdp_step(a, t, blockIdx.x - a.deliver_blocks);
-	}
+	int2* stage = lds_dyn;
+	uint32_t* cnt = reinterpret_cast<uint32_t*>(lds_dyn + a.bin_capacity);
+	deliver_block(a, t, stage, cnt, cnt + a.max_window);
 }
 
-/* STDP blocks only */
+/* STDP accumulation of step t-1, unless it is not due */
 __global__ void __launch_bounds__(NEMO_BLOCK)
 stdp_kernel(nemo_propagate_args a)
 {
-	stdp_step(a, a.clock->base_step + a.step_offset, blockIdx.x);
+	const uint64_t t = a.clock->base_step + a.step_offset;
+	if(t == 0) return;
+	const uint64_t r = t - 1;
+	if(r == a.clock->stdp_flushed || r < a.stdp.post_window) return;
+	stdp_block(a, r, blockIdx.x);
 }
 
 __global__ void __launch_bounds__(NEMO_BLOCK)
@@ -345,10 +322,10 @@ apply_stdp_kernel(nemo_apply_args a)
 		const int32_t dw = int32_t((int64_t(a.reward) * int64_t(acc)) >> a.fbits);
 		int32_t w_new = 0;
 		if(w_old > 0) {
-			const int32_t x = wrap_add(w_old, dw);
+			const int32_t x = stdp_wrap_add(w_old, dw);
 			w_new = min(a.maxE, max(x, a.minE));
 		} else if(w_old < 0) {
-			const int32_t x = wrap_add(w_old, dw);
+			const int32_t x = stdp_wrap_add(w_old, dw);
 			w_new = min(a.minI, max(x, a.maxI));
 		}
 		if(w_new != w_old) *wp = w_new;
@@ -401,7 +378,7 @@ size_t nemo_propagate_lds_bytes(unsigned max_window, unsigned bin_capacity)
 	return size_t(bin_capacity) * sizeof(int2) + 2 * size_t(max_window) * sizeof(uint32_t);
 }
 
-hipError_t nemo_launch_propagate(hipStream_t st, const nemo_propagate_args* a, unsigned stdp_blocks, int fuse_stdp)
+hipError_t nemo_launch_propagate(hipStream_t st, const nemo_propagate_args* a, unsigned stdp_blocks)
 {
 	if(a->deliver_blocks > NEMO_MAX_BIN_BLOCKS) return hipErrorInvalidValue;
 	if(a->deliver_blocks && (a->D == 0 || a->D > NEMO_MAX_DELAY || a->bin_capacity == 0
@@ -409,16 +386,9 @@ hipError_t nemo_launch_propagate(hipStream_t st, const nemo_propagate_args* a, u
 				|| !a->pair_bounds || !a->windows)) {
 		return hipErrorInvalidValue;
 	}
-	if(stdp_blocks && (a->postPreWindow > 63 || a->stdp_words == 0 || a->stdp_words > 4
-				|| size_t(stdp_blocks) * a->stdp_words < a->fired_words)) {
+	if(stdp_blocks && (a->stdp.post_window > 63 || a->stdp_words == 0 || a->stdp_words > 4
+				|| size_t(stdp_blocks) * a->stdp_words < a->fired_words || !a->stdp.rcm_ptr || !a->hist)) {
 		return hipErrorInvalidValue;
-	}
-	const size_t lds = a->deliver_blocks ? nemo_propagate_lds_bytes(a->max_window, a->bin_capacity) : 0;
-	if(fuse_stdp) {
-		const unsigned blocks = a->deliver_blocks + stdp_blocks;
-		if(blocks == 0) return hipSuccess;
-		hipLaunchKernelGGL(propagate_kernel, dim3(blocks), dim3(NEMO_BLOCK), lds, st, *a);
-		return hipGetLastError();
 	}
 	if(stdp_blocks) {
 		hipLaunchKernelGGL(stdp_kernel, dim3(stdp_blocks), dim3(NEMO_BLOCK), 0, st, *a);
@@ -426,6 +396,7 @@ hipError_t nemo_launch_propagate(hipStream_t st, const nemo_propagate_args* a, u
 		if(e != hipSuccess) return e;
 	}
 	if(a->deliver_blocks) {
+		const size_t lds = nemo_propagate_lds_bytes(a->max_window, a->bin_capacity);
 		hipLaunchKernelGGL(propagate_kernel, dim3(a->deliver_blocks), dim3(NEMO_BLOCK), lds, st, *a);
 		return hipGetLastError();
 	}

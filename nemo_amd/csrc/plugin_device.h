@@ -3,9 +3,11 @@
  *
  * An update kernel runs one block of NEMO_TILE threads per tile of its neuron
  * type, one neuron per thread, and every thread of a block calls
+ *   h = load_history(b, ng, valid);
  *   TileInput in; gather_input(b, g, in);       (block barriers inside)
  *   take_current(b, in, ng, E, I);              (valid threads)
- *   record_firing(b, ng, valid, fired);         (every thread)
+ *   record_firing(b, ng, valid, fired, h);      (every thread)
+ *   stdp_tile(b, ng, valid, h);                 (every thread; barriers inside)
  */
 #ifndef NEMO_AMD_PLUGIN_DEVICE_H
 #define NEMO_AMD_PLUGIN_DEVICE_H
@@ -200,27 +202,133 @@ __device__ inline void store_firing(const nemo_hip_step_buffers& b, unsigned ng,
 	}
 }
 
-/* Shift the firing history (whose old value h the caller loaded) and write
- * the step's firing bitmap. */
+/* The firing history after step s lives in plane s % 2. */
+__device__ inline const uint64_t* history_after(const nemo_hip_step_buffers& b, uint64_t s)
+{
+	return b.hist + (s & 1) * b.hist_pitch;
+}
+
+/* History of neuron ng before this step (after the previous one). */
+__device__ inline uint64_t load_history(const nemo_hip_step_buffers& b, unsigned ng, bool valid)
+{
+	return valid ? history_after(b, current_step(b) - 1)[ng] : 0;
+}
+
+/* Store the shifted history (h as from load_history) and write the step's
+ * firing bitmap. */
 __device__ inline void record_firing(const nemo_hip_step_buffers& b, unsigned ng, bool valid, bool fired, uint64_t h)
 {
-	if(valid) b.hist[ng] = (h << 1) | uint64_t(fired ? 1 : 0);
+	if(valid) {
+		uint64_t* next = b.hist + (current_step(b) & 1) * b.hist_pitch;
+		next[ng] = (h << 1) | uint64_t(fired ? 1 : 0);
+	}
 	store_firing(b, ng, valid, fired);
 }
 
-/* Shift the firing history and write the tile's words of the step's firing
- * bitmap. */
-__device__ inline void record_firing(const nemo_hip_step_buffers& b, unsigned ng, bool valid, bool fired)
+/* StdpProcess::updateRegion (StdpProcess.cpp:86-111): within one region of
+ * arrivals, the nearest presynaptic arrival on either side of the
+ * postsynaptic spike wins; a tie gives nothing. */
+__device__ inline int32_t stdp_update_region(uint64_t arrivals, unsigned post_window, uint64_t pre_post,
+		uint64_t post_pre, const int32_t* fn_pre, const int32_t* fn_post)
 {
-	if(valid) {
-		const uint64_t h = b.hist[ng];
-		b.hist[ng] = (h << 1) | uint64_t(fired ? 1 : 0);
+	if(!arrivals) return 0;
+	const unsigned NONE = ~0u;
+	const uint64_t pre = arrivals & pre_post;
+	const uint64_t post = arrivals & post_pre;
+	const unsigned dt_pre = pre ? unsigned(__builtin_ctzll(pre >> post_window)) : NONE;
+	const unsigned dt_post = post ? unsigned(__builtin_clzll(post << (64 - post_window))) : NONE;
+	if(dt_pre < dt_post) return fn_pre[dt_pre];
+	if(dt_post < dt_pre) return fn_post[dt_post];
+	return 0;
+}
+
+__device__ inline int32_t stdp_wrap_add(int32_t x, int32_t y)
+{
+	return int32_t(uint32_t(x) + uint32_t(y));
+}
+
+/* The STDP accumulation of the previous step for this tile's neurons (the
+ * contract of nemo_hip_stdp).  h is the thread's neuron history after the
+ * previous step (load_history).  Every thread of the block calls it. */
+__device__ inline void stdp_tile(const nemo_hip_step_buffers& b, unsigned ng, bool valid, uint64_t h)
+{
+	constexpr int ILP = 8;
+	const nemo_hip_stdp& st = b.stdp;
+	if(!st.rcm_ptr) return;
+	const uint64_t t = current_step(b);
+	if(t == 0) return;
+	const uint64_t r = t - 1;
+	if(r < st.post_window || r == b.clock->stdp_flushed) return;
+
+	__shared__ int32_t s_fn[128];
+	__shared__ uint32_t s_rb[NEMO_TILE];
+	__shared__ uint32_t s_pos[NEMO_TILE + 1];
+	const unsigned tid = threadIdx.x;
+	if(tid < 64) {
+		s_fn[tid] = st.fn_pre[tid];
+		s_fn[64 + tid] = st.fn_post[tid];
 	}
-	const uint64_t mask = __ballot(valid && fired);
-	if((threadIdx.x & 63) == 0) {
-		const uint64_t slot = current_step(b) & (NEMO_STEP_RING - 1);
-		b.fired_bits[slot * b.fired_words + (ng >> 6)] = mask;
+	const bool post = valid && ((h >> st.post_window) & 1u);
+	uint32_t rb = 0, len = 0;
+	if(post) {
+		rb = st.rcm_ptr[ng];
+		len = st.rcm_ptr[ng + 1] - rb;
 	}
+	uint32_t nfired;
+	const uint32_t idx = block_exclusive_scan(post ? 1u : 0u, &nfired);
+	uint32_t total;
+	const uint32_t pos = block_exclusive_scan(len, &total);
+	if(post) {
+		s_rb[idx] = rb;
+		s_pos[idx] = pos;
+	}
+	if(tid == 0) s_pos[nfired] = total;
+	__syncthreads();
+	if(total == 0) return;
+
+	const uint64_t* pre_hist = history_after(b, r);
+	for(uint32_t e0 = tid; e0 < total; e0 += ILP * NEMO_TILE) {
+		uint32_t k[ILP];
+		uint32_t sd[ILP];
+		int32_t acc[ILP];
+#pragma unroll
+		for(int u = 0; u < ILP; ++u) {
+			const uint32_t e = e0 + u * NEMO_TILE;
+			k[u] = 0;
+			sd[u] = 0;
+			acc[u] = 0;
+			if(e < total) {
+				unsigned lo = 0, hi = nfired; /* s_pos[lo] <= e < s_pos[hi] */
+				while(hi - lo > 1) {
+					const unsigned mid = (lo + hi) >> 1;
+					if(s_pos[mid] <= e) lo = mid; else hi = mid;
+				}
+				k[u] = s_rb[lo] + (e - s_pos[lo]);
+				sd[u] = st.rcm_sd[k[u]];
+				acc[u] = st.rcm_acc[k[u]];
+			}
+		}
+		uint64_t ph[ILP];
+#pragma unroll
+		for(int u = 0; u < ILP; ++u) {
+			ph[u] = 0;
+			if(e0 + u * NEMO_TILE < total) ph[u] = pre_hist[sd[u] & 0x3ffffffu];
+		}
+#pragma unroll
+		for(int u = 0; u < ILP; ++u) {
+			if(e0 + u * NEMO_TILE < total) {
+				const unsigned d = (sd[u] >> 26) + 1;
+				const uint64_t arrivals = d >= 64 ? 0 : (ph[u] >> d);
+				const int32_t dw = stdp_wrap_add(
+						stdp_update_region(arrivals & st.potentiation, st.post_window, st.pre_post, st.post_pre,
+							s_fn, s_fn + 64),
+						stdp_update_region(arrivals & st.depression, st.post_window, st.pre_post, st.post_pre,
+							s_fn, s_fn + 64));
+				if(dw != 0) st.rcm_acc[k[u]] = stdp_wrap_add(acc[u], dw);
+			}
+		}
+	}
+	if(tid == 0) st.examined[ng / NEMO_TILE] += total;
 }
 
 } // namespace nemo_dev

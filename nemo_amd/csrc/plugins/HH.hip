@@ -20,6 +20,7 @@ hh_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 	const unsigned nl = blockIdx.x * NEMO_UPDATE_BLOCK + threadIdx.x;
 	const bool valid = nl < g.count;
 	const unsigned nn = g.start + nl;
+	const uint64_t h = nemo_dev::load_history(b, nn, valid);
 	bool fired = false;
 
 	if(valid) {
@@ -74,7 +75,8 @@ hh_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 		s[3 * P + nl] = h;
 		s[4 * P + nl] = dir;
 	}
-	nemo_dev::record_firing(b, nn, valid, fired);
+	nemo_dev::record_firing(b, nn, valid, fired, h);
+	nemo_dev::stdp_tile(b, nn, valid, h);
 }
 
 } // namespace

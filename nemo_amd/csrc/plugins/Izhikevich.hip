@@ -42,7 +42,7 @@ izhikevich_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 		u = g.state[STATE_U * P + nl];
 		v = g.state[STATE_V * P + nl];
 		if(g.rng) s = reinterpret_cast<const uint4*>(g.rng)[nl];
-		h = b.hist[ng];
+		h = nemo_dev::load_history(b, ng, true);
 	}
 	nemo_dev::gather_input(b, g, input);
 	bool fired = false;
@@ -76,6 +76,7 @@ izhikevich_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 		g.state[STATE_V * P + nl] = v;
 	}
 	nemo_dev::record_firing(b, ng, valid, fired, h);
+	nemo_dev::stdp_tile(b, ng, valid, h);
 }
 
 } // namespace

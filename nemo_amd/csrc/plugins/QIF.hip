@@ -20,6 +20,7 @@ qif_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 	const unsigned nl = blockIdx.x * NEMO_UPDATE_BLOCK + threadIdx.x;
 	const bool valid = nl < g.count;
 	const unsigned ng = g.start + nl;
+	const uint64_t h = nemo_dev::load_history(b, ng, valid);
 	bool fired = false;
 
 	if(valid) {
@@ -54,7 +55,8 @@ qif_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 		}
 		g.state[nl] = v;
 	}
-	nemo_dev::record_firing(b, ng, valid, fired);
+	nemo_dev::record_firing(b, ng, valid, fired, h);
+	nemo_dev::stdp_tile(b, ng, valid, h);
 }
 
 } // namespace
