@@ -210,20 +210,26 @@ nemo_status_t nemo_hip_batch_get_membrane_potential(nemo_simulation_t, unsigned 
  * so that a timed run excludes graph construction. */
 nemo_status_t nemo_hip_prepare(nemo_simulation_t, unsigned nsteps, unsigned stdp_period);
 
-/* Extension: launch shape of the per-step kernels: shape[0] delivery blocks
- * (1..1024), shape[1] events a delivery block stages in LDS (64..32768),
- * shape[2] 1 if STDP accumulation runs inside the neuron-update kernels, 0 for
- * a launch of its own, shape[3] firing-bitmap words per block of that launch
- * (1..4).  Every shape gives identical results. */
-nemo_status_t nemo_hip_get_launch_shape(nemo_simulation_t, unsigned shape[4]);
-nemo_status_t nemo_hip_set_launch_shape(nemo_simulation_t, const unsigned shape[4]);
+/* Extension: launch shape of the per-step kernels, NEMO_HIP_SHAPE_FIELDS
+ * values: shape[0] delivery blocks for the delays 2..D (for all delays when
+ * shape[1] is 0), shape[1] delivery blocks for the delay 1 (both together at
+ * most 1024), shape[2] events a delivery block stages in LDS (64..32768),
+ * shape[3] where the STDP accumulation runs: 0 a launch of its own on a graph
+ * branch beside the delivery, 1 inside the neuron-update kernels, 2 in the
+ * delivery launch that follows the previous update; shape[4] firing-bitmap
+ * words per STDP block (1..4), shape[5] 1 to deliver the delays 2..D of the
+ * next step beside the current update within nemo_hip_run (when split).
+ * Every shape gives identical results. */
+#define NEMO_HIP_SHAPE_FIELDS 6
+nemo_status_t nemo_hip_get_launch_shape(nemo_simulation_t, unsigned shape[NEMO_HIP_SHAPE_FIELDS]);
+nemo_status_t nemo_hip_set_launch_shape(nemo_simulation_t, const unsigned shape[NEMO_HIP_SHAPE_FIELDS]);
 
 /* Extension: autotuner.  Times candidate launch shapes (coordinate descent
- * over the four shape parameters), each over steps_per_trial steps from the
- * same snapshot of the simulation state, then restores that state and keeps
- * the fastest shape, returned in shape.  Results of the simulation are
- * unaffected. */
-nemo_status_t nemo_hip_autotune(nemo_simulation_t, unsigned steps_per_trial, unsigned shape[4]);
+ * over the shape parameters), each over steps_per_trial steps from the same
+ * snapshot of the simulation state, then restores that state and keeps the
+ * fastest shape, returned in shape (may be NULL).  Results of the simulation
+ * are unaffected. */
+nemo_status_t nemo_hip_autotune(nemo_simulation_t, unsigned steps_per_trial, unsigned shape[NEMO_HIP_SHAPE_FIELDS]);
 
 /* Extension: float-weight mode with a hand-written adjoint (no counterpart in
  * the reference, whose only mode is the fixed-point one).  A network of one

@@ -19,8 +19,9 @@
  * in step s = clock->base_step + step_offset (the semantics of
  * cpu_update_neurons, src/nemo/cpu/plugins/neuron_model.h:25-51):
  *   - sum the step's synaptic input of the tile: the binned events of the
- *     delivery blocks in the tile's bin_range (bin_table / bins) plus the
- *     overflow accumulators acc,
+ *     delivery blocks in the tile's two bin ranges (bin_table / bins, the
+ *     buffers of step parity s % 2) plus the overflow accumulators acc (of
+ *     the same parity),
  *     excitatory (w >= 0) and inhibitory separately, saturate each sum to the
  *     int32 range and convert to float (src/nemo/fixedpoint.cpp:40-49); clear
  *     the overflow accumulators;
@@ -104,13 +105,17 @@ typedef struct nemo_hip_stdp {
 typedef struct nemo_hip_step_buffers {
 	uint64_t step_offset;      /* step = clock->base_step + step_offset */
 	const nemo_hip_clock* clock;
-	const uint32_t* bin_table; /* [tile][bin_blocks]: count << 16 | offset */
-	const uint32_t* bin_range; /* per tile: first | end << 16 of the delivery blocks
-	                              holding its events (valid table entries) */
-	const int2* bins;          /* [bin_blocks][bin_capacity] {target % NEMO_TILE, Q weight} */
-	unsigned bin_blocks;
+	/* Delivery output of step s lives in the buffers of parity s % 2: plane p
+	 * of bin_table starts at p * tiles * bin_blocks, of bins at
+	 * p * bin_blocks * bin_capacity, of acc at p * 2 * acc_pitch. */
+	const uint32_t* bin_table; /* [2][tile][bin_blocks]: count << 16 | offset */
+	const uint32_t* bin_range; /* [2][tiles]: for each of the two delivery-block sets, per tile,
+	                              first | end << 16 of the blocks holding its events */
+	const int2* bins;          /* [2][bin_blocks][bin_capacity] {target % NEMO_TILE, Q weight} */
+	unsigned bin_blocks;       /* table columns (blocks of both sets) */
 	unsigned bin_capacity;
-	int64_t* acc;              /* overflow accumulators: E plane [0, acc_pitch), then I plane */
+	unsigned tiles;
+	int64_t* acc;              /* [2][2][acc_pitch] overflow accumulators: E plane, then I plane */
 	size_t acc_pitch;
 	float* ext_current;        /* per device index, or NULL */
 	const uint32_t* fstim;     /* firing-stimulus bit-vector, or NULL */

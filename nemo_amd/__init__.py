@@ -307,21 +307,23 @@ class Simulation(_SynapseQueries):
 
     @property
     def launch_shape(self):
-        """(delivery blocks, bin capacity, STDP inside the updates (1) or in its
-        own launch (0), bitmap words per block of that launch)."""
-        a = (ctypes.c_uint * 4)()
+        """(delivery blocks for delays 2..D -- all delays when the next is 0 --,
+        delivery blocks for delay 1, bin capacity, STDP mode (0 own branch, 1
+        inside the updates, 2 in the delivery launch), bitmap words per STDP
+        block, delays 2..D delivered beside the previous update (1) or not)."""
+        a = (ctypes.c_uint * 6)()
         check(lib().nemo_hip_get_launch_shape(self._h, a))
         return tuple(a)
 
     @launch_shape.setter
     def launch_shape(self, shape):
-        a = (ctypes.c_uint * 4)(*shape)
+        a = (ctypes.c_uint * 6)(*shape)
         check(lib().nemo_hip_set_launch_shape(self._h, a))
 
     def autotune(self, steps_per_trial=128):
         """Pick the fastest launch shape by timing candidates from a snapshot of
         the current state (restored afterwards); returns the chosen shape."""
-        a = (ctypes.c_uint * 4)()
+        a = (ctypes.c_uint * 6)()
         check(lib().nemo_hip_autotune(self._h, steps_per_trial, a))
         return tuple(a)
 

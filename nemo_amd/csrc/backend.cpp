@@ -414,6 +414,11 @@ void Configuration::setHipBackend(int dev)
 
 /* ------------------------------------------------------------- simulation */
 
+static const char* stdpModeName(unsigned mode)
+{
+	return mode == 0 ? "own branch" : mode == 1 ? "in the updates" : "in the delivery launch";
+}
+
 Simulation::Simulation(const Network& net, const Configuration& conf)
 {
 	construct(std::vector<const Network*>(1, &net), conf);
@@ -455,6 +460,10 @@ void Simulation::construct(const std::vector<const Network*>& nets, const Config
 	HIP_CHECK(hipStreamCreateWithFlags(&m_stdpStream, hipStreamNonBlocking));
 	HIP_CHECK(hipEventCreateWithFlags(&m_stdpFork, hipEventDisableTiming));
 	HIP_CHECK(hipEventCreateWithFlags(&m_stdpDone, hipEventDisableTiming));
+	HIP_CHECK(hipStreamCreateWithFlags(&m_earlyStream, hipStreamNonBlocking));
+	HIP_CHECK(hipEventCreateWithFlags(&m_earlyFork, hipEventDisableTiming));
+	HIP_CHECK(hipEventCreateWithFlags(&m_earlyDone[0], hipEventDisableTiming));
+	HIP_CHECK(hipEventCreateWithFlags(&m_earlyDone[1], hipEventDisableTiming));
 	HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&m_pinnedScalars), SCALAR_SLOTS * sizeof(uint64_t),
 				hipHostMallocDefault));
 
@@ -471,9 +480,10 @@ void Simulation::construct(const std::vector<const Network*>& nets, const Config
 	HIP_CHECK(hipStreamSynchronize(m_stream));
 	if(m_logging) {
 		std::printf("nemo-hip: %u network(s), %zu neurons (%u tiles), %zu synapses (%zu plastic), max delay %u, "
-				"Q%u.%u, delivery %u blocks x %u events, STDP %u blocks (%s)\n",
+				"Q%u.%u, delivery %u + %u blocks x %u events, STDP %u blocks (%s)\n",
 				m_nets, m_ncount, m_tiles, m_nsynapses, m_nplastic, m_D, 31 - m_fbits, m_fbits,
-				m_shape.deliverBlocks, m_shape.binCapacity, m_stdpBlocks, m_shape.fuseStdp ? "in the updates" : "own launch");
+				m_shape.deliverBlocks, m_shape.lateBlocks, m_shape.binCapacity, m_stdpBlocks,
+				stdpModeName(m_shape.stdpMode));
 	}
 	resetTimer();
 }
@@ -483,7 +493,14 @@ Simulation::~Simulation()
 	(void) hipSetDevice(m_device);
 	if(m_stream) (void) hipStreamSynchronize(m_stream);
 	if(m_stdpStream) (void) hipStreamSynchronize(m_stdpStream);
+	if(m_earlyStream) (void) hipStreamSynchronize(m_earlyStream);
 	clearGraphs();
+	if(m_earlyStream) {
+		(void) hipStreamDestroy(m_earlyStream);
+		(void) hipEventDestroy(m_earlyFork);
+		(void) hipEventDestroy(m_earlyDone[0]);
+		(void) hipEventDestroy(m_earlyDone[1]);
+	}
 	if(m_stream) (void) hipStreamDestroy(m_stream);
 	if(m_stdpStream) {
 		(void) hipStreamDestroy(m_stdpStream);
@@ -898,11 +915,20 @@ struct Partition {
 	unsigned maxWindow = 0;
 };
 
-static Partition partitionPairs(const std::vector<uint64_t>& weight, const std::vector<uint2>& srcWindow,
-		unsigned D, unsigned tiles, unsigned B)
+static Partition partitionPairs(const std::vector<uint64_t>& pairWeight, const std::vector<uint2>& srcWindow,
+		unsigned D, unsigned tiles, unsigned dFirst, unsigned dCount, unsigned B)
 {
 	Partition P;
-	const size_t Q = weight.size();
+	if(B == 0 || dCount == 0) {
+		P.range.assign(tiles, 0);
+		return P;
+	}
+	/* pairs of the set: q = tile * dCount + d - dFirst */
+	const size_t Q = size_t(tiles) * dCount;
+	std::vector<uint64_t> weight(Q);
+	for(size_t p = 0; p < tiles; ++p) {
+		for(unsigned j = 0; j < dCount; ++j) weight[p * dCount + j] = pairWeight[p * D + dFirst - 1 + j];
+	}
 	uint64_t total = 0;
 	for(uint64_t w : weight) total += w;
 	P.bounds.assign(B + 1, 0);
@@ -918,7 +944,7 @@ static Partition partitionPairs(const std::vector<uint64_t>& weight, const std::
 	std::vector<uint32_t> lo(B, tiles), hi(B, 0);
 	for(unsigned b = 0; b < B; ++b) {
 		if(P.bounds[b] == P.bounds[b + 1]) continue;
-		const unsigned p0 = P.bounds[b] / D, p1 = (P.bounds[b + 1] - 1) / D;
+		const unsigned p0 = P.bounds[b] / dCount, p1 = (P.bounds[b + 1] - 1) / dCount;
 		for(unsigned p = p0; p <= p1; ++p) {
 			if(srcWindow[p].y == 0) continue;
 			lo[b] = std::min(lo[b], srcWindow[p].x);
@@ -945,39 +971,59 @@ static Partition partitionPairs(const std::vector<uint64_t>& weight, const std::
 
 /* Default shape: bin capacity 4096 events (halved while the LDS does not
  * allow it), enough delivery blocks for about 1% of the synapses per step at
- * half capacity, STDP accumulation inside the update kernels, two bitmap words per
- * STDP block.  NEMO_HIP_DELIVER_BLOCKS, NEMO_HIP_BIN_CAPACITY,
- * NEMO_HIP_FUSE_STDP and NEMO_HIP_STDP_WORDS override it (tuning). */
+ * half capacity, about a 1/D share of them for the delay-1 set, STDP on its
+ * own graph branch, two bitmap words per STDP block, delays 2..D delivered
+ * beside the previous update.  NEMO_HIP_DELIVER_BLOCKS, NEMO_HIP_LATE_BLOCKS,
+ * NEMO_HIP_BIN_CAPACITY, NEMO_HIP_FUSE_STDP, NEMO_HIP_STDP_WORDS and
+ * NEMO_HIP_OVERLAP override it (tuning). */
 LaunchShape Simulation::defaultShape() const
 {
 	LaunchShape s;
 	const uint64_t pairs = uint64_t(m_D) * m_tiles;
 	for(unsigned cap = 4096; cap >= 256; cap >>= 1) {
 		double blocks = std::ceil(0.01 * double(m_nsynapses) / (0.5 * cap));
-		blocks = std::max(1.0, std::min({blocks, double(NEMO_MAX_BIN_BLOCKS), double(pairs)}));
+		blocks = std::max(2.0, std::min({blocks, double(NEMO_MAX_BIN_BLOCKS), double(pairs) + 1.0}));
 		s.deliverBlocks = unsigned(blocks);
+		s.lateBlocks = 0;
 		s.binCapacity = cap;
-		const Partition P = partitionPairs(m_pairWeight, m_srcWindow, m_D, m_tiles, s.deliverBlocks);
-		if(nemo_propagate_lds_bytes(std::max(1u, P.maxWindow), cap) <= m_ldsBudget) break;
+		if(ldsNeeded(s) <= m_ldsBudget) break;
 	}
-	if(const char* e = std::getenv("NEMO_HIP_DELIVER_BLOCKS")) {
-		s.deliverBlocks = unsigned(std::max(1, std::min(std::atoi(e), int(NEMO_MAX_BIN_BLOCKS))));
-	}
-	if(const char* e = std::getenv("NEMO_HIP_BIN_CAPACITY")) {
-		const int c = std::atoi(e);
-		if(c >= 64 && c <= 32768) s.binCapacity = unsigned(c);
-	}
-	if(const char* e = std::getenv("NEMO_HIP_FUSE_STDP")) s.fuseStdp = std::atoi(e) ? 1 : 0;
-	if(const char* e = std::getenv("NEMO_HIP_STDP_WORDS")) {
-		s.stdpWords = unsigned(std::max(1, std::min(std::atoi(e), 4)));
-	}
+	s.stdpMode = 2;
+	auto env = [](const char* name, int lo, int hi, unsigned& field) {
+		if(const char* e = std::getenv(name)) field = unsigned(std::max(lo, std::min(std::atoi(e), hi)));
+	};
+	env("NEMO_HIP_DELIVER_BLOCKS", 0, NEMO_MAX_BIN_BLOCKS, s.deliverBlocks);
+	env("NEMO_HIP_LATE_BLOCKS", 0, NEMO_MAX_BIN_BLOCKS, s.lateBlocks);
+	env("NEMO_HIP_BIN_CAPACITY", 64, 32768, s.binCapacity);
+	env("NEMO_HIP_STDP_MODE", 0, 2, s.stdpMode);
+	env("NEMO_HIP_STDP_WORDS", 1, 4, s.stdpWords);
+	env("NEMO_HIP_OVERLAP", 0, 1, s.overlap);
 	return s;
+}
+
+/* Dynamic LDS the delivery launches of a shape need (the larger set). */
+size_t Simulation::ldsNeeded(const LaunchShape& s) const
+{
+	unsigned w = 1;
+	const bool split = s.lateBlocks > 0;
+	const unsigned dFirst = split ? 2 : 1, dCount = split ? m_D - 1 : m_D;
+	if(dCount && s.deliverBlocks) {
+		w = std::max(w, partitionPairs(m_pairWeight, m_srcWindow, m_D, m_tiles, dFirst, dCount, s.deliverBlocks).maxWindow);
+	}
+	if(split) w = std::max(w, partitionPairs(m_pairWeight, m_srcWindow, m_D, m_tiles, 1, 1, s.lateBlocks).maxWindow);
+	return nemo_propagate_lds_bytes(w, s.binCapacity);
 }
 
 void Simulation::checkShape(const LaunchShape& s) const
 {
-	if(s.deliverBlocks < 1 || s.deliverBlocks > NEMO_MAX_BIN_BLOCKS) {
-		throw Error(NEMO_INVALID_INPUT, "delivery blocks must be in [1, " + std::to_string(NEMO_MAX_BIN_BLOCKS) + "]");
+	if(s.deliverBlocks + s.lateBlocks < 1 || s.deliverBlocks + s.lateBlocks > NEMO_MAX_BIN_BLOCKS) {
+		throw Error(NEMO_INVALID_INPUT, "delivery blocks: 1 to " + std::to_string(NEMO_MAX_BIN_BLOCKS) + " in all");
+	}
+	if(s.deliverBlocks == 0 && (s.lateBlocks == 0 || m_D >= 2)) {
+		throw Error(NEMO_INVALID_INPUT, "the delays delivered by the first block set need at least one block");
+	}
+	if(s.overlap > 1) {
+		throw Error(NEMO_INVALID_INPUT, "overlap flag must be 0 or 1");
 	}
 	if(s.binCapacity < 64 || s.binCapacity > 32768) {
 		throw Error(NEMO_INVALID_INPUT, "bin capacity must be in [64, 32768]");
@@ -985,8 +1031,8 @@ void Simulation::checkShape(const LaunchShape& s) const
 	if(s.stdpWords < 1 || s.stdpWords > 4) {
 		throw Error(NEMO_INVALID_INPUT, "STDP words per block must be in [1, 4]");
 	}
-	if(s.fuseStdp > 1) {
-		throw Error(NEMO_INVALID_INPUT, "fuse flag must be 0 or 1");
+	if(s.stdpMode > 2) {
+		throw Error(NEMO_INVALID_INPUT, "STDP mode must be 0, 1 or 2");
 	}
 }
 
@@ -996,9 +1042,15 @@ void Simulation::checkShape(const LaunchShape& s) const
 void Simulation::applyShape(const LaunchShape& s)
 {
 	checkShape(s);
-	const unsigned B = s.deliverBlocks;
-	const Partition P = partitionPairs(m_pairWeight, m_srcWindow, m_D, m_tiles, B);
-	const unsigned maxWindow = std::max(1u, P.maxWindow);
+	/* lateBlocks == 0: one set for all delays; else delays 2..D and delay 1 */
+	const bool split = s.lateBlocks > 0;
+	const unsigned dFirst = split ? 2 : 1, dCount = split ? m_D - 1 : m_D;
+	const unsigned Be = dCount ? s.deliverBlocks : 0;
+	const unsigned Bl = s.lateBlocks;
+	const unsigned C = Be + Bl;
+	const Partition Pe = partitionPairs(m_pairWeight, m_srcWindow, m_D, m_tiles, dFirst, dCount, Be);
+	const Partition Pl = partitionPairs(m_pairWeight, m_srcWindow, m_D, m_tiles, 1, 1, Bl);
+	const unsigned maxWindow = std::max({1u, Pe.maxWindow, Pl.maxWindow});
 	if(nemo_propagate_lds_bytes(maxWindow, s.binCapacity) > m_ldsBudget) {
 		throw Error(NEMO_INVALID_INPUT, "launch shape needs more LDS than a block can have");
 	}
@@ -1006,14 +1058,28 @@ void Simulation::applyShape(const LaunchShape& s)
 	HIP_CHECK(hipStreamSynchronize(m_stream));
 	clearGraphs();
 	m_shape = s;
-	m_pairBounds.allocate(P.bounds.size());
-	HIP_CHECK(hipMemcpy(m_pairBounds.ptr, P.bounds.data(), m_pairBounds.bytes(), hipMemcpyHostToDevice));
-	m_windows.allocate(P.windows.size());
-	HIP_CHECK(hipMemcpy(m_windows.ptr, P.windows.data(), m_windows.bytes(), hipMemcpyHostToDevice));
-	m_binRange.allocate(P.range.size());
-	HIP_CHECK(hipMemcpy(m_binRange.ptr, P.range.data(), m_binRange.bytes(), hipMemcpyHostToDevice));
-	m_bins.allocate(size_t(B) * s.binCapacity);
-	m_binTable.allocate(size_t(m_tiles) * B);
+	m_shape.deliverBlocks = Be;
+
+	/* both sets back to back: bounds [Be+1 | Bl+1], windows [Be | Bl]; the
+	 * per-tile ranges in absolute table columns, early set then late set */
+	std::vector<uint32_t> bounds(Pe.bounds);
+	bounds.insert(bounds.end(), Pl.bounds.begin(), Pl.bounds.end());
+	std::vector<uint2> windows(Pe.windows);
+	windows.insert(windows.end(), Pl.windows.begin(), Pl.windows.end());
+	std::vector<uint32_t> range(2 * size_t(m_tiles), 0);
+	for(unsigned t = 0; t < m_tiles; ++t) {
+		range[t] = Pe.range[t];
+		const uint32_t r = Pl.range[t];
+		if((r >> 16) > (r & 0xffffu)) range[m_tiles + t] = ((r & 0xffffu) + Be) | (((r >> 16) + Be) << 16);
+	}
+	m_pairBounds.allocate(std::max<size_t>(1, bounds.size()));
+	HIP_CHECK(hipMemcpy(m_pairBounds.ptr, bounds.data(), bounds.size() * 4, hipMemcpyHostToDevice));
+	m_windows.allocate(std::max<size_t>(1, windows.size()));
+	HIP_CHECK(hipMemcpy(m_windows.ptr, windows.data(), windows.size() * sizeof(uint2), hipMemcpyHostToDevice));
+	m_binRange.allocate(range.size());
+	HIP_CHECK(hipMemcpy(m_binRange.ptr, range.data(), m_binRange.bytes(), hipMemcpyHostToDevice));
+	m_bins.allocate(2 * size_t(C) * s.binCapacity);
+	m_binTable.allocate(2 * size_t(m_tiles) * C);
 	HIP_CHECK(hipMemset(m_binTable.ptr, 0, m_binTable.bytes()));
 	m_stdpBlocks = (m_stdp && m_nplastic) ? unsigned((m_firedWords + s.stdpWords - 1) / s.stdpWords) : 0;
 
@@ -1022,9 +1088,8 @@ void Simulation::applyShape(const LaunchShape& s)
 	a.step_offset = 0;
 	a.fired_bits = m_firedBits.ptr;
 	a.fired_words = m_firedWords;
-	a.deliver_blocks = m_nsynapses ? B : 0;
-	a.pair_bounds = m_pairBounds.ptr;
-	a.windows = m_windows.ptr;
+	a.deliver_blocks = 0;
+	a.table_columns = C;
 	a.max_window = maxWindow;
 	a.rowptr = m_rowptr.ptr;
 	a.fcm = m_fcm.ptr;
@@ -1040,6 +1105,21 @@ void Simulation::applyShape(const LaunchShape& s)
 	a.hist = m_hist.ptr;
 	a.hist_pitch = m_accPitch;
 	a.stdp = m_stdpArgs;
+
+	m_early = a;
+	m_early.deliver_blocks = m_nsynapses ? Be : 0;
+	m_early.block_base = 0;
+	m_early.d_first = dFirst;
+	m_early.d_count = std::max(1u, dCount);
+	m_early.pair_bounds = m_pairBounds.ptr;
+	m_early.windows = m_windows.ptr;
+	m_late = a;
+	m_late.deliver_blocks = m_nsynapses ? Bl : 0;
+	m_late.block_base = Be;
+	m_late.d_first = 1;
+	m_late.d_count = 1;
+	m_late.pair_bounds = m_pairBounds.ptr + Pe.bounds.size();
+	m_late.windows = m_windows.ptr + Be;
 }
 
 void Simulation::setLaunchShape(const LaunchShape& s)
@@ -1065,7 +1145,7 @@ void Simulation::allocateStepState()
 	m_firedBits.allocate(size_t(NEMO_STEP_RING) * m_firedWords);
 	HIP_CHECK(hipMemset(m_firedBits.ptr, 0, m_firedBits.bytes()));
 	m_accPitch = N;
-	m_acc.allocate(2 * m_accPitch);
+	m_acc.allocate(4 * m_accPitch); /* [step parity][E, I][N] */
 	HIP_CHECK(hipMemset(m_acc.ptr, 0, m_acc.bytes()));
 	m_hist.allocate(2 * N); /* plane s % 2 holds the history after step s */
 	HIP_CHECK(hipMemset(m_hist.ptr, 0, m_hist.bytes()));
@@ -1102,8 +1182,9 @@ nemo_hip_step_buffers Simulation::stepBuffers(uint64_t offset, bool ext, bool fs
 	b.bin_table = m_binTable.ptr;
 	b.bin_range = m_binRange.ptr;
 	b.bins = m_bins.ptr;
-	b.bin_blocks = m_nsynapses ? m_shape.deliverBlocks : 0;
+	b.bin_blocks = m_nsynapses ? m_shape.deliverBlocks + m_shape.lateBlocks : 0;
 	b.bin_capacity = m_shape.binCapacity;
+	b.tiles = m_tiles;
 	b.acc = m_acc.ptr;
 	b.acc_pitch = m_accPitch;
 	b.ext_current = ext ? m_ext.ptr : nullptr;
@@ -1115,7 +1196,7 @@ nemo_hip_step_buffers Simulation::stepBuffers(uint64_t offset, bool ext, bool fs
 	b.fbits = m_fbits;
 	b.debug_current = m_debug ? m_debugCurrent.ptr : nullptr;
 	b.stdp = m_stdpArgs;
-	if(!m_shape.fuseStdp) b.stdp.rcm_ptr = nullptr; /* STDP runs in its own launch */
+	if(m_shape.stdpMode != 1) b.stdp.rcm_ptr = nullptr; /* STDP runs in a launch of its own */
 	return b;
 }
 
@@ -1136,38 +1217,75 @@ void Simulation::setBase(uint64_t base)
 	copyScalar(&m_clock.ptr->base_step, base);
 }
 
-/* One step on the stream (SimulationBackend.cpp:22-77): the propagate kernel
- * (delivery of this step; STDP accumulation of the previous step, which only
- * needs the firing history its update left), then the neuron updates.  The
- * step's own STDP accumulation runs with the next step, or on its own when
- * flushStdp() needs it earlier.  With ev, HIP events bracket the two
- * launches. */
-void Simulation::enqueueStep(uint64_t offset, bool ext, bool fstim, hipEvent_t* ev)
+void Simulation::launchDelivery(const nemo_propagate_args& set, uint64_t offset, hipStream_t st, bool withStdp)
 {
-	if(!m_shape.fuseStdp && m_stdpBlocks) {
-		/* STDP of the previous step on a parallel branch: it needs the history
-		 * its update left (plane (t-1) % 2), which the update of step t+1
-		 * overwrites, so it joins back before this step's successor. */
-		joinStdp();
-		HIP_CHECK(hipEventRecord(m_stdpFork, m_stream));
-		HIP_CHECK(hipStreamWaitEvent(m_stdpStream, m_stdpFork, 0));
-		nemo_propagate_args sa = m_propagate;
-		sa.step_offset = offset;
-		sa.deliver_blocks = 0;
-		HIP_CHECK(nemo_launch_propagate(m_stdpStream, &sa, m_stdpBlocks));
-		HIP_CHECK(hipEventRecord(m_stdpDone, m_stdpStream));
-		m_stdpPending = true;
-	}
-	if(ev) HIP_CHECK(hipEventRecord(ev[0], m_stream));
-	nemo_propagate_args a = m_propagate;
+	const unsigned stdpBlocks = withStdp ? m_stdpBlocks : 0;
+	if(!set.deliver_blocks && !stdpBlocks) return;
+	nemo_propagate_args a = set;
 	a.step_offset = offset;
-	HIP_CHECK(nemo_launch_propagate(m_stream, &a, 0));
-	if(ev) HIP_CHECK(hipEventRecord(ev[1], m_stream));
-	nemo_hip_step_buffers b = stepBuffers(offset, ext, fstim);
-	for(Group& g : m_groups) {
-		HIP_CHECK(g.plugin->update(m_stream, &g.dev, &b));
+	HIP_CHECK(nemo_launch_propagate(st, &a, stdpBlocks, 1));
+}
+
+/* Steps first..first+k-1 on the stream (SimulationBackend.cpp:22-77): spike
+ * delivery (delays 2..D, then delay 1), the neuron updates, and the STDP
+ * accumulation of the previous step.  Within a run of steps, the delivery of
+ * delays 2..D for step t+2 is forked onto a branch right after the update of
+ * step t (it needs firing up to step t and the buffers of its parity, which
+ * the update of step t has finished with) and joined before the update of
+ * step t+2, so it runs beside the update of step t+1 and the delay-1
+ * delivery of steps t+1 and t+2; the first step of a run delivers its delays
+ * 2..D in order.  STDP accumulation of step t-1 runs on its own branch beside
+ * the delivery of step t, or inside the update kernels.  With ev (three
+ * events per step), HIP events bracket the delivery and the updates. */
+void Simulation::enqueueSteps(uint64_t first, unsigned k, bool ext, bool fstim, hipEvent_t* ev)
+{
+	const bool split = m_late.deliver_blocks && m_early.deliver_blocks;
+	const bool overlap = split && m_shape.overlap && k > 1;
+	/* STDP shares the delivery launch that follows the previous update */
+	const bool fused = m_shape.stdpMode == 2 && m_stdpBlocks;
+	bool pending[2] = {false, false};
+	auto forkEarly = [&](unsigned i) {
+		HIP_CHECK(hipEventRecord(m_earlyFork, m_stream));
+		HIP_CHECK(hipStreamWaitEvent(m_earlyStream, m_earlyFork, 0));
+		launchDelivery(m_early, first + i, m_earlyStream, false);
+		HIP_CHECK(hipEventRecord(m_earlyDone[i & 1], m_earlyStream));
+		pending[i & 1] = true;
+	};
+	for(unsigned i = 0; i < k; ++i) {
+		const uint64_t off = first + i;
+		if(ev) HIP_CHECK(hipEventRecord(ev[3 * i], m_stream));
+		if(m_shape.stdpMode == 0 && m_stdpBlocks) {
+			joinStdp();
+			HIP_CHECK(hipEventRecord(m_stdpFork, m_stream));
+			HIP_CHECK(hipStreamWaitEvent(m_stdpStream, m_stdpFork, 0));
+			nemo_propagate_args sa = m_propagate;
+			sa.step_offset = off;
+			HIP_CHECK(nemo_launch_propagate(m_stdpStream, &sa, m_stdpBlocks, 0));
+			HIP_CHECK(hipEventRecord(m_stdpDone, m_stdpStream));
+			m_stdpPending = true;
+		}
+		if(split) {
+			if(i == 0 || !overlap) launchDelivery(m_early, off, m_stream, false);
+			if(i == 0 && overlap) forkEarly(1);
+			launchDelivery(m_late, off, m_stream, fused);
+		} else {
+			launchDelivery(m_early, off, m_stream, fused);
+		}
+		if(pending[i & 1] && i > 0) {
+			HIP_CHECK(hipStreamWaitEvent(m_stream, m_earlyDone[i & 1], 0));
+			pending[i & 1] = false;
+		}
+		/* the STDP branch overlaps delivery only: beside the update it slows
+		 * the update, which is on the step's critical path */
+		joinStdp();
+		if(ev) HIP_CHECK(hipEventRecord(ev[3 * i + 1], m_stream));
+		nemo_hip_step_buffers b = stepBuffers(off, ext, fstim);
+		for(Group& g : m_groups) {
+			HIP_CHECK(g.plugin->update(m_stream, &g.dev, &b));
+		}
+		if(ev) HIP_CHECK(hipEventRecord(ev[3 * i + 2], m_stream));
+		if(overlap && i + 2 < k) forkEarly(i + 2);
 	}
-	if(ev) HIP_CHECK(hipEventRecord(ev[2], m_stream));
 }
 
 /* Make the main stream wait for the STDP branch of the last enqueued step. */
@@ -1189,7 +1307,7 @@ void Simulation::flushStdp()
 	a.deliver_blocks = 0;
 	a.step_offset = 0;
 	setBase(s + 1);
-	HIP_CHECK(nemo_launch_propagate(m_stream, &a, m_stdpBlocks));
+	HIP_CHECK(nemo_launch_propagate(m_stream, &a, m_stdpBlocks, 1));
 	copyScalar(&m_clock.ptr->stdp_flushed, s);
 	m_flushed = s;
 	HIP_CHECK(hipStreamSynchronize(m_stream));
@@ -1296,8 +1414,7 @@ const std::vector<unsigned>& Simulation::step(const unsigned* fstim, size_t nf,
 	}
 	const uint64_t t = m_hostStep;
 	setBase(t);
-	enqueueStep(0, !iIdx.empty(), !fIdx.empty());
-	joinStdp();
+	enqueueSteps(0, 1, !iIdx.empty(), !fIdx.empty());
 	m_hostStep += 1;
 	if(!fIdx.empty()) {
 		HIP_CHECK(nemo_launch_fstim(m_stream, m_fstim.ptr, m_stimIdx.ptr, fIdx.size(), 0));
@@ -1318,8 +1435,7 @@ hipGraphExec_t Simulation::graphFor(unsigned k)
 	const bool debug = m_debug;
 	m_debug = false;
 	try {
-		for(unsigned i = 0; i < k; ++i) enqueueStep(i, false, false);
-		joinStdp();
+		enqueueSteps(0, k, false, false);
 	} catch(...) {
 		m_debug = debug;
 		m_stdpPending = false;
@@ -1440,8 +1556,7 @@ void Simulation::profile(unsigned nsteps, double avgUs[4], uint64_t counters[4])
 	for(hipEvent_t& e : ev) HIP_CHECK(hipEventCreate(&e));
 	const uint64_t t0 = m_hostStep;
 	setBase(t0);
-	for(unsigned s = 0; s < nsteps; ++s) enqueueStep(s, false, false, &ev[3 * size_t(s)]);
-	joinStdp();
+	for(unsigned s = 0; s < nsteps; ++s) enqueueSteps(s, 1, false, false, &ev[3 * size_t(s)]);
 	m_hostStep += nsteps;
 	HIP_CHECK(hipStreamSynchronize(m_stream));
 	double sum[3] = {0, 0, 0};
@@ -1591,16 +1706,38 @@ LaunchShape Simulation::autotune(unsigned stepsPerTrial)
 			best = c;
 		}
 	};
-	const unsigned maxBlocks = unsigned(std::min<uint64_t>(NEMO_MAX_BIN_BLOCKS, uint64_t(m_D) * m_tiles));
-	std::vector<unsigned> blocks;
-	for(double f : {0.25, 0.5, 2.0, 4.0}) {
-		const unsigned b = unsigned(std::max(1.0, std::min(double(maxBlocks), std::round(start.deliverBlocks * f))));
-		if(b != start.deliverBlocks && std::find(blocks.begin(), blocks.end(), b) == blocks.end()) blocks.push_back(b);
+	auto scaled = [](unsigned base, std::initializer_list<double> fs, unsigned lo, unsigned hi) {
+		std::vector<unsigned> v;
+		for(double f : fs) {
+			const unsigned x = unsigned(std::max(double(lo), std::min(double(hi), std::round(base * f))));
+			if(x != base && std::find(v.begin(), v.end(), x) == v.end()) v.push_back(x);
+		}
+		return v;
+	};
+	if(m_D >= 2) {
+		for(unsigned b : scaled(start.deliverBlocks, {0.25, 0.5, 2.0, 4.0}, 1, NEMO_MAX_BIN_BLOCKS - start.lateBlocks)) {
+			LaunchShape c = best;
+			c.deliverBlocks = b;
+			consider(c);
+		}
 	}
-	for(unsigned b : blocks) {
-		LaunchShape c = best;
-		c.deliverBlocks = b;
-		consider(c);
+	if(m_D >= 2) {
+		/* splitting off the delay-1 set: a share of about 1/D of the blocks, and multiples */
+		std::vector<unsigned> late;
+		const unsigned base = std::max(1u, unsigned(std::lround(double(best.deliverBlocks) / m_D)));
+		for(double f : {0.0, 1.0, 2.0, 4.0}) {
+			const unsigned x = unsigned(std::min<double>(NEMO_MAX_BIN_BLOCKS - best.deliverBlocks, std::round(base * f)));
+			if(x != best.lateBlocks && std::find(late.begin(), late.end(), x) == late.end()) late.push_back(x);
+		}
+		for(unsigned b : late) {
+			for(unsigned ov : {0u, 1u}) {
+				if(b == 0 && ov == 1) continue;
+				LaunchShape c = best;
+				c.lateBlocks = b;
+				c.overlap = ov;
+				consider(c);
+			}
+		}
 	}
 	for(unsigned cap : {1024u, 2048u, 4096u, 8192u, 16384u}) {
 		if(cap == best.binCapacity) continue;
@@ -1609,9 +1746,11 @@ LaunchShape Simulation::autotune(unsigned stepsPerTrial)
 		consider(c);
 	}
 	if(m_stdp && m_nplastic) {
-		{
+		const unsigned mode0 = best.stdpMode;
+		for(unsigned m : {0u, 1u, 2u}) {
+			if(m == mode0) continue;
 			LaunchShape c = best;
-			c.fuseStdp = 1 - best.fuseStdp;
+			c.stdpMode = m;
 			consider(c);
 		}
 		for(unsigned w : {1u, 2u, 4u}) {
@@ -1624,9 +1763,10 @@ LaunchShape Simulation::autotune(unsigned stepsPerTrial)
 	applyShape(best);
 	restoreSnapshot(snap);
 	if(m_logging) {
-		std::printf("nemo-hip: autotuned launch shape: %u delivery blocks x %u events, STDP %s, %u words "
-				"(%.1f us/step)\n", best.deliverBlocks, best.binCapacity, best.fuseStdp ? "in the updates" : "own launch",
-				best.stdpWords, 1e6 * tBest / stepsPerTrial);
+		std::printf("nemo-hip: autotuned launch shape: %u + %u delivery blocks x %u events, STDP %s, %u words, "
+				"overlap %u (%.1f us/step)\n", best.deliverBlocks, best.lateBlocks, best.binCapacity,
+				stdpModeName(best.stdpMode), best.stdpWords, best.overlap,
+				1e6 * tBest / stepsPerTrial);
 	}
 	return best;
 }

@@ -148,10 +148,13 @@ struct DeviceArray {
 
 /* Launch shape of the per-step kernels; any shape gives the same results. */
 struct LaunchShape {
-	unsigned deliverBlocks = 1;   /* delivery blocks (bin regions), <= NEMO_MAX_BIN_BLOCKS */
+	unsigned deliverBlocks = 1;   /* delivery blocks for the delays 2..D (all delays if lateBlocks == 0) */
+	unsigned lateBlocks = 0;      /* delivery blocks for the delay 1 (both sets <= NEMO_MAX_BIN_BLOCKS) */
 	unsigned binCapacity = 4096;  /* events a delivery block stages in LDS */
-	unsigned fuseStdp = 1;        /* STDP accumulation in the update kernels (1) or its own launch (0) */
+	unsigned stdpMode = 2;        /* STDP accumulation: own launch on a graph branch (0), inside the
+	                                 update kernels (1), or sharing the delivery launch (2) */
 	unsigned stdpWords = 2;       /* firing-bitmap words (64 targets each) per STDP block */
+	unsigned overlap = 1;         /* deliver the delays 2..D of step t+1 beside the update of step t */
 };
 
 /* The HIP simulation (cuda::Simulation's role, cuda/Simulation.cpp:91-380).
@@ -234,6 +237,8 @@ private:
 	hipStream_t m_stream = nullptr;
 	hipStream_t m_stdpStream = nullptr;  /* parallel branch for STDP in its own launch */
 	hipEvent_t m_stdpFork = nullptr, m_stdpDone = nullptr;
+	hipStream_t m_earlyStream = nullptr; /* branch for the delays 2..D of the next step */
+	hipEvent_t m_earlyFork = nullptr, m_earlyDone[2] = {nullptr, nullptr};
 	bool m_stdpPending = false;          /* the STDP branch has not been joined yet */
 	unsigned m_nets = 1;   /* networks in the batch */
 	size_t m_ncount = 0;   /* neurons */
@@ -287,7 +292,8 @@ private:
 	uint64_t m_flushed = ~uint64_t(0); /* step whose STDP accumulation ran on its own */
 
 	/* propagate-kernel arguments and launch shape */
-	nemo_propagate_args m_propagate;
+	nemo_propagate_args m_propagate;     /* common delivery arguments */
+	nemo_propagate_args m_early, m_late;  /* the two delivery block sets */
 	nemo_hip_stdp m_stdpArgs;            /* STDP accumulation arguments (rcm_ptr NULL: none) */
 	LaunchShape m_shape;
 	unsigned m_stdpBlocks = 0;
@@ -326,11 +332,13 @@ private:
 	void allocateStepState();
 	LaunchShape defaultShape() const;
 	void applyShape(const LaunchShape& s);
+	size_t ldsNeeded(const LaunchShape& s) const;
 	void checkShape(const LaunchShape& s) const;
 	nemo_hip_step_buffers stepBuffers(uint64_t offset, bool ext, bool fstim) const;
 	void copyScalar(uint64_t* dst, uint64_t value);
 	void setBase(uint64_t base);
-	void enqueueStep(uint64_t offset, bool ext, bool fstim, hipEvent_t* ev = nullptr);
+	void enqueueSteps(uint64_t first, unsigned k, bool ext, bool fstim, hipEvent_t* ev = nullptr);
+	void launchDelivery(const nemo_propagate_args& set, uint64_t offset, hipStream_t st, bool withStdp);
 	void flushStdp();
 	void joinStdp();
 	void decodeFiring(const uint64_t* bits, std::vector<unsigned>& out) const;

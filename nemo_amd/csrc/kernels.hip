@@ -15,10 +15,11 @@
  *   fired_bits uint64[128][N/64]  one firing bitmap per step (ring)
  *   hist       uint64[2][N] firing history (bit k: fired k steps ago); plane
  *                          s % 2 holds the history after step s
- *   bins       int2[B][CAP] each delivery block's events, sorted by tile
- *   bin_table  uint32[T][B] count << 16 | offset of tile t in block b's bins
- *                          (entries of the blocks whose window holds t)
- *   acc        int64[2][N] overflow accumulators (events beyond CAP)
+ *   bins       int2[2][B][CAP] each delivery block's events, sorted by tile,
+ *                          for steps of even and odd parity
+ *   bin_table  uint32[2][T][B] count << 16 | offset of tile t in block b's
+ *                          bins (entries of the blocks whose window holds t)
+ *   acc        int64[2][2][N] overflow accumulators (events beyond CAP)
  *   rcm_*      per target, its plastic incoming synapses (STDP only)
  */
 #include <hip/hip_runtime.h>
@@ -59,8 +60,12 @@ __device__ void deliver_block(const nemo_propagate_args& a, uint64_t t, int2* st
 
 	const unsigned CAP = a.bin_capacity;
 	const unsigned D = a.D;
+	const unsigned DC = a.d_count;
 	const unsigned tid = threadIdx.x;
 	const unsigned b = blockIdx.x;
+	const unsigned col = a.block_base + b;
+	const size_t parity = t & 1;
+	int64_t* acc = a.acc + parity * 2 * a.acc_pitch;
 	const uint2 win = a.windows[b];
 	const unsigned tlo = win.x;
 	const unsigned W = win.y - win.x; /* tiles in the window */
@@ -78,8 +83,8 @@ __device__ void deliver_block(const nemo_propagate_args& a, uint64_t t, int2* st
 		const uint64_t w = wb + tid;
 		if(w < nwords) {
 			const uint64_t q = q0 + w / 4;
-			const unsigned p = unsigned(q / D);
-			const unsigned d = unsigned(q % D) + 1;
+			const unsigned p = unsigned(q / DC);
+			const unsigned d = unsigned(q % DC) + a.d_first;
 			if(t >= d) word = a.fired_bits[((t - d) & RING_MASK) * a.fired_words + size_t(p) * 4 + (w & 3)];
 		}
 		__syncthreads(); /* previous batch done with s_word */
@@ -108,8 +113,8 @@ __device__ void deliver_block(const nemo_propagate_args& a, uint64_t t, int2* st
 					}
 					const uint64_t wg = wb + lo;
 					const uint64_t q = q0 + wg / 4;
-					const unsigned p = unsigned(q / D);
-					const unsigned d = unsigned(q % D) + 1;
+					const unsigned p = unsigned(q / DC);
+					const unsigned d = unsigned(q % DC) + a.d_first;
 					const unsigned bit = kth_set_bit(s_word[lo], r - s_wpre[lo]);
 					const uint32_t n = p * NEMO_TILE + unsigned(wg & 3) * 64 + bit;
 					const size_t row = size_t(n) * D + (d - 1);
@@ -159,7 +164,7 @@ __device__ void deliver_block(const nemo_propagate_args& a, uint64_t t, int2* st
 							/* beyond capacity (or, defensively, outside the
 							 * window): straight into the accumulators */
 							if(pos < CAP) stage[pos] = make_int2(-1, 0);
-							int64_t* dst = a.acc + (s[u].y >= 0 ? size_t(0) : a.acc_pitch) + uint32_t(s[u].x);
+							int64_t* dst = acc + (s[u].y >= 0 ? size_t(0) : a.acc_pitch) + uint32_t(s[u].x);
 							atomicAdd(reinterpret_cast<unsigned long long*>(dst),
 									(unsigned long long)(int64_t)s[u].y);
 						}
@@ -182,19 +187,19 @@ __device__ void deliver_block(const nemo_propagate_args& a, uint64_t t, int2* st
 		const uint32_t pre = nemo_dev::block_exclusive_scan(c, &total);
 		if(p < W) {
 			cur[p] = carry + pre;
-			a.bin_table[size_t(tlo + p) * a.deliver_blocks + b] = (c << 16) | (carry + pre);
+			a.bin_table[(parity * a.tiles + tlo + p) * a.table_columns + col] = (c << 16) | (carry + pre);
 		}
 		carry += total;
 	}
 	__syncthreads();
-	int2* out = a.bins + size_t(b) * CAP;
+	int2* out = a.bins + (parity * a.table_columns + col) * CAP;
 	for(uint32_t i = tid; i < staged; i += NEMO_BLOCK) {
 		const int2 e = stage[i];
 		if(e.x < 0) continue;
 		const uint32_t pos = atomicAdd(&cur[uint32_t(e.x) / NEMO_TILE - tlo], 1u);
 		out[pos] = make_int2(int(uint32_t(e.x) % NEMO_TILE), e.y);
 	}
-	if(tid == 0 && events) a.block_events[b] += events;
+	if(tid == 0 && events) a.block_events[col] += events;
 }
 
 /* ---------------------------------------------------------------- STDP */
@@ -290,25 +295,35 @@ __device__ void stdp_block(const nemo_propagate_args& a, uint64_t r, unsigned sb
 	if(tid == 0) st.examined[sb] += total;
 }
 
+/* STDP accumulation of step t-1 by STDP block sb, unless it is not due */
+__device__ inline void stdp_step(const nemo_propagate_args& a, uint64_t t, unsigned sb)
+{
+	if(t == 0) return;
+	const uint64_t r = t - 1;
+	if(r == a.clock->stdp_flushed || r < a.stdp.post_window) return;
+	stdp_block(a, r, sb);
+}
+
+/* delivery blocks, then (when fused into this launch) STDP blocks */
 __global__ void __launch_bounds__(NEMO_BLOCK)
 propagate_kernel(nemo_propagate_args a)
 {
 	extern __shared__ int2 lds_dyn[];
 	const uint64_t t = a.clock->base_step + a.step_offset;
-	int2* stage = lds_dyn;
-	uint32_t* cnt = reinterpret_cast<uint32_t*>(lds_dyn + a.bin_capacity);
-	deliver_block(a, t, stage, cnt, cnt + a.max_window);
+	if(blockIdx.x < a.deliver_blocks) {
+		int2* stage = lds_dyn;
+		uint32_t* cnt = reinterpret_cast<uint32_t*>(lds_dyn + a.bin_capacity);
+		deliver_block(a, t, stage, cnt, cnt + a.max_window);
+	} else {
+		stdp_step(a, t, blockIdx.x - a.deliver_blocks);
+	}
 }
 
-/* STDP accumulation of step t-1, unless it is not due */
+/* STDP blocks only */
 __global__ void __launch_bounds__(NEMO_BLOCK)
 stdp_kernel(nemo_propagate_args a)
 {
-	const uint64_t t = a.clock->base_step + a.step_offset;
-	if(t == 0) return;
-	const uint64_t r = t - 1;
-	if(r == a.clock->stdp_flushed || r < a.stdp.post_window) return;
-	stdp_block(a, r, blockIdx.x);
+	stdp_step(a, a.clock->base_step + a.step_offset, blockIdx.x);
 }
 
 __global__ void __launch_bounds__(NEMO_BLOCK)
@@ -378,17 +393,26 @@ size_t nemo_propagate_lds_bytes(unsigned max_window, unsigned bin_capacity)
 	return size_t(bin_capacity) * sizeof(int2) + 2 * size_t(max_window) * sizeof(uint32_t);
 }
 
-hipError_t nemo_launch_propagate(hipStream_t st, const nemo_propagate_args* a, unsigned stdp_blocks)
+hipError_t nemo_launch_propagate(hipStream_t st, const nemo_propagate_args* a, unsigned stdp_blocks, int fuse_stdp)
 {
 	if(a->deliver_blocks > NEMO_MAX_BIN_BLOCKS) return hipErrorInvalidValue;
 	if(a->deliver_blocks && (a->D == 0 || a->D > NEMO_MAX_DELAY || a->bin_capacity == 0
 				|| a->bin_capacity > 32768 || a->max_window == 0 || a->max_window > a->tiles
-				|| !a->pair_bounds || !a->windows)) {
+				|| !a->pair_bounds || !a->windows || a->d_first == 0 || a->d_count == 0
+				|| a->d_first + a->d_count - 1 > a->D || a->block_base + a->deliver_blocks > a->table_columns
+				|| a->table_columns > NEMO_MAX_BIN_BLOCKS)) {
 		return hipErrorInvalidValue;
 	}
 	if(stdp_blocks && (a->stdp.post_window > 63 || a->stdp_words == 0 || a->stdp_words > 4
 				|| size_t(stdp_blocks) * a->stdp_words < a->fired_words || !a->stdp.rcm_ptr || !a->hist)) {
 		return hipErrorInvalidValue;
+	}
+	const size_t lds = a->deliver_blocks ? nemo_propagate_lds_bytes(a->max_window, a->bin_capacity) : 0;
+	if(fuse_stdp) {
+		const unsigned blocks = a->deliver_blocks + stdp_blocks;
+		if(blocks == 0) return hipSuccess;
+		hipLaunchKernelGGL(propagate_kernel, dim3(blocks), dim3(NEMO_BLOCK), lds, st, *a);
+		return hipGetLastError();
 	}
 	if(stdp_blocks) {
 		hipLaunchKernelGGL(stdp_kernel, dim3(stdp_blocks), dim3(NEMO_BLOCK), 0, st, *a);
@@ -396,7 +420,6 @@ hipError_t nemo_launch_propagate(hipStream_t st, const nemo_propagate_args* a, u
 		if(e != hipSuccess) return e;
 	}
 	if(a->deliver_blocks) {
-		const size_t lds = nemo_propagate_lds_bytes(a->max_window, a->bin_capacity);
 		hipLaunchKernelGGL(propagate_kernel, dim3(a->deliver_blocks), dim3(NEMO_BLOCK), lds, st, *a);
 		return hipGetLastError();
 	}

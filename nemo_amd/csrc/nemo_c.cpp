@@ -485,33 +485,37 @@ nemo_status_t nemo_hip_prepare(nemo_simulation_t s, unsigned nsteps, unsigned st
 	return guarded([&] { check(s, "simulation"); s->impl.prepare(nsteps, stdp_period); });
 }
 
-static void shapeOut(const nemo_hip::LaunchShape& l, unsigned shape[4])
+static void shapeOut(const nemo_hip::LaunchShape& l, unsigned shape[NEMO_HIP_SHAPE_FIELDS])
 {
 	shape[0] = l.deliverBlocks;
-	shape[1] = l.binCapacity;
-	shape[2] = l.fuseStdp;
-	shape[3] = l.stdpWords;
+	shape[1] = l.lateBlocks;
+	shape[2] = l.binCapacity;
+	shape[3] = l.stdpMode;
+	shape[4] = l.stdpWords;
+	shape[5] = l.overlap;
 }
 
-nemo_status_t nemo_hip_get_launch_shape(nemo_simulation_t s, unsigned shape[4])
+nemo_status_t nemo_hip_get_launch_shape(nemo_simulation_t s, unsigned shape[NEMO_HIP_SHAPE_FIELDS])
 {
 	return guarded([&] { check(s, "simulation"); shapeOut(s->impl.launchShape(), shape); });
 }
 
-nemo_status_t nemo_hip_set_launch_shape(nemo_simulation_t s, const unsigned shape[4])
+nemo_status_t nemo_hip_set_launch_shape(nemo_simulation_t s, const unsigned shape[NEMO_HIP_SHAPE_FIELDS])
 {
 	return guarded([&] {
 		check(s, "simulation");
 		nemo_hip::LaunchShape l;
 		l.deliverBlocks = shape[0];
-		l.binCapacity = shape[1];
-		l.fuseStdp = shape[2];
-		l.stdpWords = shape[3];
+		l.lateBlocks = shape[1];
+		l.binCapacity = shape[2];
+		l.stdpMode = shape[3];
+		l.stdpWords = shape[4];
+		l.overlap = shape[5];
 		s->impl.setLaunchShape(l);
 	});
 }
 
-nemo_status_t nemo_hip_autotune(nemo_simulation_t s, unsigned steps_per_trial, unsigned shape[4])
+nemo_status_t nemo_hip_autotune(nemo_simulation_t s, unsigned steps_per_trial, unsigned shape[NEMO_HIP_SHAPE_FIELDS])
 {
 	return guarded([&] {
 		check(s, "simulation");

@@ -60,12 +60,13 @@ struct TileInput {
 	long long I[NEMO_TILE];
 };
 
-/* Sum the tile's binned events from every delivery block into LDS (integer
- * sums, so the order of the 64-bit LDS atomics does not matter). */
+/* Sum the tile's binned events of this step from the delivery blocks of both
+ * block sets into LDS (integer sums, so the order of the 64-bit LDS atomics
+ * does not matter). */
 __device__ inline void gather_input(const nemo_hip_step_buffers& b, const nemo_hip_neuron_group& g, TileInput& in)
 {
 	__shared__ uint32_t s_pre[NEMO_MAX_BIN_BLOCKS + 1];
-	__shared__ uint32_t s_off[NEMO_MAX_BIN_BLOCKS];
+	__shared__ uint32_t s_base[NEMO_MAX_BIN_BLOCKS];
 	const unsigned tid = threadIdx.x;
 	in.E[tid] = 0;
 	in.I[tid] = 0;
@@ -73,21 +74,27 @@ __device__ inline void gather_input(const nemo_hip_step_buffers& b, const nemo_h
 		__syncthreads();
 		return;
 	}
-	/* the delivery blocks [blo, blo + B) whose windows hold this tile */
 	const unsigned tile = g.start / NEMO_TILE + blockIdx.x;
-	const uint32_t range = b.bin_range[tile];
-	const unsigned blo = range & 0xffffu;
-	const unsigned B = (range >> 16) - blo;
-	const uint32_t* row = b.bin_table + size_t(tile) * b.bin_blocks + blo;
+	const size_t parity = current_step(b) & 1;
+	/* the table columns [lo0, lo0 + B0) and [lo1, lo1 + B1) hold this tile */
+	const uint32_t r0 = b.bin_range[tile];
+	const uint32_t r1 = b.bin_range[b.tiles + tile];
+	const unsigned lo0 = r0 & 0xffffu, B0 = (r0 >> 16) - lo0;
+	const unsigned lo1 = r1 & 0xffffu, B1 = (r1 >> 16) - lo1;
+	const unsigned B = B0 + B1;
+	const uint32_t* row = b.bin_table + (parity * b.tiles + tile) * b.bin_blocks;
+	const size_t cap = b.bin_capacity;
+	const int2* bins = b.bins + parity * b.bin_blocks * cap;
 	constexpr unsigned PER = NEMO_MAX_BIN_BLOCKS / NEMO_TILE;
 	uint32_t cnt[PER];
 	uint32_t mine = 0;
 #pragma unroll
 	for(unsigned k = 0; k < PER; ++k) {
 		const unsigned i = tid * PER + k;
-		const uint32_t e = i < B ? row[i] : 0u;
+		const unsigned col = i < B0 ? lo0 + i : lo1 + (i - B0);
+		const uint32_t e = i < B ? row[col] : 0u;
 		cnt[k] = e >> 16;
-		if(i < B) s_off[i] = e & 0xffffu;
+		if(i < B) s_base[i] = uint32_t(col * cap + (e & 0xffffu));
 		mine += cnt[k];
 	}
 	uint32_t total;
@@ -100,7 +107,6 @@ __device__ inline void gather_input(const nemo_hip_step_buffers& b, const nemo_h
 	}
 	if(tid == 0) s_pre[B] = total;
 	__syncthreads();
-	const size_t cap = b.bin_capacity;
 	/* GATHER_ILP independent loads in flight per thread before the LDS atomics */
 	constexpr int GATHER_ILP = 8;
 	for(uint32_t j0 = tid; j0 < total; j0 += GATHER_ILP * NEMO_TILE) {
@@ -115,7 +121,7 @@ __device__ inline void gather_input(const nemo_hip_step_buffers& b, const nemo_h
 					const unsigned mid = (lo + hi) >> 1;
 					if(s_pre[mid] <= j) lo = mid; else hi = mid;
 				}
-				ev[u] = b.bins[size_t(blo + lo) * cap + s_off[lo] + (j - s_pre[lo])];
+				ev[u] = bins[s_base[lo] + (j - s_pre[lo])];
 			}
 		}
 #pragma unroll
@@ -145,10 +151,11 @@ __device__ inline void take_current(const nemo_hip_step_buffers& b, const TileIn
 {
 	int64_t e = in.E[threadIdx.x];
 	int64_t i = in.I[threadIdx.x];
-	const int64_t oe = b.acc[ng];
-	const int64_t oi = b.acc[b.acc_pitch + ng];
-	if(oe) { e += oe; b.acc[ng] = 0; }
-	if(oi) { i += oi; b.acc[b.acc_pitch + ng] = 0; }
+	int64_t* acc = b.acc + (current_step(b) & 1) * 2 * b.acc_pitch;
+	const int64_t oe = acc[ng];
+	const int64_t oi = acc[b.acc_pitch + ng];
+	if(oe) { e += oe; acc[ng] = 0; }
+	if(oi) { i += oi; acc[b.acc_pitch + ng] = 0; }
 	E = wfx_to_float(e, b.fbits);
 	I = wfx_to_float(i, b.fbits);
 	if(b.debug_current) {

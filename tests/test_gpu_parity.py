@@ -342,11 +342,13 @@ def test_batch_matches_solo_runs():
 
 
 def test_launch_shapes_give_identical_results():
-    """Delivery blocks, bin capacity (down to 64 events, so that most events
-    take the overflow path), STDP fusion and STDP block width change only
+    """Delivery blocks of both sets, bin capacity (down to 64 events, so that
+    most events take the overflow path), STDP placement and block width, and
+    the overlap of the next step's delivery with the update change only
     speed: rasters, weights and accumulators are identical."""
     spec = gen.random_izhikevich(4000, 150, dmax=15, seed=21, stdp=True)
-    shapes = [None, (1, 64, 1, 1), (7, 256, 0, 4), (200, 1024, 0, 2), (1024, 4096, 1, 1), (33, 2048, 1, 3)]
+    shapes = [None, (1, 1, 64, 0, 1, 1), (7, 3, 256, 1, 4, 0), (200, 0, 1024, 2, 2, 0),
+              (900, 124, 4096, 2, 1, 1), (33, 5, 2048, 1, 3, 1), (64, 0, 512, 0, 2, 0)]
     results = []
     for shape in shapes:
         sim = sc.make(nemo_amd, spec, _conf(nemo_amd, stdp_fn()))
@@ -354,7 +356,9 @@ def test_launch_shapes_give_identical_results():
             sim.launch_shape = shape
             assert sim.launch_shape == shape
         c, f, e = sim.run(250, stdp_period=100)
-        results.append((c, f, e, sim.fixed_weights(), sim.stdp_accumulators()))
+        c2, f2, e2 = sim.run(67)  # chunks of 64 and 3 steps
+        results.append((np.concatenate([c, c2]), np.concatenate([f, f2]), e + e2, sim.fixed_weights(),
+                        sim.stdp_accumulators()))
     for r in results[1:]:
         for x, y in zip(results[0], r):
             assert np.array_equal(x, y)
