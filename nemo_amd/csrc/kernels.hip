@@ -34,9 +34,10 @@ constexpr uint64_t RING_MASK = NEMO_STEP_RING - 1;
 /* fired rows a delivery block lists in LDS at a time */
 constexpr unsigned ROWS = 1024;
 constexpr unsigned ROWS_PER_THREAD = ROWS / NEMO_BLOCK;
-/* independent global loads each thread keeps in flight in the event and
- * synapse loops */
+/* independent global loads each thread keeps in flight in the synapse loop
+ * of STDP, and in the event loop of delivery */
 constexpr int ILP = 8;
+constexpr int EVENT_ILP = 16;
 
 __device__ inline unsigned kth_set_bit(uint64_t w, unsigned k)
 {
@@ -100,28 +101,35 @@ __device__ void deliver_block(const nemo_propagate_args& a, uint64_t t, int2* st
 			/* row pointers of up to ROWS fired rows, ROWS_PER_THREAD per thread */
 			uint32_t lens[ROWS_PER_THREAD];
 			uint32_t mine = 0;
+			uint32_t rr[ROWS_PER_THREAD];
+			unsigned wl[ROWS_PER_THREAD];
+#pragma unroll
+			for(unsigned k = 0; k < ROWS_PER_THREAD; ++k) rr[k] = r0 + tid * ROWS_PER_THREAD + k;
+			nemo_dev::last_not_above_k<NEMO_BLOCK>(s_wpre, NEMO_BLOCK, rr, wl);
+			uint32_t rbv[ROWS_PER_THREAD], rev[ROWS_PER_THREAD];
 #pragma unroll
 			for(unsigned k = 0; k < ROWS_PER_THREAD; ++k) {
 				const uint32_t ri = tid * ROWS_PER_THREAD + k;
-				lens[k] = 0;
+				rbv[k] = 0;
+				rev[k] = 0;
 				if(ri < nr) {
-					const uint32_t r = r0 + ri;
-					unsigned lo = 0, hi = NEMO_BLOCK; /* s_wpre[lo] <= r < s_wpre[hi] */
-					while(hi - lo > 1) {
-						const unsigned mid = (lo + hi) >> 1;
-						if(s_wpre[mid] <= r) lo = mid; else hi = mid;
-					}
+					const unsigned lo = wl[k];
 					const uint64_t wg = wb + lo;
 					const uint64_t q = q0 + wg / 4;
 					const unsigned p = unsigned(q / DC);
 					const unsigned d = unsigned(q % DC) + a.d_first;
-					const unsigned bit = kth_set_bit(s_word[lo], r - s_wpre[lo]);
+					const unsigned bit = kth_set_bit(s_word[lo], rr[k] - s_wpre[lo]);
 					const uint32_t n = p * NEMO_TILE + unsigned(wg & 3) * 64 + bit;
 					const size_t row = size_t(n) * D + (d - 1);
-					const uint32_t rb = a.rowptr[row];
-					lens[k] = a.rowptr[row + 1] - rb;
-					s_rb[ri] = rb;
+					rbv[k] = a.rowptr[row];
+					rev[k] = a.rowptr[row + 1];
 				}
+			}
+#pragma unroll
+			for(unsigned k = 0; k < ROWS_PER_THREAD; ++k) {
+				const uint32_t ri = tid * ROWS_PER_THREAD + k;
+				lens[k] = rev[k] - rbv[k];
+				if(ri < nr) s_rb[ri] = rbv[k];
 				mine += lens[k];
 			}
 			uint32_t M;
@@ -136,23 +144,20 @@ __device__ void deliver_block(const nemo_propagate_args& a, uint64_t t, int2* st
 			__syncthreads();
 
 			/* the events of these rows, evenly over the block */
-			for(uint32_t e0 = tid; e0 < M; e0 += ILP * NEMO_BLOCK) {
-				int2 s[ILP];
+			for(uint32_t e0 = tid; e0 < M; e0 += EVENT_ILP * NEMO_BLOCK) {
+				uint32_t ee[EVENT_ILP];
+				unsigned lo[EVENT_ILP];
 #pragma unroll
-				for(int u = 0; u < ILP; ++u) {
-					const uint32_t e = e0 + u * NEMO_BLOCK;
+				for(int u = 0; u < EVENT_ILP; ++u) ee[u] = e0 + u * NEMO_BLOCK;
+				nemo_dev::last_not_above_k<ROWS>(s_rpre, nr, ee, lo);
+				int2 s[EVENT_ILP];
+#pragma unroll
+				for(int u = 0; u < EVENT_ILP; ++u) {
 					s[u] = make_int2(0, 0);
-					if(e < M) {
-						unsigned lo = 0, hi = nr; /* s_rpre[lo] <= e < s_rpre[hi] */
-						while(hi - lo > 1) {
-							const unsigned mid = (lo + hi) >> 1;
-							if(s_rpre[mid] <= e) lo = mid; else hi = mid;
-						}
-						s[u] = a.fcm[s_rb[lo] + (e - s_rpre[lo])];
-					}
+					if(ee[u] < M) s[u] = a.fcm[s_rb[lo[u]] + (ee[u] - s_rpre[lo[u]])];
 				}
 #pragma unroll
-				for(int u = 0; u < ILP; ++u) {
+				for(int u = 0; u < EVENT_ILP; ++u) {
 					const uint32_t e = e0 + u * NEMO_BLOCK;
 					if(e < M) {
 						const uint32_t pos = fill + e;
@@ -193,11 +198,24 @@ __device__ void deliver_block(const nemo_propagate_args& a, uint64_t t, int2* st
 	}
 	__syncthreads();
 	int2* out = a.bins + (parity * a.table_columns + col) * CAP;
-	for(uint32_t i = tid; i < staged; i += NEMO_BLOCK) {
-		const int2 e = stage[i];
-		if(e.x < 0) continue;
-		const uint32_t pos = atomicAdd(&cur[uint32_t(e.x) / NEMO_TILE - tlo], 1u);
-		out[pos] = make_int2(int(uint32_t(e.x) % NEMO_TILE), e.y);
+	/* four staged events per thread at a time, so that their LDS atomics
+	 * (which return the slot) overlap */
+	for(uint32_t i0 = tid; i0 < staged; i0 += 4 * NEMO_BLOCK) {
+		int2 e[4];
+		uint32_t pos[4];
+#pragma unroll
+		for(int u = 0; u < 4; ++u) {
+			const uint32_t i = i0 + u * NEMO_BLOCK;
+			e[u] = i < staged ? stage[i] : make_int2(-1, 0);
+		}
+#pragma unroll
+		for(int u = 0; u < 4; ++u) {
+			if(e[u].x >= 0) pos[u] = atomicAdd(&cur[uint32_t(e[u].x) / NEMO_TILE - tlo], 1u);
+		}
+#pragma unroll
+		for(int u = 0; u < 4; ++u) {
+			if(e[u].x >= 0) out[pos[u]] = make_int2(int(uint32_t(e[u].x) % NEMO_TILE), e[u].y);
+		}
 	}
 	if(tid == 0 && events) a.block_events[col] += events;
 }
@@ -254,18 +272,17 @@ __device__ void stdp_block(const nemo_propagate_args& a, uint64_t r, unsigned sb
 		uint32_t sd[ILP];
 		int32_t acc[ILP];
 #pragma unroll
+		for(int u = 0; u < ILP; ++u) k[u] = e0 + u * NEMO_BLOCK;
+		unsigned lo[ILP];
+		nemo_dev::last_not_above_k<NEMO_BLOCK>(s_pos, nfired, k, lo);
+#pragma unroll
 		for(int u = 0; u < ILP; ++u) {
-			const uint32_t e = e0 + u * NEMO_BLOCK;
+			const uint32_t e = k[u];
 			k[u] = 0;
 			sd[u] = 0;
 			acc[u] = 0;
 			if(e < total) {
-				unsigned lo = 0, hi = nfired; /* s_pos[lo] <= e < s_pos[hi] */
-				while(hi - lo > 1) {
-					const unsigned mid = (lo + hi) >> 1;
-					if(s_pos[mid] <= e) lo = mid; else hi = mid;
-				}
-				k[u] = s_rb[lo] + (e - s_pos[lo]);
+				k[u] = s_rb[lo[u]] + (e - s_pos[lo[u]]);
 				sd[u] = st.rcm_sd[k[u]];
 				acc[u] = st.rcm_acc[k[u]];
 			}

@@ -54,6 +54,44 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t x, uint32_t* total)
 	return before + v - x;
 }
 
+/* The largest i < n with arr[i] <= x, for a non-decreasing arr with
+ * arr[0] <= x and n <= MAXN (a power of two): a fixed number of unrolled
+ * steps, so that searches of independent items interleave. */
+template<unsigned MAXN>
+__device__ inline unsigned last_not_above(const uint32_t* arr, unsigned n, uint32_t x)
+{
+	unsigned lo = 0;
+#pragma unroll
+	for(unsigned step = MAXN / 2; step >= 1; step >>= 1) {
+		const unsigned probe = lo + step;
+		if(probe < n && arr[probe] <= x) lo = probe;
+	}
+	return lo;
+}
+
+/* last_not_above for K items at once, step by step in lock-step, so that
+ * the K LDS reads of a step are in flight together.  Requires n >= 1. */
+template<unsigned MAXN, int K>
+__device__ inline void last_not_above_k(const uint32_t* arr, unsigned n, const uint32_t (&x)[K], unsigned (&lo)[K])
+{
+#pragma unroll
+	for(int u = 0; u < K; ++u) lo[u] = 0;
+#pragma unroll
+	for(unsigned step = MAXN / 2; step >= 1; step >>= 1) {
+		uint32_t v[K];
+#pragma unroll
+		for(int u = 0; u < K; ++u) {
+			const unsigned probe = lo[u] + step;
+			v[u] = arr[probe < n ? probe : n - 1];
+		}
+#pragma unroll
+		for(int u = 0; u < K; ++u) {
+			const unsigned probe = lo[u] + step;
+			lo[u] = (probe < n && v[u] <= x[u]) ? probe : lo[u];
+		}
+	}
+}
+
 /* Synaptic input of one tile for the current step. */
 struct TileInput {
 	long long E[NEMO_TILE];
@@ -110,19 +148,16 @@ __device__ inline void gather_input(const nemo_hip_step_buffers& b, const nemo_h
 	/* GATHER_ILP independent loads in flight per thread before the LDS atomics */
 	constexpr int GATHER_ILP = 8;
 	for(uint32_t j0 = tid; j0 < total; j0 += GATHER_ILP * NEMO_TILE) {
+		uint32_t jj[GATHER_ILP];
+		unsigned lo[GATHER_ILP];
+#pragma unroll
+		for(int u = 0; u < GATHER_ILP; ++u) jj[u] = j0 + u * NEMO_TILE;
+		last_not_above_k<NEMO_MAX_BIN_BLOCKS>(s_pre, B, jj, lo);
 		int2 ev[GATHER_ILP];
 #pragma unroll
 		for(int u = 0; u < GATHER_ILP; ++u) {
-			const uint32_t j = j0 + u * NEMO_TILE;
 			ev[u] = make_int2(0, 0);
-			if(j < total) {
-				unsigned lo = 0, hi = B; /* s_pre[lo] <= j < s_pre[hi] */
-				while(hi - lo > 1) {
-					const unsigned mid = (lo + hi) >> 1;
-					if(s_pre[mid] <= j) lo = mid; else hi = mid;
-				}
-				ev[u] = bins[s_base[lo] + (j - s_pre[lo])];
-			}
+			if(jj[u] < total) ev[u] = bins[s_base[lo[u]] + (jj[u] - s_pre[lo[u]])];
 		}
 #pragma unroll
 		for(int u = 0; u < GATHER_ILP; ++u) {
@@ -299,18 +334,17 @@ __device__ inline void stdp_tile(const nemo_hip_step_buffers& b, unsigned ng, bo
 		uint32_t sd[ILP];
 		int32_t acc[ILP];
 #pragma unroll
+		for(int u = 0; u < ILP; ++u) k[u] = e0 + u * NEMO_TILE;
+		unsigned lo[ILP];
+		last_not_above_k<NEMO_TILE>(s_pos, nfired, k, lo);
+#pragma unroll
 		for(int u = 0; u < ILP; ++u) {
-			const uint32_t e = e0 + u * NEMO_TILE;
+			const uint32_t e = k[u];
 			k[u] = 0;
 			sd[u] = 0;
 			acc[u] = 0;
 			if(e < total) {
-				unsigned lo = 0, hi = nfired; /* s_pos[lo] <= e < s_pos[hi] */
-				while(hi - lo > 1) {
-					const unsigned mid = (lo + hi) >> 1;
-					if(s_pos[mid] <= e) lo = mid; else hi = mid;
-				}
-				k[u] = s_rb[lo] + (e - s_pos[lo]);
+				k[u] = s_rb[lo[u]] + (e - s_pos[lo[u]]);
 				sd[u] = st.rcm_sd[k[u]];
 				acc[u] = st.rcm_acc[k[u]];
 			}
