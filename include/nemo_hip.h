@@ -216,10 +216,15 @@ nemo_status_t nemo_hip_prepare(nemo_simulation_t, unsigned nsteps, unsigned stdp
  * most 1024), shape[2] events a delivery block stages in LDS (64..32768),
  * shape[3] where the STDP accumulation runs: 0 a launch of its own on a graph
  * branch beside the delivery, 1 inside the neuron-update kernels, 2 in the
- * delivery launch that follows the previous update; shape[4] firing-bitmap
- * words per STDP block (1..4), shape[5] 1 to deliver the delays 2..D of the
- * next step beside the current update within nemo_hip_run (when split).
- * Every shape gives identical results. */
+ * delivery launch that follows the previous update, 3 a launch of its own on
+ * a graph branch that may run until the next step's update, 4 in one launch
+ * for up to 64 steps after them (every run of steps ends with it); shape[4] firing-bitmap
+ * words per STDP block (1..8; at most 4 when the update launches carry the
+ * STDP blocks), shape[5] when split, how the delays 2..D of the next step are
+ * delivered beside the current update within nemo_hip_run: 0 not at all, 1 on
+ * a graph branch, 2 as extra blocks of the update launch (which then also
+ * carries the STDP blocks of STDP mode 2).  Every shape gives identical
+ * results. */
 #define NEMO_HIP_SHAPE_FIELDS 6
 nemo_status_t nemo_hip_get_launch_shape(nemo_simulation_t, unsigned shape[NEMO_HIP_SHAPE_FIELDS]);
 nemo_status_t nemo_hip_set_launch_shape(nemo_simulation_t, const unsigned shape[NEMO_HIP_SHAPE_FIELDS]);

@@ -29,9 +29,10 @@
  *   - apply one 1 ms step of the model; a neuron fires if the model says so or
  *     if its bit in fstim is set (fstim may be NULL);
  *   - shift the neuron's 64-bit firing history left by one, OR-ing in firing:
- *     the history after step s is in plane s % 2 of hist (plane p starts at
- *     p * hist_pitch), so the update reads plane (s-1) % 2 and writes plane
- *     s % 2;
+ *     the history after step s is in plane s & hist_mask of hist (plane p
+ *     starts at p * hist_pitch; hist_mask + 1 planes, a power of two, at
+ *     least 2), so the update reads plane (s-1) & hist_mask and writes plane
+ *     s & hist_mask;
  *   - write the tile's words of the step's firing bitmap,
  *     fired_bits[(s % NEMO_STEP_RING) * fired_words + index / 64];
  *   - when stdp.rcm_ptr is not NULL, run the STDP accumulation of step s-1 for
@@ -101,6 +102,79 @@ typedef struct nemo_hip_stdp {
 	int32_t fn_post[64];
 } nemo_hip_stdp;
 
+/* Per step, before the neuron updates:
+ *
+ * Spike delivery of step t (propagate_kernel), binned by target tile (the
+ * update kernels gather the bins), in two launches over two block sets: the
+ * delays 2..D (which only need firing up to step t-2, so this launch may run
+ * beside the update of step t-1) and the delay 1.  For every delay d of the
+ * set and every neuron n that fired at step t-d, the synapse row (n, d) is
+ * delivered: each event {target, w} is counting-sorted in LDS by target tile
+ * and written to the block's bin region, bin_table[t % 2][tile][column] =
+ * count << 16 | offset, column = block_base + block.  Events beyond the
+ * block's bin capacity are added with global atomics into acc plane t % 2 (E
+ * if w >= 0, else I).  Block b of a set covers the (source tile, delay) pairs
+ * q in [pair_bounds[b], pair_bounds[b+1]), q = tile * d_count + d - d_first,
+ * and every target of those rows lies in its tile window [windows[b].x,
+ * windows[b].y); it writes the table entries of exactly the tiles of its
+ * window.  Semantics of cpu::Simulation::deliverSpikes (cpu/Simulation.cpp:
+ * 259-308).
+ *
+ * STDP accumulation of step t-1 for the neurons that fired at step t-1-P
+ * (the contract of nemo_hip_stdp) may share the launch that delivers step t
+ * after the update of step t-1 (blocks after the delivery blocks), run as a
+ * launch of its own (stdp_kernel), or be left to the update kernels.  It
+ * reads the firing history after step t-1, plane (t-1) & hist_mask of hist.
+ * The accumulation of up to 64 past steps may also run in one batch launch
+ * after them (stdp_batch_kernel), from the history after the last of them and
+ * hist_snap.  Skipped
+ * when t == 0 or when step t-1's accumulation already ran
+ * (clock->stdp_flushed == t-1). */
+typedef struct nemo_propagate_args {
+	const nemo_hip_clock* clock;
+	uint64_t step_offset;            /* t = clock->base_step + step_offset */
+	const uint64_t* fired_bits;      /* NEMO_STEP_RING bitmaps of fired_words words */
+	size_t fired_words;
+	/* delivery: one block set */
+	unsigned deliver_blocks;         /* blocks of the set */
+	unsigned block_base;             /* table column (and bin region) of its first block */
+	unsigned table_columns;          /* blocks of both sets */
+	unsigned d_first, d_count;       /* delays of the set */
+	const uint32_t* pair_bounds;     /* [deliver_blocks+1] first pair of each block */
+	const uint2* windows;            /* [deliver_blocks] target tiles [x, y) of each block */
+	unsigned max_window;             /* largest window (tiles) */
+	const uint32_t* rowptr;          /* (neurons*D)+1 offsets, row (n,d) = n*D + d-1 */
+	const int2* fcm;                 /* {target device index, Q11.20 weight} */
+	unsigned D;                      /* rows per source (network max delay) */
+	unsigned tiles;
+	unsigned bin_capacity;           /* events per block region */
+	uint32_t* bin_table;             /* [2][tiles][table_columns] */
+	int2* bins;                      /* [2][table_columns][bin_capacity] {target % NEMO_TILE, w} */
+	int64_t* acc;                    /* [2][2][acc_pitch] overflow accumulators, E plane then I plane */
+	size_t acc_pitch;
+	unsigned long long* block_events; /* [table_columns] events delivered, accumulated */
+	/* STDP in its own launch */
+	unsigned stdp_words;             /* bitmap words (64 targets) per STDP block */
+	const uint64_t* hist;            /* hist_mask + 1 planes of hist_pitch */
+	size_t hist_pitch;
+	unsigned hist_mask;
+	uint64_t* hist_snap;             /* STDP batches: history after the step before the batch */
+	nemo_hip_stdp stdp;              /* examined: one counter per STDP block */
+} nemo_propagate_args;
+
+/* Propagate blocks an update launch carries after its tiles: the delivery
+ * blocks of args (args.step_offset names their step, the next one) and then
+ * stdp_blocks STDP blocks (of args' step minus two, the step before the
+ * update's).  They need lds_bytes of dynamic LDS, which the launch requests
+ * for all its blocks.  Both counts are zero when the launch carries none;
+ * only the launch of the first neuron group of a step carries any. */
+typedef struct nemo_hip_colaunch {
+	unsigned deliver_blocks;
+	unsigned stdp_blocks;
+	unsigned lds_bytes;
+	nemo_propagate_args args;
+} nemo_hip_colaunch;
+
 /* Per-simulation buffers shared by all groups in a step. */
 typedef struct nemo_hip_step_buffers {
 	uint64_t step_offset;      /* step = clock->base_step + step_offset */
@@ -119,13 +193,15 @@ typedef struct nemo_hip_step_buffers {
 	size_t acc_pitch;
 	float* ext_current;        /* per device index, or NULL */
 	const uint32_t* fstim;     /* firing-stimulus bit-vector, or NULL */
-	uint64_t* hist;            /* 64-bit firing history: two planes of hist_pitch */
+	uint64_t* hist;            /* 64-bit firing history: hist_mask + 1 planes of hist_pitch */
 	size_t hist_pitch;
+	unsigned hist_mask;
 	uint64_t* fired_bits;      /* NEMO_STEP_RING bitmaps of fired_words words */
 	size_t fired_words;
 	unsigned fbits;            /* fractional bits of the fixed-point currents */
 	float* debug_current;      /* NULL, or 2 planes of acc_pitch floats receiving E and I */
 	nemo_hip_stdp stdp;
+	nemo_hip_colaunch co;      /* extra blocks of this launch (the first group's only) */
 } nemo_hip_step_buffers;
 
 typedef hipError_t hip_update_neurons_t(hipStream_t stream,

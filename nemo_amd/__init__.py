@@ -309,8 +309,11 @@ class Simulation(_SynapseQueries):
     def launch_shape(self):
         """(delivery blocks for delays 2..D -- all delays when the next is 0 --,
         delivery blocks for delay 1, bin capacity, STDP mode (0 own branch, 1
-        inside the updates, 2 in the delivery launch), bitmap words per STDP
-        block, delays 2..D delivered beside the previous update (1) or not)."""
+        inside the updates, 2 in the delivery launch, 3 own branch joined one
+        step later, 4 batched after each run of up to 64 steps), bitmap words
+        per STDP block, and how the delays 2..D are
+        delivered beside the previous update: 0 not, 1 on a graph branch, 2 as
+        extra blocks of the update launch)."""
         a = (ctypes.c_uint * 6)()
         check(lib().nemo_hip_get_launch_shape(self._h, a))
         return tuple(a)

@@ -412,11 +412,16 @@ void Configuration::setHipBackend(int dev)
 	description = std::string("HIP backend (") + prop.name + ", " + prop.gcnArchName + ")";
 }
 
+/* LDS an update kernel declares itself, on top of what the propagate blocks
+ * it may carry need */
+static const size_t COLAUNCH_LDS_MARGIN = 24576;
+
 /* ------------------------------------------------------------- simulation */
 
 static const char* stdpModeName(unsigned mode)
 {
-	return mode == 0 ? "own branch" : mode == 1 ? "in the updates" : "in the delivery launch";
+	return mode == 0 ? "own branch" : mode == 1 ? "in the updates" : mode == 2 ? "in the delivery launch"
+			: mode == 3 ? "own branch, one step of slack" : "batched after each run of up to 64 steps";
 }
 
 Simulation::Simulation(const Network& net, const Configuration& conf)
@@ -460,6 +465,8 @@ void Simulation::construct(const std::vector<const Network*>& nets, const Config
 	HIP_CHECK(hipStreamCreateWithFlags(&m_stdpStream, hipStreamNonBlocking));
 	HIP_CHECK(hipEventCreateWithFlags(&m_stdpFork, hipEventDisableTiming));
 	HIP_CHECK(hipEventCreateWithFlags(&m_stdpDone, hipEventDisableTiming));
+	HIP_CHECK(hipEventCreateWithFlags(&m_stdpLag[0], hipEventDisableTiming));
+	HIP_CHECK(hipEventCreateWithFlags(&m_stdpLag[1], hipEventDisableTiming));
 	HIP_CHECK(hipStreamCreateWithFlags(&m_earlyStream, hipStreamNonBlocking));
 	HIP_CHECK(hipEventCreateWithFlags(&m_earlyFork, hipEventDisableTiming));
 	HIP_CHECK(hipEventCreateWithFlags(&m_earlyDone[0], hipEventDisableTiming));
@@ -506,6 +513,8 @@ Simulation::~Simulation()
 		(void) hipStreamDestroy(m_stdpStream);
 		(void) hipEventDestroy(m_stdpFork);
 		(void) hipEventDestroy(m_stdpDone);
+		(void) hipEventDestroy(m_stdpLag[0]);
+		(void) hipEventDestroy(m_stdpLag[1]);
 	}
 	if(m_pinnedScalars) (void) hipHostFree(m_pinnedScalars);
 	if(m_pinnedBits) (void) hipHostFree(m_pinnedBits);
@@ -995,9 +1004,9 @@ LaunchShape Simulation::defaultShape() const
 	env("NEMO_HIP_DELIVER_BLOCKS", 0, NEMO_MAX_BIN_BLOCKS, s.deliverBlocks);
 	env("NEMO_HIP_LATE_BLOCKS", 0, NEMO_MAX_BIN_BLOCKS, s.lateBlocks);
 	env("NEMO_HIP_BIN_CAPACITY", 64, 32768, s.binCapacity);
-	env("NEMO_HIP_STDP_MODE", 0, 2, s.stdpMode);
-	env("NEMO_HIP_STDP_WORDS", 1, 4, s.stdpWords);
-	env("NEMO_HIP_OVERLAP", 0, 1, s.overlap);
+	env("NEMO_HIP_STDP_MODE", 0, 4, s.stdpMode);
+	env("NEMO_HIP_STDP_WORDS", 1, NEMO_STDP_MAX_WORDS, s.stdpWords);
+	env("NEMO_HIP_OVERLAP", 0, 2, s.overlap);
 	return s;
 }
 
@@ -1011,7 +1020,14 @@ size_t Simulation::ldsNeeded(const LaunchShape& s) const
 		w = std::max(w, partitionPairs(m_pairWeight, m_srcWindow, m_D, m_tiles, dFirst, dCount, s.deliverBlocks).maxWindow);
 	}
 	if(split) w = std::max(w, partitionPairs(m_pairWeight, m_srcWindow, m_D, m_tiles, 1, 1, s.lateBlocks).maxWindow);
-	return nemo_propagate_lds_bytes(w, s.binCapacity);
+	size_t need = nemo_propagate_lds_bytes(w, s.binCapacity);
+	if(split && s.overlap == 2) {
+		/* the update launches carry delivery blocks: their LDS comes on top
+		 * of the update kernels' own */
+		const size_t co = nemo_colaunch_lds_bytes(w, s.binCapacity, 1) + COLAUNCH_LDS_MARGIN;
+		if(co > m_ldsBudget) need = std::max(need, co);
+	}
+	return need;
 }
 
 void Simulation::checkShape(const LaunchShape& s) const
@@ -1022,17 +1038,21 @@ void Simulation::checkShape(const LaunchShape& s) const
 	if(s.deliverBlocks == 0 && (s.lateBlocks == 0 || m_D >= 2)) {
 		throw Error(NEMO_INVALID_INPUT, "the delays delivered by the first block set need at least one block");
 	}
-	if(s.overlap > 1) {
-		throw Error(NEMO_INVALID_INPUT, "overlap flag must be 0 or 1");
+	if(s.overlap > 2) {
+		throw Error(NEMO_INVALID_INPUT, "overlap must be 0, 1 or 2");
+	}
+	if(s.lateBlocks && s.overlap == 2 && s.stdpMode == 2 && s.stdpWords > NEMO_TILE / 64) {
+		throw Error(NEMO_INVALID_INPUT, "STDP blocks carried by the update launches cover at most "
+				+ std::to_string(NEMO_TILE / 64) + " bitmap words");
 	}
 	if(s.binCapacity < 64 || s.binCapacity > 32768) {
 		throw Error(NEMO_INVALID_INPUT, "bin capacity must be in [64, 32768]");
 	}
-	if(s.stdpWords < 1 || s.stdpWords > 4) {
-		throw Error(NEMO_INVALID_INPUT, "STDP words per block must be in [1, 4]");
+	if(s.stdpWords < 1 || s.stdpWords > NEMO_STDP_MAX_WORDS) {
+		throw Error(NEMO_INVALID_INPUT, "STDP words per block must be in [1, " + std::to_string(NEMO_STDP_MAX_WORDS) + "]");
 	}
-	if(s.stdpMode > 2) {
-		throw Error(NEMO_INVALID_INPUT, "STDP mode must be 0, 1 or 2");
+	if(s.stdpMode > 4) {
+		throw Error(NEMO_INVALID_INPUT, "STDP mode must be 0, 1, 2, 3 or 4");
 	}
 }
 
@@ -1051,7 +1071,9 @@ void Simulation::applyShape(const LaunchShape& s)
 	const Partition Pe = partitionPairs(m_pairWeight, m_srcWindow, m_D, m_tiles, dFirst, dCount, Be);
 	const Partition Pl = partitionPairs(m_pairWeight, m_srcWindow, m_D, m_tiles, 1, 1, Bl);
 	const unsigned maxWindow = std::max({1u, Pe.maxWindow, Pl.maxWindow});
-	if(nemo_propagate_lds_bytes(maxWindow, s.binCapacity) > m_ldsBudget) {
+	if(nemo_propagate_lds_bytes(maxWindow, s.binCapacity) > m_ldsBudget
+			|| (split && s.overlap == 2
+				&& nemo_colaunch_lds_bytes(maxWindow, s.binCapacity, 1) + COLAUNCH_LDS_MARGIN > m_ldsBudget)) {
 		throw Error(NEMO_INVALID_INPUT, "launch shape needs more LDS than a block can have");
 	}
 	activate();
@@ -1104,6 +1126,8 @@ void Simulation::applyShape(const LaunchShape& s)
 	a.stdp_words = m_stdpBlocks ? s.stdpWords : 0;
 	a.hist = m_hist.ptr;
 	a.hist_pitch = m_accPitch;
+	a.hist_mask = m_histPlanes - 1;
+	a.hist_snap = m_histSnap.ptr;
 	a.stdp = m_stdpArgs;
 
 	m_early = a;
@@ -1120,6 +1144,20 @@ void Simulation::applyShape(const LaunchShape& s)
 	m_late.d_count = 1;
 	m_late.pair_bounds = m_pairBounds.ptr + Pe.bounds.size();
 	m_late.windows = m_windows.ptr + Be;
+	refreshHistSnap();
+}
+
+/* The STDP batch snapshot: the history after the last step taken. */
+void Simulation::refreshHistSnap()
+{
+	if(!m_histSnap.ptr) return;
+	if(m_hostStep == 0) {
+		HIP_CHECK(hipMemsetAsync(m_histSnap.ptr, 0, m_histSnap.bytes(), m_stream));
+	} else {
+		const uint64_t* plane = m_hist.ptr + size_t((m_hostStep - 1) & (m_histPlanes - 1)) * m_accPitch;
+		HIP_CHECK(hipMemcpyAsync(m_histSnap.ptr, plane, m_histSnap.bytes(), hipMemcpyDeviceToDevice, m_stream));
+	}
+	HIP_CHECK(hipStreamSynchronize(m_stream));
 }
 
 void Simulation::setLaunchShape(const LaunchShape& s)
@@ -1133,8 +1171,7 @@ void Simulation::allocateStepState()
 	const size_t N = m_npad;
 	int maxLds = 65536;
 	(void) hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, m_device);
-	/* the propagate kernel's static LDS is about 14 KB */
-	m_ldsBudget = size_t(maxLds) - 16384;
+	m_ldsBudget = size_t(maxLds) - std::min<size_t>(size_t(maxLds), nemo_propagate_static_lds());
 	m_clock.allocate(1);
 	nemo_hip_clock c0;
 	c0.base_step = 0;
@@ -1147,7 +1184,14 @@ void Simulation::allocateStepState()
 	m_accPitch = N;
 	m_acc.allocate(4 * m_accPitch); /* [step parity][E, I][N] */
 	HIP_CHECK(hipMemset(m_acc.ptr, 0, m_acc.bytes()));
-	m_hist.allocate(2 * N); /* plane s % 2 holds the history after step s */
+	/* plane s % 2 holds the history after step s; with STDP, a snapshot of
+	 * an older step lets a batch accumulate up to 64 steps afterwards */
+	m_histPlanes = 2;
+	m_hist.allocate(size_t(m_histPlanes) * N);
+	if(m_stdp && m_nplastic) {
+		m_histSnap.allocate(N);
+		HIP_CHECK(hipMemset(m_histSnap.ptr, 0, m_histSnap.bytes()));
+	}
 	HIP_CHECK(hipMemset(m_hist.ptr, 0, m_hist.bytes()));
 	m_ext.allocate(N);
 	HIP_CHECK(hipMemset(m_ext.ptr, 0, m_ext.bytes()));
@@ -1177,6 +1221,7 @@ void Simulation::allocateStepState()
 nemo_hip_step_buffers Simulation::stepBuffers(uint64_t offset, bool ext, bool fstim) const
 {
 	nemo_hip_step_buffers b;
+	std::memset(&b, 0, sizeof(b));
 	b.step_offset = offset;
 	b.clock = m_clock.ptr;
 	b.bin_table = m_binTable.ptr;
@@ -1191,6 +1236,7 @@ nemo_hip_step_buffers Simulation::stepBuffers(uint64_t offset, bool ext, bool fs
 	b.fstim = fstim ? m_fstim.ptr : nullptr;
 	b.hist = m_hist.ptr;
 	b.hist_pitch = m_accPitch;
+	b.hist_mask = m_histPlanes - 1;
 	b.fired_bits = m_firedBits.ptr;
 	b.fired_words = m_firedWords;
 	b.fbits = m_fbits;
@@ -1234,15 +1280,27 @@ void Simulation::launchDelivery(const nemo_propagate_args& set, uint64_t offset,
  * the update of step t has finished with) and joined before the update of
  * step t+2, so it runs beside the update of step t+1 and the delay-1
  * delivery of steps t+1 and t+2; the first step of a run delivers its delays
- * 2..D in order.  STDP accumulation of step t-1 runs on its own branch beside
- * the delivery of step t, or inside the update kernels.  With ev (three
+ * 2..D in order.  STDP accumulation of step t-1 runs inside the update
+ * kernels, shares the delivery launch of step t, or runs on its own branch
+ * forked after the update of step t-1 and joined either before the update of
+ * step t or, with one step of slack, before the update of step t+1: it reads
+ * the history plane of step t-1, which the update of step t+1 rewrites, and
+ * the accumulators it updates are serialised on the branch's stream.  The
+ * branch of the last step is joined at the end of the run.  With ev (three
  * events per step), HIP events bracket the delivery and the updates. */
 void Simulation::enqueueSteps(uint64_t first, unsigned k, bool ext, bool fstim, hipEvent_t* ev)
 {
 	const bool split = m_late.deliver_blocks && m_early.deliver_blocks;
-	const bool overlap = split && m_shape.overlap && k > 1;
-	/* STDP shares the delivery launch that follows the previous update */
+	const bool overlap = split && m_shape.overlap == 1 && k > 1;
+	/* the update of step t carries the delivery of the delays 2..D of step
+	 * t+1 as extra blocks (and, with fused STDP, the STDP blocks) */
+	const bool colaunch = split && m_shape.overlap == 2;
+	/* STDP shares the delivery launch that follows the previous update, or
+	 * the update launch when that carries propagate blocks */
 	const bool fused = m_shape.stdpMode == 2 && m_stdpBlocks;
+	const bool lagged = m_shape.stdpMode == 3 && m_stdpBlocks;
+	const bool batched = m_shape.stdpMode == 4 && m_stdpBlocks;
+	unsigned batchFrom = 0; /* first step of the pending STDP batch */
 	bool pending[2] = {false, false};
 	auto forkEarly = [&](unsigned i) {
 		HIP_CHECK(hipEventRecord(m_earlyFork, m_stream));
@@ -1264,10 +1322,18 @@ void Simulation::enqueueSteps(uint64_t first, unsigned k, bool ext, bool fstim, 
 			HIP_CHECK(hipEventRecord(m_stdpDone, m_stdpStream));
 			m_stdpPending = true;
 		}
+		if(lagged) {
+			HIP_CHECK(hipEventRecord(m_stdpFork, m_stream));
+			HIP_CHECK(hipStreamWaitEvent(m_stdpStream, m_stdpFork, 0));
+			nemo_propagate_args sa = m_propagate;
+			sa.step_offset = off;
+			HIP_CHECK(nemo_launch_propagate(m_stdpStream, &sa, m_stdpBlocks, 0));
+			HIP_CHECK(hipEventRecord(m_stdpLag[i & 1], m_stdpStream));
+		}
 		if(split) {
-			if(i == 0 || !overlap) launchDelivery(m_early, off, m_stream, false);
+			if(i == 0 || (!overlap && !colaunch)) launchDelivery(m_early, off, m_stream, false);
 			if(i == 0 && overlap) forkEarly(1);
-			launchDelivery(m_late, off, m_stream, fused);
+			launchDelivery(m_late, off, m_stream, fused && !colaunch);
 		} else {
 			launchDelivery(m_early, off, m_stream, fused);
 		}
@@ -1278,14 +1344,37 @@ void Simulation::enqueueSteps(uint64_t first, unsigned k, bool ext, bool fstim, 
 		/* the STDP branch overlaps delivery only: beside the update it slows
 		 * the update, which is on the step's critical path */
 		joinStdp();
+		if(lagged && i > 0) HIP_CHECK(hipStreamWaitEvent(m_stream, m_stdpLag[(i - 1) & 1], 0));
 		if(ev) HIP_CHECK(hipEventRecord(ev[3 * i + 1], m_stream));
 		nemo_hip_step_buffers b = stepBuffers(off, ext, fstim);
+		if(colaunch) {
+			b.co.args = m_early;
+			b.co.args.step_offset = off + 1;
+			b.co.deliver_blocks = i + 1 < k ? m_early.deliver_blocks : 0;
+			b.co.stdp_blocks = fused ? m_stdpBlocks : 0;
+			b.co.lds_bytes = b.co.deliver_blocks || b.co.stdp_blocks
+				? unsigned(nemo_colaunch_lds_bytes(m_early.max_window, m_early.bin_capacity, b.co.deliver_blocks > 0))
+				: 0;
+		}
 		for(Group& g : m_groups) {
+			if(g.dev.count == 0) continue;
 			HIP_CHECK(g.plugin->update(m_stream, &g.dev, &b));
+			b.co.deliver_blocks = 0; /* the first group's launch carries them */
+			b.co.stdp_blocks = 0;
+			b.co.lds_bytes = 0;
 		}
 		if(ev) HIP_CHECK(hipEventRecord(ev[3 * i + 2], m_stream));
 		if(overlap && i + 2 < k) forkEarly(i + 2);
+		if(batched && (i + 1 == k || i + 1 - batchFrom == 64)) {
+			/* the accumulation of the steps since the last batch: their
+			 * history planes (64 of them) are all still intact */
+			nemo_propagate_args sa = m_propagate;
+			sa.step_offset = first + batchFrom;
+			HIP_CHECK(nemo_launch_stdp_batch(m_stream, &sa, i + 1 - batchFrom, m_stdpBlocks));
+			batchFrom = i + 1;
+		}
 	}
+	if(lagged && k) HIP_CHECK(hipStreamWaitEvent(m_stream, m_stdpLag[(k - 1) & 1], 0));
 }
 
 /* Make the main stream wait for the STDP branch of the last enqueued step. */
@@ -1301,6 +1390,7 @@ void Simulation::joinStdp()
 void Simulation::flushStdp()
 {
 	joinStdp();
+	if(m_shape.stdpMode == 4) return; /* every run of steps ends with its batch */
 	if(!m_stdpBlocks || m_hostStep == 0 || m_flushed == m_hostStep - 1) return;
 	const uint64_t s = m_hostStep - 1;
 	nemo_propagate_args a = m_propagate;
@@ -1649,6 +1739,7 @@ void Simulation::restoreSnapshot(const Snapshot& s)
 	HIP_CHECK(hipStreamSynchronize(m_stream));
 	m_hostStep = s.hostStep;
 	m_flushed = s.flushed;
+	refreshHistSnap();
 }
 
 /* Advance nsteps steps in graphs of up to 64 steps; seconds on the stream. */
@@ -1729,12 +1820,25 @@ LaunchShape Simulation::autotune(unsigned stepsPerTrial)
 			const unsigned x = unsigned(std::min<double>(NEMO_MAX_BIN_BLOCKS - best.deliverBlocks, std::round(base * f)));
 			if(x != best.lateBlocks && std::find(late.begin(), late.end(), x) == late.end()) late.push_back(x);
 		}
+		for(double f : {8.0, 16.0}) {
+			const unsigned x = unsigned(std::min<double>(NEMO_MAX_BIN_BLOCKS - best.deliverBlocks, std::round(base * f)));
+			if(x != best.lateBlocks && std::find(late.begin(), late.end(), x) == late.end()) late.push_back(x);
+		}
+		const LaunchShape before = best;
 		for(unsigned b : late) {
-			for(unsigned ov : {0u, 1u}) {
-				if(b == 0 && ov == 1) continue;
-				LaunchShape c = best;
+			for(unsigned ov : {0u, 1u, 2u}) {
+				if(b == 0 && ov != 0) continue;
+				LaunchShape c = before;
 				c.lateBlocks = b;
 				c.overlap = ov;
+				consider(c);
+			}
+		}
+		if(best.lateBlocks && best.overlap == 2) {
+			/* the delays 2..D now run beside the update: retune their blocks */
+			for(unsigned b : scaled(best.deliverBlocks, {0.5, 2.0}, 1, NEMO_MAX_BIN_BLOCKS - best.lateBlocks)) {
+				LaunchShape c = best;
+				c.deliverBlocks = b;
 				consider(c);
 			}
 		}
@@ -1747,13 +1851,14 @@ LaunchShape Simulation::autotune(unsigned stepsPerTrial)
 	}
 	if(m_stdp && m_nplastic) {
 		const unsigned mode0 = best.stdpMode;
-		for(unsigned m : {0u, 1u, 2u}) {
+		for(unsigned m : {0u, 1u, 2u, 3u, 4u}) {
 			if(m == mode0) continue;
 			LaunchShape c = best;
 			c.stdpMode = m;
 			consider(c);
 		}
-		for(unsigned w : {1u, 2u, 4u}) {
+		for(unsigned w : {1u, 2u, 4u, 8u, 16u}) {
+			if(w > NEMO_STDP_MAX_WORDS) continue;
 			if(w == best.stdpWords) continue;
 			LaunchShape c = best;
 			c.stdpWords = w;
@@ -1989,7 +2094,7 @@ void Simulation::getRecentFiring(uint64_t* h, size_t n)
 	activate();
 	HIP_CHECK(hipStreamSynchronize(m_stream));
 	std::vector<uint64_t> d(m_npad);
-	const size_t plane = size_t((m_hostStep + 1) & 1); /* after the last step */
+	const size_t plane = size_t((m_hostStep - 1) & (m_histPlanes - 1)); /* after the last step */
 	HIP_CHECK(hipMemcpy(d.data(), m_hist.ptr + plane * m_accPitch, m_npad * 8, hipMemcpyDeviceToHost));
 	for(size_t i = 0; i < n; ++i) h[i] = d[m_order[i]];
 }

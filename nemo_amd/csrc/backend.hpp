@@ -151,10 +151,14 @@ struct LaunchShape {
 	unsigned deliverBlocks = 1;   /* delivery blocks for the delays 2..D (all delays if lateBlocks == 0) */
 	unsigned lateBlocks = 0;      /* delivery blocks for the delay 1 (both sets <= NEMO_MAX_BIN_BLOCKS) */
 	unsigned binCapacity = 4096;  /* events a delivery block stages in LDS */
-	unsigned stdpMode = 2;        /* STDP accumulation: own launch on a graph branch (0), inside the
-	                                 update kernels (1), or sharing the delivery launch (2) */
+	unsigned stdpMode = 2;        /* STDP accumulation: own launch on a graph branch joined before the
+	                                 update (0), inside the update kernels (1), sharing the delivery
+	                                 launch (2), own branch joined one step later (3), or in one
+	                                 launch after each run of up to 64 steps (4) */
 	unsigned stdpWords = 2;       /* firing-bitmap words (64 targets each) per STDP block */
-	unsigned overlap = 1;         /* deliver the delays 2..D of step t+1 beside the update of step t */
+	unsigned overlap = 1;         /* when split, deliver the delays 2..D of a step early: 1 on a graph
+	                                 branch beside the previous update, 2 as extra blocks of the
+	                                 previous update launch */
 };
 
 /* The HIP simulation (cuda::Simulation's role, cuda/Simulation.cpp:91-380).
@@ -237,6 +241,7 @@ private:
 	hipStream_t m_stream = nullptr;
 	hipStream_t m_stdpStream = nullptr;  /* parallel branch for STDP in its own launch */
 	hipEvent_t m_stdpFork = nullptr, m_stdpDone = nullptr;
+	hipEvent_t m_stdpLag[2] = {nullptr, nullptr}; /* mode 3: branch of step parity i done */
 	hipStream_t m_earlyStream = nullptr; /* branch for the delays 2..D of the next step */
 	hipEvent_t m_earlyFork = nullptr, m_earlyDone[2] = {nullptr, nullptr};
 	bool m_stdpPending = false;          /* the STDP branch has not been joined yet */
@@ -268,6 +273,8 @@ private:
 	DeviceArray<int64_t> m_acc;           /* overflow accumulators */
 	size_t m_accPitch = 0;
 	DeviceArray<uint64_t> m_hist;
+	unsigned m_histPlanes = 2;
+	DeviceArray<uint64_t> m_histSnap; /* STDP batches: history after the step before the batch */
 	DeviceArray<float> m_ext;
 	DeviceArray<uint32_t> m_fstim;
 	DeviceArray<float> m_debugCurrent;
@@ -333,6 +340,7 @@ private:
 	LaunchShape defaultShape() const;
 	void applyShape(const LaunchShape& s);
 	size_t ldsNeeded(const LaunchShape& s) const;
+	void refreshHistSnap();
 	void checkShape(const LaunchShape& s) const;
 	nemo_hip_step_buffers stepBuffers(uint64_t offset, bool ext, bool fstim) const;
 	void copyScalar(uint64_t* dst, uint64_t value);

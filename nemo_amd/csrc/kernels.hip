@@ -24,323 +24,65 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "kernels.h"
 #include "glibc_math.h"
 #include "plugin_device.h"
+#include "propagate_device.h"
 
 namespace {
 
-constexpr uint64_t RING_MASK = NEMO_STEP_RING - 1;
-/* fired rows a delivery block lists in LDS at a time */
-constexpr unsigned ROWS = 1024;
-constexpr unsigned ROWS_PER_THREAD = ROWS / NEMO_BLOCK;
-/* independent global loads each thread keeps in flight in the synapse loop
- * of STDP, and in the event loop of delivery */
-constexpr int ILP = 8;
-constexpr int EVENT_ILP = 16;
-
-__device__ inline unsigned kth_set_bit(uint64_t w, unsigned k)
-{
-	for(unsigned i = 0; i < k; ++i) w &= w - 1;
-	return unsigned(__builtin_ctzll(w));
-}
-
-/* ------------------------------------------------------------ delivery */
-
-/* A delivery block: list the fired rows of its (source tile, delay) pairs,
- * read their row pointers in parallel, then spread all their events evenly
- * over the block (four loads in flight per thread), staging them in LDS;
- * finally counting-sort the staged events by target tile (relative to the
- * block's window) into the block's bins. */
-__device__ void deliver_block(const nemo_propagate_args& a, uint64_t t, int2* stage, uint32_t* cnt, uint32_t* cur)
-{
-	__shared__ uint64_t s_word[NEMO_BLOCK];
-	__shared__ uint32_t s_wpre[NEMO_BLOCK + 1];
-	__shared__ uint32_t s_rb[ROWS];
-	__shared__ uint32_t s_rpre[ROWS + 1];
-
-	const unsigned CAP = a.bin_capacity;
-	const unsigned D = a.D;
-	const unsigned DC = a.d_count;
-	const unsigned tid = threadIdx.x;
-	const unsigned b = blockIdx.x;
-	const unsigned col = a.block_base + b;
-	const size_t parity = t & 1;
-	int64_t* acc = a.acc + parity * 2 * a.acc_pitch;
-	const uint2 win = a.windows[b];
-	const unsigned tlo = win.x;
-	const unsigned W = win.y - win.x; /* tiles in the window */
-	for(unsigned p = tid; p < W; p += NEMO_BLOCK) cnt[p] = 0;
-
-	const uint64_t q0 = a.pair_bounds[b];
-	const uint64_t q1 = a.pair_bounds[b + 1];
-	const uint64_t nwords = (q1 - q0) * 4;
-	uint32_t fill = 0;
-	unsigned long long events = 0;
-
-	for(uint64_t wb = 0; wb < nwords; wb += NEMO_BLOCK) {
-		/* the bitmap words of this batch, one per thread */
-		uint64_t word = 0;
-		const uint64_t w = wb + tid;
-		if(w < nwords) {
-			const uint64_t q = q0 + w / 4;
-			const unsigned p = unsigned(q / DC);
-			const unsigned d = unsigned(q % DC) + a.d_first;
-			if(t >= d) word = a.fired_bits[((t - d) & RING_MASK) * a.fired_words + size_t(p) * 4 + (w & 3)];
-		}
-		__syncthreads(); /* previous batch done with s_word */
-		s_word[tid] = word;
-		uint32_t R;
-		const uint32_t wpre = nemo_dev::block_exclusive_scan(uint32_t(__popcll(word)), &R);
-		s_wpre[tid] = wpre;
-		if(tid == 0) s_wpre[NEMO_BLOCK] = R;
-		__syncthreads();
-
-		for(uint32_t r0 = 0; r0 < R; r0 += ROWS) {
-			const uint32_t nr = min(ROWS, R - r0);
-			/* row pointers of up to ROWS fired rows, ROWS_PER_THREAD per thread */
-			uint32_t lens[ROWS_PER_THREAD];
-			uint32_t mine = 0;
-			uint32_t rr[ROWS_PER_THREAD];
-			unsigned wl[ROWS_PER_THREAD];
-#pragma unroll
-			for(unsigned k = 0; k < ROWS_PER_THREAD; ++k) rr[k] = r0 + tid * ROWS_PER_THREAD + k;
-			nemo_dev::last_not_above_k<NEMO_BLOCK>(s_wpre, NEMO_BLOCK, rr, wl);
-			uint32_t rbv[ROWS_PER_THREAD], rev[ROWS_PER_THREAD];
-#pragma unroll
-			for(unsigned k = 0; k < ROWS_PER_THREAD; ++k) {
-				const uint32_t ri = tid * ROWS_PER_THREAD + k;
-				rbv[k] = 0;
-				rev[k] = 0;
-				if(ri < nr) {
-					const unsigned lo = wl[k];
-					const uint64_t wg = wb + lo;
-					const uint64_t q = q0 + wg / 4;
-					const unsigned p = unsigned(q / DC);
-					const unsigned d = unsigned(q % DC) + a.d_first;
-					const unsigned bit = kth_set_bit(s_word[lo], rr[k] - s_wpre[lo]);
-					const uint32_t n = p * NEMO_TILE + unsigned(wg & 3) * 64 + bit;
-					const size_t row = size_t(n) * D + (d - 1);
-					rbv[k] = a.rowptr[row];
-					rev[k] = a.rowptr[row + 1];
-				}
-			}
-#pragma unroll
-			for(unsigned k = 0; k < ROWS_PER_THREAD; ++k) {
-				const uint32_t ri = tid * ROWS_PER_THREAD + k;
-				lens[k] = rev[k] - rbv[k];
-				if(ri < nr) s_rb[ri] = rbv[k];
-				mine += lens[k];
-			}
-			uint32_t M;
-			uint32_t pre = nemo_dev::block_exclusive_scan(mine, &M);
-#pragma unroll
-			for(unsigned k = 0; k < ROWS_PER_THREAD; ++k) {
-				const uint32_t ri = tid * ROWS_PER_THREAD + k;
-				if(ri < nr) s_rpre[ri] = pre;
-				pre += lens[k];
-			}
-			if(tid == 0) s_rpre[nr] = M;
-			__syncthreads();
-
-			/* the events of these rows, evenly over the block */
-			for(uint32_t e0 = tid; e0 < M; e0 += EVENT_ILP * NEMO_BLOCK) {
-				uint32_t ee[EVENT_ILP];
-				unsigned lo[EVENT_ILP];
-#pragma unroll
-				for(int u = 0; u < EVENT_ILP; ++u) ee[u] = e0 + u * NEMO_BLOCK;
-				nemo_dev::last_not_above_k<ROWS>(s_rpre, nr, ee, lo);
-				int2 s[EVENT_ILP];
-#pragma unroll
-				for(int u = 0; u < EVENT_ILP; ++u) {
-					s[u] = make_int2(0, 0);
-					if(ee[u] < M) s[u] = a.fcm[s_rb[lo[u]] + (ee[u] - s_rpre[lo[u]])];
-				}
-#pragma unroll
-				for(int u = 0; u < EVENT_ILP; ++u) {
-					const uint32_t e = e0 + u * NEMO_BLOCK;
-					if(e < M) {
-						const uint32_t pos = fill + e;
-						const uint32_t rel = uint32_t(s[u].x) / NEMO_TILE - tlo;
-						if(pos < CAP && rel < W) {
-							stage[pos] = s[u];
-							atomicAdd(&cnt[rel], 1u);
-						} else {
-							/* beyond capacity (or, defensively, outside the
-							 * window): straight into the accumulators */
-							if(pos < CAP) stage[pos] = make_int2(-1, 0);
-							int64_t* dst = acc + (s[u].y >= 0 ? size_t(0) : a.acc_pitch) + uint32_t(s[u].x);
-							atomicAdd(reinterpret_cast<unsigned long long*>(dst),
-									(unsigned long long)(int64_t)s[u].y);
-						}
-					}
-				}
-			}
-			fill += M;
-			events += M;
-			__syncthreads(); /* s_rb / s_rpre are rewritten by the next chunk */
-		}
-	}
-
-	/* counting sort of the staged events by target tile */
-	const uint32_t staged = min(fill, CAP);
-	uint32_t carry = 0;
-	for(uint32_t base = 0; base < W; base += NEMO_BLOCK) {
-		const uint32_t p = base + tid;
-		const uint32_t c = p < W ? cnt[p] : 0u;
-		uint32_t total;
-		const uint32_t pre = nemo_dev::block_exclusive_scan(c, &total);
-		if(p < W) {
-			cur[p] = carry + pre;
-			a.bin_table[(parity * a.tiles + tlo + p) * a.table_columns + col] = (c << 16) | (carry + pre);
-		}
-		carry += total;
-	}
-	__syncthreads();
-	int2* out = a.bins + (parity * a.table_columns + col) * CAP;
-	/* four staged events per thread at a time, so that their LDS atomics
-	 * (which return the slot) overlap */
-	for(uint32_t i0 = tid; i0 < staged; i0 += 4 * NEMO_BLOCK) {
-		int2 e[4];
-		uint32_t pos[4];
-#pragma unroll
-		for(int u = 0; u < 4; ++u) {
-			const uint32_t i = i0 + u * NEMO_BLOCK;
-			e[u] = i < staged ? stage[i] : make_int2(-1, 0);
-		}
-#pragma unroll
-		for(int u = 0; u < 4; ++u) {
-			if(e[u].x >= 0) pos[u] = atomicAdd(&cur[uint32_t(e[u].x) / NEMO_TILE - tlo], 1u);
-		}
-#pragma unroll
-		for(int u = 0; u < 4; ++u) {
-			if(e[u].x >= 0) out[pos[u]] = make_int2(int(uint32_t(e[u].x) % NEMO_TILE), e[u].y);
-		}
-	}
-	if(tid == 0 && events) a.block_events[col] += events;
-}
-
-/* ---------------------------------------------------------------- STDP */
+/* threads of a propagate (delivery or STDP) block */
+constexpr unsigned PBLOCK = NEMO_PROPAGATE_BLOCK;
 
 using nemo_dev::stdp_wrap_add;
 
-/* An STDP block of the STDP-only launch: the targets among its bitmap words
- * that fired P steps before step r, and all their plastic incoming synapses
- * spread evenly over its threads (ILP loads in flight per thread).  The
- * update kernels do the same per tile when STDP runs inside them. */
-__device__ void stdp_block(const nemo_propagate_args& a, uint64_t r, unsigned sb)
-{
-	const nemo_hip_stdp& st = a.stdp;
-	__shared__ int32_t s_fn[128];
-	__shared__ uint32_t s_rb[NEMO_BLOCK];
-	__shared__ uint32_t s_pos[NEMO_BLOCK + 1];
-	const unsigned P = st.post_window;
-	const unsigned tid = threadIdx.x;
-	if(tid < 64) {
-		s_fn[tid] = st.fn_pre[tid];
-		s_fn[64 + tid] = st.fn_post[tid];
-	}
-
-	const uint64_t slot = (r - P) & RING_MASK;
-	const size_t w0 = size_t(sb) * a.stdp_words;
-	const size_t word = w0 + tid / 64;
-	bool fired = false;
-	if(tid / 64 < a.stdp_words && word < a.fired_words) {
-		fired = (a.fired_bits[slot * a.fired_words + word] >> (tid & 63)) & 1u;
-	}
-	uint32_t nfired;
-	const uint32_t idx = nemo_dev::block_exclusive_scan(fired ? 1u : 0u, &nfired);
-	uint32_t len = 0, rb = 0;
-	if(fired) {
-		const uint32_t n = uint32_t(w0 * 64 + tid);
-		rb = st.rcm_ptr[n];
-		len = st.rcm_ptr[n + 1] - rb;
-	}
-	uint32_t total;
-	const uint32_t pos = nemo_dev::block_exclusive_scan(len, &total);
-	if(fired) {
-		s_rb[idx] = rb;
-		s_pos[idx] = pos;
-	}
-	if(tid == 0) s_pos[nfired] = total;
-	__syncthreads();
-	if(total == 0) return;
-
-	const uint64_t* hist = a.hist + (r & 1) * a.hist_pitch;
-	for(uint32_t e0 = tid; e0 < total; e0 += ILP * NEMO_BLOCK) {
-		uint32_t k[ILP];
-		uint32_t sd[ILP];
-		int32_t acc[ILP];
-#pragma unroll
-		for(int u = 0; u < ILP; ++u) k[u] = e0 + u * NEMO_BLOCK;
-		unsigned lo[ILP];
-		nemo_dev::last_not_above_k<NEMO_BLOCK>(s_pos, nfired, k, lo);
-#pragma unroll
-		for(int u = 0; u < ILP; ++u) {
-			const uint32_t e = k[u];
-			k[u] = 0;
-			sd[u] = 0;
-			acc[u] = 0;
-			if(e < total) {
-				k[u] = s_rb[lo[u]] + (e - s_pos[lo[u]]);
-				sd[u] = st.rcm_sd[k[u]];
-				acc[u] = st.rcm_acc[k[u]];
-			}
-		}
-		uint64_t h[ILP];
-#pragma unroll
-		for(int u = 0; u < ILP; ++u) {
-			h[u] = 0;
-			if(e0 + u * NEMO_BLOCK < total) h[u] = hist[sd[u] & 0x3ffffffu];
-		}
-#pragma unroll
-		for(int u = 0; u < ILP; ++u) {
-			if(e0 + u * NEMO_BLOCK < total) {
-				const unsigned d = (sd[u] >> 26) + 1;
-				const uint64_t preFiring = d >= 64 ? 0 : (h[u] >> d);
-				const int32_t dw = stdp_wrap_add(
-						nemo_dev::stdp_update_region(preFiring & st.potentiation, P, st.pre_post, st.post_pre,
-							s_fn, s_fn + 64),
-						nemo_dev::stdp_update_region(preFiring & st.depression, P, st.pre_post, st.post_pre,
-							s_fn, s_fn + 64));
-				if(dw != 0) {
-					st.rcm_acc[k[u]] = stdp_wrap_add(acc[u], dw);
-				}
-			}
-		}
-	}
-	if(tid == 0) st.examined[sb] += total;
-}
-
-/* STDP accumulation of step t-1 by STDP block sb, unless it is not due */
-__device__ inline void stdp_step(const nemo_propagate_args& a, uint64_t t, unsigned sb)
-{
-	if(t == 0) return;
-	const uint64_t r = t - 1;
-	if(r == a.clock->stdp_flushed || r < a.stdp.post_window) return;
-	stdp_block(a, r, sb);
-}
-
 /* delivery blocks, then (when fused into this launch) STDP blocks */
-__global__ void __launch_bounds__(NEMO_BLOCK)
+__global__ void __launch_bounds__(PBLOCK)
 propagate_kernel(nemo_propagate_args a)
 {
-	extern __shared__ int2 lds_dyn[];
+	extern __shared__ uint64_t lds_dyn[];
+	unsigned char* lds = reinterpret_cast<unsigned char*>(lds_dyn);
 	const uint64_t t = a.clock->base_step + a.step_offset;
 	if(blockIdx.x < a.deliver_blocks) {
-		int2* stage = lds_dyn;
-		uint32_t* cnt = reinterpret_cast<uint32_t*>(lds_dyn + a.bin_capacity);
-		deliver_block(a, t, stage, cnt, cnt + a.max_window);
+		nemo_dev::deliver_block<PBLOCK>(a, t, blockIdx.x, lds);
 	} else {
-		stdp_step(a, t, blockIdx.x - a.deliver_blocks);
+		nemo_dev::stdp_step<PBLOCK>(a, t, blockIdx.x - a.deliver_blocks, lds);
 	}
 }
 
 /* STDP blocks only */
-__global__ void __launch_bounds__(NEMO_BLOCK)
+__global__ void __launch_bounds__(PBLOCK)
 stdp_kernel(nemo_propagate_args a)
 {
-	stdp_step(a, a.clock->base_step + a.step_offset, blockIdx.x);
+	extern __shared__ uint64_t lds_dyn[];
+	nemo_dev::stdp_step<PBLOCK>(a, a.clock->base_step + a.step_offset, blockIdx.x,
+			reinterpret_cast<unsigned char*>(lds_dyn));
+}
+
+/* STDP accumulation of the nsteps steps from t0 = base + step_offset on,
+ * one block per group of stdp_words bitmap words (stdp_batch_block) */
+__global__ void __launch_bounds__(PBLOCK)
+stdp_batch_kernel(nemo_propagate_args a, unsigned nsteps)
+{
+	extern __shared__ uint64_t lds_dyn[];
+	nemo_dev::stdp_batch_block<PBLOCK>(a, a.clock->base_step + a.step_offset, nsteps, blockIdx.x,
+			reinterpret_cast<unsigned char*>(lds_dyn));
+}
+
+/* After a batch: the history after its last step becomes the snapshot the
+ * next batch starts from, and the last step counts as accumulated. */
+__global__ void __launch_bounds__(NEMO_BLOCK)
+stdp_snapshot_kernel(nemo_propagate_args a, unsigned nsteps)
+{
+	const uint64_t rl = a.clock->base_step + a.step_offset + nsteps - 1;
+	const uint64_t* cur = a.hist + (rl & a.hist_mask) * a.hist_pitch;
+	const size_t n = a.fired_words * 64;
+	for(size_t i = size_t(blockIdx.x) * NEMO_BLOCK + threadIdx.x; i < n; i += size_t(gridDim.x) * NEMO_BLOCK) {
+		a.hist_snap[i] = cur[i];
+	}
+	if(blockIdx.x == 0 && threadIdx.x == 0) const_cast<nemo_hip_clock*>(a.clock)->stdp_flushed = rl;
 }
 
 __global__ void __launch_bounds__(NEMO_BLOCK)
@@ -407,7 +149,37 @@ extern "C" {
 
 size_t nemo_propagate_lds_bytes(unsigned max_window, unsigned bin_capacity)
 {
-	return size_t(bin_capacity) * sizeof(int2) + 2 * size_t(max_window) * sizeof(uint32_t);
+	return nemo_dev::PropagateLds<PBLOCK>::delivery_bytes(bin_capacity, max_window);
+}
+
+hipError_t nemo_launch_stdp_batch(hipStream_t st, const nemo_propagate_args* a, unsigned nsteps, unsigned stdp_blocks)
+{
+	if(nsteps == 0 || stdp_blocks == 0) return hipSuccess;
+	if(nsteps > 64 || !a->hist_snap || a->stdp.post_window > 63 || a->stdp_words == 0
+			|| a->stdp_words > NEMO_STDP_MAX_WORDS || size_t(stdp_blocks) * a->stdp_words < a->fired_words
+			|| !a->stdp.rcm_ptr || !a->hist) {
+		return hipErrorInvalidValue;
+	}
+	hipLaunchKernelGGL(stdp_batch_kernel, dim3(stdp_blocks), dim3(PBLOCK),
+			nemo_dev::PropagateLds<PBLOCK>::stdp_bytes(), st, *a, nsteps);
+	hipError_t e = hipGetLastError();
+	if(e != hipSuccess) return e;
+	const size_t n = a->fired_words * 64;
+	const unsigned blocks = unsigned(std::min<size_t>(1024, (n + NEMO_BLOCK - 1) / NEMO_BLOCK));
+	hipLaunchKernelGGL(stdp_snapshot_kernel, dim3(blocks), dim3(NEMO_BLOCK), 0, st, *a, nsteps);
+	return hipGetLastError();
+}
+
+size_t nemo_colaunch_lds_bytes(unsigned max_window, unsigned bin_capacity, int delivery)
+{
+	return nemo_dev::PropagateLds<NEMO_TILE>::bytes(delivery != 0, bin_capacity, max_window);
+}
+
+size_t nemo_propagate_static_lds(void)
+{
+	hipFuncAttributes fa;
+	if(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(propagate_kernel)) != hipSuccess) return 32768;
+	return fa.sharedSizeBytes;
 }
 
 hipError_t nemo_launch_propagate(hipStream_t st, const nemo_propagate_args* a, unsigned stdp_blocks, int fuse_stdp)
@@ -420,24 +192,25 @@ hipError_t nemo_launch_propagate(hipStream_t st, const nemo_propagate_args* a, u
 				|| a->table_columns > NEMO_MAX_BIN_BLOCKS)) {
 		return hipErrorInvalidValue;
 	}
-	if(stdp_blocks && (a->stdp.post_window > 63 || a->stdp_words == 0 || a->stdp_words > 4
+	if(stdp_blocks && (a->stdp.post_window > 63 || a->stdp_words == 0 || a->stdp_words > NEMO_STDP_MAX_WORDS
 				|| size_t(stdp_blocks) * a->stdp_words < a->fired_words || !a->stdp.rcm_ptr || !a->hist)) {
 		return hipErrorInvalidValue;
 	}
-	const size_t lds = a->deliver_blocks ? nemo_propagate_lds_bytes(a->max_window, a->bin_capacity) : 0;
+	const size_t lds = nemo_dev::PropagateLds<PBLOCK>::bytes(a->deliver_blocks > 0, a->bin_capacity, a->max_window);
 	if(fuse_stdp) {
 		const unsigned blocks = a->deliver_blocks + stdp_blocks;
 		if(blocks == 0) return hipSuccess;
-		hipLaunchKernelGGL(propagate_kernel, dim3(blocks), dim3(NEMO_BLOCK), lds, st, *a);
+		hipLaunchKernelGGL(propagate_kernel, dim3(blocks), dim3(PBLOCK), lds, st, *a);
 		return hipGetLastError();
 	}
 	if(stdp_blocks) {
-		hipLaunchKernelGGL(stdp_kernel, dim3(stdp_blocks), dim3(NEMO_BLOCK), 0, st, *a);
+		hipLaunchKernelGGL(stdp_kernel, dim3(stdp_blocks), dim3(PBLOCK), nemo_dev::PropagateLds<PBLOCK>::stdp_bytes(),
+				st, *a);
 		const hipError_t e = hipGetLastError();
 		if(e != hipSuccess) return e;
 	}
 	if(a->deliver_blocks) {
-		hipLaunchKernelGGL(propagate_kernel, dim3(a->deliver_blocks), dim3(NEMO_BLOCK), lds, st, *a);
+		hipLaunchKernelGGL(propagate_kernel, dim3(a->deliver_blocks), dim3(PBLOCK), lds, st, *a);
 		return hipGetLastError();
 	}
 	return hipSuccess;

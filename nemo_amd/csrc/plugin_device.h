@@ -27,11 +27,12 @@ __device__ inline uint64_t current_step(const nemo_hip_step_buffers& b)
 	return b.clock->base_step + b.step_offset;
 }
 
-/* Exclusive scan of one value per thread over a block of NEMO_TILE threads;
+/* Exclusive scan of one value per thread over a block of THREADS threads;
  * returns the thread's prefix and sets *total. */
+template<unsigned THREADS = NEMO_TILE>
 __device__ inline uint32_t block_exclusive_scan(uint32_t x, uint32_t* total)
 {
-	__shared__ uint32_t s_wave[NEMO_TILE / 64];
+	__shared__ uint32_t s_wave[THREADS / 64];
 	const unsigned lane = threadIdx.x & 63;
 	const unsigned wave = threadIdx.x >> 6;
 	uint32_t v = x;
@@ -44,7 +45,7 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t x, uint32_t* total)
 	__syncthreads();
 	uint32_t before = 0, all = 0;
 #pragma unroll
-	for(unsigned w = 0; w < NEMO_TILE / 64; ++w) {
+	for(unsigned w = 0; w < THREADS / 64; ++w) {
 		const uint32_t t = s_wave[w];
 		before += w < wave ? t : 0;
 		all += t;
@@ -98,42 +99,48 @@ struct TileInput {
 	long long I[NEMO_TILE];
 };
 
-/* Sum the tile's binned events of this step from the delivery blocks of both
- * block sets into LDS (integer sums, so the order of the 64-bit LDS atomics
- * does not matter). */
+/* Sum the tile's input of this step into LDS: the overflow accumulators of
+ * its neurons (cleared here, cpu/Simulation.cpp:279-288) and the binned events
+ * of every delivery column (integer sums, so the order of the 64-bit LDS
+ * atomics does not matter).  The whole table row is read: columns whose
+ * window does not hold the tile keep a zero count, and reading them costs
+ * less than a dependent load of the tile's column range. */
 __device__ inline void gather_input(const nemo_hip_step_buffers& b, const nemo_hip_neuron_group& g, TileInput& in)
 {
 	__shared__ uint32_t s_pre[NEMO_MAX_BIN_BLOCKS + 1];
 	__shared__ uint32_t s_base[NEMO_MAX_BIN_BLOCKS];
 	const unsigned tid = threadIdx.x;
-	in.E[tid] = 0;
-	in.I[tid] = 0;
-	if(b.bin_blocks == 0) {
+	const unsigned tile = g.start / NEMO_TILE + blockIdx.x;
+	const size_t parity = current_step(b) & 1;
+	int64_t* acc = b.acc + parity * 2 * b.acc_pitch;
+	const unsigned ng = tile * NEMO_TILE + tid;
+	const int64_t oe = acc[ng];
+	const int64_t oi = acc[b.acc_pitch + ng];
+	const unsigned B = b.bin_blocks;
+	const uint32_t* row = b.bin_table + (parity * b.tiles + tile) * B;
+	const size_t cap = b.bin_capacity;
+	const int2* bins = b.bins + parity * B * cap;
+	constexpr unsigned PER = NEMO_MAX_BIN_BLOCKS / NEMO_TILE;
+	uint32_t ent[PER];
+#pragma unroll
+	for(unsigned k = 0; k < PER; ++k) {
+		const unsigned i = tid * PER + k;
+		ent[k] = i < B ? row[i] : 0u;
+	}
+	in.E[tid] = oe;
+	in.I[tid] = oi;
+	if(oe) acc[ng] = 0;
+	if(oi) acc[b.acc_pitch + ng] = 0;
+	if(B == 0) {
 		__syncthreads();
 		return;
 	}
-	const unsigned tile = g.start / NEMO_TILE + blockIdx.x;
-	const size_t parity = current_step(b) & 1;
-	/* the table columns [lo0, lo0 + B0) and [lo1, lo1 + B1) hold this tile */
-	const uint32_t r0 = b.bin_range[tile];
-	const uint32_t r1 = b.bin_range[b.tiles + tile];
-	const unsigned lo0 = r0 & 0xffffu, B0 = (r0 >> 16) - lo0;
-	const unsigned lo1 = r1 & 0xffffu, B1 = (r1 >> 16) - lo1;
-	const unsigned B = B0 + B1;
-	const uint32_t* row = b.bin_table + (parity * b.tiles + tile) * b.bin_blocks;
-	const size_t cap = b.bin_capacity;
-	const int2* bins = b.bins + parity * b.bin_blocks * cap;
-	constexpr unsigned PER = NEMO_MAX_BIN_BLOCKS / NEMO_TILE;
-	uint32_t cnt[PER];
 	uint32_t mine = 0;
 #pragma unroll
 	for(unsigned k = 0; k < PER; ++k) {
 		const unsigned i = tid * PER + k;
-		const unsigned col = i < B0 ? lo0 + i : lo1 + (i - B0);
-		const uint32_t e = i < B ? row[col] : 0u;
-		cnt[k] = e >> 16;
-		if(i < B) s_base[i] = uint32_t(col * cap + (e & 0xffffu));
-		mine += cnt[k];
+		if(i < B) s_base[i] = uint32_t(i * cap + (ent[k] & 0xffffu));
+		mine += ent[k] >> 16;
 	}
 	uint32_t total;
 	uint32_t pre = block_exclusive_scan(mine, &total);
@@ -141,18 +148,45 @@ __device__ inline void gather_input(const nemo_hip_step_buffers& b, const nemo_h
 	for(unsigned k = 0; k < PER; ++k) {
 		const unsigned i = tid * PER + k;
 		if(i < B) s_pre[i] = pre;
-		pre += cnt[k];
+		pre += ent[k] >> 16;
 	}
 	if(tid == 0) s_pre[B] = total;
 	__syncthreads();
-	/* GATHER_ILP independent loads in flight per thread before the LDS atomics */
+	/* batches of GATHER_ILP events per thread, GATHER_ILP independent loads in
+	 * flight per thread before the LDS atomics; the column of each 8th event
+	 * of a batch is searched once, an event's column is refined between its
+	 * bucket's bounds */
 	constexpr int GATHER_ILP = 8;
-	for(uint32_t j0 = tid; j0 < total; j0 += GATHER_ILP * NEMO_TILE) {
+	constexpr unsigned BATCH = GATHER_ILP * NEMO_TILE;
+	constexpr unsigned SHIFT = 3;
+	constexpr unsigned BUCKETS = BATCH >> SHIFT;
+	__shared__ uint32_t s_bucket[BUCKETS + 2];
+	for(uint32_t base = 0; base < total; base += BATCH) {
+		for(unsigned j = tid; j <= BUCKETS; j += NEMO_TILE) {
+			s_bucket[j] = last_not_above<NEMO_MAX_BIN_BLOCKS>(s_pre, B, min(base + (j << SHIFT), total - 1));
+		}
+		__syncthreads();
 		uint32_t jj[GATHER_ILP];
-		unsigned lo[GATHER_ILP];
+		unsigned lo[GATHER_ILP], hi[GATHER_ILP];
 #pragma unroll
-		for(int u = 0; u < GATHER_ILP; ++u) jj[u] = j0 + u * NEMO_TILE;
-		last_not_above_k<NEMO_MAX_BIN_BLOCKS>(s_pre, B, jj, lo);
+		for(int u = 0; u < GATHER_ILP; ++u) {
+			jj[u] = base + u * NEMO_TILE + tid;
+			const unsigned j = (u * NEMO_TILE + tid) >> SHIFT;
+			lo[u] = s_bucket[j];
+			hi[u] = s_bucket[j + 1];
+		}
+		bool open = true;
+		while(open) {
+			open = false;
+#pragma unroll
+			for(int u = 0; u < GATHER_ILP; ++u) {
+				if(lo[u] < hi[u]) {
+					const unsigned mid = (lo[u] + hi[u] + 1) >> 1;
+					if(s_pre[mid] <= jj[u]) lo[u] = mid; else hi[u] = mid - 1;
+					open = open || lo[u] < hi[u];
+				}
+			}
+		}
 		int2 ev[GATHER_ILP];
 #pragma unroll
 		for(int u = 0; u < GATHER_ILP; ++u) {
@@ -161,11 +195,12 @@ __device__ inline void gather_input(const nemo_hip_step_buffers& b, const nemo_h
 		}
 #pragma unroll
 		for(int u = 0; u < GATHER_ILP; ++u) {
-			if(j0 + u * NEMO_TILE < total) {
+			if(jj[u] < total) {
 				long long* dst = ev[u].y >= 0 ? &in.E[ev[u].x] : &in.I[ev[u].x];
 				atomicAdd(reinterpret_cast<unsigned long long*>(dst), (unsigned long long)(long long)ev[u].y);
 			}
 		}
+		__syncthreads(); /* the buckets are rewritten by the next batch */
 	}
 	__syncthreads();
 }
@@ -180,19 +215,11 @@ __device__ inline float wfx_to_float(int64_t v, unsigned fbits)
 }
 
 /* Excitatory and inhibitory current of neuron ng (thread threadIdx.x of the
- * tile): binned input plus overflow accumulators, which are cleared
- * (cpu/Simulation.cpp:279-288). */
+ * tile), as gathered by gather_input. */
 __device__ inline void take_current(const nemo_hip_step_buffers& b, const TileInput& in, unsigned ng, float& E, float& I)
 {
-	int64_t e = in.E[threadIdx.x];
-	int64_t i = in.I[threadIdx.x];
-	int64_t* acc = b.acc + (current_step(b) & 1) * 2 * b.acc_pitch;
-	const int64_t oe = acc[ng];
-	const int64_t oi = acc[b.acc_pitch + ng];
-	if(oe) { e += oe; acc[ng] = 0; }
-	if(oi) { i += oi; acc[b.acc_pitch + ng] = 0; }
-	E = wfx_to_float(e, b.fbits);
-	I = wfx_to_float(i, b.fbits);
+	E = wfx_to_float(in.E[threadIdx.x], b.fbits);
+	I = wfx_to_float(in.I[threadIdx.x], b.fbits);
 	if(b.debug_current) {
 		b.debug_current[ng] = E;
 		b.debug_current[b.acc_pitch + ng] = I;
@@ -244,10 +271,10 @@ __device__ inline void store_firing(const nemo_hip_step_buffers& b, unsigned ng,
 	}
 }
 
-/* The firing history after step s lives in plane s % 2. */
+/* The firing history after step s lives in plane s & hist_mask. */
 __device__ inline const uint64_t* history_after(const nemo_hip_step_buffers& b, uint64_t s)
 {
-	return b.hist + (s & 1) * b.hist_pitch;
+	return b.hist + (s & b.hist_mask) * b.hist_pitch;
 }
 
 /* History of neuron ng before this step (after the previous one). */
@@ -261,7 +288,7 @@ __device__ inline uint64_t load_history(const nemo_hip_step_buffers& b, unsigned
 __device__ inline void record_firing(const nemo_hip_step_buffers& b, unsigned ng, bool valid, bool fired, uint64_t h)
 {
 	if(valid) {
-		uint64_t* next = b.hist + (current_step(b) & 1) * b.hist_pitch;
+		uint64_t* next = b.hist + (current_step(b) & b.hist_mask) * b.hist_pitch;
 		next[ng] = (h << 1) | uint64_t(fired ? 1 : 0);
 	}
 	store_firing(b, ng, valid, fired);
@@ -369,9 +396,12 @@ __device__ inline void stdp_tile(const nemo_hip_step_buffers& b, unsigned ng, bo
 			}
 		}
 	}
-	if(tid == 0) st.examined[ng / NEMO_TILE] += total;
+	if(tid == 0) atomicAdd(&st.examined[ng / NEMO_TILE], (unsigned long long)total);
 }
 
 } // namespace nemo_dev
+
+/* the propagate blocks an update launch may carry (colaunched_block) */
+#include "propagate_device.h"
 
 #endif

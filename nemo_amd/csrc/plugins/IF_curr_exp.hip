@@ -23,6 +23,7 @@ enum { S_V = 0, S_IE = 1, S_II = 2, S_LASTFIRED = 3 };
 __global__ void __launch_bounds__(NEMO_UPDATE_BLOCK)
 if_curr_exp_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 {
+	if(nemo_dev::colaunched_block(b, g)) return;
 	__shared__ nemo_dev::TileInput input;
 	nemo_dev::gather_input(b, g, input);
 	const unsigned nl = blockIdx.x * NEMO_UPDATE_BLOCK + threadIdx.x;
@@ -71,8 +72,10 @@ extern "C" hipError_t
 hip_update_neurons(hipStream_t stream, const nemo_hip_neuron_group* g, const nemo_hip_step_buffers* b)
 {
 	if(g->count == 0) return hipSuccess;
-	const unsigned blocks = (g->count + NEMO_UPDATE_BLOCK - 1) / NEMO_UPDATE_BLOCK;
-	hipLaunchKernelGGL(if_curr_exp_update, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), 0, stream, *g, *b);
+	/* the group's tiles, then the propagate blocks the launch carries */
+	const unsigned blocks = (g->count + NEMO_UPDATE_BLOCK - 1) / NEMO_UPDATE_BLOCK + b->co.deliver_blocks
+		+ b->co.stdp_blocks;
+	hipLaunchKernelGGL(if_curr_exp_update, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream, *g, *b);
 	return hipGetLastError();
 }
 

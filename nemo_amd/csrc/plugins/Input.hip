@@ -10,6 +10,7 @@ namespace {
 __global__ void __launch_bounds__(NEMO_UPDATE_BLOCK)
 input_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 {
+	if(nemo_dev::colaunched_block(b, g)) return;
 	__shared__ nemo_dev::TileInput input;
 	nemo_dev::gather_input(b, g, input);
 	const unsigned nl = blockIdx.x * NEMO_UPDATE_BLOCK + threadIdx.x;
@@ -32,8 +33,10 @@ extern "C" hipError_t
 hip_update_neurons(hipStream_t stream, const nemo_hip_neuron_group* g, const nemo_hip_step_buffers* b)
 {
 	if(g->count == 0) return hipSuccess;
-	const unsigned blocks = (g->count + NEMO_UPDATE_BLOCK - 1) / NEMO_UPDATE_BLOCK;
-	hipLaunchKernelGGL(input_update, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), 0, stream, *g, *b);
+	/* the group's tiles, then the propagate blocks the launch carries */
+	const unsigned blocks = (g->count + NEMO_UPDATE_BLOCK - 1) / NEMO_UPDATE_BLOCK + b->co.deliver_blocks
+		+ b->co.stdp_blocks;
+	hipLaunchKernelGGL(input_update, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream, *g, *b);
 	return hipGetLastError();
 }
 

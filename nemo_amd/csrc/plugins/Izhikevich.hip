@@ -24,6 +24,7 @@ enum { STATE_U = 0, STATE_V = 1 };
 __global__ void __launch_bounds__(NEMO_UPDATE_BLOCK)
 izhikevich_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 {
+	if(nemo_dev::colaunched_block(b, g)) return;
 	__shared__ nemo_dev::TileInput input;
 	const unsigned nl = blockIdx.x * NEMO_UPDATE_BLOCK + threadIdx.x;
 	const bool valid = nl < g.count;
@@ -32,13 +33,15 @@ izhikevich_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 
 	/* the neuron's parameters, state, RNG and history are loaded before the
 	 * input gather, so that their latency overlaps it */
-	float sigma = 0.0f, a = 0.0f, bb = 0.0f, u = 0.0f, v = 0.0f;
+	float sigma = 0.0f, a = 0.0f, bb = 0.0f, c = 0.0f, d = 0.0f, u = 0.0f, v = 0.0f;
 	uint4 s = make_uint4(0, 0, 0, 0);
 	uint64_t h = 0;
 	if(valid) {
 		sigma = g.param[PARAM_SIGMA * P + nl];
 		a = g.param[PARAM_A * P + nl];
 		bb = g.param[PARAM_B * P + nl];
+		c = g.param[PARAM_C * P + nl];
+		d = g.param[PARAM_D * P + nl];
 		u = g.state[STATE_U * P + nl];
 		v = g.state[STATE_V * P + nl];
 		if(g.rng) s = reinterpret_cast<const uint4*>(g.rng)[nl];
@@ -69,8 +72,8 @@ izhikevich_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 
 		fired = fired || nemo_dev::stimulated(b, ng);
 		if(fired) {
-			v = g.param[PARAM_C * P + nl];
-			u += g.param[PARAM_D * P + nl];
+			v = c;
+			u += d;
 		}
 		g.state[STATE_U * P + nl] = u;
 		g.state[STATE_V * P + nl] = v;
@@ -85,8 +88,10 @@ extern "C" hipError_t
 hip_update_neurons(hipStream_t stream, const nemo_hip_neuron_group* g, const nemo_hip_step_buffers* b)
 {
 	if(g->count == 0) return hipSuccess;
-	const unsigned blocks = (g->count + NEMO_UPDATE_BLOCK - 1) / NEMO_UPDATE_BLOCK;
-	hipLaunchKernelGGL(izhikevich_update, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), 0, stream, *g, *b);
+	/* the group's tiles, then the propagate blocks the launch carries */
+	const unsigned blocks = (g->count + NEMO_UPDATE_BLOCK - 1) / NEMO_UPDATE_BLOCK + b->co.deliver_blocks
+		+ b->co.stdp_blocks;
+	hipLaunchKernelGGL(izhikevich_update, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream, *g, *b);
 	return hipGetLastError();
 }
 

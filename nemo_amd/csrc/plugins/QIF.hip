@@ -15,12 +15,16 @@ namespace {
 __global__ void __launch_bounds__(NEMO_UPDATE_BLOCK)
 qif_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 {
+	if(nemo_dev::colaunched_block(b, g)) return;
 	__shared__ nemo_dev::TileInput input;
-	nemo_dev::gather_input(b, g, input);
 	const unsigned nl = blockIdx.x * NEMO_UPDATE_BLOCK + threadIdx.x;
 	const bool valid = nl < g.count;
 	const unsigned ng = g.start + nl;
+	/* state, parameter and history load while the input is gathered */
 	const uint64_t h = nemo_dev::load_history(b, ng, valid);
+	float v = valid ? g.state[nl] : 0.0f;
+	const float a = valid ? g.param[nl] : 0.0f;
+	nemo_dev::gather_input(b, g, input);
 	bool fired = false;
 
 	if(valid) {
@@ -32,8 +36,6 @@ qif_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 		const float a2 = dt;
 		const float bb1 = dt / 6.0f;
 		const unsigned inc_max = (unsigned)(1 / dt);
-		float v = g.state[nl];
-		const float a = g.param[nl];
 		for(unsigned t = 0; t < inc_max; ++t) {
 			if(!fired) {
 				float I = Excit + Inhib + Exter;
@@ -65,8 +67,10 @@ extern "C" hipError_t
 hip_update_neurons(hipStream_t stream, const nemo_hip_neuron_group* g, const nemo_hip_step_buffers* b)
 {
 	if(g->count == 0) return hipSuccess;
-	const unsigned blocks = (g->count + NEMO_UPDATE_BLOCK - 1) / NEMO_UPDATE_BLOCK;
-	hipLaunchKernelGGL(qif_update, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), 0, stream, *g, *b);
+	/* the group's tiles, then the propagate blocks the launch carries */
+	const unsigned blocks = (g->count + NEMO_UPDATE_BLOCK - 1) / NEMO_UPDATE_BLOCK + b->co.deliver_blocks
+		+ b->co.stdp_blocks;
+	hipLaunchKernelGGL(qif_update, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream, *g, *b);
 	return hipGetLastError();
 }
 

@@ -348,7 +348,10 @@ def test_launch_shapes_give_identical_results():
     speed: rasters, weights and accumulators are identical."""
     spec = gen.random_izhikevich(4000, 150, dmax=15, seed=21, stdp=True)
     shapes = [None, (1, 1, 64, 0, 1, 1), (7, 3, 256, 1, 4, 0), (200, 0, 1024, 2, 2, 0),
-              (900, 124, 4096, 2, 1, 1), (33, 5, 2048, 1, 3, 1), (64, 0, 512, 0, 2, 0)]
+              (900, 124, 4096, 2, 1, 1), (33, 5, 2048, 1, 3, 1), (64, 0, 512, 0, 2, 0),
+              (64, 0, 512, 3, 2, 0), (33, 5, 2048, 3, 1, 1), (33, 5, 2048, 2, 3, 2), (64, 16, 1024, 1, 2, 2),
+              (200, 40, 4096, 0, 4, 2), (7, 3, 256, 3, 1, 2), (120, 30, 512, 2, 4, 2), (64, 0, 512, 4, 2, 0),
+              (33, 5, 2048, 4, 3, 2), (200, 0, 1024, 4, 8, 1)]
     results = []
     for shape in shapes:
         sim = sc.make(nemo_amd, spec, _conf(nemo_amd, stdp_fn()))
@@ -357,11 +360,13 @@ def test_launch_shapes_give_identical_results():
             assert sim.launch_shape == shape
         c, f, e = sim.run(250, stdp_period=100)
         c2, f2, e2 = sim.run(67)  # chunks of 64 and 3 steps
+        single = [sim.step() for _ in range(5)]
         results.append((np.concatenate([c, c2]), np.concatenate([f, f2]), e + e2, sim.fixed_weights(),
-                        sim.stdp_accumulators()))
-    for r in results[1:]:
-        for x, y in zip(results[0], r):
-            assert np.array_equal(x, y)
+                        sim.stdp_accumulators(), single))
+    for shape, r in zip(shapes[1:], results[1:]):
+        for name, x, y in zip(("counts", "fired", "events", "weights", "accumulators"), results[0][:5], r[:5]):
+            assert np.array_equal(x, y), (shape, name)
+        assert results[0][5] == r[5], (shape, "single steps")
     assert results[0][0].sum() > 0
 
 
