@@ -979,25 +979,24 @@ static Partition partitionPairs(const std::vector<uint64_t>& pairWeight, const s
 }
 
 /* Default shape: bin capacity 4096 events (halved while the LDS does not
- * allow it), enough delivery blocks for about 1% of the synapses per step at
- * half capacity, about a 1/D share of them for the delay-1 set, STDP on its
- * own graph branch, two bitmap words per STDP block, delays 2..D delivered
- * beside the previous update.  NEMO_HIP_DELIVER_BLOCKS, NEMO_HIP_LATE_BLOCKS,
- * NEMO_HIP_BIN_CAPACITY, NEMO_HIP_FUSE_STDP, NEMO_HIP_STDP_WORDS and
- * NEMO_HIP_OVERLAP override it (tuning). */
+ * allow it), one block set with enough delivery blocks for about 1% of the
+ * synapses per step, STDP accumulated in one batch after each run of steps,
+ * two bitmap words per STDP block.  NEMO_HIP_DELIVER_BLOCKS,
+ * NEMO_HIP_LATE_BLOCKS, NEMO_HIP_BIN_CAPACITY, NEMO_HIP_STDP_MODE,
+ * NEMO_HIP_STDP_WORDS and NEMO_HIP_OVERLAP override it (tuning). */
 LaunchShape Simulation::defaultShape() const
 {
 	LaunchShape s;
 	const uint64_t pairs = uint64_t(m_D) * m_tiles;
 	for(unsigned cap = 4096; cap >= 256; cap >>= 1) {
-		double blocks = std::ceil(0.01 * double(m_nsynapses) / (0.5 * cap));
+		double blocks = std::ceil(0.01 * double(m_nsynapses) / cap);
 		blocks = std::max(2.0, std::min({blocks, double(NEMO_MAX_BIN_BLOCKS), double(pairs) + 1.0}));
 		s.deliverBlocks = unsigned(blocks);
 		s.lateBlocks = 0;
 		s.binCapacity = cap;
 		if(ldsNeeded(s) <= m_ldsBudget) break;
 	}
-	s.stdpMode = 2;
+	s.stdpMode = 4;
 	auto env = [](const char* name, int lo, int hi, unsigned& field) {
 		if(const char* e = std::getenv(name)) field = unsigned(std::max(lo, std::min(std::atoi(e), hi)));
 	};

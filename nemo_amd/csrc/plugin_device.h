@@ -142,38 +142,54 @@ __device__ inline void gather_input(const nemo_hip_step_buffers& b, const nemo_h
 		if(i < B) s_base[i] = uint32_t(i * cap + (ent[k] & 0xffffu));
 		mine += ent[k] >> 16;
 	}
-	uint32_t total;
-	uint32_t pre = block_exclusive_scan(mine, &total);
-#pragma unroll
-	for(unsigned k = 0; k < PER; ++k) {
-		const unsigned i = tid * PER + k;
-		if(i < B) s_pre[i] = pre;
-		pre += ent[k] >> 16;
-	}
-	if(tid == 0) s_pre[B] = total;
-	__syncthreads();
 	/* batches of GATHER_ILP events per thread, GATHER_ILP independent loads in
-	 * flight per thread before the LDS atomics; the column of each 8th event
-	 * of a batch is searched once, an event's column is refined between its
-	 * bucket's bounds */
+	 * flight per thread before the LDS atomics.  An event's column lies
+	 * between the columns of the bucket bounds around it (every 8th event),
+	 * which each column marks when the tile has at most MARKED events, or
+	 * which are searched per batch; it is refined between them. */
 	constexpr int GATHER_ILP = 8;
 	constexpr unsigned BATCH = GATHER_ILP * NEMO_TILE;
 	constexpr unsigned SHIFT = 3;
-	constexpr unsigned BUCKETS = BATCH >> SHIFT;
-	__shared__ uint32_t s_bucket[BUCKETS + 2];
-	for(uint32_t base = 0; base < total; base += BATCH) {
-		for(unsigned j = tid; j <= BUCKETS; j += NEMO_TILE) {
-			s_bucket[j] = last_not_above<NEMO_MAX_BIN_BLOCKS>(s_pre, B, min(base + (j << SHIFT), total - 1));
+	constexpr unsigned MARKED = 4096;
+	__shared__ uint32_t s_bucket[(MARKED > BATCH ? MARKED : BATCH) / (1u << SHIFT) + 2];
+	uint32_t total;
+	uint32_t pre = block_exclusive_scan(mine, &total);
+	const bool marked = total <= MARKED;
+#pragma unroll
+	for(unsigned k = 0; k < PER; ++k) {
+		const unsigned i = tid * PER + k;
+		const uint32_t c = ent[k] >> 16;
+		if(i < B) {
+			s_pre[i] = pre;
+			if(marked) {
+				for(uint32_t j = (pre + (1u << SHIFT) - 1) >> SHIFT; (j << SHIFT) < pre + c; ++j) s_bucket[j] = i;
+			}
 		}
-		__syncthreads();
+		pre += c;
+	}
+	if(tid == 0) {
+		s_pre[B] = total;
+		if(marked && total) s_bucket[((total - 1) >> SHIFT) + 1] = B - 1;
+	}
+	__syncthreads();
+	for(uint32_t base = 0; base < total; base += BATCH) {
+		const uint32_t b0 = marked ? 0u : base; /* first event of the buckets */
+		if(!marked) {
+			for(unsigned j = tid; j <= (BATCH >> SHIFT); j += NEMO_TILE) {
+				s_bucket[j] = last_not_above<NEMO_MAX_BIN_BLOCKS>(s_pre, B, min(base + (j << SHIFT), total - 1));
+			}
+			__syncthreads();
+		}
 		uint32_t jj[GATHER_ILP];
 		unsigned lo[GATHER_ILP], hi[GATHER_ILP];
 #pragma unroll
 		for(int u = 0; u < GATHER_ILP; ++u) {
+			/* buckets past the last event may be unset: only the events of
+			 * the tile read theirs */
 			jj[u] = base + u * NEMO_TILE + tid;
-			const unsigned j = (u * NEMO_TILE + tid) >> SHIFT;
-			lo[u] = s_bucket[j];
-			hi[u] = s_bucket[j + 1];
+			const unsigned j = (jj[u] - b0) >> SHIFT;
+			lo[u] = jj[u] < total ? s_bucket[j] : 0u;
+			hi[u] = jj[u] < total ? s_bucket[j + 1] : 0u;
 		}
 		bool open = true;
 		while(open) {
@@ -181,7 +197,7 @@ __device__ inline void gather_input(const nemo_hip_step_buffers& b, const nemo_h
 #pragma unroll
 			for(int u = 0; u < GATHER_ILP; ++u) {
 				if(lo[u] < hi[u]) {
-					const unsigned mid = (lo[u] + hi[u] + 1) >> 1;
+					const unsigned mid = lo[u] + ((hi[u] - lo[u] + 1) >> 1);
 					if(s_pre[mid] <= jj[u]) lo[u] = mid; else hi[u] = mid - 1;
 					open = open || lo[u] < hi[u];
 				}
@@ -200,7 +216,7 @@ __device__ inline void gather_input(const nemo_hip_step_buffers& b, const nemo_h
 				atomicAdd(reinterpret_cast<unsigned long long*>(dst), (unsigned long long)(long long)ev[u].y);
 			}
 		}
-		__syncthreads(); /* the buckets are rewritten by the next batch */
+		if(!marked) __syncthreads(); /* the buckets are rewritten by the next batch */
 	}
 	__syncthreads();
 }

@@ -44,10 +44,13 @@ struct PropagateLds {
 	uint32_t* cnt;
 	uint32_t* cur;
 	uint32_t* bucket;
-	/* events of a batch of the event loop, and events per row bucket */
+	/* events of a batch of the event loop, events per row bucket, and the
+	 * most events of a chunk whose buckets the rows mark (buckets hold the
+	 * larger of a batch and that) */
 	static constexpr unsigned BATCH = EVENT_ILP * THREADS;
 	static constexpr unsigned BUCKET_SHIFT = 5;
-	static constexpr unsigned BUCKETS = BATCH >> BUCKET_SHIFT;
+	static constexpr unsigned MARKED = 16384;
+	static constexpr unsigned BUCKETS = (BATCH > MARKED ? BATCH : MARKED) >> BUCKET_SHIFT;
 	__device__ PropagateLds(unsigned char* p, unsigned cap, unsigned max_window)
 	{
 		stage = reinterpret_cast<int2*>(p);
@@ -182,24 +185,59 @@ __device__ inline void deliver_block(const nemo_propagate_args& a, uint64_t t, u
 			if(tid == 0) s_rpre[nr] = M;
 			__syncthreads();
 
-			/* the events of these rows, evenly over the block, in batches:
-			 * the row of each 32nd event of a batch is searched once, and an
-			 * event's row is refined between those of its bucket's bounds */
+			/* staging of event number e of this chunk: into the stage with a
+			 * count for its tile, or (beyond capacity or, defensively,
+			 * outside the window) straight into the accumulators */
+			auto stage_event = [&](int2 ev, uint32_t e) {
+				const uint32_t pos = fill + e;
+				const uint32_t rel = uint32_t(ev.x) / NEMO_TILE - tlo;
+				if(pos < CAP && rel < W) {
+					stage[pos] = ev;
+					atomicAdd(&cnt[rel], 1u);
+				} else {
+					if(pos < CAP) stage[pos] = make_int2(-1, 0);
+					int64_t* dst = acc + (ev.y >= 0 ? size_t(0) : a.acc_pitch) + uint32_t(ev.x);
+					atomicAdd(reinterpret_cast<unsigned long long*>(dst), (unsigned long long)(int64_t)ev.y);
+				}
+			};
+			/* the events of these rows evenly over the block, in batches of
+			 * EVENT_ILP per thread.  An event's row lies between the rows of
+			 * the bucket bounds around it (every 32nd event), which each row
+			 * marks when the chunk has at most MARKED events, or which are
+			 * searched per batch; it is refined between them. */
 			constexpr unsigned BATCH = PropagateLds<THREADS>::BATCH;
 			constexpr unsigned SHIFT = PropagateLds<THREADS>::BUCKET_SHIFT;
-			for(uint32_t base = 0; base < M; base += BATCH) {
-				for(unsigned j = tid; j <= PropagateLds<THREADS>::BUCKETS; j += THREADS) {
-					L.bucket[j] = last_not_above<ROWS>(s_rpre, nr, min(base + (j << SHIFT), M - 1));
+			const bool marked = M <= PropagateLds<THREADS>::MARKED;
+			if(marked) {
+#pragma unroll
+				for(unsigned k = 0; k < ROWS_PER_THREAD; ++k) {
+					const uint32_t ri = tid * ROWS_PER_THREAD + k;
+					if(ri < nr) {
+						const uint32_t p0 = s_rpre[ri], p1 = s_rpre[ri + 1];
+						for(uint32_t j = (p0 + (1u << SHIFT) - 1) >> SHIFT; (j << SHIFT) < p1; ++j) L.bucket[j] = ri;
+					}
 				}
+				if(tid == 0 && M) L.bucket[((M - 1) >> SHIFT) + 1] = nr - 1;
 				__syncthreads();
+			}
+			for(uint32_t base = 0; base < M; base += BATCH) {
+				const uint32_t b0 = marked ? 0u : base; /* first event of the buckets */
+				if(!marked) {
+					for(unsigned j = tid; j <= (BATCH >> SHIFT); j += THREADS) {
+						L.bucket[j] = last_not_above<ROWS>(s_rpre, nr, min(base + (j << SHIFT), M - 1));
+					}
+					__syncthreads();
+				}
 				uint32_t ee[EVENT_ILP];
 				unsigned lo[EVENT_ILP], hi[EVENT_ILP];
 #pragma unroll
 				for(int u = 0; u < EVENT_ILP; ++u) {
+					/* buckets past the last event may be unset: only the
+					 * events of the chunk read theirs */
 					ee[u] = base + u * THREADS + tid;
-					const unsigned j = (u * THREADS + tid) >> SHIFT;
-					lo[u] = L.bucket[j];
-					hi[u] = L.bucket[j + 1];
+					const unsigned j = (ee[u] - b0) >> SHIFT;
+					lo[u] = ee[u] < M ? L.bucket[j] : 0u;
+					hi[u] = ee[u] < M ? L.bucket[j + 1] : 0u;
 				}
 				bool open = true;
 				while(open) {
@@ -207,38 +245,23 @@ __device__ inline void deliver_block(const nemo_propagate_args& a, uint64_t t, u
 #pragma unroll
 					for(int u = 0; u < EVENT_ILP; ++u) {
 						if(lo[u] < hi[u]) {
-							const unsigned mid = (lo[u] + hi[u] + 1) >> 1;
+							const unsigned mid = lo[u] + ((hi[u] - lo[u] + 1) >> 1);
 							if(s_rpre[mid] <= ee[u]) lo[u] = mid; else hi[u] = mid - 1;
 							open = open || lo[u] < hi[u];
 						}
 					}
 				}
-				int2 s[EVENT_ILP];
+				int2 ev[EVENT_ILP];
 #pragma unroll
 				for(int u = 0; u < EVENT_ILP; ++u) {
-					s[u] = make_int2(0, 0);
-					if(ee[u] < M) s[u] = a.fcm[s_rb[lo[u]] + (ee[u] - s_rpre[lo[u]])];
+					ev[u] = make_int2(0, 0);
+					if(ee[u] < M) ev[u] = a.fcm[s_rb[lo[u]] + (ee[u] - s_rpre[lo[u]])];
 				}
 #pragma unroll
 				for(int u = 0; u < EVENT_ILP; ++u) {
-					const uint32_t e = ee[u];
-					if(e < M) {
-						const uint32_t pos = fill + e;
-						const uint32_t rel = uint32_t(s[u].x) / NEMO_TILE - tlo;
-						if(pos < CAP && rel < W) {
-							stage[pos] = s[u];
-							atomicAdd(&cnt[rel], 1u);
-						} else {
-							/* beyond capacity (or, defensively, outside the
-							 * window): straight into the accumulators */
-							if(pos < CAP) stage[pos] = make_int2(-1, 0);
-							int64_t* dst = acc + (s[u].y >= 0 ? size_t(0) : a.acc_pitch) + uint32_t(s[u].x);
-							atomicAdd(reinterpret_cast<unsigned long long*>(dst),
-									(unsigned long long)(int64_t)s[u].y);
-						}
-					}
+					if(ee[u] < M) stage_event(ev[u], ee[u]);
 				}
-				__syncthreads(); /* the buckets are rewritten by the next batch */
+				if(!marked) __syncthreads(); /* the buckets are rewritten by the next batch */
 			}
 			fill += M;
 			events += M;

@@ -177,13 +177,16 @@ def stdp_fn():
     return (pre, post, 0.0, 10.0, 0.0, -10.0)
 
 
-@pytest.mark.parametrize("n,m,steps,period", [(2000, 200, 1200, 300), (5000, 100, 600, 200)])
-def test_stdp_network_run_parity(n, m, steps, period):
+@pytest.mark.parametrize("n,m,dmax,steps,period", [(2000, 200, 20, 1200, 300), (5000, 100, 20, 600, 200),
+                                                    (1500, 1000, 20, 400, 150), (1000, 1500, 12, 300, 100)])
+def test_stdp_network_run_parity(n, m, dmax, steps, period):
     """Izhikevich network with plastic excitatory synapses (the STDP benchmark
     rule, reduced size): the run() graph path against the oracle stepped one
     step at a time, applyStdp(1.0) every `period` steps.  Raster, weights and
-    accumulators bit-exact."""
-    spec = gen.random_izhikevich(n, m, dmax=20, seed=5, stdp=True)
+    accumulators bit-exact.  Rows of about 10, 5, 50 and 125 synapses per
+    (source, delay): short rows, and rows a delivery wave takes whole, over
+    one or several passes of 64 events."""
+    spec = gen.random_izhikevich(n, m, dmax=dmax, seed=5, stdp=True)
     ref, hip = both(spec, stdp_fn())
     cr, fr, er = ref.run(steps, stdp_period=period, reward=1.0)
     ch, fh, eh = hip.run(steps, stdp_period=period, reward=1.0)
