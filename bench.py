@@ -281,13 +281,18 @@ def main():
 
     kern, work = sim.profile(args.profile_steps)
     steps_p = work["steps"]
+    # the propagate launch carries the STDP accumulation only in STDP mode 2;
+    # otherwise it runs in launches of its own (mode 4: one batch per run of
+    # up to 64 steps), whose bytes count towards the step but not the kernel
+    stdp_bytes = BYTES_PER_STDP_SYNAPSE * work["stdp_synapses"] / steps_p
+    fused_stdp = shape[3] == 2
     per_launch = {
-        "propagate": (BYTES_PER_EVENT * work["events"] + BYTES_PER_STDP_SYNAPSE * work["stdp_synapses"]) / steps_p,
+        "propagate": BYTES_PER_EVENT * work["events"] / steps_p + (stdp_bytes if fused_stdp else 0.0),
         "update": BYTES_PER_IZH_NEURON * n,
     }
     dominant = max(per_launch, key=lambda k: kern[k])
     achieved = per_launch[dominant] / (kern[dominant] * 1e-6) / 1e9
-    step_bytes = sum(per_launch.values())
+    step_bytes = sum(per_launch.values()) + (0.0 if fused_stdp else stdp_bytes)
 
     result = {
         "metric": METRIC,

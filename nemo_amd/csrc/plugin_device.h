@@ -105,7 +105,16 @@ struct TileInput {
  * atomics does not matter).  The whole table row is read: columns whose
  * window does not hold the tile keep a zero count, and reading them costs
  * less than a dependent load of the tile's column range. */
-__device__ inline void gather_input(const nemo_hip_step_buffers& b, const nemo_hip_neuron_group& g, TileInput& in)
+struct NoPrefetch {
+	__device__ void operator()() const {}
+};
+
+/* prefetch() is called once the tile's table row and accumulators are
+ * requested: the loads it issues (the model's state, say) queue behind them,
+ * so that waiting for the table does not wait for them too. */
+template<typename Prefetch = NoPrefetch>
+__device__ inline void gather_input(const nemo_hip_step_buffers& b, const nemo_hip_neuron_group& g, TileInput& in,
+		Prefetch prefetch = Prefetch())
 {
 	__shared__ uint32_t s_pre[NEMO_MAX_BIN_BLOCKS + 1];
 	__shared__ uint32_t s_base[NEMO_MAX_BIN_BLOCKS];
@@ -114,8 +123,6 @@ __device__ inline void gather_input(const nemo_hip_step_buffers& b, const nemo_h
 	const size_t parity = current_step(b) & 1;
 	int64_t* acc = b.acc + parity * 2 * b.acc_pitch;
 	const unsigned ng = tile * NEMO_TILE + tid;
-	const int64_t oe = acc[ng];
-	const int64_t oi = acc[b.acc_pitch + ng];
 	const unsigned B = b.bin_blocks;
 	const uint32_t* row = b.bin_table + (parity * b.tiles + tile) * B;
 	const size_t cap = b.bin_capacity;
@@ -127,6 +134,9 @@ __device__ inline void gather_input(const nemo_hip_step_buffers& b, const nemo_h
 		const unsigned i = tid * PER + k;
 		ent[k] = i < B ? row[i] : 0u;
 	}
+	const int64_t oe = acc[ng];
+	const int64_t oi = acc[b.acc_pitch + ng];
+	prefetch();
 	in.E[tid] = oe;
 	in.I[tid] = oi;
 	if(oe) acc[ng] = 0;

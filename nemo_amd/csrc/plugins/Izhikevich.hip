@@ -31,23 +31,24 @@ izhikevich_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 	const unsigned ng = g.start + nl;
 	const size_t P = g.pitch;
 
-	/* the neuron's parameters, state, RNG and history are loaded before the
-	 * input gather, so that their latency overlaps it */
+	/* the neuron's parameters, state, RNG and history are requested right
+	 * after the tile's input table, so that their latency overlaps the gather */
 	float sigma = 0.0f, a = 0.0f, bb = 0.0f, c = 0.0f, d = 0.0f, u = 0.0f, v = 0.0f;
 	uint4 s = make_uint4(0, 0, 0, 0);
 	uint64_t h = 0;
-	if(valid) {
-		sigma = g.param[PARAM_SIGMA * P + nl];
-		a = g.param[PARAM_A * P + nl];
-		bb = g.param[PARAM_B * P + nl];
-		c = g.param[PARAM_C * P + nl];
-		d = g.param[PARAM_D * P + nl];
-		u = g.state[STATE_U * P + nl];
-		v = g.state[STATE_V * P + nl];
-		if(g.rng) s = reinterpret_cast<const uint4*>(g.rng)[nl];
-		h = nemo_dev::load_history(b, ng, true);
-	}
-	nemo_dev::gather_input(b, g, input);
+	nemo_dev::gather_input(b, g, input, [&]() {
+		if(valid) {
+			sigma = g.param[PARAM_SIGMA * P + nl];
+			a = g.param[PARAM_A * P + nl];
+			bb = g.param[PARAM_B * P + nl];
+			c = g.param[PARAM_C * P + nl];
+			d = g.param[PARAM_D * P + nl];
+			u = g.state[STATE_U * P + nl];
+			v = g.state[STATE_V * P + nl];
+			if(g.rng) s = reinterpret_cast<const uint4*>(g.rng)[nl];
+			h = nemo_dev::load_history(b, ng, true);
+		}
+	});
 	bool fired = false;
 
 	if(valid) {

@@ -21,10 +21,13 @@ qif_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 	const bool valid = nl < g.count;
 	const unsigned ng = g.start + nl;
 	/* state, parameter and history load while the input is gathered */
-	const uint64_t h = nemo_dev::load_history(b, ng, valid);
-	float v = valid ? g.state[nl] : 0.0f;
-	const float a = valid ? g.param[nl] : 0.0f;
-	nemo_dev::gather_input(b, g, input);
+	uint64_t h = 0;
+	float v = 0.0f, a = 0.0f;
+	nemo_dev::gather_input(b, g, input, [&]() {
+		h = nemo_dev::load_history(b, ng, valid);
+		v = valid ? g.state[nl] : 0.0f;
+		a = valid ? g.param[nl] : 0.0f;
+	});
 	bool fired = false;
 
 	if(valid) {
