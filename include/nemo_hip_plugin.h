@@ -29,10 +29,10 @@
  *   - apply one 1 ms step of the model; a neuron fires if the model says so or
  *     if its bit in fstim is set (fstim may be NULL);
  *   - shift the neuron's 64-bit firing history left by one, OR-ing in firing:
- *     the history after step s is in plane s & hist_mask of hist (plane p
- *     starts at p * hist_pitch; hist_mask + 1 planes, a power of two, at
- *     least 2), so the update reads plane (s-1) & hist_mask and writes plane
- *     s & hist_mask;
+ *     the history of neuron n after step s is
+ *     hist[NEMO_HIST_INDEX(n, s, hist_pitch)] (plane s % 2, word 2n; word
+ *     2n+1 of both planes holds the snapshot an STDP batch starts from), so
+ *     the update reads the word of step s-1 and writes the word of step s;
  *   - write the tile's words of the step's firing bitmap,
  *     fired_bits[(s % NEMO_STEP_RING) * fired_words + index / 64];
  *   - when stdp.rcm_ptr is not NULL, run the STDP accumulation of step s-1 for
@@ -59,6 +59,11 @@ extern "C" {
 #define NEMO_STEP_RING 128
 /* Most delivery blocks (bin regions) a step's input can come from. */
 #define NEMO_MAX_BIN_BLOCKS 1024
+/* Firing history: two planes of hist_pitch words, plane s % 2 after step s;
+ * in a plane, neuron n's history at word 2n and, at 2n+1, the snapshot an
+ * STDP batch starts from, so that a history word and the snapshot are one
+ * 16-byte load. */
+#define NEMO_HIST_INDEX(n, s, pitch) (((s) & 1) * size_t(pitch) + 2 * size_t(n))
 
 /* Device-resident step base, set by the host before launches. */
 typedef struct nemo_hip_clock {
@@ -124,10 +129,10 @@ typedef struct nemo_hip_stdp {
  * (the contract of nemo_hip_stdp) may share the launch that delivers step t
  * after the update of step t-1 (blocks after the delivery blocks), run as a
  * launch of its own (stdp_kernel), or be left to the update kernels.  It
- * reads the firing history after step t-1, plane (t-1) & hist_mask of hist.
- * The accumulation of up to 64 past steps may also run in one batch launch
- * after them (stdp_batch_kernel), from the history after the last of them and
- * hist_snap.  Skipped
+ * reads the firing history after step t-1.  The accumulation of up to 64
+ * past steps may also run in one batch launch after them (stdp_batch_kernel),
+ * from the history after the last of them and the snapshot words of the
+ * history records.  Skipped
  * when t == 0 or when step t-1's accumulation already ran
  * (clock->stdp_flushed == t-1). */
 typedef struct nemo_propagate_args {
@@ -155,10 +160,8 @@ typedef struct nemo_propagate_args {
 	unsigned long long* block_events; /* [table_columns] events delivered, accumulated */
 	/* STDP in its own launch */
 	unsigned stdp_words;             /* bitmap words (64 targets) per STDP block */
-	const uint64_t* hist;            /* hist_mask + 1 planes of hist_pitch */
+	uint64_t* hist;                  /* two planes of hist_pitch words (NEMO_HIST_INDEX) */
 	size_t hist_pitch;
-	unsigned hist_mask;
-	uint64_t* hist_snap;             /* STDP batches: history after the step before the batch */
 	nemo_hip_stdp stdp;              /* examined: one counter per STDP block */
 } nemo_propagate_args;
 
@@ -193,9 +196,8 @@ typedef struct nemo_hip_step_buffers {
 	size_t acc_pitch;
 	float* ext_current;        /* per device index, or NULL */
 	const uint32_t* fstim;     /* firing-stimulus bit-vector, or NULL */
-	uint64_t* hist;            /* 64-bit firing history: hist_mask + 1 planes of hist_pitch */
+	uint64_t* hist;            /* 64-bit firing history: two planes of hist_pitch words (NEMO_HIST_INDEX) */
 	size_t hist_pitch;
-	unsigned hist_mask;
 	uint64_t* fired_bits;      /* NEMO_STEP_RING bitmaps of fired_words words */
 	size_t fired_words;
 	unsigned fbits;            /* fractional bits of the fixed-point currents */

@@ -1124,9 +1124,7 @@ void Simulation::applyShape(const LaunchShape& s)
 	a.block_events = m_blockEvents.ptr;
 	a.stdp_words = m_stdpBlocks ? s.stdpWords : 0;
 	a.hist = m_hist.ptr;
-	a.hist_pitch = m_accPitch;
-	a.hist_mask = m_histPlanes - 1;
-	a.hist_snap = m_histSnap.ptr;
+	a.hist_pitch = 2 * m_accPitch;
 	a.stdp = m_stdpArgs;
 
 	m_early = a;
@@ -1149,12 +1147,17 @@ void Simulation::applyShape(const LaunchShape& s)
 /* The STDP batch snapshot: the history after the last step taken. */
 void Simulation::refreshHistSnap()
 {
-	if(!m_histSnap.ptr) return;
-	if(m_hostStep == 0) {
-		HIP_CHECK(hipMemsetAsync(m_histSnap.ptr, 0, m_histSnap.bytes(), m_stream));
-	} else {
-		const uint64_t* plane = m_hist.ptr + size_t((m_hostStep - 1) & (m_histPlanes - 1)) * m_accPitch;
-		HIP_CHECK(hipMemcpyAsync(m_histSnap.ptr, plane, m_histSnap.bytes(), hipMemcpyDeviceToDevice, m_stream));
+	if(!(m_stdp && m_nplastic)) return;
+	const size_t pitch = 2 * m_accPitch, rec = 2 * sizeof(uint64_t);
+	const uint64_t* last = m_hist.ptr + (m_hostStep == 0 ? 0 : NEMO_HIST_INDEX(0, m_hostStep - 1, pitch));
+	for(size_t p = 0; p < 2; ++p) {
+		uint64_t* snap = m_hist.ptr + p * pitch + 1;
+		if(m_hostStep == 0) {
+			HIP_CHECK(hipMemset2DAsync(snap, rec, 0, sizeof(uint64_t), m_accPitch, m_stream));
+		} else {
+			HIP_CHECK(hipMemcpy2DAsync(snap, rec, last, rec, sizeof(uint64_t), m_accPitch,
+						hipMemcpyDeviceToDevice, m_stream));
+		}
 	}
 	HIP_CHECK(hipStreamSynchronize(m_stream));
 }
@@ -1183,14 +1186,10 @@ void Simulation::allocateStepState()
 	m_accPitch = N;
 	m_acc.allocate(4 * m_accPitch); /* [step parity][E, I][N] */
 	HIP_CHECK(hipMemset(m_acc.ptr, 0, m_acc.bytes()));
-	/* plane s % 2 holds the history after step s; with STDP, a snapshot of
-	 * an older step lets a batch accumulate up to 64 steps afterwards */
-	m_histPlanes = 2;
-	m_hist.allocate(size_t(m_histPlanes) * N);
-	if(m_stdp && m_nplastic) {
-		m_histSnap.allocate(N);
-		HIP_CHECK(hipMemset(m_histSnap.ptr, 0, m_histSnap.bytes()));
-	}
+	/* two planes (step parity) of {history, snapshot} per neuron: the
+	 * snapshot of an older step lets an STDP batch accumulate up to 64 steps
+	 * afterwards */
+	m_hist.allocate(4 * N);
 	HIP_CHECK(hipMemset(m_hist.ptr, 0, m_hist.bytes()));
 	m_ext.allocate(N);
 	HIP_CHECK(hipMemset(m_ext.ptr, 0, m_ext.bytes()));
@@ -1234,8 +1233,7 @@ nemo_hip_step_buffers Simulation::stepBuffers(uint64_t offset, bool ext, bool fs
 	b.ext_current = ext ? m_ext.ptr : nullptr;
 	b.fstim = fstim ? m_fstim.ptr : nullptr;
 	b.hist = m_hist.ptr;
-	b.hist_pitch = m_accPitch;
-	b.hist_mask = m_histPlanes - 1;
+	b.hist_pitch = 2 * m_accPitch;
 	b.fired_bits = m_firedBits.ptr;
 	b.fired_words = m_firedWords;
 	b.fbits = m_fbits;
@@ -2093,8 +2091,10 @@ void Simulation::getRecentFiring(uint64_t* h, size_t n)
 	activate();
 	HIP_CHECK(hipStreamSynchronize(m_stream));
 	std::vector<uint64_t> d(m_npad);
-	const size_t plane = size_t((m_hostStep - 1) & (m_histPlanes - 1)); /* after the last step */
-	HIP_CHECK(hipMemcpy(d.data(), m_hist.ptr + plane * m_accPitch, m_npad * 8, hipMemcpyDeviceToHost));
+	/* the history words after the last step */
+	const size_t rec = 2 * sizeof(uint64_t);
+	const uint64_t* last = m_hist.ptr + NEMO_HIST_INDEX(0, m_hostStep == 0 ? 0 : m_hostStep - 1, 2 * m_accPitch);
+	HIP_CHECK(hipMemcpy2D(d.data(), sizeof(uint64_t), last, rec, sizeof(uint64_t), m_npad, hipMemcpyDeviceToHost));
 	for(size_t i = 0; i < n; ++i) h[i] = d[m_order[i]];
 }
 

@@ -272,9 +272,7 @@ private:
 	size_t m_firedWords = 0;
 	DeviceArray<int64_t> m_acc;           /* overflow accumulators */
 	size_t m_accPitch = 0;
-	DeviceArray<uint64_t> m_hist;
-	unsigned m_histPlanes = 2;
-	DeviceArray<uint64_t> m_histSnap; /* STDP batches: history after the step before the batch */
+	DeviceArray<uint64_t> m_hist;     /* [2][N][2]: history, STDP snapshot (NEMO_HIST_INDEX) */
 	DeviceArray<float> m_ext;
 	DeviceArray<uint32_t> m_fstim;
 	DeviceArray<float> m_debugCurrent;

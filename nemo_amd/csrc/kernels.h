@@ -34,9 +34,9 @@ extern "C" {
 size_t nemo_propagate_lds_bytes(unsigned max_window, unsigned bin_capacity);
 /* STDP accumulation of the nsteps (1..64) steps from a->step_offset on, in
  * one launch of stdp_blocks blocks, from the history after the last of them
- * and a->hist_snap (the history after the step before the first); then
- * hist_snap takes the history after the last step and clock->stdp_flushed
- * is set to it. */
+ * and the snapshot words of the history records (the history after the step
+ * before the first); then the snapshot takes the history after the last step
+ * and clock->stdp_flushed is set to it. */
 hipError_t nemo_launch_stdp_batch(hipStream_t, const nemo_propagate_args*, unsigned nsteps, unsigned stdp_blocks);
 /* dynamic LDS an update launch carrying propagate blocks needs (delivery
  * blocks when delivery is set, else STDP blocks only) */

@@ -13,8 +13,9 @@
  *                          {target, Q11.20 weight}, 8 bytes, so consecutive
  *                          lanes read consecutive 8-byte words of a row
  *   fired_bits uint64[128][N/64]  one firing bitmap per step (ring)
- *   hist       uint64[2][N] firing history (bit k: fired k steps ago); plane
- *                          s % 2 holds the history after step s
+ *   hist       uint64[2][N][2] firing history (bit k: fired k steps ago):
+ *                          plane s % 2 holds {history after step s, STDP
+ *                          snapshot} per neuron (NEMO_HIST_INDEX)
  *   bins       int2[2][B][CAP] each delivery block's events, sorted by tile,
  *                          for steps of even and odd parity
  *   bin_table  uint32[2][T][B] count << 16 | offset of tile t in block b's
@@ -77,10 +78,11 @@ __global__ void __launch_bounds__(NEMO_BLOCK)
 stdp_snapshot_kernel(nemo_propagate_args a, unsigned nsteps)
 {
 	const uint64_t rl = a.clock->base_step + a.step_offset + nsteps - 1;
-	const uint64_t* cur = a.hist + (rl & a.hist_mask) * a.hist_pitch;
 	const size_t n = a.fired_words * 64;
 	for(size_t i = size_t(blockIdx.x) * NEMO_BLOCK + threadIdx.x; i < n; i += size_t(gridDim.x) * NEMO_BLOCK) {
-		a.hist_snap[i] = cur[i];
+		const uint64_t h = a.hist[NEMO_HIST_INDEX(i, rl, a.hist_pitch)];
+		a.hist[2 * i + 1] = h;
+		a.hist[a.hist_pitch + 2 * i + 1] = h;
 	}
 	if(blockIdx.x == 0 && threadIdx.x == 0) const_cast<nemo_hip_clock*>(a.clock)->stdp_flushed = rl;
 }
@@ -155,7 +157,7 @@ size_t nemo_propagate_lds_bytes(unsigned max_window, unsigned bin_capacity)
 hipError_t nemo_launch_stdp_batch(hipStream_t st, const nemo_propagate_args* a, unsigned nsteps, unsigned stdp_blocks)
 {
 	if(nsteps == 0 || stdp_blocks == 0) return hipSuccess;
-	if(nsteps > 64 || !a->hist_snap || a->stdp.post_window > 63 || a->stdp_words == 0
+	if(nsteps > 64 || a->stdp.post_window > 63 || a->stdp_words == 0
 			|| a->stdp_words > NEMO_STDP_MAX_WORDS || size_t(stdp_blocks) * a->stdp_words < a->fired_words
 			|| !a->stdp.rcm_ptr || !a->hist) {
 		return hipErrorInvalidValue;

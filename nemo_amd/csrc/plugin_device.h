@@ -297,16 +297,10 @@ __device__ inline void store_firing(const nemo_hip_step_buffers& b, unsigned ng,
 	}
 }
 
-/* The firing history after step s lives in plane s & hist_mask. */
-__device__ inline const uint64_t* history_after(const nemo_hip_step_buffers& b, uint64_t s)
-{
-	return b.hist + (s & b.hist_mask) * b.hist_pitch;
-}
-
 /* History of neuron ng before this step (after the previous one). */
 __device__ inline uint64_t load_history(const nemo_hip_step_buffers& b, unsigned ng, bool valid)
 {
-	return valid ? history_after(b, current_step(b) - 1)[ng] : 0;
+	return valid ? b.hist[NEMO_HIST_INDEX(ng, current_step(b) - 1, b.hist_pitch)] : 0;
 }
 
 /* Store the shifted history (h as from load_history) and write the step's
@@ -314,8 +308,7 @@ __device__ inline uint64_t load_history(const nemo_hip_step_buffers& b, unsigned
 __device__ inline void record_firing(const nemo_hip_step_buffers& b, unsigned ng, bool valid, bool fired, uint64_t h)
 {
 	if(valid) {
-		uint64_t* next = b.hist + (current_step(b) & b.hist_mask) * b.hist_pitch;
-		next[ng] = (h << 1) | uint64_t(fired ? 1 : 0);
+		b.hist[NEMO_HIST_INDEX(ng, current_step(b), b.hist_pitch)] = (h << 1) | uint64_t(fired ? 1 : 0);
 	}
 	store_firing(b, ng, valid, fired);
 }
@@ -381,7 +374,7 @@ __device__ inline void stdp_tile(const nemo_hip_step_buffers& b, unsigned ng, bo
 	__syncthreads();
 	if(total == 0) return;
 
-	const uint64_t* pre_hist = history_after(b, r);
+	const uint64_t* pre_hist = b.hist + NEMO_HIST_INDEX(0, r, b.hist_pitch);
 	for(uint32_t e0 = tid; e0 < total; e0 += ILP * NEMO_TILE) {
 		uint32_t k[ILP];
 		uint32_t sd[ILP];
@@ -406,7 +399,7 @@ __device__ inline void stdp_tile(const nemo_hip_step_buffers& b, unsigned ng, bo
 #pragma unroll
 		for(int u = 0; u < ILP; ++u) {
 			ph[u] = 0;
-			if(e0 + u * NEMO_TILE < total) ph[u] = pre_hist[sd[u] & 0x3ffffffu];
+			if(e0 + u * NEMO_TILE < total) ph[u] = pre_hist[2 * size_t(sd[u] & 0x3ffffffu)];
 		}
 #pragma unroll
 		for(int u = 0; u < ILP; ++u) {

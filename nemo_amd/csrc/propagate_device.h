@@ -375,7 +375,7 @@ __device__ inline void stdp_block(const nemo_propagate_args& a, uint64_t r, unsi
 	__syncthreads();
 	if(total == 0) return;
 
-	const uint64_t* hist = a.hist + (r & a.hist_mask) * a.hist_pitch;
+	const uint64_t* hist = a.hist + NEMO_HIST_INDEX(0, r, a.hist_pitch);
 	for(uint32_t e0 = tid; e0 < total; e0 += ILP * THREADS) {
 		uint32_t k[ILP];
 		uint32_t sd[ILP];
@@ -400,7 +400,7 @@ __device__ inline void stdp_block(const nemo_propagate_args& a, uint64_t r, unsi
 #pragma unroll
 		for(int u = 0; u < ILP; ++u) {
 			h[u] = 0;
-			if(e0 + u * THREADS < total) h[u] = hist[sd[u] & 0x3ffffffu];
+			if(e0 + u * THREADS < total) h[u] = hist[2 * size_t(sd[u] & 0x3ffffffu)];
 		}
 #pragma unroll
 		for(int u = 0; u < ILP; ++u) {
@@ -434,8 +434,8 @@ __device__ inline uint64_t history_window(uint64_t c, uint64_t s, unsigned n, un
 
 /* STDP accumulation of the n (1..64) steps r0 .. r0+n-1 for the targets of
  * block sb's bitmap words, from two history planes only: the history after
- * the last of them (plane rl of hist) and after the step before the first
- * (a.hist_snap).  Together they hold every neuron's firing over the 128 steps
+ * the last of them and after the step before the first (the snapshot words
+ * of the history records).  Together they hold every neuron's firing over the 128 steps
  * the n accumulations look at: a target fired at step r - P when bit
  * rl - r + P of its record is set, and a source's history after step r is
  * the window of its record at rl - r.  Each thread owns a synapse for all the
@@ -458,8 +458,8 @@ __device__ inline void stdp_batch_block(const nemo_propagate_args& a, uint64_t r
 		s_fn[64 + tid] = st.fn_post[tid];
 	}
 	const uint64_t rl = r0 + n - 1;
-	const uint64_t* cur = a.hist + (rl & a.hist_mask) * a.hist_pitch;
-	const uint64_t* snap = a.hist_snap;
+	/* the history after rl and the snapshot: one 16-byte load per neuron */
+	const ulonglong2* rec = reinterpret_cast<const ulonglong2*>(a.hist + NEMO_HIST_INDEX(0, rl, a.hist_pitch));
 	/* bit y: step rl - y is accumulated (in the batch, r >= P, not flushed) */
 	uint64_t due = n >= 64 ? ~uint64_t(0) : (uint64_t(1) << n) - 1;
 	if(rl < P) {
@@ -475,7 +475,8 @@ __device__ inline void stdp_batch_block(const nemo_propagate_args& a, uint64_t r
 	uint64_t post = 0; /* bit y: the target fired at step rl - y - P */
 	if(tid / 64 < a.stdp_words && word < a.fired_words) {
 		const uint32_t tn = uint32_t(w0 * 64 + tid);
-		post = history_window(cur[tn], snap[tn], n, P) & due;
+		const ulonglong2 cs = rec[tn];
+		post = history_window(cs.x, cs.y, n, P) & due;
 	}
 	const bool fired = post != 0;
 	uint32_t nfired;
@@ -528,8 +529,9 @@ __device__ inline void stdp_batch_block(const nemo_propagate_args& a, uint64_t r
 			hc[u] = 0;
 			hs[u] = 0;
 			if(e0 + u * THREADS < total) {
-				hc[u] = cur[sd[u] & 0x3ffffffu];
-				hs[u] = snap[sd[u] & 0x3ffffffu];
+				const ulonglong2 cs = rec[sd[u] & 0x3ffffffu];
+				hc[u] = cs.x;
+				hs[u] = cs.y;
 			}
 		}
 #pragma unroll
