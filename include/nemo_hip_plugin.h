@@ -30,9 +30,8 @@
  *     if its bit in fstim is set (fstim may be NULL);
  *   - shift the neuron's 64-bit firing history left by one, OR-ing in firing:
  *     the history of neuron n after step s is
- *     hist[NEMO_HIST_INDEX(n, s, hist_pitch)] (plane s % 2, word 2n; word
- *     2n+1 of both planes holds the snapshot an STDP batch starts from), so
- *     the update reads the word of step s-1 and writes the word of step s;
+ *     hist[NEMO_HIST_INDEX(n, s, hist_pitch)] (plane s % 2), so the update
+ *     reads the word of step s-1 and writes the word of step s;
  *   - write the tile's words of the step's firing bitmap,
  *     fired_bits[(s % NEMO_STEP_RING) * fired_words + index / 64];
  *   - when stdp.rcm_ptr is not NULL, run the STDP accumulation of step s-1 for
@@ -59,11 +58,8 @@ extern "C" {
 #define NEMO_STEP_RING 128
 /* Most delivery blocks (bin regions) a step's input can come from. */
 #define NEMO_MAX_BIN_BLOCKS 1024
-/* Firing history: two planes of hist_pitch words, plane s % 2 after step s;
- * in a plane, neuron n's history at word 2n and, at 2n+1, the snapshot an
- * STDP batch starts from, so that a history word and the snapshot are one
- * 16-byte load. */
-#define NEMO_HIST_INDEX(n, s, pitch) (((s) & 1) * size_t(pitch) + 2 * size_t(n))
+/* Firing history: two planes of hist_pitch words, plane s % 2 after step s. */
+#define NEMO_HIST_INDEX(n, s, pitch) (((s) & 1) * size_t(pitch) + size_t(n))
 
 /* Device-resident step base, set by the host before launches. */
 typedef struct nemo_hip_clock {
@@ -71,6 +67,14 @@ typedef struct nemo_hip_clock {
 	uint64_t cycle_origin; /* step of the last timer reset: cycle = step - cycle_origin
 	                          (Timer, src/nemo/Timer.hpp:23-79) */
 	uint64_t stdp_flushed; /* step whose STDP accumulation already ran on its own (~0: none) */
+	/* STDP batches: snapshots of the firing history taken so far (the last
+	 * in slot stdp_snaps % 2, the one before in the other slot), the step of
+	 * the last, and the steps [stdp_first, stdp_first + stdp_count) between
+	 * the two whose accumulation is pending (stdp_count 0: none) */
+	uint64_t stdp_snaps;
+	uint64_t stdp_last;
+	uint64_t stdp_first;
+	uint64_t stdp_count;
 } nemo_hip_clock;
 
 /* One neuron type's state: structure of arrays, plane p of a quantity starts
@@ -130,9 +134,9 @@ typedef struct nemo_hip_stdp {
  * after the update of step t-1 (blocks after the delivery blocks), run as a
  * launch of its own (stdp_kernel), or be left to the update kernels.  It
  * reads the firing history after step t-1.  The accumulation of up to 64
- * past steps may also run in one batch launch after them (stdp_batch_kernel),
- * from the history after the last of them and the snapshot words of the
- * history records.  Skipped
+ * past steps may also run in one batch launch (stdp_batch_kernel) from two
+ * snapshots of the firing history in stdp_snap, taken after the last of them
+ * and after the step before the first.  Skipped
  * when t == 0 or when step t-1's accumulation already ran
  * (clock->stdp_flushed == t-1). */
 typedef struct nemo_propagate_args {
@@ -162,6 +166,7 @@ typedef struct nemo_propagate_args {
 	unsigned stdp_words;             /* bitmap words (64 targets) per STDP block */
 	uint64_t* hist;                  /* two planes of hist_pitch words (NEMO_HIST_INDEX) */
 	size_t hist_pitch;
+	uint64_t* stdp_snap;             /* STDP batches: two history snapshot slots per neuron (2n, 2n+1) */
 	nemo_hip_stdp stdp;              /* examined: one counter per STDP block */
 } nemo_propagate_args;
 

@@ -1124,7 +1124,8 @@ void Simulation::applyShape(const LaunchShape& s)
 	a.block_events = m_blockEvents.ptr;
 	a.stdp_words = m_stdpBlocks ? s.stdpWords : 0;
 	a.hist = m_hist.ptr;
-	a.hist_pitch = 2 * m_accPitch;
+	a.hist_pitch = m_accPitch;
+	a.stdp_snap = m_stdpSnap.ptr;
 	a.stdp = m_stdpArgs;
 
 	m_early = a;
@@ -1141,24 +1142,25 @@ void Simulation::applyShape(const LaunchShape& s)
 	m_late.d_count = 1;
 	m_late.pair_bounds = m_pairBounds.ptr + Pe.bounds.size();
 	m_late.windows = m_windows.ptr + Be;
-	refreshHistSnap();
+	resetStdpBatches();
 }
 
-/* The STDP batch snapshot: the history after the last step taken. */
-void Simulation::refreshHistSnap()
+/* STDP batches start over from a snapshot of the history after the last
+ * step taken, with nothing pending (every earlier step is accumulated). */
+void Simulation::resetStdpBatches()
 {
-	if(!(m_stdp && m_nplastic)) return;
-	const size_t pitch = 2 * m_accPitch, rec = 2 * sizeof(uint64_t);
-	const uint64_t* last = m_hist.ptr + (m_hostStep == 0 ? 0 : NEMO_HIST_INDEX(0, m_hostStep - 1, pitch));
-	for(size_t p = 0; p < 2; ++p) {
-		uint64_t* snap = m_hist.ptr + p * pitch + 1;
-		if(m_hostStep == 0) {
-			HIP_CHECK(hipMemset2DAsync(snap, rec, 0, sizeof(uint64_t), m_accPitch, m_stream));
-		} else {
-			HIP_CHECK(hipMemcpy2DAsync(snap, rec, last, rec, sizeof(uint64_t), m_accPitch,
-						hipMemcpyDeviceToDevice, m_stream));
-		}
+	if(!m_stdpSnap.ptr) return;
+	if(m_hostStep == 0) {
+		HIP_CHECK(hipMemsetAsync(m_stdpSnap.ptr, 0, m_stdpSnap.bytes(), m_stream));
+	} else {
+		const uint64_t* last = m_hist.ptr + NEMO_HIST_INDEX(0, m_hostStep - 1, m_accPitch);
+		HIP_CHECK(hipMemcpy2DAsync(m_stdpSnap.ptr, 2 * sizeof(uint64_t), last, sizeof(uint64_t), sizeof(uint64_t),
+					m_accPitch, hipMemcpyDeviceToDevice, m_stream));
 	}
+	copyScalar(&m_clock.ptr->stdp_snaps, 0);
+	copyScalar(&m_clock.ptr->stdp_last, m_hostStep - 1);
+	copyScalar(&m_clock.ptr->stdp_first, m_hostStep);
+	copyScalar(&m_clock.ptr->stdp_count, 0);
 	HIP_CHECK(hipStreamSynchronize(m_stream));
 }
 
@@ -1186,10 +1188,12 @@ void Simulation::allocateStepState()
 	m_accPitch = N;
 	m_acc.allocate(4 * m_accPitch); /* [step parity][E, I][N] */
 	HIP_CHECK(hipMemset(m_acc.ptr, 0, m_acc.bytes()));
-	/* two planes (step parity) of {history, snapshot} per neuron: the
-	 * snapshot of an older step lets an STDP batch accumulate up to 64 steps
-	 * afterwards */
-	m_hist.allocate(4 * N);
+	m_hist.allocate(2 * N); /* plane s % 2 holds the history after step s */
+	if(m_stdp && m_nplastic) {
+		/* two history snapshots per neuron, from which an STDP batch
+		 * accumulates up to 64 steps between them */
+		m_stdpSnap.allocate(2 * N);
+	}
 	HIP_CHECK(hipMemset(m_hist.ptr, 0, m_hist.bytes()));
 	m_ext.allocate(N);
 	HIP_CHECK(hipMemset(m_ext.ptr, 0, m_ext.bytes()));
@@ -1233,7 +1237,7 @@ nemo_hip_step_buffers Simulation::stepBuffers(uint64_t offset, bool ext, bool fs
 	b.ext_current = ext ? m_ext.ptr : nullptr;
 	b.fstim = fstim ? m_fstim.ptr : nullptr;
 	b.hist = m_hist.ptr;
-	b.hist_pitch = 2 * m_accPitch;
+	b.hist_pitch = m_accPitch;
 	b.fired_bits = m_firedBits.ptr;
 	b.fired_words = m_firedWords;
 	b.fbits = m_fbits;
@@ -1297,7 +1301,16 @@ void Simulation::enqueueSteps(uint64_t first, unsigned k, bool ext, bool fstim, 
 	const bool fused = m_shape.stdpMode == 2 && m_stdpBlocks;
 	const bool lagged = m_shape.stdpMode == 3 && m_stdpBlocks;
 	const bool batched = m_shape.stdpMode == 4 && m_stdpBlocks;
-	unsigned batchFrom = 0; /* first step of the pending STDP batch */
+	/* the pending STDP batch runs on its own branch beside the steps; it
+	 * reads only the history snapshots, and is joined before the next
+	 * snapshot replaces the older of them */
+	auto forkBatch = [&]() {
+		HIP_CHECK(hipEventRecord(m_stdpFork, m_stream));
+		HIP_CHECK(hipStreamWaitEvent(m_stdpStream, m_stdpFork, 0));
+		HIP_CHECK(nemo_launch_stdp_batch(m_stdpStream, &m_propagate, m_stdpBlocks));
+		HIP_CHECK(hipEventRecord(m_stdpDone, m_stdpStream));
+	};
+	if(batched && k) forkBatch();
 	bool pending[2] = {false, false};
 	auto forkEarly = [&](unsigned i) {
 		HIP_CHECK(hipEventRecord(m_earlyFork, m_stream));
@@ -1362,13 +1375,12 @@ void Simulation::enqueueSteps(uint64_t first, unsigned k, bool ext, bool fstim, 
 		}
 		if(ev) HIP_CHECK(hipEventRecord(ev[3 * i + 2], m_stream));
 		if(overlap && i + 2 < k) forkEarly(i + 2);
-		if(batched && (i + 1 == k || i + 1 - batchFrom == 64)) {
-			/* the accumulation of the steps since the last batch: their
-			 * history planes (64 of them) are all still intact */
-			nemo_propagate_args sa = m_propagate;
-			sa.step_offset = first + batchFrom;
-			HIP_CHECK(nemo_launch_stdp_batch(m_stream, &sa, i + 1 - batchFrom, m_stdpBlocks));
-			batchFrom = i + 1;
+		if(batched && (i + 1 == k || (i + 1) % 64 == 0)) {
+			/* snapshot after this step: the steps since the last one are
+			 * pending for the next batch */
+			HIP_CHECK(hipStreamWaitEvent(m_stream, m_stdpDone, 0));
+			HIP_CHECK(nemo_launch_stdp_snapshot(m_stream, &m_propagate, first + i));
+			if(i + 1 < k) forkBatch();
 		}
 	}
 	if(lagged && k) HIP_CHECK(hipStreamWaitEvent(m_stream, m_stdpLag[(k - 1) & 1], 0));
@@ -1387,8 +1399,16 @@ void Simulation::joinStdp()
 void Simulation::flushStdp()
 {
 	joinStdp();
-	if(m_shape.stdpMode == 4) return; /* every run of steps ends with its batch */
 	if(!m_stdpBlocks || m_hostStep == 0 || m_flushed == m_hostStep - 1) return;
+	if(m_shape.stdpMode == 4) {
+		/* the pending batch: every run of steps ends with a snapshot */
+		HIP_CHECK(nemo_launch_stdp_batch(m_stream, &m_propagate, m_stdpBlocks));
+		copyScalar(&m_clock.ptr->stdp_count, 0);
+		copyScalar(&m_clock.ptr->stdp_flushed, m_hostStep - 1);
+		m_flushed = m_hostStep - 1;
+		HIP_CHECK(hipStreamSynchronize(m_stream));
+		return;
+	}
 	const uint64_t s = m_hostStep - 1;
 	nemo_propagate_args a = m_propagate;
 	a.deliver_blocks = 0;
@@ -1709,6 +1729,7 @@ void Simulation::takeSnapshot(Snapshot& s)
 	add(m_firedBits.ptr, m_firedBits.bytes());
 	add(m_acc.ptr, m_acc.bytes());
 	add(m_hist.ptr, m_hist.bytes());
+	add(m_stdpSnap.ptr, m_stdpSnap.bytes());
 	add(m_ext.ptr, m_ext.bytes());
 	add(m_rcmAcc.ptr, m_rcmAcc.bytes());
 	add(m_blockEvents.ptr, m_blockEvents.bytes());
@@ -1736,7 +1757,6 @@ void Simulation::restoreSnapshot(const Snapshot& s)
 	HIP_CHECK(hipStreamSynchronize(m_stream));
 	m_hostStep = s.hostStep;
 	m_flushed = s.flushed;
-	refreshHistSnap();
 }
 
 /* Advance nsteps steps in graphs of up to 64 steps; seconds on the stream. */
@@ -1771,6 +1791,7 @@ LaunchShape Simulation::autotune(unsigned stepsPerTrial)
 	activate();
 	if(m_nsynapses == 0 || stepsPerTrial == 0) return m_shape;
 	stepsPerTrial = std::min(stepsPerTrial, 1024u);
+	flushStdp(); /* nothing pending across shapes */
 	Snapshot snap;
 	takeSnapshot(snap);
 	const LaunchShape start = m_shape;
@@ -2092,9 +2113,8 @@ void Simulation::getRecentFiring(uint64_t* h, size_t n)
 	HIP_CHECK(hipStreamSynchronize(m_stream));
 	std::vector<uint64_t> d(m_npad);
 	/* the history words after the last step */
-	const size_t rec = 2 * sizeof(uint64_t);
-	const uint64_t* last = m_hist.ptr + NEMO_HIST_INDEX(0, m_hostStep == 0 ? 0 : m_hostStep - 1, 2 * m_accPitch);
-	HIP_CHECK(hipMemcpy2D(d.data(), sizeof(uint64_t), last, rec, sizeof(uint64_t), m_npad, hipMemcpyDeviceToHost));
+	const uint64_t* last = m_hist.ptr + NEMO_HIST_INDEX(0, m_hostStep == 0 ? 1 : m_hostStep - 1, m_accPitch);
+	HIP_CHECK(hipMemcpy(d.data(), last, m_npad * 8, hipMemcpyDeviceToHost));
 	for(size_t i = 0; i < n; ++i) h[i] = d[m_order[i]];
 }
 

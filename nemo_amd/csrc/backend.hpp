@@ -272,7 +272,8 @@ private:
 	size_t m_firedWords = 0;
 	DeviceArray<int64_t> m_acc;           /* overflow accumulators */
 	size_t m_accPitch = 0;
-	DeviceArray<uint64_t> m_hist;     /* [2][N][2]: history, STDP snapshot (NEMO_HIST_INDEX) */
+	DeviceArray<uint64_t> m_hist;     /* [2][N] firing history (NEMO_HIST_INDEX) */
+	DeviceArray<uint64_t> m_stdpSnap; /* [N][2] history snapshots for STDP batches */
 	DeviceArray<float> m_ext;
 	DeviceArray<uint32_t> m_fstim;
 	DeviceArray<float> m_debugCurrent;
@@ -338,7 +339,7 @@ private:
 	LaunchShape defaultShape() const;
 	void applyShape(const LaunchShape& s);
 	size_t ldsNeeded(const LaunchShape& s) const;
-	void refreshHistSnap();
+	void resetStdpBatches();
 	void checkShape(const LaunchShape& s) const;
 	nemo_hip_step_buffers stepBuffers(uint64_t offset, bool ext, bool fstim) const;
 	void copyScalar(uint64_t* dst, uint64_t value);

@@ -32,12 +32,14 @@ extern "C" {
 
 /* dynamic LDS bytes a delivery block needs */
 size_t nemo_propagate_lds_bytes(unsigned max_window, unsigned bin_capacity);
-/* STDP accumulation of the nsteps (1..64) steps from a->step_offset on, in
- * one launch of stdp_blocks blocks, from the history after the last of them
- * and the snapshot words of the history records (the history after the step
- * before the first); then the snapshot takes the history after the last step
- * and clock->stdp_flushed is set to it. */
-hipError_t nemo_launch_stdp_batch(hipStream_t, const nemo_propagate_args*, unsigned nsteps, unsigned stdp_blocks);
+/* The pending STDP accumulation (clock->stdp_first, stdp_count) in one launch
+ * of stdp_blocks blocks, from the two history snapshots.  It reads neither
+ * the live history nor the bitmaps, so it may run beside later steps. */
+hipError_t nemo_launch_stdp_batch(hipStream_t, const nemo_propagate_args*, unsigned stdp_blocks);
+/* A snapshot of the history after step base + offset (at most 64 steps after
+ * the last one); the steps since the last snapshot become the pending
+ * accumulation.  The pending batch must have run. */
+hipError_t nemo_launch_stdp_snapshot(hipStream_t, const nemo_propagate_args*, uint64_t offset);
 /* dynamic LDS an update launch carrying propagate blocks needs (delivery
  * blocks when delivery is set, else STDP blocks only) */
 size_t nemo_colaunch_lds_bytes(unsigned max_window, unsigned bin_capacity, int delivery);

@@ -13,9 +13,9 @@
  *                          {target, Q11.20 weight}, 8 bytes, so consecutive
  *                          lanes read consecutive 8-byte words of a row
  *   fired_bits uint64[128][N/64]  one firing bitmap per step (ring)
- *   hist       uint64[2][N][2] firing history (bit k: fired k steps ago):
- *                          plane s % 2 holds {history after step s, STDP
- *                          snapshot} per neuron (NEMO_HIST_INDEX)
+ *   hist       uint64[2][N] firing history (bit k: fired k steps ago); plane
+ *                          s % 2 holds the history after step s
+ *   stdp_snap  uint64[N][2] two history snapshots per neuron (STDP batches)
  *   bins       int2[2][B][CAP] each delivery block's events, sorted by tile,
  *                          for steps of even and odd parity
  *   bin_table  uint32[2][T][B] count << 16 | offset of tile t in block b's
@@ -62,29 +62,36 @@ stdp_kernel(nemo_propagate_args a)
 			reinterpret_cast<unsigned char*>(lds_dyn));
 }
 
-/* STDP accumulation of the nsteps steps from t0 = base + step_offset on,
- * one block per group of stdp_words bitmap words (stdp_batch_block) */
+/* The pending STDP accumulation, one block per group of stdp_words bitmap
+ * words (stdp_batch_block) */
 __global__ void __launch_bounds__(PBLOCK)
-stdp_batch_kernel(nemo_propagate_args a, unsigned nsteps)
+stdp_batch_kernel(nemo_propagate_args a)
 {
 	extern __shared__ uint64_t lds_dyn[];
-	nemo_dev::stdp_batch_block<PBLOCK>(a, a.clock->base_step + a.step_offset, nsteps, blockIdx.x,
-			reinterpret_cast<unsigned char*>(lds_dyn));
+	nemo_dev::stdp_batch_block<PBLOCK>(a, blockIdx.x, reinterpret_cast<unsigned char*>(lds_dyn));
 }
 
-/* After a batch: the history after its last step becomes the snapshot the
- * next batch starts from, and the last step counts as accumulated. */
+/* A history snapshot after step base + offset into the free slot (the one
+ * the last snapshot is not in) */
 __global__ void __launch_bounds__(NEMO_BLOCK)
-stdp_snapshot_kernel(nemo_propagate_args a, unsigned nsteps)
+stdp_snapshot_kernel(nemo_propagate_args a, uint64_t offset)
 {
-	const uint64_t rl = a.clock->base_step + a.step_offset + nsteps - 1;
+	const uint64_t s = a.clock->base_step + offset;
+	const unsigned slot = unsigned(a.clock->stdp_snaps + 1) & 1;
 	const size_t n = a.fired_words * 64;
 	for(size_t i = size_t(blockIdx.x) * NEMO_BLOCK + threadIdx.x; i < n; i += size_t(gridDim.x) * NEMO_BLOCK) {
-		const uint64_t h = a.hist[NEMO_HIST_INDEX(i, rl, a.hist_pitch)];
-		a.hist[2 * i + 1] = h;
-		a.hist[a.hist_pitch + 2 * i + 1] = h;
+		a.stdp_snap[2 * i + slot] = a.hist[NEMO_HIST_INDEX(i, s, a.hist_pitch)];
 	}
-	if(blockIdx.x == 0 && threadIdx.x == 0) const_cast<nemo_hip_clock*>(a.clock)->stdp_flushed = rl;
+}
+
+/* ... after which the steps since the last snapshot are pending */
+__global__ void stdp_snapshot_done_kernel(nemo_hip_clock* clock, uint64_t offset)
+{
+	const uint64_t s = clock->base_step + offset;
+	clock->stdp_first = clock->stdp_last + 1;
+	clock->stdp_count = s - clock->stdp_last;
+	clock->stdp_last = s;
+	clock->stdp_snaps += 1;
 }
 
 __global__ void __launch_bounds__(NEMO_BLOCK)
@@ -154,21 +161,28 @@ size_t nemo_propagate_lds_bytes(unsigned max_window, unsigned bin_capacity)
 	return nemo_dev::PropagateLds<PBLOCK>::delivery_bytes(bin_capacity, max_window);
 }
 
-hipError_t nemo_launch_stdp_batch(hipStream_t st, const nemo_propagate_args* a, unsigned nsteps, unsigned stdp_blocks)
+hipError_t nemo_launch_stdp_batch(hipStream_t st, const nemo_propagate_args* a, unsigned stdp_blocks)
 {
-	if(nsteps == 0 || stdp_blocks == 0) return hipSuccess;
-	if(nsteps > 64 || a->stdp.post_window > 63 || a->stdp_words == 0
-			|| a->stdp_words > NEMO_STDP_MAX_WORDS || size_t(stdp_blocks) * a->stdp_words < a->fired_words
-			|| !a->stdp.rcm_ptr || !a->hist) {
+	if(stdp_blocks == 0) return hipSuccess;
+	if(!a->stdp_snap || a->stdp.post_window > 63 || a->stdp_words == 0 || a->stdp_words > NEMO_STDP_MAX_WORDS
+			|| size_t(stdp_blocks) * a->stdp_words < a->fired_words || !a->stdp.rcm_ptr) {
 		return hipErrorInvalidValue;
 	}
 	hipLaunchKernelGGL(stdp_batch_kernel, dim3(stdp_blocks), dim3(PBLOCK),
-			nemo_dev::PropagateLds<PBLOCK>::stdp_bytes(), st, *a, nsteps);
-	hipError_t e = hipGetLastError();
-	if(e != hipSuccess) return e;
+			nemo_dev::PropagateLds<PBLOCK>::stdp_bytes(), st, *a);
+	return hipGetLastError();
+}
+
+hipError_t nemo_launch_stdp_snapshot(hipStream_t st, const nemo_propagate_args* a, uint64_t offset)
+{
+	if(!a->stdp_snap || !a->hist) return hipErrorInvalidValue;
 	const size_t n = a->fired_words * 64;
 	const unsigned blocks = unsigned(std::min<size_t>(1024, (n + NEMO_BLOCK - 1) / NEMO_BLOCK));
-	hipLaunchKernelGGL(stdp_snapshot_kernel, dim3(blocks), dim3(NEMO_BLOCK), 0, st, *a, nsteps);
+	hipLaunchKernelGGL(stdp_snapshot_kernel, dim3(blocks), dim3(NEMO_BLOCK), 0, st, *a, offset);
+	hipError_t e = hipGetLastError();
+	if(e != hipSuccess) return e;
+	hipLaunchKernelGGL(stdp_snapshot_done_kernel, dim3(1), dim3(1), 0, st, const_cast<nemo_hip_clock*>(a->clock),
+			offset);
 	return hipGetLastError();
 }
 

@@ -399,7 +399,7 @@ __device__ inline void stdp_tile(const nemo_hip_step_buffers& b, unsigned ng, bo
 #pragma unroll
 		for(int u = 0; u < ILP; ++u) {
 			ph[u] = 0;
-			if(e0 + u * NEMO_TILE < total) ph[u] = pre_hist[2 * size_t(sd[u] & 0x3ffffffu)];
+			if(e0 + u * NEMO_TILE < total) ph[u] = pre_hist[sd[u] & 0x3ffffffu];
 		}
 #pragma unroll
 		for(int u = 0; u < ILP; ++u) {

@@ -400,7 +400,7 @@ __device__ inline void stdp_block(const nemo_propagate_args& a, uint64_t r, unsi
 #pragma unroll
 		for(int u = 0; u < ILP; ++u) {
 			h[u] = 0;
-			if(e0 + u * THREADS < total) h[u] = hist[2 * size_t(sd[u] & 0x3ffffffu)];
+			if(e0 + u * THREADS < total) h[u] = hist[sd[u] & 0x3ffffffu];
 		}
 #pragma unroll
 		for(int u = 0; u < ILP; ++u) {
@@ -432,20 +432,25 @@ __device__ inline uint64_t history_window(uint64_t c, uint64_t s, unsigned n, un
 	return y == 0 ? lo : y < 64 ? (lo >> y) | (hi << (64 - y)) : hi;
 }
 
-/* STDP accumulation of the n (1..64) steps r0 .. r0+n-1 for the targets of
- * block sb's bitmap words, from two history planes only: the history after
- * the last of them and after the step before the first (the snapshot words
- * of the history records).  Together they hold every neuron's firing over the 128 steps
- * the n accumulations look at: a target fired at step r - P when bit
- * rl - r + P of its record is set, and a source's history after step r is
- * the window of its record at rl - r.  Each thread owns a synapse for all the
- * steps, so the accumulator is read and written once per batch and no two
- * threads touch it; the sum of 32-bit wrap-around additions does not depend
- * on their order, so the result equals n single-step accumulations. */
+/* The pending STDP accumulation (clock->stdp_first, stdp_count: 1..64
+ * steps r0 .. rl) for the targets of block sb's bitmap words, from the two
+ * history snapshots only: after rl (slot stdp_snaps % 2 of stdp_snap) and
+ * after r0 - 1 (the other slot).  Together they hold every neuron's firing
+ * over the 128 steps the n accumulations look at: a target fired at step
+ * r - P when bit rl - r + P of its record is set, and a source's history
+ * after step r is the window of its record at rl - r.  Each thread owns a
+ * synapse for all the steps, so the accumulator is read and written once per
+ * batch and no two threads touch it; the sum of 32-bit wrap-around additions
+ * does not depend on their order, so the result equals n single-step
+ * accumulations.  Nothing here reads the live history, so the batch may run
+ * beside the steps that follow. */
 template<unsigned THREADS>
-__device__ inline void stdp_batch_block(const nemo_propagate_args& a, uint64_t r0, unsigned n, unsigned sb,
-		unsigned char* lds)
+__device__ inline void stdp_batch_block(const nemo_propagate_args& a, unsigned sb, unsigned char* lds)
 {
+	const uint64_t r0 = a.clock->stdp_first;
+	const unsigned n = unsigned(a.clock->stdp_count);
+	const unsigned cur_slot = unsigned(a.clock->stdp_snaps & 1);
+	if(n == 0 || n > 64) return;
 	const nemo_hip_stdp& st = a.stdp;
 	int32_t* s_fn = reinterpret_cast<int32_t*>(lds);
 	uint32_t* s_rb = reinterpret_cast<uint32_t*>(lds + 128 * sizeof(int32_t));
@@ -458,8 +463,13 @@ __device__ inline void stdp_batch_block(const nemo_propagate_args& a, uint64_t r
 		s_fn[64 + tid] = st.fn_post[tid];
 	}
 	const uint64_t rl = r0 + n - 1;
-	/* the history after rl and the snapshot: one 16-byte load per neuron */
-	const ulonglong2* rec = reinterpret_cast<const ulonglong2*>(a.hist + NEMO_HIST_INDEX(0, rl, a.hist_pitch));
+	/* the two snapshots, after rl and after r0 - 1: one 16-byte load per
+	 * neuron */
+	const ulonglong2* rec = reinterpret_cast<const ulonglong2*>(a.stdp_snap);
+	auto split = [cur_slot](const ulonglong2 v, uint64_t& cur, uint64_t& old) {
+		cur = cur_slot ? v.y : v.x;
+		old = cur_slot ? v.x : v.y;
+	};
 	/* bit y: step rl - y is accumulated (in the batch, r >= P, not flushed) */
 	uint64_t due = n >= 64 ? ~uint64_t(0) : (uint64_t(1) << n) - 1;
 	if(rl < P) {
@@ -475,8 +485,9 @@ __device__ inline void stdp_batch_block(const nemo_propagate_args& a, uint64_t r
 	uint64_t post = 0; /* bit y: the target fired at step rl - y - P */
 	if(tid / 64 < a.stdp_words && word < a.fired_words) {
 		const uint32_t tn = uint32_t(w0 * 64 + tid);
-		const ulonglong2 cs = rec[tn];
-		post = history_window(cs.x, cs.y, n, P) & due;
+		uint64_t c, o;
+		split(rec[tn], c, o);
+		post = history_window(c, o, n, P) & due;
 	}
 	const bool fired = post != 0;
 	uint32_t nfired;
@@ -529,9 +540,7 @@ __device__ inline void stdp_batch_block(const nemo_propagate_args& a, uint64_t r
 			hc[u] = 0;
 			hs[u] = 0;
 			if(e0 + u * THREADS < total) {
-				const ulonglong2 cs = rec[sd[u] & 0x3ffffffu];
-				hc[u] = cs.x;
-				hs[u] = cs.y;
+				split(rec[sd[u] & 0x3ffffffu], hc[u], hs[u]);
 			}
 		}
 #pragma unroll
