@@ -980,8 +980,8 @@ static Partition partitionPairs(const std::vector<uint64_t>& pairWeight, const s
 
 /* Default shape: bin capacity 4096 events (halved while the LDS does not
  * allow it), one block set with enough delivery blocks for about 1% of the
- * synapses per step, STDP accumulated in one batch after each run of steps,
- * two bitmap words per STDP block.  NEMO_HIP_DELIVER_BLOCKS,
+ * synapses per step, STDP accumulated in batches beside the steps, one
+ * bitmap word per 64 threads of an STDP block.  NEMO_HIP_DELIVER_BLOCKS,
  * NEMO_HIP_LATE_BLOCKS, NEMO_HIP_BIN_CAPACITY, NEMO_HIP_STDP_MODE,
  * NEMO_HIP_STDP_WORDS and NEMO_HIP_OVERLAP override it (tuning). */
 LaunchShape Simulation::defaultShape() const
@@ -997,6 +997,7 @@ LaunchShape Simulation::defaultShape() const
 		if(ldsNeeded(s) <= m_ldsBudget) break;
 	}
 	s.stdpMode = 4;
+	s.stdpWords = NEMO_STDP_MAX_WORDS; /* few, wide batch blocks disturb the steps least */
 	auto env = [](const char* name, int lo, int hi, unsigned& field) {
 		if(const char* e = std::getenv(name)) field = unsigned(std::max(lo, std::min(std::atoi(e), hi)));
 	};
@@ -1004,7 +1005,8 @@ LaunchShape Simulation::defaultShape() const
 	env("NEMO_HIP_LATE_BLOCKS", 0, NEMO_MAX_BIN_BLOCKS, s.lateBlocks);
 	env("NEMO_HIP_BIN_CAPACITY", 64, 32768, s.binCapacity);
 	env("NEMO_HIP_STDP_MODE", 0, 4, s.stdpMode);
-	env("NEMO_HIP_STDP_WORDS", 1, NEMO_STDP_MAX_WORDS, s.stdpWords);
+	env("NEMO_HIP_STDP_WORDS", 1, NEMO_STDP_BATCH_MAX_WORDS, s.stdpWords);
+	if(s.stdpMode != 4) s.stdpWords = std::min<unsigned>(s.stdpWords, NEMO_STDP_MAX_WORDS);
 	env("NEMO_HIP_OVERLAP", 0, 2, s.overlap);
 	return s;
 }
@@ -1047,8 +1049,10 @@ void Simulation::checkShape(const LaunchShape& s) const
 	if(s.binCapacity < 64 || s.binCapacity > 32768) {
 		throw Error(NEMO_INVALID_INPUT, "bin capacity must be in [64, 32768]");
 	}
-	if(s.stdpWords < 1 || s.stdpWords > NEMO_STDP_MAX_WORDS) {
-		throw Error(NEMO_INVALID_INPUT, "STDP words per block must be in [1, " + std::to_string(NEMO_STDP_MAX_WORDS) + "]");
+	const unsigned maxWords = s.stdpMode == 4 ? NEMO_STDP_BATCH_MAX_WORDS : NEMO_STDP_MAX_WORDS;
+	if(s.stdpWords < 1 || s.stdpWords > maxWords) {
+		throw Error(NEMO_INVALID_INPUT, "STDP words per block must be in [1, " + std::to_string(maxWords)
+				+ "] for this STDP mode");
 	}
 	if(s.stdpMode > 4) {
 		throw Error(NEMO_INVALID_INPUT, "STDP mode must be 0, 1, 2, 3 or 4");
@@ -1875,8 +1879,8 @@ LaunchShape Simulation::autotune(unsigned stepsPerTrial)
 			c.stdpMode = m;
 			consider(c);
 		}
-		for(unsigned w : {1u, 2u, 4u, 8u, 16u}) {
-			if(w > NEMO_STDP_MAX_WORDS) continue;
+		for(unsigned w : {1u, 2u, 4u, 8u, 16u, 32u, 64u}) {
+			if(w > (best.stdpMode == 4 ? NEMO_STDP_BATCH_MAX_WORDS : NEMO_STDP_MAX_WORDS)) continue;
 			if(w == best.stdpWords) continue;
 			LaunchShape c = best;
 			c.stdpWords = w;

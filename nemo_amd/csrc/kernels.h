@@ -26,8 +26,10 @@ extern "C" {
 #ifndef NEMO_PROPAGATE_BLOCK
 #define NEMO_PROPAGATE_BLOCK 512
 #endif
-/* Firing-bitmap words (64 targets each) an STDP block may cover. */
+/* Firing-bitmap words (64 targets each) an STDP block may cover: one target
+ * per thread, or, in STDP batches, groups of that many in turn. */
 #define NEMO_STDP_MAX_WORDS (NEMO_PROPAGATE_BLOCK / 64)
+#define NEMO_STDP_BATCH_MAX_WORDS 64
 
 
 /* dynamic LDS bytes a delivery block needs */

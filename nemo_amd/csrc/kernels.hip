@@ -164,7 +164,7 @@ size_t nemo_propagate_lds_bytes(unsigned max_window, unsigned bin_capacity)
 hipError_t nemo_launch_stdp_batch(hipStream_t st, const nemo_propagate_args* a, unsigned stdp_blocks)
 {
 	if(stdp_blocks == 0) return hipSuccess;
-	if(!a->stdp_snap || a->stdp.post_window > 63 || a->stdp_words == 0 || a->stdp_words > NEMO_STDP_MAX_WORDS
+	if(!a->stdp_snap || a->stdp.post_window > 63 || a->stdp_words == 0 || a->stdp_words > NEMO_STDP_BATCH_MAX_WORDS
 			|| size_t(stdp_blocks) * a->stdp_words < a->fired_words || !a->stdp.rcm_ptr) {
 		return hipErrorInvalidValue;
 	}

@@ -480,91 +480,97 @@ __device__ inline void stdp_batch_block(const nemo_propagate_args& a, unsigned s
 	const uint64_t flushed = a.clock->stdp_flushed;
 	if(flushed >= r0 && flushed <= rl) due &= ~(uint64_t(1) << (rl - flushed));
 
-	const size_t w0 = size_t(sb) * a.stdp_words;
-	const size_t word = w0 + tid / 64;
-	uint64_t post = 0; /* bit y: the target fired at step rl - y - P */
-	if(tid / 64 < a.stdp_words && word < a.fired_words) {
-		const uint32_t tn = uint32_t(w0 * 64 + tid);
-		uint64_t c, o;
-		split(rec[tn], c, o);
-		post = history_window(c, o, n, P) & due;
-	}
-	const bool fired = post != 0;
-	uint32_t nfired;
-	const uint32_t idx = block_exclusive_scan<THREADS>(fired ? 1u : 0u, &nfired);
-	uint32_t len = 0, rb = 0;
-	if(fired) {
-		const uint32_t tn = uint32_t(w0 * 64 + tid);
-		rb = st.rcm_ptr[tn];
-		len = st.rcm_ptr[tn + 1] - rb;
-	}
-	uint32_t total;
-	const uint32_t pos = block_exclusive_scan<THREADS>(len, &total);
-	uint32_t examined;
-	block_exclusive_scan<THREADS>(len * unsigned(__popcll(post)), &examined);
-	if(fired) {
-		s_rb[idx] = rb;
-		s_pos[idx] = pos;
-		s_post[idx] = post;
-	}
-	if(tid == 0) s_pos[nfired] = total;
-	__syncthreads();
-	if(total == 0) return;
+	/* the block's words, THREADS / 64 (one target per thread) at a time */
+	unsigned long long examined_all = 0;
+	for(unsigned wg = 0; wg < a.stdp_words; wg += THREADS / 64) {
+		__syncthreads(); /* the previous group is done with the LDS lists */
+		const size_t w0 = size_t(sb) * a.stdp_words + wg;
+		const size_t word = w0 + tid / 64;
+		uint64_t post = 0; /* bit y: the target fired at step rl - y - P */
+		if(tid / 64 < min(THREADS / 64, a.stdp_words - wg) && word < a.fired_words) {
+			const uint32_t tn = uint32_t(w0 * 64 + tid);
+			uint64_t c, o;
+			split(rec[tn], c, o);
+			post = history_window(c, o, n, P) & due;
+		}
+		const bool fired = post != 0;
+		uint32_t nfired;
+		const uint32_t idx = block_exclusive_scan<THREADS>(fired ? 1u : 0u, &nfired);
+		uint32_t len = 0, rb = 0;
+		if(fired) {
+			const uint32_t tn = uint32_t(w0 * 64 + tid);
+			rb = st.rcm_ptr[tn];
+			len = st.rcm_ptr[tn + 1] - rb;
+		}
+		uint32_t total;
+		const uint32_t pos = block_exclusive_scan<THREADS>(len, &total);
+		uint32_t examined;
+		block_exclusive_scan<THREADS>(len * unsigned(__popcll(post)), &examined);
+		examined_all += examined;
+		if(fired) {
+			s_rb[idx] = rb;
+			s_pos[idx] = pos;
+			s_post[idx] = post;
+		}
+		if(tid == 0) s_pos[nfired] = total;
+		__syncthreads();
+		if(total == 0) continue;
 
-	for(uint32_t e0 = tid; e0 < total; e0 += ILP * THREADS) {
-		uint32_t k[ILP];
-		uint32_t sd[ILP];
-		int32_t acc[ILP];
-		uint64_t pm[ILP];
-#pragma unroll
-		for(int u = 0; u < ILP; ++u) k[u] = e0 + u * THREADS;
-		unsigned lo[ILP];
-		last_not_above_k<THREADS>(s_pos, nfired, k, lo);
-#pragma unroll
-		for(int u = 0; u < ILP; ++u) {
-			const uint32_t e = k[u];
-			k[u] = 0;
-			sd[u] = 0;
-			acc[u] = 0;
-			pm[u] = 0;
-			if(e < total) {
-				k[u] = s_rb[lo[u]] + (e - s_pos[lo[u]]);
-				pm[u] = s_post[lo[u]];
-				sd[u] = st.rcm_sd[k[u]];
-				acc[u] = st.rcm_acc[k[u]];
-			}
-		}
-		uint64_t hc[ILP], hs[ILP];
-#pragma unroll
-		for(int u = 0; u < ILP; ++u) {
-			hc[u] = 0;
-			hs[u] = 0;
-			if(e0 + u * THREADS < total) {
-				split(rec[sd[u] & 0x3ffffffu], hc[u], hs[u]);
-			}
-		}
-#pragma unroll
-		for(int u = 0; u < ILP; ++u) {
-			if(e0 + u * THREADS < total) {
-				const unsigned d = (sd[u] >> 26) + 1;
-				int32_t dw = 0;
-				for(uint64_t m = pm[u]; m; m &= m - 1) {
-					const unsigned y = unsigned(__builtin_ctzll(m));
-					const uint64_t h = history_window(hc[u], hs[u], n, y);
-					const uint64_t preFiring = d >= 64 ? 0 : (h >> d);
-					dw = stdp_wrap_add(dw, stdp_wrap_add(
-							stdp_update_region(preFiring & st.potentiation, P, st.pre_post, st.post_pre,
-								s_fn, s_fn + 64),
-							stdp_update_region(preFiring & st.depression, P, st.pre_post, st.post_pre,
-								s_fn, s_fn + 64)));
+		for(uint32_t e0 = tid; e0 < total; e0 += ILP * THREADS) {
+			uint32_t k[ILP];
+			uint32_t sd[ILP];
+			int32_t acc[ILP];
+			uint64_t pm[ILP];
+	#pragma unroll
+			for(int u = 0; u < ILP; ++u) k[u] = e0 + u * THREADS;
+			unsigned lo[ILP];
+			last_not_above_k<THREADS>(s_pos, nfired, k, lo);
+	#pragma unroll
+			for(int u = 0; u < ILP; ++u) {
+				const uint32_t e = k[u];
+				k[u] = 0;
+				sd[u] = 0;
+				acc[u] = 0;
+				pm[u] = 0;
+				if(e < total) {
+					k[u] = s_rb[lo[u]] + (e - s_pos[lo[u]]);
+					pm[u] = s_post[lo[u]];
+					sd[u] = st.rcm_sd[k[u]];
+					acc[u] = st.rcm_acc[k[u]];
 				}
-				if(dw != 0) {
-					st.rcm_acc[k[u]] = stdp_wrap_add(acc[u], dw);
+			}
+			uint64_t hc[ILP], hs[ILP];
+	#pragma unroll
+			for(int u = 0; u < ILP; ++u) {
+				hc[u] = 0;
+				hs[u] = 0;
+				if(e0 + u * THREADS < total) {
+					split(rec[sd[u] & 0x3ffffffu], hc[u], hs[u]);
+				}
+			}
+	#pragma unroll
+			for(int u = 0; u < ILP; ++u) {
+				if(e0 + u * THREADS < total) {
+					const unsigned d = (sd[u] >> 26) + 1;
+					int32_t dw = 0;
+					for(uint64_t m = pm[u]; m; m &= m - 1) {
+						const unsigned y = unsigned(__builtin_ctzll(m));
+						const uint64_t h = history_window(hc[u], hs[u], n, y);
+						const uint64_t preFiring = d >= 64 ? 0 : (h >> d);
+						dw = stdp_wrap_add(dw, stdp_wrap_add(
+								stdp_update_region(preFiring & st.potentiation, P, st.pre_post, st.post_pre,
+									s_fn, s_fn + 64),
+								stdp_update_region(preFiring & st.depression, P, st.pre_post, st.post_pre,
+									s_fn, s_fn + 64)));
+					}
+					if(dw != 0) {
+						st.rcm_acc[k[u]] = stdp_wrap_add(acc[u], dw);
+					}
 				}
 			}
 		}
 	}
-	if(tid == 0) atomicAdd(&st.examined[sb], (unsigned long long)examined);
+	if(tid == 0) atomicAdd(&st.examined[sb], examined_all);
 }
 
 /* STDP accumulation of step t-1 by STDP block sb, unless it is not due */

@@ -354,7 +354,7 @@ def test_launch_shapes_give_identical_results():
               (900, 124, 4096, 2, 1, 1), (33, 5, 2048, 1, 3, 1), (64, 0, 512, 0, 2, 0),
               (64, 0, 512, 3, 2, 0), (33, 5, 2048, 3, 1, 1), (33, 5, 2048, 2, 3, 2), (64, 16, 1024, 1, 2, 2),
               (200, 40, 4096, 0, 4, 2), (7, 3, 256, 3, 1, 2), (120, 30, 512, 2, 4, 2), (64, 0, 512, 4, 2, 0),
-              (33, 5, 2048, 4, 3, 2), (200, 0, 1024, 4, 8, 1)]
+              (33, 5, 2048, 4, 3, 2), (200, 0, 1024, 4, 8, 1), (120, 0, 1024, 4, 32, 1), (90, 0, 2048, 4, 64, 0)]
     results = []
     for shape in shapes:
         sim = sc.make(nemo_amd, spec, _conf(nemo_amd, stdp_fn()))
