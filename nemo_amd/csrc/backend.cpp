@@ -1652,8 +1652,8 @@ void Simulation::run(unsigned nsteps, unsigned stdpPeriod, float reward, bool re
 }
 
 /* Eager steps with HIP events around each launch on the simulation's stream:
- * average microseconds of the propagate kernel (delivery and STDP
- * accumulation) and of the neuron updates (all groups), and the work counters
+ * average microseconds of the propagate kernel (delivery, and STDP
+ * accumulation in STDP mode 2) and of the neuron updates (all groups), and the work counters
  * of the profiled steps (events delivered, neurons fired, plastic synapses
  * examined, steps).  The steps are real simulation steps; their firing is not
  * returned. */
@@ -1667,7 +1667,10 @@ void Simulation::profile(unsigned nsteps, double avgUs[4], uint64_t counters[4])
 	for(hipEvent_t& e : ev) HIP_CHECK(hipEventCreate(&e));
 	const uint64_t t0 = m_hostStep;
 	setBase(t0);
-	for(unsigned s = 0; s < nsteps; ++s) enqueueSteps(s, 1, false, false, &ev[3 * size_t(s)]);
+	/* runs of up to 64 steps, as in a graph, with events between the launches */
+	for(unsigned s = 0; s < nsteps; s += 64) {
+		enqueueSteps(s, std::min(64u, nsteps - s), false, false, &ev[3 * size_t(s)]);
+	}
 	m_hostStep += nsteps;
 	HIP_CHECK(hipStreamSynchronize(m_stream));
 	double sum[3] = {0, 0, 0};
