@@ -209,6 +209,8 @@ typedef struct nemo_hip_step_buffers {
 	float* debug_current;      /* NULL, or 2 planes of acc_pitch floats receiving E and I */
 	nemo_hip_stdp stdp;
 	nemo_hip_colaunch co;      /* extra blocks of this launch (the first group's only) */
+	hipEvent_t time_start;     /* when not NULL, recorded when the launch's kernel starts */
+	hipEvent_t time_stop;      /* when not NULL, recorded when it ends (hipExtLaunchKernelGGL) */
 } nemo_hip_step_buffers;
 
 typedef hipError_t hip_update_neurons_t(hipStream_t stream,

@@ -1268,13 +1268,18 @@ void Simulation::setBase(uint64_t base)
 	copyScalar(&m_clock.ptr->base_step, base);
 }
 
-void Simulation::launchDelivery(const nemo_propagate_args& set, uint64_t offset, hipStream_t st, bool withStdp)
+void Simulation::launchDelivery(const nemo_propagate_args& set, uint64_t offset, hipStream_t st, bool withStdp,
+		hipEvent_t start, hipEvent_t stop)
 {
 	const unsigned stdpBlocks = withStdp ? m_stdpBlocks : 0;
-	if(!set.deliver_blocks && !stdpBlocks) return;
+	if(!set.deliver_blocks && !stdpBlocks) {
+		if(start) HIP_CHECK(hipEventRecord(start, st));
+		if(stop) HIP_CHECK(hipEventRecord(stop, st));
+		return;
+	}
 	nemo_propagate_args a = set;
 	a.step_offset = offset;
-	HIP_CHECK(nemo_launch_propagate(st, &a, stdpBlocks, 1));
+	HIP_CHECK(nemo_launch_propagate(st, &a, stdpBlocks, 1, start, stop));
 }
 
 /* Steps first..first+k-1 on the stream (SimulationBackend.cpp:22-77): spike
@@ -1291,8 +1296,11 @@ void Simulation::launchDelivery(const nemo_propagate_args& set, uint64_t offset,
  * step t or, with one step of slack, before the update of step t+1: it reads
  * the history plane of step t-1, which the update of step t+1 rewrites, and
  * the accumulators it updates are serialised on the branch's stream.  The
- * branch of the last step is joined at the end of the run.  With ev (three
- * events per step), HIP events bracket the delivery and the updates. */
+ * branch of the last step is joined at the end of the run.  In STDP mode 4
+ * the pending batch runs on a branch beside all the steps and a history
+ * snapshot ends the run (and every 64 steps).  With ev (four events per
+ * step), the kernels record the start and end of the step's delivery on this
+ * stream and of its neuron updates. */
 void Simulation::enqueueSteps(uint64_t first, unsigned k, bool ext, bool fstim, hipEvent_t* ev)
 {
 	const bool split = m_late.deliver_blocks && m_early.deliver_blocks;
@@ -1325,14 +1333,13 @@ void Simulation::enqueueSteps(uint64_t first, unsigned k, bool ext, bool fstim, 
 	};
 	for(unsigned i = 0; i < k; ++i) {
 		const uint64_t off = first + i;
-		if(ev) HIP_CHECK(hipEventRecord(ev[3 * i], m_stream));
 		if(m_shape.stdpMode == 0 && m_stdpBlocks) {
 			joinStdp();
 			HIP_CHECK(hipEventRecord(m_stdpFork, m_stream));
 			HIP_CHECK(hipStreamWaitEvent(m_stdpStream, m_stdpFork, 0));
 			nemo_propagate_args sa = m_propagate;
 			sa.step_offset = off;
-			HIP_CHECK(nemo_launch_propagate(m_stdpStream, &sa, m_stdpBlocks, 0));
+			HIP_CHECK(nemo_launch_propagate(m_stdpStream, &sa, m_stdpBlocks, 0, nullptr, nullptr));
 			HIP_CHECK(hipEventRecord(m_stdpDone, m_stdpStream));
 			m_stdpPending = true;
 		}
@@ -1341,15 +1348,19 @@ void Simulation::enqueueSteps(uint64_t first, unsigned k, bool ext, bool fstim, 
 			HIP_CHECK(hipStreamWaitEvent(m_stdpStream, m_stdpFork, 0));
 			nemo_propagate_args sa = m_propagate;
 			sa.step_offset = off;
-			HIP_CHECK(nemo_launch_propagate(m_stdpStream, &sa, m_stdpBlocks, 0));
+			HIP_CHECK(nemo_launch_propagate(m_stdpStream, &sa, m_stdpBlocks, 0, nullptr, nullptr));
 			HIP_CHECK(hipEventRecord(m_stdpLag[i & 1], m_stdpStream));
 		}
+		/* with ev: four events per step, recorded by the kernels: start
+		 * and end of the delivery on this stream, of the neuron updates */
+		hipEvent_t ps = ev ? ev[4 * i] : nullptr, pe = ev ? ev[4 * i + 1] : nullptr;
 		if(split) {
-			if(i == 0 || (!overlap && !colaunch)) launchDelivery(m_early, off, m_stream, false);
+			const bool early = i == 0 || (!overlap && !colaunch);
+			if(early) launchDelivery(m_early, off, m_stream, false, ps, nullptr);
 			if(i == 0 && overlap) forkEarly(1);
-			launchDelivery(m_late, off, m_stream, fused && !colaunch);
+			launchDelivery(m_late, off, m_stream, fused && !colaunch, early ? nullptr : ps, pe);
 		} else {
-			launchDelivery(m_early, off, m_stream, fused);
+			launchDelivery(m_early, off, m_stream, fused, ps, pe);
 		}
 		if(pending[i & 1] && i > 0) {
 			HIP_CHECK(hipStreamWaitEvent(m_stream, m_earlyDone[i & 1], 0));
@@ -1359,7 +1370,6 @@ void Simulation::enqueueSteps(uint64_t first, unsigned k, bool ext, bool fstim, 
 		 * the update, which is on the step's critical path */
 		joinStdp();
 		if(lagged && i > 0) HIP_CHECK(hipStreamWaitEvent(m_stream, m_stdpLag[(i - 1) & 1], 0));
-		if(ev) HIP_CHECK(hipEventRecord(ev[3 * i + 1], m_stream));
 		nemo_hip_step_buffers b = stepBuffers(off, ext, fstim);
 		if(colaunch) {
 			b.co.args = m_early;
@@ -1370,14 +1380,25 @@ void Simulation::enqueueSteps(uint64_t first, unsigned k, bool ext, bool fstim, 
 				? unsigned(nemo_colaunch_lds_bytes(m_early.max_window, m_early.bin_capacity, b.co.deliver_blocks > 0))
 				: 0;
 		}
-		for(Group& g : m_groups) {
+		size_t last = m_groups.size();
+		for(size_t gi = 0; gi < m_groups.size(); ++gi) {
+			if(m_groups[gi].dev.count) last = gi;
+		}
+		if(ev) b.time_start = ev[4 * i + 2];
+		for(size_t gi = 0; gi < m_groups.size(); ++gi) {
+			Group& g = m_groups[gi];
 			if(g.dev.count == 0) continue;
+			if(ev && gi == last) b.time_stop = ev[4 * i + 3];
 			HIP_CHECK(g.plugin->update(m_stream, &g.dev, &b));
 			b.co.deliver_blocks = 0; /* the first group's launch carries them */
 			b.co.stdp_blocks = 0;
 			b.co.lds_bytes = 0;
+			b.time_start = nullptr;
 		}
-		if(ev) HIP_CHECK(hipEventRecord(ev[3 * i + 2], m_stream));
+		if(ev && last == m_groups.size()) {
+			HIP_CHECK(hipEventRecord(ev[4 * i + 2], m_stream));
+			HIP_CHECK(hipEventRecord(ev[4 * i + 3], m_stream));
+		}
 		if(overlap && i + 2 < k) forkEarly(i + 2);
 		if(batched && (i + 1 == k || (i + 1) % 64 == 0)) {
 			/* snapshot after this step: the steps since the last one are
@@ -1418,7 +1439,7 @@ void Simulation::flushStdp()
 	a.deliver_blocks = 0;
 	a.step_offset = 0;
 	setBase(s + 1);
-	HIP_CHECK(nemo_launch_propagate(m_stream, &a, m_stdpBlocks, 1));
+	HIP_CHECK(nemo_launch_propagate(m_stream, &a, m_stdpBlocks, 1, nullptr, nullptr));
 	copyScalar(&m_clock.ptr->stdp_flushed, s);
 	m_flushed = s;
 	HIP_CHECK(hipStreamSynchronize(m_stream));
@@ -1663,13 +1684,13 @@ void Simulation::profile(unsigned nsteps, double avgUs[4], uint64_t counters[4])
 	HIP_CHECK(hipStreamSynchronize(m_stream));
 	const uint64_t e0 = sumCounters(m_blockEvents);
 	const uint64_t x0 = sumCounters(m_blockExamined);
-	std::vector<hipEvent_t> ev(3 * size_t(nsteps));
+	std::vector<hipEvent_t> ev(4 * size_t(nsteps));
 	for(hipEvent_t& e : ev) HIP_CHECK(hipEventCreate(&e));
 	const uint64_t t0 = m_hostStep;
 	setBase(t0);
 	/* runs of up to 64 steps, as in a graph, with events between the launches */
 	for(unsigned s = 0; s < nsteps; s += 64) {
-		enqueueSteps(s, std::min(64u, nsteps - s), false, false, &ev[3 * size_t(s)]);
+		enqueueSteps(s, std::min(64u, nsteps - s), false, false, &ev[4 * size_t(s)]);
 	}
 	m_hostStep += nsteps;
 	HIP_CHECK(hipStreamSynchronize(m_stream));
@@ -1677,7 +1698,7 @@ void Simulation::profile(unsigned nsteps, double avgUs[4], uint64_t counters[4])
 	for(unsigned s = 0; s < nsteps; ++s) {
 		for(int k = 0; k < 2; ++k) {
 			float ms = 0;
-			HIP_CHECK(hipEventElapsedTime(&ms, ev[3 * size_t(s) + k], ev[3 * size_t(s) + k + 1]));
+			HIP_CHECK(hipEventElapsedTime(&ms, ev[4 * size_t(s) + 2 * k], ev[4 * size_t(s) + 2 * k + 1]));
 			sum[k] += ms;
 		}
 	}

@@ -345,7 +345,8 @@ private:
 	void copyScalar(uint64_t* dst, uint64_t value);
 	void setBase(uint64_t base);
 	void enqueueSteps(uint64_t first, unsigned k, bool ext, bool fstim, hipEvent_t* ev = nullptr);
-	void launchDelivery(const nemo_propagate_args& set, uint64_t offset, hipStream_t st, bool withStdp);
+	void launchDelivery(const nemo_propagate_args& set, uint64_t offset, hipStream_t st, bool withStdp,
+			hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 	void flushStdp();
 	void joinStdp();
 	void decodeFiring(const uint64_t* bits, std::vector<unsigned>& out) const;

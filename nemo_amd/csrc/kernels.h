@@ -49,8 +49,11 @@ size_t nemo_colaunch_lds_bytes(unsigned max_window, unsigned bin_capacity, int d
 size_t nemo_propagate_static_lds(void);
 /* Delivery (a->deliver_blocks blocks; none if 0) and STDP accumulation
  * (stdp_blocks blocks; none if 0): one launch with the STDP blocks after the
- * delivery blocks when fuse_stdp is set, else the STDP launch first. */
-hipError_t nemo_launch_propagate(hipStream_t, const nemo_propagate_args*, unsigned stdp_blocks, int fuse_stdp);
+ * delivery blocks when fuse_stdp is set, else the STDP launch first.  start
+ * and stop (may be NULL) are recorded at the start of the first kernel and
+ * the end of the last. */
+hipError_t nemo_launch_propagate(hipStream_t, const nemo_propagate_args*, unsigned stdp_blocks, int fuse_stdp,
+		hipEvent_t start, hipEvent_t stop);
 
 /* applyStdp (ConnectivityMatrix::applyStdp, ConnectivityMatrix.cpp:272-325 and
  * StdpProcess::updatedWeight, StdpProcess.cpp:128-138). */

@@ -24,6 +24,7 @@
  *   rcm_*      per target, its plastic incoming synapses (STDP only)
  */
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 
@@ -198,7 +199,8 @@ size_t nemo_propagate_static_lds(void)
 	return fa.sharedSizeBytes;
 }
 
-hipError_t nemo_launch_propagate(hipStream_t st, const nemo_propagate_args* a, unsigned stdp_blocks, int fuse_stdp)
+hipError_t nemo_launch_propagate(hipStream_t st, const nemo_propagate_args* a, unsigned stdp_blocks, int fuse_stdp,
+		hipEvent_t start, hipEvent_t stop)
 {
 	if(a->deliver_blocks > NEMO_MAX_BIN_BLOCKS) return hipErrorInvalidValue;
 	if(a->deliver_blocks && (a->D == 0 || a->D > NEMO_MAX_DELAY || a->bin_capacity == 0
@@ -216,17 +218,18 @@ hipError_t nemo_launch_propagate(hipStream_t st, const nemo_propagate_args* a, u
 	if(fuse_stdp) {
 		const unsigned blocks = a->deliver_blocks + stdp_blocks;
 		if(blocks == 0) return hipSuccess;
-		hipLaunchKernelGGL(propagate_kernel, dim3(blocks), dim3(PBLOCK), lds, st, *a);
+		hipExtLaunchKernelGGL(propagate_kernel, dim3(blocks), dim3(PBLOCK), lds, st, start, stop, 0, *a);
 		return hipGetLastError();
 	}
 	if(stdp_blocks) {
-		hipLaunchKernelGGL(stdp_kernel, dim3(stdp_blocks), dim3(PBLOCK), nemo_dev::PropagateLds<PBLOCK>::stdp_bytes(),
-				st, *a);
+		hipExtLaunchKernelGGL(stdp_kernel, dim3(stdp_blocks), dim3(PBLOCK), nemo_dev::PropagateLds<PBLOCK>::stdp_bytes(),
+				st, start, a->deliver_blocks ? nullptr : stop, 0, *a);
 		const hipError_t e = hipGetLastError();
 		if(e != hipSuccess) return e;
 	}
 	if(a->deliver_blocks) {
-		hipLaunchKernelGGL(propagate_kernel, dim3(a->deliver_blocks), dim3(PBLOCK), lds, st, *a);
+		hipExtLaunchKernelGGL(propagate_kernel, dim3(a->deliver_blocks), dim3(PBLOCK), lds, st,
+				stdp_blocks ? nullptr : start, stop, 0, *a);
 		return hipGetLastError();
 	}
 	return hipSuccess;

@@ -12,6 +12,8 @@
  * Divisions are correctly rounded (-fhip-fp32-correctly-rounded-divide-sqrt).
  * Indices from src/nemo/plugins/IF_curr_exp.h:4-18.
  */
+#include <hip/hip_ext.h>
+
 #include "../plugin_device.h"
 
 namespace {
@@ -75,7 +77,8 @@ hip_update_neurons(hipStream_t stream, const nemo_hip_neuron_group* g, const nem
 	/* the group's tiles, then the propagate blocks the launch carries */
 	const unsigned blocks = (g->count + NEMO_UPDATE_BLOCK - 1) / NEMO_UPDATE_BLOCK + b->co.deliver_blocks
 		+ b->co.stdp_blocks;
-	hipLaunchKernelGGL(if_curr_exp_update, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream, *g, *b);
+	hipExtLaunchKernelGGL(if_curr_exp_update, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream, b->time_start,
+			b->time_stop, 0, *g, *b);
 	return hipGetLastError();
 }
 

@@ -3,6 +3,8 @@
  * (src/nemo/cpu/plugins/Input.cpp:6-26).  Currents are ignored; the
  * accumulators are read and cleared.
  */
+#include <hip/hip_ext.h>
+
 #include "../plugin_device.h"
 
 namespace {
@@ -36,7 +38,8 @@ hip_update_neurons(hipStream_t stream, const nemo_hip_neuron_group* g, const nem
 	/* the group's tiles, then the propagate blocks the launch carries */
 	const unsigned blocks = (g->count + NEMO_UPDATE_BLOCK - 1) / NEMO_UPDATE_BLOCK + b->co.deliver_blocks
 		+ b->co.stdp_blocks;
-	hipLaunchKernelGGL(input_update, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream, *g, *b);
+	hipExtLaunchKernelGGL(input_update, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream, b->time_start,
+			b->time_stop, 0, *g, *b);
 	return hipGetLastError();
 }
 

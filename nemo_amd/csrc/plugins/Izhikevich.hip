@@ -14,6 +14,8 @@
  * Parameters a, b, c, d, sigma and state u, v use the indices of
  * src/nemo/plugins/Izhikevich.h:6-15.
  */
+#include <hip/hip_ext.h>
+
 #include "../plugin_device.h"
 
 namespace {
@@ -92,7 +94,8 @@ hip_update_neurons(hipStream_t stream, const nemo_hip_neuron_group* g, const nem
 	/* the group's tiles, then the propagate blocks the launch carries */
 	const unsigned blocks = (g->count + NEMO_UPDATE_BLOCK - 1) / NEMO_UPDATE_BLOCK + b->co.deliver_blocks
 		+ b->co.stdp_blocks;
-	hipLaunchKernelGGL(izhikevich_update, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream, *g, *b);
+	hipExtLaunchKernelGGL(izhikevich_update, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream, b->time_start,
+			b->time_stop, 0, *g, *b);
 	return hipGetLastError();
 }
 

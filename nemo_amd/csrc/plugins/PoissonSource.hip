@@ -7,6 +7,8 @@
  * Synaptic and external currents are ignored (and, as in the reference, the
  * external current is not cleared); the accumulators are still read and cleared.
  */
+#include <hip/hip_ext.h>
+
 #include "../plugin_device.h"
 
 namespace {
@@ -45,7 +47,8 @@ hip_update_neurons(hipStream_t stream, const nemo_hip_neuron_group* g, const nem
 	/* the group's tiles, then the propagate blocks the launch carries */
 	const unsigned blocks = (g->count + NEMO_UPDATE_BLOCK - 1) / NEMO_UPDATE_BLOCK + b->co.deliver_blocks
 		+ b->co.stdp_blocks;
-	hipLaunchKernelGGL(poisson_update, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream, *g, *b);
+	hipExtLaunchKernelGGL(poisson_update, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream, b->time_start,
+			b->time_stop, 0, *g, *b);
 	return hipGetLastError();
 }
 

@@ -8,6 +8,8 @@
  * the type is the first one added; both agree in that case).
  * Parameter a and state v use the indices of src/nemo/plugins/QIF.h.
  */
+#include <hip/hip_ext.h>
+
 #include "../plugin_device.h"
 
 namespace {
@@ -73,7 +75,8 @@ hip_update_neurons(hipStream_t stream, const nemo_hip_neuron_group* g, const nem
 	/* the group's tiles, then the propagate blocks the launch carries */
 	const unsigned blocks = (g->count + NEMO_UPDATE_BLOCK - 1) / NEMO_UPDATE_BLOCK + b->co.deliver_blocks
 		+ b->co.stdp_blocks;
-	hipLaunchKernelGGL(qif_update, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream, *g, *b);
+	hipExtLaunchKernelGGL(qif_update, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream, b->time_start,
+			b->time_stop, 0, *g, *b);
 	return hipGetLastError();
 }
 
