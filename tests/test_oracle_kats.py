@@ -30,7 +30,7 @@ def test_seed_words_match_independent_mt19937():
     assert np.array_equal(oa.rng_seeds(0, 49), expected)
     # skipping 4*first words gives the same words for later neurons
     assert np.array_equal(oa.rng_seeds(17, 49), expected[17:])
-    # the first eight words quoted in the survey of the reference
+    # the first eight words of mt19937(5489) shifted right by one
     assert expected.ravel()[:8].tolist() == [1749605806, 290934651, 1945173367, 1793167292,
                                               272702102, 2080627695, 1961459714, 474666992]
 
