@@ -873,6 +873,9 @@ void Simulation::buildConnectivity(const std::vector<const Network*>& nets, cons
 		HIP_CHECK(hipMemcpy(m_rcmSd.ptr, rcmSd.data(), m_rcmSd.bytes(), hipMemcpyHostToDevice));
 		HIP_CHECK(hipMemcpy(m_rcmFwd.ptr, rcmFwd.data(), m_rcmFwd.bytes(), hipMemcpyHostToDevice));
 		HIP_CHECK(hipMemset(m_rcmAcc.ptr, 0, m_rcmAcc.bytes()));
+		m_rcmOf.allocate(m_fcm.n);
+		HIP_CHECK(nemo_launch_invert_rcm(nullptr, m_rcmFwd.ptr, m_rcmOf.ptr, rcmFwd.size(), m_fcm.n));
+		HIP_CHECK(hipDeviceSynchronize());
 	}
 }
 
@@ -1937,7 +1940,8 @@ void Simulation::applyStdp(float reward)
 	}
 	flushStdp();
 	nemo_apply_args a;
-	a.rcm_fwd = m_rcmFwd.ptr;
+	a.rcm_of = m_rcmOf.ptr;
+	a.fcm_count = m_fcm.n;
 	a.rcm_acc = m_rcmAcc.ptr;
 	a.fcm = m_fcm.ptr;
 	a.count = m_nplastic;

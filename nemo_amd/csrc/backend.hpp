@@ -281,6 +281,7 @@ private:
 	DeviceArray<uint32_t> m_rowptr;
 	DeviceArray<int2> m_fcm;
 	DeviceArray<uint32_t> m_rcmPtr, m_rcmSd, m_rcmFwd;
+	DeviceArray<uint32_t> m_rcmOf;    /* per FCM synapse: accumulator index, ~0 if static */
 	DeviceArray<int32_t> m_rcmAcc;
 	DeviceArray<uint32_t> m_stimIdx;
 	DeviceArray<float> m_stimVal;

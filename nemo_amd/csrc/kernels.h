@@ -58,16 +58,21 @@ hipError_t nemo_launch_propagate(hipStream_t, const nemo_propagate_args*, unsign
 /* applyStdp (ConnectivityMatrix::applyStdp, ConnectivityMatrix.cpp:272-325 and
  * StdpProcess::updatedWeight, StdpProcess.cpp:128-138). */
 typedef struct nemo_apply_args {
-	const uint32_t* rcm_fwd;         /* fcm index of each plastic synapse */
+	const uint32_t* rcm_of;          /* per FCM synapse: its accumulator index, or ~0 if static */
 	int32_t* rcm_acc;
 	int2* fcm;
-	size_t count;
+	size_t count;                    /* plastic synapses */
+	size_t fcm_count;                /* FCM synapses */
 	int32_t reward;                  /* Q11.20 reward; 0 clears without adding */
 	int32_t minE, maxE, minI, maxI;
 	unsigned fbits;
 } nemo_apply_args;
 
 hipError_t nemo_launch_apply_stdp(hipStream_t, const nemo_apply_args*);
+/* rcm_of (fcm_count entries): the accumulator index of every plastic
+ * synapse, ~0 for the others, from rcm_fwd (count entries). */
+hipError_t nemo_launch_invert_rcm(hipStream_t, const uint32_t* rcm_fwd, uint32_t* rcm_of, size_t count,
+		size_t fcm_count);
 
 /* Firing stimulus: set (or clear) the bits of the n given device indices. */
 hipError_t nemo_launch_fstim(hipStream_t, uint32_t* bits, const uint32_t* idx, size_t n, int set);

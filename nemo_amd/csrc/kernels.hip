@@ -95,14 +95,20 @@ __global__ void stdp_snapshot_done_kernel(nemo_hip_clock* clock, uint64_t offset
 	clock->stdp_snaps += 1;
 }
 
+/* applyStdp in forward order: every synapse of the FCM that is plastic
+ * (rcm_of[j] its accumulator) takes its weight change; reading the weights in
+ * order and the accumulators scattered costs one random read per synapse,
+ * where the reverse order costs a random read and write of the weight.  The
+ * accumulators are cleared afterwards by a separate fill. */
 __global__ void __launch_bounds__(NEMO_BLOCK)
 apply_stdp_kernel(nemo_apply_args a)
 {
-	for(size_t k = size_t(blockIdx.x) * NEMO_BLOCK + threadIdx.x; k < a.count;
-			k += size_t(gridDim.x) * NEMO_BLOCK) {
+	for(size_t j = size_t(blockIdx.x) * NEMO_BLOCK + threadIdx.x; j < a.fcm_count;
+			j += size_t(gridDim.x) * NEMO_BLOCK) {
+		const uint32_t k = a.rcm_of[j];
+		if(k == ~0u) continue;
 		const int32_t acc = a.reward == 0 ? 0 : a.rcm_acc[k];
-		int32_t* wp = reinterpret_cast<int32_t*>(a.fcm + a.rcm_fwd[k]) + 1;
-		const int32_t w_old = *wp;
+		const int32_t w_old = a.fcm[j].y;
 		const int32_t dw = int32_t((int64_t(a.reward) * int64_t(acc)) >> a.fbits);
 		int32_t w_new = 0;
 		if(w_old > 0) {
@@ -112,8 +118,16 @@ apply_stdp_kernel(nemo_apply_args a)
 			const int32_t x = stdp_wrap_add(w_old, dw);
 			w_new = min(a.minI, max(x, a.maxI));
 		}
-		if(w_new != w_old) *wp = w_new;
-		if(a.rcm_acc[k] != 0) a.rcm_acc[k] = 0;
+		if(w_new != w_old) a.fcm[j].y = w_new;
+	}
+}
+
+/* rcm_of[rcm_fwd[k]] = k for every plastic synapse k (rcm_of filled with ~0 before) */
+__global__ void __launch_bounds__(NEMO_BLOCK)
+invert_rcm_kernel(const uint32_t* rcm_fwd, uint32_t* rcm_of, size_t count)
+{
+	for(size_t k = size_t(blockIdx.x) * NEMO_BLOCK + threadIdx.x; k < count; k += size_t(gridDim.x) * NEMO_BLOCK) {
+		rcm_of[rcm_fwd[k]] = uint32_t(k);
 	}
 }
 
@@ -238,9 +252,22 @@ hipError_t nemo_launch_propagate(hipStream_t st, const nemo_propagate_args* a, u
 hipError_t nemo_launch_apply_stdp(hipStream_t st, const nemo_apply_args* a)
 {
 	if(a->count == 0) return hipSuccess;
-	size_t blocks = (a->count + NEMO_BLOCK - 1) / NEMO_BLOCK;
-	if(blocks > 8192) blocks = 8192;
+	size_t blocks = (a->fcm_count + NEMO_BLOCK - 1) / NEMO_BLOCK;
+	if(blocks > 16384) blocks = 16384;
 	hipLaunchKernelGGL(apply_stdp_kernel, dim3(unsigned(blocks)), dim3(NEMO_BLOCK), 0, st, *a);
+	hipError_t e = hipGetLastError();
+	if(e != hipSuccess) return e;
+	return hipMemsetAsync(a->rcm_acc, 0, a->count * sizeof(int32_t), st);
+}
+
+hipError_t nemo_launch_invert_rcm(hipStream_t st, const uint32_t* rcm_fwd, uint32_t* rcm_of, size_t count,
+		size_t fcm_count)
+{
+	hipError_t e = hipMemsetAsync(rcm_of, 0xff, fcm_count * sizeof(uint32_t), st);
+	if(e != hipSuccess || count == 0) return e;
+	size_t blocks = (count + NEMO_BLOCK - 1) / NEMO_BLOCK;
+	if(blocks > 16384) blocks = 16384;
+	hipLaunchKernelGGL(invert_rcm_kernel, dim3(unsigned(blocks)), dim3(NEMO_BLOCK), 0, st, rcm_fwd, rcm_of, count);
 	return hipGetLastError();
 }
 
