@@ -395,6 +395,9 @@ private:
 	DeviceArray<double> m_v, m_u, m_seg;  /* state in double; segment: entry v, u and input I */
 	DeviceArray<double> m_LV, m_LU, m_LI; /* adjoint state and recent input adjoints */
 	unsigned m_segK = 0;
+	DeviceArray<uint32_t> m_bits;         /* [T][m_pitch] spike bitmaps */
+	unsigned m_pitch = 0, m_bitsT = 0;
+	bool m_stage = false;
 
 	void prepare(unsigned T, unsigned K);
 };

@@ -25,7 +25,7 @@ typedef struct nemo_float_step_args {
 	const float* pc;
 	const float* pd;
 	const float* x;            /* [T][n] external current */
-	float* S;                  /* [T][n] spikes (read for t-d, written at t when store) */
+	float* S;                  /* [T][n] spikes (written at t when store) */
 	float* V;                  /* [T][n] membrane potential after the step (store) */
 	double* v;                 /* [n] state, updated */
 	double* u;
@@ -37,7 +37,20 @@ typedef struct nemo_float_step_args {
 	double* seg_I;
 	double* ckpt;              /* NULL, or [ceil(T/K)][2][n]: (v, u) before every K-th step */
 	unsigned K;
+	uint32_t* bits;            /* [T][pitch] spike bitmaps (read for t-d, written at t when store) */
+	unsigned pitch;            /* words per step, nemo_float_bitmap_pitch(n) or more, a multiple of 4 */
+	unsigned dmax;             /* longest delay */
+	int stage;                 /* stage the last dmax bitmaps in LDS; needs nemo_float_stage_bytes
+	                              of at most NEMO_FLOAT_STAGE_MAX */
 } nemo_float_step_args;
+
+/* Largest LDS stage of spike bitmaps a forward block takes. */
+#define NEMO_FLOAT_STAGE_MAX 49152
+unsigned nemo_float_bitmap_pitch(unsigned n);
+size_t nemo_float_stage_bytes(unsigned n, unsigned dmax);
+/* bits[t][.] = spike bitmap of S[t][.] for t < T */
+hipError_t nemo_launch_pack_spikes(hipStream_t, uint32_t* bits, unsigned pitch, const float* S, unsigned n,
+		unsigned T);
 
 /* One backward step t (in decreasing t) of the adjoint. */
 typedef struct nemo_float_back_args {

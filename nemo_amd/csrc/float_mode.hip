@@ -26,9 +26,13 @@
  * << 26.  Weights are gathered per call into both orders.  S, V, x and the
  * gradients are [T][N] row-major float.
  *
- * One wave (64 lanes) per neuron: lanes stride over the neuron's synapses,
- * a fixed butterfly sums them (so results do not depend on scheduling),
- * lane 0 does the neuron arithmetic.
+ * A group of lanes per neuron: they stride over the neuron's synapses, a
+ * fixed butterfly sums them (so results do not depend on scheduling), the
+ * group's first lane does the neuron arithmetic.  The forward step tests a
+ * synapse's spike in a bitmap of recent steps ([T][pitch] words, bit j of
+ * step t set when neuron j fired), which each block stages in LDS when the
+ * last dmax steps' bitmaps are small; a block of 32 neurons writes its step's
+ * bitmap word itself.
  */
 #include <hip/hip_runtime.h>
 
@@ -36,15 +40,33 @@
 
 namespace {
 
+/* Backward: LANES lanes per neuron, BLOCK threads per block. */
 constexpr unsigned BLOCK = 256;
-constexpr unsigned WAVES = BLOCK / 64;
+#ifndef NEMO_FLOAT_LANES
+#define NEMO_FLOAT_LANES 32
+#endif
+constexpr unsigned LANES = NEMO_FLOAT_LANES;
+constexpr unsigned GROUPS = BLOCK / LANES;
+static_assert(LANES >= 8 && LANES <= 64 && (LANES & (LANES - 1)) == 0, "LANES: a power of two, 8..64");
+/* Forward: FLANES lanes per neuron and 32 neurons per block, so that a
+ * block's spikes make one word of the spike bitmap. */
+#ifndef NEMO_FLOAT_FWD_LANES
+#define NEMO_FLOAT_FWD_LANES 32
+#endif
+constexpr unsigned FLANES = NEMO_FLOAT_FWD_LANES;
+constexpr unsigned FGROUPS = 32;
+constexpr unsigned FBLOCK = FGROUPS * FLANES;
+static_assert(FLANES >= 8 && FLANES <= 32 && (FLANES & (FLANES - 1)) == 0, "FLANES: a power of two, 8..32");
 constexpr uint32_t IDX_MASK = 0x3ffffffu;
 constexpr double H = 0.25;
+/* synapses in flight per lane */
+template<unsigned L> constexpr unsigned unroll_for() { return L >= 16 ? 256 / L : 16; }
 
-__device__ inline double wave_sum(double x)
+template<unsigned L>
+__device__ inline double group_sum(double x)
 {
 #pragma unroll
-	for(int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+	for(int o = L / 2; o > 0; o >>= 1) x += __shfl_xor(x, o);
 	return x;
 }
 
@@ -54,99 +76,188 @@ __device__ inline double surrogate(double v)
 	return 1.0 / (z * z);
 }
 
-__global__ void __launch_bounds__(BLOCK)
+/* Spike test of synapse (source src, delay d) at step t: bit src of the
+ * bitmap of step t - d, from the block's staged rows (row d-1 holds step
+ * t-d) or from global memory. */
+template<bool STAGED>
+__device__ inline bool spiked(const nemo_float_step_args& a, const uint32_t* stage, unsigned t, unsigned d, uint32_t src)
+{
+	const uint32_t w = STAGED ? stage[(d - 1) * a.pitch + (src >> 5)] : a.bits[size_t(t - d) * a.pitch + (src >> 5)];
+	return (w >> (src & 31)) & 1u;
+}
+
+template<bool STAGED>
+__global__ void __launch_bounds__(FBLOCK)
 forward_step_kernel(nemo_float_step_args a)
 {
-	const unsigned j = blockIdx.x * WAVES + threadIdx.x / 64;
-	const unsigned lane = threadIdx.x & 63;
-	if(j >= a.n) return;
+	constexpr unsigned UNROLL = unroll_for<FLANES>();
+	extern __shared__ uint4 stage4[];
+	__shared__ uint32_t fired;
 	const unsigned t = a.t;
 	const size_t n = a.n;
-	double acc = 0.0;
-	for(uint32_t k = a.rcm_ptr[j] + lane; k < a.rcm_ptr[j + 1]; k += 64) {
-		const uint32_t sd = a.rcm_sd[k];
-		const unsigned d = (sd >> 26) + 1;
-		if(t >= d && a.S[size_t(t - d) * n + (sd & IDX_MASK)] != 0.0f) acc += double(a.wr[k]);
+	const unsigned g = threadIdx.x / FLANES;
+	const unsigned j = blockIdx.x * FGROUPS + g;
+	const unsigned lane = threadIdx.x % FLANES;
+	const bool live = j < n;
+	uint32_t beg = 0, end = 0;
+	float xj = 0.0f, C = 0.0f, Dd = 0.0f;
+	double v0 = 0.0, u0 = 0.0, A = 0.0, B = 0.0;
+	if(live) {
+		/* the neuron's own operands, loaded before the staging and the
+		 * synapses so that their latency overlaps */
+		beg = a.rcm_ptr[j];
+		end = a.rcm_ptr[j + 1];
+		xj = a.x[size_t(t) * n + j];
+		v0 = a.v[j];
+		u0 = a.u[j];
+		A = a.pa[j];
+		B = a.pb[j];
+		C = a.pc[j];
+		Dd = a.pd[j];
 	}
-	acc = wave_sum(acc);
-	if(lane != 0) return;
-
-	const double I = acc + double(a.x[size_t(t) * n + j]);
-	const double v0 = a.v[j];
-	const double u0 = a.u[j];
-	if(a.record) {
-		const size_t r = size_t(t - a.t0) * n + j;
-		a.seg_v[r] = v0;
-		a.seg_u[r] = u0;
-		a.seg_I[r] = I;
-	}
-	if(a.ckpt && t % a.K == 0) {
-		a.ckpt[size_t(t / a.K) * 2 * n + j] = v0;
-		a.ckpt[size_t(t / a.K) * 2 * n + n + j] = u0;
-	}
-	const double A = a.pa[j], B = a.pb[j];
-	double v = v0, u = u0;
-	bool m = true;
-#pragma unroll
-	for(int k = 0; k < 4; ++k) {
-		if(m) {
-			v = v + H * (0.04 * v * v + 5.0 * v + 140.0 - u + I);
-			u = u + H * A * (B * v - u);
-			m = v < 30.0;
+	if(STAGED) {
+		/* bitmaps of steps t-1 .. t-min(dmax, t), pitch a multiple of 4 words */
+		const unsigned rows = min(a.dmax, t);
+		const unsigned q4 = a.pitch / 4;
+		const unsigned total = rows * q4;
+		for(unsigned e = threadIdx.x; e < total; e += FBLOCK) {
+			const unsigned r = e / q4;
+			stage4[e] = reinterpret_cast<const uint4*>(a.bits + size_t(t - 1 - r) * a.pitch)[e - r * q4];
 		}
 	}
-	const bool s = v >= 30.0;
-	if(s) {
-		v = double(a.pc[j]);
-		u = u + double(a.pd[j]);
+	if(threadIdx.x == 0) fired = 0;
+	__syncthreads();
+	const uint32_t* stage = reinterpret_cast<const uint32_t*>(stage4);
+
+	bool s = false;
+	if(live) {
+		/* UNROLL synapses per lane in flight; the sum runs in ascending
+		 * synapse order per lane */
+		double acc = 0.0;
+		for(uint32_t k0 = beg + lane; k0 < end; k0 += FLANES * UNROLL) {
+			uint32_t sd[UNROLL];
+			float w[UNROLL];
+#pragma unroll
+			for(unsigned q = 0; q < UNROLL; ++q) {
+				const uint32_t k = k0 + FLANES * q;
+				sd[q] = k < end ? a.rcm_sd[k] : 0u;
+				w[q] = k < end ? a.wr[k] : 0.0f;
+			}
+#pragma unroll
+			for(unsigned q = 0; q < UNROLL; ++q) {
+				const unsigned d = (sd[q] >> 26) + 1;
+				if(k0 + FLANES * q < end && t >= d && spiked<STAGED>(a, stage, t, d, sd[q] & IDX_MASK))
+					acc += double(w[q]);
+			}
+		}
+		acc = group_sum<FLANES>(acc);
+		if(lane == 0) {
+			const double I = acc + double(xj);
+			if(a.record) {
+				const size_t r = size_t(t - a.t0) * n + j;
+				a.seg_v[r] = v0;
+				a.seg_u[r] = u0;
+				a.seg_I[r] = I;
+			}
+			if(a.ckpt && t % a.K == 0) {
+				a.ckpt[size_t(t / a.K) * 2 * n + j] = v0;
+				a.ckpt[size_t(t / a.K) * 2 * n + n + j] = u0;
+			}
+			double v = v0, u = u0;
+			bool m = true;
+#pragma unroll
+			for(int k = 0; k < 4; ++k) {
+				if(m) {
+					v = v + H * (0.04 * v * v + 5.0 * v + 140.0 - u + I);
+					u = u + H * A * (B * v - u);
+					m = v < 30.0;
+				}
+			}
+			s = v >= 30.0;
+			if(s) {
+				v = double(C);
+				u = u + double(Dd);
+			}
+			a.v[j] = v;
+			a.u[j] = u;
+			if(a.store) {
+				a.S[size_t(t) * n + j] = s ? 1.0f : 0.0f;
+				a.V[size_t(t) * n + j] = float(v);
+			}
+		}
 	}
-	a.v[j] = v;
-	a.u[j] = u;
 	if(a.store) {
-		a.S[size_t(t) * n + j] = s ? 1.0f : 0.0f;
-		a.V[size_t(t) * n + j] = float(v);
+		/* the block's 32 neurons are word blockIdx.x of step t's bitmap */
+		if(s) atomicOr(&fired, 1u << g);
+		__syncthreads();
+		if(threadIdx.x == 0) a.bits[size_t(t) * a.pitch + blockIdx.x] = fired;
 	}
 }
 
 __global__ void __launch_bounds__(BLOCK)
 backward_step_kernel(nemo_float_back_args a)
 {
-	const unsigned i = blockIdx.x * WAVES + threadIdx.x / 64;
-	const unsigned lane = threadIdx.x & 63;
+	const unsigned i = blockIdx.x * GROUPS + threadIdx.x / LANES;
+	const unsigned lane = threadIdx.x % LANES;
 	if(i >= a.n) return;
 	const unsigned tau = a.t;
 	const size_t n = a.n;
 	const unsigned R = a.ring;
-	const bool fired = a.S[size_t(tau) * n + i] != 0.0f;
+	const uint32_t beg = a.fcm_ptr[i], end = a.fcm_ptr[i + 1];
+	const size_t r = size_t(tau) * n + i;
+	const bool fired = a.S[r] != 0.0f;
+	/* the neuron's own operands, loaded before the synapses so that their
+	 * latency overlaps the gathers' */
+	const size_t sq = size_t(tau - a.t0) * n + i;
+	const float gSr = a.gS ? a.gS[r] : 0.0f;
+	const float gVr = a.gV ? a.gV[r] : 0.0f;
+	const double LVi = a.LV[i], LUi = a.LU[i];
+	const double I = a.seg_I[sq];
+	const double sv = a.seg_v[sq], su = a.seg_u[sq];
+	const double A = a.pa[i], B = a.pb[i];
 
 	/* adjoint of this step's spike from the steps it reaches (transposed
-	 * delivery), and the weight gradient of the synapses it drove */
+	 * delivery), and the weight gradient of the synapses it drove; UNROLL
+	 * synapses per lane in flight, summed in ascending synapse order */
+	constexpr unsigned UNROLL = unroll_for<LANES>();
 	double acc = 0.0;
-	for(uint32_t k = a.fcm_ptr[i] + lane; k < a.fcm_ptr[i + 1]; k += 64) {
-		const uint32_t td = a.fcm_td[k];
-		const unsigned d = (td >> 26) + 1;
-		if(tau + d < a.T) {
-			const double l = a.LI[size_t((tau + d) % R) * n + (td & IDX_MASK)];
-			acc += double(a.wf[k]) * l;
-			if(fired) a.gw[k] = float(double(a.gw[k]) + l);
+	for(uint32_t k0 = beg + lane; k0 < end; k0 += LANES * UNROLL) {
+		uint32_t td[UNROLL];
+		float w[UNROLL], g[UNROLL];
+#pragma unroll
+		for(unsigned q = 0; q < UNROLL; ++q) {
+			const uint32_t k = k0 + LANES * q;
+			td[q] = k < end ? a.fcm_td[k] : 0u;
+			w[q] = k < end ? a.wf[k] : 0.0f;
+			g[q] = k < end && fired ? a.gw[k] : 0.0f;
+		}
+		bool live[UNROLL];
+		double l[UNROLL];
+#pragma unroll
+		for(unsigned q = 0; q < UNROLL; ++q) {
+			const unsigned d = (td[q] >> 26) + 1;
+			live[q] = k0 + LANES * q < end && tau + d < a.T;
+			l[q] = live[q] ? a.LI[size_t((tau + d) % R) * n + (td[q] & IDX_MASK)] : 0.0;
+		}
+#pragma unroll
+		for(unsigned q = 0; q < UNROLL; ++q) {
+			if(!live[q]) continue;
+			acc += double(w[q]) * l[q];
+			if(fired) a.gw[k0 + LANES * q] = float(double(g[q]) + l[q]);
 		}
 	}
-	acc = wave_sum(acc);
+	acc = group_sum<LANES>(acc);
 	if(lane != 0) return;
 
-	const size_t r = size_t(tau) * n + i;
-	const double lS = (a.gS ? double(a.gS[r]) : 0.0) + acc;
-	double lv = a.LV[i] + (a.gV ? double(a.gV[r]) : 0.0); /* d L / d v(tau+1) */
-	double lu = a.LU[i];                                  /* d L / d u(tau+1) */
+	const double lS = double(gSr) + acc;
+	double lv = LVi + double(gVr); /* d L / d v(tau+1) */
+	double lu = LUi;               /* d L / d u(tau+1) */
 
 	/* recompute the step's substeps from the recorded state and input */
-	const size_t q = size_t(tau - a.t0) * n + i;
-	const double I = a.seg_I[q];
-	const double A = a.pa[i], B = a.pb[i];
 	double vk[5], uk[5];
 	bool mk[4];
-	vk[0] = a.seg_v[q];
-	uk[0] = a.seg_u[q];
+	vk[0] = sv;
+	uk[0] = su;
 	bool m = true;
 #pragma unroll
 	for(int k = 0; k < 4; ++k) {
@@ -189,6 +300,22 @@ backward_step_kernel(nemo_float_back_args a)
 	a.ga[i] += float(ga);
 	a.gb[i] += float(gb);
 	a.gd[i] += float(gd);
+}
+
+/* Spike bitmaps [T][pitch] of S [T][n]: one wave per 64 neurons of a step. */
+__global__ void __launch_bounds__(BLOCK)
+pack_spikes_kernel(uint32_t* bits, unsigned pitch, const float* S, unsigned n)
+{
+	const unsigned t = blockIdx.y;
+	const unsigned j = blockIdx.x * BLOCK + threadIdx.x;
+	const unsigned j0 = j & ~63u;
+	const uint64_t m = __ballot(j < n && S[size_t(t) * n + j] != 0.0f);
+	if(j0 < n && (threadIdx.x & 63) == 0) {
+		/* j0 < n: word j0/32 < ceil(n/32); word j0/32 + 1 is at most
+		 * ceil(n/32), below the pitch when ceil(n/32) is odd */
+		bits[size_t(t) * pitch + j0 / 32] = uint32_t(m);
+		bits[size_t(t) * pitch + j0 / 32 + 1] = uint32_t(m >> 32);
+	}
 }
 
 __global__ void gather_kernel(float* out, const float* in, const uint32_t* idx, size_t n)
@@ -241,7 +368,35 @@ hipError_t nemo_launch_float_forward_step(hipStream_t st, const nemo_float_step_
 	if(a->n == 0) return hipSuccess;
 	if(a->record && (!a->seg_v || !a->seg_u || !a->seg_I || a->t < a->t0)) return hipErrorInvalidValue;
 	if(a->ckpt && a->K == 0) return hipErrorInvalidValue;
-	hipLaunchKernelGGL(forward_step_kernel, dim3(unsigned((a->n + WAVES - 1) / WAVES)), dim3(BLOCK), 0, st, *a);
+	if(!a->bits || a->pitch % 4 || a->pitch < nemo_float_bitmap_pitch(a->n) || a->dmax == 0)
+		return hipErrorInvalidValue;
+	const dim3 grid(unsigned((a->n + FGROUPS - 1) / FGROUPS));
+	if(a->stage) {
+		const size_t lds = nemo_float_stage_bytes(a->n, a->dmax);
+		if(lds > NEMO_FLOAT_STAGE_MAX) return hipErrorInvalidValue;
+		hipLaunchKernelGGL(forward_step_kernel<true>, grid, dim3(FBLOCK), lds, st, *a);
+	} else {
+		hipLaunchKernelGGL(forward_step_kernel<false>, grid, dim3(FBLOCK), 0, st, *a);
+	}
+	return hipGetLastError();
+}
+
+unsigned nemo_float_bitmap_pitch(unsigned n)
+{
+	return ((n + 31) / 32 + 3) / 4 * 4;
+}
+
+size_t nemo_float_stage_bytes(unsigned n, unsigned dmax)
+{
+	return size_t(dmax) * nemo_float_bitmap_pitch(n) * sizeof(uint32_t);
+}
+
+hipError_t nemo_launch_pack_spikes(hipStream_t st, uint32_t* bits, unsigned pitch, const float* S, unsigned n,
+		unsigned T)
+{
+	if(n == 0 || T == 0) return hipSuccess;
+	if(pitch % 4 || pitch < nemo_float_bitmap_pitch(n)) return hipErrorInvalidValue;
+	hipLaunchKernelGGL(pack_spikes_kernel, dim3(blocks_for(n), T), dim3(BLOCK), 0, st, bits, pitch, S, n);
 	return hipGetLastError();
 }
 
@@ -249,7 +404,7 @@ hipError_t nemo_launch_float_backward_step(hipStream_t st, const nemo_float_back
 {
 	if(a->n == 0) return hipSuccess;
 	if(a->ring == 0 || a->t >= a->T || a->t < a->t0) return hipErrorInvalidValue;
-	hipLaunchKernelGGL(backward_step_kernel, dim3(unsigned((a->n + WAVES - 1) / WAVES)), dim3(BLOCK), 0, st, *a);
+	hipLaunchKernelGGL(backward_step_kernel, dim3(unsigned((a->n + GROUPS - 1) / GROUPS)), dim3(BLOCK), 0, st, *a);
 	return hipGetLastError();
 }
 

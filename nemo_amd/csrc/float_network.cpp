@@ -7,6 +7,7 @@
 #include "float_mode.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -88,6 +89,15 @@ FloatNetwork::FloatNetwork(const Network& net, int device) : m_device(device < 0
 	m_LV.allocate(m_n);
 	m_LU.allocate(m_n);
 	m_LI.allocate(size_t(m_D + 1) * m_n);
+	m_pitch = nemo_float_bitmap_pitch(unsigned(m_n));
+	/* stage the recent spike bitmaps in LDS when they fit and cost fewer
+	 * cache lines than a block's (32 neurons') spike gathers would */
+	const size_t stage = nemo_float_stage_bytes(unsigned(m_n), m_D);
+	m_stage = stage <= NEMO_FLOAT_STAGE_MAX && stage / 64 <= 32 * (m_s / std::max<size_t>(1, m_n));
+	/* NEMO_HIP_FLOAT_STAGE=0 reads the bitmaps from global memory instead */
+	if(const char* e = std::getenv("NEMO_HIP_FLOAT_STAGE")) {
+		if(std::string(e) == "0") m_stage = false;
+	}
 }
 
 void FloatNetwork::prepare(unsigned T, unsigned K)
@@ -99,6 +109,10 @@ void FloatNetwork::prepare(unsigned T, unsigned K)
 	if(k > m_segK) {
 		m_seg.allocate(3 * size_t(k) * m_n);
 		m_segK = k;
+	}
+	if(T > m_bitsT) {
+		m_bits.allocate(size_t(T) * m_pitch);
+		m_bitsT = T;
 	}
 }
 
@@ -128,6 +142,10 @@ void FloatNetwork::forward(hipStream_t st, unsigned T, unsigned K, const float* 
 	f.record = 0;
 	f.ckpt = ckpt;
 	f.K = K;
+	f.bits = m_bits.ptr;
+	f.pitch = m_pitch;
+	f.dmax = m_D;
+	f.stage = m_stage;
 	for(unsigned t = 0; t < T; ++t) {
 		f.t = t;
 		HIP_CHECK(nemo_launch_float_forward_step(st, &f));
@@ -175,6 +193,12 @@ void FloatNetwork::backward(hipStream_t st, unsigned T, unsigned K, const float*
 	f.seg_I = m_seg.ptr + 2 * segPlane;
 	f.ckpt = nullptr;
 	f.K = K;
+	f.bits = m_bits.ptr;
+	f.pitch = m_pitch;
+	f.dmax = m_D;
+	f.stage = m_stage;
+	/* the replayed steps read the spikes of S through its bitmaps */
+	HIP_CHECK(nemo_launch_pack_spikes(st, m_bits.ptr, m_pitch, S, unsigned(m_n), T));
 
 	nemo_float_back_args g = {};
 	g.n = unsigned(m_n);

@@ -134,8 +134,13 @@ def _rel(a, b):
 
 
 @pytest.mark.gpu
-def test_forward_matches_fp64_reference():
+@pytest.mark.parametrize("stage", ["auto", "0"])
+def test_forward_matches_fp64_reference(stage, monkeypatch):
+    """Spike bitmaps staged in LDS (auto for this size) and read from global
+    memory (NEMO_HIP_FLOAT_STAGE=0)."""
     _gpu_or_fail()
+    if stage == "0":
+        monkeypatch.setenv("NEMO_HIP_FLOAT_STAGE", "0")
     spec, p = instance(1000, 100, 10, seed=5, sigma=5.0, steps=100)
     fnet = _float_network(spec)
     q = _to_dev(p, ("w", "a", "b", "c", "d", "x", "v0", "u0"))
@@ -152,10 +157,13 @@ def test_forward_matches_fp64_reference():
 
 
 @pytest.mark.gpu
-def test_gradients_match_fp64_autograd():
+@pytest.mark.parametrize("stage", ["auto", "0"])
+def test_gradients_match_fp64_autograd(stage, monkeypatch):
     """1k neurons, 100 synapses each, 100 steps, spike-count loss over the
-    last 50 steps against random target rates."""
+    last 50 steps against random target rates; spike bitmaps staged and not."""
     _gpu_or_fail()
+    if stage == "0":
+        monkeypatch.setenv("NEMO_HIP_FLOAT_STAGE", "0")
     T, last = 100, 50
     spec, p = instance(1000, 100, 10, seed=5, sigma=5.0, steps=T)
     target = torch.rand(1000, generator=torch.Generator().manual_seed(2), dtype=torch.float64) * 18 + 2
