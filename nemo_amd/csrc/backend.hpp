@@ -395,7 +395,8 @@ private:
 	DeviceArray<double> m_v, m_u, m_seg;  /* state in double; segment: entry v, u and input I */
 	DeviceArray<double> m_LV, m_LU, m_LI; /* adjoint state and recent input adjoints */
 	unsigned m_segK = 0;
-	DeviceArray<uint32_t> m_bits;         /* [T][m_pitch] spike bitmaps */
+	DeviceArray<uint32_t> m_bits;         /* [T][m_pitch] spike bitmaps of the forward */
+	DeviceArray<uint64_t> m_hist;         /* firing histories for the replay */
 	unsigned m_pitch = 0, m_bitsT = 0;
 	bool m_stage = false;
 

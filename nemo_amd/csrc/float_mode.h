@@ -29,12 +29,7 @@ typedef struct nemo_float_step_args {
 	float* V;                  /* [T][n] membrane potential after the step (store) */
 	double* v;                 /* [n] state, updated */
 	double* u;
-	int store;                 /* write S[t], V[t] */
-	int record;                /* write the step's entry state and input to seg_* */
-	unsigned t0;               /* first step of the recorded segment */
-	double* seg_v;             /* [K][n] */
-	double* seg_u;
-	double* seg_I;
+	int store;                 /* write S[t], V[t] and the bitmap of step t */
 	double* ckpt;              /* NULL, or [ceil(T/K)][2][n]: (v, u) before every K-th step */
 	unsigned K;
 	uint32_t* bits;            /* [T][pitch] spike bitmaps (read for t-d, written at t when store) */
@@ -48,10 +43,38 @@ typedef struct nemo_float_step_args {
 #define NEMO_FLOAT_STAGE_MAX 49152
 unsigned nemo_float_bitmap_pitch(unsigned n);
 size_t nemo_float_stage_bytes(unsigned n, unsigned dmax);
-/* bits[t][.] = spike bitmap of S[t][.] for t < T */
-hipError_t nemo_launch_pack_spikes(hipStream_t, uint32_t* bits, unsigned pitch, const float* S, unsigned n,
-		unsigned T);
 
+/* Replay of forward steps t0..t1-1 from the state before t0 and the
+ * recorded spikes (as firing histories): each step's entry state and input,
+ * equal to the forward's. */
+typedef struct nemo_float_replay_args {
+	unsigned n;
+	unsigned t0, t1;
+	unsigned dmax;             /* longest delay, at most 64 */
+	const uint32_t* rcm_ptr;
+	const uint32_t* rcm_sd;
+	const float* wr;
+	const float* pa;
+	const float* pb;
+	const float* pc;
+	const float* pd;
+	const float* x;            /* [T][n] */
+	const uint64_t* hist;      /* [hist_rows][n] from nemo_launch_pack_history */
+	unsigned hist_rows;
+	const double* v0;          /* [n] state before t0 */
+	const double* u0;
+	double* seg_v;             /* [t1 - t0][n] entry state and input of each step */
+	double* seg_u;
+	double* seg_I;
+} nemo_float_replay_args;
+
+hipError_t nemo_launch_float_replay(hipStream_t, const nemo_float_replay_args*);
+/* rows of the firing histories of a T-step run */
+unsigned nemo_float_history_rows(unsigned T);
+/* hist[r][j], bit k: neuron j fired at step 64 (r - 1) + k (S [T][n]); row 0
+ * and steps from T on are zero */
+hipError_t nemo_launch_pack_history(hipStream_t, uint64_t* hist, unsigned rows, const float* S, unsigned n,
+		unsigned T);
 /* One backward step t (in decreasing t) of the adjoint. */
 typedef struct nemo_float_back_args {
 	unsigned n;

@@ -76,6 +76,28 @@ __device__ inline double surrogate(double v)
 	return 1.0 / (z * z);
 }
 
+/* One step of the neuron from (v, u) with input I: four substeps while
+ * v < 30, then the reset; returns the spike.  The forward and the replay
+ * share it, so that the replay reproduces the forward's state exactly. */
+__device__ inline bool neuron_step(double& v, double& u, double I, double A, double B, float C, float D)
+{
+	bool m = true;
+#pragma unroll
+	for(int k = 0; k < 4; ++k) {
+		if(m) {
+			v = v + H * (0.04 * v * v + 5.0 * v + 140.0 - u + I);
+			u = u + H * A * (B * v - u);
+			m = v < 30.0;
+		}
+	}
+	const bool s = v >= 30.0;
+	if(s) {
+		v = double(C);
+		u = u + double(D);
+	}
+	return s;
+}
+
 /* Spike test of synapse (source src, delay d) at step t: bit src of the
  * bitmap of step t - d, from the block's staged rows (row d-1 holds step
  * t-d) or from global memory. */
@@ -115,11 +137,22 @@ forward_step_kernel(nemo_float_step_args a)
 		C = a.pc[j];
 		Dd = a.pd[j];
 	}
+	/* UNROLL synapses per lane in flight; the sum runs in ascending synapse
+	 * order per lane.  The first batch is issued before the staging so that
+	 * both are in flight together. */
+	uint32_t sd[UNROLL];
+	float w[UNROLL];
+	uint32_t k0 = beg + lane;
+#pragma unroll
+	for(unsigned q = 0; q < UNROLL; ++q) {
+		const uint32_t k = k0 + FLANES * q;
+		sd[q] = k < end ? a.rcm_sd[k] : 0u;
+		w[q] = k < end ? a.wr[k] : 0.0f;
+	}
 	if(STAGED) {
 		/* bitmaps of steps t-1 .. t-min(dmax, t), pitch a multiple of 4 words */
-		const unsigned rows = min(a.dmax, t);
 		const unsigned q4 = a.pitch / 4;
-		const unsigned total = rows * q4;
+		const unsigned total = min(a.dmax, t) * q4;
 		for(unsigned e = threadIdx.x; e < total; e += FBLOCK) {
 			const unsigned r = e / q4;
 			stage4[e] = reinterpret_cast<const uint4*>(a.bits + size_t(t - 1 - r) * a.pitch)[e - r * q4];
@@ -131,53 +164,31 @@ forward_step_kernel(nemo_float_step_args a)
 
 	bool s = false;
 	if(live) {
-		/* UNROLL synapses per lane in flight; the sum runs in ascending
-		 * synapse order per lane */
 		double acc = 0.0;
-		for(uint32_t k0 = beg + lane; k0 < end; k0 += FLANES * UNROLL) {
-			uint32_t sd[UNROLL];
-			float w[UNROLL];
-#pragma unroll
-			for(unsigned q = 0; q < UNROLL; ++q) {
-				const uint32_t k = k0 + FLANES * q;
-				sd[q] = k < end ? a.rcm_sd[k] : 0u;
-				w[q] = k < end ? a.wr[k] : 0.0f;
-			}
+		while(k0 < end) {
 #pragma unroll
 			for(unsigned q = 0; q < UNROLL; ++q) {
 				const unsigned d = (sd[q] >> 26) + 1;
 				if(k0 + FLANES * q < end && t >= d && spiked<STAGED>(a, stage, t, d, sd[q] & IDX_MASK))
 					acc += double(w[q]);
 			}
+			k0 += FLANES * UNROLL;
+#pragma unroll
+			for(unsigned q = 0; q < UNROLL; ++q) {
+				const uint32_t k = k0 + FLANES * q;
+				sd[q] = k < end ? a.rcm_sd[k] : 0u;
+				w[q] = k < end ? a.wr[k] : 0.0f;
+			}
 		}
 		acc = group_sum<FLANES>(acc);
 		if(lane == 0) {
 			const double I = acc + double(xj);
-			if(a.record) {
-				const size_t r = size_t(t - a.t0) * n + j;
-				a.seg_v[r] = v0;
-				a.seg_u[r] = u0;
-				a.seg_I[r] = I;
-			}
 			if(a.ckpt && t % a.K == 0) {
 				a.ckpt[size_t(t / a.K) * 2 * n + j] = v0;
 				a.ckpt[size_t(t / a.K) * 2 * n + n + j] = u0;
 			}
 			double v = v0, u = u0;
-			bool m = true;
-#pragma unroll
-			for(int k = 0; k < 4; ++k) {
-				if(m) {
-					v = v + H * (0.04 * v * v + 5.0 * v + 140.0 - u + I);
-					u = u + H * A * (B * v - u);
-					m = v < 30.0;
-				}
-			}
-			s = v >= 30.0;
-			if(s) {
-				v = double(C);
-				u = u + double(Dd);
-			}
+			s = neuron_step(v, u, I, A, B, C, Dd);
 			a.v[j] = v;
 			a.u[j] = u;
 			if(a.store) {
@@ -215,11 +226,14 @@ backward_step_kernel(nemo_float_back_args a)
 	const double I = a.seg_I[sq];
 	const double sv = a.seg_v[sq], su = a.seg_u[sq];
 	const double A = a.pa[i], B = a.pb[i];
+	const float ga0 = a.ga[i], gb0 = a.gb[i], gd0 = a.gd[i];
 
 	/* adjoint of this step's spike from the steps it reaches (transposed
 	 * delivery), and the weight gradient of the synapses it drove; UNROLL
 	 * synapses per lane in flight, summed in ascending synapse order */
-	constexpr unsigned UNROLL = unroll_for<LANES>();
+	/* 4 in flight per lane keeps the kernel at 7 waves per SIMD; 8 would
+	 * need more registers than 5 waves allow */
+	constexpr unsigned UNROLL = 4;
 	double acc = 0.0;
 	for(uint32_t k0 = beg + lane; k0 < end; k0 += LANES * UNROLL) {
 		uint32_t td[UNROLL];
@@ -297,24 +311,100 @@ backward_step_kernel(nemo_float_back_args a)
 	a.LU[i] = Lu;
 	a.LI[size_t(tau % R) * n + i] = lI;
 	if(a.gx) a.gx[r] = float(lI);
-	a.ga[i] += float(ga);
-	a.gb[i] += float(gb);
-	a.gd[i] += float(gd);
+	a.ga[i] = ga0 + float(ga);
+	a.gb[i] = gb0 + float(gb);
+	a.gd[i] = gd0 + float(gd);
 }
 
-/* Spike bitmaps [T][pitch] of S [T][n]: one wave per 64 neurons of a step. */
+/* Firing histories for the replay: hist[r][j] bit k set when neuron j
+ * fired at step 64 (r - 1) + k; row 0 (steps -64..-1) and steps from T on
+ * are zero. */
 __global__ void __launch_bounds__(BLOCK)
-pack_spikes_kernel(uint32_t* bits, unsigned pitch, const float* S, unsigned n)
+pack_history_kernel(uint64_t* hist, unsigned rows, const float* S, unsigned n, unsigned T)
 {
-	const unsigned t = blockIdx.y;
 	const unsigned j = blockIdx.x * BLOCK + threadIdx.x;
-	const unsigned j0 = j & ~63u;
-	const uint64_t m = __ballot(j < n && S[size_t(t) * n + j] != 0.0f);
-	if(j0 < n && (threadIdx.x & 63) == 0) {
-		/* j0 < n: word j0/32 < ceil(n/32); word j0/32 + 1 is at most
-		 * ceil(n/32), below the pitch when ceil(n/32) is odd */
-		bits[size_t(t) * pitch + j0 / 32] = uint32_t(m);
-		bits[size_t(t) * pitch + j0 / 32 + 1] = uint32_t(m >> 32);
+	const unsigned r = blockIdx.y;
+	if(j >= n || r >= rows) return;
+	uint64_t word = 0;
+	if(r > 0) {
+		const unsigned t0 = 64 * (r - 1);
+		for(unsigned k = 0; k < 64 && t0 + k < T; ++k)
+			if(S[size_t(t0 + k) * n + j] != 0.0f) word |= uint64_t(1) << k;
+	}
+	hist[size_t(r) * n + j] = word;
+}
+
+/* Steps t0..t1-1 of the forward in one launch, from the state before t0 and
+ * the recorded spikes: each step's entry state and input are recorded.  The
+ * synaptic sums are those of forward_step_kernel, operation for operation
+ * (same lanes per neuron, same synapse-to-lane assignment and order, same
+ * butterfly), so the replayed state equals the forward's.  CH steps at a
+ * time: each synapse's spikes over the CH steps come from at most two
+ * history words. */
+constexpr unsigned CH = 16;
+
+__global__ void __launch_bounds__(BLOCK)
+replay_kernel(nemo_float_replay_args a)
+{
+	constexpr unsigned UNROLL = unroll_for<FLANES>();
+	constexpr unsigned RGROUPS = BLOCK / FLANES;
+	const unsigned j = blockIdx.x * RGROUPS + threadIdx.x / FLANES;
+	const unsigned lane = threadIdx.x % FLANES;
+	if(j >= a.n) return;
+	const size_t n = a.n;
+	const uint32_t beg = a.rcm_ptr[j], end = a.rcm_ptr[j + 1];
+	const double A = a.pa[j], B = a.pb[j];
+	const float C = a.pc[j], Dd = a.pd[j];
+	double v = a.v0[j], u = a.u0[j];
+	for(unsigned c0 = a.t0; c0 < a.t1; c0 += CH) {
+		const unsigned steps = min(CH, a.t1 - c0);
+		float xs[CH];
+#pragma unroll
+		for(unsigned q = 0; q < CH; ++q) xs[q] = lane == 0 && q < steps ? a.x[size_t(c0 + q) * n + j] : 0.0f;
+		double acc[CH];
+#pragma unroll
+		for(unsigned q = 0; q < CH; ++q) acc[q] = 0.0;
+		for(uint32_t k0 = beg + lane; k0 < end; k0 += FLANES * UNROLL) {
+			uint32_t sd[UNROLL];
+			float w[UNROLL];
+#pragma unroll
+			for(unsigned q = 0; q < UNROLL; ++q) {
+				const uint32_t k = k0 + FLANES * q;
+				sd[q] = k < end ? a.rcm_sd[k] : 0u;
+				w[q] = k < end ? a.wr[k] : 0.0f;
+			}
+			/* the source's spikes at steps c0-d .. c0-d+CH-1: history bits
+			 * from p = c0 - d + 64 (d <= 64, so p >= 0) */
+			uint64_t lo[UNROLL], hi[UNROLL];
+#pragma unroll
+			for(unsigned q = 0; q < UNROLL; ++q) {
+				const bool valid = k0 + FLANES * q < end;
+				const unsigned p = c0 + 64 - ((sd[q] >> 26) + 1);
+				const size_t row = size_t(p >> 6) * n + (sd[q] & IDX_MASK);
+				lo[q] = valid ? a.hist[row] : 0;
+				hi[q] = valid && (p & 63) + CH > 64 ? a.hist[row + n] : 0;
+			}
+#pragma unroll
+			for(unsigned q = 0; q < UNROLL; ++q) {
+				const unsigned o = (c0 + 64 - ((sd[q] >> 26) + 1)) & 63;
+				const uint64_t win = o ? (lo[q] >> o) | (hi[q] << (64 - o)) : lo[q];
+#pragma unroll
+				for(unsigned c = 0; c < CH; ++c)
+					if((win >> c) & 1) acc[c] += double(w[q]);
+			}
+		}
+#pragma unroll
+		for(unsigned c = 0; c < CH; ++c) {
+			const double tot = group_sum<FLANES>(acc[c]);
+			if(lane == 0 && c < steps) {
+				const double I = tot + double(xs[c]);
+				const size_t rec = size_t(c0 + c - a.t0) * n + j;
+				a.seg_v[rec] = v;
+				a.seg_u[rec] = u;
+				a.seg_I[rec] = I;
+				neuron_step(v, u, I, A, B, C, Dd);
+			}
+		}
 	}
 }
 
@@ -366,7 +456,6 @@ extern "C" {
 hipError_t nemo_launch_float_forward_step(hipStream_t st, const nemo_float_step_args* a)
 {
 	if(a->n == 0) return hipSuccess;
-	if(a->record && (!a->seg_v || !a->seg_u || !a->seg_I || a->t < a->t0)) return hipErrorInvalidValue;
 	if(a->ckpt && a->K == 0) return hipErrorInvalidValue;
 	if(!a->bits || a->pitch % 4 || a->pitch < nemo_float_bitmap_pitch(a->n) || a->dmax == 0)
 		return hipErrorInvalidValue;
@@ -391,12 +480,27 @@ size_t nemo_float_stage_bytes(unsigned n, unsigned dmax)
 	return size_t(dmax) * nemo_float_bitmap_pitch(n) * sizeof(uint32_t);
 }
 
-hipError_t nemo_launch_pack_spikes(hipStream_t st, uint32_t* bits, unsigned pitch, const float* S, unsigned n,
+hipError_t nemo_launch_pack_history(hipStream_t st, uint64_t* hist, unsigned rows, const float* S, unsigned n,
 		unsigned T)
 {
-	if(n == 0 || T == 0) return hipSuccess;
-	if(pitch % 4 || pitch < nemo_float_bitmap_pitch(n)) return hipErrorInvalidValue;
-	hipLaunchKernelGGL(pack_spikes_kernel, dim3(blocks_for(n), T), dim3(BLOCK), 0, st, bits, pitch, S, n);
+	if(n == 0 || rows == 0) return hipSuccess;
+	if(rows < nemo_float_history_rows(T)) return hipErrorInvalidValue;
+	hipLaunchKernelGGL(pack_history_kernel, dim3(blocks_for(n), rows), dim3(BLOCK), 0, st, hist, rows, S, n, T);
+	return hipGetLastError();
+}
+
+unsigned nemo_float_history_rows(unsigned T)
+{
+	return (T + 63) / 64 + 2;
+}
+
+hipError_t nemo_launch_float_replay(hipStream_t st, const nemo_float_replay_args* a)
+{
+	if(a->n == 0 || a->t1 <= a->t0) return hipSuccess;
+	if(!a->hist || a->hist_rows < nemo_float_history_rows(a->t1) || a->dmax == 0 || a->dmax > 64)
+		return hipErrorInvalidValue;
+	constexpr unsigned RGROUPS = BLOCK / FLANES;
+	hipLaunchKernelGGL(replay_kernel, dim3(unsigned((a->n + RGROUPS - 1) / RGROUPS)), dim3(BLOCK), 0, st, *a);
 	return hipGetLastError();
 }
 

@@ -139,7 +139,6 @@ void FloatNetwork::forward(hipStream_t st, unsigned T, unsigned K, const float* 
 	f.v = m_v.ptr;
 	f.u = m_u.ptr;
 	f.store = 1;
-	f.record = 0;
 	f.ckpt = ckpt;
 	f.K = K;
 	f.bits = m_bits.ptr;
@@ -152,9 +151,9 @@ void FloatNetwork::forward(hipStream_t st, unsigned T, unsigned K, const float* 
 	}
 }
 
-/* Segments of K steps in reverse: restore the segment's checkpoint, replay
- * its forward steps recording entry state and input, then run its adjoint
- * steps backwards. */
+/* Segments of K steps in reverse: replay the segment's forward steps from its
+ * checkpoint (one launch), recording entry state and input, then run its
+ * adjoint steps backwards. */
 void FloatNetwork::backward(hipStream_t st, unsigned T, unsigned K, const float* w, const float* a,
 		const float* b, const float* c, const float* d, const float* x, const float* S, const double* ckpt,
 		const float* gS, const float* gV, float* gw, float* ga, float* gb, float* gd, float* gx,
@@ -172,8 +171,13 @@ void FloatNetwork::backward(hipStream_t st, unsigned T, unsigned K, const float*
 	const unsigned Kseg = std::min(K, T);
 	const size_t segPlane = size_t(Kseg) * m_n;
 
-	nemo_float_step_args f = {};
+	/* the replay reads the recorded spikes as firing histories */
+	const unsigned rows = nemo_float_history_rows(T);
+	if(size_t(rows) * m_n > m_hist.n) m_hist.allocate(size_t(rows) * m_n);
+	HIP_CHECK(nemo_launch_pack_history(st, m_hist.ptr, rows, S, unsigned(m_n), T));
+	nemo_float_replay_args f = {};
 	f.n = unsigned(m_n);
+	f.dmax = m_D;
 	f.rcm_ptr = m_rcmPtr.ptr;
 	f.rcm_sd = m_rcmSd.ptr;
 	f.wr = m_wr.ptr;
@@ -182,23 +186,11 @@ void FloatNetwork::backward(hipStream_t st, unsigned T, unsigned K, const float*
 	f.pc = c;
 	f.pd = d;
 	f.x = x;
-	f.S = const_cast<float*>(S); /* read only: store == 0 */
-	f.V = nullptr;
-	f.v = m_v.ptr;
-	f.u = m_u.ptr;
-	f.store = 0;
-	f.record = 1;
+	f.hist = m_hist.ptr;
+	f.hist_rows = rows;
 	f.seg_v = m_seg.ptr;
 	f.seg_u = m_seg.ptr + segPlane;
 	f.seg_I = m_seg.ptr + 2 * segPlane;
-	f.ckpt = nullptr;
-	f.K = K;
-	f.bits = m_bits.ptr;
-	f.pitch = m_pitch;
-	f.dmax = m_D;
-	f.stage = m_stage;
-	/* the replayed steps read the spikes of S through its bitmaps */
-	HIP_CHECK(nemo_launch_pack_spikes(st, m_bits.ptr, m_pitch, S, unsigned(m_n), T));
 
 	nemo_float_back_args g = {};
 	g.n = unsigned(m_n);
@@ -230,15 +222,11 @@ void FloatNetwork::backward(hipStream_t st, unsigned T, unsigned K, const float*
 	for(unsigned s = segments; s-- > 0;) {
 		const unsigned t0 = s * K;
 		const unsigned t1 = std::min(T, t0 + K);
-		HIP_CHECK(hipMemcpyAsync(m_v.ptr, ckpt + size_t(s) * 2 * m_n, m_n * sizeof(double),
-					hipMemcpyDeviceToDevice, st));
-		HIP_CHECK(hipMemcpyAsync(m_u.ptr, ckpt + size_t(s) * 2 * m_n + m_n, m_n * sizeof(double),
-					hipMemcpyDeviceToDevice, st));
 		f.t0 = t0;
-		for(unsigned t = t0; t < t1; ++t) {
-			f.t = t;
-			HIP_CHECK(nemo_launch_float_forward_step(st, &f));
-		}
+		f.t1 = t1;
+		f.v0 = ckpt + size_t(s) * 2 * m_n;
+		f.u0 = ckpt + size_t(s) * 2 * m_n + m_n;
+		HIP_CHECK(nemo_launch_float_replay(st, &f));
 		g.t0 = t0;
 		for(unsigned t = t1; t-- > t0;) {
 			g.t = t;

@@ -211,12 +211,17 @@ def test_gradients_match_finite_differences_spike_free():
 
 
 @pytest.mark.gpu
-def test_checkpoint_interval_does_not_change_gradients():
+@pytest.mark.parametrize("n,m,dmax,T", [(800, 60, 10, 60), (300, 40, 64, 150)])
+def test_checkpoint_interval_does_not_change_gradients(n, m, dmax, T):
+    """The adjoint replays each checkpoint segment in one launch from the
+    recorded spikes (16 steps at a time, each synapse's spikes from 64-step
+    history words); gradients must not depend on the segment length, so the
+    replayed state equals the forward's bit for bit.  The second case has
+    delays up to 64 and segments that cross history words."""
     _gpu_or_fail()
-    T = 60
-    spec, p = instance(800, 60, 10, seed=7, sigma=5.0, steps=T)
+    spec, p = instance(n, m, dmax, seed=7, sigma=5.0, steps=T)
     fnet = _float_network(spec)
-    target = torch.full((800,), 10.0, device="cuda")
+    target = torch.full((n,), 10.0, device="cuda")
     grads = []
     for K in (1, 7, 50, T):
         q = _to_dev(p, ("w", "a", "b", "c", "d", "x", "v0", "u0"), grad=PARAMS)
