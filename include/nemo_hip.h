@@ -267,10 +267,11 @@ nemo_status_t nemo_hip_float_forward(nemo_hip_float_network_t, void* stream, uns
 /* The adjoint of nemo_hip_float_forward: from dL/dspikes and dL/dv_out (either
  * may be NULL) and the forward's spikes and checkpoints, the gradients of the
  * loss with respect to w, a, b, d and x (grad_x may be NULL) and the initial
- * state v0, u0 (either may be NULL).  Each checkpoint segment is recomputed
- * forward, then stepped backward in time: the transposed delivery gathers
- * each source's spike adjoint from the current adjoints of its targets at
- * step + delay. */
+ * state v0, u0 (either may be NULL).  Each checkpoint segment is replayed
+ * forward from its checkpoint and the recorded spikes (reproducing the
+ * forward's state exactly), then stepped backward in time: the transposed
+ * delivery gathers each source's spike adjoint from the current adjoints of
+ * its targets at step + delay. */
 nemo_status_t nemo_hip_float_backward(nemo_hip_float_network_t, void* stream, unsigned steps,
 		unsigned checkpoint_every, const float* w, const float* a, const float* b, const float* c,
 		const float* d, const float* x, const float* spikes, const double* checkpoints,

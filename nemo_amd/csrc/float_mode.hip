@@ -234,6 +234,7 @@ backward_step_kernel(nemo_float_back_args a)
 	/* 4 in flight per lane keeps the kernel at 7 waves per SIMD; 8 would
 	 * need more registers than 5 waves allow */
 	constexpr unsigned UNROLL = 4;
+	const unsigned tm = tau % R; /* slot of step tau + d: tm + d, less R once past it (d < R) */
 	double acc = 0.0;
 	for(uint32_t k0 = beg + lane; k0 < end; k0 += LANES * UNROLL) {
 		uint32_t td[UNROLL];
@@ -251,7 +252,8 @@ backward_step_kernel(nemo_float_back_args a)
 		for(unsigned q = 0; q < UNROLL; ++q) {
 			const unsigned d = (td[q] >> 26) + 1;
 			live[q] = k0 + LANES * q < end && tau + d < a.T;
-			l[q] = live[q] ? a.LI[size_t((tau + d) % R) * n + (td[q] & IDX_MASK)] : 0.0;
+			const unsigned slot = tm + d >= R ? tm + d - R : tm + d;
+			l[q] = live[q] ? a.LI[size_t(slot) * n + (td[q] & IDX_MASK)] : 0.0;
 		}
 #pragma unroll
 		for(unsigned q = 0; q < UNROLL; ++q) {
@@ -346,8 +348,9 @@ constexpr unsigned CH = 16;
 __global__ void __launch_bounds__(BLOCK)
 replay_kernel(nemo_float_replay_args a)
 {
-	constexpr unsigned UNROLL = unroll_for<FLANES>();
+	constexpr unsigned UNROLL = 4;
 	constexpr unsigned RGROUPS = BLOCK / FLANES;
+	static_assert(CH <= FLANES, "lane c holds the input of step c of a chunk");
 	const unsigned j = blockIdx.x * RGROUPS + threadIdx.x / FLANES;
 	const unsigned lane = threadIdx.x % FLANES;
 	if(j >= a.n) return;
@@ -358,9 +361,7 @@ replay_kernel(nemo_float_replay_args a)
 	double v = a.v0[j], u = a.u0[j];
 	for(unsigned c0 = a.t0; c0 < a.t1; c0 += CH) {
 		const unsigned steps = min(CH, a.t1 - c0);
-		float xs[CH];
-#pragma unroll
-		for(unsigned q = 0; q < CH; ++q) xs[q] = lane == 0 && q < steps ? a.x[size_t(c0 + q) * n + j] : 0.0f;
+		const float xl = lane < steps ? a.x[size_t(c0 + lane) * n + j] : 0.0f;
 		double acc[CH];
 #pragma unroll
 		for(unsigned q = 0; q < CH; ++q) acc[q] = 0.0;
@@ -396,8 +397,9 @@ replay_kernel(nemo_float_replay_args a)
 #pragma unroll
 		for(unsigned c = 0; c < CH; ++c) {
 			const double tot = group_sum<FLANES>(acc[c]);
+			const float xc = __shfl(xl, c, FLANES);
 			if(lane == 0 && c < steps) {
-				const double I = tot + double(xs[c]);
+				const double I = tot + double(xc);
 				const size_t rec = size_t(c0 + c - a.t0) * n + j;
 				a.seg_v[rec] = v;
 				a.seg_u[rec] = u;
