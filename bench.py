@@ -93,13 +93,15 @@ def cpu_baseline(spec, seconds, max_steps):
     }
 
 
-def batched(api, gen, count, steps, autotune_steps):
+def batched(api, gen, count, steps, autotune_steps, warmup=200):
     """The batched workload: `count` independent networks (log-uniform sizes
     1k..50k, U{100..1000} synapses per neuron, STDP on every other one) in one
     simulation, against the same networks run one after another as plain
-    simulations.  Both from construction, `steps` steps each, firing read
-    back every step, graphs captured before the timed region; each network's
-    per-step firing counts are compared with its solo run."""
+    simulations.  Both run `warmup` untimed steps after construction (the
+    first steps of a new simulation run about 10% slower), then `steps` timed
+    steps, firing read back every step, graphs captured before the timed
+    region; each network's per-step firing counts are compared with its solo
+    run."""
     t = time.perf_counter()
     specs = gen.batch_izhikevich(count=count)
     nets = []
@@ -117,6 +119,8 @@ def batched(api, gen, count, steps, autotune_steps):
           % (count, nneu, nsyn, time.perf_counter() - t), file=sys.stderr, flush=True)
     default_shape = batch.launch_shape
     shape = batch.autotune(autotune_steps) if autotune_steps else default_shape
+    if warmup:
+        batch.run(warmup, stdp_period=1000, reward=1.0, raster=False)
     batch.prepare(steps, stdp_period=1000)
     t0 = time.perf_counter()
     bcounts, _, bevents = batch.run(steps, stdp_period=1000, reward=1.0)
@@ -133,6 +137,8 @@ def batched(api, gen, count, steps, autotune_steps):
     for k, net in enumerate(nets):
         stdp = k % 2 == 0
         sim = api.Simulation(net, stdp_conf if stdp else plain_conf)
+        if warmup:
+            sim.run(warmup, stdp_period=1000 if stdp else 0, reward=1.0, raster=False)
         sim.prepare(steps, stdp_period=1000 if stdp else 0)
         t0 = time.perf_counter()
         c, _, e = sim.run(steps, stdp_period=1000 if stdp else 0, reward=1.0)
@@ -143,7 +149,7 @@ def batched(api, gen, count, steps, autotune_steps):
     print("sequential runs %.3f s" % swall, file=sys.stderr, flush=True)
     return {
         "workload": "izhikevich_80_20_batch_%d_nets" % count,
-        "networks": count, "neurons": nneu, "synapses": nsyn, "steps": steps,
+        "networks": count, "neurons": nneu, "synapses": nsyn, "steps": steps, "warmup": warmup,
         "events_per_s": bevents / bwall,
         "ms_per_step": 1000.0 * bwall / steps,
         "sequential_events_per_s": sevents / swall,
@@ -154,7 +160,7 @@ def batched(api, gen, count, steps, autotune_steps):
         "launch_shape": list(shape),
         "default_launch_shape": list(default_shape),
         "note": "sequential = the same networks as separate plain simulations (default launch shapes), "
-                "one after another; construction and graph capture excluded from both",
+                "one after another; construction, warm-up steps and graph capture excluded from both",
     }
 
 
