@@ -216,7 +216,7 @@ def differentiable(api, gen, steps=500, checkpoint_every=50, reps=3):
         "forward_ms_per_step": f / steps,
         "adjoint_ms_per_step": bk / steps,
         "adjoint_over_forward": bk / f,
-        "firing_rate_hz": float(S.sum()) / n / (steps * 1e-3),
+        "firing_rate_hz": float(S.detach().sum()) / n / (steps * 1e-3),
         "loss": float(loss),
         "grad_norms": {k: float(p.grad.norm()) for k, p in (("w", w), ("a", a), ("b", b), ("d", d))},
         "dtype": "f32 (neuron substeps in f64)",
