@@ -164,6 +164,44 @@ def batched(api, gen, count, steps, autotune_steps, warmup=200):
     }
 
 
+def mixed(api, gen, steps=2000, warmup=200, autotune_steps=128, profile_steps=300):
+    """The mixed-model workload: 50k neurons on the unit torus (40k
+    IF_curr_exp, 8k QIF, 2k HH) plus 1k Poisson sources at 10 Hz, 500
+    outgoing synapses each with probability falling with distance (sigma
+    0.1) and delay ceil(20 d) + 1 ms, static weights; `warmup` untimed steps,
+    then `steps` timed steps with the firing read back every step."""
+    t = time.perf_counter()
+    spec = gen.torus_mixed()
+    net = api.Network()
+    spec.load(net)
+    nsyn, nneu = int(spec.synapse_count), int(spec.neuron_count)
+    del spec
+    sim = api.Simulation(net, api.Configuration())
+    print("mixed torus: %d neurons, %d synapses, built in %.1f s" % (nneu, nsyn, time.perf_counter() - t),
+          file=sys.stderr, flush=True)
+    if warmup:
+        sim.run(warmup, raster=False)
+    default_shape = sim.launch_shape
+    shape = sim.autotune(autotune_steps) if autotune_steps else default_shape
+    sim.prepare(steps)
+    t0 = time.perf_counter()
+    counts, _, events = sim.run(steps)
+    wall = time.perf_counter() - t0
+    kern, work = sim.profile(profile_steps)
+    del sim
+    return {
+        "workload": "torus_mixed_lif_qif_hh_poisson_51k_x_500",
+        "neurons": nneu, "synapses": nsyn, "steps": steps, "warmup": warmup,
+        "events_per_s": events / wall,
+        "ms_per_step": 1000.0 * wall / steps,
+        "sim_to_wall": steps * 1e-3 / wall,
+        "firing_rate_hz": float(counts.sum()) / nneu / (steps * 1e-3),
+        "events_per_step": events / steps,
+        "kernel_us": kern,
+        "launch_shape": list(shape), "default_launch_shape": list(default_shape),
+    }
+
+
 def differentiable(api, gen, steps=500, checkpoint_every=50, reps=3):
     """The differentiable workload: 10k Izhikevich neurons x 200 synapses
     (delays U{1..10}), FP32 weights and per-neuron a, b, d trainable, Gaussian
@@ -237,7 +275,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--autotune-steps", type=int, default=128, help="steps per autotuner trial (0: default shape)")
     ap.add_argument("--diff-steps", type=int, default=500, help="steps of the differentiable run (0: skip it)")
-    ap.add_argument("--only", choices=["batched", "differentiable"],
+    ap.add_argument("--mixed-steps", type=int, default=2000, help="steps of the mixed-model torus run (0: skip it)")
+    ap.add_argument("--only", choices=["batched", "differentiable", "mixed"],
                     help="run one of the secondary workloads alone and print its JSON (profiling)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel on BYTES (counter calibration)")
@@ -263,6 +302,9 @@ def main():
         return
     if args.only == "differentiable":
         print(json.dumps(differentiable(nemo_amd, gen, steps=args.diff_steps or 500)))
+        return
+    if args.only == "mixed":
+        print(json.dumps(mixed(nemo_amd, gen, steps=args.mixed_steps or 2000, autotune_steps=args.autotune_steps)))
         return
 
     n, m = args.neurons, args.synapses
@@ -342,6 +384,9 @@ def main():
     if args.diff_steps:
         result["differentiable"] = differentiable(nemo_amd, gen, steps=args.diff_steps)
         print("differentiable done", file=sys.stderr, flush=True)
+    if args.mixed_steps:
+        result["mixed"] = mixed(nemo_amd, gen, steps=args.mixed_steps, autotune_steps=args.autotune_steps)
+        print("mixed done", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         print("cpu baseline ...", file=sys.stderr, flush=True)
         cb = cpu_baseline(spec, args.cpu_seconds, 100000)
