@@ -468,6 +468,7 @@ void Simulation::construct(const std::vector<const Network*>& nets, const Config
 	HIP_CHECK(hipEventCreateWithFlags(&m_stdpLag[0], hipEventDisableTiming));
 	HIP_CHECK(hipEventCreateWithFlags(&m_stdpLag[1], hipEventDisableTiming));
 	HIP_CHECK(hipStreamCreateWithFlags(&m_earlyStream, hipStreamNonBlocking));
+	HIP_CHECK(hipStreamCreateWithFlags(&m_copyStream, hipStreamNonBlocking));
 	HIP_CHECK(hipEventCreateWithFlags(&m_earlyFork, hipEventDisableTiming));
 	HIP_CHECK(hipEventCreateWithFlags(&m_earlyDone[0], hipEventDisableTiming));
 	HIP_CHECK(hipEventCreateWithFlags(&m_earlyDone[1], hipEventDisableTiming));
@@ -501,7 +502,9 @@ Simulation::~Simulation()
 	if(m_stream) (void) hipStreamSynchronize(m_stream);
 	if(m_stdpStream) (void) hipStreamSynchronize(m_stdpStream);
 	if(m_earlyStream) (void) hipStreamSynchronize(m_earlyStream);
+	if(m_copyStream) (void) hipStreamSynchronize(m_copyStream);
 	clearGraphs();
+	if(m_copyStream) (void) hipStreamDestroy(m_copyStream);
 	if(m_earlyStream) {
 		(void) hipStreamDestroy(m_earlyStream);
 		(void) hipEventDestroy(m_earlyFork);
@@ -1474,7 +1477,9 @@ void Simulation::decodeFiringBatch(const uint64_t* bits, std::vector<unsigned>& 
 		counts.push_back(unsigned(out.size() - before));
 		return;
 	}
-	std::vector<std::vector<unsigned> > per(m_nets);
+	std::vector<std::vector<unsigned> >& per = m_decodePer; /* kept between steps: no allocation per step */
+	per.resize(m_nets);
+	for(std::vector<unsigned>& v : per) v.clear();
 	for(size_t w = 0; w < m_firedWords; ++w) {
 		uint64_t word = bits[w];
 		while(word) {
@@ -1493,17 +1498,19 @@ void Simulation::decodeFiringBatch(const uint64_t* bits, std::vector<unsigned>& 
 	}
 }
 
-/* Copy the bitmaps of steps [t0, t0+k) (k <= NEMO_STEP_RING) to m_pinnedBits. */
-void Simulation::readBitmaps(uint64_t t0, unsigned k, uint64_t* dst)
+/* Copy the bitmaps of steps [t0, t0+k) (k <= NEMO_STEP_RING) to m_pinnedBits
+ * (or dst), on the simulation's stream (or st). */
+void Simulation::readBitmaps(uint64_t t0, unsigned k, uint64_t* dst, hipStream_t st)
 {
 	const size_t W = m_firedWords;
 	if(!dst) dst = m_pinnedBits;
+	if(!st) st = m_stream;
 	unsigned done = 0;
 	while(done < k) {
 		const uint64_t slot = (t0 + done) & (NEMO_STEP_RING - 1);
 		const unsigned len = unsigned(std::min<uint64_t>(k - done, NEMO_STEP_RING - slot));
 		HIP_CHECK(hipMemcpyAsync(dst + size_t(done) * W, m_firedBits.ptr + slot * W,
-					size_t(len) * W * sizeof(uint64_t), hipMemcpyDeviceToHost, m_stream));
+					size_t(len) * W * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
 		done += len;
 	}
 }
@@ -1614,14 +1621,32 @@ void Simulation::run(unsigned nsteps, unsigned stdpPeriod, float reward, bool re
 	const uint64_t events0 = events ? sumCounters(m_blockEvents) : 0;
 	uint64_t stepsSinceApply = 0;
 	const unsigned CHUNK = NEMO_STEP_RING / 2;
+	/* room for the fired list at the rate of earlier runs, so that it is not
+	 * reallocated (and copied) while the device waits for the host */
+	const size_t firedBefore = readRaster && fired ? fired->size() : 0;
+	if(readRaster && fired && counts) {
+		fired->reserve(firedBefore + size_t(1.25 * m_firedPerStep * nsteps) + 1024);
+		counts->reserve(counts->size() + size_t(nsteps) * m_nets);
+	}
 	/* Chunks alternate between the two halves of the pinned bitmap buffer:
 	 * chunk i+1 is queued before chunk i's bitmaps are decoded on the host,
-	 * so the device does not wait for the host between chunks.  The device
-	 * ring holds 128 steps, so the bitmaps of chunk i are still intact while
-	 * chunk i+1 runs. */
-	hipEvent_t copied[2];
+	 * so the device does not wait for the host between chunks.  The copy of
+	 * chunk i runs on its own stream beside chunk i+1; the device ring holds
+	 * 128 steps, so chunk i's bitmaps are intact until chunk i+2, which waits
+	 * for that copy (the copy of chunk i is the last use of its half's event
+	 * before chunk i+2 records it again). */
+	hipEvent_t copied[2], ready[2];
+	bool inFlight[2] = {false, false};
 	HIP_CHECK(hipEventCreateWithFlags(&copied[0], hipEventDisableTiming));
 	HIP_CHECK(hipEventCreateWithFlags(&copied[1], hipEventDisableTiming));
+	HIP_CHECK(hipEventCreateWithFlags(&ready[0], hipEventDisableTiming));
+	HIP_CHECK(hipEventCreateWithFlags(&ready[1], hipEventDisableTiming));
+	auto destroyEvents = [&]() {
+		for(int i = 0; i < 2; ++i) {
+			(void) hipEventDestroy(copied[i]);
+			(void) hipEventDestroy(ready[i]);
+		}
+	};
 	unsigned pendingSteps = 0; /* steps of the chunk waiting to be decoded */
 	int pendingHalf = -1;
 	auto decodePending = [&]() {
@@ -1642,11 +1667,16 @@ void Simulation::run(unsigned nsteps, unsigned stdpPeriod, float reward, bool re
 			const uint64_t t0 = m_hostStep;
 			hipGraphExec_t g = graphFor(k);
 			setBase(t0);
+			/* the ring slots this chunk overwrites were copied out */
+			if(inFlight[half]) HIP_CHECK(hipStreamWaitEvent(m_stream, copied[half], 0));
 			HIP_CHECK(hipGraphLaunch(g, m_stream));
 			m_hostStep += k;
 			if(readRaster) {
-				readBitmaps(t0, k, m_pinnedBits + size_t(half) * CHUNK * m_firedWords);
-				HIP_CHECK(hipEventRecord(copied[half], m_stream));
+				HIP_CHECK(hipEventRecord(ready[half], m_stream));
+				HIP_CHECK(hipStreamWaitEvent(m_copyStream, ready[half], 0));
+				readBitmaps(t0, k, m_pinnedBits + size_t(half) * CHUNK * m_firedWords, m_copyStream);
+				HIP_CHECK(hipEventRecord(copied[half], m_copyStream));
+				inFlight[half] = true;
 			}
 			decodePending();
 			if(readRaster) {
@@ -1664,13 +1694,14 @@ void Simulation::run(unsigned nsteps, unsigned stdpPeriod, float reward, bool re
 		decodePending();
 	} catch(...) {
 		(void) hipStreamSynchronize(m_stream);
-		(void) hipEventDestroy(copied[0]);
-		(void) hipEventDestroy(copied[1]);
+		(void) hipStreamSynchronize(m_copyStream);
+		destroyEvents();
 		throw;
 	}
-	(void) hipEventDestroy(copied[0]);
-	(void) hipEventDestroy(copied[1]);
+	HIP_CHECK(hipStreamSynchronize(m_copyStream));
+	destroyEvents();
 	HIP_CHECK(hipStreamSynchronize(m_stream));
+	if(readRaster && fired && nsteps) m_firedPerStep = double(fired->size() - firedBefore) / nsteps;
 	if(events) *events = sumCounters(m_blockEvents) - events0;
 	m_firedOut.clear();
 }

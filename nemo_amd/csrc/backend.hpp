@@ -243,6 +243,9 @@ private:
 	hipEvent_t m_stdpFork = nullptr, m_stdpDone = nullptr;
 	hipEvent_t m_stdpLag[2] = {nullptr, nullptr}; /* mode 3: branch of step parity i done */
 	hipStream_t m_earlyStream = nullptr; /* branch for the delays 2..D of the next step */
+	hipStream_t m_copyStream = nullptr;  /* firing-bitmap readout of run(), beside the next steps */
+	double m_firedPerStep = 0.0;          /* fired neurons per step in the last run() with a raster */
+	mutable std::vector<std::vector<unsigned> > m_decodePer; /* decodeFiringBatch's per-network lists */
 	hipEvent_t m_earlyFork = nullptr, m_earlyDone[2] = {nullptr, nullptr};
 	bool m_stdpPending = false;          /* the STDP branch has not been joined yet */
 	unsigned m_nets = 1;   /* networks in the batch */
@@ -352,7 +355,7 @@ private:
 	void joinStdp();
 	void decodeFiring(const uint64_t* bits, std::vector<unsigned>& out) const;
 	void decodeFiringBatch(const uint64_t* bits, std::vector<unsigned>& out, std::vector<unsigned>& counts) const;
-	void readBitmaps(uint64_t t0, unsigned k, uint64_t* dst = nullptr);
+	void readBitmaps(uint64_t t0, unsigned k, uint64_t* dst = nullptr, hipStream_t st = nullptr);
 	uint64_t sumCounters(const DeviceArray<unsigned long long>& a);
 	hipGraphExec_t graphFor(unsigned k);
 	void clearGraphs();
