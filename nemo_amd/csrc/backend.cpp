@@ -476,6 +476,14 @@ void Simulation::construct(const std::vector<const Network*>& nets, const Config
 				hipHostMallocDefault));
 
 	buildNeurons(nets);
+	/* single-type batches (and any whose networks occupy increasing device
+	 * ranges in their own order) decode their fired lists in one pass */
+	m_decodeInOrder = m_nets > 1 && std::find(m_reorder.begin(), m_reorder.end(), 1) == m_reorder.end();
+	for(size_t l = 0, prev = 0; m_decodeInOrder && l < m_npad; ++l) {
+		if(m_netOf[l] == uint16_t(0xffff)) continue;
+		if(m_netOf[l] < prev) m_decodeInOrder = false;
+		prev = m_netOf[l];
+	}
 	buildConnectivity(nets, conf);
 	buildStdp(conf);
 	allocateStepState();
@@ -1475,6 +1483,20 @@ void Simulation::decodeFiringBatch(const uint64_t* bits, std::vector<unsigned>& 
 		const size_t before = out.size();
 		decodeFiring(bits, out);
 		counts.push_back(unsigned(out.size() - before));
+		return;
+	}
+	if(m_decodeInOrder) {
+		const size_t c0 = counts.size();
+		counts.resize(c0 + m_nets, 0u);
+		for(size_t w = 0; w < m_firedWords; ++w) {
+			uint64_t word = bits[w];
+			while(word) {
+				const unsigned l = unsigned(w * 64 + unsigned(__builtin_ctzll(word)));
+				word &= word - 1;
+				out.push_back(m_l2g[l]);
+				++counts[c0 + m_netOf[l]];
+			}
+		}
 		return;
 	}
 	std::vector<std::vector<unsigned> >& per = m_decodePer; /* kept between steps: no allocation per step */

@@ -373,6 +373,33 @@ def test_batch_matches_solo_runs():
     assert np.array_equal(batch.fixed_weights(), np.concatenate(weights))
 
 
+@pytest.mark.gpu
+def test_single_type_batch_matches_solo_runs():
+    """A batch of one neuron type lists each network's neurons in turn on the
+    device, and its fired lists are decoded in one pass: each network's raster
+    must still equal its own simulation's."""
+    specs = gen.batch_izhikevich(count=5, nmin=200, nmax=2000, smin=20, smax=100, seed=12)
+    nets = []
+    for spec, _ in specs:
+        net = nemo_amd.Network()
+        spec.load(net)
+        nets.append(net)
+    batch = nemo_amd.Batch(nets, _conf(nemo_amd, stdp_fn()))
+    steps = 200
+    counts, fired, events = batch.run(steps, stdp_period=100, reward=1.0)
+    per = nemo_amd.Batch.split(counts, fired)
+    solo_events = 0
+    for k, ((spec, stdp), net) in enumerate(zip(specs, nets)):
+        solo = nemo_amd.Simulation(net, _conf(nemo_amd, stdp_fn()) if stdp else nemo_amd.Configuration())
+        c, f, e = solo.run(steps, stdp_period=100 if stdp else 0, reward=1.0)
+        assert np.array_equal(per[k][0], c), ("counts", k)
+        assert np.array_equal(per[k][1], f), ("fired", k)
+        assert c.sum() > 0
+        solo_events += e
+    assert events == solo_events
+
+
+@pytest.mark.gpu
 def test_launch_shapes_give_identical_results():
     """Delivery blocks of both sets, bin capacity (down to 64 events, so that
     most events take the overflow path), STDP placement and block width, and
