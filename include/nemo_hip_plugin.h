@@ -197,6 +197,8 @@ typedef struct nemo_hip_step_buffers {
 	unsigned bin_blocks;       /* table columns (blocks of both sets) */
 	unsigned bin_capacity;
 	unsigned tiles;
+	unsigned bin_ranged;       /* nonzero: a tile's events lie in its bin_range columns only, and those
+	                              are much fewer than bin_blocks (batches): read just them */
 	int64_t* acc;              /* [2][2][acc_pitch] overflow accumulators: E plane, then I plane */
 	size_t acc_pitch;
 	float* ext_current;        /* per device index, or NULL */

@@ -1117,6 +1117,15 @@ void Simulation::applyShape(const LaunchShape& s)
 	HIP_CHECK(hipMemcpy(m_windows.ptr, windows.data(), windows.size() * sizeof(uint2), hipMemcpyHostToDevice));
 	m_binRange.allocate(range.size());
 	HIP_CHECK(hipMemcpy(m_binRange.ptr, range.data(), m_binRange.bytes(), hipMemcpyHostToDevice));
+	/* the update reads only a tile's column ranges (one dependent load more)
+	 * when they cover under a quarter of the table row on average;
+	 * NEMO_HIP_BIN_RANGED=0 or 1 forces the choice */
+	{
+		uint64_t width = 0;
+		for(uint32_t r : range) width += (r >> 16) - (r & 0xffffu);
+		m_binRanged = width * 4 < uint64_t(m_tiles) * C;
+		if(const char* e = std::getenv("NEMO_HIP_BIN_RANGED")) m_binRanged = std::atoi(e) != 0;
+	}
 	m_bins.allocate(2 * size_t(C) * s.binCapacity);
 	m_binTable.allocate(2 * size_t(m_tiles) * C);
 	HIP_CHECK(hipMemset(m_binTable.ptr, 0, m_binTable.bytes()));
@@ -1250,6 +1259,7 @@ nemo_hip_step_buffers Simulation::stepBuffers(uint64_t offset, bool ext, bool fs
 	b.bin_blocks = m_nsynapses ? m_shape.deliverBlocks + m_shape.lateBlocks : 0;
 	b.bin_capacity = m_shape.binCapacity;
 	b.tiles = m_tiles;
+	b.bin_ranged = m_binRanged ? 1u : 0u;
 	b.acc = m_acc.ptr;
 	b.acc_pitch = m_accPitch;
 	b.ext_current = ext ? m_ext.ptr : nullptr;

@@ -247,6 +247,7 @@ private:
 	double m_firedPerStep = 0.0;          /* fired neurons per step in the last run() with a raster */
 	mutable std::vector<std::vector<unsigned> > m_decodePer; /* decodeFiringBatch's per-network lists */
 	bool m_decodeInOrder = false;         /* batch whose device order lists each network in turn, in order */
+	bool m_binRanged = false;             /* updates read only their tiles' column ranges */
 	hipEvent_t m_earlyFork = nullptr, m_earlyDone[2] = {nullptr, nullptr};
 	bool m_stdpPending = false;          /* the STDP branch has not been joined yet */
 	unsigned m_nets = 1;   /* networks in the batch */

@@ -123,16 +123,28 @@ __device__ inline void gather_input(const nemo_hip_step_buffers& b, const nemo_h
 	const size_t parity = current_step(b) & 1;
 	int64_t* acc = b.acc + parity * 2 * b.acc_pitch;
 	const unsigned ng = tile * NEMO_TILE + tid;
-	const unsigned B = b.bin_blocks;
-	const uint32_t* row = b.bin_table + (parity * b.tiles + tile) * B;
+	const unsigned BB = b.bin_blocks;
+	const uint32_t* row = b.bin_table + (parity * b.tiles + tile) * BB;
 	const size_t cap = b.bin_capacity;
-	const int2* bins = b.bins + parity * B * cap;
+	const int2* bins = b.bins + parity * BB * cap;
+	/* the columns read: the whole row, or (bin_ranged) the tile's two ranges,
+	 * column first0 + i for i < n0, then first1 + i - n0 */
+	unsigned B = BB, first0 = 0, n0 = BB, first1 = 0;
+	if(b.bin_ranged) {
+		const uint32_t r0 = b.bin_range[tile], r1 = b.bin_range[b.tiles + tile];
+		first0 = r0 & 0xffffu;
+		n0 = (r0 >> 16) - first0;
+		first1 = r1 & 0xffffu;
+		B = n0 + ((r1 >> 16) - first1);
+	}
 	constexpr unsigned PER = NEMO_MAX_BIN_BLOCKS / NEMO_TILE;
 	uint32_t ent[PER];
+	unsigned col[PER];
 #pragma unroll
 	for(unsigned k = 0; k < PER; ++k) {
 		const unsigned i = tid * PER + k;
-		ent[k] = i < B ? row[i] : 0u;
+		col[k] = i < n0 ? first0 + i : first1 + (i - n0);
+		ent[k] = i < B ? row[col[k]] : 0u;
 	}
 	const int64_t oe = acc[ng];
 	const int64_t oi = acc[b.acc_pitch + ng];
@@ -149,7 +161,7 @@ __device__ inline void gather_input(const nemo_hip_step_buffers& b, const nemo_h
 #pragma unroll
 	for(unsigned k = 0; k < PER; ++k) {
 		const unsigned i = tid * PER + k;
-		if(i < B) s_base[i] = uint32_t(i * cap + (ent[k] & 0xffffu));
+		if(i < B) s_base[i] = uint32_t(col[k] * cap + (ent[k] & 0xffffu));
 		mine += ent[k] >> 16;
 	}
 	/* batches of GATHER_ILP events per thread, GATHER_ILP independent loads in

@@ -373,7 +373,6 @@ def test_batch_matches_solo_runs():
     assert np.array_equal(batch.fixed_weights(), np.concatenate(weights))
 
 
-@pytest.mark.gpu
 def test_single_type_batch_matches_solo_runs():
     """A batch of one neuron type lists each network's neurons in turn on the
     device, and its fired lists are decoded in one pass: each network's raster
@@ -399,7 +398,6 @@ def test_single_type_batch_matches_solo_runs():
     assert events == solo_events
 
 
-@pytest.mark.gpu
 def test_launch_shapes_give_identical_results():
     """Delivery blocks of both sets, bin capacity (down to 64 events, so that
     most events take the overflow path), STDP placement and block width, and
@@ -427,6 +425,34 @@ def test_launch_shapes_give_identical_results():
             assert np.array_equal(x, y), (shape, name)
         assert results[0][5] == r[5], (shape, "single steps")
     assert results[0][0].sum() > 0
+
+
+@pytest.mark.parametrize("which", ["random", "torus"])
+def test_ranged_table_reads_give_identical_results(which, monkeypatch):
+    """The update reads either a tile's whole table row or only the columns of
+    the delivery blocks whose windows hold the tile (NEMO_HIP_BIN_RANGED): the
+    same events either way, over several launch shapes (both delivery sets,
+    small bins with overflow, co-launched delivery)."""
+    if which == "random":
+        spec = gen.random_izhikevich(4000, 150, dmax=15, seed=21, stdp=True)
+    else:
+        spec = gen.torus_mixed(n_lif=4000, n_qif=800, n_hh=200, n_poisson=100, m=50, grid=15)
+    stdp = which == "random"
+    shapes = [None, (1, 1, 64, 0, 1, 1), (33, 5, 2048, 4, 3, 2), (7, 3, 256, 3, 1, 2), (200, 0, 1024, 4, 8, 1)]
+    results = []
+    for ranged in ("0", "1"):
+        monkeypatch.setenv("NEMO_HIP_BIN_RANGED", ranged)
+        for shape in shapes:
+            sim = sc.make(nemo_amd, spec, _conf(nemo_amd, stdp_fn()) if stdp else None)
+            if shape is not None:
+                sim.launch_shape = shape
+            c, f, e = sim.run(150, stdp_period=100 if stdp else 0)
+            results.append(((ranged, shape), c, f, e))
+    for key, c, f, e in results[1:]:
+        assert np.array_equal(results[0][1], c), (key, "counts")
+        assert np.array_equal(results[0][2], f), (key, "fired")
+        assert results[0][3] == e, (key, "events")
+    assert results[0][1].sum() > 0
 
 
 def test_autotune_keeps_the_simulation_state():
