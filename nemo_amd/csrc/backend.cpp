@@ -489,6 +489,18 @@ void Simulation::construct(const std::vector<const Network*>& nets, const Config
 	allocateStepState();
 	applyShape(defaultShape());
 
+	size_t models = 0;
+	for(const Group& g : m_groups) models += g.dev.count ? 1 : 0;
+	if(models > 1) {
+		HIP_CHECK(hipEventCreateWithFlags(&m_groupFork, hipEventDisableTiming));
+		m_groupStreams.resize(models - 1, nullptr);
+		m_groupDone.resize(models - 1, nullptr);
+		for(size_t q = 0; q + 1 < models; ++q) {
+			HIP_CHECK(hipStreamCreateWithFlags(&m_groupStreams[q], hipStreamNonBlocking));
+			HIP_CHECK(hipEventCreateWithFlags(&m_groupDone[q], hipEventDisableTiming));
+		}
+	}
+
 	nemo_hip_step_buffers b = stepBuffers(0, false, false);
 	for(Group& g : m_groups) {
 		HIP_CHECK(g.plugin->init(m_stream, &g.dev, &b));
@@ -511,8 +523,18 @@ Simulation::~Simulation()
 	if(m_stdpStream) (void) hipStreamSynchronize(m_stdpStream);
 	if(m_earlyStream) (void) hipStreamSynchronize(m_earlyStream);
 	if(m_copyStream) (void) hipStreamSynchronize(m_copyStream);
+	for(hipStream_t q : m_groupStreams) {
+		if(q) (void) hipStreamSynchronize(q);
+	}
 	clearGraphs();
 	if(m_copyStream) (void) hipStreamDestroy(m_copyStream);
+	for(hipStream_t q : m_groupStreams) {
+		if(q) (void) hipStreamDestroy(q);
+	}
+	for(hipEvent_t e : m_groupDone) {
+		if(e) (void) hipEventDestroy(e);
+	}
+	if(m_groupFork) (void) hipEventDestroy(m_groupFork);
 	if(m_earlyStream) {
 		(void) hipStreamDestroy(m_earlyStream);
 		(void) hipEventDestroy(m_earlyFork);
@@ -1408,20 +1430,49 @@ void Simulation::enqueueSteps(uint64_t first, unsigned k, bool ext, bool fstim, 
 		for(size_t gi = 0; gi < m_groups.size(); ++gi) {
 			if(m_groups[gi].dev.count) last = gi;
 		}
-		if(ev) b.time_start = ev[4 * i + 2];
-		for(size_t gi = 0; gi < m_groups.size(); ++gi) {
-			Group& g = m_groups[gi];
-			if(g.dev.count == 0) continue;
-			if(ev && gi == last) b.time_stop = ev[4 * i + 3];
-			HIP_CHECK(g.plugin->update(m_stream, &g.dev, &b));
-			b.co.deliver_blocks = 0; /* the first group's launch carries them */
-			b.co.stdp_blocks = 0;
-			b.co.lds_bytes = 0;
-			b.time_start = nullptr;
-		}
-		if(ev && last == m_groups.size()) {
-			HIP_CHECK(hipEventRecord(ev[4 * i + 2], m_stream));
-			HIP_CHECK(hipEventRecord(ev[4 * i + 3], m_stream));
+		if(m_groupStreams.empty()) {
+			if(ev) b.time_start = ev[4 * i + 2];
+			for(size_t gi = 0; gi < m_groups.size(); ++gi) {
+				Group& g = m_groups[gi];
+				if(g.dev.count == 0) continue;
+				if(ev && gi == last) b.time_stop = ev[4 * i + 3];
+				HIP_CHECK(g.plugin->update(m_stream, &g.dev, &b));
+				b.co.deliver_blocks = 0; /* the first group's launch carries them */
+				b.co.stdp_blocks = 0;
+				b.co.lds_bytes = 0;
+				b.time_start = nullptr;
+			}
+			if(ev && last == m_groups.size()) {
+				HIP_CHECK(hipEventRecord(ev[4 * i + 2], m_stream));
+				HIP_CHECK(hipEventRecord(ev[4 * i + 3], m_stream));
+			}
+		} else {
+			/* the models' updates touch disjoint tiles: the first on this
+			 * stream (with any propagate blocks the launch carries), the
+			 * others on branches joined before the step ends */
+			if(ev) HIP_CHECK(hipEventRecord(ev[4 * i + 2], m_stream));
+			HIP_CHECK(hipEventRecord(m_groupFork, m_stream));
+			size_t side = 0;
+			bool first = true;
+			for(size_t gi = 0; gi < m_groups.size(); ++gi) {
+				Group& g = m_groups[gi];
+				if(g.dev.count == 0) continue;
+				if(first) {
+					HIP_CHECK(g.plugin->update(m_stream, &g.dev, &b));
+					b.co.deliver_blocks = 0; /* the first group's launch carries them */
+					b.co.stdp_blocks = 0;
+					b.co.lds_bytes = 0;
+					first = false;
+					continue;
+				}
+				hipStream_t q = m_groupStreams[side];
+				HIP_CHECK(hipStreamWaitEvent(q, m_groupFork, 0));
+				HIP_CHECK(g.plugin->update(q, &g.dev, &b));
+				HIP_CHECK(hipEventRecord(m_groupDone[side], q));
+				++side;
+			}
+			for(size_t q = 0; q < side; ++q) HIP_CHECK(hipStreamWaitEvent(m_stream, m_groupDone[q], 0));
+			if(ev) HIP_CHECK(hipEventRecord(ev[4 * i + 3], m_stream));
 		}
 		if(overlap && i + 2 < k) forkEarly(i + 2);
 		if(batched && (i + 1 == k || (i + 1) % 64 == 0)) {

@@ -244,6 +244,11 @@ private:
 	hipEvent_t m_stdpLag[2] = {nullptr, nullptr}; /* mode 3: branch of step parity i done */
 	hipStream_t m_earlyStream = nullptr; /* branch for the delays 2..D of the next step */
 	hipStream_t m_copyStream = nullptr;  /* firing-bitmap readout of run(), beside the next steps */
+	/* with several neuron models, every model's update after the first runs
+	 * on a branch of its own, so a step waits for the slowest model */
+	std::vector<hipStream_t> m_groupStreams;
+	std::vector<hipEvent_t> m_groupDone;
+	hipEvent_t m_groupFork = nullptr;
 	double m_firedPerStep = 0.0;          /* fired neurons per step in the last run() with a raster */
 	mutable std::vector<std::vector<unsigned> > m_decodePer; /* decodeFiringBatch's per-network lists */
 	bool m_decodeInOrder = false;         /* batch whose device order lists each network in turn, in order */
