@@ -86,6 +86,9 @@ typedef struct nemo_hip_neuron_group {
 	float* param;      /* nParam planes */
 	float* state;      /* nState planes (history length 1) */
 	uint32_t* rng;     /* 4 xorshift words per neuron, interleaved; may be NULL */
+	float* scratch;    /* the planes the model asks for with scratch-planes under [hip] in its .ini
+	                      (pitch apart), owned by the simulation; NULL if none.  Contents are the
+	                      model's own between its launches of a step; nothing else reads them */
 } nemo_hip_neuron_group;
 
 /* STDP accumulation of step r = s-1 for the neurons of an update block's tile

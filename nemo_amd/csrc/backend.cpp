@@ -176,6 +176,8 @@ NeuronType loadNeuronType(const std::string& name)
 				if(parseBool(val)) t.usesRcm = true;
 			} else if(section == "backends") {
 				if(key == "hip") hip = parseBool(val);
+			} else if(section == "hip") {
+				if(key == "scratch-planes") t.scratchPlanes = unsigned(std::stoul(val));
 			} else if(section == "param_names") {
 				t.paramNames[key] = unsigned(std::stoul(val));
 			} else if(section == "state_names") {
@@ -701,6 +703,11 @@ void Simulation::buildNeurons(const std::vector<const Network*>& nets)
 		g.dev.param = g.param.ptr;
 		g.dev.state = g.state.ptr;
 		g.dev.rng = g.rng.ptr;
+		g.dev.scratch = nullptr;
+		if(g.type.scratchPlanes && g.count) {
+			g.scratch.allocate(size_t(g.type.scratchPlanes) * pitch);
+			g.dev.scratch = g.scratch.ptr;
+		}
 		start += (g.count + NEMO_TILE - 1) / NEMO_TILE * NEMO_TILE;
 	}
 }

@@ -47,6 +47,7 @@ struct NeuronType {
 	unsigned history = 1;
 	bool rngNormal = false;
 	bool usesRcm = false;
+	unsigned scratchPlanes = 0; /* [hip] scratch-planes: per-neuron device planes the plugin may use */
 	std::string pluginDir;
 	std::map<std::string, unsigned> paramNames;
 	std::map<std::string, unsigned> stateNames;
@@ -232,7 +233,7 @@ private:
 		NeuronType type;
 		std::unique_ptr<Plugin> plugin;
 		unsigned start = 0, count = 0;
-		DeviceArray<float> param, state;
+		DeviceArray<float> param, state, scratch;
 		DeviceArray<uint32_t> rng;
 		nemo_hip_neuron_group dev;
 	};

@@ -6,6 +6,16 @@
  * neuron fires on the first downturn of v (change < -1e-9) once the timer
  * cycle is >= 10.  State v, n, m, h, dir (HH.ini: 5 state variables, no
  * parameters).  Per-type arrays are indexed within the type (see QIF).
+ *
+ * A step is three launches when the group has its four scratch planes
+ * (HH.ini [hip] scratch-planes=4): the tiles' input gather (one thread per
+ * neuron, as every model), the substeps with eight lanes per neuron in
+ * one-wave blocks -- lane q evaluates the q-th of the six rate functions
+ * (each an expf), the lanes exchange them and all carry the same state -- and
+ * the tiles' firing record.  One wave of a tile block otherwise spends each
+ * substep on six exponentials for its 64 neurons, one wave per SIMD on a few
+ * CUs; the split spreads the same arithmetic, operation for operation, over
+ * eight times as many waves.  Without scratch planes the single launch runs.
  */
 #include <hip/hip_ext.h>
 
@@ -13,17 +23,54 @@
 
 namespace {
 
+using nemo_math::expf_exact;
+
+/* HH.cpp:26-39 */
+constexpr float DT = 0.001f;
+constexpr float G_NA = 120.0f;
+constexpr float G_K = 36.0f;
+constexpr float G_L = 0.3f;
+constexpr float E_NA = 115.0f - 65.0f;
+constexpr float E_K = -12.0f - 65.0f;
+constexpr float E_L = 10.6f - 65.0f;
+constexpr float C_M = 1.0f;
+constexpr float REV_E = 0.0f;
+constexpr float REV_I = -70.0f;
+
+/* scratch planes of the split step */
+enum { SCR_E = 0, SCR_I = 1, SCR_X = 2, SCR_FIRED = 3 };
+constexpr unsigned LANES = 8;      /* per neuron in the substep launch */
+constexpr unsigned SUB_BLOCK = 64; /* one wave: eight neurons */
+
+/* The gating and membrane update of one substep from the six rates (HH.cpp:44-62). */
+__device__ inline void hh_substep(float& v, float& n, float& m, float& h, float& dir, bool& fired, uint64_t cycle,
+		float Excit, float Inhib, float Exter, float alphan, float alpham, float alphah, float betan,
+		float betam, float betah)
+{
+	const float I = (Excit * (REV_E - v)) + (Inhib * ((REV_I - v) / -1)) + Exter;
+	m = m + DT * (alpham * (1.0f - m) - betam * m);
+	n = n + DT * (alphan * (1.0f - n) - betan * n);
+	h = h + DT * (alphah * (1.0f - h) - betah * h);
+	const float Ik = G_NA * (m * m * m) * h * (v - E_NA) + G_K * (n * n * n * n) * (v - E_K) + G_L * (v - E_L);
+	const float newv = v + DT * (-Ik + I) / C_M;
+	const float new_dir = (newv - v);
+	const float change = (dir < 0) | (newv < -45) ? 0 : new_dir;
+	dir = new_dir;
+	if(!fired && cycle >= 10) fired = double(change) < -0.000000001;
+	v = newv;
+}
+
+/* Single launch: gather, substeps and firing record per tile. */
 __global__ void __launch_bounds__(NEMO_UPDATE_BLOCK)
 hh_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 {
 	if(nemo_dev::colaunched_block(b, g)) return;
-	using nemo_math::expf_exact;
 	__shared__ nemo_dev::TileInput input;
 	nemo_dev::gather_input(b, g, input);
 	const unsigned nl = blockIdx.x * NEMO_UPDATE_BLOCK + threadIdx.x;
 	const bool valid = nl < g.count;
 	const unsigned nn = g.start + nl;
-	const uint64_t h = nemo_dev::load_history(b, nn, valid);
+	const uint64_t hist = nemo_dev::load_history(b, nn, valid);
 	bool fired = false;
 
 	if(valid) {
@@ -38,38 +85,15 @@ hh_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 		nemo_dev::take_current(b, input, nn, Excit, Inhib);
 		const float Exter = nemo_dev::take_external(b, nn);
 		const uint64_t cycle = nemo_dev::current_step(b) - b.clock->cycle_origin;
-
-		const float dt = 0.001f;
-		const float gNa = 120.0f;
-		const float gK = 36.0f;
-		const float gL = 0.3f;
-		const float ENa = 115.0f - 65.0f;
-		const float EK = -12.0f - 65.0f;
-		const float EL = 10.6f - 65.0f;
-		const float C = 1.0f;
-		const float RevE = 0.0f;
-		const float RevI = -70.0f;
-		const int inc_max = (int)(1 / dt);
-
+		const int inc_max = (int)(1 / DT);
 		for(int t = 0; t < inc_max; ++t) {
-			float I = (Excit * (RevE - v)) + (Inhib * ((RevI - v) / -1)) + Exter;
-			float alphan = (0.1f - 0.01f * (v + 65.0f)) / (expf_exact(1.0f - 0.1f * (v + 65.0f)) - 1.0f);
-			float alpham = (2.5f - 0.1f * (v + 65.0f)) / (expf_exact(2.5f - 0.1f * (v + 65.0f)) - 1.0f);
-			float alphah = 0.07f * expf_exact(-(v + 65.0f) / 20.0f);
-			float betan = 0.125f * expf_exact(-(v + 65.0f) / 80.0f);
-			float betam = 4.0f * expf_exact(-(v + 65.0f) / 18.0f);
-			float betah = 1.0f / (expf_exact(3.0f - 0.1f * (v + 65.0f)) + 1.0f);
-			m = m + dt * (alpham * (1.0f - m) - betam * m);
-			n = n + dt * (alphan * (1.0f - n) - betan * n);
-			h = h + dt * (alphah * (1.0f - h) - betah * h);
-			float Ik = gNa * (m * m * m) * h * (v - ENa) + gK * (n * n * n * n) * (v - EK) + gL * (v - EL);
-			float newv = v + dt * (-Ik + I) / C;
-			float new_dir = (newv - v);
-			float change = (dir < 0) | (newv < -45) ? 0 : new_dir;
-			dir = new_dir;
-			if(!fired && cycle >= 10)
-				fired = double(change) < -0.000000001;
-			v = newv;
+			const float alphan = (0.1f - 0.01f * (v + 65.0f)) / (expf_exact(1.0f - 0.1f * (v + 65.0f)) - 1.0f);
+			const float alpham = (2.5f - 0.1f * (v + 65.0f)) / (expf_exact(2.5f - 0.1f * (v + 65.0f)) - 1.0f);
+			const float alphah = 0.07f * expf_exact(-(v + 65.0f) / 20.0f);
+			const float betan = 0.125f * expf_exact(-(v + 65.0f) / 80.0f);
+			const float betam = 4.0f * expf_exact(-(v + 65.0f) / 18.0f);
+			const float betah = 1.0f / (expf_exact(3.0f - 0.1f * (v + 65.0f)) + 1.0f);
+			hh_substep(v, n, m, h, dir, fired, cycle, Excit, Inhib, Exter, alphan, alpham, alphah, betan, betam, betah);
 		}
 		fired = fired || nemo_dev::stimulated(b, nn);
 		s[0 * P + nl] = v;
@@ -78,8 +102,98 @@ hh_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 		s[3 * P + nl] = h;
 		s[4 * P + nl] = dir;
 	}
-	nemo_dev::record_firing(b, nn, valid, fired, h);
-	nemo_dev::stdp_tile(b, nn, valid, h);
+	nemo_dev::record_firing(b, nn, valid, fired, hist);
+	nemo_dev::stdp_tile(b, nn, valid, hist);
+}
+
+/* Split step, launch 1: the tiles' synaptic and external input. */
+__global__ void __launch_bounds__(NEMO_UPDATE_BLOCK)
+hh_input(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
+{
+	if(nemo_dev::colaunched_block(b, g)) return;
+	__shared__ nemo_dev::TileInput input;
+	nemo_dev::gather_input(b, g, input);
+	const unsigned nl = blockIdx.x * NEMO_UPDATE_BLOCK + threadIdx.x;
+	if(nl < g.count) {
+		const unsigned nn = g.start + nl;
+		const size_t P = g.pitch;
+		float Excit, Inhib;
+		nemo_dev::take_current(b, input, nn, Excit, Inhib);
+		g.scratch[SCR_E * P + nl] = Excit;
+		g.scratch[SCR_I * P + nl] = Inhib;
+		g.scratch[SCR_X * P + nl] = nemo_dev::take_external(b, nn);
+	}
+}
+
+/* Split step, launch 2: the substeps, LANES lanes per neuron.  Lane q < 6
+ * evaluates rate q (alphan, alpham, alphah, betan, betam, betah; lanes 6 and
+ * 7 repeat betah) from exactly the expression of the single launch; every
+ * lane then applies the same update, so all lanes of a neuron hold the same
+ * state and the first writes it. */
+__global__ void __launch_bounds__(SUB_BLOCK)
+hh_substeps(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
+{
+	const unsigned t = blockIdx.x * SUB_BLOCK + threadIdx.x;
+	const unsigned nl = t / LANES;
+	const unsigned q = t % LANES;
+	if(nl >= g.count) return; /* whole groups of LANES lanes */
+	const size_t P = g.pitch;
+	float* s = g.state;
+	float v = s[0 * P + nl];
+	float n = s[1 * P + nl];
+	float m = s[2 * P + nl];
+	float h = s[3 * P + nl];
+	float dir = s[4 * P + nl];
+	const float Excit = g.scratch[SCR_E * P + nl];
+	const float Inhib = g.scratch[SCR_I * P + nl];
+	const float Exter = g.scratch[SCR_X * P + nl];
+	const uint64_t cycle = nemo_dev::current_step(b) - b.clock->cycle_origin;
+	/* per lane: the constant of its linear argument (alphan, alpham, betah),
+	 * the divisor of its scaled argument (alphah, betan, betam) and the
+	 * factor of its exponential */
+	const unsigned r = q < 6 ? q : 5;
+	const float lin0 = r == 0 ? 1.0f : r == 1 ? 2.5f : 3.0f;
+	const float div = r == 2 ? 20.0f : r == 3 ? 80.0f : 18.0f;
+	const float fac = r == 2 ? 0.07f : r == 3 ? 0.125f : 4.0f;
+	const bool linear = r == 0 || r == 1 || r == 5;
+	bool fired = false;
+	const int inc_max = (int)(1 / DT);
+	for(int it = 0; it < inc_max; ++it) {
+		const float x = v + 65.0f;
+		const float arg = linear ? lin0 - 0.1f * x : -x / div;
+		const float e = expf_exact(arg);
+		const float num = r == 0 ? 0.1f - 0.01f * x : 2.5f - 0.1f * x;
+		const float rate = r <= 1 ? num / (e - 1.0f) : r == 5 ? 1.0f / (e + 1.0f) : fac * e;
+		const float alphan = __shfl(rate, 0, LANES);
+		const float alpham = __shfl(rate, 1, LANES);
+		const float alphah = __shfl(rate, 2, LANES);
+		const float betan = __shfl(rate, 3, LANES);
+		const float betam = __shfl(rate, 4, LANES);
+		const float betah = __shfl(rate, 5, LANES);
+		hh_substep(v, n, m, h, dir, fired, cycle, Excit, Inhib, Exter, alphan, alpham, alphah, betan, betam, betah);
+	}
+	if(q == 0) {
+		s[0 * P + nl] = v;
+		s[1 * P + nl] = n;
+		s[2 * P + nl] = m;
+		s[3 * P + nl] = h;
+		s[4 * P + nl] = dir;
+		g.scratch[SCR_FIRED * P + nl] = fired ? 1.0f : 0.0f;
+	}
+}
+
+/* Split step, launch 3: the tiles' firing record (and per-tile STDP). */
+__global__ void __launch_bounds__(NEMO_UPDATE_BLOCK)
+hh_record(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
+{
+	const unsigned nl = blockIdx.x * NEMO_UPDATE_BLOCK + threadIdx.x;
+	const bool valid = nl < g.count;
+	const unsigned nn = g.start + nl;
+	const uint64_t hist = nemo_dev::load_history(b, nn, valid);
+	bool fired = false;
+	if(valid) fired = g.scratch[SCR_FIRED * g.pitch + nl] != 0.0f || nemo_dev::stimulated(b, nn);
+	nemo_dev::record_firing(b, nn, valid, fired, hist);
+	nemo_dev::stdp_tile(b, nn, valid, hist);
 }
 
 } // namespace
@@ -88,16 +202,29 @@ extern "C" hipError_t
 hip_update_neurons(hipStream_t stream, const nemo_hip_neuron_group* g, const nemo_hip_step_buffers* b)
 {
 	if(g->count == 0) return hipSuccess;
+	const unsigned tiles = (g->count + NEMO_UPDATE_BLOCK - 1) / NEMO_UPDATE_BLOCK;
 	/* the group's tiles, then the propagate blocks the launch carries */
-	const unsigned blocks = (g->count + NEMO_UPDATE_BLOCK - 1) / NEMO_UPDATE_BLOCK + b->co.deliver_blocks
-		+ b->co.stdp_blocks;
-	hipExtLaunchKernelGGL(hh_update, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream, b->time_start,
-			b->time_stop, 0, *g, *b);
+	const unsigned blocks = tiles + b->co.deliver_blocks + b->co.stdp_blocks;
+	if(!g->scratch) {
+		hipExtLaunchKernelGGL(hh_update, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream,
+				b->time_start, b->time_stop, 0, *g, *b);
+		return hipGetLastError();
+	}
+	hipExtLaunchKernelGGL(hh_input, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream, b->time_start,
+			nullptr, 0, *g, *b);
+	hipError_t e = hipGetLastError();
+	if(e != hipSuccess) return e;
+	const unsigned sub = unsigned((size_t(g->count) * LANES + SUB_BLOCK - 1) / SUB_BLOCK);
+	hipLaunchKernelGGL(hh_substeps, dim3(sub), dim3(SUB_BLOCK), 0, stream, *g, *b);
+	e = hipGetLastError();
+	if(e != hipSuccess) return e;
+	hipExtLaunchKernelGGL(hh_record, dim3(tiles), dim3(NEMO_UPDATE_BLOCK), 0, stream, nullptr, b->time_stop, 0, *g,
+			*b);
 	return hipGetLastError();
 }
 
 extern "C" hipError_t
 hip_init_neurons(hipStream_t, const nemo_hip_neuron_group*, const nemo_hip_step_buffers*)
 {
-	return hipSuccess;
+	return hipSuccess; /* src/nemo/cpu/plugins/default_init.c:3-21: nothing to do */
 }
