@@ -305,7 +305,15 @@ __device__ inline void deliver_block(const nemo_propagate_args& a, uint64_t t, u
 			if(e[u].x >= 0) stage[pos[u]] = make_int2(int(uint32_t(e[u].x) % NEMO_TILE), e[u].y);
 		}
 		__syncthreads();
-		for(uint32_t i = tid; i < carry; i += THREADS) out[i] = stage[i];
+		/* two events per 16-byte store (the stage and every bin region start
+		 * 16-byte aligned), the odd last one alone; non-temporal, since the
+		 * update reads them on other XCDs */
+		typedef int v4i __attribute__((ext_vector_type(4)));
+		const uint32_t pairs = carry / 2;
+		const v4i* s4 = reinterpret_cast<const v4i*>(stage);
+		v4i* o4 = reinterpret_cast<v4i*>(out);
+		for(uint32_t i = tid; i < pairs; i += THREADS) __builtin_nontemporal_store(s4[i], &o4[i]);
+		if((carry & 1) && tid == 0) out[carry - 1] = stage[carry - 1];
 	} else {
 		/* four staged events per thread at a time, so that their LDS
 		 * atomics (which return the slot) overlap */
