@@ -61,10 +61,13 @@ __device__ inline void hh_substep(float& v, float& n, float& m, float& h, float&
 }
 
 /* Single launch: gather, substeps and firing record per tile. */
+/* CO: the launch may carry propagate blocks after the tiles (compiled
+ * out otherwise, which leaves the update far fewer registers) */
+template<bool CO>
 __global__ void __launch_bounds__(NEMO_UPDATE_BLOCK)
 hh_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 {
-	if(nemo_dev::colaunched_block(b, g)) return;
+	if(CO && nemo_dev::colaunched_block(b, g)) return;
 	__shared__ nemo_dev::TileInput input;
 	nemo_dev::gather_input(b, g, input);
 	const unsigned nl = blockIdx.x * NEMO_UPDATE_BLOCK + threadIdx.x;
@@ -107,10 +110,13 @@ hh_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 }
 
 /* Split step, launch 1: the tiles' synaptic and external input. */
+/* CO: the launch may carry propagate blocks after the tiles (compiled
+ * out otherwise, which leaves the update far fewer registers) */
+template<bool CO>
 __global__ void __launch_bounds__(NEMO_UPDATE_BLOCK)
 hh_input(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 {
-	if(nemo_dev::colaunched_block(b, g)) return;
+	if(CO && nemo_dev::colaunched_block(b, g)) return;
 	__shared__ nemo_dev::TileInput input;
 	nemo_dev::gather_input(b, g, input);
 	const unsigned nl = blockIdx.x * NEMO_UPDATE_BLOCK + threadIdx.x;
@@ -206,12 +212,22 @@ hip_update_neurons(hipStream_t stream, const nemo_hip_neuron_group* g, const nem
 	/* the group's tiles, then the propagate blocks the launch carries */
 	const unsigned blocks = tiles + b->co.deliver_blocks + b->co.stdp_blocks;
 	if(!g->scratch) {
-		hipExtLaunchKernelGGL(hh_update, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream,
-				b->time_start, b->time_stop, 0, *g, *b);
+		if(b->co.deliver_blocks || b->co.stdp_blocks) {
+			hipExtLaunchKernelGGL(hh_update<true>, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream,
+					b->time_start, b->time_stop, 0, *g, *b);
+		} else {
+			hipExtLaunchKernelGGL(hh_update<false>, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream,
+					b->time_start, b->time_stop, 0, *g, *b);
+		}
 		return hipGetLastError();
 	}
-	hipExtLaunchKernelGGL(hh_input, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream, b->time_start,
-			nullptr, 0, *g, *b);
+	if(b->co.deliver_blocks || b->co.stdp_blocks) {
+		hipExtLaunchKernelGGL(hh_input<true>, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream,
+				b->time_start, nullptr, 0, *g, *b);
+	} else {
+		hipExtLaunchKernelGGL(hh_input<false>, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream,
+				b->time_start, nullptr, 0, *g, *b);
+	}
 	hipError_t e = hipGetLastError();
 	if(e != hipSuccess) return e;
 	const unsigned sub = unsigned((size_t(g->count) * LANES + SUB_BLOCK - 1) / SUB_BLOCK);

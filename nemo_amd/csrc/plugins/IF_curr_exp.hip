@@ -22,10 +22,13 @@ enum { V_REST = 0, V_RESET = 1, C_M = 2, TAU_M = 3, TAU_REFRAC = 4, TAU_SYN_E = 
 	V_THRESH = 7, I_OFFSET = 8 };
 enum { S_V = 0, S_IE = 1, S_II = 2, S_LASTFIRED = 3 };
 
+/* CO: the launch may carry propagate blocks after the tiles (compiled
+ * out otherwise, which leaves the update far fewer registers) */
+template<bool CO>
 __global__ void __launch_bounds__(NEMO_UPDATE_BLOCK)
 if_curr_exp_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 {
-	if(nemo_dev::colaunched_block(b, g)) return;
+	if(CO && nemo_dev::colaunched_block(b, g)) return;
 	__shared__ nemo_dev::TileInput input;
 	nemo_dev::gather_input(b, g, input);
 	const unsigned nl = blockIdx.x * NEMO_UPDATE_BLOCK + threadIdx.x;
@@ -77,8 +80,13 @@ hip_update_neurons(hipStream_t stream, const nemo_hip_neuron_group* g, const nem
 	/* the group's tiles, then the propagate blocks the launch carries */
 	const unsigned blocks = (g->count + NEMO_UPDATE_BLOCK - 1) / NEMO_UPDATE_BLOCK + b->co.deliver_blocks
 		+ b->co.stdp_blocks;
-	hipExtLaunchKernelGGL(if_curr_exp_update, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream, b->time_start,
-			b->time_stop, 0, *g, *b);
+	if(b->co.deliver_blocks || b->co.stdp_blocks) {
+		hipExtLaunchKernelGGL(if_curr_exp_update<true>, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream,
+				b->time_start, b->time_stop, 0, *g, *b);
+	} else {
+		hipExtLaunchKernelGGL(if_curr_exp_update<false>, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream,
+				b->time_start, b->time_stop, 0, *g, *b);
+	}
 	return hipGetLastError();
 }
 

@@ -9,10 +9,13 @@
 
 namespace {
 
+/* CO: the launch may carry propagate blocks after the tiles (compiled
+ * out otherwise, which leaves the update far fewer registers) */
+template<bool CO>
 __global__ void __launch_bounds__(NEMO_UPDATE_BLOCK)
 input_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 {
-	if(nemo_dev::colaunched_block(b, g)) return;
+	if(CO && nemo_dev::colaunched_block(b, g)) return;
 	__shared__ nemo_dev::TileInput input;
 	nemo_dev::gather_input(b, g, input);
 	const unsigned nl = blockIdx.x * NEMO_UPDATE_BLOCK + threadIdx.x;
@@ -38,8 +41,13 @@ hip_update_neurons(hipStream_t stream, const nemo_hip_neuron_group* g, const nem
 	/* the group's tiles, then the propagate blocks the launch carries */
 	const unsigned blocks = (g->count + NEMO_UPDATE_BLOCK - 1) / NEMO_UPDATE_BLOCK + b->co.deliver_blocks
 		+ b->co.stdp_blocks;
-	hipExtLaunchKernelGGL(input_update, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream, b->time_start,
-			b->time_stop, 0, *g, *b);
+	if(b->co.deliver_blocks || b->co.stdp_blocks) {
+		hipExtLaunchKernelGGL(input_update<true>, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream,
+				b->time_start, b->time_stop, 0, *g, *b);
+	} else {
+		hipExtLaunchKernelGGL(input_update<false>, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream,
+				b->time_start, b->time_stop, 0, *g, *b);
+	}
 	return hipGetLastError();
 }
 
