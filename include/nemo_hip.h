@@ -186,6 +186,15 @@ nemo_status_t nemo_reset_timer(nemo_simulation_t);
 nemo_status_t nemo_hip_run(nemo_simulation_t, unsigned nsteps,
 		unsigned stdp_period, float reward,
 		unsigned counts[], unsigned* fired[], size_t* fired_len, uint64_t* events);
+/* Extension (no counterpart in the reference): nemo_hip_run with the raster
+ * read back and the fired neurons written straight into the caller's
+ * fired_buf (fired_cap entries; *fired_len of them used), as they are decoded
+ * beside the later steps, so that no copy follows the run.  Neurons beyond
+ * fired_cap continue in the simulation's own list, *spill (*spill_len
+ * entries), valid until the next call on the simulation. */
+nemo_status_t nemo_hip_run_into(nemo_simulation_t, unsigned nsteps, unsigned stdp_period, float reward,
+		unsigned counts[], unsigned fired_buf[], size_t fired_cap, size_t* fired_len, unsigned* spill[],
+		size_t* spill_len, uint64_t* events);
 
 /* Extension: a batch of independent networks (trials, parameter sweeps)
  * advanced together in the same kernel launches.  Each network keeps its own

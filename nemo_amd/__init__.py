@@ -286,16 +286,37 @@ class Simulation(_SynapseQueries):
         per-step fired counts, the concatenated fired neurons (each step in
         simulation order) and the synaptic events delivered."""
         counts = np.zeros(nsteps, dtype=np.uint32) if raster else None
-        out = ctypes.POINTER(ctypes.c_uint)()
-        n = ctypes.c_size_t(0)
+        fired, events = self._run(nsteps, stdp_period, reward, counts)
+        return counts, fired, events
+
+    def _run(self, nsteps, stdp_period, reward, counts):
+        """nemo_hip_run, or with counts (the raster read back into it)
+        nemo_hip_run_into: the fired neurons are decoded on the host beside
+        the later steps straight into an array sized from the previous run's
+        firing rate, and any beyond it are appended from the simulation's own
+        list.  Returns (fired or None, events)."""
         ev = ctypes.c_uint64(0)
-        check(lib().nemo_hip_run(self._h, nsteps, stdp_period, reward,
-                                 _ptr(counts, ctypes.c_uint) if raster else None,
-                                 ctypes.byref(out), ctypes.byref(n), ctypes.byref(ev)))
-        if not raster:
-            return None, None, ev.value
-        fired = np.ctypeslib.as_array(out, shape=(n.value,)).copy() if n.value else np.zeros(0, np.uint32)
-        return counts, fired, ev.value
+        if counts is None:
+            out = ctypes.POINTER(ctypes.c_uint)()
+            n = ctypes.c_size_t(0)
+            check(lib().nemo_hip_run(self._h, nsteps, stdp_period, reward, None, ctypes.byref(out),
+                                     ctypes.byref(n), ctypes.byref(ev)))
+            return None, ev.value
+        rate = getattr(self, "_fired_per_step", 0.0)
+        cap = int(1.25 * rate * nsteps) + 1024 if rate else 0
+        buf = np.empty(cap, np.uint32)
+        used = ctypes.c_size_t(0)
+        spill = ctypes.POINTER(ctypes.c_uint)()
+        nspill = ctypes.c_size_t(0)
+        check(lib().nemo_hip_run_into(self._h, nsteps, stdp_period, reward, _ptr(counts, ctypes.c_uint),
+                                      _ptr(buf, ctypes.c_uint), cap, ctypes.byref(used), ctypes.byref(spill),
+                                      ctypes.byref(nspill), ctypes.byref(ev)))
+        fired = buf[:used.value]
+        if nspill.value:
+            fired = np.concatenate([fired, np.ctypeslib.as_array(spill, shape=(nspill.value,))])
+        if nsteps:
+            self._fired_per_step = (used.value + nspill.value) / nsteps
+        return fired, ev.value
 
     def apply_stdp(self, reward):
         check(lib().nemo_apply_stdp(self._h, reward))
@@ -435,16 +456,8 @@ class Batch(Simulation):
         """As Simulation.run; counts has shape (nsteps, size) and fired lists,
         step by step, each network's fired neurons in turn."""
         counts = np.zeros((nsteps, self.size), dtype=np.uint32) if raster else None
-        out = ctypes.POINTER(ctypes.c_uint)()
-        n = ctypes.c_size_t(0)
-        ev = ctypes.c_uint64(0)
-        check(lib().nemo_hip_run(self._h, nsteps, stdp_period, reward,
-                                 _ptr(counts, ctypes.c_uint) if raster else None,
-                                 ctypes.byref(out), ctypes.byref(n), ctypes.byref(ev)))
-        if not raster:
-            return None, None, ev.value
-        fired = np.ctypeslib.as_array(out, shape=(n.value,)).copy() if n.value else np.zeros(0, np.uint32)
-        return counts, fired, ev.value
+        fired, events = self._run(nsteps, stdp_period, reward, counts)
+        return counts, fired, events
 
     @staticmethod
     def split(counts, fired):

@@ -82,6 +82,8 @@ _PROTOTYPES = [
     ("nemo_reset_timer", ctypes.c_int, [vp]),
     ("nemo_hip_run", ctypes.c_int, [vp, ctypes.c_uint, ctypes.c_uint, ctypes.c_float, c_uint_p,
                                     ctypes.POINTER(c_uint_p), c_size_p, c_u64_p]),
+    ("nemo_hip_run_into", ctypes.c_int, [vp, ctypes.c_uint, ctypes.c_uint, ctypes.c_float, c_uint_p, c_uint_p,
+                                         ctypes.c_size_t, c_size_p, ctypes.POINTER(c_uint_p), c_size_p, c_u64_p]),
     ("nemo_hip_new_batch", ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_size_t, vp, ctypes.POINTER(vp)]),
     ("nemo_hip_batch_size", ctypes.c_int, [vp, c_uint_p]),
     ("nemo_hip_batch_get_neuron_state", ctypes.c_int, [vp, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, c_float_p]),

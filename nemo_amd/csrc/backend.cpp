@@ -1529,7 +1529,8 @@ void Simulation::flushStdp()
 
 /* Neurons (user indices) set in the firing bitmap of one step, in simulation
  * order (cpu/Simulation.cpp:200-209). */
-void Simulation::decodeFiring(const uint64_t* bits, std::vector<unsigned>& out) const
+template<typename Out>
+void Simulation::decodeFiring(const uint64_t* bits, Out& out) const
 {
 	for(size_t w = 0; w < m_firedWords; ++w) {
 		uint64_t word = bits[w];
@@ -1544,8 +1545,8 @@ void Simulation::decodeFiring(const uint64_t* bits, std::vector<unsigned>& out) 
 /* Fired neurons of every network of a batch for one step: out gets each
  * network's user indices in turn, in that network's simulation order, and
  * counts one entry per network. */
-void Simulation::decodeFiringBatch(const uint64_t* bits, std::vector<unsigned>& out,
-		std::vector<unsigned>& counts) const
+template<typename Out>
+void Simulation::decodeFiringBatch(const uint64_t* bits, Out& out, std::vector<unsigned>& counts) const
 {
 	if(m_nets == 1) {
 		const size_t before = out.size();
@@ -1701,7 +1702,7 @@ uint64_t Simulation::sumCounters(const DeviceArray<unsigned long long>& a)
 /* Steps without stimulus, captured in graphs of up to 64 steps; the firing
  * bitmaps of each graph's steps are read back after it. */
 void Simulation::run(unsigned nsteps, unsigned stdpPeriod, float reward, bool readRaster,
-		std::vector<unsigned>* counts, std::vector<unsigned>* fired, uint64_t* events)
+		std::vector<unsigned>* counts, FiredSink* fired, uint64_t* events)
 {
 	activate();
 	if(stdpPeriod && !m_stdp) {
@@ -1715,7 +1716,7 @@ void Simulation::run(unsigned nsteps, unsigned stdpPeriod, float reward, bool re
 	 * reallocated (and copied) while the device waits for the host */
 	const size_t firedBefore = readRaster && fired ? fired->size() : 0;
 	if(readRaster && fired && counts) {
-		fired->reserve(firedBefore + size_t(1.25 * m_firedPerStep * nsteps) + 1024);
+		if(fired->cap == 0) fired->spill->reserve(fired->spill->size() + size_t(1.25 * m_firedPerStep * nsteps) + 1024);
 		counts->reserve(counts->size() + size_t(nsteps) * m_nets);
 	}
 	/* Chunks alternate between the two halves of the pinned bitmap buffer:

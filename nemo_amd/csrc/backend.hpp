@@ -180,11 +180,24 @@ public:
 	const std::vector<unsigned>& step(const unsigned* fstim, size_t nf,
 			const unsigned* istimIdx, const float* istimVal, size_t ni);
 	void applyStdp(float reward);
+	/* Where run() writes fired neurons: a caller's buffer while it has room,
+	 * then a vector. */
+	struct FiredSink {
+		unsigned* buf = nullptr;
+		size_t cap = 0, len = 0;
+		std::vector<unsigned>* spill = nullptr;
+		void push_back(unsigned v)
+		{
+			if(len < cap) buf[len++] = v;
+			else spill->push_back(v);
+		}
+		size_t size() const { return len + spill->size(); }
+	};
 	/* nsteps steps without stimulus; with readRaster, counts gets [step][network]
 	 * firing counts and fired the fired neurons (by step, then network, then
 	 * ascending simulation order) */
 	void run(unsigned nsteps, unsigned stdpPeriod, float reward, bool readRaster,
-			std::vector<unsigned>* counts, std::vector<unsigned>* fired, uint64_t* events);
+			std::vector<unsigned>* counts, FiredSink* fired, uint64_t* events);
 	/* capture the step graphs a run(nsteps, stdpPeriod) uses */
 	void prepare(unsigned nsteps, unsigned stdpPeriod);
 	void profile(unsigned nsteps, double avgUs[4], uint64_t counters[4]);
@@ -361,8 +374,8 @@ private:
 			hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 	void flushStdp();
 	void joinStdp();
-	void decodeFiring(const uint64_t* bits, std::vector<unsigned>& out) const;
-	void decodeFiringBatch(const uint64_t* bits, std::vector<unsigned>& out, std::vector<unsigned>& counts) const;
+	template<typename Out> void decodeFiring(const uint64_t* bits, Out& out) const;
+	template<typename Out> void decodeFiringBatch(const uint64_t* bits, Out& out, std::vector<unsigned>& counts) const;
 	void readBitmaps(uint64_t t0, unsigned k, uint64_t* dst = nullptr, hipStream_t st = nullptr);
 	uint64_t sumCounters(const DeviceArray<unsigned long long>& a);
 	hipGraphExec_t graphFor(unsigned k);

@@ -434,12 +434,37 @@ nemo_status_t nemo_hip_run(nemo_simulation_t s, unsigned nsteps, unsigned stdp_p
 		s->runCounts.clear();
 		s->runFired.clear();
 		const bool raster = counts != nullptr;
-		s->impl.run(nsteps, stdp_period, reward, raster, &s->runCounts, &s->runFired, events);
+		nemo_hip::Simulation::FiredSink sink;
+		sink.spill = &s->runFired;
+		s->impl.run(nsteps, stdp_period, reward, raster, &s->runCounts, &sink, events);
 		if(raster) {
 			std::memcpy(counts, s->runCounts.data(), s->runCounts.size() * sizeof(unsigned));
 			if(fired) *fired = s->runFired.data();
 			if(fired_len) *fired_len = s->runFired.size();
 		}
+	});
+}
+
+nemo_status_t nemo_hip_run_into(nemo_simulation_t s, unsigned nsteps, unsigned stdp_period, float reward,
+		unsigned counts[], unsigned fired_buf[], size_t fired_cap, size_t* fired_len, unsigned* spill[],
+		size_t* spill_len, uint64_t* events)
+{
+	return guarded([&] {
+		check(s, "simulation");
+		if(!counts || !fired_len || !spill || !spill_len || (fired_cap && !fired_buf)) {
+			throw nemo_hip::Error(NEMO_INVALID_INPUT, "null argument");
+		}
+		s->runCounts.clear();
+		s->runFired.clear();
+		nemo_hip::Simulation::FiredSink sink;
+		sink.buf = fired_buf;
+		sink.cap = fired_cap;
+		sink.spill = &s->runFired;
+		s->impl.run(nsteps, stdp_period, reward, true, &s->runCounts, &sink, events);
+		std::memcpy(counts, s->runCounts.data(), s->runCounts.size() * sizeof(unsigned));
+		*fired_len = sink.len;
+		*spill = s->runFired.data();
+		*spill_len = s->runFired.size();
 	});
 }
 

@@ -373,6 +373,34 @@ def test_batch_matches_solo_runs():
     assert np.array_equal(batch.fixed_weights(), np.concatenate(weights))
 
 
+def test_run_into_caller_buffer_and_spill():
+    """nemo_hip_run_into writes fired neurons into the caller's buffer and the
+    rest into the simulation's list: with a buffer of 7 entries (almost all
+    spill), with none, and through Simulation.run (sized from the previous
+    run), the fired lists and counts equal nemo_hip_run's."""
+    from nemo_amd._lib import check, lib
+    spec = gen.random_izhikevich(3000, 100, dmax=20, seed=17, stdp=True)
+    sims = [sc.make(nemo_amd, spec, _conf(nemo_amd, stdp_fn())) for _ in range(3)]
+    c_uint_p = ctypes.POINTER(ctypes.c_uint)
+    for nsteps in (100, 150, 70):
+        ref_counts, ref_fired, ref_ev = sims[0].run(nsteps, stdp_period=100)
+        for sim, cap in ((sims[1], 7), (sims[2], 0)):
+            counts = np.zeros(nsteps, np.uint32)
+            buf = np.zeros(max(cap, 1), np.uint32)
+            used, nspill, ev = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_uint64()
+            spill = c_uint_p()
+            check(lib().nemo_hip_run_into(sim._h, nsteps, 100, 1.0, counts.ctypes.data_as(c_uint_p),
+                                          buf.ctypes.data_as(c_uint_p), cap, ctypes.byref(used),
+                                          ctypes.byref(spill), ctypes.byref(nspill), ctypes.byref(ev)))
+            rest = np.ctypeslib.as_array(spill, shape=(nspill.value,)) if nspill.value else np.zeros(0, np.uint32)
+            fired = np.concatenate([buf[:used.value], rest])
+            assert used.value == min(cap, fired.size)
+            assert np.array_equal(counts, ref_counts), (nsteps, cap)
+            assert np.array_equal(fired, ref_fired), (nsteps, cap)
+            assert ev.value == ref_ev
+        assert ref_counts.sum() > 0
+
+
 def test_single_type_batch_matches_solo_runs():
     """A batch of one neuron type lists each network's neurons in turn on the
     device, and its fired lists are decoded in one pass: each network's raster
