@@ -40,8 +40,9 @@ constexpr unsigned PBLOCK = NEMO_PROPAGATE_BLOCK;
 
 using nemo_dev::stdp_wrap_add;
 
-/* delivery blocks, then (when fused into this launch) STDP blocks */
-__global__ void __launch_bounds__(PBLOCK)
+/* delivery blocks, then (when fused into this launch) STDP blocks; at least
+ * 6 waves per SIMD (80 VGPRs), so that three 512-thread blocks share a CU */
+__global__ void __launch_bounds__(PBLOCK, 6)
 propagate_kernel(nemo_propagate_args a)
 {
 	extern __shared__ uint64_t lds_dyn[];

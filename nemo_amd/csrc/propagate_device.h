@@ -23,7 +23,10 @@ constexpr uint64_t PROPAGATE_RING_MASK = NEMO_STEP_RING - 1;
 /* independent global loads each thread keeps in flight in the synapse loop
  * of STDP, and in the event loop of delivery */
 constexpr int ILP = 8;
-constexpr int EVENT_ILP = 16;
+/* 8, not 16: with the launch held to 80 VGPRs a CU runs three 512-thread
+ * delivery blocks at once instead of two, which pays when there are more
+ * blocks than CUs (batches) and costs nothing with one block per CU */
+constexpr int EVENT_ILP = 8;
 /* staged events per thread the delivery sort keeps in registers */
 constexpr unsigned SORT_REGS = 8;
 
