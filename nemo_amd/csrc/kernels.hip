@@ -104,22 +104,35 @@ __global__ void stdp_snapshot_done_kernel(nemo_hip_clock* clock, uint64_t offset
 __global__ void __launch_bounds__(NEMO_BLOCK)
 apply_stdp_kernel(nemo_apply_args a)
 {
-	for(size_t j = size_t(blockIdx.x) * NEMO_BLOCK + threadIdx.x; j < a.fcm_count;
-			j += size_t(gridDim.x) * NEMO_BLOCK) {
-		const uint32_t k = a.rcm_of[j];
-		if(k == ~0u) continue;
-		const int32_t acc = a.reward == 0 ? 0 : a.rcm_acc[k];
-		const int32_t w_old = a.fcm[j].y;
-		const int32_t dw = int32_t((int64_t(a.reward) * int64_t(acc)) >> a.fbits);
-		int32_t w_new = 0;
-		if(w_old > 0) {
-			const int32_t x = stdp_wrap_add(w_old, dw);
-			w_new = min(a.maxE, max(x, a.minE));
-		} else if(w_old < 0) {
-			const int32_t x = stdp_wrap_add(w_old, dw);
-			w_new = min(a.minI, max(x, a.maxI));
+	/* APPLY_ILP synapses per thread at a time: their index and weight loads,
+	 * then their accumulator gathers, are in flight together */
+	constexpr unsigned APPLY_ILP = 4;
+	const size_t stride = size_t(gridDim.x) * NEMO_BLOCK * APPLY_ILP;
+	for(size_t j0 = size_t(blockIdx.x) * NEMO_BLOCK * APPLY_ILP + threadIdx.x; j0 < a.fcm_count; j0 += stride) {
+		uint32_t k[APPLY_ILP];
+		int32_t w_old[APPLY_ILP], acc[APPLY_ILP];
+#pragma unroll
+		for(unsigned u = 0; u < APPLY_ILP; ++u) {
+			const size_t j = j0 + u * NEMO_BLOCK;
+			k[u] = j < a.fcm_count ? a.rcm_of[j] : ~0u;
+			w_old[u] = j < a.fcm_count ? a.fcm[j].y : 0;
 		}
-		if(w_new != w_old) a.fcm[j].y = w_new;
+#pragma unroll
+		for(unsigned u = 0; u < APPLY_ILP; ++u) acc[u] = k[u] != ~0u && a.reward != 0 ? a.rcm_acc[k[u]] : 0;
+#pragma unroll
+		for(unsigned u = 0; u < APPLY_ILP; ++u) {
+			if(k[u] == ~0u) continue;
+			const int32_t dw = int32_t((int64_t(a.reward) * int64_t(acc[u])) >> a.fbits);
+			int32_t w_new = 0;
+			if(w_old[u] > 0) {
+				const int32_t x = stdp_wrap_add(w_old[u], dw);
+				w_new = min(a.maxE, max(x, a.minE));
+			} else if(w_old[u] < 0) {
+				const int32_t x = stdp_wrap_add(w_old[u], dw);
+				w_new = min(a.minI, max(x, a.maxI));
+			}
+			if(w_new != w_old[u]) a.fcm[j0 + u * NEMO_BLOCK].y = w_new;
+		}
 	}
 }
 
@@ -253,7 +266,7 @@ hipError_t nemo_launch_propagate(hipStream_t st, const nemo_propagate_args* a, u
 hipError_t nemo_launch_apply_stdp(hipStream_t st, const nemo_apply_args* a)
 {
 	if(a->count == 0) return hipSuccess;
-	size_t blocks = (a->fcm_count + NEMO_BLOCK - 1) / NEMO_BLOCK;
+	size_t blocks = (a->fcm_count + 4 * NEMO_BLOCK - 1) / (4 * NEMO_BLOCK);
 	if(blocks > 16384) blocks = 16384;
 	hipLaunchKernelGGL(apply_stdp_kernel, dim3(unsigned(blocks)), dim3(NEMO_BLOCK), 0, st, *a);
 	hipError_t e = hipGetLastError();
