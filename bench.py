@@ -98,8 +98,9 @@ def batched(api, gen, count, steps, autotune_steps, warmup=200):
     1k..50k, U{100..1000} synapses per neuron, STDP on every other one) in one
     simulation, against the same networks run one after another as plain
     simulations.  Both run `warmup` untimed steps after construction (the
-    first steps of a new simulation run about 10% slower), then `steps` timed
-    steps, firing read back every step, graphs captured before the timed
+    first steps of a new simulation run about 10% slower; the warm-up reads
+    the firing back too, which sizes the fired-list buffer of the timed run),
+    then `steps` timed steps, firing read back every step, graphs captured before the timed
     region; each network's per-step firing counts are compared with its solo
     run."""
     t = time.perf_counter()
@@ -120,7 +121,7 @@ def batched(api, gen, count, steps, autotune_steps, warmup=200):
     default_shape = batch.launch_shape
     shape = batch.autotune(autotune_steps) if autotune_steps else default_shape
     if warmup:
-        batch.run(warmup, stdp_period=1000, reward=1.0, raster=False)
+        batch.run(warmup, stdp_period=1000, reward=1.0)
     batch.prepare(steps, stdp_period=1000)
     t0 = time.perf_counter()
     bcounts, _, bevents = batch.run(steps, stdp_period=1000, reward=1.0)
@@ -138,7 +139,7 @@ def batched(api, gen, count, steps, autotune_steps, warmup=200):
         stdp = k % 2 == 0
         sim = api.Simulation(net, stdp_conf if stdp else plain_conf)
         if warmup:
-            sim.run(warmup, stdp_period=1000 if stdp else 0, reward=1.0, raster=False)
+            sim.run(warmup, stdp_period=1000 if stdp else 0, reward=1.0)
         sim.prepare(steps, stdp_period=1000 if stdp else 0)
         t0 = time.perf_counter()
         c, _, e = sim.run(steps, stdp_period=1000 if stdp else 0, reward=1.0)
@@ -180,7 +181,7 @@ def mixed(api, gen, steps=2000, warmup=200, autotune_steps=128, profile_steps=30
     print("mixed torus: %d neurons, %d synapses, built in %.1f s" % (nneu, nsyn, time.perf_counter() - t),
           file=sys.stderr, flush=True)
     if warmup:
-        sim.run(warmup, raster=False)
+        sim.run(warmup)
     default_shape = sim.launch_shape
     shape = sim.autotune(autotune_steps) if autotune_steps else default_shape
     sim.prepare(steps)
