@@ -9,13 +9,13 @@
  *
  * A step is three launches when the group has its four scratch planes
  * (HH.ini [hip] scratch-planes=4): the tiles' input gather (one thread per
- * neuron, as every model), the substeps with eight lanes per neuron in
+ * neuron, as every model), the substeps with sixteen lanes per neuron in
  * one-wave blocks -- lane q evaluates the q-th of the six rate functions
  * (each an expf), the lanes exchange them and all carry the same state -- and
  * the tiles' firing record.  One wave of a tile block otherwise spends each
  * substep on six exponentials for its 64 neurons, one wave per SIMD on a few
  * CUs; the split spreads the same arithmetic, operation for operation, over
- * eight times as many waves.  Without scratch planes the single launch runs.
+ * sixteen times as many waves.  Without scratch planes the single launch runs.
  */
 #include <hip/hip_ext.h>
 
@@ -39,8 +39,15 @@ constexpr float REV_I = -70.0f;
 
 /* scratch planes of the split step */
 enum { SCR_E = 0, SCR_I = 1, SCR_X = 2, SCR_FIRED = 3 };
-constexpr unsigned LANES = 8;      /* per neuron in the substep launch */
+constexpr unsigned LANES = 16;     /* per neuron in the substep launch: one DPP row */
 constexpr unsigned SUB_BLOCK = 64; /* one wave: eight neurons */
+
+/* The value of lane Q of this lane's row of 16 (DPP row_share). */
+template<int Q>
+__device__ inline float row_lane(float v)
+{
+	return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x150 + Q, 0xf, 0xf, false));
+}
 
 /* The gating and membrane update of one substep from the six rates (HH.cpp:44-62). */
 __device__ inline void hh_substep(float& v, float& n, float& m, float& h, float& dir, bool& fired, uint64_t cycle,
@@ -132,8 +139,9 @@ hh_input(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 }
 
 /* Split step, launch 2: the substeps, LANES lanes per neuron.  Lane q < 6
- * evaluates rate q (alphan, alpham, alphah, betan, betam, betah; lanes 6 and
- * 7 repeat betah) from exactly the expression of the single launch; every
+ * evaluates rate q (alphan, alpham, alphah, betan, betam, betah; lanes 6 to
+ * 15 repeat betah) from exactly the expression of the single launch, and a
+ * DPP row broadcast hands each rate to the whole row (the neuron); every
  * lane then applies the same update, so all lanes of a neuron hold the same
  * state and the first writes it. */
 __global__ void __launch_bounds__(SUB_BLOCK)
@@ -170,12 +178,12 @@ hh_substeps(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 		const float e = expf_exact(arg);
 		const float num = r == 0 ? 0.1f - 0.01f * x : 2.5f - 0.1f * x;
 		const float rate = r <= 1 ? num / (e - 1.0f) : r == 5 ? 1.0f / (e + 1.0f) : fac * e;
-		const float alphan = __shfl(rate, 0, LANES);
-		const float alpham = __shfl(rate, 1, LANES);
-		const float alphah = __shfl(rate, 2, LANES);
-		const float betan = __shfl(rate, 3, LANES);
-		const float betam = __shfl(rate, 4, LANES);
-		const float betah = __shfl(rate, 5, LANES);
+		const float alphan = row_lane<0>(rate);
+		const float alpham = row_lane<1>(rate);
+		const float alphah = row_lane<2>(rate);
+		const float betan = row_lane<3>(rate);
+		const float betam = row_lane<4>(rate);
+		const float betah = row_lane<5>(rate);
 		hh_substep(v, n, m, h, dir, fired, cycle, Excit, Inhib, Exter, alphan, alpham, alphah, betan, betam, betah);
 	}
 	if(q == 0) {
