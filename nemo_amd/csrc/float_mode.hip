@@ -62,11 +62,37 @@ constexpr double H = 0.25;
 /* synapses in flight per lane */
 template<unsigned L> constexpr unsigned unroll_for() { return L >= 16 ? 256 / L : 16; }
 
+/* x of the partner lane under a DPP control (one 32-bit half at a time):
+ * quad_perm [1,0,3,2] pairs lanes 4k+0/1 and 4k+2/3, [2,3,0,1] pairs 4k+0/2
+ * and 4k+1/3, row_half_mirror pairs i and 7-i in each 8, row_mirror pairs i
+ * and 15-i in each 16. */
+template<int CTRL>
+__device__ inline double dpp_partner(double x)
+{
+	const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, 0xf, 0xf, false);
+	const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, 0xf, 0xf, false);
+	return __hiloint2double(hi, lo);
+}
+constexpr int DPP_QUAD_1032 = 0xb1;
+constexpr int DPP_QUAD_2301 = 0x4e;
+constexpr int DPP_ROW_HALF_MIRROR = 0x141;
+constexpr int DPP_ROW_MIRROR = 0x140;
+
+/* Sum over groups of L consecutive lanes (L a power of two up to 64), the
+ * same fixed pairing in every call: within quads, eights and rows of 16 by
+ * DPP, across rows by shuffles.  The first lane of a group (the one that
+ * reads it) always adds in the same order, so a sum is reproducible bit for
+ * bit. */
 template<unsigned L>
 __device__ inline double group_sum(double x)
 {
-#pragma unroll
-	for(int o = L / 2; o > 0; o >>= 1) x += __shfl_xor(x, o);
+	static_assert(L >= 1 && L <= 64 && (L & (L - 1)) == 0, "L: a power of two up to 64");
+	if(L > 1) x += dpp_partner<DPP_QUAD_1032>(x);
+	if(L > 2) x += dpp_partner<DPP_QUAD_2301>(x);
+	if(L > 4) x += dpp_partner<DPP_ROW_HALF_MIRROR>(x);
+	if(L > 8) x += dpp_partner<DPP_ROW_MIRROR>(x);
+	if(L > 16) x += __shfl_xor(x, 16);
+	if(L > 32) x += __shfl_xor(x, 32);
 	return x;
 }
 
