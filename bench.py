@@ -35,20 +35,45 @@ BYTES_PER_IZH_NEURON = 84      # 20 B params + 16 B u,v r/w + 32 B RNG r/w + 16 
 BYTES_PER_STDP_SYNAPSE = 20    # 4 B packed source/delay + 8 B presynaptic history + 8 B accumulator r/w
 
 
-def measured_traffic(kernel):
+def measured_traffic(kernel, shape):
     """HBM bytes per launch of `kernel` from the committed rocprof counter
     summary of this workload (FETCH_SIZE and WRITE_SIZE passes scaled by the
-    copy-kernel calibration), or None if there is none."""
+    copy-kernel calibration), when it was measured at launch shape `shape`;
+    None otherwise (the traffic of a launch depends on its shape)."""
     path = os.path.join(ROOT, "profiles", "propagate_update_izh100k_stdp.md")
     names = {"propagate": "propagate_kernel", "update": "izhikevich_update"}
     try:
+        measured_shape = None
+        traffic = None
         for line in open(path):
+            if line.startswith("traffic_launch_shape:"):
+                measured_shape = [int(x) for x in line.split(":", 1)[1].split()]
             if line.startswith("traffic_bytes_per_launch:"):
                 fields = dict(f.split("=") for f in line.split(":", 1)[1].split())
-                return float(fields[names[kernel]])
+                traffic = float(fields[names[kernel]])
+        if measured_shape == list(shape):
+            return traffic
     except (OSError, KeyError, ValueError):
         pass
     return None
+
+
+def host_description():
+    """CPU model and C library of this host (the CPU baseline's libm is part
+    of the numerics)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        libc = os.confstr("CS_GNU_LIBC_VERSION")
+    except (ValueError, OSError):
+        libc = "unknown"
+    return model, libc
 
 
 def stdp_function():
@@ -66,30 +91,43 @@ def build_network(api, spec, stdp):
     return api.Simulation(net, conf)
 
 
-def cpu_baseline(spec, seconds, max_steps):
-    """The CPU oracle (restated reference CPU backend) on the same network, on
-    this host's cores, over a bounded sample of steps from construction."""
+def cpu_baseline(spec, warmup, steps, seconds, period=1000):
+    """The CPU oracle (restated reference CPU backend) on the same network and
+    the same step window as the GPU's timed run: `warmup` untimed steps from
+    construction, then up to `steps` timed steps (fewer if they would take
+    more than `seconds`), applyStdp(1.0) every `period` steps of each of the
+    two runs as in the GPU's, on this host's cores."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_api as oa
     sim = build_network(oa, spec, True)
+    t_w = time.perf_counter()
+    for s in range(warmup):
+        sim.step()
+        if (s + 1) % period == 0:
+            sim.apply_stdp(1.0)
+    t_w = time.perf_counter() - t_w
     ev0 = sim.delivered_events()
     t0 = time.perf_counter()
-    steps = 0
-    while steps < max_steps and time.perf_counter() - t0 < seconds:
+    done = 0
+    while done < steps and time.perf_counter() - t0 < seconds:
         sim.step()
-        steps += 1
-        if steps % 1000 == 0:
+        done += 1
+        if done % period == 0:
             sim.apply_stdp(1.0)
     wall = time.perf_counter() - t0
     events = sim.delivered_events() - ev0
+    model, libc = host_description()
     return {
         "value": events / wall,
         "unit": "events/s",
         "cores": int(oa.lib().oracle_num_threads()),
         "kind": "port",
-        "sample": "first %d steps of the same network after construction (%.1f s wall, %.0f ms/step)"
-                  % (steps, wall, 1000.0 * wall / max(1, steps)),
-        "ms_per_step": 1000.0 * wall / max(1, steps),
+        "sample": "steps %d..%d of the same network (the GPU's timed window is steps %d..%d), %d untimed "
+                  "warm-up steps first (%.1f s); %.1f s wall, %.2f ms/step"
+                  % (warmup, warmup + done, warmup, warmup + steps, warmup, t_w, wall, 1000.0 * wall / max(1, done)),
+        "ms_per_step": 1000.0 * wall / max(1, done),
+        "cpu_model": model,
+        "libc": libc,
     }
 
 
@@ -287,15 +325,25 @@ def main():
               "multi-GPU path", file=sys.stderr)
         sys.exit(2)
 
-    import ctypes
     import nemo_amd
     from nemo_amd import generators as gen
-    from nemo_amd._lib import check, lib
 
     if args.copy_probe:
-        gbps = ctypes.c_double()
-        check(lib().nemo_hip_copy_probe(args.copy_probe, 10, ctypes.byref(gbps)))
-        print(json.dumps({"copy_probe_bytes": args.copy_probe, "launches": 11, "achieved_GBps": gbps.value}))
+        # counter calibration: torch's device-to-device copy of a known size
+        import torch
+        n = args.copy_probe // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(10):
+            dst.copy_(src)
+        e1.record()
+        torch.cuda.synchronize()
+        gbps = 2.0 * 4 * n * 10 / (e0.elapsed_time(e1) * 1e-3) / 1e9
+        print(json.dumps({"copy_probe_bytes": 4 * n, "launches": 11, "achieved_GBps": gbps}))
         return
 
     if args.only == "batched":
@@ -374,7 +422,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(dominant),
+            "traffic": measured_traffic(dominant, shape),
             "algorithmic_bytes_per_launch": per_launch[dominant],
         },
         "step_roofline_frac": (step_bytes / (wall / args.steps)) / 1e9 / HBM_PEAK_GBS,
@@ -390,8 +438,8 @@ def main():
         print("mixed done", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         print("cpu baseline ...", file=sys.stderr, flush=True)
-        cb = cpu_baseline(spec, args.cpu_seconds, 100000)
-        result["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        cb = cpu_baseline(spec, args.warmup, args.steps, args.cpu_seconds)
+        result["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample", "cpu_model", "libc")}
         result["speedup_vs_cpu"] = result["value"] / cb["value"]
     print(json.dumps(result), flush=True)
 

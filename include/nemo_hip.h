@@ -131,6 +131,10 @@ nemo_status_t nemo_get_neuron_state_n(nemo_network_t, unsigned neuron, unsigned 
 nemo_status_t nemo_get_neuron_parameter_n(nemo_network_t, unsigned neuron, unsigned param, float* val);
 nemo_status_t nemo_set_neuron_n(nemo_network_t, unsigned idx, unsigned nargs, float args[]);
 nemo_status_t nemo_set_neuron_state_n(nemo_network_t, unsigned neuron, unsigned var, float val);
+/* src/nemo.h:640-643 (deprecated in the reference; kept for drop-in):
+ * Izhikevich neuron, a, b, c, d, u, v, sigma */
+nemo_status_t nemo_set_neuron_iz_n(nemo_network_t, unsigned idx,
+		float a, float b, float c, float d, float u, float v, float sigma);
 nemo_status_t nemo_set_neuron_parameter_n(nemo_network_t, unsigned neuron, unsigned param, float val);
 /* src/nemo.h:467-567 (network synapse getters) */
 nemo_status_t nemo_get_synapse_source_n(nemo_network_t, synapse_id, unsigned* source);
@@ -160,6 +164,9 @@ nemo_status_t nemo_get_neuron_parameter_s(nemo_simulation_t, unsigned neuron, un
 /* src/nemo.h:625-732 (simulation setters) */
 nemo_status_t nemo_set_neuron_s(nemo_simulation_t, unsigned idx, unsigned nargs, float args[]);
 nemo_status_t nemo_set_neuron_state_s(nemo_simulation_t, unsigned neuron, unsigned var, float val);
+/* src/nemo.h:655-658 (deprecated in the reference; kept for drop-in) */
+nemo_status_t nemo_set_neuron_iz_s(nemo_simulation_t, unsigned idx,
+		float a, float b, float c, float d, float u, float v, float sigma);
 nemo_status_t nemo_set_neuron_parameter_s(nemo_simulation_t, unsigned neuron, unsigned param, float val);
 /* src/nemo.h:478-581 (simulation synapse getters) */
 nemo_status_t nemo_get_synapse_source_s(nemo_simulation_t, synapse_id, unsigned* source);
@@ -300,6 +307,11 @@ nemo_status_t nemo_hip_local_to_global(nemo_simulation_t, unsigned gidx[], size_
 nemo_status_t nemo_hip_get_recent_firing(nemo_simulation_t, uint64_t hist[], size_t n);
 /* Extension (parity inspection): RNG state, 4 words per neuron, simulation order */
 nemo_status_t nemo_hip_get_rng_state(nemo_simulation_t, uint32_t state[], size_t n);
+/* Extension (parity inspection): state variable var of every neuron in
+ * simulation order (NaN for neurons whose model has fewer state variables);
+ * one copy instead of one nemo_get_neuron_state_s call per neuron
+ * (src/nemo.h:437-446). */
+nemo_status_t nemo_hip_get_neuron_states(nemo_simulation_t, unsigned var, float out[], size_t n);
 /* Extension (parity inspection): Q11.20 weight and STDP accumulator of every
  * synapse in network order (source ascending, then insertion order) */
 nemo_status_t nemo_hip_get_fixed_weights(nemo_simulation_t, int32_t w[], size_t n);
@@ -314,13 +326,6 @@ nemo_status_t nemo_hip_counts(nemo_simulation_t, size_t* neurons, size_t* synaps
  * delivered, neurons fired, plastic synapses examined and steps run.  The
  * reference's Timer (src/nemo/Timer.hpp:23-79) only measures whole steps. */
 nemo_status_t nemo_hip_profile(nemo_simulation_t, unsigned nsteps, double avg_us[4], uint64_t counters[4]);
-
-/* Extension (measurement): copy `bytes` (rounded down to 16) between two
- * device buffers reps times with a plain streaming kernel; *gbps receives the
- * mean achieved bandwidth (read + write bytes per second / 1e9).  Used to
- * calibrate hardware byte counters and as the achievable-bandwidth
- * reference; no counterpart in the reference. */
-nemo_status_t nemo_hip_copy_probe(size_t bytes, unsigned reps, double* gbps);
 
 /* Extension (diagnostics): evaluate the device's glibc-exact logf (fn 0),
  * sinf (fn 1) or expf (fn 2) on n host inputs, results to host out.  The

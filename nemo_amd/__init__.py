@@ -427,6 +427,13 @@ class Simulation(_SynapseQueries):
         check(lib().nemo_hip_get_rng_state(self._h, _ptr(s, ctypes.c_uint32), self.neuron_count))
         return s.reshape(-1, 4)
 
+    def neuron_states(self, var):
+        """State variable var of every neuron, in simulation order (NaN where
+        the neuron's model has fewer state variables)."""
+        out = np.zeros(self.neuron_count, np.float32)
+        check(lib().nemo_hip_get_neuron_states(self._h, var, _ptr(out, ctypes.c_float), out.size))
+        return out
+
     def fixed_weights(self):
         w = np.zeros(self.synapse_count, np.int32)
         check(lib().nemo_hip_get_fixed_weights(self._h, _ptr(w, ctypes.c_int32), w.size))

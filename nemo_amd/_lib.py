@@ -53,6 +53,7 @@ _PROTOTYPES = [
     ("nemo_get_neuron_parameter_n", ctypes.c_int, [vp, ctypes.c_uint, ctypes.c_uint, c_float_p]),
     ("nemo_set_neuron_n", ctypes.c_int, [vp, ctypes.c_uint, ctypes.c_uint, c_float_p]),
     ("nemo_set_neuron_state_n", ctypes.c_int, [vp, ctypes.c_uint, ctypes.c_uint, ctypes.c_float]),
+    ("nemo_set_neuron_iz_n", ctypes.c_int, [vp, ctypes.c_uint] + [ctypes.c_float] * 7),
     ("nemo_set_neuron_parameter_n", ctypes.c_int, [vp, ctypes.c_uint, ctypes.c_uint, ctypes.c_float]),
     ("nemo_get_synapse_source_n", ctypes.c_int, [vp, ctypes.c_uint64, c_uint_p]),
     ("nemo_get_synapse_target_n", ctypes.c_int, [vp, ctypes.c_uint64, c_uint_p]),
@@ -70,6 +71,7 @@ _PROTOTYPES = [
     ("nemo_get_neuron_parameter_s", ctypes.c_int, [vp, ctypes.c_uint, ctypes.c_uint, c_float_p]),
     ("nemo_set_neuron_s", ctypes.c_int, [vp, ctypes.c_uint, ctypes.c_uint, c_float_p]),
     ("nemo_set_neuron_state_s", ctypes.c_int, [vp, ctypes.c_uint, ctypes.c_uint, ctypes.c_float]),
+    ("nemo_set_neuron_iz_s", ctypes.c_int, [vp, ctypes.c_uint] + [ctypes.c_float] * 7),
     ("nemo_set_neuron_parameter_s", ctypes.c_int, [vp, ctypes.c_uint, ctypes.c_uint, ctypes.c_float]),
     ("nemo_get_synapse_source_s", ctypes.c_int, [vp, ctypes.c_uint64, c_uint_p]),
     ("nemo_get_synapse_target_s", ctypes.c_int, [vp, ctypes.c_uint64, c_uint_p]),
@@ -104,11 +106,11 @@ _PROTOTYPES = [
     ("nemo_hip_local_to_global", ctypes.c_int, [vp, c_uint_p, ctypes.c_size_t]),
     ("nemo_hip_get_recent_firing", ctypes.c_int, [vp, c_u64_p, ctypes.c_size_t]),
     ("nemo_hip_get_rng_state", ctypes.c_int, [vp, c_u32_p, ctypes.c_size_t]),
+    ("nemo_hip_get_neuron_states", ctypes.c_int, [vp, ctypes.c_uint, c_float_p, ctypes.c_size_t]),
     ("nemo_hip_get_fixed_weights", ctypes.c_int, [vp, c_i32_p, ctypes.c_size_t]),
     ("nemo_hip_get_stdp_accumulators", ctypes.c_int, [vp, c_i32_p, ctypes.c_size_t]),
     ("nemo_hip_counts", ctypes.c_int, [vp, c_size_p, c_size_p, c_size_p]),
     ("nemo_hip_profile", ctypes.c_int, [vp, ctypes.c_uint, ctypes.POINTER(ctypes.c_double), c_u64_p]),
-    ("nemo_hip_copy_probe", ctypes.c_int, [ctypes.c_size_t, ctypes.c_uint, ctypes.POINTER(ctypes.c_double)]),
     ("nemo_hip_eval_math", ctypes.c_int, [ctypes.c_int, c_float_p, c_float_p, ctypes.c_size_t]),
     ("hip_simulation", vp, [vp, vp]),
     ("hip_set_configuration", ctypes.c_int, [vp, ctypes.c_int]),

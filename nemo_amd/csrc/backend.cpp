@@ -1305,11 +1305,15 @@ nemo_hip_step_buffers Simulation::stepBuffers(uint64_t offset, bool ext, bool fs
 }
 
 /* Copy a host scalar to the device on the stream.  The source is a fresh
- * pinned slot of a ring, so it stays untouched until the stream has passed
- * the copy (the stream is synchronised at least once per SCALAR_SLOTS copies:
- * every step, run chunk and flush synchronises). */
+ * pinned slot of a ring; before the ring wraps, the stream is synchronised,
+ * so every copy queued from a slot has run before the slot is written again
+ * (copy j reuses the slot of copy j - SCALAR_SLOTS, which precedes that
+ * synchronisation).  Never called while the stream is being captured. */
 void Simulation::copyScalar(uint64_t* dst, uint64_t value)
 {
+	if(m_scalarNext % SCALAR_SLOTS == 0 && m_scalarNext) {
+		HIP_CHECK(hipStreamSynchronize(m_stream));
+	}
 	uint64_t* slot = &m_pinnedScalars[m_scalarNext++ % SCALAR_SLOTS];
 	*slot = value;
 	HIP_CHECK(hipMemcpyAsync(dst, slot, sizeof(uint64_t), hipMemcpyHostToDevice, m_stream));
@@ -1505,16 +1509,22 @@ void Simulation::joinStdp()
  * next propagate launch to skip it. */
 void Simulation::flushStdp()
 {
+	if(enqueueFlush()) HIP_CHECK(hipStreamSynchronize(m_stream));
+}
+
+/* flushStdp's work queued on the stream without waiting for it; false if
+ * nothing was pending. */
+bool Simulation::enqueueFlush()
+{
 	joinStdp();
-	if(!m_stdpBlocks || m_hostStep == 0 || m_flushed == m_hostStep - 1) return;
+	if(!m_stdpBlocks || m_hostStep == 0 || m_flushed == m_hostStep - 1) return false;
 	if(m_shape.stdpMode == 4) {
 		/* the pending batch: every run of steps ends with a snapshot */
 		HIP_CHECK(nemo_launch_stdp_batch(m_stream, &m_propagate, m_stdpBlocks));
 		copyScalar(&m_clock.ptr->stdp_count, 0);
 		copyScalar(&m_clock.ptr->stdp_flushed, m_hostStep - 1);
 		m_flushed = m_hostStep - 1;
-		HIP_CHECK(hipStreamSynchronize(m_stream));
-		return;
+		return true;
 	}
 	const uint64_t s = m_hostStep - 1;
 	nemo_propagate_args a = m_propagate;
@@ -1524,7 +1534,7 @@ void Simulation::flushStdp()
 	HIP_CHECK(nemo_launch_propagate(m_stream, &a, m_stdpBlocks, 1, nullptr, nullptr));
 	copyScalar(&m_clock.ptr->stdp_flushed, s);
 	m_flushed = s;
-	HIP_CHECK(hipStreamSynchronize(m_stream));
+	return true;
 }
 
 /* Neurons (user indices) set in the firing bitmap of one step, in simulation
@@ -1782,6 +1792,9 @@ void Simulation::run(unsigned nsteps, unsigned stdpPeriod, float reward, bool re
 				stepsSinceApply = 0;
 			}
 		}
+		/* a run returns with the STDP accumulation of all its steps done
+		 * (the last batch, or the last step's in the per-step placements) */
+		enqueueFlush();
 		decodePending();
 	} catch(...) {
 		(void) hipStreamSynchronize(m_stream);
@@ -1912,7 +1925,9 @@ void Simulation::restoreSnapshot(const Snapshot& s)
 	m_flushed = s.flushed;
 }
 
-/* Advance nsteps steps in graphs of up to 64 steps; seconds on the stream. */
+/* Advance nsteps steps in graphs of up to 64 steps, then their pending STDP
+ * accumulation (so that every STDP placement is timed with all its work);
+ * seconds on the stream. */
 double Simulation::timeSteps(unsigned nsteps)
 {
 	hipEvent_t e0, e1;
@@ -1928,6 +1943,7 @@ double Simulation::timeSteps(unsigned nsteps)
 		m_hostStep += k;
 		done += k;
 	}
+	enqueueFlush();
 	HIP_CHECK(hipEventRecord(e1, m_stream));
 	HIP_CHECK(hipStreamSynchronize(m_stream));
 	float ms = 0;
@@ -1948,15 +1964,19 @@ LaunchShape Simulation::autotune(unsigned stepsPerTrial)
 	Snapshot snap;
 	takeSnapshot(snap);
 	const LaunchShape start = m_shape;
+	/* The snapshot holds the STDP batch clock of the shape it was taken
+	 * under; a restore is therefore followed by the shape's own batch reset
+	 * (applyShape does one), from the restored step with nothing pending. */
 	auto trial = [&](const LaunchShape& c) -> double {
+		restoreSnapshot(snap);
 		try {
 			applyShape(c);
 		} catch(const Error&) {
 			return 1e30;
 		}
-		restoreSnapshot(snap);
 		timeSteps(std::min(64u, stepsPerTrial)); /* captures the graph, warms caches */
 		restoreSnapshot(snap);
+		resetStdpBatches();
 		return timeSteps(stepsPerTrial);
 	};
 	LaunchShape best = start;
@@ -2036,8 +2056,8 @@ LaunchShape Simulation::autotune(unsigned stepsPerTrial)
 			consider(c);
 		}
 	}
-	applyShape(best);
 	restoreSnapshot(snap);
+	applyShape(best);
 	if(m_logging) {
 		std::printf("nemo-hip: autotuned launch shape: %u + %u delivery blocks x %u events, STDP %s, %u words, "
 				"overlap %u (%.1f us/step)\n", best.deliverBlocks, best.lateBlocks, best.binCapacity,
@@ -2285,6 +2305,24 @@ void Simulation::getRngState(uint32_t* s, size_t n)
 			if(l >= g.start && l < g.start + g.count) {
 				std::copy(h.begin() + 4 * size_t(l - g.start), h.begin() + 4 * size_t(l - g.start + 1), s + 4 * i);
 			}
+		}
+	}
+}
+
+void Simulation::getNeuronStates(unsigned var, float* out, size_t n)
+{
+	if(n != m_ncount) throw Error(NEMO_INVALID_INPUT, "size mismatch");
+	activate();
+	HIP_CHECK(hipStreamSynchronize(m_stream));
+	std::fill(out, out + n, std::nanf(""));
+	for(Group& g : m_groups) {
+		if(var >= g.type.nState || g.count == 0) continue;
+		std::vector<float> h(g.count);
+		HIP_CHECK(hipMemcpy(h.data(), g.state.ptr + size_t(var) * g.count, g.count * sizeof(float),
+					hipMemcpyDeviceToHost));
+		for(size_t i = 0; i < n; ++i) {
+			const unsigned l = m_order[i];
+			if(l >= g.start && l < g.start + g.count) out[i] = h[l - g.start];
 		}
 	}
 }

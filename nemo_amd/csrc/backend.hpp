@@ -232,6 +232,7 @@ public:
 	void localToGlobal(unsigned* g, size_t n) const;
 	void getRecentFiring(uint64_t* h, size_t n);
 	void getRngState(uint32_t* s, size_t n);
+	void getNeuronStates(unsigned var, float* out, size_t n);
 	void getFixedWeights(int32_t* w, size_t n);
 	void getStdpAccumulators(int32_t* acc, size_t n);
 	size_t neuronCount() const { return m_ncount; }
@@ -373,6 +374,7 @@ private:
 	void launchDelivery(const nemo_propagate_args& set, uint64_t offset, hipStream_t st, bool withStdp,
 			hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 	void flushStdp();
+	bool enqueueFlush();
 	void joinStdp();
 	template<typename Out> void decodeFiring(const uint64_t* bits, Out& out) const;
 	template<typename Out> void decodeFiringBatch(const uint64_t* bits, Out& out, std::vector<unsigned>& counts) const;

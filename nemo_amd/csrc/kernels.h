@@ -80,9 +80,6 @@ hipError_t nemo_launch_fstim(hipStream_t, uint32_t* bits, const uint32_t* idx, s
 /* Current stimulus: ext[idx[i]] = val[i] (indices must be distinct). */
 hipError_t nemo_launch_istim(hipStream_t, float* ext, const uint32_t* idx, const float* val, size_t n);
 
-/* Plain device-to-device copy of bytes (a multiple of 16) (measurement). */
-hipError_t nemo_launch_copy(hipStream_t, const void* src, void* dst, size_t bytes);
-
 /* Device logf (fn 0), sinf (fn 1) or expf (fn 2) of n inputs (diagnostics). */
 hipError_t nemo_launch_eval_math(hipStream_t, int fn, const float* in, float* out, size_t n);
 

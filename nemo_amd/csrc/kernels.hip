@@ -163,15 +163,6 @@ __global__ void istim_kernel(float* ext, const uint32_t* idx, const float* val, 
 	if(i < n) ext[idx[i]] = val[i];
 }
 
-/* plain streaming copy, 16 bytes per thread per iteration */
-__global__ void __launch_bounds__(NEMO_BLOCK)
-copy_kernel(const int4* __restrict__ src, int4* __restrict__ dst, size_t n)
-{
-	for(size_t i = size_t(blockIdx.x) * NEMO_BLOCK + threadIdx.x; i < n; i += size_t(gridDim.x) * NEMO_BLOCK) {
-		dst[i] = src[i];
-	}
-}
-
 __global__ void eval_math_kernel(int fn, const float* in, float* out, size_t n)
 {
 	const size_t i = size_t(blockIdx.x) * NEMO_BLOCK + threadIdx.x;
@@ -298,17 +289,6 @@ hipError_t nemo_launch_istim(hipStream_t st, float* ext, const uint32_t* idx, co
 	if(n == 0) return hipSuccess;
 	hipLaunchKernelGGL(istim_kernel, dim3(unsigned((n + NEMO_BLOCK - 1) / NEMO_BLOCK)), dim3(NEMO_BLOCK), 0, st,
 			ext, idx, val, n);
-	return hipGetLastError();
-}
-
-hipError_t nemo_launch_copy(hipStream_t st, const void* src, void* dst, size_t bytes)
-{
-	const size_t n = bytes / sizeof(int4);
-	if(n == 0) return hipSuccess;
-	size_t blocks = (n + NEMO_BLOCK - 1) / NEMO_BLOCK;
-	if(blocks > 16384) blocks = 16384;
-	hipLaunchKernelGGL(copy_kernel, dim3(unsigned(blocks)), dim3(NEMO_BLOCK), 0, st,
-			static_cast<const int4*>(src), static_cast<int4*>(dst), n);
 	return hipGetLastError();
 }
 

@@ -226,6 +226,17 @@ nemo_status_t nemo_set_neuron_n(nemo_network_t n, unsigned idx, unsigned nargs, 
 	return guarded([&] { check(n, "network"); n->impl.setNeuron(idx, nargs, args); });
 }
 
+/* Network::setNeuron (src/nemo/Network.cpp:67-73): args {a,b,c,d,sigma,u,v} */
+nemo_status_t nemo_set_neuron_iz_n(nemo_network_t n, unsigned idx,
+		float a, float b, float c, float d, float u, float v, float sigma)
+{
+	return guarded([&] {
+		check(n, "network");
+		float args[7] = {a, b, c, d, sigma, u, v};
+		n->impl.setNeuron(idx, 7, args);
+	});
+}
+
 nemo_status_t nemo_set_neuron_state_n(nemo_network_t n, unsigned neuron, unsigned var, float val)
 {
 	return guarded([&] { check(n, "network"); n->impl.setNeuronState(neuron, var, val); });
@@ -360,6 +371,17 @@ nemo_status_t nemo_get_neuron_parameter_s(nemo_simulation_t s, unsigned neuron, 
 nemo_status_t nemo_set_neuron_s(nemo_simulation_t s, unsigned idx, unsigned nargs, float args[])
 {
 	return guarded([&] { check(s, "simulation"); s->impl.setNeuron(idx, nargs, args); });
+}
+
+/* Simulation::setNeuron (src/nemo/Simulation.cpp:21-27): args {a,b,c,d,sigma,u,v} */
+nemo_status_t nemo_set_neuron_iz_s(nemo_simulation_t s, unsigned idx,
+		float a, float b, float c, float d, float u, float v, float sigma)
+{
+	return guarded([&] {
+		check(s, "simulation");
+		const float args[7] = {a, b, c, d, sigma, u, v};
+		s->impl.setNeuron(idx, 7, args);
+	});
 }
 
 nemo_status_t nemo_set_neuron_state_s(nemo_simulation_t s, unsigned neuron, unsigned var, float val)
@@ -574,6 +596,11 @@ nemo_status_t nemo_hip_get_rng_state(nemo_simulation_t s, uint32_t state[], size
 	return guarded([&] { check(s, "simulation"); s->impl.getRngState(state, n); });
 }
 
+nemo_status_t nemo_hip_get_neuron_states(nemo_simulation_t s, unsigned var, float out[], size_t n)
+{
+	return guarded([&] { check(s, "simulation"); s->impl.getNeuronStates(var, out, n); });
+}
+
 nemo_status_t nemo_hip_get_fixed_weights(nemo_simulation_t s, int32_t w[], size_t n)
 {
 	return guarded([&] { check(s, "simulation"); s->impl.getFixedWeights(w, n); });
@@ -597,29 +624,6 @@ nemo_status_t nemo_hip_counts(nemo_simulation_t s, size_t* neurons, size_t* syna
 nemo_status_t nemo_hip_profile(nemo_simulation_t s, unsigned nsteps, double avg_us[4], uint64_t counters[4])
 {
 	return guarded([&] { check(s, "simulation"); s->impl.profile(nsteps, avg_us, counters); });
-}
-
-nemo_status_t nemo_hip_copy_probe(size_t bytes, unsigned reps, double* gbps)
-{
-	return guarded([&] {
-		bytes &= ~size_t(15);
-		if(bytes == 0 || reps == 0) throw Error(NEMO_INVALID_INPUT, "empty copy probe");
-		nemo_hip::DeviceArray<int4> src(bytes / 16), dst(bytes / 16);
-		HIP_CHECK(hipMemset(src.ptr, 1, bytes));
-		hipEvent_t e0, e1;
-		HIP_CHECK(hipEventCreate(&e0));
-		HIP_CHECK(hipEventCreate(&e1));
-		HIP_CHECK(nemo_launch_copy(nullptr, src.ptr, dst.ptr, bytes)); /* warm */
-		HIP_CHECK(hipEventRecord(e0, nullptr));
-		for(unsigned r = 0; r < reps; ++r) HIP_CHECK(nemo_launch_copy(nullptr, src.ptr, dst.ptr, bytes));
-		HIP_CHECK(hipEventRecord(e1, nullptr));
-		HIP_CHECK(hipEventSynchronize(e1));
-		float ms = 0;
-		HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-		(void) hipEventDestroy(e0);
-		(void) hipEventDestroy(e1);
-		*gbps = 2.0 * double(bytes) * reps / (double(ms) * 1e-3) / 1e9;
-	});
 }
 
 nemo_status_t nemo_hip_eval_math(int fn, const float in[], float out[], size_t n)

@@ -1244,6 +1244,11 @@ float oracle_nrand(uint32_t* state4)
 	return v;
 }
 
+void oracle_nrand_n(uint32_t* states, size_t n, float* out)
+{
+	for(size_t i = 0; i < n; ++i) out[i] = oracle_nrand(states + 4 * i);
+}
+
 uint32_t oracle_mt19937_nth(uint32_t seed, unsigned n)
 {
 	std::mt19937 mt(seed);

@@ -90,6 +90,8 @@ int oracle_num_threads(void);
 void oracle_rng_seeds(unsigned first, unsigned last, uint32_t* out /* 4*(last-first+1) */);
 uint32_t oracle_urand(uint32_t* state4);
 float oracle_nrand(uint32_t* state4);
+/* nrand of n generators (4 words each, advanced in place) */
+void oracle_nrand_n(uint32_t* states, size_t n, float* out);
 uint32_t oracle_mt19937_nth(uint32_t seed, unsigned n);
 
 /* --- fixed point (src/nemo/fixedpoint.cpp:17-49) --- */

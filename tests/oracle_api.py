@@ -11,7 +11,9 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
-ORACLE_LIB = os.path.join(ORACLE_DIR, "_build", "liboracle.so")
+# NEMO_ORACLE_LIB: another build of the oracle (e.g. one with a planted
+# mistake, to check that the pinning tests catch it)
+ORACLE_LIB = os.environ.get("NEMO_ORACLE_LIB", os.path.join(ORACLE_DIR, "_build", "liboracle.so"))
 
 _lib = None
 
@@ -73,6 +75,7 @@ def lib():
             ("oracle_rng_seeds", None, [ctypes.c_uint, ctypes.c_uint, P(ctypes.c_uint32)]),
             ("oracle_urand", ctypes.c_uint32, [P(ctypes.c_uint32)]),
             ("oracle_nrand", ctypes.c_float, [P(ctypes.c_uint32)]),
+            ("oracle_nrand_n", None, [P(ctypes.c_uint32), ctypes.c_size_t, P(ctypes.c_float)]),
             ("oracle_mt19937_nth", ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint]),
             ("oracle_fx_tofix", ctypes.c_int, [ctypes.c_float, ctypes.c_uint, P(ctypes.c_int32)]),
             ("oracle_wfx_tofloat", ctypes.c_float, [ctypes.c_int64, ctypes.c_uint]),
@@ -314,6 +317,15 @@ def nrand(state):
     v = lib().oracle_nrand(_ptr(s, ctypes.c_uint32))
     state[:] = s
     return v
+
+
+def nrand_n(states):
+    """nrand of every generator in states (n x 4, advanced in place)."""
+    s = np.ascontiguousarray(states, dtype=np.uint32)
+    out = np.zeros(s.shape[0], np.float32)
+    lib().oracle_nrand_n(_ptr(s, ctypes.c_uint32), s.shape[0], _ptr(out, ctypes.c_float))
+    states[...] = s
+    return out
 
 
 def mt19937_nth(seed, n):

@@ -260,3 +260,149 @@ def _takes_count(sim):
         return len(inspect.signature(sim.fixed_weights).parameters) == 1
     except (TypeError, ValueError):
         return False
+
+
+# src/test/test.cpp:775-968 testSetNeuron: neuron setters and getters of the
+# network and of a running simulation (Izhikevich: parameters a, b, c, d,
+# sigma; state u, v), with the reference's float arithmetic
+def set_neuron(api, error):
+    import pytest
+    f = np.float32
+    a, b = f(0.02), f(0.2)
+    c = f(-65.0) + f(15.0) * f(0.25)
+    d = f(8.0) - f(6.0) * f(0.25)
+    v = f(-65.0)
+    u = b * v
+    sigma = f(5.0)
+    e = f(0.1)
+
+    def iz(a_, b_, c_, d_, u_, v_, s_):
+        """the deprecated Izhikevich argument order -> parameters, state"""
+        return [a_, b_, c_, d_, s_, u_, v_]
+
+    net = api.Network()
+    with pytest.raises(error):
+        net.set_neuron(0, *iz(a, b, c, d, u, v, sigma))
+    t = net.add_neuron_type("Izhikevich")
+    net.add_neuron(t, 0, *iz(a, b, c - f(0.1), d, u, v - f(1.0), sigma))
+    for bad in (lambda: net.get_neuron_parameter(1, 0), lambda: net.get_neuron_state(1, 0),
+                lambda: net.get_neuron_parameter(0, 5), lambda: net.get_neuron_state(0, 2)):
+        with pytest.raises(error):
+            bad()
+    net.set_neuron(0, *iz(a - e, b - e, c - e, d - e, u - e, v - e, sigma - e))
+    assert [net.get_neuron_parameter(0, p) for p in range(5)] == [a - e, b - e, c - e, d - e, sigma - e]
+    assert [net.get_neuron_state(0, s) for s in range(2)] == [u - e, v - e]
+    for p, x in enumerate((a, b, c, d, sigma)):
+        net.set_neuron_parameter(0, p, x)
+        assert net.get_neuron_parameter(0, p) == x
+    for s, x in enumerate((u, v)):
+        net.set_neuron_state(0, s, x)
+        assert net.get_neuron_state(0, s) == x
+    for bad in (lambda: net.set_neuron_parameter(1, 0, 0.0), lambda: net.set_neuron_state(1, 0, 0.0),
+                lambda: net.set_neuron_parameter(0, 5, 0.0), lambda: net.set_neuron_state(0, 2, 0.0)):
+        with pytest.raises(error):
+            bad()
+
+    conf = api.Configuration()
+    sim = api.Simulation(net, conf)
+    sim.step()
+    sim.set_neuron_state(0, 0, u - e)
+    assert sim.get_neuron_state(0, 0) == u - e
+    sim.set_neuron_state(0, 1, v - e)
+    assert sim.get_neuron_state(0, 1) == v - e
+    u0, v0 = sim.get_neuron_state(0, 0), sim.get_neuron_state(0, 1)
+    for p, x in enumerate((a - e, b - e, c - e, d - e, sigma - e)):
+        sim.set_neuron_parameter(0, p, x)
+        assert sim.get_neuron_parameter(0, p) == x
+    sim.step()
+    assert [sim.get_neuron_parameter(0, p) for p in range(5)] == [a - e, b - e, c - e, d - e, sigma - e]
+    assert sim.get_neuron_state(0, 0) != u0
+    assert sim.get_neuron_state(0, 1) != v0
+    for bad in (lambda: sim.set_neuron_parameter(1, 0, 0.0), lambda: sim.set_neuron_state(1, 0, 0.0),
+                lambda: sim.get_neuron_parameter(1, 0), lambda: sim.get_neuron_state(1, 0),
+                lambda: sim.set_neuron_parameter(0, 5, 0.0), lambda: sim.set_neuron_state(0, 2, 0.0),
+                lambda: sim.get_neuron_parameter(0, 5), lambda: sim.get_neuron_state(0, 2)):
+        with pytest.raises(error):
+            bad()
+    del sim
+
+    sim = api.Simulation(net, conf)
+    sim.step()
+    v_ref = sim.get_membrane_potential(0)
+    del sim
+
+    # changing c, which only matters when the neuron fires, leaves the step alone
+    sim = api.Simulation(net, conf)
+    assert sim.get_neuron_state(0, 0) == u and sim.get_neuron_state(0, 1) == v
+    sim.set_neuron(0, *iz(a, b, c + f(1.0), d, u, v, sigma))
+    sim.step()
+    assert sim.get_membrane_potential(0) == v_ref
+    assert [sim.get_neuron_parameter(0, p) for p in range(5)] == [a, b, c + f(1.0), d, sigma]
+    del sim
+
+    # setting the state is not applied more than once
+    net0 = api.Network()
+    t0 = net0.add_neuron_type("Izhikevich")
+    net0.add_neuron(t0, 0, *iz(a, b, c, d, u, v, f(0.0)))
+    sim0 = api.Simulation(net0, conf)
+    sim0.step()
+    sim0.step()
+    sim1 = api.Simulation(net0, conf)
+    sim1.step()
+    sim1.set_neuron(0, *iz(a, b, c, d, u, v, f(0.0)))
+    sim1.step()
+    sim1.step()
+    assert sim0.get_membrane_potential(0) == sim1.get_membrane_potential(0)
+
+    # the membrane potential set after construction is the one simulated
+    net1 = api.Network()
+    t1 = net1.add_neuron_type("Izhikevich")
+    net1.add_neuron(t1, 0, *iz(a, b, c, d, u, v - f(1.0), sigma))
+    sim = api.Simulation(net1, conf)
+    sim.set_neuron(0, *iz(a, b, c, d, u, v, sigma))
+    assert sim.get_membrane_potential(0) == v
+    sim.step()
+    assert sim.get_membrane_potential(0) == v_ref
+
+
+# src/test/PoissonSource.cpp:7-30 testRate: the mean rate of one Poisson
+# source over a long run approaches its rate parameter (crude bound of the
+# reference), alone or beside a ring of neurons that never fire
+def poisson_rate(api, duration, other_neurons):
+    net = api.Network()
+    if other_neurons:
+        gen.ring(1024, 0, delay=1).load(net)
+        idx = 1024
+    else:
+        idx = 0
+    t = net.add_neuron_type("PoissonSource")
+    rate = np.float32(0.010)
+    net.add_neuron(t, idx, rate)
+    sim = api.Simulation(net, api.Configuration())
+    nfired = 0
+    for _ in range(duration):
+        nfired += len(sim.step())
+    expected = int(abs(np.float32(duration) * rate))
+    deviation = (nfired - expected) % (1 << 32)  # the reference's unsigned difference
+    assert nfired > 0
+    assert deviation < expected * 2
+    return nfired
+
+
+# src/test/Input.cpp:16-65: input neurons (no parameters, no state) fire
+# exactly as stimulated, alone or followed by as many Izhikevich neurons
+def input_neurons(api, ncount, izhikevich, seed=9):
+    net = api.Network()
+    t = net.add_neuron_type("Input")
+    net.add_neuron(t, np.arange(ncount, dtype=np.uint32))
+    if izhikevich:
+        iz = net.add_neuron_type("Izhikevich")
+        net.add_neuron(iz, np.arange(ncount, 2 * ncount, dtype=np.uint32),
+                       0.02, 0.2, -65.0, 8.0, 0.0, np.float32(0.2) * np.float32(-65.0), -65.0)
+    sim = api.Simulation(net, api.Configuration())
+    rng = np.random.default_rng(seed)
+    for _ in range(1000):
+        p = 0.5 * rng.random()
+        fstim = [n for n in range(ncount) if rng.random() < p]
+        fired = sim.step(fstim=fstim)
+        assert list(fired) == fstim
