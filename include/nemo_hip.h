@@ -164,6 +164,20 @@ nemo_status_t nemo_get_neuron_parameter_s(nemo_simulation_t, unsigned neuron, un
 /* src/nemo.h:625-732 (simulation setters) */
 nemo_status_t nemo_set_neuron_s(nemo_simulation_t, unsigned idx, unsigned nargs, float args[]);
 nemo_status_t nemo_set_neuron_state_s(nemo_simulation_t, unsigned neuron, unsigned var, float val);
+/* Extension: vector forms of nemo_set_neuron_s (args n x nargs row-major),
+ * nemo_get/set_neuron_state_s and nemo_get/set_neuron_parameter_s over n
+ * neurons, as the Python binding's list arguments (src/api/python/
+ * nemo_py.cpp); same errors as n single calls, raised before anything is
+ * changed.  For a neuron listed twice the last value wins. */
+nemo_status_t nemo_set_neurons_s(nemo_simulation_t, size_t n, const unsigned idx[], unsigned nargs,
+		const float args[]);
+nemo_status_t nemo_get_neuron_states_s(nemo_simulation_t, size_t n, const unsigned idx[], unsigned var, float out[]);
+nemo_status_t nemo_get_neuron_parameters_s(nemo_simulation_t, size_t n, const unsigned idx[], unsigned param,
+		float out[]);
+nemo_status_t nemo_set_neuron_states_s(nemo_simulation_t, size_t n, const unsigned idx[], unsigned var,
+		const float val[]);
+nemo_status_t nemo_set_neuron_parameters_s(nemo_simulation_t, size_t n, const unsigned idx[], unsigned param,
+		const float val[]);
 /* src/nemo.h:655-658 (deprecated in the reference; kept for drop-in) */
 nemo_status_t nemo_set_neuron_iz_s(nemo_simulation_t, unsigned idx,
 		float a, float b, float c, float d, float u, float v, float sigma);

@@ -30,6 +30,7 @@ def version():
 
 def add_plugin_path(path):
     check(lib().nemo_add_plugin_path(path.encode()))
+    _extra_paths.append(path)
 
 
 def hip_device_count():
@@ -56,12 +57,30 @@ def _ptr(a, t):
     return a.ctypes.data_as(ctypes.POINTER(t))
 
 
+# Named parameters and state variables of each model, in index order: the
+# names NeMo's Python module maps (src/api/python/wrappers.py:72-150).
+_NAMED_ARGS = {
+    "Izhikevich": (("a", "b", "c", "d", "sigma"), ("u", "v")),
+    "IF_curr_exp": (("v_rest", "v_reset", "c_m", "tau_m", "tau_refrac", "tau_synE", "tau_synI", "v_thresh",
+                     "I_offset"), ("v", "ie", "ii", "lastfired")),
+    "PoissonSource": (("lambda",), ()),
+    "Input": ((), ()),
+    "Kuramoto": (("omega", "Cmean"), ("theta",)),
+}
+
+
 def _ini_names(model):
-    """Parameter and state names of a model from its .ini file."""
-    path = os.path.join(plugin_dir(), model + ".ini")
-    params, state, section = {}, {}, ""
-    if not os.path.exists(path):
-        return params, state
+    """Parameter and state names of a model: the table above, or else the key
+    order of the first [params:...] section and of the [state_names] section
+    of its .ini file (which NeMo's .ini files list in index order for the
+    other models)."""
+    if model in _NAMED_ARGS:
+        return _NAMED_ARGS[model]
+    path = next((os.path.join(d, model + ".ini") for d in _search_paths()
+                 if os.path.exists(os.path.join(d, model + ".ini"))), None)
+    if path is None:
+        return (), ()
+    params, state, section, params_section = [], [], "", None
     with open(path) as f:
         for line in f:
             line = line.split("#", 1)[0].strip()
@@ -69,11 +88,31 @@ def _ini_names(model):
                 continue
             if line.startswith("[") and line.endswith("]"):
                 section = line[1:-1]
+                if section.startswith("params:") and params_section is None:
+                    params_section = section
                 continue
-            if "=" in line and section in ("param_names", "state_names"):
-                k, v = (x.strip() for x in line.split("=", 1))
-                (params if section == "param_names" else state)[k] = int(v)
-    return params, state
+            if "=" not in line:
+                continue
+            key = line.split("=", 1)[0].strip()
+            if section == params_section:
+                params.append(key)
+            elif section == "state_names":
+                state.append(key)
+    return tuple(params), tuple(state)
+
+
+_extra_paths = []
+
+
+def _search_paths():
+    """Directories searched for <Model>.ini, in the library's order: added
+    paths, NEMO_PLUGIN_PATH, then the package's plugins directory."""
+    env = [p for p in os.environ.get("NEMO_PLUGIN_PATH", "").split(":") if p]
+    return _extra_paths + env + [plugin_dir()]
+
+
+def _is_seq(x):
+    return isinstance(x, (list, tuple, range, np.ndarray))
 
 
 class Configuration:
@@ -178,19 +217,25 @@ class Network(_SynapseQueries):
         return t.value
 
     def add_neuron(self, neuron_type, neuron_idx, *args, paramDict=None, stateDict=None):
-        """Add one neuron or a vector of neurons.  args are the model's
-        parameters followed by its state variables, each a scalar or a sequence
-        of the same length as neuron_idx; alternatively name them in paramDict
-        and stateDict (wrappers.py:72-155)."""
+        """Add one neuron or a vector of neurons.  As in NeMo's Python module
+        (wrappers.py:72-155), the model's parameters and state variables are
+        given as two dictionaries of named values, positionally or by
+        keyword: add_neuron(type, idx, paramDict, stateDict).  Alternatively
+        args are the parameters followed by the state variables, in index
+        order.  Each value is a scalar or a sequence as long as neuron_idx."""
+        if neuron_type not in self._types:
+            raise RuntimeError("That neuron type is not in this network. Use add_neuron_type first.")
+        if args and isinstance(args[0], dict):
+            paramDict = args[0]
+            stateDict = args[1] if len(args) > 1 else {}
+            args = ()
         if paramDict is not None or stateDict is not None:
             pnames, snames = _ini_names(self._types[neuron_type])
             paramDict, stateDict = paramDict or {}, stateDict or {}
-            ordered = [None] * (len(pnames) + len(snames))
-            for k, i in pnames.items():
-                ordered[i] = paramDict[k]
-            for k, i in snames.items():
-                ordered[len(pnames) + i] = stateDict[k]
-            args = tuple(ordered)
+            missing = [k for k in pnames if k not in paramDict] + [k for k in snames if k not in stateDict]
+            if missing:
+                raise KeyError("missing neuron parameters or state variables: " + ", ".join(missing))
+            args = tuple(paramDict[k] for k in pnames) + tuple(stateDict[k] for k in snames)
         idx = np.atleast_1d(_u32(neuron_idx))
         n = idx.size
         cols = [np.broadcast_to(np.asarray(a, dtype=np.float32), (n,)) for a in args]
@@ -202,6 +247,7 @@ class Network(_SynapseQueries):
         """Add one synapse (returns its id) or vectors of synapses (returns a
         numpy array of ids); scalars broadcast against vectors."""
         scalar = all(np.isscalar(x) for x in (source, target, delay, weight, plastic))
+        as_list = any(isinstance(x, (list, tuple, range)) for x in (source, target, delay, weight, plastic))
         n = max(np.size(x) for x in (source, target, delay, weight, plastic))
         s = np.ascontiguousarray(np.broadcast_to(_u32(source), (n,)))
         t = np.ascontiguousarray(np.broadcast_to(_u32(target), (n,)))
@@ -212,32 +258,62 @@ class Network(_SynapseQueries):
         check(lib().nemo_add_synapses(self._h, n, _ptr(s, ctypes.c_uint), _ptr(t, ctypes.c_uint),
                                       _ptr(d, ctypes.c_uint), _ptr(w, ctypes.c_float),
                                       _ptr(p, ctypes.c_ubyte), _ptr(ids, ctypes.c_uint64)))
-        return int(ids[0]) if scalar else ids
+        if scalar:
+            return int(ids[0])
+        return ids.tolist() if as_list else ids
 
     def neuron_count(self):
         n = ctypes.c_uint(0)
         check(lib().nemo_neuron_count(self._h, ctypes.byref(n)))
         return n.value
 
-    def get_neuron_state(self, idx, var):
+    def _get1(self, fn, idx, k):
         v = ctypes.c_float()
-        check(lib().nemo_get_neuron_state_n(self._h, idx, var, ctypes.byref(v)))
+        check(fn(self._h, int(idx), k, ctypes.byref(v)))
         return v.value
+
+    def get_neuron_state(self, idx, var):
+        """State variable var of a neuron, or a list of them for a list of
+        neurons."""
+        if _is_seq(idx):
+            return [self._get1(lib().nemo_get_neuron_state_n, i, var) for i in idx]
+        return self._get1(lib().nemo_get_neuron_state_n, idx, var)
 
     def get_neuron_parameter(self, idx, param):
-        v = ctypes.c_float()
-        check(lib().nemo_get_neuron_parameter_n(self._h, idx, param, ctypes.byref(v)))
-        return v.value
+        if _is_seq(idx):
+            return [self._get1(lib().nemo_get_neuron_parameter_n, i, param) for i in idx]
+        return self._get1(lib().nemo_get_neuron_parameter_n, idx, param)
 
     def set_neuron(self, idx, *args):
-        a = _f32(args)
-        check(lib().nemo_set_neuron_n(self._h, idx, a.size, _ptr(a, ctypes.c_float)))
+        """Replace the parameters and state of one neuron, or of a list of
+        neurons (each argument then a scalar or a list as long)."""
+        idx, mat = _neuron_rows(idx, args)
+        for i, row in zip(idx, mat):
+            check(lib().nemo_set_neuron_n(self._h, int(i), row.size, _ptr(row, ctypes.c_float)))
 
     def set_neuron_state(self, idx, var, val):
-        check(lib().nemo_set_neuron_state_n(self._h, idx, var, val))
+        idx, vals = _pairs(idx, val)
+        for i, v in zip(idx, vals):
+            check(lib().nemo_set_neuron_state_n(self._h, int(i), var, float(v)))
 
     def set_neuron_parameter(self, idx, param, val):
-        check(lib().nemo_set_neuron_parameter_n(self._h, idx, param, val))
+        idx, vals = _pairs(idx, val)
+        for i, v in zip(idx, vals):
+            check(lib().nemo_set_neuron_parameter_n(self._h, int(i), param, float(v)))
+
+
+def _neuron_rows(idx, args):
+    """(indices, n x nargs float32 rows) from a neuron or list of neurons and
+    per-argument scalars or lists."""
+    idx = np.atleast_1d(_u32(idx))
+    cols = [np.broadcast_to(np.asarray(a, dtype=np.float32), (idx.size,)) for a in args]
+    mat = np.ascontiguousarray(np.stack(cols, axis=1) if cols else np.zeros((idx.size, 0), np.float32))
+    return idx, mat
+
+
+def _pairs(idx, val):
+    idx = np.atleast_1d(_u32(idx))
+    return idx, np.ascontiguousarray(np.broadcast_to(np.asarray(val, dtype=np.float32), (idx.size,)))
 
 
 class Simulation(_SynapseQueries):
@@ -268,6 +344,9 @@ class Simulation(_SynapseQueries):
         """One 1 ms step; fstim: neurons forced to fire, istim: list of
         (neuron, current) pairs.  Returns the fired neurons in simulation order
         (wrappers.py:252-276)."""
+        if istim and (not isinstance(istim[0], tuple) or len(istim[0]) != 2):
+            raise ValueError("Current input must be specified as a list of (idx, I) pairs, "
+                             "e.g. [(idx1, I1), (idx2, I2), ...]")
         f = _u32(fstim if fstim is not None else [])
         if istim:
             ii = _u32([x[0] for x in istim])
@@ -279,6 +358,18 @@ class Simulation(_SynapseQueries):
         check(lib().nemo_step(self._h, _ptr(f, ctypes.c_uint), f.size, _ptr(ii, ctypes.c_uint),
                               _ptr(iv, ctypes.c_float), ii.size, ctypes.byref(out), ctypes.byref(n)))
         return [out[i] for i in range(n.value)]
+
+    def step_noinput(self):
+        return self.step()
+
+    def step_f(self, fstim):
+        return self.step(fstim=fstim)
+
+    def step_i(self, istim):
+        return self.step(istim=istim)
+
+    def step_fi(self, fstim, istim):
+        return self.step(fstim=fstim, istim=istim)
 
     def run(self, nsteps, stdp_period=0, reward=1.0, raster=True):
         """Run nsteps steps on the device (no stimulus), applying STDP with the
@@ -356,25 +447,37 @@ class Simulation(_SynapseQueries):
         check(lib().nemo_get_membrane_potential(self._h, idx, ctypes.byref(v)))
         return v.value
 
+    def _get_values(self, fn, idx, k):
+        seq = _is_seq(idx)
+        ii = np.atleast_1d(_u32(idx))
+        out = np.zeros(ii.size, np.float32)
+        check(fn(self._h, ii.size, _ptr(ii, ctypes.c_uint), k, _ptr(out, ctypes.c_float)))
+        return [float(x) for x in out] if seq else float(out[0])
+
     def get_neuron_state(self, idx, var):
-        v = ctypes.c_float()
-        check(lib().nemo_get_neuron_state_s(self._h, idx, var, ctypes.byref(v)))
-        return v.value
+        """State variable var of a neuron, or a list of them for a list of
+        neurons."""
+        return self._get_values(lib().nemo_get_neuron_states_s, idx, var)
 
     def get_neuron_parameter(self, idx, param):
-        v = ctypes.c_float()
-        check(lib().nemo_get_neuron_parameter_s(self._h, idx, param, ctypes.byref(v)))
-        return v.value
+        return self._get_values(lib().nemo_get_neuron_parameters_s, idx, param)
 
     def set_neuron(self, idx, *args):
-        a = _f32(args)
-        check(lib().nemo_set_neuron_s(self._h, idx, a.size, _ptr(a, ctypes.c_float)))
+        """Replace the parameters and state of one neuron, or of a list of
+        neurons (each argument then a scalar or a list as long)."""
+        ii, mat = _neuron_rows(idx, args)
+        check(lib().nemo_set_neurons_s(self._h, ii.size, _ptr(ii, ctypes.c_uint), mat.shape[1],
+                                       _ptr(mat, ctypes.c_float)))
 
     def set_neuron_state(self, idx, var, val):
-        check(lib().nemo_set_neuron_state_s(self._h, idx, var, val))
+        ii, vals = _pairs(idx, val)
+        check(lib().nemo_set_neuron_states_s(self._h, ii.size, _ptr(ii, ctypes.c_uint), var,
+                                             _ptr(vals, ctypes.c_float)))
 
     def set_neuron_parameter(self, idx, param, val):
-        check(lib().nemo_set_neuron_parameter_s(self._h, idx, param, val))
+        ii, vals = _pairs(idx, val)
+        check(lib().nemo_set_neuron_parameters_s(self._h, ii.size, _ptr(ii, ctypes.c_uint), param,
+                                                 _ptr(vals, ctypes.c_float)))
 
     def elapsed_wallclock(self):
         v = ctypes.c_ulong()

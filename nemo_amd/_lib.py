@@ -72,6 +72,11 @@ _PROTOTYPES = [
     ("nemo_set_neuron_s", ctypes.c_int, [vp, ctypes.c_uint, ctypes.c_uint, c_float_p]),
     ("nemo_set_neuron_state_s", ctypes.c_int, [vp, ctypes.c_uint, ctypes.c_uint, ctypes.c_float]),
     ("nemo_set_neuron_iz_s", ctypes.c_int, [vp, ctypes.c_uint] + [ctypes.c_float] * 7),
+    ("nemo_set_neurons_s", ctypes.c_int, [vp, ctypes.c_size_t, c_uint_p, ctypes.c_uint, c_float_p]),
+    ("nemo_get_neuron_states_s", ctypes.c_int, [vp, ctypes.c_size_t, c_uint_p, ctypes.c_uint, c_float_p]),
+    ("nemo_get_neuron_parameters_s", ctypes.c_int, [vp, ctypes.c_size_t, c_uint_p, ctypes.c_uint, c_float_p]),
+    ("nemo_set_neuron_states_s", ctypes.c_int, [vp, ctypes.c_size_t, c_uint_p, ctypes.c_uint, c_float_p]),
+    ("nemo_set_neuron_parameters_s", ctypes.c_int, [vp, ctypes.c_size_t, c_uint_p, ctypes.c_uint, c_float_p]),
     ("nemo_set_neuron_parameter_s", ctypes.c_int, [vp, ctypes.c_uint, ctypes.c_uint, ctypes.c_float]),
     ("nemo_get_synapse_source_s", ctypes.c_int, [vp, ctypes.c_uint64, c_uint_p]),
     ("nemo_get_synapse_target_s", ctypes.c_int, [vp, ctypes.c_uint64, c_uint_p]),

@@ -4,6 +4,7 @@
 #include "backend.hpp"
 
 #include <algorithm>
+#include <cctype>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -133,32 +134,64 @@ static std::string trim(const std::string& s)
 	return s.substr(a, b - a + 1);
 }
 
-static bool parseBool(const std::string& v)
+/* boost::program_options' spellings of a bool (NeuronType.cpp reads the
+ * .ini with program_options) */
+static bool parseBool(const std::string& v, const std::string& key, const std::string& path)
 {
-	return v == "true" || v == "1" || v == "yes";
+	std::string l(v);
+	std::transform(l.begin(), l.end(), l.begin(), [](unsigned char c) { return char(std::tolower(c)); });
+	if(l == "true" || l == "1" || l == "yes" || l == "on") return true;
+	if(l == "false" || l == "0" || l == "no" || l == "off") return false;
+	throw Error(NEMO_INVALID_INPUT, "Error parsing neuron model configuration file " + path + ": invalid value '"
+			+ v + "' for " + key);
 }
 
-/* <Model>.ini: key=value lines, [section] headers, '#' comments
- * (NeuronType.cpp:93-158) */
+static unsigned parseUnsigned(const std::string& v, const std::string& key, const std::string& path)
+{
+	size_t used = 0;
+	unsigned long x = 0;
+	try {
+		x = std::stoul(v, &used);
+	} catch(const std::exception&) {
+		used = 0;
+	}
+	if(used == 0 || used != v.size() || v[0] == '-') {
+		throw Error(NEMO_INVALID_INPUT, "Error parsing neuron model configuration file " + path
+				+ ": invalid value '" + v + "' for " + key);
+	}
+	return unsigned(x);
+}
+
+/* <Model>.ini in NeMo's format (NeuronType.cpp:93-158): key=value lines,
+ * [section] headers, '#' comments.  Required: parameters, state-variables,
+ * membrane-potential and rng.normal; optional: history (1), rcm.sources,
+ * rcm.delays, rcm.forward, rcm.weights (false) and the backend flags, of
+ * which this backend needs backends.hip.  Other keys and sections -- the
+ * [params:...] and [state_names] defaults NeMo's tools read -- are
+ * ignored, as NeMo's parser does.  [hip] scratch-planes is this backend's
+ * own. */
 NeuronType loadNeuronType(const std::string& name)
 {
 	for(const std::string& dir : pluginPaths()) {
 		std::string path = dir + "/" + name + ".ini";
 		if(!fileExists(path)) continue;
 		std::ifstream in(path);
+		if(!in) {
+			throw Error(NEMO_IO_ERROR, "Failed to open neuron model configuration file " + path);
+		}
 		NeuronType t;
 		t.name = name;
 		t.pluginDir = dir;
 		std::string section, line;
 		bool hip = false;
-		bool haveParams = false, haveState = false;
+		bool haveParams = false, haveState = false, haveMp = false, haveRng = false;
 		while(std::getline(in, line)) {
 			size_t hash = line.find('#');
 			if(hash != std::string::npos) line = line.substr(0, hash);
 			line = trim(line);
 			if(line.empty()) continue;
 			if(line.front() == '[' && line.back() == ']') {
-				section = line.substr(1, line.size() - 2);
+				section = trim(line.substr(1, line.size() - 2));
 				continue;
 			}
 			size_t eq = line.find('=');
@@ -166,26 +199,28 @@ NeuronType loadNeuronType(const std::string& name)
 			std::string key = trim(line.substr(0, eq));
 			std::string val = trim(line.substr(eq + 1));
 			if(section.empty()) {
-				if(key == "parameters") { t.nParam = unsigned(std::stoul(val)); haveParams = true; }
-				else if(key == "state-variables") { t.nState = unsigned(std::stoul(val)); haveState = true; }
-				else if(key == "membrane-potential") t.membranePotential = unsigned(std::stoul(val));
-				else if(key == "history") t.history = unsigned(std::stoul(val));
+				if(key == "parameters") { t.nParam = parseUnsigned(val, key, path); haveParams = true; }
+				else if(key == "state-variables") { t.nState = parseUnsigned(val, key, path); haveState = true; }
+				else if(key == "membrane-potential") { t.membranePotential = parseUnsigned(val, key, path); haveMp = true; }
+				else if(key == "history") t.history = parseUnsigned(val, key, path);
 			} else if(section == "rng") {
-				if(key == "normal") t.rngNormal = parseBool(val);
+				if(key == "normal") { t.rngNormal = parseBool(val, "rng.normal", path); haveRng = true; }
 			} else if(section == "rcm") {
-				if(parseBool(val)) t.usesRcm = true;
+				if(key == "sources") t.rcmSources = parseBool(val, "rcm.sources", path);
+				else if(key == "delays") t.rcmDelays = parseBool(val, "rcm.delays", path);
+				else if(key == "forward") t.rcmForward = parseBool(val, "rcm.forward", path);
+				else if(key == "weights") t.rcmWeights = parseBool(val, "rcm.weights", path);
 			} else if(section == "backends") {
-				if(key == "hip") hip = parseBool(val);
+				if(key == "hip") hip = parseBool(val, "backends.hip", path);
 			} else if(section == "hip") {
-				if(key == "scratch-planes") t.scratchPlanes = unsigned(std::stoul(val));
-			} else if(section == "param_names") {
-				t.paramNames[key] = unsigned(std::stoul(val));
-			} else if(section == "state_names") {
-				t.stateNames[key] = unsigned(std::stoul(val));
+				if(key == "scratch-planes") t.scratchPlanes = parseUnsigned(val, "hip.scratch-planes", path);
 			}
 		}
-		if(!haveParams || !haveState) {
-			throw Error(NEMO_INVALID_INPUT, "neuron type description " + path + " lacks parameter or state counts");
+		const char* missing = !haveParams ? "parameters" : !haveState ? "state-variables"
+			: !haveMp ? "membrane-potential" : !haveRng ? "rng.normal" : nullptr;
+		if(missing) {
+			throw Error(NEMO_INVALID_INPUT, std::string("Missing parameter '") + missing + "' in configuration file "
+					+ path);
 		}
 		if(!hip) {
 			throw Error(NEMO_API_UNSUPPORTED, "neuron type " + name + " has no HIP backend");
@@ -2172,6 +2207,129 @@ void Simulation::setNeuronParameter(unsigned idx, unsigned param, float val)
 	if(param >= g.type.nParam) throw Error(NEMO_INVALID_INPUT, "Invalid parameter index " + std::to_string(param));
 	HIP_CHECK(hipStreamSynchronize(m_stream));
 	HIP_CHECK(hipMemcpy(g.param.ptr + size_t(param) * g.count + (l - g.start), &val, 4, hipMemcpyHostToDevice));
+}
+
+/* Vector forms of the neuron getters and setters (the Python binding's list
+ * arguments, src/api/python/nemo_py.cpp): a few neurons are read or written
+ * element by element; more, plane by plane (each plane of a model that the
+ * call touches is copied to the host once, and written back once). */
+static const size_t ELEMENTWISE_MAX = 16;
+
+void Simulation::getNeuronValues(bool param, unsigned var, size_t n, const unsigned* idx, float* out)
+{
+	activate();
+	std::vector<unsigned> loc(n);
+	for(size_t i = 0; i < n; ++i) {
+		loc[i] = localIdx(idx[i]);
+		const Group& g = groupOf(loc[i]);
+		if(var >= (param ? g.type.nParam : g.type.nState)) {
+			throw Error(NEMO_INVALID_INPUT, std::string(param ? "Invalid parameter index " : "Invalid state variable index ")
+					+ std::to_string(var));
+		}
+	}
+	HIP_CHECK(hipStreamSynchronize(m_stream));
+	std::vector<std::vector<float> > planes(m_groups.size());
+	for(size_t i = 0; i < n; ++i) {
+		const int gi = m_typeOfLocal[loc[i]];
+		Group& g = m_groups[gi];
+		const float* plane = (param ? g.param.ptr : g.state.ptr) + size_t(var) * g.count;
+		if(n <= ELEMENTWISE_MAX) {
+			HIP_CHECK(hipMemcpy(out + i, plane + (loc[i] - g.start), 4, hipMemcpyDeviceToHost));
+			continue;
+		}
+		if(planes[gi].empty()) {
+			planes[gi].resize(g.count);
+			HIP_CHECK(hipMemcpy(planes[gi].data(), plane, g.count * sizeof(float), hipMemcpyDeviceToHost));
+		}
+		out[i] = planes[gi][loc[i] - g.start];
+	}
+}
+
+void Simulation::setNeuronValues(bool param, unsigned var, size_t n, const unsigned* idx, const float* vals)
+{
+	activate();
+	std::vector<unsigned> loc(n);
+	for(size_t i = 0; i < n; ++i) {
+		loc[i] = localIdx(idx[i]);
+		const Group& g = groupOf(loc[i]);
+		if(var >= (param ? g.type.nParam : g.type.nState)) {
+			throw Error(NEMO_INVALID_INPUT, std::string(param ? "Invalid parameter index " : "Invalid state variable index ")
+					+ std::to_string(var));
+		}
+	}
+	HIP_CHECK(hipStreamSynchronize(m_stream));
+	if(n <= ELEMENTWISE_MAX) {
+		for(size_t i = 0; i < n; ++i) {
+			Group& g = groupOf(loc[i]);
+			float* plane = (param ? g.param.ptr : g.state.ptr) + size_t(var) * g.count;
+			HIP_CHECK(hipMemcpy(plane + (loc[i] - g.start), vals + i, 4, hipMemcpyHostToDevice));
+		}
+		return;
+	}
+	std::vector<std::vector<float> > planes(m_groups.size());
+	for(size_t i = 0; i < n; ++i) {
+		const int gi = m_typeOfLocal[loc[i]];
+		Group& g = m_groups[gi];
+		if(planes[gi].empty()) {
+			planes[gi].resize(g.count);
+			HIP_CHECK(hipMemcpy(planes[gi].data(), (param ? g.param.ptr : g.state.ptr) + size_t(var) * g.count,
+						g.count * sizeof(float), hipMemcpyDeviceToHost));
+		}
+		planes[gi][loc[i] - g.start] = vals[i]; /* for repeated neurons the last value wins */
+	}
+	for(size_t gi = 0; gi < m_groups.size(); ++gi) {
+		if(planes[gi].empty()) continue;
+		Group& g = m_groups[gi];
+		HIP_CHECK(hipMemcpy((param ? g.param.ptr : g.state.ptr) + size_t(var) * g.count, planes[gi].data(),
+					g.count * sizeof(float), hipMemcpyHostToDevice));
+	}
+}
+
+void Simulation::setNeurons(size_t n, const unsigned* idx, unsigned nargs, const float* args)
+{
+	activate();
+	std::vector<unsigned> loc(n);
+	for(size_t i = 0; i < n; ++i) {
+		loc[i] = localIdx(idx[i]);
+		const Group& g = groupOf(loc[i]);
+		if(nargs != g.type.nParam + g.type.nState) {
+			std::ostringstream msg;
+			msg << "Unexpected number of parameters/state variables when modifying neuron. Expected "
+				<< (g.type.nParam + g.type.nState) << ", found " << nargs;
+			throw Error(NEMO_INVALID_INPUT, msg.str());
+		}
+	}
+	if(n <= ELEMENTWISE_MAX) {
+		for(size_t i = 0; i < n; ++i) setNeuron(idx[i], nargs, args + i * nargs);
+		return;
+	}
+	HIP_CHECK(hipStreamSynchronize(m_stream));
+	/* per touched group: all its parameter and state planes, [plane][i] */
+	std::vector<std::vector<float> > param(m_groups.size()), state(m_groups.size());
+	for(size_t i = 0; i < n; ++i) {
+		const int gi = m_typeOfLocal[loc[i]];
+		Group& g = m_groups[gi];
+		const size_t np = size_t(g.type.nParam) * g.count, ns = size_t(g.type.nState) * g.count;
+		if(param[gi].empty() && state[gi].empty()) {
+			param[gi].resize(np);
+			state[gi].resize(ns);
+			if(np) HIP_CHECK(hipMemcpy(param[gi].data(), g.param.ptr, np * sizeof(float), hipMemcpyDeviceToHost));
+			if(ns) HIP_CHECK(hipMemcpy(state[gi].data(), g.state.ptr, ns * sizeof(float), hipMemcpyDeviceToHost));
+			if(param[gi].empty() && state[gi].empty()) continue; /* a model without parameters or state */
+		}
+		const size_t k = loc[i] - g.start;
+		for(unsigned p = 0; p < g.type.nParam; ++p) param[gi][size_t(p) * g.count + k] = args[i * nargs + p];
+		for(unsigned v = 0; v < g.type.nState; ++v) state[gi][size_t(v) * g.count + k] = args[i * nargs + g.type.nParam + v];
+	}
+	for(size_t gi = 0; gi < m_groups.size(); ++gi) {
+		Group& g = m_groups[gi];
+		if(!param[gi].empty()) {
+			HIP_CHECK(hipMemcpy(g.param.ptr, param[gi].data(), param[gi].size() * sizeof(float), hipMemcpyHostToDevice));
+		}
+		if(!state[gi].empty()) {
+			HIP_CHECK(hipMemcpy(g.state.ptr, state[gi].data(), state[gi].size() * sizeof(float), hipMemcpyHostToDevice));
+		}
+	}
 }
 
 /* -------------------------------------------------------- synapse access */

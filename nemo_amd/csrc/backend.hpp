@@ -46,11 +46,10 @@ struct NeuronType {
 	unsigned membranePotential = 0;
 	unsigned history = 1;
 	bool rngNormal = false;
-	bool usesRcm = false;
+	/* fields of the reverse matrix the model reads ([rcm]) */
+	bool rcmSources = false, rcmDelays = false, rcmForward = false, rcmWeights = false;
 	unsigned scratchPlanes = 0; /* [hip] scratch-planes: per-neuron device planes the plugin may use */
 	std::string pluginDir;
-	std::map<std::string, unsigned> paramNames;
-	std::map<std::string, unsigned> stateNames;
 };
 
 NeuronType loadNeuronType(const std::string& name);
@@ -215,6 +214,10 @@ public:
 	void setNeuron(unsigned idx, unsigned nargs, const float* args);
 	void setNeuronState(unsigned idx, unsigned var, float val);
 	void setNeuronParameter(unsigned idx, unsigned param, float val);
+	/* vector forms: parameter (param) or state variable var of n neurons */
+	void getNeuronValues(bool param, unsigned var, size_t n, const unsigned* idx, float* out);
+	void setNeuronValues(bool param, unsigned var, size_t n, const unsigned* idx, const float* vals);
+	void setNeurons(size_t n, const unsigned* idx, unsigned nargs, const float* args);
 
 	const std::vector<synapse_id>& getSynapsesFrom(unsigned source);
 	unsigned getSynapseTarget(synapse_id id);

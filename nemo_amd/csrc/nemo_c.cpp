@@ -373,6 +373,35 @@ nemo_status_t nemo_set_neuron_s(nemo_simulation_t s, unsigned idx, unsigned narg
 	return guarded([&] { check(s, "simulation"); s->impl.setNeuron(idx, nargs, args); });
 }
 
+nemo_status_t nemo_set_neurons_s(nemo_simulation_t s, size_t n, const unsigned idx[], unsigned nargs,
+		const float args[])
+{
+	return guarded([&] { check(s, "simulation"); s->impl.setNeurons(n, idx, nargs, args); });
+}
+
+nemo_status_t nemo_get_neuron_states_s(nemo_simulation_t s, size_t n, const unsigned idx[], unsigned var, float out[])
+{
+	return guarded([&] { check(s, "simulation"); s->impl.getNeuronValues(false, var, n, idx, out); });
+}
+
+nemo_status_t nemo_get_neuron_parameters_s(nemo_simulation_t s, size_t n, const unsigned idx[], unsigned param,
+		float out[])
+{
+	return guarded([&] { check(s, "simulation"); s->impl.getNeuronValues(true, param, n, idx, out); });
+}
+
+nemo_status_t nemo_set_neuron_states_s(nemo_simulation_t s, size_t n, const unsigned idx[], unsigned var,
+		const float val[])
+{
+	return guarded([&] { check(s, "simulation"); s->impl.setNeuronValues(false, var, n, idx, val); });
+}
+
+nemo_status_t nemo_set_neuron_parameters_s(nemo_simulation_t s, size_t n, const unsigned idx[], unsigned param,
+		const float val[])
+{
+	return guarded([&] { check(s, "simulation"); s->impl.setNeuronValues(true, param, n, idx, val); });
+}
+
 /* Simulation::setNeuron (src/nemo/Simulation.cpp:21-27): args {a,b,c,d,sigma,u,v} */
 nemo_status_t nemo_set_neuron_iz_s(nemo_simulation_t s, unsigned idx,
 		float a, float b, float c, float d, float u, float v, float sigma)

@@ -53,7 +53,8 @@ def test_network_construction_and_queries():
     ids = net.add_synapse(0, [2, 1], [1, 2], [-0.25, 0.75], 0)
     assert sid == 0 and list(ids) == [1, 2]
     assert net.get_synapses_from(0) == [0, 1, 2]
-    assert net.get_synapse_target(ids.tolist()) == [2, 1]
+    assert isinstance(ids, list)  # list input gives a list of ids, as NeMo's module
+    assert net.get_synapse_target(ids) == [2, 1]
     assert net.get_synapse_delay(sid) == 3
     assert net.get_synapse_weight(1) == -0.25
     assert net.get_synapse_plastic(sid) == 1
@@ -65,9 +66,10 @@ def test_network_construction_and_queries():
 def test_network_named_parameters():
     net = nemo_amd.Network()
     t = net.add_neuron_type("IF_curr_exp")
-    params = dict(v_rest=-65.0, v_reset=-70.0, cm=1.0, tau_m=20.0, tau_refrac=2.0, tau_syn_E=5.0,
-                  tau_syn_I=5.0, v_thresh=-50.0, i_offset=0.1)
-    net.add_neuron(t, 7, paramDict=params, stateDict=dict(v=-65.0, ie=0.0, ii=0.0, lastfire=100.0))
+    # the key names of NeMo's Python module (wrappers.py:117-121)
+    params = dict(v_rest=-65.0, v_reset=-70.0, c_m=1.0, tau_m=20.0, tau_refrac=2.0, tau_synE=5.0,
+                  tau_synI=5.0, v_thresh=-50.0, I_offset=0.1)
+    net.add_neuron(t, 7, paramDict=params, stateDict=dict(v=-65.0, ie=0.0, ii=0.0, lastfired=100.0))
     assert net.get_neuron_parameter(7, 1) == -70.0
     assert net.get_neuron_parameter(7, 8) == np.float32(0.1)
     assert net.get_neuron_state(7, 3) == 100.0
