@@ -346,17 +346,38 @@ nemo_status_t nemo_hip_profile(nemo_simulation_t, unsigned nsteps, double avg_us
  * reference calls the host C library for these (RNG.cpp:28-32, HH.cpp:92-98). */
 nemo_status_t nemo_hip_eval_math(int fn, const float in[], float out[], size_t n);
 
-/* ------------------------------------------------- backend factory (dlopen) */
+/* Brian's propagate (src/nemo/Simulation.hpp:93-113; cpu/Simulation.cpp:
+ * 115-137 and cuda/Simulation.cpp:328-346, which NeMo compiles with
+ * NEMO_BRIAN_ENABLED): instead of updating the neurons, the firing of the
+ * next step is given -- the nfired neuron indices in fired -- its spikes are
+ * delivered, and E and I (n = neuron count entries each, simulation order as
+ * nemo_hip_local_to_global lists it) receive the excitatory and inhibitory
+ * synaptic input of the step after it, as the neurons would see it.  Neuron
+ * state is untouched; the simulation advances one step.  Refused when STDP
+ * is configured (as in the CUDA backend) and for batches. */
+nemo_status_t nemo_hip_propagate(nemo_simulation_t, const unsigned fired[], size_t nfired, float E[], float I[],
+		size_t n);
+/* The device form (the CUDA backend's): fired is a device array of one
+ * uint32 flag per neuron in simulation order (nonzero: fired); *E and *I
+ * receive device arrays of the input in simulation order, valid until the
+ * next call on the simulation.  The work is complete on return. */
+nemo_status_t nemo_hip_propagate_device(nemo_simulation_t, const uint32_t* fired, float** E, float** I);
 
-/* src/nemo/cuda/create_simulation.hpp:14-22 (cuda_simulation): build a HIP
- * simulation of net under conf.  Returns NULL and sets nemo_strerror() on failure. */
-nemo_simulation_t hip_simulation(nemo_network_t net, nemo_configuration_t conf);
-/* src/nemo/cuda/devices.hpp:21-56 (cuda_set_configuration, cuda_device_count,
- * cuda_device_description, cuda_test_device) */
-nemo_status_t hip_set_configuration(nemo_configuration_t conf, int dev);
-unsigned hip_device_count(void);
-const char* hip_device_description(unsigned device);
-nemo_status_t hip_test_device(unsigned device);
+/* --------------------------------------------------------- backend factory */
+
+/* NeMo resolves a backend's factory by name (src/nemo/nemo.cpp:92-101,
+ * 120-133): cuda_simulation, cuda_set_configuration, cuda_test_device,
+ * cuda_device_count and cuda_device_description for the CUDA backend
+ * (src/nemo/cuda/create_simulation.hpp:14-22, src/nemo/cuda/devices.hpp:
+ * 21-56), whose arguments are NeMo's own C++ objects.  Their HIP
+ * counterparts, hip_simulation and the rest with the same signatures, are
+ * defined by integration/nemo_hip_backend.cpp, which NeMo compiles against
+ * its headers and which implements nemo::SimulationBackend over this C ABI.
+ * This library provides the device test they need: */
+
+/* cuda_test_device (src/nemo/cuda/devices.hpp:33-37), as a status: the
+ * device exists and accepts an allocation */
+nemo_status_t nemo_hip_test_device(unsigned device);
 
 #ifdef __cplusplus
 }

@@ -412,6 +412,20 @@ class Simulation(_SynapseQueries):
     def apply_stdp(self, reward):
         check(lib().nemo_apply_stdp(self._h, reward))
 
+    def propagate(self, fired):
+        """Brian's propagate (Simulation.hpp:93-113): the given neurons fire
+        in the next step instead of the model deciding; their spikes are
+        delivered and the excitatory and inhibitory input of the step after
+        is returned, two float32 arrays in simulation order.  Neuron state is
+        untouched.  Without STDP only."""
+        f = _u32(fired if fired is not None else [])
+        n = self.neuron_count
+        E = np.zeros(n, np.float32)
+        I = np.zeros(n, np.float32)
+        check(lib().nemo_hip_propagate(self._h, _ptr(f, ctypes.c_uint), f.size, _ptr(E, ctypes.c_float),
+                                       _ptr(I, ctypes.c_float), n))
+        return E, I
+
     def prepare(self, nsteps, stdp_period=0):
         """Capture the step graphs run(nsteps, stdp_period) uses (so that a timed
         run excludes graph construction)."""

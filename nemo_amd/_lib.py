@@ -117,11 +117,9 @@ _PROTOTYPES = [
     ("nemo_hip_counts", ctypes.c_int, [vp, c_size_p, c_size_p, c_size_p]),
     ("nemo_hip_profile", ctypes.c_int, [vp, ctypes.c_uint, ctypes.POINTER(ctypes.c_double), c_u64_p]),
     ("nemo_hip_eval_math", ctypes.c_int, [ctypes.c_int, c_float_p, c_float_p, ctypes.c_size_t]),
-    ("hip_simulation", vp, [vp, vp]),
-    ("hip_set_configuration", ctypes.c_int, [vp, ctypes.c_int]),
-    ("hip_device_count", ctypes.c_uint, []),
-    ("hip_device_description", ctypes.c_char_p, [ctypes.c_uint]),
-    ("hip_test_device", ctypes.c_int, [ctypes.c_uint]),
+    ("nemo_hip_propagate", ctypes.c_int, [vp, c_uint_p, ctypes.c_size_t, c_float_p, c_float_p, ctypes.c_size_t]),
+    ("nemo_hip_propagate_device", ctypes.c_int, [vp, vp, ctypes.POINTER(c_float_p), ctypes.POINTER(c_float_p)]),
+    ("nemo_hip_test_device", ctypes.c_int, [ctypes.c_uint]),
 ]
 
 

@@ -80,6 +80,9 @@ def build(verbose=False):
             os.path.join(BUILD, "backend.o"), os.path.join(BUILD, "float_network.o"),
             os.path.join(BUILD, "nemo_c.o"),
             "-L" + os.path.join(ROCM, "lib"), "-lamdhip64", "-fopenmp", "-ldl",
+            # calls between the library's own entry points bind inside it, even
+            # where a process also holds NeMo's libnemo (same nemo_* names)
+            "-Wl,-Bsymbolic",
             "-Wl,-rpath," + os.path.join(ROCM, "lib")]
     _run(link)
     return os.path.join(LIBDIR, "libnemo_hip.so")

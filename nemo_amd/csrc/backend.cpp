@@ -2132,6 +2132,65 @@ void Simulation::applyStdp(float reward)
 	m_fcmHostValid = false;
 }
 
+/* ------------------------------------------------ externally given firing */
+
+/* cuda/Simulation.cpp:328-346 and cpu/Simulation.cpp:115-137: the firing of
+ * step s (the next step) is given; the histories and the step's bitmap are
+ * written from it, the spikes arriving at step s + 1 are delivered and
+ * gathered, and the timer advances by one step.  Defined only without STDP,
+ * as in the CUDA backend. */
+void Simulation::propagateDevice(const uint32_t* flags, float** E, float** I)
+{
+	activate();
+	if(m_nets > 1) {
+		throw Error(NEMO_API_UNSUPPORTED, "propagate drives a single network");
+	}
+	if(m_stdp) {
+		throw Error(NEMO_LOGIC_ERROR, "Brian-specific function propagate only well-defined when STDP is not enabled");
+	}
+	if(!m_devToSim.ptr) {
+		std::vector<uint32_t> d2s(m_npad, ~0u);
+		for(size_t i = 0; i < m_ncount; ++i) d2s[m_order[i]] = uint32_t(i);
+		m_devToSim.allocate(m_npad);
+		HIP_CHECK(hipMemcpy(m_devToSim.ptr, d2s.data(), m_devToSim.bytes(), hipMemcpyHostToDevice));
+		m_extE.allocate(std::max<size_t>(1, m_ncount));
+		m_extI.allocate(std::max<size_t>(1, m_ncount));
+	}
+	joinStdp();
+	const uint64_t s = m_hostStep;
+	setBase(s);
+	nemo_hip_step_buffers b = stepBuffers(0, false, false);
+	b.debug_current = nullptr;
+	HIP_CHECK(nemo_launch_external_fire(m_stream, &b, flags, m_devToSim.ptr, m_tiles));
+	setBase(s + 1);
+	launchDelivery(m_early, 0, m_stream, false);
+	launchDelivery(m_late, 0, m_stream, false);
+	HIP_CHECK(nemo_launch_gather_currents(m_stream, &b, m_devToSim.ptr, m_extE.ptr, m_extI.ptr, m_tiles));
+	m_hostStep = s + 1;
+	HIP_CHECK(hipStreamSynchronize(m_stream));
+	*E = m_extE.ptr;
+	*I = m_extI.ptr;
+}
+
+void Simulation::propagate(const unsigned* fired, size_t nfired, float* E, float* I, size_t n)
+{
+	activate();
+	if(n != m_ncount) throw Error(NEMO_INVALID_INPUT, "current arrays must hold one entry per neuron");
+	std::vector<uint32_t> flags(std::max<size_t>(1, m_ncount), 0u);
+	if(nfired) {
+		std::vector<uint32_t> d2s(m_npad, ~0u);
+		for(size_t i = 0; i < m_ncount; ++i) d2s[m_order[i]] = uint32_t(i);
+		for(size_t k = 0; k < nfired; ++k) flags[d2s[localIdx(fired[k])]] = 1u;
+	}
+	if(m_extFlags.n < flags.size()) m_extFlags.allocate(flags.size());
+	HIP_CHECK(hipMemcpy(m_extFlags.ptr, flags.data(), flags.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+	float* dE = nullptr;
+	float* dI = nullptr;
+	propagateDevice(m_extFlags.ptr, &dE, &dI);
+	HIP_CHECK(hipMemcpy(E, dE, n * sizeof(float), hipMemcpyDeviceToHost));
+	HIP_CHECK(hipMemcpy(I, dI, n * sizeof(float), hipMemcpyDeviceToHost));
+}
+
 /* --------------------------------------------------------- neuron access */
 
 float Simulation::getNeuronState(unsigned idx, unsigned var, unsigned net)

@@ -179,6 +179,14 @@ public:
 	const std::vector<unsigned>& step(const unsigned* fstim, size_t nf,
 			const unsigned* istimIdx, const float* istimVal, size_t ni);
 	void applyStdp(float reward);
+	/* Brian's propagate (Simulation.hpp:93-113): the firing of the next step
+	 * is given (fired neuron indices, or one device flag per neuron in
+	 * simulation order) instead of computed; the step's spikes are delivered
+	 * and the synaptic input of the step after it is returned, E and I in
+	 * simulation order (host arrays of neuronCount() floats, or device
+	 * arrays valid until the next call).  Neuron state is not updated. */
+	void propagate(const unsigned* fired, size_t nfired, float* E, float* I, size_t n);
+	void propagateDevice(const uint32_t* flags, float** E, float** I);
 	/* Where run() writes fired neurons: a caller's buffer while it has room,
 	 * then a vector. */
 	struct FiredSink {
@@ -318,6 +326,10 @@ private:
 	DeviceArray<uint32_t> m_pairBounds, m_binRange;
 	DeviceArray<uint2> m_windows;
 	DeviceArray<unsigned long long> m_blockEvents, m_blockExamined;
+	/* propagate: device index -> simulation index (~0 for padding), the
+	 * given firing, and the input it produces */
+	DeviceArray<uint32_t> m_devToSim, m_extFlags;
+	DeviceArray<float> m_extE, m_extI;
 	std::vector<uint2> m_srcWindow;       /* per source tile: target tiles [x, y) of its synapses */
 	std::vector<uint64_t> m_pairWeight;   /* synapses per (source tile, delay) pair */
 
@@ -409,6 +421,19 @@ public:
 	void forward(hipStream_t st, unsigned T, unsigned K, const float* w, const float* a, const float* b,
 			const float* c, const float* d, const float* v0, const float* u0, const float* x,
 			float* S, float* V, double* ckpt);
+	/* The device memory the forward uses beyond its arguments */
+	struct ForwardWorkspace {
+		float* wr;        /* weights in reverse order */
+		double* v;        /* state */
+		double* u;
+		uint32_t* bits;   /* [T][pitch] spike bitmaps */
+	};
+	size_t forwardWorkspaceBytes(unsigned T) const;
+	ForwardWorkspace carveWorkspace(void* base, unsigned T) const;
+	void forwardWith(const ForwardWorkspace& ws, hipStream_t st, unsigned T, unsigned K, const float* w,
+			const float* a, const float* b, const float* c, const float* d, const float* v0, const float* u0,
+			const float* x, float* S, float* V, double* ckpt);
+	int device() const { return m_device; }
 	void backward(hipStream_t st, unsigned T, unsigned K, const float* w, const float* a, const float* b,
 			const float* c, const float* d, const float* x, const float* S, const double* ckpt,
 			const float* gS, const float* gV, float* gw, float* ga, float* gb, float* gd, float* gx,

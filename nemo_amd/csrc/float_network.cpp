@@ -121,14 +121,57 @@ void FloatNetwork::forward(hipStream_t st, unsigned T, unsigned K, const float* 
 		float* S, float* V, double* ckpt)
 {
 	prepare(T, K);
-	HIP_CHECK(nemo_launch_gather(st, m_wr.ptr, w, m_rcmUser.ptr, m_s));
-	HIP_CHECK(nemo_launch_widen(st, m_v.ptr, v0, m_n));
-	HIP_CHECK(nemo_launch_widen(st, m_u.ptr, u0, m_n));
+	ForwardWorkspace ws;
+	ws.wr = m_wr.ptr;
+	ws.v = m_v.ptr;
+	ws.u = m_u.ptr;
+	ws.bits = m_bits.ptr;
+	forwardWith(ws, st, T, K, w, a, b, c, d, v0, u0, x, S, V, ckpt);
+}
+
+/* Bytes of the forward's workspace for T steps, each part 256-byte aligned:
+ * weights in reverse order (4 B per synapse), v and u in double (16 B per
+ * neuron) and the spike bitmaps of every step. */
+size_t FloatNetwork::forwardWorkspaceBytes(unsigned T) const
+{
+	auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+	return up(std::max<size_t>(1, m_s) * sizeof(float)) + 2 * up(m_n * sizeof(double))
+		+ up(size_t(T) * m_pitch * sizeof(uint32_t));
+}
+
+FloatNetwork::ForwardWorkspace FloatNetwork::carveWorkspace(void* base, unsigned T) const
+{
+	auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+	unsigned char* p = static_cast<unsigned char*>(base);
+	ForwardWorkspace ws;
+	ws.wr = reinterpret_cast<float*>(p);
+	p += up(std::max<size_t>(1, m_s) * sizeof(float));
+	ws.v = reinterpret_cast<double*>(p);
+	p += up(m_n * sizeof(double));
+	ws.u = reinterpret_cast<double*>(p);
+	p += up(m_n * sizeof(double));
+	ws.bits = reinterpret_cast<uint32_t*>(p);
+	(void) T;
+	return ws;
+}
+
+/* The forward's kernels on a given workspace; launches only (nothing is
+ * allocated, freed or waited for), so it may be captured in a graph. */
+void FloatNetwork::forwardWith(const ForwardWorkspace& ws, hipStream_t st, unsigned T, unsigned K, const float* w,
+		const float* a, const float* b, const float* c, const float* d, const float* v0, const float* u0,
+		const float* x, float* S, float* V, double* ckpt)
+{
+	if(T == 0) throw Error(NEMO_INVALID_INPUT, "the float-weight mode needs at least one step");
+	if(K == 0) throw Error(NEMO_INVALID_INPUT, "checkpoint interval must be positive");
+	HIP_CHECK(hipSetDevice(m_device));
+	HIP_CHECK(nemo_launch_gather(st, ws.wr, w, m_rcmUser.ptr, m_s));
+	HIP_CHECK(nemo_launch_widen(st, ws.v, v0, m_n));
+	HIP_CHECK(nemo_launch_widen(st, ws.u, u0, m_n));
 	nemo_float_step_args f = {};
 	f.n = unsigned(m_n);
 	f.rcm_ptr = m_rcmPtr.ptr;
 	f.rcm_sd = m_rcmSd.ptr;
-	f.wr = m_wr.ptr;
+	f.wr = ws.wr;
 	f.pa = a;
 	f.pb = b;
 	f.pc = c;
@@ -136,12 +179,12 @@ void FloatNetwork::forward(hipStream_t st, unsigned T, unsigned K, const float* 
 	f.x = x;
 	f.S = S;
 	f.V = V;
-	f.v = m_v.ptr;
-	f.u = m_u.ptr;
+	f.v = ws.v;
+	f.u = ws.u;
 	f.store = 1;
 	f.ckpt = ckpt;
 	f.K = K;
-	f.bits = m_bits.ptr;
+	f.bits = ws.bits;
 	f.pitch = m_pitch;
 	f.dmax = m_D;
 	f.stage = m_stage;

@@ -74,6 +74,17 @@ hipError_t nemo_launch_apply_stdp(hipStream_t, const nemo_apply_args*);
 hipError_t nemo_launch_invert_rcm(hipStream_t, const uint32_t* rcm_fwd, uint32_t* rcm_of, size_t count,
 		size_t fcm_count);
 
+/* Firing of step base_step given from outside (one flag per neuron in
+ * simulation order; dev_to_sim maps device index -> simulation index, ~0
+ * for padding): histories and the step's bitmap written as by an update.
+ * One block per tile. */
+hipError_t nemo_launch_external_fire(hipStream_t, const nemo_hip_step_buffers* b, const uint32_t* flags,
+		const uint32_t* dev_to_sim, unsigned tiles);
+/* Synaptic input of step base_step gathered per tile (overflow accumulators
+ * cleared) and written as floats in simulation order. */
+hipError_t nemo_launch_gather_currents(hipStream_t, const nemo_hip_step_buffers* b, const uint32_t* dev_to_sim,
+		float* E, float* I, unsigned tiles);
+
 /* Firing stimulus: set (or clear) the bits of the n given device indices. */
 hipError_t nemo_launch_fstim(hipStream_t, uint32_t* bits, const uint32_t* idx, size_t n, int set);
 

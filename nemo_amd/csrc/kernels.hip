@@ -145,6 +145,48 @@ invert_rcm_kernel(const uint32_t* rcm_fwd, uint32_t* rcm_of, size_t count)
 	}
 }
 
+/* ------------------------------------------- externally given firing */
+
+/* The firing of step s given from outside (Brian's propagate,
+ * cpu/Simulation.cpp:115-137): neuron ng fired if flags[sim] is nonzero for
+ * its simulation index sim = dev_to_sim[ng] (~0 for padding); its history
+ * and the step's bitmap word are written as an update would. */
+__global__ void __launch_bounds__(NEMO_TILE)
+external_fire_kernel(nemo_hip_step_buffers b, const uint32_t* flags, const uint32_t* dev_to_sim)
+{
+	const unsigned ng = blockIdx.x * NEMO_TILE + threadIdx.x;
+	const uint32_t sim = dev_to_sim[ng];
+	const bool valid = sim != ~0u;
+	const uint64_t h = nemo_dev::load_history(b, ng, valid);
+	nemo_dev::record_firing(b, ng, valid, valid && flags[sim] != 0u, h);
+}
+
+/* The synaptic input of a step, gathered per tile as the neuron updates
+ * gather it (overflow accumulators cleared), converted to float and written
+ * in simulation order: E[sim], I[sim]. */
+__global__ void __launch_bounds__(NEMO_TILE)
+gather_currents_kernel(nemo_hip_step_buffers b, const uint32_t* dev_to_sim, float* E, float* I)
+{
+	__shared__ nemo_dev::TileInput input;
+	nemo_hip_neuron_group g;
+	g.start = 0;
+	g.count = gridDim.x * NEMO_TILE;
+	g.pitch = 0;
+	g.param = nullptr;
+	g.state = nullptr;
+	g.rng = nullptr;
+	g.scratch = nullptr;
+	nemo_dev::gather_input(b, g, input);
+	const unsigned ng = blockIdx.x * NEMO_TILE + threadIdx.x;
+	float e, i;
+	nemo_dev::take_current(b, input, ng, e, i);
+	const uint32_t sim = dev_to_sim[ng];
+	if(sim != ~0u) {
+		E[sim] = e;
+		I[sim] = i;
+	}
+}
+
 /* ------------------------------------------------------------ stimulus */
 
 __global__ void fstim_kernel(uint32_t* bits, const uint32_t* idx, size_t n, int set)
@@ -273,6 +315,22 @@ hipError_t nemo_launch_invert_rcm(hipStream_t st, const uint32_t* rcm_fwd, uint3
 	size_t blocks = (count + NEMO_BLOCK - 1) / NEMO_BLOCK;
 	if(blocks > 16384) blocks = 16384;
 	hipLaunchKernelGGL(invert_rcm_kernel, dim3(unsigned(blocks)), dim3(NEMO_BLOCK), 0, st, rcm_fwd, rcm_of, count);
+	return hipGetLastError();
+}
+
+hipError_t nemo_launch_external_fire(hipStream_t st, const nemo_hip_step_buffers* b, const uint32_t* flags,
+		const uint32_t* dev_to_sim, unsigned tiles)
+{
+	if(tiles == 0) return hipSuccess;
+	hipLaunchKernelGGL(external_fire_kernel, dim3(tiles), dim3(NEMO_TILE), 0, st, *b, flags, dev_to_sim);
+	return hipGetLastError();
+}
+
+hipError_t nemo_launch_gather_currents(hipStream_t st, const nemo_hip_step_buffers* b, const uint32_t* dev_to_sim,
+		float* E, float* I, unsigned tiles)
+{
+	if(tiles == 0) return hipSuccess;
+	hipLaunchKernelGGL(gather_currents_kernel, dim3(tiles), dim3(NEMO_TILE), 0, st, *b, dev_to_sim, E, I);
 	return hipGetLastError();
 }
 

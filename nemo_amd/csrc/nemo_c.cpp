@@ -328,11 +328,6 @@ nemo_simulation_t nemo_new_simulation(nemo_network_t n, nemo_configuration_t c)
 	return s;
 }
 
-nemo_simulation_t hip_simulation(nemo_network_t n, nemo_configuration_t c)
-{
-	return nemo_new_simulation(n, c);
-}
-
 void nemo_delete_simulation(nemo_simulation_t s) { delete s; }
 
 nemo_status_t nemo_step(nemo_simulation_t s,
@@ -668,28 +663,20 @@ nemo_status_t nemo_hip_eval_math(int fn, const float in[], float out[], size_t n
 	});
 }
 
-/* ------------------------------------------------- backend factory (dlopen) */
-
-nemo_status_t hip_set_configuration(nemo_configuration_t c, int dev)
+nemo_status_t nemo_hip_propagate(nemo_simulation_t s, const unsigned fired[], size_t nfired, float E[], float I[],
+		size_t n)
 {
-	return nemo_set_hip_backend(c, dev);
+	return guarded([&] { check(s, "simulation"); s->impl.propagate(fired, nfired, E, I, n); });
 }
 
-unsigned hip_device_count(void)
+nemo_status_t nemo_hip_propagate_device(nemo_simulation_t s, const uint32_t* fired, float** E, float** I)
 {
-	int n = 0;
-	if(hipGetDeviceCount(&n) != hipSuccess) return 0;
-	return unsigned(n);
+	return guarded([&] { check(s, "simulation"); s->impl.propagateDevice(fired, E, I); });
 }
 
-const char* hip_device_description(unsigned device)
-{
-	const char* d = nullptr;
-	if(nemo_hip_device_description(device, &d) != NEMO_OK) return nullptr;
-	return d;
-}
-
-nemo_status_t hip_test_device(unsigned device)
+/* cuda_test_device (src/nemo/cuda/devices.hpp:33-37; devices.cpp), as a
+ * status: the device exists and accepts an allocation */
+nemo_status_t nemo_hip_test_device(unsigned device)
 {
 	return guarded([&] {
 		int n = 0;

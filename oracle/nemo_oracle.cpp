@@ -167,7 +167,7 @@ void initialiseRng(unsigned minIdx, unsigned maxIdx, std::vector<RNG>& rngs)
 /* neuron types: the .ini files in src/nemo/plugins/                   */
 /* ------------------------------------------------------------------ */
 
-enum Model { IZHIKEVICH, QIF, IF_CURR_EXP, HH, POISSON, INPUT };
+enum Model { IZHIKEVICH, QIF, IF_CURR_EXP, HH, POISSON, INPUT, KURAMOTO };
 
 struct NeuronType {
 	std::string name;
@@ -175,6 +175,8 @@ struct NeuronType {
 	unsigned nParam;
 	unsigned nState;
 	unsigned membranePotential;
+	unsigned history = 1; /* state history length (NeuronType.hpp:67) */
+	bool rcm = false;     /* reads sources, delays and weights of all incoming synapses */
 };
 
 NeuronType lookupType(const std::string& name)
@@ -188,6 +190,8 @@ NeuronType lookupType(const std::string& name)
 	if(name == "HH")            return NeuronType{name, HH, 0, 5, 0};
 	if(name == "PoissonSource") return NeuronType{name, POISSON, 1, 0, 0};
 	if(name == "Input")         return NeuronType{name, INPUT, 0, 0, 0};
+	/* Kuramoto.ini:2-20: history=32, rcm sources/delays/weights */
+	if(name == "Kuramoto")      return NeuronType{name, KURAMOTO, 2, 1, 0, 32, true};
 	throw oracle_error(NEMO_INVALID_INPUT, "unknown neuron type " + name);
 }
 
@@ -327,8 +331,10 @@ struct oracle_simulation {
 		NeuronType type;
 		unsigned base, size;
 		std::vector<float> param; /* [nParam][size] */
-		std::vector<float> state; /* [nState][size], history length 1 */
+		std::vector<float> state; /* [history][nState][size] (cpu/Neurons.cpp:15) */
+		unsigned current = 0;     /* history slot of the current state (cpu/Neurons.cpp:123) */
 		std::vector<RNG> rng;
+		float& at(unsigned var, unsigned i) { return state[(size_t(current) * type.nState + var) * size + i]; }
 	};
 	std::vector<TypeGroup> groups;
 	std::vector<unsigned> typeOfLocal;
@@ -362,6 +368,14 @@ struct oracle_simulation {
 	std::vector<size_t> rcmForward; /* fcm index of the synapse's weight */
 	std::vector<fix_t> rcmAcc;
 
+	/* reverse matrix of all synapses, for models that read presynaptic
+	 * state (construction/RCM.cpp:60-98 with rcm sources/delays/weights):
+	 * per local target, source, delay and float weight in the network's
+	 * synapse order */
+	std::vector<size_t> rcmAllStart;
+	std::vector<uint32_t> rcmAllSource, rcmAllDelay;
+	std::vector<float> rcmAllWeight;
+
 	std::vector<unsigned> firedOut;
 	unsigned long cycle = 0; /* Timer::elapsedSimulation, reset by resetTimer */
 	uint64_t delivered = 0;
@@ -383,6 +397,8 @@ void updateIFCurrExp(oracle_simulation& s, oracle_simulation::TypeGroup& g);
 void updateHH(oracle_simulation& s, oracle_simulation::TypeGroup& g);
 void updatePoisson(oracle_simulation& s, oracle_simulation::TypeGroup& g);
 void updateInput(oracle_simulation& s, oracle_simulation::TypeGroup& g);
+void updateKuramoto(oracle_simulation& s, oracle_simulation::TypeGroup& g);
+void initKuramoto(oracle_simulation::TypeGroup& g);
 
 void build(oracle_simulation& s, const oracle_network& net, const oracle_configuration& conf)
 {
@@ -416,7 +432,7 @@ void build(oracle_simulation& s, const oracle_network& net, const oracle_configu
 		tg.base = l_idx;
 		tg.size = unsigned(ng.gidx.size());
 		tg.param.assign(size_t(tg.type.nParam) * tg.size, 0.0f);
-		tg.state.assign(size_t(tg.type.nState) * tg.size, 0.0f);
+		tg.state.assign(size_t(tg.type.history) * tg.type.nState * tg.size, 0.0f);
 		for(unsigned i = 0; i < tg.size; ++i) {
 			unsigned sim = tg.base + i;
 			if(!s.g2l.emplace(ng.gidx[i], sim).second) {
@@ -429,6 +445,7 @@ void build(oracle_simulation& s, const oracle_network& net, const oracle_configu
 		}
 		tg.rng.resize(tg.size);
 		initialiseRng(tg.base, tg.base + tg.size - 1, tg.rng);
+		if(tg.type.model == KURAMOTO) initKuramoto(tg);
 		l_idx += tg.size;
 		s.groups.push_back(std::move(tg));
 	}
@@ -455,6 +472,15 @@ void build(oracle_simulation& s, const oracle_network& net, const oracle_configu
 	size_t nrows = ncount * D;
 	std::vector<size_t> rowCount(nrows + 1, 0);
 	std::vector<unsigned> rcmCount(ncount + 1, 0);
+	/* construction/RCM.cpp:83-91: STDP and a model that reads the reverse
+	 * matrix do not mix */
+	bool rcmAll = false;
+	for(const auto& g : s.groups) rcmAll = rcmAll || g.type.rcm;
+	if(rcmAll && conf.stdp) {
+		throw oracle_error(NEMO_API_UNSUPPORTED, "The current version does not support a mix of STDP with neuron "
+				"types which require reverse connectivity for other purposes");
+	}
+	std::vector<unsigned> rcmAllCount(ncount + 1, 0);
 
 	/* pass 1: validate, count rows */
 	for(auto& kv : net.fcm) {
@@ -464,8 +490,18 @@ void build(oracle_simulation& s, const oracle_network& net, const oracle_configu
 			unsigned tgt = s.localIdx(ax.target[k]);
 			rowCount[size_t(src) * D + ax.delay[k] - 1] += 1;
 			if(conf.stdp && ax.plastic[k]) rcmCount[tgt] += 1;
+			if(rcmAll) rcmAllCount[tgt] += 1;
 		}
 	}
+	if(rcmAll) {
+		s.rcmAllStart.assign(ncount + 1, 0);
+		for(size_t n = 0; n < ncount; ++n) s.rcmAllStart[n + 1] = s.rcmAllStart[n] + rcmAllCount[n];
+		s.rcmAllSource.resize(s.rcmAllStart[ncount]);
+		s.rcmAllDelay.resize(s.rcmAllStart[ncount]);
+		s.rcmAllWeight.resize(s.rcmAllStart[ncount]);
+	}
+	std::vector<size_t> rcmAllFill(s.rcmAllStart.begin(),
+			s.rcmAllStart.empty() ? s.rcmAllStart.begin() : s.rcmAllStart.end() - 1);
 	s.rowStart.assign(nrows + 1, 0);
 	for(size_t r = 0; r < nrows; ++r) s.rowStart[r + 1] = s.rowStart[r] + rowCount[r];
 	s.fcm.resize(net.synapseCount);
@@ -506,6 +542,12 @@ void build(oracle_simulation& s, const oracle_network& net, const oracle_configu
 				s.rcmSource[r] = src;
 				s.rcmDelay[r] = d;
 				s.rcmForward[r] = addr;
+			}
+			if(rcmAll) {
+				size_t r = rcmAllFill[tgt]++;
+				s.rcmAllSource[r] = src;
+				s.rcmAllDelay[r] = d;
+				s.rcmAllWeight[r] = ax.weight[k];
 			}
 		}
 	}
@@ -598,6 +640,7 @@ void fire(oracle_simulation& s)
 			case HH: updateHH(s, g); break;
 			case POISSON: updatePoisson(s, g); break;
 			case INPUT: updateInput(s, g); break;
+			case KURAMOTO: updateKuramoto(s, g); break;
 		}
 	}
 	accumulateStdp(s);
@@ -855,6 +898,78 @@ void updateInput(oracle_simulation& s, oracle_simulation::TypeGroup& g)
 	}
 }
 
+/* cpu/plugins/Kuramoto.cpp:14-134 (parameters plugins/Kuramoto.h: frequency,
+ * mean coupling; state: the phase, 32 steps of history).  The phase of each
+ * incoming synapse's source is read d-1 steps back; per-type arrays indexed
+ * locally (deviation 1), every source being an oscillator of the same
+ * group.  The source phase in the reference is taken from slot
+ * (cycle - (delay-1)) mod 32 of the unsigned cycle. */
+void updateKuramoto(oracle_simulation& s, oracle_simulation::TypeGroup& g)
+{
+	const unsigned H = g.type.history;
+	const unsigned cycle = unsigned(s.cycle);
+	const size_t stride = size_t(g.type.nState) * g.size; /* one history slot */
+	const float* frequency = &g.param[0];
+	const float* cmean = &g.param[g.size];
+	const float* phase0 = &g.state[size_t(cycle % H) * stride];
+	float* phase1 = &g.state[size_t((cycle + 1) % H) * stride];
+	g.current = (cycle + 1) % H; /* cpu/Neurons.cpp:123 */
+	std::vector<float> weight, sourcePhase;
+	for(unsigned nl = 0; nl < g.size; ++nl) {
+		const unsigned n = g.base + nl;
+		const float h = 0.05f;
+		const float f = frequency[nl];
+		float targetPhase = phase0[nl];
+		/* loadIncoming, Kuramoto.cpp:52-78 */
+		const size_t r0 = s.rcmAllStart[n], r1 = s.rcmAllStart[n + 1];
+		weight.resize(r1 - r0);
+		sourcePhase.resize(r1 - r0);
+		for(size_t r = r0; r < r1; ++r) {
+			const unsigned src = s.rcmAllSource[r];
+			if(src < g.base || src >= g.base + g.size) {
+				throw oracle_error(NEMO_API_UNSUPPORTED, "a Kuramoto oscillator's sources must be oscillators of its type");
+			}
+			weight[r - r0] = s.rcmAllWeight[r];
+			const unsigned slot = (cycle - unsigned(int(s.rcmAllDelay[r] - 1))) % H;
+			sourcePhase[r - r0] = g.state[size_t(slot) * stride + (src - g.base)];
+		}
+		/* sumN, Kuramoto.cpp:22-34 */
+		auto sumN = [&](float target) {
+			float sum = 0.0f;
+			for(size_t k = 0; k < weight.size(); ++k) sum += weight[k] * sinf(sourcePhase[k] - target);
+			return sum;
+		};
+		const float Cmean = cmean[nl] > 0 ? cmean[nl] : 1;
+		const float k0 = f + (sumN(targetPhase) / Cmean);
+		const float k1 = f + (sumN(targetPhase + k0 * 0.5f * h) / Cmean);
+		const float k2 = f + (sumN(targetPhase + k1 * 0.5f * h) / Cmean);
+		const float k3 = f + (sumN(targetPhase + k2 * h) / Cmean);
+		targetPhase += h * (k0 + 2 * k1 + 2 * k2 + k3) / 6.0f;
+		/* float fmodf with 2*pi rounded to float, then a double add */
+		phase1[nl] = float(double(fmodf(targetPhase, float(2.0f * M_PI)))
+				+ (targetPhase < 0.0f ? 2.0f * M_PI : 0.0f));
+	}
+}
+
+/* cpu/plugins/Kuramoto.cpp:137-159: the history before the first step, by
+ * running each oscillator backwards one frequency (not h times it) per step
+ * without coupling, from slot 0 into slots 31, 30, .., 1 */
+void initKuramoto(oracle_simulation::TypeGroup& g)
+{
+	const unsigned H = g.type.history;
+	const size_t stride = size_t(g.type.nState) * g.size;
+	for(unsigned t = 0; t + 1 < H; ++t) {
+		const unsigned current = 0u - t;
+		const unsigned previous = 0u - (t + 1u);
+		const float* phase0 = &g.state[size_t(current % H) * stride];
+		float* phase1 = &g.state[size_t(previous % H) * stride];
+		for(unsigned nl = 0; nl < g.size; ++nl) {
+			const float phase = phase0[nl] - g.param[nl];
+			phase1[nl] = float(double(fmodf(phase, float(2.0f * M_PI))) + (phase < 0.0f ? 2.0f * M_PI : 0.0f));
+		}
+	}
+}
+
 template<typename F>
 int guarded(F f)
 {
@@ -1046,7 +1161,7 @@ int oracle_get_membrane_potential(oracle_simulation_t s, unsigned neuron, float*
 		if(g.type.membranePotential >= g.type.nState) {
 			throw oracle_error(NEMO_INVALID_INPUT, "neuron type has no membrane potential");
 		}
-		*v = g.state[size_t(g.type.membranePotential) * g.size + (l - g.base)];
+		*v = g.at(g.type.membranePotential, l - g.base);
 	});
 }
 
@@ -1056,7 +1171,7 @@ int oracle_get_neuron_state(oracle_simulation_t s, unsigned neuron, unsigned var
 		unsigned l = s->localIdx(neuron);
 		auto& g = groupOf(s, l);
 		if(var >= g.type.nState) throw oracle_error(NEMO_INVALID_INPUT, "Invalid state variable index");
-		*val = g.state[size_t(var) * g.size + (l - g.base)];
+		*val = g.at(var, l - g.base);
 	});
 }
 
@@ -1076,7 +1191,7 @@ int oracle_set_neuron_state(oracle_simulation_t s, unsigned neuron, unsigned var
 		unsigned l = s->localIdx(neuron);
 		auto& g = groupOf(s, l);
 		if(var >= g.type.nState) throw oracle_error(NEMO_INVALID_INPUT, "Invalid state variable index");
-		g.state[size_t(var) * g.size + (l - g.base)] = val;
+		g.at(var, l - g.base) = val;
 	});
 }
 
@@ -1242,6 +1357,37 @@ float oracle_nrand(uint32_t* state4)
 	float v = nrand(&r);
 	std::memcpy(state4, r.state, 16);
 	return v;
+}
+
+/* state variable var of every neuron in local order (NaN where its model has
+ * fewer) */
+int oracle_get_neuron_states(oracle_simulation_t s, unsigned var, float* out, size_t n)
+{
+	return guarded([&] {
+		if(n != s->ncount) throw oracle_error(NEMO_INVALID_INPUT, "size mismatch");
+		for(size_t l = 0; l < n; ++l) {
+			oracle_simulation::TypeGroup& g = groupOf(s, unsigned(l));
+			out[l] = var < g.type.nState ? g.at(var, unsigned(l - g.base)) : std::nanf("");
+		}
+	});
+}
+
+/* cpu/Simulation.cpp:115-137 (propagate_raw, NEMO_BRIAN_ENABLED): every
+ * history shifts by one, the given neurons (local indices) fire, and the
+ * spikes arriving now are delivered; the timer advances. */
+int oracle_propagate(oracle_simulation_t s, const unsigned* fired, size_t nfired, const float** E, const float** I)
+{
+	return guarded([&] {
+		for(size_t n = 0; n < s->ncount; ++n) s->recentFiring[n] <<= 1;
+		for(size_t i = 0; i < nfired; ++i) {
+			if(fired[i] >= s->ncount) throw oracle_error(NEMO_INVALID_INPUT, "propagate: neuron index out of range");
+			s->recentFiring[fired[i]] |= uint64_t(1);
+		}
+		deliverSpikes(*s);
+		s->cycle += 1;
+		*E = s->currentE.data();
+		*I = s->currentI.data();
+	});
 }
 
 void oracle_nrand_n(uint32_t* states, size_t n, float* out)

@@ -86,6 +86,14 @@ uint64_t oracle_delivered_events(oracle_simulation_t);
 /* OpenMP threads the oracle will use */
 int oracle_num_threads(void);
 
+/* state variable var of every neuron in local order (NaN if the model has fewer) */
+int oracle_get_neuron_states(oracle_simulation_t, unsigned var, float* out, size_t n);
+
+/* --- Brian's propagate (src/nemo/cpu/Simulation.cpp:115-137) --- */
+/* the neurons with the given local indices fire; E and I point at the input
+ * delivered now, in local order (valid until the next call) */
+int oracle_propagate(oracle_simulation_t, const unsigned* fired, size_t nfired, const float** E, const float** I);
+
 /* --- RNG primitives (src/nemo/RNG.cpp:9-63) --- */
 void oracle_rng_seeds(unsigned first, unsigned last, uint32_t* out /* 4*(last-first+1) */);
 uint32_t oracle_urand(uint32_t* state4);

@@ -76,6 +76,9 @@ def lib():
             ("oracle_urand", ctypes.c_uint32, [P(ctypes.c_uint32)]),
             ("oracle_nrand", ctypes.c_float, [P(ctypes.c_uint32)]),
             ("oracle_nrand_n", None, [P(ctypes.c_uint32), ctypes.c_size_t, P(ctypes.c_float)]),
+            ("oracle_get_neuron_states", ctypes.c_int, [vp, ctypes.c_uint, P(ctypes.c_float), ctypes.c_size_t]),
+            ("oracle_propagate", ctypes.c_int, [vp, P(ctypes.c_uint), ctypes.c_size_t, P(P(ctypes.c_float)),
+                                                P(P(ctypes.c_float))]),
             ("oracle_mt19937_nth", ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint]),
             ("oracle_fx_tofix", ctypes.c_int, [ctypes.c_float, ctypes.c_uint, P(ctypes.c_int32)]),
             ("oracle_wfx_tofloat", ctypes.c_float, [ctypes.c_int64, ctypes.c_uint]),
@@ -297,6 +300,24 @@ class Simulation:
 
     def delivered_events(self):
         return lib().oracle_delivered_events(self._h)
+
+    def neuron_states(self, var):
+        out = np.zeros(self.neuron_count, np.float32)
+        check(lib().oracle_get_neuron_states(self._h, var, _ptr(out, ctypes.c_float), out.size))
+        return out
+
+    def propagate(self, fired):
+        """cpu/Simulation.cpp:115-137 with the fired neurons given by neuron
+        index (mapped to local indices here); returns copies of E and I in
+        local order."""
+        if not hasattr(self, "_g2l"):
+            g = self.local_to_global()
+            self._g2l = {int(x): i for i, x in enumerate(g)}
+        f = np.ascontiguousarray([self._g2l[int(x)] for x in fired], dtype=np.uint32)
+        e, i = P(ctypes.c_float)(), P(ctypes.c_float)()
+        check(lib().oracle_propagate(self._h, _ptr(f, ctypes.c_uint), f.size, ctypes.byref(e), ctypes.byref(i)))
+        n = self.neuron_count
+        return (np.ctypeslib.as_array(e, shape=(n,)).copy(), np.ctypeslib.as_array(i, shape=(n,)).copy())
 
 
 def rng_seeds(first, last):

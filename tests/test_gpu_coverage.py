@@ -61,8 +61,9 @@ def test_full_size_stdp_network_bit_exact():
     excitatory synapses: 1000 steps with applyStdp(1.0) after step 1000, then
     50 more steps, in the HIP backend (graph runs) and in the oracle stepped
     one step at a time.  Counts, fired lists, events, every weight after
-    applyStdp, every STDP accumulator, u and v of every neuron, firing
-    histories and RNG states are bit-exact."""
+    applyStdp, every STDP accumulator, u and v of every neuron (including the
+    three the reference dynamics drive to NaN), firing histories and RNG
+    states are bit-exact."""
     spec = gen.random_izhikevich(100000, 1000, dmax=20, seed=2, stdp=True)
     ref = sc.make(oa, spec, conf_of(oa, True))
     hip = sc.make(nemo_amd, spec, conf_of(nemo_amd, True))
@@ -86,10 +87,9 @@ def test_full_size_stdp_network_bit_exact():
     assert np.array_equal(ref.recent_firing(), hip.recent_firing())
     assert np.array_equal(ref.rng_state(), hip.rng_state())
     for var in (0, 1):
-        h = hip.neuron_states(var)
-        r = np.array([ref.get_neuron_state(i, var) for i in range(0, 100000, 97)], np.float32)
-        assert same_f32(r, h[::97]), var
-        assert np.all(np.isfinite(h))
+        # every neuron; a few (3 of 100k) diverge to NaN under the reference
+        # dynamics, and do so here too
+        assert same_f32(ref.neuron_states(var), hip.neuron_states(var)), var
 
 
 # -------------------------------------------------------------- C4 batches
@@ -428,3 +428,72 @@ def test_lif_create_and_run(multiple):
         sims.append(sim)
     for var in range(4):
         assert same_f32(sims[0].get_neuron_state(2000, var), sims[1].get_neuron_state(2000, var))
+
+
+# ------------------------------------------------ Brian's propagate (F4)
+def _hip_runtime():
+    """hipMemcpy of the HIP runtime the process already uses (torch's, which
+    libnemo_hip.so binds to by soname): loading the library by soname
+    reuses it.  A second runtime in the process would not know the first's
+    device pointers, so the test refuses to go on if two are mapped."""
+    import ctypes
+    from nemo_amd._lib import lib
+    lib()
+    rt = ctypes.CDLL("libamdhip64.so.7")
+    runtimes = {l.split()[-1] for l in open("/proc/self/maps") if "libamdhip64" in l}
+    assert len(runtimes) == 1, runtimes
+    rt.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    return rt
+
+
+def test_propagate_matches_oracle():
+    """Brian's propagate (Simulation.hpp:93-113) on a 3000-neuron network
+    after 50 ordinary steps: 100 calls with random fired sets return the
+    oracle's E and I bit for bit (host form, then the device form with a
+    device flag array), leave the neuron state alone, and ordinary steps
+    afterwards stay identical to the oracle's."""
+    import ctypes
+    from nemo_amd._lib import check, lib
+    spec = gen.random_izhikevich(3000, 100, dmax=20, seed=51)
+    ref = sc.make(oa, spec)
+    hip = sc.make(nemo_amd, spec)
+    for t in range(50):
+        assert ref.step() == hip.step(), t
+    v_before = hip.neuron_states(1)
+    rng = np.random.default_rng(51)
+    for j in range(60):
+        F = np.sort(rng.choice(3000, int(rng.integers(0, 300)), replace=False)).astype(np.uint32)
+        er, ir = ref.propagate(F)
+        eh, ih = hip.propagate(F)
+        assert same_f32(er, eh) and same_f32(ir, ih), j
+    assert same_f32(v_before, hip.neuron_states(1))
+    import torch
+    rt = _hip_runtime()
+    n = hip.neuron_count
+    l2g = hip.local_to_global()
+    for j in range(40):
+        F = np.sort(rng.choice(3000, int(rng.integers(0, 300)), replace=False)).astype(np.uint32)
+        flags = torch.tensor(np.isin(l2g, F).astype(np.int32), device="cuda:0")
+        torch.cuda.synchronize()
+        pe, pi = ctypes.POINTER(ctypes.c_float)(), ctypes.POINTER(ctypes.c_float)()
+        check(lib().nemo_hip_propagate_device(hip._h, ctypes.c_void_p(flags.data_ptr()), ctypes.byref(pe),
+                                              ctypes.byref(pi)))
+        eh, ih = np.zeros(n, np.float32), np.zeros(n, np.float32)
+        assert rt.hipMemcpy(eh.ctypes.data, ctypes.cast(pe, ctypes.c_void_p), 4 * n, 2) == 0  # device to host
+        assert rt.hipMemcpy(ih.ctypes.data, ctypes.cast(pi, ctypes.c_void_p), 4 * n, 2) == 0
+        er, ir = ref.propagate(F)
+        assert same_f32(er, eh) and same_f32(ir, ih), ("device", j)
+    assert ref.elapsed_simulation() == hip.elapsed_simulation()
+    for t in range(100):
+        assert ref.step() == hip.step(), ("after", t)
+    for var in (0, 1):
+        r = np.array([ref.get_neuron_state(i, var) for i in range(3000)], np.float32)
+        assert same_f32(r, hip.neuron_states(var))
+
+
+def test_propagate_refused_with_stdp():
+    """The CUDA backend asserts that STDP is off (cuda/Simulation.cpp:332)."""
+    spec = gen.random_izhikevich(500, 20, dmax=10, seed=52, stdp=True)
+    hip = sc.make(nemo_amd, spec, conf_of(nemo_amd, True))
+    with pytest.raises(nemo_amd.NemoError):
+        hip.propagate([1, 2, 3])

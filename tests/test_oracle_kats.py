@@ -261,3 +261,41 @@ def test_random_network_events_match_raster():
     expected = sum(int(rowlen[f, d]) for f, s in zip(fired, step_of) for d in range(1, D + 1) if s + d < steps)
     assert events == expected
     assert counts.sum() > 0
+
+
+def test_propagate_brute_force():
+    """Brian's propagate (cpu/Simulation.cpp:115-137): call j makes the given
+    set F_j fire and returns the input of the step after it, so neuron t
+    receives the sum of w over synapses (s, t, d, w) with s in F_{j+1-d},
+    excitatory and inhibitory apart.  Checked against that sum computed by
+    brute force on a small random network with delays up to 5, and the
+    neuron state is left alone."""
+    rng = np.random.default_rng(12)
+    n, m, D = 40, 6, 5
+    net = oa.Network()
+    t = net.add_neuron_type("Izhikevich")
+    net.add_neuron(t, np.arange(n, dtype=np.uint32), 0.02, 0.2, -65.0, 8.0, 0.0, -13.0, -65.0)
+    src = np.repeat(np.arange(n), m)
+    tgt = rng.integers(0, n, src.size)
+    dly = rng.integers(1, D + 1, src.size)
+    w = np.round(rng.uniform(-2, 2, src.size) * 1024) / 1024  # exact in Q11.20
+    net.add_synapse(src.astype(np.uint32), tgt.astype(np.uint32), dly.astype(np.uint32), w.astype(np.float32), 0)
+    sim = oa.Simulation(net, oa.Configuration())
+    v0 = [sim.get_neuron_state(i, 1) for i in range(n)]
+    fired = []
+    for j in range(30):
+        F = sorted(rng.choice(n, rng.integers(0, 8), replace=False).tolist())
+        fired.append(set(F))
+        E, I = sim.propagate(F)
+        eE, eI = np.zeros(n), np.zeros(n)
+        for s, tt, d, ww in zip(src, tgt, dly, w):
+            k = j + 1 - d
+            if k >= 0 and s in fired[k]:
+                if ww >= 0:
+                    eE[tt] += ww
+                else:
+                    eI[tt] += ww
+        assert np.array_equal(E, eE.astype(np.float32)), j
+        assert np.array_equal(I, eI.astype(np.float32)), j
+    assert [sim.get_neuron_state(i, 1) for i in range(n)] == v0
+    assert sim.elapsed_simulation() == 30

@@ -288,3 +288,127 @@ def test_poisson_rate(duration, mix):
 @pytest.mark.parametrize("ncount,iz", [(1, False), (1000, False), (1, True), (1000, True)])
 def test_input_fires_exactly_as_stimulated(ncount, iz):
     sc.input_neurons(oa, ncount, iz)
+
+
+# --------------------------------------------------------------- Kuramoto
+TWO_PI32 = float(np.float32(2 * math.pi))
+
+
+def kuramoto_net(api, freq, phase, cmean=0.0, synapses=()):
+    net = api.Network()
+    t = net.add_neuron_type("Kuramoto")
+    for i, (f, p) in enumerate(zip(freq, phase)):
+        net.add_neuron(t, i, f, cmean, p)
+    for src, tgt, d, w in synapses:
+        net.add_synapse(src, tgt, d, w, 0)
+    return api.Simulation(net, api.Configuration())
+
+
+def kuramoto_fp64(freq, phase0, synapses, steps, cmean=0.0, h=0.05, H=32):
+    """Independent double-precision model of cpu/plugins/Kuramoto.cpp: the
+    history before step 0 runs each oscillator back one frequency per step
+    (Kuramoto.cpp:137-159); in step c, oscillator j takes one RK4 step of h
+    of theta' = w_j + (1/C) sum_k W_kj sin(theta_k(c - d_kj + 1) - theta),
+    the source phases held at their history values, then wraps into
+    [0, 2 pi)."""
+    n = len(freq)
+    hist = {0: np.array(phase0, np.float64)}
+    for t in range(1, H):
+        hist[-t] = np.mod(hist[-t + 1] - np.array(freq), 2 * math.pi)
+    C = cmean if cmean > 0 else 1.0
+    out = []
+    for c in range(steps):
+        cur = hist[c]
+        nxt = np.empty(n)
+        for j in range(n):
+            inc = [(hist[c - (d - 1)][k], w) for k, tgt, d, w in synapses if tgt == j]
+
+            def k_of(theta):
+                return freq[j] + sum(w * math.sin(p - theta) for p, w in inc) / C
+            k0 = k_of(cur[j])
+            k1 = k_of(cur[j] + 0.5 * h * k0)
+            k2 = k_of(cur[j] + 0.5 * h * k1)
+            k3 = k_of(cur[j] + h * k2)
+            nxt[j] = math.fmod(cur[j] + h * (k0 + 2 * k1 + 2 * k2 + k3) / 6, 2 * math.pi) % (2 * math.pi)
+        hist[c + 1] = nxt
+        out.append(nxt)
+    return np.array(out)
+
+
+def wrapped_diff(a, b):
+    d = np.abs(np.asarray(a, np.float64) - np.asarray(b, np.float64))
+    return np.minimum(d, 2 * math.pi - d)
+
+
+def test_kuramoto_uncoupled_advance_by_h_omega():
+    """Oscillators without coupling advance by h * omega = 0.05 omega per
+    step (the step of cpu/plugins/Kuramoto.cpp).  NeMo's own tests
+    (src/test/Kuramoto.cpp:48-82) expect a whole omega per step, which no
+    NeMo plugin does; this checks the plugins' behaviour.  Catches a step
+    other than h, a dropped RK4 weight, or a wrap other than into
+    [0, 2 pi)."""
+    freq = [0.1, 1.0, 7.0, 0.0]
+    phase = [0.0, 3.0, 6.0, 1.0]
+    sim = kuramoto_net(oa, freq, phase)
+    assert [sim.get_neuron_state(i, 0) for i in range(4)] == [np.float32(p) for p in phase]
+    got = []
+    for _ in range(500):
+        sim.step()
+        got.append([sim.get_neuron_state(i, 0) for i in range(4)])
+    ref = kuramoto_fp64(freq, phase, [], 500)
+    assert np.max(wrapped_diff(got, ref)) < 2e-4
+    assert np.all((np.array(got) >= 0) & (np.array(got) < TWO_PI32))
+    assert abs(got[-1][0] - 500 * 0.05 * 0.1) < 1e-4
+
+
+@pytest.mark.parametrize("delay", [1, 3, 20])
+def test_kuramoto_delayed_coupling(delay):
+    """Two oscillators, 0 -> 1 with strength 1 (the coupling of
+    src/test/Kuramoto.cpp:85-124, at h 0.05): oscillator 1 follows the
+    double-precision model within 1e-3 rad over 200 steps, including the
+    first steps, where a delay > 1 reads the history filled backwards at
+    construction.  Catches sin(target - source), a source phase read from
+    the wrong history slot, or a history filled with h omega."""
+    syn = [(0, 1, delay, 1.0)]
+    sim = kuramoto_net(oa, [0.1, 0.1], [0.0, 1.57], synapses=syn)
+    got = []
+    for _ in range(200):
+        sim.step()
+        got.append([sim.get_neuron_state(i, 0) for i in range(2)])
+    ref = kuramoto_fp64([0.1, 0.1], [0.0, 1.57], syn, 200)
+    assert np.max(wrapped_diff(got, ref)) < 1e-3
+    # the coupling pulls oscillator 1 towards oscillator 0
+    assert wrapped_diff(got[-1][1], got[-1][0]) < wrapped_diff(1.57, 0.0)
+
+
+def test_kuramoto_n_to_1_with_mean_coupling():
+    """src/test/Kuramoto.cpp:139-185 at h 0.05, with noise connections of
+    strength 0 and the coupling divided by Cmean = 2: the target follows the
+    double-precision model."""
+    n = 20
+    freq = [0.1] * (n + 1)
+    phase = [0.0] + [1.0] * n
+    syn = [(k + 1, 0, 1, 1.0 / n) for k in range(n)]
+    syn += [(k + 1, t, 2, 0.0) for k in range(n) for t in range(1, n + 1) if t != k + 1][:100]
+    sim = kuramoto_net(oa, freq, phase, cmean=2.0, synapses=syn)
+    got = []
+    for _ in range(50):
+        sim.step()
+        got.append([sim.get_neuron_state(i, 0) for i in (0, 1)])
+    ref = kuramoto_fp64(freq, phase, syn, 50, cmean=2.0)[:, :2]
+    assert np.max(wrapped_diff(got, ref)) < 1e-4
+
+
+def test_kuramoto_refused_with_stdp():
+    """construction/RCM.cpp:83-91"""
+    net = oa.Network()
+    t = net.add_neuron_type("Kuramoto")
+    net.add_neuron(t, 0, 0.1, 0.0, 0.0)
+    net.add_neuron(t, 1, 0.1, 0.0, 0.0)
+    net.add_synapse(0, 1, 1, 1.0, 1)
+    conf = oa.Configuration()
+    pre, post = gen.stdp_exp_function(0.1, -0.12, 20.0, 20)
+    conf.set_stdp_function(pre, post, 0.0, 10.0, 0.0, -10.0)
+    with pytest.raises(oa.OracleError) as e:
+        oa.Simulation(net, conf)
+    assert e.value.code == 5
