@@ -147,19 +147,34 @@ NEMO_HD float sinf_exact(float y)
 
 /* ---------------------------------------------------------------- expf */
 
-NEMO_HD float expf_exact(float x)
+/* The published expf's table: the bit patterns of 2^(i/32) with i << 47
+ * subtracted, i < 32. */
+#define NEMO_EXPF_TABLE { \
+		0x3ff0000000000000ULL, 0x3fefd9b0d3158574ULL, 0x3fefb5586cf9890fULL, 0x3fef9301d0125b51ULL, \
+		0x3fef72b83c7d517bULL, 0x3fef54873168b9aaULL, 0x3fef387a6e756238ULL, 0x3fef1e9df51fdee1ULL, \
+		0x3fef06fe0a31b715ULL, 0x3feef1a7373aa9cbULL, 0x3feedea64c123422ULL, 0x3feece086061892dULL, \
+		0x3feebfdad5362a27ULL, 0x3feeb42b569d4f82ULL, 0x3feeab07dd485429ULL, 0x3feea47eb03a5585ULL, \
+		0x3feea09e667f3bcdULL, 0x3fee9f75e8ec5f74ULL, 0x3feea11473eb0187ULL, 0x3feea589994cce13ULL, \
+		0x3feeace5422aa0dbULL, 0x3feeb737b0cdc5e5ULL, 0x3feec49182a3f090ULL, 0x3feed503b23e255dULL, \
+		0x3feee89f995ad3adULL, 0x3feeff76f2fb5e47ULL, 0x3fef199bdd85529cULL, 0x3fef3720dcef9069ULL, \
+		0x3fef5818dcfba487ULL, 0x3fef7c97337b9b5fULL, 0x3fefa4afa2a490daULL, 0x3fefd0765b6e4540ULL, \
+	}
+
+/* Entry i of the table (i < 32). */
+NEMO_HD uint64_t expf_table(unsigned i)
 {
-	/* bit patterns of 2^(i/32) with i << 47 subtracted */
-	const uint64_t T[32] = {
-		0x3ff0000000000000ULL, 0x3fefd9b0d3158574ULL, 0x3fefb5586cf9890fULL, 0x3fef9301d0125b51ULL,
-		0x3fef72b83c7d517bULL, 0x3fef54873168b9aaULL, 0x3fef387a6e756238ULL, 0x3fef1e9df51fdee1ULL,
-		0x3fef06fe0a31b715ULL, 0x3feef1a7373aa9cbULL, 0x3feedea64c123422ULL, 0x3feece086061892dULL,
-		0x3feebfdad5362a27ULL, 0x3feeb42b569d4f82ULL, 0x3feeab07dd485429ULL, 0x3feea47eb03a5585ULL,
-		0x3feea09e667f3bcdULL, 0x3fee9f75e8ec5f74ULL, 0x3feea11473eb0187ULL, 0x3feea589994cce13ULL,
-		0x3feeace5422aa0dbULL, 0x3feeb737b0cdc5e5ULL, 0x3feec49182a3f090ULL, 0x3feed503b23e255dULL,
-		0x3feee89f995ad3adULL, 0x3feeff76f2fb5e47ULL, 0x3fef199bdd85529cULL, 0x3fef3720dcef9069ULL,
-		0x3fef5818dcfba487ULL, 0x3fef7c97337b9b5fULL, 0x3fefa4afa2a490daULL, 0x3fefd0765b6e4540ULL,
-	};
+	const uint64_t T[32] = NEMO_EXPF_TABLE;
+	return T[i % 32];
+}
+
+/* expf with the table at tab (tab[i] == expf_table(i); a caller may keep it
+ * in LDS).  The special cases of the published routine (|x| >= 88: -inf
+ * gives 0, NaN and +inf give x + x, overflow inf, underflow 0) are selected
+ * after the main path instead of branched around it: the main path is
+ * harmless for those inputs (any table index is in range), and a wave whose
+ * lanes take different cases stays on one instruction stream. */
+NEMO_HD float expf_exact_tab(float x, const uint64_t* tab)
+{
 	const double InvLn2N = 0x1.71547652b82fep+0 * 32;
 	const double SHIFT = 0x1.8p+52;
 	const double C0 = 0x1.c6af84b912394p-5 / 32 / 32 / 32;
@@ -167,18 +182,11 @@ NEMO_HD float expf_exact(float x)
 	const double C2 = 0x1.62e42ff0c52d6p-1 / 32;
 
 	double xd = double(x);
-	uint32_t abstop = (asuint(x) >> 20) & 0x7ff;
-	if(abstop >= (asuint(88.0f) >> 20)) {
-		if(asuint(x) == asuint(-__builtin_inff())) return 0.0f;
-		if(abstop >= (asuint(__builtin_inff()) >> 20)) return x + x;
-		if(x > 0x1.62e42ep6f) return __builtin_inff();
-		if(x < -0x1.9fe368p6f) return 0.0f;
-	}
 	double kd = fma_d(InvLn2N, xd, SHIFT);
 	uint64_t ki = asuint64(kd);
 	kd -= SHIFT;
 	double r = fma_d(InvLn2N, xd, -kd);
-	uint64_t t = T[ki % 32];
+	uint64_t t = tab[ki % 32];
 	t += ki << (52 - 5);
 	double s = asdouble(t);
 	double z = fma_d(C0, r, C1);
@@ -186,7 +194,20 @@ NEMO_HD float expf_exact(float x)
 	double y = fma_d(C2, r, 1.0);
 	y = fma_d(z, r2, y);
 	y = y * s;
-	return float(y);
+	const float main = float(y);
+
+	const uint32_t abstop = (asuint(x) >> 20) & 0x7ff;
+	float special = main;
+	special = x < -0x1.9fe368p6f ? 0.0f : special;          /* underflow, and -inf */
+	special = x > 0x1.62e42ep6f ? __builtin_inff() : special; /* overflow, and +inf (= x + x) */
+	special = x != x ? x + x : special;                      /* NaN */
+	return abstop >= (asuint(88.0f) >> 20) ? special : main;
+}
+
+NEMO_HD float expf_exact(float x)
+{
+	const uint64_t T[32] = NEMO_EXPF_TABLE;
+	return expf_exact_tab(x, T);
 }
 
 } // namespace nemo_math

@@ -143,10 +143,20 @@ hh_input(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
  * 15 repeat betah) from exactly the expression of the single launch, and a
  * DPP row broadcast hands each rate to the whole row (the neuron); every
  * lane then applies the same update, so all lanes of a neuron hold the same
- * state and the first writes it. */
+ * state and the first writes it.
+ *
+ * The substeps are one dependent chain, so their latency is the launch's
+ * length: every lane runs the same instructions (the six rate formulas are
+ * one division of selected operands: a division by 1 is exact, so
+ * "fac * e / 1" and "(-x / 1)" are fac * e and -x bit for bit), expf's
+ * special cases are selected rather than branched, and its table is read
+ * from LDS. */
 __global__ void __launch_bounds__(SUB_BLOCK)
 hh_substeps(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 {
+	__shared__ uint64_t s_exp[32];
+	if(threadIdx.x < 32) s_exp[threadIdx.x] = nemo_math::expf_table(threadIdx.x);
+	__syncthreads();
 	const unsigned t = blockIdx.x * SUB_BLOCK + threadIdx.x;
 	const unsigned nl = t / LANES;
 	const unsigned q = t % LANES;
@@ -162,22 +172,32 @@ hh_substeps(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 	const float Inhib = g.scratch[SCR_I * P + nl];
 	const float Exter = g.scratch[SCR_X * P + nl];
 	const uint64_t cycle = nemo_dev::current_step(b) - b.clock->cycle_origin;
-	/* per lane: the constant of its linear argument (alphan, alpham, betah),
-	 * the divisor of its scaled argument (alphah, betan, betam) and the
-	 * factor of its exponential */
+	/* per lane (HH.cpp:92-98), with x = v + 65:
+	 *   exponent  (lin0 - 0.1 x) / 1   (alphan 1, alpham 2.5, betah 3)
+	 *             (-x) / div           (alphah 20, betan 80, betam 18)
+	 *   rate      (c0 - c1 x) / (e - 1)  (alphan 0.1, 0.01; alpham 2.5, 0.1)
+	 *             1 / (e + 1)            (betah)
+	 *             (fac e) / 1            (alphah 0.07, betan 0.125, betam 4) */
 	const unsigned r = q < 6 ? q : 5;
-	const float lin0 = r == 0 ? 1.0f : r == 1 ? 2.5f : 3.0f;
-	const float div = r == 2 ? 20.0f : r == 3 ? 80.0f : 18.0f;
-	const float fac = r == 2 ? 0.07f : r == 3 ? 0.125f : 4.0f;
 	const bool linear = r == 0 || r == 1 || r == 5;
+	const float lin0 = r == 0 ? 1.0f : r == 1 ? 2.5f : 3.0f;
+	const float adiv = linear ? 1.0f : r == 2 ? 20.0f : r == 3 ? 80.0f : 18.0f;
+	const float fac = r == 2 ? 0.07f : r == 3 ? 0.125f : 4.0f;
+	const float c0 = r == 0 ? 0.1f : 2.5f;
+	const float c1 = r == 0 ? 0.01f : 0.1f;
+	const float eoff = r <= 1 ? -1.0f : 1.0f;
 	bool fired = false;
 	const int inc_max = (int)(1 / DT);
 	for(int it = 0; it < inc_max; ++it) {
 		const float x = v + 65.0f;
-		const float arg = linear ? lin0 - 0.1f * x : -x / div;
-		const float e = expf_exact(arg);
-		const float num = r == 0 ? 0.1f - 0.01f * x : 2.5f - 0.1f * x;
-		const float rate = r <= 1 ? num / (e - 1.0f) : r == 5 ? 1.0f / (e + 1.0f) : fac * e;
+		const float anum = linear ? lin0 - 0.1f * x : -x;
+		const float e = nemo_math::expf_exact_tab(anum / adiv, s_exp);
+		const float poly = c0 - c1 * x;
+		const float fe = fac * e;
+		const float rnum = r <= 1 ? poly : r == 5 ? 1.0f : fe;
+		const float eden = e + eoff;
+		const float rden = r <= 1 || r == 5 ? eden : 1.0f;
+		const float rate = rnum / rden;
 		const float alphan = row_lane<0>(rate);
 		const float alpham = row_lane<1>(rate);
 		const float alphah = row_lane<2>(rate);
