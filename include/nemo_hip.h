@@ -294,6 +294,22 @@ nemo_status_t nemo_hip_float_forward(nemo_hip_float_network_t, void* stream, uns
 		unsigned checkpoint_every, const float* w, const float* a, const float* b, const float* c,
 		const float* d, const float* v0, const float* u0, const float* x,
 		float* spikes, float* v_out, double* checkpoints);
+/* Extension (no counterpart in the reference): nemo_hip_float_forward for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results; the forward's workspace -- weights in reverse order (4 B per
+ * synapse), the state in double (16 B per neuron) and the spike bitmaps
+ * (4 B per 32 neurons, rounded up to 16 B, per step) -- is carved from a
+ * static device array of 64 MiB per device instead of the handle's own
+ * allocations, and nothing is allocated or freed.  Fails with
+ * NEMO_INVALID_INPUT when the workspace for `steps` steps exceeds it.  Every
+ * call on one device shares that array (it keeps the last call's contents,
+ * none of which a call reads before writing): calls that overlap on one
+ * device race. */
+nemo_status_t nemo_hip_float_forward_static(nemo_hip_float_network_t, void* stream, unsigned steps,
+		unsigned checkpoint_every, const float* w, const float* a, const float* b, const float* c,
+		const float* d, const float* v0, const float* u0, const float* x,
+		float* spikes, float* v_out, double* checkpoints);
 /* The adjoint of nemo_hip_float_forward: from dL/dspikes and dL/dv_out (either
  * may be NULL) and the forward's spikes and checkpoints, the gradients of the
  * loss with respect to w, a, b, d and x (grad_x may be NULL) and the initial

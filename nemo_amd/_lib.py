@@ -105,6 +105,7 @@ _PROTOTYPES = [
     ("nemo_hip_delete_float_network", None, [vp]),
     ("nemo_hip_float_network_sizes", ctypes.c_int, [vp, c_size_p, c_size_p, c_uint_p]),
     ("nemo_hip_float_forward", ctypes.c_int, [vp, vp, ctypes.c_uint, ctypes.c_uint] + [vp] * 11),
+    ("nemo_hip_float_forward_static", ctypes.c_int, [vp, vp, ctypes.c_uint, ctypes.c_uint] + [vp] * 11),
     ("nemo_hip_float_backward", ctypes.c_int, [vp, vp, ctypes.c_uint, ctypes.c_uint] + [vp] * 17),
     ("nemo_hip_set_debug", ctypes.c_int, [vp, ctypes.c_int]),
     ("nemo_hip_get_currents", ctypes.c_int, [vp, c_float_p, c_float_p, ctypes.c_size_t]),

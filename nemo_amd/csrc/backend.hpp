@@ -433,6 +433,11 @@ public:
 	void forwardWith(const ForwardWorkspace& ws, hipStream_t st, unsigned T, unsigned K, const float* w,
 			const float* a, const float* b, const float* c, const float* d, const float* v0, const float* u0,
 			const float* x, float* S, float* V, double* ckpt);
+	/* forward on the device's static workspace (nemo_float_scratch): calls
+	 * no allocator; fails if the workspace is too small for T steps */
+	void forwardStatic(hipStream_t st, unsigned T, unsigned K, const float* w, const float* a, const float* b,
+			const float* c, const float* d, const float* v0, const float* u0, const float* x,
+			float* S, float* V, double* ckpt);
 	int device() const { return m_device; }
 	void backward(hipStream_t st, unsigned T, unsigned K, const float* w, const float* a, const float* b,
 			const float* c, const float* d, const float* x, const float* S, const double* ckpt,

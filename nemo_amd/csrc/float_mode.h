@@ -39,6 +39,16 @@ typedef struct nemo_float_step_args {
 	                              of at most NEMO_FLOAT_STAGE_MAX */
 } nemo_float_step_args;
 
+/* The static workspace of the allocation-free forward: a file-scope device
+ * array of NEMO_FLOAT_SCRATCH_BYTES, one per device (each device holds it
+ * until the process ends).  Its address on the current device, looked up
+ * once per device (with hipGetSymbolAddress, the only valid device address
+ * of the array in host code); devices numbered below
+ * NEMO_FLOAT_SCRATCH_DEVICES. */
+#define NEMO_FLOAT_SCRATCH_BYTES (64u << 20)
+#define NEMO_FLOAT_SCRATCH_DEVICES 64
+hipError_t nemo_float_scratch(void** ptr, size_t* bytes);
+
 /* Largest LDS stage of spike bitmaps a forward block takes. */
 #define NEMO_FLOAT_STAGE_MAX 49152
 unsigned nemo_float_bitmap_pitch(unsigned n);

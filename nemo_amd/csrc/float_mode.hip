@@ -36,6 +36,8 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "float_mode.h"
 
 namespace {
@@ -479,7 +481,33 @@ inline unsigned blocks_for(size_t n)
 
 } // namespace
 
+/* The workspace of the allocation-free forward (see nemo_float_scratch).
+ * Host code never passes its name to a kernel: the name is a host-side
+ * stand-in, and the device address comes from hipGetSymbolAddress. */
+static __device__ __align__(256) unsigned char scratch[NEMO_FLOAT_SCRATCH_BYTES];
+
 extern "C" {
+
+hipError_t nemo_float_scratch(void** ptr, size_t* bytes)
+{
+	static std::mutex lock;
+	static void* addr[NEMO_FLOAT_SCRATCH_DEVICES] = {};
+	int dev = 0;
+	hipError_t e = hipGetDevice(&dev);
+	if(e != hipSuccess) return e;
+	if(dev < 0 || dev >= NEMO_FLOAT_SCRATCH_DEVICES) return hipErrorInvalidDevice;
+	std::lock_guard<std::mutex> held(lock);
+	if(!addr[dev]) {
+		void* p = nullptr;
+		e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if(e != hipSuccess) return e;
+		if(!p) return hipErrorInvalidSymbol;
+		addr[dev] = p;
+	}
+	*ptr = addr[dev];
+	*bytes = sizeof(scratch);
+	return hipSuccess;
+}
 
 hipError_t nemo_launch_float_forward_step(hipStream_t st, const nemo_float_step_args* a)
 {

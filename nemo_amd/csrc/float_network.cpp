@@ -129,6 +129,29 @@ void FloatNetwork::forward(hipStream_t st, unsigned T, unsigned K, const float* 
 	forwardWith(ws, st, T, K, w, a, b, c, d, v0, u0, x, S, V, ckpt);
 }
 
+/* The forward on the device's static workspace: the kernels and arguments of
+ * forward(), with the workspace carved from the file-scope array of
+ * float_mode.hip instead of this object's allocations, so that nothing is
+ * allocated (the array's address is looked up on the device's first call).
+ * Calls on one device share the array, so they must not overlap. */
+void FloatNetwork::forwardStatic(hipStream_t st, unsigned T, unsigned K, const float* w, const float* a,
+		const float* b, const float* c, const float* d, const float* v0, const float* u0, const float* x,
+		float* S, float* V, double* ckpt)
+{
+	if(T == 0) throw Error(NEMO_INVALID_INPUT, "the float-weight mode needs at least one step");
+	if(K == 0) throw Error(NEMO_INVALID_INPUT, "checkpoint interval must be positive");
+	HIP_CHECK(hipSetDevice(m_device));
+	void* base = nullptr;
+	size_t cap = 0;
+	HIP_CHECK(nemo_float_scratch(&base, &cap));
+	const size_t need = forwardWorkspaceBytes(T);
+	if(need > cap) {
+		throw Error(NEMO_INVALID_INPUT, "the allocation-free forward of " + std::to_string(T) + " steps needs "
+				+ std::to_string(need) + " bytes of workspace; its static workspace holds " + std::to_string(cap));
+	}
+	forwardWith(carveWorkspace(base, T), st, T, K, w, a, b, c, d, v0, u0, x, S, V, ckpt);
+}
+
 /* Bytes of the forward's workspace for T steps, each part 256-byte aligned:
  * weights in reverse order (4 B per synapse), v and u in double (16 B per
  * neuron) and the spike bitmaps of every step. */

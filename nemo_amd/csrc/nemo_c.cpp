@@ -740,6 +740,21 @@ nemo_status_t nemo_hip_float_forward(nemo_hip_float_network_t f, void* stream, u
 	});
 }
 
+nemo_status_t nemo_hip_float_forward_static(nemo_hip_float_network_t f, void* stream, unsigned steps,
+		unsigned checkpoint_every, const float* w, const float* a, const float* b, const float* c,
+		const float* d, const float* v0, const float* u0, const float* x,
+		float* spikes, float* v_out, double* checkpoints)
+{
+	return guarded([&] {
+		check(f, "float network");
+		if(!w || !a || !b || !c || !d || !v0 || !u0 || !x || !spikes || !v_out || !checkpoints) {
+			throw nemo_hip::Error(NEMO_INVALID_INPUT, "null argument");
+		}
+		f->impl.forwardStatic(static_cast<hipStream_t>(stream), steps, checkpoint_every, w, a, b, c, d, v0, u0, x,
+				spikes, v_out, checkpoints);
+	});
+}
+
 nemo_status_t nemo_hip_float_backward(nemo_hip_float_network_t f, void* stream, unsigned steps,
 		unsigned checkpoint_every, const float* w, const float* a, const float* b, const float* c,
 		const float* d, const float* x, const float* spikes, const double* checkpoints,
