@@ -35,11 +35,19 @@ BYTES_PER_IZH_NEURON = 84      # 20 B params + 16 B u,v r/w + 32 B RNG r/w + 16 
 BYTES_PER_STDP_SYNAPSE = 20    # 4 B packed source/delay + 8 B presynaptic history + 8 B accumulator r/w
 
 
+def traffic_key(shape):
+    """The launch-shape fields the propagate launch's traffic depends on:
+    delivery blocks of both sets, bin capacity and STDP mode, and the STDP
+    words when STDP is fused into the launch (mode 2)."""
+    return list(shape[:4]) + ([shape[4]] if shape[3] == 2 else [])
+
+
 def measured_traffic(kernel, shape):
     """HBM bytes per launch of `kernel` from the committed rocprof counter
     summary of this workload (FETCH_SIZE and WRITE_SIZE passes scaled by the
-    copy-kernel calibration), when it was measured at launch shape `shape`;
-    None otherwise (the traffic of a launch depends on its shape)."""
+    copy-kernel calibration), when it was measured at a launch shape that
+    agrees with `shape` in the fields that matter (traffic_key); None
+    otherwise (the traffic of a launch depends on its shape)."""
     path = os.path.join(ROOT, "profiles", "propagate_update_izh100k_stdp.md")
     names = {"propagate": "propagate_kernel", "update": "izhikevich_update"}
     try:
@@ -51,7 +59,7 @@ def measured_traffic(kernel, shape):
             if line.startswith("traffic_bytes_per_launch:"):
                 fields = dict(f.split("=") for f in line.split(":", 1)[1].split())
                 traffic = float(fields[names[kernel]])
-        if measured_shape == list(shape):
+        if measured_shape is not None and traffic_key(measured_shape) == traffic_key(shape):
             return traffic
     except (OSError, KeyError, ValueError):
         pass

@@ -89,7 +89,44 @@ typedef struct nemo_hip_neuron_group {
 	float* scratch;    /* the planes the model asks for with scratch-planes under [hip] in its .ini
 	                      (pitch apart), owned by the simulation; NULL if none.  Contents are the
 	                      model's own between its launches of a step; nothing else reads them */
+	unsigned history;  /* state history length H (the .ini's history): state holds H slots of
+	                      nState planes, slot h plane v at (h * nState + v) * pitch.  With H > 1
+	                      the update of step s reads slot c % H and writes slot (c + 1) % H, c =
+	                      s - clock->cycle_origin (cpu/Neurons.cpp:123; the reference's 32-bit
+	                      cycle, which H divides); the initial state is slot 0 */
+	/* Update order of a model with H > 1 that reads its sources' state at
+	 * slot (c - (d-1)) % H through the all-synapse reverse matrix: a synapse
+	 * with (d - 1) % H == H - 1 reads slot (c + 1) % H, the slot the step
+	 * writes.  NeMo's CPU backend updates the neurons one after another in
+	 * index order (cpu/plugins/Kuramoto.cpp:100-126), so such a read sees
+	 * the source's new state when its index is below the target's and its
+	 * old state otherwise.  level_ptr NULL: no synapse reads that slot.
+	 * Otherwise the plugin keeps the slot's old contents (in its scratch)
+	 * and updates the neurons in `levels` passes, pass l the group-local
+	 * indices level_idx[level_ptr[l] .. level_ptr[l+1]); every lower-index
+	 * source such a synapse of a neuron reads is updated in an earlier pass.
+	 * level_ptr is host memory (levels + 1 entries, for the launches),
+	 * level_idx device memory. */
+	unsigned levels;
+	const uint32_t* level_ptr;
+	const uint32_t* level_idx;
 } nemo_hip_neuron_group;
+
+/* The all-synapse reverse matrix (construction/RCM.cpp:76-98), built when a
+ * model's .ini asks for [rcm] sources, delays or weights: for each device
+ * index n, its incoming synapses k in [ptr[n], ptr[n+1]) in the order NeMo's
+ * CPU backend builds them (sources ascending, then each source's synapses in
+ * insertion order), source_delay[k] = source device index | (delay-1) << 26,
+ * weight[k] the synapse's float weight.  ptr is NULL when no model asks. */
+typedef struct nemo_hip_rcm {
+	const uint32_t* ptr;
+	const uint32_t* source_delay;
+	const float* weight;
+} nemo_hip_rcm;
+
+/* Bits of the status word a launch may set for the host to report after the
+ * step (the results of the step are not those of the reference). */
+#define NEMO_HIP_STATUS_SINF_RANGE 1u /* a sine argument outside (-120, 120) */
 
 /* STDP accumulation of step r = s-1 for the neurons of an update block's tile
  * (ConnectivityMatrix::accumulateStdp, ConnectivityMatrix.cpp:224-258;
@@ -216,6 +253,8 @@ typedef struct nemo_hip_step_buffers {
 	nemo_hip_colaunch co;      /* extra blocks of this launch (the first group's only) */
 	hipEvent_t time_start;     /* when not NULL, recorded when the launch's kernel starts */
 	hipEvent_t time_stop;      /* when not NULL, recorded when it ends (hipExtLaunchKernelGGL) */
+	nemo_hip_rcm rcm_all;      /* all-synapse reverse matrix, for the models that read it */
+	unsigned* status;          /* NEMO_HIP_STATUS_* bits (atomicOr) */
 } nemo_hip_step_buffers;
 
 typedef hipError_t hip_update_neurons_t(hipStream_t stream,

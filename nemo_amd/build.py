@@ -25,7 +25,7 @@ LIBDIR = os.path.join(PKG, "lib")
 PLUGIN_OUT = os.path.join(PKG, "plugins", "hip")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
-MODELS = ["Izhikevich", "QIF", "IF_curr_exp", "HH", "PoissonSource", "Input"]
+MODELS = ["Izhikevich", "QIF", "IF_curr_exp", "HH", "PoissonSource", "Input", "Kuramoto"]
 
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",

@@ -225,8 +225,8 @@ NeuronType loadNeuronType(const std::string& name)
 		if(!hip) {
 			throw Error(NEMO_API_UNSUPPORTED, "neuron type " + name + " has no HIP backend");
 		}
-		if(t.history != 1) {
-			throw Error(NEMO_API_UNSUPPORTED, "neuron type " + name + ": only state history length 1 is supported");
+		if(t.history < 1 || t.history > 64) {
+			throw Error(NEMO_API_UNSUPPORTED, "neuron type " + name + ": state history length must be 1 to 64");
 		}
 		return t;
 	}
@@ -693,7 +693,8 @@ void Simulation::buildNeurons(const std::vector<const Network*>& nets)
 		g.count = unsigned(typeCount[ds]);
 		const size_t pitch = g.count;
 		std::vector<float> param(std::max<size_t>(1, g.type.nParam * pitch));
-		std::vector<float> state(std::max<size_t>(1, g.type.nState * pitch));
+		/* slot 0 of the state history holds the initial state */
+		std::vector<float> state(std::max<size_t>(1, size_t(g.type.history) * g.type.nState * pitch));
 		std::vector<uint32_t> rng(4 * size_t(g.count));
 		size_t pos = 0;
 		for(unsigned k = 0; k < nets.size(); ++k) {
@@ -739,6 +740,10 @@ void Simulation::buildNeurons(const std::vector<const Network*>& nets)
 		g.dev.state = g.state.ptr;
 		g.dev.rng = g.rng.ptr;
 		g.dev.scratch = nullptr;
+		g.dev.history = g.type.history;
+		g.dev.levels = 1;
+		g.dev.level_ptr = nullptr;
+		g.dev.level_idx = nullptr;
 		if(g.type.scratchPlanes && g.count) {
 			g.scratch.allocate(size_t(g.type.scratchPlanes) * pitch);
 			g.dev.scratch = g.scratch.ptr;
@@ -935,6 +940,91 @@ void Simulation::buildConnectivity(const std::vector<const Network*>& nets, cons
 				}
 			}
 		}
+	}
+	/* all-synapse reverse matrix, for models that read their incoming
+	 * synapses' sources, delays or weights (construction/RCM.cpp:76-98):
+	 * per target, sources in ascending order and each source's synapses in
+	 * insertion order, as the reference's CPU backend adds them */
+	bool rcmAll = false;
+	for(const Group& g : m_groups) {
+		rcmAll = rcmAll || g.type.rcmSources || g.type.rcmDelays || g.type.rcmWeights || g.type.rcmForward;
+	}
+	if(rcmAll) {
+		if(m_stdp) {
+			throw Error(NEMO_API_UNSUPPORTED, "The current version does not support a mix of STDP with neuron types "
+					"which require reverse connectivity for other purposes");
+		}
+		std::vector<uint32_t> ptr(N + 1, 0);
+		for(const Src& s : srcs) {
+			for(size_t k = 0; k < s.ax->target.size(); ++k) ptr[ltgt[s.seq + k] + 1] += 1;
+		}
+		for(size_t n = 0; n < N; ++n) ptr[n + 1] += ptr[n];
+		std::vector<uint32_t> fill(ptr.begin(), ptr.end() - 1);
+		std::vector<uint32_t> sd(std::max<size_t>(1, m_nsynapses));
+		std::vector<float> w(std::max<size_t>(1, m_nsynapses));
+		for(const Src& s : srcs) {
+			for(size_t k = 0; k < s.ax->target.size(); ++k) {
+				const uint32_t tg = ltgt[s.seq + k];
+				const int tgroup = m_typeOfLocal[tg];
+				const Group& g = m_groups[size_t(tgroup)];
+				const bool reads = g.type.rcmSources || g.type.rcmDelays || g.type.rcmWeights || g.type.rcmForward;
+				if(reads && m_typeOfLocal[s.l] != tgroup) {
+					/* the models that read the matrix index their own state by
+					 * the source (cpu/plugins/Kuramoto.cpp:52-78) */
+					throw Error(NEMO_API_UNSUPPORTED, "synapse from neuron " + std::to_string(s.g) + " to a neuron of type "
+							+ g.type.name + ", whose model reads its sources' state: sources must be of the same type");
+				}
+				const uint32_t pos = fill[tg]++;
+				sd[pos] = uint32_t(s.l) | (uint32_t(s.ax->delay[k] - 1) << 26);
+				w[pos] = s.ax->weight[k];
+			}
+		}
+		/* update passes of the models with a state history that read the
+		 * slot the step writes (nemo_hip_neuron_group.levels): a neuron's
+		 * pass follows the passes of the lower-index sources it reads that
+		 * slot from; indices ascend, so one pass in index order sets them */
+		for(Group& g : m_groups) {
+			const unsigned H = g.type.history;
+			const bool reads = g.type.rcmSources || g.type.rcmDelays || g.type.rcmWeights || g.type.rcmForward;
+			if(H < 2 || !reads || g.count == 0) continue;
+			bool any = false;
+			std::vector<uint32_t> level(g.count, 0);
+			uint32_t levels = 1;
+			for(unsigned i = 0; i < g.count; ++i) {
+				const uint32_t tg = g.start + i;
+				for(uint32_t k = ptr[tg]; k < ptr[tg + 1]; ++k) {
+					const unsigned d = (sd[k] >> 26) + 1;
+					if((d - 1) % H != H - 1) continue;
+					any = true;
+					const uint32_t src = (sd[k] & 0x3ffffffu) - g.start;
+					if(src < i) level[i] = std::max(level[i], level[src] + 1);
+				}
+				levels = std::max(levels, level[i] + 1);
+			}
+			if(!any) continue;
+			std::vector<uint32_t> lptr(levels + 1, 0), lidx(g.count);
+			for(unsigned i = 0; i < g.count; ++i) lptr[level[i] + 1] += 1;
+			for(unsigned l = 0; l < levels; ++l) lptr[l + 1] += lptr[l];
+			std::vector<uint32_t> lfill(lptr.begin(), lptr.end() - 1);
+			for(unsigned i = 0; i < g.count; ++i) lidx[lfill[level[i]]++] = i;
+			g.levelPtr = lptr;
+			g.levelIdx.allocate(lidx.size());
+			HIP_CHECK(hipMemcpy(g.levelIdx.ptr, lidx.data(), g.levelIdx.bytes(), hipMemcpyHostToDevice));
+			g.dev.levels = levels;
+			g.dev.level_ptr = g.levelPtr.data();
+			g.dev.level_idx = g.levelIdx.ptr;
+			if(g.type.scratchPlanes < 1) {
+				throw Error(NEMO_API_UNSUPPORTED, "neuron type " + g.type.name + " reads the state slot a step "
+						"writes and needs a scratch plane for its old contents ([hip] scratch-planes)");
+			}
+		}
+		m_rcmAllPtr.allocate(ptr.size());
+		m_rcmAllSd.allocate(sd.size());
+		m_rcmAllW.allocate(w.size());
+		HIP_CHECK(hipMemcpy(m_rcmAllPtr.ptr, ptr.data(), m_rcmAllPtr.bytes(), hipMemcpyHostToDevice));
+		HIP_CHECK(hipMemcpy(m_rcmAllSd.ptr, sd.data(), m_rcmAllSd.bytes(), hipMemcpyHostToDevice));
+		HIP_CHECK(hipMemcpy(m_rcmAllW.ptr, w.data(), m_rcmAllW.bytes(), hipMemcpyHostToDevice));
+		m_checkStatus = true;
 	}
 	if(m_writeOnly) {
 		std::vector<uint32_t>().swap(m_synAddr);
@@ -1290,6 +1380,8 @@ void Simulation::allocateStepState()
 	HIP_CHECK(hipMemset(m_ext.ptr, 0, m_ext.bytes()));
 	m_fstim.allocate((N + 31) / 32);
 	HIP_CHECK(hipMemset(m_fstim.ptr, 0, m_fstim.bytes()));
+	m_status.allocate(1);
+	HIP_CHECK(hipMemset(m_status.ptr, 0, m_status.bytes()));
 	m_debugCurrent.allocate(2 * m_accPitch);
 	HIP_CHECK(hipMemset(m_debugCurrent.ptr, 0, m_debugCurrent.bytes()));
 	HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&m_pinnedBits),
@@ -1336,7 +1428,33 @@ nemo_hip_step_buffers Simulation::stepBuffers(uint64_t offset, bool ext, bool fs
 	b.debug_current = m_debug ? m_debugCurrent.ptr : nullptr;
 	b.stdp = m_stdpArgs;
 	if(m_shape.stdpMode != 1) b.stdp.rcm_ptr = nullptr; /* STDP runs in a launch of its own */
+	b.rcm_all.ptr = m_rcmAllPtr.ptr;
+	b.rcm_all.source_delay = m_rcmAllSd.ptr;
+	b.rcm_all.weight = m_rcmAllW.ptr;
+	b.status = m_status.ptr;
 	return b;
+}
+
+float* Simulation::statePlane(const Group& g, unsigned var) const
+{
+	const unsigned H = g.type.history;
+	const size_t slot = H > 1 && m_updated ? size_t((m_lastCycle + 1) % H) : 0;
+	return g.state.ptr + (slot * g.type.nState + var) * g.count;
+}
+
+/* Report what the kernels flagged (NEMO_HIP_STATUS_*) since the last check;
+ * the stream must be synchronised. */
+void Simulation::checkStatus()
+{
+	if(!m_checkStatus) return;
+	unsigned st = 0;
+	HIP_CHECK(hipMemcpy(&st, m_status.ptr, sizeof(st), hipMemcpyDeviceToHost));
+	if(!st) return;
+	HIP_CHECK(hipMemset(m_status.ptr, 0, sizeof(unsigned)));
+	if(st & NEMO_HIP_STATUS_SINF_RANGE) {
+		throw Error(NEMO_API_UNSUPPORTED, "a neuron model took the sine of an argument outside (-120, 120), "
+				"which the HIP backend does not evaluate as the C library does; the step's state is not the reference's");
+	}
 }
 
 /* Copy a host scalar to the device on the stream.  The source is a fresh
@@ -1532,6 +1650,15 @@ void Simulation::enqueueSteps(uint64_t first, unsigned k, bool ext, bool fstim, 
 	if(lagged && k) HIP_CHECK(hipStreamWaitEvent(m_stream, m_stdpLag[(k - 1) & 1], 0));
 }
 
+/* k steps with neuron updates were enqueued after m_hostStep */
+void Simulation::advanceSteps(uint64_t k)
+{
+	if(!k) return;
+	m_hostStep += k;
+	m_lastCycle = m_hostStep - 1 - m_cycleOrigin;
+	m_updated = true;
+}
+
 /* Make the main stream wait for the STDP branch of the last enqueued step. */
 void Simulation::joinStdp()
 {
@@ -1693,12 +1820,13 @@ const std::vector<unsigned>& Simulation::step(const unsigned* fstim, size_t nf,
 	const uint64_t t = m_hostStep;
 	setBase(t);
 	enqueueSteps(0, 1, !iIdx.empty(), !fIdx.empty());
-	m_hostStep += 1;
+	advanceSteps(1);
 	if(!fIdx.empty()) {
 		HIP_CHECK(nemo_launch_fstim(m_stream, m_fstim.ptr, m_stimIdx.ptr, fIdx.size(), 0));
 	}
 	readBitmaps(t, 1);
 	HIP_CHECK(hipStreamSynchronize(m_stream));
+	checkStatus();
 	m_firedOut.clear();
 	decodeFiring(m_pinnedBits, m_firedOut);
 	return m_firedOut;
@@ -1806,7 +1934,7 @@ void Simulation::run(unsigned nsteps, unsigned stdpPeriod, float reward, bool re
 			/* the ring slots this chunk overwrites were copied out */
 			if(inFlight[half]) HIP_CHECK(hipStreamWaitEvent(m_stream, copied[half], 0));
 			HIP_CHECK(hipGraphLaunch(g, m_stream));
-			m_hostStep += k;
+			advanceSteps(k);
 			if(readRaster) {
 				HIP_CHECK(hipEventRecord(ready[half], m_stream));
 				HIP_CHECK(hipStreamWaitEvent(m_copyStream, ready[half], 0));
@@ -1843,6 +1971,7 @@ void Simulation::run(unsigned nsteps, unsigned stdpPeriod, float reward, bool re
 	if(readRaster && fired && nsteps) m_firedPerStep = double(fired->size() - firedBefore) / nsteps;
 	if(events) *events = sumCounters(m_blockEvents) - events0;
 	m_firedOut.clear();
+	checkStatus();
 }
 
 /* Eager steps with HIP events around each launch on the simulation's stream:
@@ -1865,7 +1994,7 @@ void Simulation::profile(unsigned nsteps, double avgUs[4], uint64_t counters[4])
 	for(unsigned s = 0; s < nsteps; s += 64) {
 		enqueueSteps(s, std::min(64u, nsteps - s), false, false, &ev[4 * size_t(s)]);
 	}
-	m_hostStep += nsteps;
+	advanceSteps(nsteps);
 	HIP_CHECK(hipStreamSynchronize(m_stream));
 	double sum[3] = {0, 0, 0};
 	for(unsigned s = 0; s < nsteps; ++s) {
@@ -1912,6 +2041,8 @@ struct Simulation::Snapshot {
 	std::vector<DeviceArray<unsigned char> > copies;
 	uint64_t hostStep = 0;
 	uint64_t flushed = 0;
+	uint64_t lastCycle = 0;
+	bool updated = false;
 };
 
 void Simulation::takeSnapshot(Snapshot& s)
@@ -1945,6 +2076,8 @@ void Simulation::takeSnapshot(Snapshot& s)
 	HIP_CHECK(hipStreamSynchronize(m_stream));
 	s.hostStep = m_hostStep;
 	s.flushed = m_flushed;
+	s.lastCycle = m_lastCycle;
+	s.updated = m_updated;
 }
 
 void Simulation::restoreSnapshot(const Snapshot& s)
@@ -1958,6 +2091,8 @@ void Simulation::restoreSnapshot(const Snapshot& s)
 	HIP_CHECK(hipStreamSynchronize(m_stream));
 	m_hostStep = s.hostStep;
 	m_flushed = s.flushed;
+	m_lastCycle = s.lastCycle;
+	m_updated = s.updated;
 }
 
 /* Advance nsteps steps in graphs of up to 64 steps, then their pending STDP
@@ -1975,7 +2110,7 @@ double Simulation::timeSteps(unsigned nsteps)
 		hipGraphExec_t g = graphFor(k);
 		setBase(m_hostStep);
 		HIP_CHECK(hipGraphLaunch(g, m_stream));
-		m_hostStep += k;
+		advanceSteps(k);
 		done += k;
 	}
 	enqueueFlush();
@@ -2201,7 +2336,7 @@ float Simulation::getNeuronState(unsigned idx, unsigned var, unsigned net)
 	if(var >= g.type.nState) throw Error(NEMO_INVALID_INPUT, "Invalid state variable index " + std::to_string(var));
 	float v;
 	HIP_CHECK(hipStreamSynchronize(m_stream));
-	HIP_CHECK(hipMemcpy(&v, g.state.ptr + size_t(var) * g.count + (l - g.start), 4, hipMemcpyDeviceToHost));
+	HIP_CHECK(hipMemcpy(&v, statePlane(g, var) + (l - g.start), 4, hipMemcpyDeviceToHost));
 	return v;
 }
 
@@ -2243,7 +2378,7 @@ void Simulation::setNeuron(unsigned idx, unsigned nargs, const float* args)
 		HIP_CHECK(hipMemcpy(g.param.ptr + size_t(p) * g.count + (l - g.start), args + p, 4, hipMemcpyHostToDevice));
 	}
 	for(unsigned s = 0; s < g.type.nState; ++s) {
-		HIP_CHECK(hipMemcpy(g.state.ptr + size_t(s) * g.count + (l - g.start), args + g.type.nParam + s, 4,
+		HIP_CHECK(hipMemcpy(statePlane(g, s) + (l - g.start), args + g.type.nParam + s, 4,
 					hipMemcpyHostToDevice));
 	}
 }
@@ -2255,7 +2390,7 @@ void Simulation::setNeuronState(unsigned idx, unsigned var, float val)
 	Group& g = groupOf(l);
 	if(var >= g.type.nState) throw Error(NEMO_INVALID_INPUT, "Invalid state variable index " + std::to_string(var));
 	HIP_CHECK(hipStreamSynchronize(m_stream));
-	HIP_CHECK(hipMemcpy(g.state.ptr + size_t(var) * g.count + (l - g.start), &val, 4, hipMemcpyHostToDevice));
+	HIP_CHECK(hipMemcpy(statePlane(g, var) + (l - g.start), &val, 4, hipMemcpyHostToDevice));
 }
 
 void Simulation::setNeuronParameter(unsigned idx, unsigned param, float val)
@@ -2291,7 +2426,7 @@ void Simulation::getNeuronValues(bool param, unsigned var, size_t n, const unsig
 	for(size_t i = 0; i < n; ++i) {
 		const int gi = m_typeOfLocal[loc[i]];
 		Group& g = m_groups[gi];
-		const float* plane = (param ? g.param.ptr : g.state.ptr) + size_t(var) * g.count;
+		const float* plane = (param ? g.param.ptr + size_t(var) * g.count : statePlane(g, var));
 		if(n <= ELEMENTWISE_MAX) {
 			HIP_CHECK(hipMemcpy(out + i, plane + (loc[i] - g.start), 4, hipMemcpyDeviceToHost));
 			continue;
@@ -2320,7 +2455,7 @@ void Simulation::setNeuronValues(bool param, unsigned var, size_t n, const unsig
 	if(n <= ELEMENTWISE_MAX) {
 		for(size_t i = 0; i < n; ++i) {
 			Group& g = groupOf(loc[i]);
-			float* plane = (param ? g.param.ptr : g.state.ptr) + size_t(var) * g.count;
+			float* plane = (param ? g.param.ptr + size_t(var) * g.count : statePlane(g, var));
 			HIP_CHECK(hipMemcpy(plane + (loc[i] - g.start), vals + i, 4, hipMemcpyHostToDevice));
 		}
 		return;
@@ -2331,7 +2466,7 @@ void Simulation::setNeuronValues(bool param, unsigned var, size_t n, const unsig
 		Group& g = m_groups[gi];
 		if(planes[gi].empty()) {
 			planes[gi].resize(g.count);
-			HIP_CHECK(hipMemcpy(planes[gi].data(), (param ? g.param.ptr : g.state.ptr) + size_t(var) * g.count,
+			HIP_CHECK(hipMemcpy(planes[gi].data(), (param ? g.param.ptr + size_t(var) * g.count : statePlane(g, var)),
 						g.count * sizeof(float), hipMemcpyDeviceToHost));
 		}
 		planes[gi][loc[i] - g.start] = vals[i]; /* for repeated neurons the last value wins */
@@ -2339,7 +2474,7 @@ void Simulation::setNeuronValues(bool param, unsigned var, size_t n, const unsig
 	for(size_t gi = 0; gi < m_groups.size(); ++gi) {
 		if(planes[gi].empty()) continue;
 		Group& g = m_groups[gi];
-		HIP_CHECK(hipMemcpy((param ? g.param.ptr : g.state.ptr) + size_t(var) * g.count, planes[gi].data(),
+		HIP_CHECK(hipMemcpy((param ? g.param.ptr + size_t(var) * g.count : statePlane(g, var)), planes[gi].data(),
 					g.count * sizeof(float), hipMemcpyHostToDevice));
 	}
 }
@@ -2373,7 +2508,7 @@ void Simulation::setNeurons(size_t n, const unsigned* idx, unsigned nargs, const
 			param[gi].resize(np);
 			state[gi].resize(ns);
 			if(np) HIP_CHECK(hipMemcpy(param[gi].data(), g.param.ptr, np * sizeof(float), hipMemcpyDeviceToHost));
-			if(ns) HIP_CHECK(hipMemcpy(state[gi].data(), g.state.ptr, ns * sizeof(float), hipMemcpyDeviceToHost));
+			if(ns) HIP_CHECK(hipMemcpy(state[gi].data(), statePlane(g, 0), ns * sizeof(float), hipMemcpyDeviceToHost));
 			if(param[gi].empty() && state[gi].empty()) continue; /* a model without parameters or state */
 		}
 		const size_t k = loc[i] - g.start;
@@ -2386,7 +2521,8 @@ void Simulation::setNeurons(size_t n, const unsigned* idx, unsigned nargs, const
 			HIP_CHECK(hipMemcpy(g.param.ptr, param[gi].data(), param[gi].size() * sizeof(float), hipMemcpyHostToDevice));
 		}
 		if(!state[gi].empty()) {
-			HIP_CHECK(hipMemcpy(g.state.ptr, state[gi].data(), state[gi].size() * sizeof(float), hipMemcpyHostToDevice));
+			HIP_CHECK(hipMemcpy(statePlane(g, 0), state[gi].data(), state[gi].size() * sizeof(float),
+						hipMemcpyHostToDevice));
 		}
 	}
 }
@@ -2535,7 +2671,7 @@ void Simulation::getNeuronStates(unsigned var, float* out, size_t n)
 	for(Group& g : m_groups) {
 		if(var >= g.type.nState || g.count == 0) continue;
 		std::vector<float> h(g.count);
-		HIP_CHECK(hipMemcpy(h.data(), g.state.ptr + size_t(var) * g.count, g.count * sizeof(float),
+		HIP_CHECK(hipMemcpy(h.data(), statePlane(g, var), g.count * sizeof(float),
 					hipMemcpyDeviceToHost));
 		for(size_t i = 0; i < n; ++i) {
 			const unsigned l = m_order[i];

@@ -260,8 +260,14 @@ private:
 		unsigned start = 0, count = 0;
 		DeviceArray<float> param, state, scratch;
 		DeviceArray<uint32_t> rng;
+		std::vector<uint32_t> levelPtr;   /* update passes (nemo_hip_neuron_group.levels), host */
+		DeviceArray<uint32_t> levelIdx;
 		nemo_hip_neuron_group dev;
 	};
+	/* plane var of the group's current state (its history slot) */
+	float* statePlane(const Group& g, unsigned var) const;
+	void advanceSteps(uint64_t k);
+	void checkStatus();
 
 	int m_device = 0;
 	hipStream_t m_stream = nullptr;
@@ -318,6 +324,11 @@ private:
 	DeviceArray<int2> m_fcm;
 	DeviceArray<uint32_t> m_rcmPtr, m_rcmSd, m_rcmFwd;
 	DeviceArray<uint32_t> m_rcmOf;    /* per FCM synapse: accumulator index, ~0 if static */
+	/* all-synapse reverse matrix (models whose .ini asks for [rcm] fields) */
+	DeviceArray<uint32_t> m_rcmAllPtr, m_rcmAllSd;
+	DeviceArray<float> m_rcmAllW;
+	DeviceArray<unsigned> m_status;   /* NEMO_HIP_STATUS_* bits set by the kernels */
+	bool m_checkStatus = false;       /* some model may set status bits */
 	DeviceArray<int32_t> m_rcmAcc;
 	DeviceArray<uint32_t> m_stimIdx;
 	DeviceArray<float> m_stimVal;
@@ -359,6 +370,11 @@ private:
 	/* host staging */
 	uint64_t m_hostStep = 0;           /* steps launched */
 	uint64_t m_cycleOrigin = 0;        /* step of the last timer reset */
+	/* cycle (step - m_cycleOrigin) of the last neuron update, when any ran:
+	 * a model with state history H holds its current state in slot
+	 * (m_lastCycle + 1) % H (cpu/Neurons.cpp:123), else in slot 0 */
+	uint64_t m_lastCycle = 0;
+	bool m_updated = false;
 	static const unsigned SCALAR_SLOTS = 256;
 	uint64_t* m_pinnedScalars = nullptr; /* ring of sources for host-to-device scalars */
 	unsigned m_scalarNext = 0;

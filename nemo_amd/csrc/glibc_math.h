@@ -210,6 +210,65 @@ NEMO_HD float expf_exact(float x)
 	return expf_exact_tab(x, T);
 }
 
+/* ---------------------------------------------------------------- fmodf */
+
+/* x - n y for the integer n = trunc(x / y), which IEEE 754 defines exactly
+ * (the result is always representable), so any correct implementation --
+ * the C library's, this one -- gives the same bits.  Shift-and-subtract
+ * long division of the significands: the remainder of the aligned
+ * significand of x by that of y, one exponent step at a time. */
+NEMO_HD float fmodf_exact(float x, float y)
+{
+	uint32_t ux = asuint(x), uy = asuint(y);
+	const uint32_t sign = ux & 0x80000000u;
+	int ex = int((ux >> 23) & 0xff), ey = int((uy >> 23) & 0xff);
+	if((uy << 1) == 0 || (uy << 1) > 0xff000000u || ex == 0xff) return (x * y) / (x * y); /* NaN */
+	if((ux << 1) <= (uy << 1)) return (ux << 1) == (uy << 1) ? 0.0f * x : x;
+	/* significands with the implicit bit; subnormals normalised, their
+	 * exponent going below 1 */
+	if(ex == 0) {
+		ux <<= 9;
+		while(!(ux & 0x80000000u)) {
+			--ex;
+			ux <<= 1;
+		}
+		ux >>= 8;
+	} else {
+		ux = (ux & 0x007fffffu) | 0x00800000u;
+	}
+	if(ey == 0) {
+		uy <<= 9;
+		while(!(uy & 0x80000000u)) {
+			--ey;
+			uy <<= 1;
+		}
+		uy >>= 8;
+	} else {
+		uy = (uy & 0x007fffffu) | 0x00800000u;
+	}
+	for(; ex > ey; --ex) {
+		if(ux >= uy) {
+			ux -= uy;
+			if(ux == 0) return 0.0f * x;
+		}
+		ux <<= 1;
+	}
+	if(ux >= uy) {
+		ux -= uy;
+		if(ux == 0) return 0.0f * x;
+	}
+	while(!(ux & 0x00800000u)) {
+		ux <<= 1;
+		--ex;
+	}
+	if(ex > 0) {
+		ux = (ux & 0x007fffffu) | (uint32_t(ex) << 23);
+	} else {
+		ux >>= 1 - ex;
+	}
+	return asfloat(ux | sign);
+}
+
 } // namespace nemo_math
 
 #endif

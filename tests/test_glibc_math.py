@@ -3,6 +3,8 @@ host C library, over every float the callers can produce:
 
   logf(1 - b), b = float(u32) * 2^-32       (Box-Muller radius, RNG.cpp:30)
   sinf(a),     a = float(u32) * 1.4629181e-9 (Box-Muller angle, RNG.cpp:28)
+  fmodf(x, y)  (exact by definition) against the C library's, mostly for
+               y = float(2 pi), Kuramoto's phase wrap (Kuramoto.cpp:124)
   expf(x),     every float (HH.cpp:92-98 calls it on the membrane
                potential's functions; the special cases are selected, not
                branched, so they are checked too)
@@ -46,7 +48,32 @@ int main() {
         float x; memcpy(&x, &w, 4);
         n_exp++; if (bits(expf(x)) != bits(expf_exact(x))) bad_exp++;
     }
-    printf("%ld %ld %ld %ld %ld %ld\n", bad_log, n_log, bad_sin, n_sin, bad_exp, n_exp);
+    /* fmodf: by 2 pi rounded to float (Kuramoto's phase wrap) every float
+     * below 1024 in magnitude, every 97th float beyond; other divisors,
+     * subnormal ones included, every 4099th float */
+    long bad_fmod = 0, n_fmod = 0;
+    const float y0 = (float)(2.0f * M_PI);
+    #pragma omp parallel for reduction(+:bad_fmod, n_fmod) schedule(dynamic, 1 << 16)
+    for (int64_t u = 0; u < (int64_t(1) << 32); u += 1) {
+        uint32_t w = (uint32_t)u;
+        if ((w & 0x7fffffffu) >= 0x44800000u && u % 97) continue;
+        float x; memcpy(&x, &w, 4);
+        float a = fmodf(x, y0), b = fmodf_exact(x, y0);
+        n_fmod++;
+        if (bits(a) != bits(b) && !(a != a && b != b)) bad_fmod++;
+    }
+    const float ys[] = {1.0f, 3.0f, 1e-40f, -2.5f, 1e30f, 0.0f};
+    for (float y : ys) {
+        #pragma omp parallel for reduction(+:bad_fmod, n_fmod) schedule(dynamic, 1 << 12)
+        for (int64_t u = 0; u < (int64_t(1) << 32); u += 4099) {
+            uint32_t w = (uint32_t)u;
+            float x; memcpy(&x, &w, 4);
+            float a = fmodf(x, y), b = fmodf_exact(x, y);
+            n_fmod++;
+            if (bits(a) != bits(b) && !(a != a && b != b)) bad_fmod++;
+        }
+    }
+    printf("%ld %ld %ld %ld %ld %ld %ld %ld\n", bad_log, n_log, bad_sin, n_sin, bad_exp, n_exp, bad_fmod, n_fmod);
     return 0;
 }
 """
@@ -79,4 +106,10 @@ def test_sinf_matches_host_over_box_muller_domain(results):
 def test_expf_matches_host_over_every_float(results):
     bad, n = results[4], results[5]
     assert n == 1 << 32
+    assert bad == 0
+
+
+def test_fmodf_matches_host(results):
+    bad, n = results[6], results[7]
+    assert n > 1 << 31
     assert bad == 0
