@@ -255,6 +255,9 @@ typedef struct nemo_hip_step_buffers {
 	hipEvent_t time_stop;      /* when not NULL, recorded when it ends (hipExtLaunchKernelGGL) */
 	nemo_hip_rcm rcm_all;      /* all-synapse reverse matrix, for the models that read it */
 	unsigned* status;          /* NEMO_HIP_STATUS_* bits (atomicOr) */
+	const float* ext_held;     /* the external current of every device index, also in steps
+	                              without a current stimulus (ext_current NULL): for a model
+	                              that reads it without clearing it, as a held value */
 } nemo_hip_step_buffers;
 
 typedef hipError_t hip_update_neurons_t(hipStream_t stream,

@@ -66,6 +66,19 @@ _NAMED_ARGS = {
     "PoissonSource": (("lambda",), ()),
     "Input": ((), ()),
     "Kuramoto": (("omega", "Cmean"), ("theta",)),
+    "KuramotoLag": (("omega", "Cmean", "alpha"), ("theta",)),
+    "IzhikevichChaos": (("a", "b", "c", "d", "sigma"), ("u", "v")),
+    # NeMo's module passes these three models' state as v, u, ge, gi although
+    # their plugins read index 0 as u and 1 as v (plugins/Izhikevich2007.h,
+    # IzhikevichSyn.h); a drop-in keeps the module's mapping
+    "IzhikevichSyn": (("a", "b", "c", "d", "vpeak", "sigma", "tMdtOt_exc", "G_exc", "E_exc", "tMdtOt_inh", "G_inh",
+                       "E_inh"), ("v", "u", "ge", "gi")),
+    "Izhikevich2007": (("a", "b", "c", "d", "vpeak", "Vr", "Vt", "k", "Cm", "sigma", "tMdtOt_exc", "G_exc", "E_exc",
+                        "tMdtOt_inh", "G_inh", "E_inh"), ("v", "u", "ge", "gi")),
+    "Izhikevich2007DopSyn": (("a", "b", "c", "d", "vpeak", "Vr", "Vt", "k", "Cm", "sigma", "d1", "d2", "tMdtOt_exc",
+                              "G_exc", "E_exc", "tMdtOt_inh", "G_inh", "E_inh"), ("v", "u", "ge", "gi")),
+    "BayesianNeuron": (("rOn", "rOff", "g0"), ("L", "G")),
+    "InhomogeneousPoissonSource": ((), ()),
 }
 
 

@@ -25,7 +25,9 @@ LIBDIR = os.path.join(PKG, "lib")
 PLUGIN_OUT = os.path.join(PKG, "plugins", "hip")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
-MODELS = ["Izhikevich", "QIF", "IF_curr_exp", "HH", "PoissonSource", "Input", "Kuramoto"]
+MODELS = ["Izhikevich", "QIF", "IF_curr_exp", "HH", "PoissonSource", "Input", "Kuramoto", "KuramotoLag",
+          "IzhikevichChaos", "IzhikevichSyn", "Izhikevich2007", "Izhikevich2007DopSyn", "BayesianNeuron",
+          "InhomogeneousPoissonSource"]
 
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",

@@ -1432,6 +1432,7 @@ nemo_hip_step_buffers Simulation::stepBuffers(uint64_t offset, bool ext, bool fs
 	b.rcm_all.source_delay = m_rcmAllSd.ptr;
 	b.rcm_all.weight = m_rcmAllW.ptr;
 	b.status = m_status.ptr;
+	b.ext_held = m_ext.ptr;
 	return b;
 }
 

@@ -1,6 +1,9 @@
 /*
  * Kuramoto oscillator plugin (libKuramoto.so), bit-exact with
- * src/nemo/cpu/plugins/Kuramoto.cpp:14-159.
+ * src/nemo/cpu/plugins/Kuramoto.cpp:14-159; built with NEMO_KURAMOTO_LAG
+ * defined (plugins/KuramotoLag.hip) it is libKuramotoLag.so, bit-exact with
+ * src/nemo/cpu/plugins/KuramotoLag.cpp:15-138: a third parameter alpha is
+ * subtracted inside every sine, sin(theta_k - theta - alpha).
  *
  * Parameters (plugins/Kuramoto.h): omega, the phase advance per unit time,
  * and Cmean, the coupling normalisation (1 when not positive).  State: the
@@ -27,7 +30,13 @@
 
 namespace {
 
-enum { PARAM_FREQUENCY = 0, PARAM_CMEAN = 1 };
+enum { PARAM_FREQUENCY = 0, PARAM_CMEAN = 1, PARAM_ALPHA = 2 };
+
+#ifdef NEMO_KURAMOTO_LAG
+constexpr bool LAG = true;
+#else
+constexpr bool LAG = false;
+#endif
 
 /* 2.0f * M_PI is evaluated in double (Kuramoto.cpp:124) */
 constexpr double TWO_PI = 2.0 * 3.14159265358979323846;
@@ -56,6 +65,7 @@ __device__ inline float kuramoto_phase(const nemo_hip_neuron_group& g, const nem
 	const float f = g.param[PARAM_FREQUENCY * P + nl];
 	const float cm = g.param[PARAM_CMEAN * P + nl];
 	const float Cmean = cm > 0 ? cm : 1;
+	const float alpha = LAG ? g.param[PARAM_ALPHA * P + nl] : 0.0f;
 	const float h = 0.05f;
 	const float target = theta[size_t(cycle % H) * P + nl];
 	const unsigned ng = g.start + nl;
@@ -70,7 +80,7 @@ __device__ inline float kuramoto_phase(const nemo_hip_neuron_group& g, const nem
 			const unsigned src = (sd & 0x3ffffffu) - g.start;
 			const float sourcePhase = levelled && slot == next && src >= nl ? g.scratch[src]
 				: theta[size_t(slot) * P + src];
-			const float x = sourcePhase - targetPhase;
+			const float x = LAG ? sourcePhase - targetPhase - alpha : sourcePhase - targetPhase;
 			*range = *range || __builtin_fabsf(x) >= 120.0f;
 			sum += b.rcm_all.weight[k] * nemo_math::sinf_exact(x);
 		}

@@ -110,10 +110,14 @@ def batch_izhikevich(count=64, nmin=1000, nmax=50000, smin=100, smax=1000, dmax=
 
 
 def torus_mixed(n_lif=40000, n_qif=8000, n_hh=2000, n_poisson=1000, m=500, sigma=0.1, seed=3,
-                cm=1.0, i_offset=0.7, qif_a=0.1, rate=0.01, grid=50):
+                cm=400.0, i_offset=280.0, qif_a=0.1, rate=0.01, grid=50):
     """Mixed-model network on the unit torus: IF_curr_exp (v_rest -65 mV,
-    v_reset -65, cm 1 nF, tau_m 20 ms, refractory 2 ms, tau_syn 5 ms, threshold
-    -50 mV, i_offset 0.7 nA: resting just below threshold), QIF (a = 0.1), HH
+    v_reset -65, tau_m 20 ms, refractory 2 ms, tau_syn 5 ms, threshold -50 mV;
+    cm 400 nF and i_offset 280 nA, so that the neuron rests 1 mV below
+    threshold (v_rest + tau_m i_offset / cm = -51 mV) and one synaptic event
+    of weight w moves v by about tau_syn w / cm, a few microvolts: the LIF
+    population fires at about 10 Hz, where cm 1 nF drives the whole network
+    to its refractory ceiling), QIF (a = 0.1), HH
     (resting gating state) and Poisson sources (rate per ms; 0.01 = 10 Hz),
     added in that order, neuron i at a position U[0,1)^2.  Every neuron has m
     outgoing synapses to non-source neurons with probability proportional to

@@ -167,7 +167,8 @@ void initialiseRng(unsigned minIdx, unsigned maxIdx, std::vector<RNG>& rngs)
 /* neuron types: the .ini files in src/nemo/plugins/                   */
 /* ------------------------------------------------------------------ */
 
-enum Model { IZHIKEVICH, QIF, IF_CURR_EXP, HH, POISSON, INPUT, KURAMOTO };
+enum Model { IZHIKEVICH, QIF, IF_CURR_EXP, HH, POISSON, INPUT, KURAMOTO, KURAMOTO_LAG, IZHIKEVICH_CHAOS,
+	IZHIKEVICH_SYN, IZHIKEVICH_2007, IZHIKEVICH_2007_DOPSYN, BAYESIAN, INHOMOGENEOUS_POISSON };
 
 struct NeuronType {
 	std::string name;
@@ -192,6 +193,17 @@ NeuronType lookupType(const std::string& name)
 	if(name == "Input")         return NeuronType{name, INPUT, 0, 0, 0};
 	/* Kuramoto.ini:2-20: history=32, rcm sources/delays/weights */
 	if(name == "Kuramoto")      return NeuronType{name, KURAMOTO, 2, 1, 0, 32, true};
+	/* KuramotoLag.ini:1-14: a third parameter, the phase lag alpha */
+	if(name == "KuramotoLag")   return NeuronType{name, KURAMOTO_LAG, 3, 1, 0, 32, true};
+	/* the CPU-only models: IzhikevichChaos.ini, IzhikevichSyn.h (12 parameters),
+	 * Izhikevich2007.ini, Izhikevich2007DopSyn.ini, BayesianNeuron.ini,
+	 * InhomogeneousPoissonSource.ini */
+	if(name == "IzhikevichChaos")      return NeuronType{name, IZHIKEVICH_CHAOS, 5, 2, 1};
+	if(name == "IzhikevichSyn")        return NeuronType{name, IZHIKEVICH_SYN, 12, 4, 1};
+	if(name == "Izhikevich2007")       return NeuronType{name, IZHIKEVICH_2007, 16, 4, 1};
+	if(name == "Izhikevich2007DopSyn") return NeuronType{name, IZHIKEVICH_2007_DOPSYN, 18, 4, 1};
+	if(name == "BayesianNeuron")       return NeuronType{name, BAYESIAN, 3, 2, 1};
+	if(name == "InhomogeneousPoissonSource") return NeuronType{name, INHOMOGENEOUS_POISSON, 0, 0, 0};
 	throw oracle_error(NEMO_INVALID_INPUT, "unknown neuron type " + name);
 }
 
@@ -398,6 +410,9 @@ void updateHH(oracle_simulation& s, oracle_simulation::TypeGroup& g);
 void updatePoisson(oracle_simulation& s, oracle_simulation::TypeGroup& g);
 void updateInput(oracle_simulation& s, oracle_simulation::TypeGroup& g);
 void updateKuramoto(oracle_simulation& s, oracle_simulation::TypeGroup& g);
+void updateIzhikevichVariant(oracle_simulation& s, oracle_simulation::TypeGroup& g);
+void updateBayesian(oracle_simulation& s, oracle_simulation::TypeGroup& g);
+void updateInhomogeneousPoisson(oracle_simulation& s, oracle_simulation::TypeGroup& g);
 void initKuramoto(oracle_simulation::TypeGroup& g);
 
 void build(oracle_simulation& s, const oracle_network& net, const oracle_configuration& conf)
@@ -445,7 +460,7 @@ void build(oracle_simulation& s, const oracle_network& net, const oracle_configu
 		}
 		tg.rng.resize(tg.size);
 		initialiseRng(tg.base, tg.base + tg.size - 1, tg.rng);
-		if(tg.type.model == KURAMOTO) initKuramoto(tg);
+		if(tg.type.model == KURAMOTO || tg.type.model == KURAMOTO_LAG) initKuramoto(tg);
 		l_idx += tg.size;
 		s.groups.push_back(std::move(tg));
 	}
@@ -640,7 +655,12 @@ void fire(oracle_simulation& s)
 			case HH: updateHH(s, g); break;
 			case POISSON: updatePoisson(s, g); break;
 			case INPUT: updateInput(s, g); break;
-			case KURAMOTO: updateKuramoto(s, g); break;
+			case KURAMOTO: case KURAMOTO_LAG: updateKuramoto(s, g); break;
+			case IZHIKEVICH_CHAOS: case IZHIKEVICH_SYN: case IZHIKEVICH_2007: case IZHIKEVICH_2007_DOPSYN:
+				updateIzhikevichVariant(s, g);
+				break;
+			case BAYESIAN: updateBayesian(s, g); break;
+			case INHOMOGENEOUS_POISSON: updateInhomogeneousPoisson(s, g); break;
 		}
 	}
 	accumulateStdp(s);
@@ -898,8 +918,237 @@ void updateInput(oracle_simulation& s, oracle_simulation::TypeGroup& g)
 	}
 }
 
+/* The Izhikevich variants of NeMo's CPU-only plugins (per-type arrays and
+ * RNG indexed locally, as every model here):
+ *   IzhikevichChaos (cpu/plugins/IzhikevichChaos.cpp:14-101): the Izhikevich
+ *     step with 20 substeps of 0.05 ms, and the noise sigma * nrand drawn
+ *     every step and added to the u update instead of the input;
+ *   IzhikevichSyn (IzhikevichSyn.cpp:14-140, plugins/IzhikevichSyn.h):
+ *     conductances ge, gi decaying by a factor per step, incremented by the
+ *     excitatory and inhibitory input and clamped to G_exc and -G_inh, input
+ *     ge (E_exc - v) - gi (E_inh - v) + external (+ sigma nrand), the
+ *     Izhikevich substeps with threshold vpeak;
+ *   Izhikevich2007 (Izhikevich2007.cpp:14-134, plugins/Izhikevich2007.h):
+ *     C v' = k (v - Vr)(v - Vt) - u + I, u' = a (b (v - Vr) - u) in float,
+ *     conductances reset to 1 by any positive input, input
+ *     G_exc ge (E_exc - v) + G_inh gi (E_inh - v) + external (+ C sigma nrand);
+ *   Izhikevich2007DopSyn (Izhikevich2007DopSyn.cpp:14-156): dopamine levels
+ *     d1, d2 scale the excitatory input, Vr, k and d (constants of
+ *     plugins/Izhikevich2007DopSyn.h, in double), with IzhikevichSyn's
+ *     conductances and input. */
+void updateIzhikevichVariant(oracle_simulation& s, oracle_simulation::TypeGroup& g)
+{
+	const Model m = g.type.model;
+	const size_t stride = g.size;
+	auto P = [&](unsigned i) { return &g.param[i * stride]; };
+	const int nn = int(g.size);
+	float* currentEPSP = s.currentE.data();
+	float* currentIPSP = s.currentI.data();
+	float* currentExternal = s.currentExt.data();
+	unsigned* fstim = s.fstim.data();
+	unsigned* fired = s.fired.data();
+	uint64_t* recentFiring = s.recentFiring.data();
+	RNG* rng = g.rng.data();
+	float* u0 = &g.state[0 * stride];
+	float* v0 = &g.state[1 * stride];
+	float* ge0 = g.type.nState > 2 ? &g.state[2 * stride] : nullptr;
+	float* gi0 = g.type.nState > 3 ? &g.state[3 * stride] : nullptr;
+#pragma omp parallel for default(shared)
+	for(int nl = 0; nl < nn; nl++) {
+		const unsigned ng = g.base + nl;
+		float u = u0[nl];
+		float v = v0[nl];
+		if(m == IZHIKEVICH_CHAOS) {
+			const float *a = P(0), *b = P(1), *c = P(2), *d = P(3), *sigma = P(4);
+			const unsigned SUBSTEPS = 20;
+			const float SUBSTEP_MULT = 0.05f;
+			float I = currentEPSP[ng] + currentIPSP[ng] + currentExternal[ng];
+			currentExternal[ng] = 0.0f;
+			float randNoise = sigma[nl] * nrand(&rng[nl]);
+			fired[ng] = 0;
+			for(unsigned t = 0; t < SUBSTEPS; ++t) {
+				if(!fired[ng]) {
+					v += SUBSTEP_MULT * ((0.04 * v + 5.0) * v + 140.0 - u + I);
+					u += SUBSTEP_MULT * (a[nl] * (b[nl] * v - u) + randNoise);
+					fired[ng] = v >= 30.0;
+				}
+			}
+			fired[ng] |= fstim[ng];
+			fstim[ng] = 0;
+			recentFiring[ng] = (recentFiring[ng] << 1) | (uint64_t) fired[ng];
+			if(fired[ng]) {
+				v = c[nl];
+				u += d[nl];
+			}
+		} else if(m == IZHIKEVICH_SYN) {
+			const float *a = P(0), *b = P(1), *c = P(2), *d = P(3), *vpeak = P(4), *sigma = P(5);
+			const float *tMdtOt_exc = P(6), *G_exc = P(7), *E_exc = P(8);
+			const float *tMdtOt_inh = P(9), *G_inh = P(10), *E_inh = P(11);
+			const unsigned SUBSTEPS = 4;
+			const float SUBSTEP_MULT = 0.25f;
+			float ge = ge0[nl];
+			float gi = gi0[nl];
+			ge = ge * tMdtOt_exc[nl];
+			gi = gi * tMdtOt_inh[nl];
+			ge += currentEPSP[ng];
+			gi += currentIPSP[ng];
+			if(ge > G_exc[nl]) ge = G_exc[nl];
+			if(gi < -G_inh[nl]) gi = -G_inh[nl];
+			ge0[nl] = ge;
+			gi0[nl] = gi;
+			float I = ge * (E_exc[nl] - v) - gi * (E_inh[nl] - v) + currentExternal[ng];
+			currentExternal[ng] = 0.0f;
+			if(sigma[nl] != 0.0f) {
+				I += sigma[nl] * nrand(&rng[nl]);
+			}
+			fired[ng] = 0;
+			for(unsigned t = 0; t < SUBSTEPS; ++t) {
+				if(!fired[ng]) {
+					v += SUBSTEP_MULT * ((0.04 * v + 5.0) * v + 140.0 - u + I);
+					u += SUBSTEP_MULT * (a[nl] * (b[nl] * v - u));
+					fired[ng] = v >= vpeak[nl];
+				}
+			}
+			fired[ng] |= fstim[ng];
+			fstim[ng] = 0;
+			recentFiring[ng] = (recentFiring[ng] << 1) | (uint64_t) fired[ng];
+			if(fired[ng]) {
+				v = c[nl];
+				u += d[nl];
+			}
+		} else {
+			const bool dop = m == IZHIKEVICH_2007_DOPSYN;
+			const unsigned off = dop ? 2 : 0; /* d1, d2 sit between sigma and the synapse parameters */
+			const float *a = P(0), *b = P(1), *c = P(2), *d = P(3), *vpeak = P(4), *Vr = P(5), *Vt = P(6);
+			const float *k = P(7), *C = P(8), *sigma = P(9);
+			const float *d1 = dop ? P(10) : nullptr, *d2 = dop ? P(11) : nullptr;
+			const float *tMdtOt_exc = P(10 + off), *G_exc = P(11 + off), *E_exc = P(12 + off);
+			const float *tMdtOt_inh = P(13 + off), *G_inh = P(14 + off), *E_inh = P(15 + off);
+			const unsigned SUBSTEPS = 4;
+			const float SUBSTEP_MULT = 0.25f;
+			const double MSN_BETA1 = 6.3, MSN_BETA2 = 0.215, MSN_K = 0.0289, MSN_L = 0.331, MSN_ALPHA = 0.032;
+			float ge = ge0[nl];
+			float gi = gi0[nl];
+			ge = ge * tMdtOt_exc[nl];
+			gi = gi * tMdtOt_inh[nl];
+			float I;
+			if(dop) {
+				ge += (1.0f + MSN_BETA1 * d1[nl]) * (1.0f - MSN_BETA2 * d2[nl]) * currentEPSP[ng];
+				gi += currentIPSP[ng];
+				if(ge > G_exc[nl]) ge = G_exc[nl];
+				if(gi < -G_inh[nl]) gi = -G_inh[nl];
+				ge0[nl] = ge;
+				gi0[nl] = gi;
+				I = ge * (E_exc[nl] - v) - gi * (E_inh[nl] - v) + currentExternal[ng];
+			} else {
+				if(currentEPSP[ng] > 0.0) ge = 1.0f;
+				if(currentIPSP[ng] > 0.0) gi = 1.0f;
+				ge0[nl] = ge;
+				gi0[nl] = gi;
+				I = G_exc[nl] * ge * (E_exc[nl] - v) + G_inh[nl] * gi * (E_inh[nl] - v) + currentExternal[ng];
+			}
+			currentExternal[ng] = 0.0f;
+			if(sigma[nl] != 0.0f) {
+				if(dop) I += sigma[nl] * nrand(&rng[nl]);
+				else I += C[nl] * sigma[nl] * nrand(&rng[nl]);
+			}
+			fired[ng] = 0;
+			const float oneOverC = 1.0f / C[nl];
+			float msn_d = d[nl];
+			for(unsigned t = 0; t < SUBSTEPS; ++t) {
+				if(!fired[ng]) {
+					if(dop) {
+						const float msn_Vr = Vr[nl] * (1 + MSN_K * d1[nl]);
+						msn_d = d[nl] * (1 - MSN_L * d1[nl]);
+						const float msn_k = k[nl] * (1 - MSN_ALPHA * d2[nl]);
+						v += SUBSTEP_MULT * oneOverC * (msn_k * (v - msn_Vr) * (v - Vt[nl]) - u + I);
+						u += SUBSTEP_MULT * (a[nl] * (b[nl] * (v - msn_Vr) - u));
+					} else {
+						v += SUBSTEP_MULT * oneOverC * (k[nl] * (v - Vr[nl]) * (v - Vt[nl]) - u + I);
+						u += SUBSTEP_MULT * (a[nl] * (b[nl] * (v - Vr[nl]) - u));
+					}
+					fired[ng] = v >= vpeak[nl];
+				}
+			}
+			fired[ng] |= fstim[ng];
+			fstim[ng] = 0;
+			recentFiring[ng] = (recentFiring[ng] << 1) | (uint64_t) fired[ng];
+			if(fired[ng]) {
+				v = c[nl];
+				u += dop ? msn_d : d[nl];
+			}
+		}
+		u0[nl] = u;
+		v0[nl] = v;
+	}
+}
+
+/* cpu/plugins/BayesianNeuron.cpp:14-93 (plugins/BayesianNeuron.h: rOn, rOff,
+ * g0; state L, G): four Euler substeps of 0.25 of the log-odds L and the
+ * adaptation G, with the float exp; fires when L exceeds G + g0/2 (in
+ * double), and then G += g0. */
+void updateBayesian(oracle_simulation& s, oracle_simulation::TypeGroup& g)
+{
+	const size_t stride = g.size;
+	const float* rOn = &g.param[0 * stride];
+	const float* rOff = &g.param[1 * stride];
+	const float* g0 = &g.param[2 * stride];
+	float* L0 = &g.state[0 * stride];
+	float* G0 = &g.state[1 * stride];
+	const unsigned SUBSTEPS = 4;
+	const float SUBSTEP_MULT = 0.25f;
+	const int nn = int(g.size);
+#pragma omp parallel for default(shared)
+	for(int nl = 0; nl < nn; nl++) {
+		const unsigned ng = g.base + nl;
+		float I = s.currentE[ng] + s.currentI[ng] + s.currentExt[ng];
+		s.currentExt[ng] = 0.0f;
+		s.fired[ng] = 0;
+		float L = L0[nl];
+		float G = G0[nl];
+		const float rOn_this = rOn[nl];
+		const float rOff_this = rOff[nl];
+		const float g0_this = g0[nl];
+		for(unsigned t = 0; t < SUBSTEPS; ++t) {
+			if(!s.fired[ng]) {
+				/* exp of a float is the float overload (libstdc++ <math.h>) */
+				L += SUBSTEP_MULT * (rOn_this * (1 + std::exp(-L)) - rOff_this * (1 + std::exp(L)) + I);
+				G += SUBSTEP_MULT * (rOn_this * (1 + std::exp(-G)) - rOff_this * (1 + std::exp(G)));
+				s.fired[ng] = L > (G + g0_this / 2.0);
+			}
+		}
+		s.fired[ng] |= s.fstim[ng];
+		s.fstim[ng] = 0;
+		s.recentFiring[ng] = (s.recentFiring[ng] << 1) | (uint64_t) s.fired[ng];
+		if(s.fired[ng]) {
+			G += g0_this;
+		}
+		L0[nl] = L;
+		G0[nl] = G;
+	}
+}
+
+/* cpu/plugins/InhomogeneousPoissonSource.cpp:6-27: the firing rate (Hz) is
+ * the neuron's external current, which the model reads but does not clear
+ * (it holds from step to step until the current stimulus changes it);
+ * indexed by the neuron's simulation index (the reference indexes the
+ * current by the local one, correct when the type comes first). */
+void updateInhomogeneousPoisson(oracle_simulation& s, oracle_simulation::TypeGroup& g)
+{
+	for(unsigned ng = g.base, nl = 0; nl < g.size; ng++, nl++) {
+		const float rate = s.currentExt[ng];
+		unsigned p0 = unsigned(0.001 * rate * float(1 << 16));
+		unsigned p1 = urand(&g.rng[nl]) & 0xffff;
+		s.fired[ng] = p1 < p0 || s.fstim[ng];
+		s.fstim[ng] = 0;
+		s.recentFiring[ng] = (s.recentFiring[ng] << 1) | (uint64_t) s.fired[ng];
+	}
+}
+
 /* cpu/plugins/Kuramoto.cpp:14-134 (parameters plugins/Kuramoto.h: frequency,
- * mean coupling; state: the phase, 32 steps of history).  The phase of each
+ * mean coupling; state: the phase, 32 steps of history), and KuramotoLag
+ * (cpu/plugins/KuramotoLag.cpp:15-138, plugins/KuramotoLag.h: a third
+ * parameter alpha, subtracted inside every sine: sin(source - target - alpha)).  The phase of each
  * incoming synapse's source is read d-1 steps back; per-type arrays indexed
  * locally (deviation 1), every source being an oscillator of the same
  * group.  The source phase in the reference is taken from slot
@@ -911,6 +1160,7 @@ void updateKuramoto(oracle_simulation& s, oracle_simulation::TypeGroup& g)
 	const size_t stride = size_t(g.type.nState) * g.size; /* one history slot */
 	const float* frequency = &g.param[0];
 	const float* cmean = &g.param[g.size];
+	const bool lag = g.type.model == KURAMOTO_LAG;
 	const float* phase0 = &g.state[size_t(cycle % H) * stride];
 	float* phase1 = &g.state[size_t((cycle + 1) % H) * stride];
 	g.current = (cycle + 1) % H; /* cpu/Neurons.cpp:123 */
@@ -934,9 +1184,12 @@ void updateKuramoto(oracle_simulation& s, oracle_simulation::TypeGroup& g)
 			sourcePhase[r - r0] = g.state[size_t(slot) * stride + (src - g.base)];
 		}
 		/* sumN, Kuramoto.cpp:22-34 */
+		const float alpha = lag ? g.param[2 * size_t(g.size) + nl] : 0.0f;
 		auto sumN = [&](float target) {
 			float sum = 0.0f;
-			for(size_t k = 0; k < weight.size(); ++k) sum += weight[k] * sinf(sourcePhase[k] - target);
+			for(size_t k = 0; k < weight.size(); ++k) {
+				sum += weight[k] * (lag ? sinf(sourcePhase[k] - target - alpha) : sinf(sourcePhase[k] - target));
+			}
 			return sum;
 		};
 		const float Cmean = cmean[nl] > 0 ? cmean[nl] : 1;

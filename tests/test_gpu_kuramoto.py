@@ -154,3 +154,27 @@ def test_kuramoto_refusals():
     with pytest.raises(nemo_amd.NemoError) as e:
         nemo_amd.Simulation(net, nemo_amd.Configuration())
     assert "sources must be of the same type" in str(e.value)
+
+
+def test_kuramoto_lag_network_matches_oracle():
+    """KuramotoLag (a phase lag alpha per oscillator, cpu/plugins/
+    KuramotoLag.cpp): 350 oscillators, 50 incoming synapses each, delays
+    1..64, lags U[-1, 1): phases bit-exact with the oracle every step for 100
+    steps."""
+    rng = np.random.default_rng(77)
+    freq, phase, syn, cmean = oscillators(350, 50, 64, seed=77, cmean=50.0)
+    alpha = (rng.random(350) * 2 - 1).astype(np.float32)
+    sims = []
+    for api in (oa, nemo_amd):
+        net = api.Network()
+        t = net.add_neuron_type("KuramotoLag")
+        for i in range(350):
+            net.add_neuron(t, i, float(freq[i]), cmean, float(alpha[i]), float(phase[i]))
+        for s, g, d, w in syn:
+            net.add_synapse(s, g, d, w, 0)
+        sims.append(api.Simulation(net, api.Configuration()))
+    ref, hip = sims
+    for t in range(100):
+        ref.step()
+        hip.step()
+        assert same_f32(ref.neuron_states(0), hip.neuron_states(0)), t
