@@ -249,11 +249,12 @@ nemo_status_t nemo_hip_prepare(nemo_simulation_t, unsigned nsteps, unsigned stdp
  * delivery launch that follows the previous update, 3 a launch of its own on
  * a graph branch that may run until the next step's update, 4 in one launch
  * for up to 64 steps after them (every run of steps ends with it); shape[4] firing-bitmap
- * words per STDP block (1..8, 1..64 in STDP mode 4; at most 4 when the update
- * launches carry the STDP blocks), shape[5] when split, how the delays 2..D of the next step are
+ * words per STDP block (1..8, 1..64 in STDP mode 4), shape[5] when split, how the delays 2..D of the next step are
  * delivered beside the current update within nemo_hip_run: 0 not at all, 1 on
  * a graph branch, 2 as extra blocks of the update launch (which then also
- * carries the STDP blocks of STDP mode 2).  Every shape gives identical
+ * carries the STDP blocks of STDP mode 2), 3 as in 2 with the update's tiles
+ * delivering the delay-1 events of their own firing too, so that every step
+ * of a run after the first is one launch.  Every shape gives identical
  * results. */
 #define NEMO_HIP_SHAPE_FIELDS 6
 nemo_status_t nemo_hip_get_launch_shape(nemo_simulation_t, unsigned shape[NEMO_HIP_SHAPE_FIELDS]);

@@ -20,8 +20,8 @@
  * cpu_update_neurons, src/nemo/cpu/plugins/neuron_model.h:25-51):
  *   - sum the step's synaptic input of the tile: the binned events of the
  *     delivery blocks in the tile's two bin ranges (bin_table / bins, the
- *     buffers of step parity s % 2) plus the overflow accumulators acc (of
- *     the same parity),
+ *     buffers of step parity s % 2; the first set's only when bin_first_set
+ *     is nonzero) plus the overflow accumulators acc (of the same parity),
  *     excitatory (w >= 0) and inhibitory separately, saturate each sum to the
  *     int32 range and convert to float (src/nemo/fixedpoint.cpp:40-49); clear
  *     the overflow accumulators;
@@ -34,6 +34,8 @@
  *     reads the word of step s-1 and writes the word of step s;
  *   - write the tile's words of the step's firing bitmap,
  *     fired_bits[(s % NEMO_STEP_RING) * fired_words + index / 64];
+ *   - when co.push is set, deliver the delay-1 events of the step's firing
+ *     for step s+1 (see nemo_hip_colaunch);
  *   - when stdp.rcm_ptr is not NULL, run the STDP accumulation of step s-1 for
  *     the tile's neurons (see nemo_hip_stdp).
  * The helpers in nemo_amd/csrc/plugin_device.h implement all of this but the
@@ -56,6 +58,10 @@ extern "C" {
 #define NEMO_TILE 256
 /* Steps of firing bitmaps kept (power of two, > maximum delay + STDP window). */
 #define NEMO_STEP_RING 128
+/* Threads per block of an update launch that carries propagate blocks
+ * (nemo_hip_colaunch): the propagate blocks use all of them, a tile's block
+ * its first NEMO_TILE (the others end at once). */
+#define NEMO_COLAUNCH_BLOCK 512
 /* Most delivery blocks (bin regions) a step's input can come from. */
 #define NEMO_MAX_BIN_BLOCKS 1024
 /* Firing history: two planes of hist_pitch words, plane s % 2 after step s. */
@@ -201,7 +207,8 @@ typedef struct nemo_propagate_args {
 	int2* bins;                      /* [2][table_columns][bin_capacity] {target % NEMO_TILE, w} */
 	int64_t* acc;                    /* [2][2][acc_pitch] overflow accumulators, E plane then I plane */
 	size_t acc_pitch;
-	unsigned long long* block_events; /* [table_columns] events delivered, accumulated */
+	unsigned long long* block_events; /* events delivered, accumulated: [NEMO_MAX_BIN_BLOCKS] per table
+	                                     column, then one per 64 device indices (pushed events) */
 	/* STDP in its own launch */
 	unsigned stdp_words;             /* bitmap words (64 targets) per STDP block */
 	uint64_t* hist;                  /* two planes of hist_pitch words (NEMO_HIST_INDEX) */
@@ -213,13 +220,22 @@ typedef struct nemo_propagate_args {
 /* Propagate blocks an update launch carries after its tiles: the delivery
  * blocks of args (args.step_offset names their step, the next one) and then
  * stdp_blocks STDP blocks (of args' step minus two, the step before the
- * update's).  They need lds_bytes of dynamic LDS, which the launch requests
- * for all its blocks.  Both counts are zero when the launch carries none;
- * only the launch of the first neuron group of a step carries any. */
+ * update's).  They need lds_bytes of dynamic LDS and NEMO_COLAUNCH_BLOCK
+ * threads, which the launch requests for all its blocks.  Both counts are zero when the launch carries none;
+ * only the launch of the first neuron group of a step carries any.
+ *
+ * push (every group's launch of the step, when set): record_firing delivers
+ * the delay-1 events of the step's firing itself, for the next step, into
+ * that step's overflow accumulators (acc of the next step's parity; args
+ * gives fcm, acc and acc_pitch; push_rows the delay-1 row of every device
+ * index, which gather_input reads ahead) and counts them in the block_events
+ * entry of their wave (NEMO_MAX_BIN_BLOCKS + device index / 64). */
 typedef struct nemo_hip_colaunch {
 	unsigned deliver_blocks;
 	unsigned stdp_blocks;
 	unsigned lds_bytes;
+	unsigned push;
+	const uint2* push_rows;          /* per device index n: fcm [x, y) of row (n, delay 1) */
 	nemo_propagate_args args;
 } nemo_hip_colaunch;
 
@@ -258,6 +274,10 @@ typedef struct nemo_hip_step_buffers {
 	const float* ext_held;     /* the external current of every device index, also in steps
 	                              without a current stimulus (ext_current NULL): for a model
 	                              that reads it without clearing it, as a held value */
+	unsigned bin_first_set;    /* nonzero: this step's binned events lie in the first delivery-block
+	                              set's columns only, [0, bin_first_set) of each table row (with
+	                              bin_ranged: in the tile's first range only); the delay-1 events
+	                              were pushed into acc by the previous step's update */
 } nemo_hip_step_buffers;
 
 typedef hipError_t hip_update_neurons_t(hipStream_t stream,

@@ -451,7 +451,9 @@ void Configuration::setHipBackend(int dev)
 
 /* LDS an update kernel declares itself, on top of what the propagate blocks
  * it may carry need */
-static const size_t COLAUNCH_LDS_MARGIN = 24576;
+/* static LDS of an update kernel that carries propagate blocks, beside the
+ * dynamic buffer both kinds of block share (block scans' wave totals) */
+static const size_t COLAUNCH_LDS_MARGIN = 1024;
 
 /* ------------------------------------------------------------- simulation */
 
@@ -880,6 +882,13 @@ void Simulation::buildConnectivity(const std::vector<const Network*>& nets, cons
 
 	m_rowptr.allocate(rowptr.size());
 	HIP_CHECK(hipMemcpy(m_rowptr.ptr, rowptr.data(), m_rowptr.bytes(), hipMemcpyHostToDevice));
+	{
+		/* the delay-1 rows by themselves, for updates that push them */
+		std::vector<uint2> d1(N);
+		for(size_t n = 0; n < N; ++n) d1[n] = make_uint2(rowptr[n * D], rowptr[n * D + 1]);
+		m_d1Rows.allocate(N);
+		HIP_CHECK(hipMemcpy(m_d1Rows.ptr, d1.data(), m_d1Rows.bytes(), hipMemcpyHostToDevice));
+	}
 	m_fcm.allocate(fcm.size());
 	HIP_CHECK(hipMemcpy(m_fcm.ptr, fcm.data(), m_fcm.bytes(), hipMemcpyHostToDevice));
 	size_t rows = 0;
@@ -1175,7 +1184,7 @@ LaunchShape Simulation::defaultShape() const
 	env("NEMO_HIP_STDP_MODE", 0, 4, s.stdpMode);
 	env("NEMO_HIP_STDP_WORDS", 1, NEMO_STDP_BATCH_MAX_WORDS, s.stdpWords);
 	if(s.stdpMode != 4) s.stdpWords = std::min<unsigned>(s.stdpWords, NEMO_STDP_MAX_WORDS);
-	env("NEMO_HIP_OVERLAP", 0, 2, s.overlap);
+	env("NEMO_HIP_OVERLAP", 0, 3, s.overlap);
 	return s;
 }
 
@@ -1190,9 +1199,9 @@ size_t Simulation::ldsNeeded(const LaunchShape& s) const
 	}
 	if(split) w = std::max(w, partitionPairs(m_pairWeight, m_srcWindow, m_D, m_tiles, 1, 1, s.lateBlocks).maxWindow);
 	size_t need = nemo_propagate_lds_bytes(w, s.binCapacity);
-	if(split && s.overlap == 2) {
-		/* the update launches carry delivery blocks: their LDS comes on top
-		 * of the update kernels' own */
+	if(split && s.overlap >= 2) {
+		/* the update launches carry delivery blocks, which share their
+		 * dynamic LDS with the update blocks */
 		const size_t co = nemo_colaunch_lds_bytes(w, s.binCapacity, 1) + COLAUNCH_LDS_MARGIN;
 		if(co > m_ldsBudget) need = std::max(need, co);
 	}
@@ -1207,12 +1216,12 @@ void Simulation::checkShape(const LaunchShape& s) const
 	if(s.deliverBlocks == 0 && (s.lateBlocks == 0 || m_D >= 2)) {
 		throw Error(NEMO_INVALID_INPUT, "the delays delivered by the first block set need at least one block");
 	}
-	if(s.overlap > 2) {
-		throw Error(NEMO_INVALID_INPUT, "overlap must be 0, 1 or 2");
+	if(s.overlap > 3) {
+		throw Error(NEMO_INVALID_INPUT, "overlap must be 0, 1, 2 or 3");
 	}
-	if(s.lateBlocks && s.overlap == 2 && s.stdpMode == 2 && s.stdpWords > NEMO_TILE / 64) {
+	if(s.lateBlocks && s.overlap >= 2 && s.stdpMode == 2 && s.stdpWords > NEMO_COLAUNCH_BLOCK / 64) {
 		throw Error(NEMO_INVALID_INPUT, "STDP blocks carried by the update launches cover at most "
-				+ std::to_string(NEMO_TILE / 64) + " bitmap words");
+				+ std::to_string(NEMO_COLAUNCH_BLOCK / 64) + " bitmap words");
 	}
 	if(s.binCapacity < 64 || s.binCapacity > 32768) {
 		throw Error(NEMO_INVALID_INPUT, "bin capacity must be in [64, 32768]");
@@ -1243,7 +1252,7 @@ void Simulation::applyShape(const LaunchShape& s)
 	const Partition Pl = partitionPairs(m_pairWeight, m_srcWindow, m_D, m_tiles, 1, 1, Bl);
 	const unsigned maxWindow = std::max({1u, Pe.maxWindow, Pl.maxWindow});
 	if(nemo_propagate_lds_bytes(maxWindow, s.binCapacity) > m_ldsBudget
-			|| (split && s.overlap == 2
+			|| (split && s.overlap >= 2
 				&& nemo_colaunch_lds_bytes(maxWindow, s.binCapacity, 1) + COLAUNCH_LDS_MARGIN > m_ldsBudget)) {
 		throw Error(NEMO_INVALID_INPUT, "launch shape needs more LDS than a block can have");
 	}
@@ -1388,7 +1397,8 @@ void Simulation::allocateStepState()
 				size_t(NEMO_STEP_RING) * m_firedWords * sizeof(uint64_t), hipHostMallocDefault));
 
 	/* work counters, sized for any launch shape */
-	m_blockEvents.allocate(NEMO_MAX_BIN_BLOCKS);
+	/* per table column, then per 64 device indices (events pushed by updates) */
+	m_blockEvents.allocate(NEMO_MAX_BIN_BLOCKS + m_npad / 64);
 	HIP_CHECK(hipMemset(m_blockEvents.ptr, 0, m_blockEvents.bytes()));
 	m_blockExamined.allocate(std::max<size_t>(1, m_firedWords));
 	HIP_CHECK(hipMemset(m_blockExamined.ptr, 0, m_blockExamined.bytes()));
@@ -1507,7 +1517,12 @@ void Simulation::launchDelivery(const nemo_propagate_args& set, uint64_t offset,
  * step t or, with one step of slack, before the update of step t+1: it reads
  * the history plane of step t-1, which the update of step t+1 rewrites, and
  * the accumulators it updates are serialised on the branch's stream.  The
- * branch of the last step is joined at the end of the run.  In STDP mode 4
+ * branch of the last step is joined at the end of the run.  With overlap 2
+ * the update launch of step t carries the delivery of delays 2..D of step
+ * t+1 instead, and with overlap 3 its tiles also push their delay-1 events
+ * for step t+1, whose update then reads only the first set's bins.  Within
+ * the run only: the next run (after an applyStdp, say) delivers its first
+ * step from scratch, with the weights of that time.  In STDP mode 4
  * the pending batch runs on a branch beside all the steps and a history
  * snapshot ends the run (and every 64 steps).  With ev (four events per
  * step), the kernels record the start and end of the step's delivery on this
@@ -1517,8 +1532,11 @@ void Simulation::enqueueSteps(uint64_t first, unsigned k, bool ext, bool fstim, 
 	const bool split = m_late.deliver_blocks && m_early.deliver_blocks;
 	const bool overlap = split && m_shape.overlap == 1 && k > 1;
 	/* the update of step t carries the delivery of the delays 2..D of step
-	 * t+1 as extra blocks (and, with fused STDP, the STDP blocks) */
-	const bool colaunch = split && m_shape.overlap == 2;
+	 * t+1 as extra blocks (and, with fused STDP, the STDP blocks); with
+	 * overlap 3 its tiles also push the delay-1 events of their firing into
+	 * the accumulators of step t+1, which then needs no delivery launch */
+	const bool colaunch = split && m_shape.overlap >= 2;
+	const bool pushed = split && m_shape.overlap == 3;
 	/* STDP shares the delivery launch that follows the previous update, or
 	 * the update launch when that carries propagate blocks */
 	const bool fused = m_shape.stdpMode == 2 && m_stdpBlocks;
@@ -1565,7 +1583,13 @@ void Simulation::enqueueSteps(uint64_t first, unsigned k, bool ext, bool fstim, 
 		/* with ev: four events per step, recorded by the kernels: start
 		 * and end of the delivery on this stream, of the neuron updates */
 		hipEvent_t ps = ev ? ev[4 * i] : nullptr, pe = ev ? ev[4 * i + 1] : nullptr;
-		if(split) {
+		if(split && pushed && i > 0) {
+			/* delivered by the previous step's update launch */
+			if(ev) {
+				HIP_CHECK(hipEventRecord(ps, m_stream));
+				HIP_CHECK(hipEventRecord(pe, m_stream));
+			}
+		} else if(split) {
 			const bool early = i == 0 || (!overlap && !colaunch);
 			if(early) launchDelivery(m_early, off, m_stream, false, ps, nullptr);
 			if(i == 0 && overlap) forkEarly(1);
@@ -1590,7 +1614,10 @@ void Simulation::enqueueSteps(uint64_t first, unsigned k, bool ext, bool fstim, 
 			b.co.lds_bytes = b.co.deliver_blocks || b.co.stdp_blocks
 				? unsigned(nemo_colaunch_lds_bytes(m_early.max_window, m_early.bin_capacity, b.co.deliver_blocks > 0))
 				: 0;
+			b.co.push = pushed && i + 1 < k ? 1u : 0u;
+			b.co.push_rows = m_d1Rows.ptr;
 		}
+		if(pushed && i > 0) b.bin_first_set = m_early.deliver_blocks;
 		size_t last = m_groups.size();
 		for(size_t gi = 0; gi < m_groups.size(); ++gi) {
 			if(m_groups[gi].dev.count) last = gi;
@@ -2188,7 +2215,7 @@ LaunchShape Simulation::autotune(unsigned stepsPerTrial)
 		}
 		const LaunchShape before = best;
 		for(unsigned b : late) {
-			for(unsigned ov : {0u, 1u, 2u}) {
+			for(unsigned ov : {0u, 1u, 2u, 3u}) {
 				if(b == 0 && ov != 0) continue;
 				LaunchShape c = before;
 				c.lateBlocks = b;
@@ -2196,7 +2223,7 @@ LaunchShape Simulation::autotune(unsigned stepsPerTrial)
 				consider(c);
 			}
 		}
-		if(best.lateBlocks && best.overlap == 2) {
+		if(best.lateBlocks && best.overlap >= 2) {
 			/* the delays 2..D now run beside the update: retune their blocks */
 			for(unsigned b : scaled(best.deliverBlocks, {0.5, 2.0}, 1, NEMO_MAX_BIN_BLOCKS - best.lateBlocks)) {
 				LaunchShape c = best;

@@ -158,7 +158,8 @@ struct LaunchShape {
 	unsigned stdpWords = 2;       /* firing-bitmap words (64 targets each) per STDP block */
 	unsigned overlap = 1;         /* when split, deliver the delays 2..D of a step early: 1 on a graph
 	                                 branch beside the previous update, 2 as extra blocks of the
-	                                 previous update launch */
+	                                 previous update launch, 3 as in 2 with the previous update's
+	                                 tiles pushing the delay-1 events too (one launch per step) */
 };
 
 /* The HIP simulation (cuda::Simulation's role, cuda/Simulation.cpp:91-380).
@@ -321,6 +322,7 @@ private:
 	DeviceArray<float> m_debugCurrent;
 	bool m_debug = false;
 	DeviceArray<uint32_t> m_rowptr;
+	DeviceArray<uint2> m_d1Rows;      /* [N] the delay-1 row of each device index: fcm [x, y) */
 	DeviceArray<int2> m_fcm;
 	DeviceArray<uint32_t> m_rcmPtr, m_rcmSd, m_rcmFwd;
 	DeviceArray<uint32_t> m_rcmOf;    /* per FCM synapse: accumulator index, ~0 if static */

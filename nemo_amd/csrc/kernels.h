@@ -43,7 +43,8 @@ hipError_t nemo_launch_stdp_batch(hipStream_t, const nemo_propagate_args*, unsig
  * accumulation.  The pending batch must have run. */
 hipError_t nemo_launch_stdp_snapshot(hipStream_t, const nemo_propagate_args*, uint64_t offset);
 /* dynamic LDS an update launch carrying propagate blocks needs (delivery
- * blocks when delivery is set, else STDP blocks only) */
+ * blocks when delivery is set, else STDP blocks only; the launch's update
+ * blocks share the buffer) */
 size_t nemo_colaunch_lds_bytes(unsigned max_window, unsigned bin_capacity, int delivery);
 /* Static LDS of the propagate kernel (bytes). */
 size_t nemo_propagate_static_lds(void);

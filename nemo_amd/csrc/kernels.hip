@@ -167,7 +167,7 @@ external_fire_kernel(nemo_hip_step_buffers b, const uint32_t* flags, const uint3
 __global__ void __launch_bounds__(NEMO_TILE)
 gather_currents_kernel(nemo_hip_step_buffers b, const uint32_t* dev_to_sim, float* E, float* I)
 {
-	__shared__ nemo_dev::TileInput input;
+	nemo_dev::UpdateLds& L = nemo_dev::update_lds<false>();
 	nemo_hip_neuron_group g;
 	g.start = 0;
 	g.count = gridDim.x * NEMO_TILE;
@@ -176,10 +176,10 @@ gather_currents_kernel(nemo_hip_step_buffers b, const uint32_t* dev_to_sim, floa
 	g.state = nullptr;
 	g.rng = nullptr;
 	g.scratch = nullptr;
-	nemo_dev::gather_input(b, g, input);
+	nemo_dev::gather_input(b, g, L);
 	const unsigned ng = blockIdx.x * NEMO_TILE + threadIdx.x;
 	float e, i;
-	nemo_dev::take_current(b, input, ng, e, i);
+	nemo_dev::take_current(b, L.input, ng, e, i);
 	const uint32_t sim = dev_to_sim[ng];
 	if(sim != ~0u) {
 		E[sim] = e;
@@ -250,7 +250,9 @@ hipError_t nemo_launch_stdp_snapshot(hipStream_t st, const nemo_propagate_args* 
 
 size_t nemo_colaunch_lds_bytes(unsigned max_window, unsigned bin_capacity, int delivery)
 {
-	return nemo_dev::PropagateLds<NEMO_TILE>::bytes(delivery != 0, bin_capacity, max_window);
+	/* the update blocks of the launch take their LDS from the same buffer */
+	return std::max(nemo_dev::PropagateLds<NEMO_COLAUNCH_BLOCK>::bytes(delivery != 0, bin_capacity, max_window),
+			sizeof(nemo_dev::UpdateLds));
 }
 
 size_t nemo_propagate_static_lds(void)

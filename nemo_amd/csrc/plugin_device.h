@@ -3,11 +3,12 @@
  *
  * An update kernel runs one block of NEMO_TILE threads per tile of its neuron
  * type, one neuron per thread, and every thread of a block calls
+ *   UpdateLds& L = update_lds<CO>();            (CO: the launch carries propagate blocks)
  *   h = load_history(b, ng, valid);
- *   TileInput in; gather_input(b, g, in);       (block barriers inside)
- *   take_current(b, in, ng, E, I);              (valid threads)
- *   record_firing(b, ng, valid, fired, h);      (every thread)
- *   stdp_tile(b, ng, valid, h);                 (every thread; barriers inside)
+ *   gather_input(b, g, L);                      (block barriers inside)
+ *   take_current(b, L.input, ng, E, I);         (valid threads)
+ *   record_firing(b, L, ng, valid, fired, h);   (every thread; pushes the delay-1 events when asked)
+ *   stdp_tile(b, L, ng, valid, h);              (every thread; barriers inside)
  */
 #ifndef NEMO_AMD_PLUGIN_DEVICE_H
 #define NEMO_AMD_PLUGIN_DEVICE_H
@@ -99,6 +100,45 @@ struct TileInput {
 	long long I[NEMO_TILE];
 };
 
+/* Gather batches: GATHER_ILP events per thread in flight; an event's column
+ * is found from buckets of 2^GATHER_SHIFT events, marked per column when the
+ * tile has at most GATHER_MARKED events. */
+constexpr int GATHER_ILP = 8;
+constexpr unsigned GATHER_BATCH = GATHER_ILP * NEMO_TILE;
+constexpr unsigned GATHER_SHIFT = 3;
+constexpr unsigned GATHER_MARKED = 4096;
+constexpr unsigned GATHER_BUCKETS = (GATHER_MARKED > GATHER_BATCH ? GATHER_MARKED : GATHER_BATCH) / (1u << GATHER_SHIFT)
+	+ 2;
+
+/* The LDS an update block uses: its tile's input sums, the gather's column
+ * prefix, bases and buckets, and the per-tile STDP lists.  A kernel takes it
+ * from update_lds<DYNAMIC>(): a static array, or -- in launches that also run
+ * propagate blocks -- the start of the launch's dynamic LDS, which those
+ * blocks use for their own purposes, so that a block of either kind needs
+ * only the larger of the two. */
+struct UpdateLds {
+	TileInput input;
+	uint32_t pre[NEMO_MAX_BIN_BLOCKS + 1];
+	uint32_t base[NEMO_MAX_BIN_BLOCKS];
+	uint32_t bucket[GATHER_BUCKETS];
+	int32_t fn[128];
+	uint32_t rb[NEMO_TILE];
+	uint32_t pos[NEMO_TILE + 1];
+	uint2 push_rows[NEMO_TILE]; /* the tile's delay-1 rows, read ahead for push_delay1 */
+};
+
+template<bool DYNAMIC>
+__device__ inline UpdateLds& update_lds()
+{
+	if constexpr(DYNAMIC) {
+		extern __shared__ uint64_t nemo_update_lds_dyn[];
+		return *reinterpret_cast<UpdateLds*>(nemo_update_lds_dyn);
+	} else {
+		__shared__ UpdateLds lds;
+		return lds;
+	}
+}
+
 /* Sum the tile's input of this step into LDS: the overflow accumulators of
  * its neurons (cleared here, cpu/Simulation.cpp:279-288) and the binned events
  * of every delivery column (integer sums, so the order of the 64-bit LDS
@@ -109,15 +149,24 @@ struct NoPrefetch {
 	__device__ void operator()() const {}
 };
 
+/* The delay-1 row of device index ng when the launch pushes (co.push), for
+ * L.push_rows[threadIdx.x]: gather_input reads it ahead; a kernel that
+ * records firing without gathering stores it itself. */
+__device__ inline uint2 push_row(const nemo_hip_step_buffers& b, unsigned ng)
+{
+	return b.co.push ? b.co.push_rows[ng] : make_uint2(0, 0);
+}
+
 /* prefetch() is called once the tile's table row and accumulators are
  * requested: the loads it issues (the model's state, say) queue behind them,
  * so that waiting for the table does not wait for them too. */
 template<typename Prefetch = NoPrefetch>
-__device__ inline void gather_input(const nemo_hip_step_buffers& b, const nemo_hip_neuron_group& g, TileInput& in,
+__device__ inline void gather_input(const nemo_hip_step_buffers& b, const nemo_hip_neuron_group& g, UpdateLds& L,
 		Prefetch prefetch = Prefetch())
 {
-	__shared__ uint32_t s_pre[NEMO_MAX_BIN_BLOCKS + 1];
-	__shared__ uint32_t s_base[NEMO_MAX_BIN_BLOCKS];
+	TileInput& in = L.input;
+	uint32_t* s_pre = L.pre;
+	uint32_t* s_base = L.base;
 	const unsigned tid = threadIdx.x;
 	const unsigned tile = g.start / NEMO_TILE + blockIdx.x;
 	const size_t parity = current_step(b) & 1;
@@ -129,13 +178,14 @@ __device__ inline void gather_input(const nemo_hip_step_buffers& b, const nemo_h
 	const int2* bins = b.bins + parity * BB * cap;
 	/* the columns read: the whole row, or (bin_ranged) the tile's two ranges,
 	 * column first0 + i for i < n0, then first1 + i - n0 */
-	unsigned B = BB, first0 = 0, n0 = BB, first1 = 0;
+	const unsigned BR = b.bin_first_set ? b.bin_first_set : BB; /* columns holding events */
+	unsigned B = BR, first0 = 0, n0 = BR, first1 = 0;
 	if(b.bin_ranged) {
 		const uint32_t r0 = b.bin_range[tile], r1 = b.bin_range[b.tiles + tile];
 		first0 = r0 & 0xffffu;
 		n0 = (r0 >> 16) - first0;
 		first1 = r1 & 0xffffu;
-		B = n0 + ((r1 >> 16) - first1);
+		B = n0 + (b.bin_first_set ? 0u : (r1 >> 16) - first1);
 	}
 	constexpr unsigned PER = NEMO_MAX_BIN_BLOCKS / NEMO_TILE;
 	uint32_t ent[PER];
@@ -148,9 +198,11 @@ __device__ inline void gather_input(const nemo_hip_step_buffers& b, const nemo_h
 	}
 	const int64_t oe = acc[ng];
 	const int64_t oi = acc[b.acc_pitch + ng];
+	const uint2 prow = push_row(b, ng);
 	prefetch();
 	in.E[tid] = oe;
 	in.I[tid] = oi;
+	L.push_rows[tid] = prow;
 	if(oe) acc[ng] = 0;
 	if(oi) acc[b.acc_pitch + ng] = 0;
 	if(B == 0) {
@@ -169,11 +221,10 @@ __device__ inline void gather_input(const nemo_hip_step_buffers& b, const nemo_h
 	 * between the columns of the bucket bounds around it (every 8th event),
 	 * which each column marks when the tile has at most MARKED events, or
 	 * which are searched per batch; it is refined between them. */
-	constexpr int GATHER_ILP = 8;
-	constexpr unsigned BATCH = GATHER_ILP * NEMO_TILE;
-	constexpr unsigned SHIFT = 3;
-	constexpr unsigned MARKED = 4096;
-	__shared__ uint32_t s_bucket[(MARKED > BATCH ? MARKED : BATCH) / (1u << SHIFT) + 2];
+	constexpr unsigned BATCH = GATHER_BATCH;
+	constexpr unsigned SHIFT = GATHER_SHIFT;
+	constexpr unsigned MARKED = GATHER_MARKED;
+	uint32_t* s_bucket = L.bucket;
 	uint32_t total;
 	uint32_t pre = block_exclusive_scan(mine, &total);
 	const bool marked = total <= MARKED;
@@ -315,6 +366,65 @@ __device__ inline uint64_t load_history(const nemo_hip_step_buffers& b, unsigned
 	return valid ? b.hist[NEMO_HIST_INDEX(ng, current_step(b) - 1, b.hist_pitch)] : 0;
 }
 
+/* Delay-1 delivery of this step's firing for the next step, straight into
+ * that step's overflow accumulators (nemo_hip_colaunch.push), as the delay-1
+ * part of cpu::Simulation::deliverSpikes (src/nemo/cpu/Simulation.cpp:
+ * 259-308).  Each wave spreads the delay-1 rows of its fired neurons (read
+ * ahead into L.push_rows by gather_input) over its lanes, PUSH_ILP loads in
+ * flight per lane before their 64-bit atomics; the sums are integer, so
+ * their order does not matter.  Every thread of the wave calls it. */
+constexpr int PUSH_ILP = 4;
+__device__ inline void push_delay1(const nemo_hip_step_buffers& b, const UpdateLds& L, unsigned ng, bool valid,
+		bool fired)
+{
+	const nemo_propagate_args& a = b.co.args;
+	const bool f = valid && fired;
+	if(!__any(f)) return;
+	const unsigned lane = threadIdx.x & 63;
+	uint32_t rb = 0, len = 0;
+	if(f) {
+		const uint2 r = L.push_rows[threadIdx.x];
+		rb = r.x;
+		len = r.y - r.x;
+	}
+	uint32_t incl = len;
+#pragma unroll
+	for(int o = 1; o < 64; o <<= 1) {
+		const uint32_t y = __shfl_up(incl, o);
+		if(int(lane) >= o) incl += y;
+	}
+	const uint32_t total = __shfl(incl, 63);
+	const uint32_t excl = incl - len;
+	int64_t* acc = a.acc + ((current_step(b) + 1) & 1) * 2 * a.acc_pitch;
+	for(uint32_t e0 = 0; e0 < total; e0 += 64 * PUSH_ILP) {
+		int2 ev[PUSH_ILP];
+#pragma unroll
+		for(int u = 0; u < PUSH_ILP; ++u) {
+			const uint32_t e = e0 + u * 64 + lane;
+			/* the lane whose row holds event e: the first whose inclusive
+			 * prefix exceeds e (the prefix is non-decreasing over lanes) */
+			unsigned s = 0;
+#pragma unroll
+			for(unsigned step = 32; step >= 1; step >>= 1) {
+				if(__shfl(incl, int(s + step - 1)) <= e) s += step;
+			}
+			const uint32_t srb = __shfl(rb, int(s));
+			const uint32_t sex = __shfl(excl, int(s));
+			ev[u] = make_int2(0, 0);
+			if(e < total) ev[u] = a.fcm[srb + (e - sex)];
+		}
+#pragma unroll
+		for(int u = 0; u < PUSH_ILP; ++u) {
+			if(e0 + u * 64 + lane < total) {
+				int64_t* dst = acc + (ev[u].y >= 0 ? size_t(0) : a.acc_pitch) + uint32_t(ev[u].x);
+				atomicAdd(reinterpret_cast<unsigned long long*>(dst), (unsigned long long)(int64_t)ev[u].y);
+			}
+		}
+	}
+	/* the wave's own counter: atomics of many waves on one address queue */
+	if(lane == 0) atomicAdd(&a.block_events[NEMO_MAX_BIN_BLOCKS + (ng >> 6)], (unsigned long long)total);
+}
+
 /* Store the shifted history (h as from load_history) and write the step's
  * firing bitmap. */
 __device__ inline void record_firing(const nemo_hip_step_buffers& b, unsigned ng, bool valid, bool fired, uint64_t h)
@@ -323,6 +433,15 @@ __device__ inline void record_firing(const nemo_hip_step_buffers& b, unsigned ng
 		b.hist[NEMO_HIST_INDEX(ng, current_step(b), b.hist_pitch)] = (h << 1) | uint64_t(fired ? 1 : 0);
 	}
 	store_firing(b, ng, valid, fired);
+}
+
+/* The same in an update block whose input gather_input summed into L: with
+ * co.push, also deliver the firing's delay-1 events. */
+__device__ inline void record_firing(const nemo_hip_step_buffers& b, const UpdateLds& L, unsigned ng, bool valid,
+		bool fired, uint64_t h)
+{
+	record_firing(b, ng, valid, fired, h);
+	if(b.co.push) push_delay1(b, L, ng, valid, fired);
 }
 
 /* StdpProcess::updateRegion (StdpProcess.cpp:86-111): within one region of
@@ -350,7 +469,7 @@ __device__ inline int32_t stdp_wrap_add(int32_t x, int32_t y)
 /* The STDP accumulation of the previous step for this tile's neurons (the
  * contract of nemo_hip_stdp).  h is the thread's neuron history after the
  * previous step (load_history).  Every thread of the block calls it. */
-__device__ inline void stdp_tile(const nemo_hip_step_buffers& b, unsigned ng, bool valid, uint64_t h)
+__device__ inline void stdp_tile(const nemo_hip_step_buffers& b, UpdateLds& L, unsigned ng, bool valid, uint64_t h)
 {
 	constexpr int ILP = 8;
 	const nemo_hip_stdp& st = b.stdp;
@@ -360,9 +479,9 @@ __device__ inline void stdp_tile(const nemo_hip_step_buffers& b, unsigned ng, bo
 	const uint64_t r = t - 1;
 	if(r < st.post_window || r == b.clock->stdp_flushed) return;
 
-	__shared__ int32_t s_fn[128];
-	__shared__ uint32_t s_rb[NEMO_TILE];
-	__shared__ uint32_t s_pos[NEMO_TILE + 1];
+	int32_t* s_fn = L.fn;
+	uint32_t* s_rb = L.rb;
+	uint32_t* s_pos = L.pos;
 	const unsigned tid = threadIdx.x;
 	if(tid < 64) {
 		s_fn[tid] = st.fn_pre[tid];

@@ -71,12 +71,12 @@ __device__ inline void hh_substep(float& v, float& n, float& m, float& h, float&
 /* CO: the launch may carry propagate blocks after the tiles (compiled
  * out otherwise, which leaves the update far fewer registers) */
 template<bool CO>
-__global__ void __launch_bounds__(NEMO_UPDATE_BLOCK)
+__global__ void __launch_bounds__(CO ? NEMO_COLAUNCH_BLOCK : NEMO_UPDATE_BLOCK)
 hh_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 {
 	if(CO && nemo_dev::colaunched_block(b, g)) return;
-	__shared__ nemo_dev::TileInput input;
-	nemo_dev::gather_input(b, g, input);
+	nemo_dev::UpdateLds& L = nemo_dev::update_lds<CO>();
+	nemo_dev::gather_input(b, g, L);
 	const unsigned nl = blockIdx.x * NEMO_UPDATE_BLOCK + threadIdx.x;
 	const bool valid = nl < g.count;
 	const unsigned nn = g.start + nl;
@@ -92,7 +92,7 @@ hh_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 		float h = s[3 * P + nl];
 		float dir = s[4 * P + nl];
 		float Excit, Inhib;
-		nemo_dev::take_current(b, input, nn, Excit, Inhib);
+		nemo_dev::take_current(b, L.input, nn, Excit, Inhib);
 		const float Exter = nemo_dev::take_external(b, nn);
 		const uint64_t cycle = nemo_dev::current_step(b) - b.clock->cycle_origin;
 		const int inc_max = (int)(1 / DT);
@@ -112,26 +112,26 @@ hh_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 		s[3 * P + nl] = h;
 		s[4 * P + nl] = dir;
 	}
-	nemo_dev::record_firing(b, nn, valid, fired, hist);
-	nemo_dev::stdp_tile(b, nn, valid, hist);
+	nemo_dev::record_firing(b, L, nn, valid, fired, hist);
+	nemo_dev::stdp_tile(b, L, nn, valid, hist);
 }
 
 /* Split step, launch 1: the tiles' synaptic and external input. */
 /* CO: the launch may carry propagate blocks after the tiles (compiled
  * out otherwise, which leaves the update far fewer registers) */
 template<bool CO>
-__global__ void __launch_bounds__(NEMO_UPDATE_BLOCK)
+__global__ void __launch_bounds__(CO ? NEMO_COLAUNCH_BLOCK : NEMO_UPDATE_BLOCK)
 hh_input(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 {
 	if(CO && nemo_dev::colaunched_block(b, g)) return;
-	__shared__ nemo_dev::TileInput input;
-	nemo_dev::gather_input(b, g, input);
+	nemo_dev::UpdateLds& L = nemo_dev::update_lds<CO>();
+	nemo_dev::gather_input(b, g, L);
 	const unsigned nl = blockIdx.x * NEMO_UPDATE_BLOCK + threadIdx.x;
 	if(nl < g.count) {
 		const unsigned nn = g.start + nl;
 		const size_t P = g.pitch;
 		float Excit, Inhib;
-		nemo_dev::take_current(b, input, nn, Excit, Inhib);
+		nemo_dev::take_current(b, L.input, nn, Excit, Inhib);
 		g.scratch[SCR_E * P + nl] = Excit;
 		g.scratch[SCR_I * P + nl] = Inhib;
 		g.scratch[SCR_X * P + nl] = nemo_dev::take_external(b, nn);
@@ -220,14 +220,16 @@ hh_substeps(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 __global__ void __launch_bounds__(NEMO_UPDATE_BLOCK)
 hh_record(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 {
+	nemo_dev::UpdateLds& L = nemo_dev::update_lds<false>();
 	const unsigned nl = blockIdx.x * NEMO_UPDATE_BLOCK + threadIdx.x;
 	const bool valid = nl < g.count;
 	const unsigned nn = g.start + nl;
 	const uint64_t hist = nemo_dev::load_history(b, nn, valid);
+	L.push_rows[threadIdx.x] = nemo_dev::push_row(b, nn);
 	bool fired = false;
 	if(valid) fired = g.scratch[SCR_FIRED * g.pitch + nl] != 0.0f || nemo_dev::stimulated(b, nn);
-	nemo_dev::record_firing(b, nn, valid, fired, hist);
-	nemo_dev::stdp_tile(b, nn, valid, hist);
+	nemo_dev::record_firing(b, L, nn, valid, fired, hist);
+	nemo_dev::stdp_tile(b, L, nn, valid, hist);
 }
 
 } // namespace
@@ -241,7 +243,7 @@ hip_update_neurons(hipStream_t stream, const nemo_hip_neuron_group* g, const nem
 	const unsigned blocks = tiles + b->co.deliver_blocks + b->co.stdp_blocks;
 	if(!g->scratch) {
 		if(b->co.deliver_blocks || b->co.stdp_blocks) {
-			hipExtLaunchKernelGGL(hh_update<true>, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream,
+			hipExtLaunchKernelGGL(hh_update<true>, dim3(blocks), dim3(NEMO_COLAUNCH_BLOCK), b->co.lds_bytes, stream,
 					b->time_start, b->time_stop, 0, *g, *b);
 		} else {
 			hipExtLaunchKernelGGL(hh_update<false>, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream,
@@ -250,7 +252,7 @@ hip_update_neurons(hipStream_t stream, const nemo_hip_neuron_group* g, const nem
 		return hipGetLastError();
 	}
 	if(b->co.deliver_blocks || b->co.stdp_blocks) {
-		hipExtLaunchKernelGGL(hh_input<true>, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream,
+		hipExtLaunchKernelGGL(hh_input<true>, dim3(blocks), dim3(NEMO_COLAUNCH_BLOCK), b->co.lds_bytes, stream,
 				b->time_start, nullptr, 0, *g, *b);
 	} else {
 		hipExtLaunchKernelGGL(hh_input<false>, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream,

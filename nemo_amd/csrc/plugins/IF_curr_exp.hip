@@ -25,12 +25,12 @@ enum { S_V = 0, S_IE = 1, S_II = 2, S_LASTFIRED = 3 };
 /* CO: the launch may carry propagate blocks after the tiles (compiled
  * out otherwise, which leaves the update far fewer registers) */
 template<bool CO>
-__global__ void __launch_bounds__(NEMO_UPDATE_BLOCK)
+__global__ void __launch_bounds__(CO ? NEMO_COLAUNCH_BLOCK : NEMO_UPDATE_BLOCK)
 if_curr_exp_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 {
 	if(CO && nemo_dev::colaunched_block(b, g)) return;
-	__shared__ nemo_dev::TileInput input;
-	nemo_dev::gather_input(b, g, input);
+	nemo_dev::UpdateLds& L = nemo_dev::update_lds<CO>();
+	nemo_dev::gather_input(b, g, L);
 	const unsigned nl = blockIdx.x * NEMO_UPDATE_BLOCK + threadIdx.x;
 	const bool valid = nl < g.count;
 	const unsigned ng = g.start + nl;
@@ -42,7 +42,7 @@ if_curr_exp_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 		const float* p = g.param;
 		float* s = g.state;
 		float E, In;
-		nemo_dev::take_current(b, input, ng, E, In);
+		nemo_dev::take_current(b, L.input, ng, E, In);
 		float Ie = ((1.0f - 1.0f / p[TAU_SYN_E * P + nl]) * s[S_IE * P + nl]) + E;
 		float Ii = ((1.0f - 1.0f / p[TAU_SYN_I * P + nl]) * s[S_II * P + nl]) + In;
 		float I = Ie + Ii + nemo_dev::take_external(b, ng) + p[I_OFFSET * P + nl];
@@ -67,8 +67,8 @@ if_curr_exp_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 			s[S_V * P + nl] = v;
 		}
 	}
-	nemo_dev::record_firing(b, ng, valid, fired, h);
-	nemo_dev::stdp_tile(b, ng, valid, h);
+	nemo_dev::record_firing(b, L, ng, valid, fired, h);
+	nemo_dev::stdp_tile(b, L, ng, valid, h);
 }
 
 } // namespace
@@ -81,7 +81,7 @@ hip_update_neurons(hipStream_t stream, const nemo_hip_neuron_group* g, const nem
 	const unsigned blocks = (g->count + NEMO_UPDATE_BLOCK - 1) / NEMO_UPDATE_BLOCK + b->co.deliver_blocks
 		+ b->co.stdp_blocks;
 	if(b->co.deliver_blocks || b->co.stdp_blocks) {
-		hipExtLaunchKernelGGL(if_curr_exp_update<true>, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream,
+		hipExtLaunchKernelGGL(if_curr_exp_update<true>, dim3(blocks), dim3(NEMO_COLAUNCH_BLOCK), b->co.lds_bytes, stream,
 				b->time_start, b->time_stop, 0, *g, *b);
 	} else {
 		hipExtLaunchKernelGGL(if_curr_exp_update<false>, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream,

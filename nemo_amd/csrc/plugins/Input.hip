@@ -12,12 +12,12 @@ namespace {
 /* CO: the launch may carry propagate blocks after the tiles (compiled
  * out otherwise, which leaves the update far fewer registers) */
 template<bool CO>
-__global__ void __launch_bounds__(NEMO_UPDATE_BLOCK)
+__global__ void __launch_bounds__(CO ? NEMO_COLAUNCH_BLOCK : NEMO_UPDATE_BLOCK)
 input_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 {
 	if(CO && nemo_dev::colaunched_block(b, g)) return;
-	__shared__ nemo_dev::TileInput input;
-	nemo_dev::gather_input(b, g, input);
+	nemo_dev::UpdateLds& L = nemo_dev::update_lds<CO>();
+	nemo_dev::gather_input(b, g, L);
 	const unsigned nl = blockIdx.x * NEMO_UPDATE_BLOCK + threadIdx.x;
 	const bool valid = nl < g.count;
 	const unsigned ng = g.start + nl;
@@ -25,11 +25,11 @@ input_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 	bool fired = false;
 	if(valid) {
 		float E, I;
-		nemo_dev::take_current(b, input, ng, E, I); /* clears the overflow accumulators */
+		nemo_dev::take_current(b, L.input, ng, E, I); /* clears the overflow accumulators */
 		fired = nemo_dev::stimulated(b, ng);
 	}
-	nemo_dev::record_firing(b, ng, valid, fired, h);
-	nemo_dev::stdp_tile(b, ng, valid, h);
+	nemo_dev::record_firing(b, L, ng, valid, fired, h);
+	nemo_dev::stdp_tile(b, L, ng, valid, h);
 }
 
 } // namespace
@@ -42,7 +42,7 @@ hip_update_neurons(hipStream_t stream, const nemo_hip_neuron_group* g, const nem
 	const unsigned blocks = (g->count + NEMO_UPDATE_BLOCK - 1) / NEMO_UPDATE_BLOCK + b->co.deliver_blocks
 		+ b->co.stdp_blocks;
 	if(b->co.deliver_blocks || b->co.stdp_blocks) {
-		hipExtLaunchKernelGGL(input_update<true>, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream,
+		hipExtLaunchKernelGGL(input_update<true>, dim3(blocks), dim3(NEMO_COLAUNCH_BLOCK), b->co.lds_bytes, stream,
 				b->time_start, b->time_stop, 0, *g, *b);
 	} else {
 		hipExtLaunchKernelGGL(input_update<false>, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream,

@@ -26,11 +26,11 @@ enum { STATE_U = 0, STATE_V = 1 };
 /* CO: the launch may carry propagate blocks after the tiles (compiled
  * out otherwise, which leaves the update far fewer registers) */
 template<bool CO>
-__global__ void __launch_bounds__(NEMO_UPDATE_BLOCK)
+__global__ void __launch_bounds__(CO ? NEMO_COLAUNCH_BLOCK : NEMO_UPDATE_BLOCK)
 izhikevich_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 {
 	if(CO && nemo_dev::colaunched_block(b, g)) return;
-	__shared__ nemo_dev::TileInput input;
+	nemo_dev::UpdateLds& L = nemo_dev::update_lds<CO>();
 	const unsigned nl = blockIdx.x * NEMO_UPDATE_BLOCK + threadIdx.x;
 	const bool valid = nl < g.count;
 	const unsigned ng = g.start + nl;
@@ -41,7 +41,7 @@ izhikevich_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 	float sigma = 0.0f, a = 0.0f, bb = 0.0f, c = 0.0f, d = 0.0f, u = 0.0f, v = 0.0f;
 	uint4 s = make_uint4(0, 0, 0, 0);
 	uint64_t h = 0;
-	nemo_dev::gather_input(b, g, input, [&]() {
+	nemo_dev::gather_input(b, g, L, [&]() {
 		if(valid) {
 			sigma = g.param[PARAM_SIGMA * P + nl];
 			a = g.param[PARAM_A * P + nl];
@@ -58,7 +58,7 @@ izhikevich_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 
 	if(valid) {
 		float E, In;
-		nemo_dev::take_current(b, input, ng, E, In);
+		nemo_dev::take_current(b, L.input, ng, E, In);
 		float I = E + In + nemo_dev::take_external(b, ng);
 
 		if(sigma != 0.0f) {
@@ -84,8 +84,8 @@ izhikevich_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 		g.state[STATE_U * P + nl] = u;
 		g.state[STATE_V * P + nl] = v;
 	}
-	nemo_dev::record_firing(b, ng, valid, fired, h);
-	nemo_dev::stdp_tile(b, ng, valid, h);
+	nemo_dev::record_firing(b, L, ng, valid, fired, h);
+	nemo_dev::stdp_tile(b, L, ng, valid, h);
 }
 
 } // namespace
@@ -98,7 +98,7 @@ hip_update_neurons(hipStream_t stream, const nemo_hip_neuron_group* g, const nem
 	const unsigned blocks = (g->count + NEMO_UPDATE_BLOCK - 1) / NEMO_UPDATE_BLOCK + b->co.deliver_blocks
 		+ b->co.stdp_blocks;
 	if(b->co.deliver_blocks || b->co.stdp_blocks) {
-		hipExtLaunchKernelGGL(izhikevich_update<true>, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream,
+		hipExtLaunchKernelGGL(izhikevich_update<true>, dim3(blocks), dim3(NEMO_COLAUNCH_BLOCK), b->co.lds_bytes, stream,
 				b->time_start, b->time_stop, 0, *g, *b);
 	} else {
 		hipExtLaunchKernelGGL(izhikevich_update<false>, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream,

@@ -102,10 +102,11 @@ __device__ inline uint32_t cycle_of(const nemo_hip_step_buffers& b)
 /* One launch per step: each thread one oscillator of its tile, then the
  * tile's (empty) firing record. */
 template<bool CO>
-__global__ void __launch_bounds__(NEMO_UPDATE_BLOCK)
+__global__ void __launch_bounds__(CO ? NEMO_COLAUNCH_BLOCK : NEMO_UPDATE_BLOCK)
 kuramoto_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 {
 	if(CO && nemo_dev::colaunched_block(b, g)) return;
+	nemo_dev::UpdateLds& L = nemo_dev::update_lds<CO>();
 	const unsigned nl = blockIdx.x * NEMO_UPDATE_BLOCK + threadIdx.x;
 	const bool valid = nl < g.count;
 	const unsigned ng = g.start + nl;
@@ -117,8 +118,8 @@ kuramoto_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 		g.state[size_t((cycle + 1) % g.history) * g.pitch + nl] = phase;
 		if(range) atomicOr(b.status, NEMO_HIP_STATUS_SINF_RANGE);
 	}
-	nemo_dev::record_firing(b, ng, valid, false, hist);
-	nemo_dev::stdp_tile(b, ng, valid, hist);
+	nemo_dev::record_firing(b, L, ng, valid, false, hist);
+	nemo_dev::stdp_tile(b, L, ng, valid, hist);
 }
 
 /* Update passes (nemo_hip_neuron_group.levels): first the old contents of
@@ -145,16 +146,17 @@ kuramoto_pass(nemo_hip_neuron_group g, nemo_hip_step_buffers b, uint32_t first, 
 }
 
 template<bool CO>
-__global__ void __launch_bounds__(NEMO_UPDATE_BLOCK)
+__global__ void __launch_bounds__(CO ? NEMO_COLAUNCH_BLOCK : NEMO_UPDATE_BLOCK)
 kuramoto_record(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 {
 	if(CO && nemo_dev::colaunched_block(b, g)) return;
+	nemo_dev::UpdateLds& L = nemo_dev::update_lds<CO>();
 	const unsigned nl = blockIdx.x * NEMO_UPDATE_BLOCK + threadIdx.x;
 	const bool valid = nl < g.count;
 	const unsigned ng = g.start + nl;
 	const uint64_t hist = nemo_dev::load_history(b, ng, valid);
-	nemo_dev::record_firing(b, ng, valid, false, hist);
-	nemo_dev::stdp_tile(b, ng, valid, hist);
+	nemo_dev::record_firing(b, L, ng, valid, false, hist);
+	nemo_dev::stdp_tile(b, L, ng, valid, hist);
 }
 
 /* Kuramoto.cpp:137-159: the history before the first cycle, each
@@ -189,7 +191,7 @@ hip_update_neurons(hipStream_t stream, const nemo_hip_neuron_group* g, const nem
 	const bool co = b->co.deliver_blocks || b->co.stdp_blocks;
 	if(!g->level_ptr) {
 		if(co) {
-			hipExtLaunchKernelGGL(kuramoto_update<true>, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes,
+			hipExtLaunchKernelGGL(kuramoto_update<true>, dim3(blocks), dim3(NEMO_COLAUNCH_BLOCK), b->co.lds_bytes,
 					stream, b->time_start, b->time_stop, 0, *g, *b);
 		} else {
 			hipExtLaunchKernelGGL(kuramoto_update<false>, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes,
@@ -210,7 +212,7 @@ hip_update_neurons(hipStream_t stream, const nemo_hip_neuron_group* g, const nem
 	}
 	if(e != hipSuccess) return e;
 	if(co) {
-		hipExtLaunchKernelGGL(kuramoto_record<true>, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream,
+		hipExtLaunchKernelGGL(kuramoto_record<true>, dim3(blocks), dim3(NEMO_COLAUNCH_BLOCK), b->co.lds_bytes, stream,
 				nullptr, b->time_stop, 0, *g, *b);
 	} else {
 		hipExtLaunchKernelGGL(kuramoto_record<false>, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream,

@@ -17,18 +17,18 @@ namespace {
 /* CO: the launch may carry propagate blocks after the tiles (compiled
  * out otherwise, which leaves the update far fewer registers) */
 template<bool CO>
-__global__ void __launch_bounds__(NEMO_UPDATE_BLOCK)
+__global__ void __launch_bounds__(CO ? NEMO_COLAUNCH_BLOCK : NEMO_UPDATE_BLOCK)
 qif_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 {
 	if(CO && nemo_dev::colaunched_block(b, g)) return;
-	__shared__ nemo_dev::TileInput input;
+	nemo_dev::UpdateLds& L = nemo_dev::update_lds<CO>();
 	const unsigned nl = blockIdx.x * NEMO_UPDATE_BLOCK + threadIdx.x;
 	const bool valid = nl < g.count;
 	const unsigned ng = g.start + nl;
 	/* state, parameter and history load while the input is gathered */
 	uint64_t h = 0;
 	float v = 0.0f, a = 0.0f;
-	nemo_dev::gather_input(b, g, input, [&]() {
+	nemo_dev::gather_input(b, g, L, [&]() {
 		h = nemo_dev::load_history(b, ng, valid);
 		v = valid ? g.state[nl] : 0.0f;
 		a = valid ? g.param[nl] : 0.0f;
@@ -37,7 +37,7 @@ qif_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 
 	if(valid) {
 		float Excit, Inhib;
-		nemo_dev::take_current(b, input, ng, Excit, Inhib);
+		nemo_dev::take_current(b, L.input, ng, Excit, Inhib);
 		const float Exter = nemo_dev::take_external(b, ng);
 		const float dt = 0.1f;
 		const float a1 = 0.5f * dt;
@@ -65,8 +65,8 @@ qif_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 		}
 		g.state[nl] = v;
 	}
-	nemo_dev::record_firing(b, ng, valid, fired, h);
-	nemo_dev::stdp_tile(b, ng, valid, h);
+	nemo_dev::record_firing(b, L, ng, valid, fired, h);
+	nemo_dev::stdp_tile(b, L, ng, valid, h);
 }
 
 } // namespace
@@ -79,7 +79,7 @@ hip_update_neurons(hipStream_t stream, const nemo_hip_neuron_group* g, const nem
 	const unsigned blocks = (g->count + NEMO_UPDATE_BLOCK - 1) / NEMO_UPDATE_BLOCK + b->co.deliver_blocks
 		+ b->co.stdp_blocks;
 	if(b->co.deliver_blocks || b->co.stdp_blocks) {
-		hipExtLaunchKernelGGL(qif_update<true>, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream,
+		hipExtLaunchKernelGGL(qif_update<true>, dim3(blocks), dim3(NEMO_COLAUNCH_BLOCK), b->co.lds_bytes, stream,
 				b->time_start, b->time_stop, 0, *g, *b);
 	} else {
 		hipExtLaunchKernelGGL(qif_update<false>, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream,

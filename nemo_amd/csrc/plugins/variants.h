@@ -178,12 +178,12 @@ __device__ inline bool variant_step(const nemo_hip_neuron_group& g, const nemo_h
 }
 
 template<bool CO>
-__global__ void __launch_bounds__(NEMO_UPDATE_BLOCK)
+__global__ void __launch_bounds__(CO ? NEMO_COLAUNCH_BLOCK : NEMO_UPDATE_BLOCK)
 variant_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 {
 	if(CO && nemo_dev::colaunched_block(b, g)) return;
-	__shared__ nemo_dev::TileInput input;
-	nemo_dev::gather_input(b, g, input);
+	nemo_dev::UpdateLds& L = nemo_dev::update_lds<CO>();
+	nemo_dev::gather_input(b, g, L);
 	const unsigned nl = blockIdx.x * NEMO_UPDATE_BLOCK + threadIdx.x;
 	const bool valid = nl < g.count;
 	const unsigned ng = g.start + nl;
@@ -191,11 +191,11 @@ variant_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 	bool fired = false;
 	if(valid) {
 		float E, I;
-		nemo_dev::take_current(b, input, ng, E, I);
+		nemo_dev::take_current(b, L.input, ng, E, I);
 		fired = variant_step(g, b, nl, ng, E, I);
 	}
-	nemo_dev::record_firing(b, ng, valid, fired, h);
-	nemo_dev::stdp_tile(b, ng, valid, h);
+	nemo_dev::record_firing(b, L, ng, valid, fired, h);
+	nemo_dev::stdp_tile(b, L, ng, valid, h);
 }
 
 } // namespace
@@ -208,7 +208,7 @@ hip_update_neurons(hipStream_t stream, const nemo_hip_neuron_group* g, const nem
 	const unsigned blocks = (g->count + NEMO_UPDATE_BLOCK - 1) / NEMO_UPDATE_BLOCK + b->co.deliver_blocks
 		+ b->co.stdp_blocks;
 	if(b->co.deliver_blocks || b->co.stdp_blocks) {
-		hipExtLaunchKernelGGL(variant_update<true>, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream,
+		hipExtLaunchKernelGGL(variant_update<true>, dim3(blocks), dim3(NEMO_COLAUNCH_BLOCK), b->co.lds_bytes, stream,
 				b->time_start, b->time_stop, 0, *g, *b);
 	} else {
 		hipExtLaunchKernelGGL(variant_update<false>, dim3(blocks), dim3(NEMO_UPDATE_BLOCK), b->co.lds_bytes, stream,

@@ -595,23 +595,27 @@ __device__ inline void stdp_step(const nemo_propagate_args& a, uint64_t t, unsig
 }
 
 
-/* In an update launch: run this block as one of the propagate blocks the
- * launch carries after the group's tiles (nemo_hip_colaunch) and return
- * true, or return false for a block of the group's own tiles.  Every update
- * kernel calls it first. */
+/* In an update launch of NEMO_COLAUNCH_BLOCK threads per block: run this
+ * block as one of the propagate blocks the launch carries after the group's
+ * tiles (nemo_hip_colaunch) and return true, or, in a block of the group's
+ * own tiles, return false for its first NEMO_TILE threads (the tile's) and
+ * true for the others, whose waves then end (a block barrier waits only for
+ * the waves still running).  Every update kernel of such a launch calls it
+ * first and returns when it returns true. */
 __device__ inline bool colaunched_block(const nemo_hip_step_buffers& b, const nemo_hip_neuron_group& g)
 {
+	static_assert(NEMO_COLAUNCH_BLOCK % 64 == 0 && NEMO_COLAUNCH_BLOCK >= NEMO_TILE, "whole waves beyond the tile");
 	const unsigned tiles = (g.count + NEMO_TILE - 1) / NEMO_TILE;
-	if(blockIdx.x < tiles) return false;
+	if(blockIdx.x < tiles) return threadIdx.x >= NEMO_TILE;
 	extern __shared__ uint64_t colaunch_lds[];
 	unsigned char* lds = reinterpret_cast<unsigned char*>(colaunch_lds);
 	const unsigned k = blockIdx.x - tiles;
 	const nemo_propagate_args& a = b.co.args;
 	const uint64_t t = a.clock->base_step + a.step_offset;
 	if(k < b.co.deliver_blocks) {
-		deliver_block<NEMO_TILE>(a, t, k, lds);
+		deliver_block<NEMO_COLAUNCH_BLOCK>(a, t, k, lds);
 	} else {
-		stdp_step<NEMO_TILE>(a, t - 1, k - b.co.deliver_blocks, lds);
+		stdp_step<NEMO_COLAUNCH_BLOCK>(a, t - 1, k - b.co.deliver_blocks, lds);
 	}
 	return true;
 }

@@ -429,14 +429,17 @@ def test_single_type_batch_matches_solo_runs():
 def test_launch_shapes_give_identical_results():
     """Delivery blocks of both sets, bin capacity (down to 64 events, so that
     most events take the overflow path), STDP placement and block width, and
-    the overlap of the next step's delivery with the update change only
-    speed: rasters, weights and accumulators are identical."""
+    the overlap of the next step's delivery with the update (including the
+    update's own push of its delay-1 events) change only speed: rasters,
+    weights and accumulators are identical."""
     spec = gen.random_izhikevich(4000, 150, dmax=15, seed=21, stdp=True)
     shapes = [None, (1, 1, 64, 0, 1, 1), (7, 3, 256, 1, 4, 0), (200, 0, 1024, 2, 2, 0),
               (900, 124, 4096, 2, 1, 1), (33, 5, 2048, 1, 3, 1), (64, 0, 512, 0, 2, 0),
               (64, 0, 512, 3, 2, 0), (33, 5, 2048, 3, 1, 1), (33, 5, 2048, 2, 3, 2), (64, 16, 1024, 1, 2, 2),
               (200, 40, 4096, 0, 4, 2), (7, 3, 256, 3, 1, 2), (120, 30, 512, 2, 4, 2), (64, 0, 512, 4, 2, 0),
-              (33, 5, 2048, 4, 3, 2), (200, 0, 1024, 4, 8, 1), (120, 0, 1024, 4, 32, 1), (90, 0, 2048, 4, 64, 0)]
+              (33, 5, 2048, 4, 3, 2), (200, 0, 1024, 4, 8, 1), (120, 0, 1024, 4, 32, 1), (90, 0, 2048, 4, 64, 0),
+              (33, 5, 2048, 4, 3, 3), (7, 3, 256, 3, 1, 3), (1, 1, 64, 2, 2, 3), (64, 16, 1024, 1, 2, 3),
+              (120, 30, 512, 0, 4, 3)]
     results = []
     for shape in shapes:
         sim = sc.make(nemo_amd, spec, _conf(nemo_amd, stdp_fn()))
@@ -460,13 +463,14 @@ def test_ranged_table_reads_give_identical_results(which, monkeypatch):
     """The update reads either a tile's whole table row or only the columns of
     the delivery blocks whose windows hold the tile (NEMO_HIP_BIN_RANGED): the
     same events either way, over several launch shapes (both delivery sets,
-    small bins with overflow, co-launched delivery)."""
+    small bins with overflow, co-launched delivery, pushed delay-1 events)."""
     if which == "random":
         spec = gen.random_izhikevich(4000, 150, dmax=15, seed=21, stdp=True)
     else:
         spec = gen.torus_mixed(n_lif=4000, n_qif=800, n_hh=200, n_poisson=100, m=50, grid=15)
     stdp = which == "random"
-    shapes = [None, (1, 1, 64, 0, 1, 1), (33, 5, 2048, 4, 3, 2), (7, 3, 256, 3, 1, 2), (200, 0, 1024, 4, 8, 1)]
+    shapes = [None, (1, 1, 64, 0, 1, 1), (33, 5, 2048, 4, 3, 2), (7, 3, 256, 3, 1, 2), (200, 0, 1024, 4, 8, 1),
+              (33, 5, 2048, 4, 3, 3), (7, 3, 256, 3, 1, 3)]
     results = []
     for ranged in ("0", "1"):
         monkeypatch.setenv("NEMO_HIP_BIN_RANGED", ranged)
