@@ -36,10 +36,39 @@ BYTES_PER_STDP_SYNAPSE = 20    # 4 B packed source/delay + 8 B presynaptic histo
 
 
 def traffic_key(shape):
-    """The launch-shape fields the propagate launch's traffic depends on:
-    delivery blocks of both sets, bin capacity and STDP mode, and the STDP
-    words when STDP is fused into the launch (mode 2)."""
-    return list(shape[:4]) + ([shape[4]] if shape[3] == 2 else [])
+    """The launch-shape fields the launches' traffic depends on: delivery
+    blocks of both sets, bin capacity and STDP mode, the STDP words when STDP
+    is fused into a launch (mode 2), and the overlap when the sets are split
+    (it moves delivery into the update launch)."""
+    return list(shape[:4]) + ([shape[4]] if shape[3] == 2 else []) + [shape[5] if shape[1] else 0]
+
+
+def launch_bytes(shape, kern, work, n, spec):
+    """Algorithmic bytes per launch of the propagate and update launches of
+    the plain workload under a launch shape, and each launch's time per step
+    (us).  Overlap 3 (shape[5], split sets): the update launch of step t also
+    delivers all events of step t+1 (delays 2..D as carried blocks, delay 1
+    pushed by its tiles), and only the first step of each run of 64 has a
+    propagate launch.  Overlap 2: the update launch delivers delays 2..D,
+    their share of events taken as their share of the synapses (uniform
+    firing).  The STDP accumulation counts towards a launch only in STDP
+    mode 2, where it shares that launch; otherwise it runs in launches of its
+    own (mode 4: one batch per run of up to 64 steps)."""
+    steps = work["steps"]
+    ev = BYTES_PER_EVENT * work["events"] / steps
+    stdp = BYTES_PER_STDP_SYNAPSE * work["stdp_synapses"] / steps if shape[3] == 2 else 0.0
+    neurons = BYTES_PER_IZH_NEURON * n
+    split = shape[1] > 0
+    overlap = shape[5] if split else 0
+    if overlap == 3:
+        per_launch = {"propagate": ev, "update": neurons + ev + stdp}
+    elif overlap == 2:
+        late = float(np.count_nonzero(spec.delay == 1)) / max(1, spec.delay.size)
+        per_launch = {"propagate": ev * late, "update": neurons + ev * (1.0 - late) + stdp}
+    else:
+        per_launch = {"propagate": ev + stdp, "update": neurons}
+    per_step_us = {"propagate": kern["propagate"] * kern.get("propagate_launches", 1.0), "update": kern["update"]}
+    return per_launch, per_step_us
 
 
 def measured_traffic(kernel, shape):
@@ -386,18 +415,11 @@ def main():
 
     kern, work = sim.profile(args.profile_steps)
     steps_p = work["steps"]
-    # the propagate launch carries the STDP accumulation only in STDP mode 2;
-    # otherwise it runs in launches of its own (mode 4: one batch per run of
-    # up to 64 steps), whose bytes count towards the step but not the kernel
-    stdp_bytes = BYTES_PER_STDP_SYNAPSE * work["stdp_synapses"] / steps_p
-    fused_stdp = shape[3] == 2
-    per_launch = {
-        "propagate": BYTES_PER_EVENT * work["events"] / steps_p + (stdp_bytes if fused_stdp else 0.0),
-        "update": BYTES_PER_IZH_NEURON * n,
-    }
-    dominant = max(per_launch, key=lambda k: kern[k])
+    per_launch, per_step_us = launch_bytes(shape, kern, work, n, spec)
+    dominant = max(per_step_us, key=per_step_us.get)
     achieved = per_launch[dominant] / (kern[dominant] * 1e-6) / 1e9
-    step_bytes = sum(per_launch.values()) + (0.0 if fused_stdp else stdp_bytes)
+    stdp_bytes = BYTES_PER_STDP_SYNAPSE * work["stdp_synapses"] / steps_p
+    step_bytes = BYTES_PER_EVENT * work["events"] / steps_p + BYTES_PER_IZH_NEURON * n + stdp_bytes
 
     result = {
         "metric": METRIC,

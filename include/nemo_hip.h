@@ -352,9 +352,12 @@ nemo_status_t nemo_hip_counts(nemo_simulation_t, size_t* neurons, size_t* synaps
 
 /* Extension (measurement): run nsteps steps (no stimulus) with HIP events
  * around each phase on the simulation's stream.  avg_us receives the mean
- * duration of the propagate phase (delivery and STDP accumulation), of the
- * neuron updates (all models), and two zeros (reserved); counters the events
- * delivered, neurons fired, plastic synapses examined and steps run.  The
+ * duration of the propagate launch (delivery and STDP accumulation) over the
+ * steps that have one, of the neuron updates (all models; with launch-shape
+ * overlap 2 or 3 the update launch also delivers the next step) per step,
+ * the propagate launches per step (below 1 with overlap 3), and a zero
+ * (reserved); counters the events delivered, neurons fired, plastic synapses
+ * examined and steps run.  The
  * reference's Timer (src/nemo/Timer.hpp:23-79) only measures whole steps. */
 nemo_status_t nemo_hip_profile(nemo_simulation_t, unsigned nsteps, double avg_us[4], uint64_t counters[4]);
 

@@ -520,14 +520,17 @@ class Simulation(_SynapseQueries):
         check(lib().nemo_reset_timer(self._h))
 
     def profile(self, nsteps):
-        """Mean time (us) of the two launches of a step -- "propagate"
-        (delivery of the step and STDP accumulation of the previous one) and
-        "update" (all neuron models) -- over nsteps eager steps, measured with
-        HIP events on the simulation's stream, and the work done."""
+        """Mean time (us) of the launches of a step -- "propagate" (delivery
+        of the step and STDP accumulation of the previous one, per launch) and
+        "update" (all neuron models, per step; with launch-shape overlap 2 or
+        3 it also delivers the next step) -- and "propagate_launches" per step
+        (below 1 with overlap 3, whose steps after the first of each run of 64
+        need none), over nsteps eager steps measured with HIP events on the
+        simulation's stream, and the work done."""
         avg = (ctypes.c_double * 4)()
         cnt = (ctypes.c_uint64 * 4)()
         check(lib().nemo_hip_profile(self._h, nsteps, avg, cnt))
-        keys = ("propagate", "update")
+        keys = ("propagate", "update", "propagate_launches")
         return ({k: avg[i] for i, k in enumerate(keys)},
                 {"events": cnt[0], "fired": cnt[1], "stdp_synapses": cnt[2], "steps": cnt[3]})
 

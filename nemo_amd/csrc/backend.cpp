@@ -590,6 +590,7 @@ Simulation::~Simulation()
 	}
 	if(m_pinnedScalars) (void) hipHostFree(m_pinnedScalars);
 	if(m_pinnedBits) (void) hipHostFree(m_pinnedBits);
+	if(m_pinnedCounters) (void) hipHostFree(m_pinnedCounters);
 }
 
 void Simulation::activate() const
@@ -751,6 +752,26 @@ void Simulation::buildNeurons(const std::vector<const Network*>& nets)
 			g.dev.scratch = g.scratch.ptr;
 		}
 		start += (g.count + NEMO_TILE - 1) / NEMO_TILE * NEMO_TILE;
+	}
+	/* words of 64 device indices whose neurons have consecutive user indices
+	 * (padding aside): the readout decodes them without the lookup table,
+	 * whose scattered reads miss the host caches when a run is short */
+	m_wordBase.assign(m_npad / 64, ~0u);
+	for(size_t w = 0; w < m_npad / 64; ++w) {
+		bool linear = true, any = false;
+		unsigned base = 0;
+		for(unsigned j = 0; j < 64 && linear; ++j) {
+			const unsigned u = m_l2g[w * 64 + j];
+			if(u == ~0u) continue;
+			if(!any) {
+				any = u >= j;
+				linear = any;
+				base = u - j;
+			} else {
+				linear = u == base + j;
+			}
+		}
+		if(linear && any) m_wordBase[w] = base;
 	}
 }
 
@@ -1156,8 +1177,10 @@ static Partition partitionPairs(const std::vector<uint64_t>& pairWeight, const s
 }
 
 /* Default shape: bin capacity 4096 events (halved while the LDS does not
- * allow it), one block set with enough delivery blocks for about 1% of the
- * synapses per step, STDP accumulated in batches beside the steps, one
+ * allow it), enough delivery blocks for about 1% of the synapses per step,
+ * and, with delays above 1, a delay-1 set of about 1/D of them, so that the
+ * update launches carry the next step's delivery and push its delay-1
+ * events (overlap 3); STDP accumulated in batches beside the steps, one
  * bitmap word per 64 threads of an STDP block.  NEMO_HIP_DELIVER_BLOCKS,
  * NEMO_HIP_LATE_BLOCKS, NEMO_HIP_BIN_CAPACITY, NEMO_HIP_STDP_MODE,
  * NEMO_HIP_STDP_WORDS and NEMO_HIP_OVERLAP override it (tuning). */
@@ -1172,6 +1195,13 @@ LaunchShape Simulation::defaultShape() const
 		s.lateBlocks = 0;
 		s.binCapacity = cap;
 		if(ldsNeeded(s) <= m_ldsBudget) break;
+	}
+	if(m_D >= 2) {
+		LaunchShape c = s;
+		c.lateBlocks = std::max(1u, unsigned(std::lround(double(s.deliverBlocks) / m_D)));
+		c.deliverBlocks = std::min(s.deliverBlocks, NEMO_MAX_BIN_BLOCKS - c.lateBlocks);
+		c.overlap = 3;
+		if(ldsNeeded(c) <= m_ldsBudget) s = c;
 	}
 	s.stdpMode = 4;
 	s.stdpWords = NEMO_STDP_MAX_WORDS; /* few, wide batch blocks disturb the steps least */
@@ -1395,11 +1425,16 @@ void Simulation::allocateStepState()
 	HIP_CHECK(hipMemset(m_debugCurrent.ptr, 0, m_debugCurrent.bytes()));
 	HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&m_pinnedBits),
 				size_t(NEMO_STEP_RING) * m_firedWords * sizeof(uint64_t), hipHostMallocDefault));
+	/* touched now, so that the first readout of a page does not fault it in */
+	std::memset(m_pinnedBits, 0, size_t(NEMO_STEP_RING) * m_firedWords * sizeof(uint64_t));
 
 	/* work counters, sized for any launch shape */
 	/* per table column, then per 64 device indices (events pushed by updates) */
 	m_blockEvents.allocate(NEMO_MAX_BIN_BLOCKS + m_npad / 64);
 	HIP_CHECK(hipMemset(m_blockEvents.ptr, 0, m_blockEvents.bytes()));
+	HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&m_pinnedCounters), 2 * m_blockEvents.bytes(),
+				hipHostMallocDefault));
+	std::memset(m_pinnedCounters, 0, 2 * m_blockEvents.bytes());
 	m_blockExamined.allocate(std::max<size_t>(1, m_firedWords));
 	HIP_CHECK(hipMemset(m_blockExamined.ptr, 0, m_blockExamined.bytes()));
 
@@ -1584,16 +1619,21 @@ void Simulation::enqueueSteps(uint64_t first, unsigned k, bool ext, bool fstim, 
 		 * and end of the delivery on this stream, of the neuron updates */
 		hipEvent_t ps = ev ? ev[4 * i] : nullptr, pe = ev ? ev[4 * i + 1] : nullptr;
 		if(split && pushed && i > 0) {
-			/* delivered by the previous step's update launch */
-			if(ev) {
-				HIP_CHECK(hipEventRecord(ps, m_stream));
-				HIP_CHECK(hipEventRecord(pe, m_stream));
-			}
+			/* delivered by the previous step's update launch (no propagate
+			 * launch to time: ps and pe stay unrecorded) */
 		} else if(split) {
 			const bool early = i == 0 || (!overlap && !colaunch);
-			if(early) launchDelivery(m_early, off, m_stream, false, ps, nullptr);
-			if(i == 0 && overlap) forkEarly(1);
-			launchDelivery(m_late, off, m_stream, fused && !colaunch, early ? nullptr : ps, pe);
+			if(early && !(i == 0 && overlap)) {
+				/* both sets in one launch */
+				nemo_propagate_args a = m_early, a2 = m_late;
+				a.step_offset = a2.step_offset = off;
+				HIP_CHECK(nemo_launch_propagate_sets(m_stream, &a, &a2, fused && !colaunch ? m_stdpBlocks : 0, ps,
+							pe));
+			} else {
+				if(early) launchDelivery(m_early, off, m_stream, false, ps, nullptr);
+				if(i == 0 && overlap) forkEarly(1);
+				launchDelivery(m_late, off, m_stream, fused && !colaunch, early ? nullptr : ps, pe);
+			}
 		} else {
 			launchDelivery(m_early, off, m_stream, fused, ps, pe);
 		}
@@ -1709,8 +1749,13 @@ bool Simulation::enqueueFlush()
 	joinStdp();
 	if(!m_stdpBlocks || m_hostStep == 0 || m_flushed == m_hostStep - 1) return false;
 	if(m_shape.stdpMode == 4) {
-		/* the pending batch: every run of steps ends with a snapshot */
-		HIP_CHECK(nemo_launch_stdp_batch(m_stream, &m_propagate, m_stdpBlocks));
+		/* the pending batch: every run of steps ends with a snapshot.  Here
+		 * nothing runs beside it, so it takes narrower blocks (more of them)
+		 * than the batches beside the steps */
+		nemo_propagate_args a = m_propagate;
+		a.stdp_words = std::min(a.stdp_words, 4u);
+		const unsigned blocks = unsigned((m_firedWords + a.stdp_words - 1) / a.stdp_words);
+		HIP_CHECK(nemo_launch_stdp_batch(m_stream, &a, blocks));
 		copyScalar(&m_clock.ptr->stdp_count, 0);
 		copyScalar(&m_clock.ptr->stdp_flushed, m_hostStep - 1);
 		m_flushed = m_hostStep - 1;
@@ -1729,17 +1774,62 @@ bool Simulation::enqueueFlush()
 
 /* Neurons (user indices) set in the firing bitmap of one step, in simulation
  * order (cpu/Simulation.cpp:200-209). */
+static inline unsigned lowestBit(uint64_t w)
+{
+	return w ? unsigned(__builtin_ctzll(w)) : 0u;
+}
+
+template<typename Out>
+static inline void appendFired(Out& out, const unsigned* p, size_t n)
+{
+	out.insert(out.end(), p, p + n);
+}
+
+template<>
+inline void appendFired(Simulation::FiredSink& out, const unsigned* p, size_t n)
+{
+	out.append(p, n);
+}
+
 template<typename Out>
 void Simulation::decodeFiring(const uint64_t* bits, Out& out) const
 {
+	/* into a buffer with four entries of slack, then appended in one go.  In
+	 * a word whose neurons have consecutive user indices, its first four set
+	 * bits are written unconditionally (the slots past its count are
+	 * overwritten by the next word): most words hold at most four, so the
+	 * loop's branches follow the words, not their random bits */
+	m_decodeBuf.resize(m_firedWords * 64 + 4);
+	unsigned* d = m_decodeBuf.data();
+	size_t pos = 0;
 	for(size_t w = 0; w < m_firedWords; ++w) {
 		uint64_t word = bits[w];
-		while(word) {
-			const unsigned bit = unsigned(__builtin_ctzll(word));
+		const unsigned base = m_wordBase[w];
+		if(base != ~0u) {
+			const unsigned cnt = unsigned(__builtin_popcountll(word));
+			unsigned* o = d + pos;
+			o[0] = base + lowestBit(word);
 			word &= word - 1;
-			out.push_back(m_l2g[w * 64 + bit]);
+			o[1] = base + lowestBit(word);
+			word &= word - 1;
+			o[2] = base + lowestBit(word);
+			word &= word - 1;
+			o[3] = base + lowestBit(word);
+			word &= word - 1;
+			for(unsigned i = 4; word; ++i) {
+				o[i] = base + unsigned(__builtin_ctzll(word));
+				word &= word - 1;
+			}
+			pos += cnt;
+		} else {
+			while(word) {
+				const unsigned bit = unsigned(__builtin_ctzll(word));
+				word &= word - 1;
+				d[pos++] = m_l2g[w * 64 + bit];
+			}
 		}
 	}
+	appendFired(out, d, pos);
 }
 
 /* Fired neurons of every network of a batch for one step: out gets each
@@ -1881,6 +1971,8 @@ hipGraphExec_t Simulation::graphFor(unsigned k)
 	hipGraphExec_t exec;
 	HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
 	HIP_CHECK(hipGraphDestroy(graph));
+	/* its launch packets go to the device now, not in its first (timed) launch */
+	HIP_CHECK(hipGraphUpload(exec, m_stream));
 	m_graphs[k] = exec;
 	return exec;
 }
@@ -1910,7 +2002,13 @@ void Simulation::run(unsigned nsteps, unsigned stdpPeriod, float reward, bool re
 		throw Error(NEMO_LOGIC_ERROR, "applyStdp called, but no STDP model specified");
 	}
 	HIP_CHECK(hipStreamSynchronize(m_stream));
-	const uint64_t events0 = events ? sumCounters(m_blockEvents) : 0;
+	/* the event counters before and after the run's work, copied on its
+	 * stream (no blocking copy at either end) */
+	const size_t nCounters = m_blockEvents.n;
+	if(events) {
+		HIP_CHECK(hipMemcpyAsync(m_pinnedCounters, m_blockEvents.ptr, m_blockEvents.bytes(), hipMemcpyDeviceToHost,
+					m_stream));
+	}
 	uint64_t stepsSinceApply = 0;
 	const unsigned CHUNK = NEMO_STEP_RING / 2;
 	/* room for the fired list at the rate of earlier runs, so that it is not
@@ -1986,6 +2084,10 @@ void Simulation::run(unsigned nsteps, unsigned stdpPeriod, float reward, bool re
 		/* a run returns with the STDP accumulation of all its steps done
 		 * (the last batch, or the last step's in the per-step placements) */
 		enqueueFlush();
+		if(events) {
+			HIP_CHECK(hipMemcpyAsync(m_pinnedCounters + nCounters, m_blockEvents.ptr, m_blockEvents.bytes(),
+						hipMemcpyDeviceToHost, m_stream));
+		}
 		decodePending();
 	} catch(...) {
 		(void) hipStreamSynchronize(m_stream);
@@ -1997,16 +2099,26 @@ void Simulation::run(unsigned nsteps, unsigned stdpPeriod, float reward, bool re
 	destroyEvents();
 	HIP_CHECK(hipStreamSynchronize(m_stream));
 	if(readRaster && fired && nsteps) m_firedPerStep = double(fired->size() - firedBefore) / nsteps;
-	if(events) *events = sumCounters(m_blockEvents) - events0;
+	if(events) {
+		uint64_t before = 0, after = 0;
+		for(size_t i = 0; i < nCounters; ++i) {
+			before += m_pinnedCounters[i];
+			after += m_pinnedCounters[nCounters + i];
+		}
+		*events = after - before;
+	}
 	m_firedOut.clear();
 	checkStatus();
 }
 
 /* Eager steps with HIP events around each launch on the simulation's stream:
  * average microseconds of the propagate kernel (delivery, and STDP
- * accumulation in STDP mode 2) and of the neuron updates (all groups), and the work counters
- * of the profiled steps (events delivered, neurons fired, plastic synapses
- * examined, steps).  The steps are real simulation steps; their firing is not
+ * accumulation in STDP mode 2) over the steps that have one, of the neuron
+ * updates (all groups) per step, and the propagate launches per step (with
+ * overlap 3 only the first step of each run of 64 has one: the others are
+ * delivered by the previous update launch); and the work counters of the
+ * profiled steps (events delivered, neurons fired, plastic synapses examined,
+ * steps).  The steps are real simulation steps; their firing is not
  * returned. */
 void Simulation::profile(unsigned nsteps, double avgUs[4], uint64_t counters[4])
 {
@@ -2024,9 +2136,13 @@ void Simulation::profile(unsigned nsteps, double avgUs[4], uint64_t counters[4])
 	}
 	advanceSteps(nsteps);
 	HIP_CHECK(hipStreamSynchronize(m_stream));
+	const bool pushed = m_late.deliver_blocks && m_early.deliver_blocks && m_shape.overlap == 3;
 	double sum[3] = {0, 0, 0};
+	unsigned launches = 0;
 	for(unsigned s = 0; s < nsteps; ++s) {
-		for(int k = 0; k < 2; ++k) {
+		const bool propagate = !(pushed && s % 64 != 0);
+		launches += propagate ? 1 : 0;
+		for(int k = propagate ? 0 : 1; k < 2; ++k) {
 			float ms = 0;
 			HIP_CHECK(hipEventElapsedTime(&ms, ev[4 * size_t(s) + 2 * k], ev[4 * size_t(s) + 2 * k + 1]));
 			sum[k] += ms;
@@ -2039,8 +2155,9 @@ void Simulation::profile(unsigned nsteps, double avgUs[4], uint64_t counters[4])
 	HIP_CHECK(hipStreamSynchronize(m_stream));
 	uint64_t firedCount = 0;
 	for(size_t i = 0; i < size_t(kb) * m_firedWords; ++i) firedCount += __builtin_popcountll(m_pinnedBits[i]);
-	for(int k = 0; k < 2; ++k) avgUs[k] = nsteps ? 1000.0 * sum[k] / nsteps : 0.0;
-	avgUs[2] = 0.0;
+	avgUs[0] = launches ? 1000.0 * sum[0] / launches : 0.0;
+	avgUs[1] = nsteps ? 1000.0 * sum[1] / nsteps : 0.0;
+	avgUs[2] = nsteps ? double(launches) / nsteps : 0.0;
 	avgUs[3] = 0.0;
 	counters[0] = sumCounters(m_blockEvents) - e0;
 	counters[1] = kb ? firedCount * nsteps / kb : 0;

@@ -19,6 +19,7 @@
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
+#include <algorithm>
 #include <vector>
 
 #include <hip/hip_runtime_api.h>
@@ -199,6 +200,13 @@ public:
 			if(len < cap) buf[len++] = v;
 			else spill->push_back(v);
 		}
+		void append(const unsigned* p, size_t n)
+		{
+			const size_t k = std::min(n, cap - len);
+			std::copy(p, p + k, buf + len);
+			len += k;
+			spill->insert(spill->end(), p + k, p + n);
+		}
 		size_t size() const { return len + spill->size(); }
 	};
 	/* nsteps steps without stimulus; with readRaster, counts gets [step][network]
@@ -284,6 +292,7 @@ private:
 	hipEvent_t m_groupFork = nullptr;
 	double m_firedPerStep = 0.0;          /* fired neurons per step in the last run() with a raster */
 	mutable std::vector<std::vector<unsigned> > m_decodePer; /* decodeFiringBatch's per-network lists */
+	mutable std::vector<unsigned> m_decodeBuf;               /* decodeFiring's list of one step */
 	bool m_decodeInOrder = false;         /* batch whose device order lists each network in turn, in order */
 	bool m_binRanged = false;             /* updates read only their tiles' column ranges */
 	hipEvent_t m_earlyFork = nullptr, m_earlyDone[2] = {nullptr, nullptr};
@@ -299,6 +308,8 @@ private:
 	bool m_logging = false;
 
 	std::vector<unsigned> m_l2g;          /* device index -> user index (~0 for padding) */
+	std::vector<unsigned> m_wordBase;     /* per 64 device indices: u such that m_l2g[64w + j] = u + j
+	                                         for all its neurons, or ~0 */
 	std::vector<uint16_t> m_netOf;        /* device index -> network (batches only) */
 	std::vector<uint32_t> m_simOf;        /* device index -> simulation index in its network
 	                                         (batches whose device type order differs from a
@@ -381,6 +392,8 @@ private:
 	uint64_t* m_pinnedScalars = nullptr; /* ring of sources for host-to-device scalars */
 	unsigned m_scalarNext = 0;
 	uint64_t* m_pinnedBits = nullptr;  /* firing bitmaps read back */
+	unsigned long long* m_pinnedCounters = nullptr; /* [2][m_blockEvents.n]: the event counters at the
+	                                                   start and at the end of a run */
 	std::vector<unsigned> m_firedOut;
 	std::chrono::steady_clock::time_point m_wallStart;
 

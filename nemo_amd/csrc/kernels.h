@@ -56,6 +56,12 @@ size_t nemo_propagate_static_lds(void);
 hipError_t nemo_launch_propagate(hipStream_t, const nemo_propagate_args*, unsigned stdp_blocks, int fuse_stdp,
 		hipEvent_t start, hipEvent_t stop);
 
+/* Both delivery sets of one step (a's blocks, then a2's; the same step,
+ * clock, bin capacity, window and table) in one launch, with stdp_blocks STDP
+ * blocks of a's arguments after them. */
+hipError_t nemo_launch_propagate_sets(hipStream_t, const nemo_propagate_args* a, const nemo_propagate_args* a2,
+		unsigned stdp_blocks, hipEvent_t start, hipEvent_t stop);
+
 /* applyStdp (ConnectivityMatrix::applyStdp, ConnectivityMatrix.cpp:272-325 and
  * StdpProcess::updatedWeight, StdpProcess.cpp:128-138). */
 typedef struct nemo_apply_args {

@@ -55,6 +55,24 @@ propagate_kernel(nemo_propagate_args a)
 	}
 }
 
+/* Both delivery sets of one step in one launch: the first set's blocks, the
+ * second's, then (when fused into this launch) STDP blocks of the first
+ * set's arguments */
+__global__ void __launch_bounds__(PBLOCK, 6)
+propagate_sets_kernel(nemo_propagate_args a, nemo_propagate_args a2)
+{
+	extern __shared__ uint64_t lds_dyn[];
+	unsigned char* lds = reinterpret_cast<unsigned char*>(lds_dyn);
+	const uint64_t t = a.clock->base_step + a.step_offset;
+	if(blockIdx.x < a.deliver_blocks) {
+		nemo_dev::deliver_block<PBLOCK>(a, t, blockIdx.x, lds);
+	} else if(blockIdx.x < a.deliver_blocks + a2.deliver_blocks) {
+		nemo_dev::deliver_block<PBLOCK>(a2, t, blockIdx.x - a.deliver_blocks, lds);
+	} else {
+		nemo_dev::stdp_step<PBLOCK>(a, t, blockIdx.x - a.deliver_blocks - a2.deliver_blocks, lds);
+	}
+}
+
 /* STDP blocks only */
 __global__ void __launch_bounds__(PBLOCK)
 stdp_kernel(nemo_propagate_args a)
@@ -296,6 +314,37 @@ hipError_t nemo_launch_propagate(hipStream_t st, const nemo_propagate_args* a, u
 		return hipGetLastError();
 	}
 	return hipSuccess;
+}
+
+hipError_t nemo_launch_propagate_sets(hipStream_t st, const nemo_propagate_args* a, const nemo_propagate_args* a2,
+		unsigned stdp_blocks, hipEvent_t start, hipEvent_t stop)
+{
+	if(!a2->deliver_blocks) return nemo_launch_propagate(st, a, stdp_blocks, 1, start, stop);
+	if(!a->deliver_blocks) {
+		if(stdp_blocks) return hipErrorInvalidValue; /* the STDP blocks take the first set's arguments */
+		return nemo_launch_propagate(st, a2, 0, 1, start, stop);
+	}
+	if(a->step_offset != a2->step_offset || a->clock != a2->clock || a->bin_capacity != a2->bin_capacity
+			|| a->max_window != a2->max_window || a->table_columns != a2->table_columns
+			|| a->deliver_blocks + a2->deliver_blocks > NEMO_MAX_BIN_BLOCKS) {
+		return hipErrorInvalidValue;
+	}
+	for(const nemo_propagate_args* x : {a, a2}) {
+		if(x->D == 0 || x->D > NEMO_MAX_DELAY || x->bin_capacity == 0 || x->bin_capacity > 32768
+				|| x->max_window == 0 || x->max_window > x->tiles || !x->pair_bounds || !x->windows
+				|| x->d_first == 0 || x->d_count == 0 || x->d_first + x->d_count - 1 > x->D
+				|| x->block_base + x->deliver_blocks > x->table_columns || x->table_columns > NEMO_MAX_BIN_BLOCKS) {
+			return hipErrorInvalidValue;
+		}
+	}
+	if(stdp_blocks && (a->stdp.post_window > 63 || a->stdp_words == 0 || a->stdp_words > NEMO_STDP_MAX_WORDS
+				|| size_t(stdp_blocks) * a->stdp_words < a->fired_words || !a->stdp.rcm_ptr || !a->hist)) {
+		return hipErrorInvalidValue;
+	}
+	const size_t lds = nemo_dev::PropagateLds<PBLOCK>::bytes(true, a->bin_capacity, a->max_window);
+	const unsigned blocks = a->deliver_blocks + a2->deliver_blocks + stdp_blocks;
+	hipExtLaunchKernelGGL(propagate_sets_kernel, dim3(blocks), dim3(PBLOCK), lds, st, start, stop, 0, *a, *a2);
+	return hipGetLastError();
 }
 
 hipError_t nemo_launch_apply_stdp(hipStream_t st, const nemo_apply_args* a)
