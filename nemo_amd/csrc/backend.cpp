@@ -508,6 +508,7 @@ void Simulation::construct(const std::vector<const Network*>& nets, const Config
 	HIP_CHECK(hipEventCreateWithFlags(&m_stdpLag[1], hipEventDisableTiming));
 	HIP_CHECK(hipStreamCreateWithFlags(&m_earlyStream, hipStreamNonBlocking));
 	HIP_CHECK(hipStreamCreateWithFlags(&m_copyStream, hipStreamNonBlocking));
+	HIP_CHECK(hipStreamCreateWithFlags(&m_restStream, hipStreamNonBlocking));
 	HIP_CHECK(hipEventCreateWithFlags(&m_earlyFork, hipEventDisableTiming));
 	HIP_CHECK(hipEventCreateWithFlags(&m_earlyDone[0], hipEventDisableTiming));
 	HIP_CHECK(hipEventCreateWithFlags(&m_earlyDone[1], hipEventDisableTiming));
@@ -562,11 +563,13 @@ Simulation::~Simulation()
 	if(m_stdpStream) (void) hipStreamSynchronize(m_stdpStream);
 	if(m_earlyStream) (void) hipStreamSynchronize(m_earlyStream);
 	if(m_copyStream) (void) hipStreamSynchronize(m_copyStream);
+	if(m_restStream) (void) hipStreamSynchronize(m_restStream);
 	for(hipStream_t q : m_groupStreams) {
 		if(q) (void) hipStreamSynchronize(q);
 	}
 	clearGraphs();
 	if(m_copyStream) (void) hipStreamDestroy(m_copyStream);
+	if(m_restStream) (void) hipStreamDestroy(m_restStream);
 	for(hipStream_t q : m_groupStreams) {
 		if(q) (void) hipStreamDestroy(q);
 	}
@@ -591,6 +594,10 @@ Simulation::~Simulation()
 	if(m_pinnedScalars) (void) hipHostFree(m_pinnedScalars);
 	if(m_pinnedBits) (void) hipHostFree(m_pinnedBits);
 	if(m_pinnedCounters) (void) hipHostFree(m_pinnedCounters);
+	if(m_pinnedListCounts) (void) hipHostFree(m_pinnedListCounts);
+	for(uint32_t* p : m_pinnedList) {
+		if(p) (void) hipHostFree(p);
+	}
 }
 
 void Simulation::activate() const
@@ -1992,8 +1999,45 @@ uint64_t Simulation::sumCounters(const DeviceArray<unsigned long long>& a)
 	return s;
 }
 
+/* The device-compacted firing readout of run() for a single network, set
+ * up on first use; false (bitmaps decoded on the host instead) for batches
+ * or when its lists would take more than 1 GiB. */
+bool Simulation::ensureListReadout()
+{
+	if(m_nets != 1) return false;
+	const size_t CHUNK = NEMO_STEP_RING / 2;
+	if(m_listDev.n) return true;
+	if(2 * CHUNK * m_npad * sizeof(uint32_t) > (size_t(1) << 30)) return false;
+	m_l2gDev.allocate(m_npad);
+	HIP_CHECK(hipMemcpy(m_l2gDev.ptr, m_l2g.data(), m_l2gDev.bytes(), hipMemcpyHostToDevice));
+	m_listDev.allocate(2 * CHUNK * m_npad);
+	m_listCountsDev.allocate(2 * CHUNK);
+	HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&m_pinnedListCounts), 2 * CHUNK * sizeof(uint32_t),
+				hipHostMallocDefault));
+	std::memset(m_pinnedListCounts, 0, 2 * CHUNK * sizeof(uint32_t));
+	for(int h = 0; h < 2; ++h) growPinnedList(h, std::max<size_t>(65536, CHUNK * m_npad / 32), 0);
+	return true;
+}
+
+/* Room for need entries in half h's pinned list, keeping its first keep. */
+void Simulation::growPinnedList(int h, size_t need, size_t keep)
+{
+	if(need <= m_pinnedListCap[h]) return;
+	const size_t cap = std::max(need, 2 * m_pinnedListCap[h]);
+	uint32_t* p = nullptr;
+	HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&p), cap * sizeof(uint32_t), hipHostMallocDefault));
+	std::memset(p, 0, cap * sizeof(uint32_t)); /* its pages faulted in now, not by the readout */
+	if(keep) std::memcpy(p, m_pinnedList[h], keep * sizeof(uint32_t));
+	if(m_pinnedList[h]) HIP_CHECK(hipHostFree(m_pinnedList[h]));
+	m_pinnedList[h] = p;
+	m_pinnedListCap[h] = cap;
+}
+
 /* Steps without stimulus, captured in graphs of up to 64 steps; the firing
- * bitmaps of each graph's steps are read back after it. */
+ * of each graph's steps is read back after it: for a single network as
+ * per-step counts and a fired list compacted on the device (the list's
+ * expected length copied with the counts, any rest once they are known),
+ * for a batch as bitmaps decoded on the host. */
 void Simulation::run(unsigned nsteps, unsigned stdpPeriod, float reward, bool readRaster,
 		std::vector<unsigned>* counts, FiredSink* fired, uint64_t* events)
 {
@@ -2018,12 +2062,13 @@ void Simulation::run(unsigned nsteps, unsigned stdpPeriod, float reward, bool re
 		if(fired->cap == 0) fired->spill->reserve(fired->spill->size() + size_t(1.25 * m_firedPerStep * nsteps) + 1024);
 		counts->reserve(counts->size() + size_t(nsteps) * m_nets);
 	}
-	/* Chunks alternate between the two halves of the pinned bitmap buffer:
-	 * chunk i+1 is queued before chunk i's bitmaps are decoded on the host,
-	 * so the device does not wait for the host between chunks.  The copy of
-	 * chunk i runs on its own stream beside chunk i+1; the device ring holds
-	 * 128 steps, so chunk i's bitmaps are intact until chunk i+2, which waits
-	 * for that copy (the copy of chunk i is the last use of its half's event
+	/* Chunks alternate between the two halves of the readout buffers (pinned
+	 * bitmaps, or device and pinned lists): chunk i+1 is queued before chunk
+	 * i's firing is decoded on the host, so the device does not wait for the
+	 * host between chunks.  The copy of chunk i runs on its own stream beside
+	 * chunk i+1; the device ring holds 128 steps, so chunk i's bitmaps (and
+	 * its half's device list) are intact until chunk i+2, which waits for
+	 * that copy (the copy of chunk i is the last use of its half's event
 	 * before chunk i+2 records it again). */
 	hipEvent_t copied[2], ready[2];
 	bool inFlight[2] = {false, false};
@@ -2039,12 +2084,34 @@ void Simulation::run(unsigned nsteps, unsigned stdpPeriod, float reward, bool re
 	};
 	unsigned pendingSteps = 0; /* steps of the chunk waiting to be decoded */
 	int pendingHalf = -1;
+	const bool devList = readRaster && ensureListReadout();
+	size_t listCopied[2] = {0, 0}; /* entries of each half's list copied with its counts */
 	auto decodePending = [&]() {
 		if(pendingHalf < 0) return;
 		HIP_CHECK(hipEventSynchronize(copied[pendingHalf]));
-		const uint64_t* bits = m_pinnedBits + size_t(pendingHalf) * CHUNK * m_firedWords;
-		for(unsigned s = 0; s < pendingSteps; ++s) {
-			decodeFiringBatch(bits + size_t(s) * m_firedWords, *fired, *counts);
+		if(devList) {
+			const int h = pendingHalf;
+			const uint32_t* cnt = m_pinnedListCounts + size_t(h) * CHUNK;
+			size_t total = 0;
+			for(unsigned s = 0; s < pendingSteps; ++s) {
+				counts->push_back(cnt[s]);
+				total += cnt[s];
+			}
+			if(total > listCopied[h]) {
+				/* the rest, on a stream of its own: the copy stream may
+				 * already wait for the next chunk */
+				growPinnedList(h, total, listCopied[h]);
+				HIP_CHECK(hipMemcpyAsync(m_pinnedList[h] + listCopied[h],
+							m_listDev.ptr + size_t(h) * CHUNK * m_npad + listCopied[h],
+							(total - listCopied[h]) * sizeof(uint32_t), hipMemcpyDeviceToHost, m_restStream));
+				HIP_CHECK(hipStreamSynchronize(m_restStream));
+			}
+			fired->append(m_pinnedList[h], total);
+		} else {
+			const uint64_t* bits = m_pinnedBits + size_t(pendingHalf) * CHUNK * m_firedWords;
+			for(unsigned s = 0; s < pendingSteps; ++s) {
+				decodeFiringBatch(bits + size_t(s) * m_firedWords, *fired, *counts);
+			}
 		}
 		pendingHalf = -1;
 	};
@@ -2062,9 +2129,27 @@ void Simulation::run(unsigned nsteps, unsigned stdpPeriod, float reward, bool re
 			HIP_CHECK(hipGraphLaunch(g, m_stream));
 			advanceSteps(k);
 			if(readRaster) {
+				uint32_t* listDev = devList ? m_listDev.ptr + size_t(half) * CHUNK * m_npad : nullptr;
+				uint32_t* countsDev = devList ? m_listCountsDev.ptr + size_t(half) * CHUNK : nullptr;
+				if(devList) {
+					HIP_CHECK(nemo_launch_fired_list(m_stream, m_firedBits.ptr, m_firedWords, t0, k, m_l2gDev.ptr,
+								countsDev, listDev));
+				}
 				HIP_CHECK(hipEventRecord(ready[half], m_stream));
 				HIP_CHECK(hipStreamWaitEvent(m_copyStream, ready[half], 0));
-				readBitmaps(t0, k, m_pinnedBits + size_t(half) * CHUNK * m_firedWords, m_copyStream);
+				if(devList) {
+					/* the counts and as much of the list as recent rates suggest,
+					 * at least 1/64 of the neurons per step (half the bytes of
+					 * the bitmaps) */
+					const size_t expect = std::max(size_t(1.5 * m_firedPerStep * k), size_t(k) * m_npad / 64) + 4096;
+					listCopied[half] = std::min({expect, m_pinnedListCap[half], size_t(k) * m_npad});
+					HIP_CHECK(hipMemcpyAsync(m_pinnedListCounts + size_t(half) * CHUNK, countsDev, k * sizeof(uint32_t),
+								hipMemcpyDeviceToHost, m_copyStream));
+					HIP_CHECK(hipMemcpyAsync(m_pinnedList[half], listDev, listCopied[half] * sizeof(uint32_t),
+								hipMemcpyDeviceToHost, m_copyStream));
+				} else {
+					readBitmaps(t0, k, m_pinnedBits + size_t(half) * CHUNK * m_firedWords, m_copyStream);
+				}
 				HIP_CHECK(hipEventRecord(copied[half], m_copyStream));
 				inFlight[half] = true;
 			}

@@ -285,6 +285,7 @@ private:
 	hipEvent_t m_stdpLag[2] = {nullptr, nullptr}; /* mode 3: branch of step parity i done */
 	hipStream_t m_earlyStream = nullptr; /* branch for the delays 2..D of the next step */
 	hipStream_t m_copyStream = nullptr;  /* firing-bitmap readout of run(), beside the next steps */
+	hipStream_t m_restStream = nullptr;  /* the rest of a fired list longer than expected */
 	/* with several neuron models, every model's update after the first runs
 	 * on a branch of its own, so a step waits for the slowest model */
 	std::vector<hipStream_t> m_groupStreams;
@@ -392,6 +393,16 @@ private:
 	uint64_t* m_pinnedScalars = nullptr; /* ring of sources for host-to-device scalars */
 	unsigned m_scalarNext = 0;
 	uint64_t* m_pinnedBits = nullptr;  /* firing bitmaps read back */
+	/* run()'s firing readout of a single network, compacted on the device:
+	 * per half (chunks alternate), the steps' counts and fired lists */
+	DeviceArray<uint32_t> m_l2gDev;       /* m_l2g on the device */
+	DeviceArray<uint32_t> m_listDev;      /* [2][CHUNK * m_npad] fired lists */
+	DeviceArray<uint32_t> m_listCountsDev; /* [2][CHUNK] */
+	uint32_t* m_pinnedListCounts = nullptr; /* [2][CHUNK] */
+	uint32_t* m_pinnedList[2] = {nullptr, nullptr};
+	size_t m_pinnedListCap[2] = {0, 0};
+	bool ensureListReadout();
+	void growPinnedList(int half, size_t need, size_t keep);
 	unsigned long long* m_pinnedCounters = nullptr; /* [2][m_blockEvents.n]: the event counters at the
 	                                                   start and at the end of a run */
 	std::vector<unsigned> m_firedOut;

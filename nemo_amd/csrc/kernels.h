@@ -92,6 +92,13 @@ hipError_t nemo_launch_external_fire(hipStream_t, const nemo_hip_step_buffers* b
 hipError_t nemo_launch_gather_currents(hipStream_t, const nemo_hip_step_buffers* b, const uint32_t* dev_to_sim,
 		float* E, float* I, unsigned tiles);
 
+/* Firing readout of k steps from step t0 (their firing-bitmap ring slots,
+ * words words each): counts[s] the neurons fired in step t0 + s, and list
+ * their l2g entries, step after step, each step in ascending device index
+ * (list holds up to k * words * 64 entries). */
+hipError_t nemo_launch_fired_list(hipStream_t, const uint64_t* bits, size_t words, uint64_t t0, unsigned k,
+		const uint32_t* l2g, uint32_t* counts, uint32_t* list);
+
 /* Firing stimulus: set (or clear) the bits of the n given device indices. */
 hipError_t nemo_launch_fstim(hipStream_t, uint32_t* bits, const uint32_t* idx, size_t n, int set);
 

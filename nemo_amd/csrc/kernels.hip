@@ -163,6 +163,53 @@ invert_rcm_kernel(const uint32_t* rcm_fwd, uint32_t* rcm_of, size_t count)
 	}
 }
 
+/* ------------------------------------------------------ firing readout */
+
+/* Neurons fired in step t0 + blockIdx.x (its firing-bitmap ring slot) */
+__global__ void __launch_bounds__(NEMO_BLOCK)
+fired_count_kernel(const uint64_t* bits, size_t words, uint64_t t0, uint32_t* counts)
+{
+	const uint64_t* b = bits + ((t0 + blockIdx.x) & (NEMO_STEP_RING - 1)) * words;
+	uint32_t c = 0;
+	for(size_t w = threadIdx.x; w < words; w += NEMO_BLOCK) c += uint32_t(__popcll(b[w]));
+	uint32_t total;
+	nemo_dev::block_exclusive_scan<NEMO_BLOCK>(c, &total);
+	if(threadIdx.x == 0) counts[blockIdx.x] = total;
+}
+
+/* The neurons fired in step t0 + blockIdx.x, in ascending device index, as
+ * l2g's indices, written after those of the steps before it (counts) */
+__global__ void __launch_bounds__(NEMO_BLOCK)
+fired_list_kernel(const uint64_t* bits, size_t words, uint64_t t0, const uint32_t* counts, const uint32_t* l2g,
+		uint32_t* list)
+{
+	__shared__ uint32_t s_off;
+	if(threadIdx.x < 64) {
+		uint32_t v = 0;
+		for(unsigned s = threadIdx.x; s < blockIdx.x; s += 64) v += counts[s];
+#pragma unroll
+		for(int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+		if(threadIdx.x == 0) s_off = v;
+	}
+	__syncthreads();
+	const uint64_t* b = bits + ((t0 + blockIdx.x) & (NEMO_STEP_RING - 1)) * words;
+	/* a contiguous run of words per thread keeps the list in device order */
+	const size_t per = (words + NEMO_BLOCK - 1) / NEMO_BLOCK;
+	const size_t w0 = min(words, size_t(threadIdx.x) * per), w1 = min(words, w0 + per);
+	uint32_t c = 0;
+	for(size_t w = w0; w < w1; ++w) c += uint32_t(__popcll(b[w]));
+	uint32_t total;
+	uint32_t pos = s_off + nemo_dev::block_exclusive_scan<NEMO_BLOCK>(c, &total);
+	for(size_t w = w0; w < w1; ++w) {
+		uint64_t x = b[w];
+		while(x) {
+			const unsigned bit = unsigned(__builtin_ctzll(x));
+			x &= x - 1;
+			list[pos++] = l2g[w * 64 + bit];
+		}
+	}
+}
+
 /* ------------------------------------------- externally given firing */
 
 /* The firing of step s given from outside (Brian's propagate,
@@ -382,6 +429,18 @@ hipError_t nemo_launch_gather_currents(hipStream_t st, const nemo_hip_step_buffe
 {
 	if(tiles == 0) return hipSuccess;
 	hipLaunchKernelGGL(gather_currents_kernel, dim3(tiles), dim3(NEMO_TILE), 0, st, *b, dev_to_sim, E, I);
+	return hipGetLastError();
+}
+
+hipError_t nemo_launch_fired_list(hipStream_t st, const uint64_t* bits, size_t words, uint64_t t0, unsigned k,
+		const uint32_t* l2g, uint32_t* counts, uint32_t* list)
+{
+	if(k == 0) return hipSuccess;
+	if(k > NEMO_STEP_RING || !bits || !l2g || !counts || !list) return hipErrorInvalidValue;
+	hipLaunchKernelGGL(fired_count_kernel, dim3(k), dim3(NEMO_BLOCK), 0, st, bits, words, t0, counts);
+	const hipError_t e = hipGetLastError();
+	if(e != hipSuccess) return e;
+	hipLaunchKernelGGL(fired_list_kernel, dim3(k), dim3(NEMO_BLOCK), 0, st, bits, words, t0, counts, l2g, list);
 	return hipGetLastError();
 }
 
