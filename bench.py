@@ -73,25 +73,25 @@ def launch_bytes(shape, kern, work, n, spec):
 
 def measured_traffic(kernel, shape):
     """HBM bytes per launch of `kernel` from the committed rocprof counter
-    summary of this workload (FETCH_SIZE and WRITE_SIZE passes scaled by the
-    copy-kernel calibration), when it was measured at a launch shape that
-    agrees with `shape` in the fields that matter (traffic_key); None
-    otherwise (the traffic of a launch depends on its shape)."""
-    path = os.path.join(ROOT, "profiles", "propagate_update_izh100k_stdp.md")
-    names = {"propagate": "propagate_kernel", "update": "izhikevich_update"}
-    try:
-        measured_shape = None
-        traffic = None
-        for line in open(path):
-            if line.startswith("traffic_launch_shape:"):
-                measured_shape = [int(x) for x in line.split(":", 1)[1].split()]
-            if line.startswith("traffic_bytes_per_launch:"):
-                fields = dict(f.split("=") for f in line.split(":", 1)[1].split())
-                traffic = float(fields[names[kernel]])
-        if measured_shape is not None and traffic_key(measured_shape) == traffic_key(shape):
-            return traffic
-    except (OSError, KeyError, ValueError):
-        pass
+    summaries of this workload (FETCH_SIZE and WRITE_SIZE passes scaled by the
+    copy-kernel calibration), from the one measured at a launch shape that
+    agrees with `shape` in the fields that matter (traffic_key); None if
+    there is none (the traffic of a launch depends on its shape)."""
+    split = shape[1] > 0
+    names = {"propagate": "propagate_sets_kernel" if split else "propagate_kernel", "update": "izhikevich_update"}
+    for name in ("update_delivery_izh100k_stdp.md", "propagate_update_izh100k_stdp.md"):
+        try:
+            measured_shape = None
+            fields = {}
+            for line in open(os.path.join(ROOT, "profiles", name)):
+                if line.startswith("traffic_launch_shape:"):
+                    measured_shape = [int(x) for x in line.split(":", 1)[1].split()]
+                if line.startswith("traffic_bytes_per_launch:"):
+                    fields = dict(f.split("=") for f in line.split(":", 1)[1].split())
+            if measured_shape is not None and traffic_key(measured_shape) == traffic_key(shape):
+                return float(fields[names[kernel]])
+        except (OSError, KeyError, ValueError):
+            pass
     return None
 
 
@@ -349,7 +349,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch-nets", type=int, default=64, help="networks in the batched run (0: skip it)")
     ap.add_argument("--batch-steps", type=int, default=1000)
-    ap.add_argument("--autotune-steps", type=int, default=128, help="steps per autotuner trial (0: default shape)")
+    ap.add_argument("--autotune-steps", type=int, default=128,
+                    help="steps per autotuner trial of the batched and mixed runs (0: default shape)")
+    ap.add_argument("--autotune-plain", type=int, default=0, metavar="STEPS",
+                    help="autotune the plain run too, with STEPS per trial (default: its default launch shape, "
+                         "whose traffic the committed counter profile measured)")
     ap.add_argument("--diff-steps", type=int, default=500, help="steps of the differentiable run (0: skip it)")
     ap.add_argument("--mixed-steps", type=int, default=2000, help="steps of the mixed-model torus run (0: skip it)")
     ap.add_argument("--only", choices=["batched", "differentiable", "mixed"],
@@ -402,11 +406,14 @@ def main():
     print("built %d neurons, %d synapses (%d plastic) in %.1f s (generation %.1f s)"
           % (n, sim.synapse_count, sim.plastic_count, t_ready - t_build, t_build - t_gen), file=sys.stderr, flush=True)
 
-    sim.run(args.warmup, stdp_period=1000, reward=1.0)
     default_shape = sim.launch_shape
-    shape = sim.autotune(args.autotune_steps) if args.autotune_steps else default_shape
+    shape = sim.autotune(args.autotune_plain) if args.autotune_plain else default_shape
+    print("launch shape %s (default %s)" % (shape, default_shape), file=sys.stderr, flush=True)
+    # the timed run's graphs are built first, so that the warm-up steps run
+    # right before it (an idle GPU clocks down, and a short timed run would
+    # pay for the ramp back up)
     sim.prepare(args.steps, stdp_period=1000)
-    print("warmup done; launch shape %s (default %s)" % (shape, default_shape), file=sys.stderr, flush=True)
+    sim.run(args.warmup, stdp_period=1000, reward=1.0)
 
     t0 = time.perf_counter()
     counts, fired, events = sim.run(args.steps, stdp_period=1000, reward=1.0)

@@ -1525,6 +1525,37 @@ void Simulation::copyScalar(uint64_t* dst, uint64_t value)
 	HIP_CHECK(hipMemcpyAsync(dst, slot, sizeof(uint64_t), hipMemcpyHostToDevice, m_stream));
 }
 
+/* Wait for an event (or a stream) by polling it for up to 50 ms before the
+ * blocking wait: a blocking wait can put the thread to sleep and wake it
+ * milliseconds after the work ended, which costs more than a whole short
+ * run of steps. */
+static void pollThenWait(hipError_t (*query)(void*), hipError_t (*wait)(void*), void* x)
+{
+	const auto t0 = std::chrono::steady_clock::now();
+	for(;;) {
+		const hipError_t e = query(x);
+		if(e == hipSuccess) return;
+		if(e != hipErrorNotReady) HIP_CHECK(e);
+		if(std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(50)) break;
+#if defined(__x86_64__)
+		__builtin_ia32_pause();
+#endif
+	}
+	HIP_CHECK(wait(x));
+}
+
+void Simulation::waitEvent(hipEvent_t e)
+{
+	pollThenWait([](void* x) { return hipEventQuery(static_cast<hipEvent_t>(x)); },
+			[](void* x) { return hipEventSynchronize(static_cast<hipEvent_t>(x)); }, e);
+}
+
+void Simulation::waitStream(hipStream_t s)
+{
+	pollThenWait([](void* x) { return hipStreamQuery(static_cast<hipStream_t>(x)); },
+			[](void* x) { return hipStreamSynchronize(static_cast<hipStream_t>(x)); }, s);
+}
+
 /* The step base used by the next launches: step = base + launch offset. */
 void Simulation::setBase(uint64_t base)
 {
@@ -1950,7 +1981,7 @@ const std::vector<unsigned>& Simulation::step(const unsigned* fstim, size_t nf,
 		HIP_CHECK(nemo_launch_fstim(m_stream, m_fstim.ptr, m_stimIdx.ptr, fIdx.size(), 0));
 	}
 	readBitmaps(t, 1);
-	HIP_CHECK(hipStreamSynchronize(m_stream));
+	waitStream(m_stream);
 	checkStatus();
 	m_firedOut.clear();
 	decodeFiring(m_pinnedBits, m_firedOut);
@@ -2045,7 +2076,7 @@ void Simulation::run(unsigned nsteps, unsigned stdpPeriod, float reward, bool re
 	if(stdpPeriod && !m_stdp) {
 		throw Error(NEMO_LOGIC_ERROR, "applyStdp called, but no STDP model specified");
 	}
-	HIP_CHECK(hipStreamSynchronize(m_stream));
+	waitStream(m_stream);
 	/* the event counters before and after the run's work, copied on its
 	 * stream (no blocking copy at either end) */
 	const size_t nCounters = m_blockEvents.n;
@@ -2088,7 +2119,7 @@ void Simulation::run(unsigned nsteps, unsigned stdpPeriod, float reward, bool re
 	size_t listCopied[2] = {0, 0}; /* entries of each half's list copied with its counts */
 	auto decodePending = [&]() {
 		if(pendingHalf < 0) return;
-		HIP_CHECK(hipEventSynchronize(copied[pendingHalf]));
+		waitEvent(copied[pendingHalf]);
 		if(devList) {
 			const int h = pendingHalf;
 			const uint32_t* cnt = m_pinnedListCounts + size_t(h) * CHUNK;
@@ -2104,7 +2135,7 @@ void Simulation::run(unsigned nsteps, unsigned stdpPeriod, float reward, bool re
 				HIP_CHECK(hipMemcpyAsync(m_pinnedList[h] + listCopied[h],
 							m_listDev.ptr + size_t(h) * CHUNK * m_npad + listCopied[h],
 							(total - listCopied[h]) * sizeof(uint32_t), hipMemcpyDeviceToHost, m_restStream));
-				HIP_CHECK(hipStreamSynchronize(m_restStream));
+				waitStream(m_restStream);
 			}
 			fired->append(m_pinnedList[h], total);
 		} else {
@@ -2180,9 +2211,9 @@ void Simulation::run(unsigned nsteps, unsigned stdpPeriod, float reward, bool re
 		destroyEvents();
 		throw;
 	}
-	HIP_CHECK(hipStreamSynchronize(m_copyStream));
+	waitStream(m_copyStream);
 	destroyEvents();
-	HIP_CHECK(hipStreamSynchronize(m_stream));
+	waitStream(m_stream);
 	if(readRaster && fired && nsteps) m_firedPerStep = double(fired->size() - firedBefore) / nsteps;
 	if(events) {
 		uint64_t before = 0, after = 0;

@@ -402,6 +402,8 @@ private:
 	uint32_t* m_pinnedList[2] = {nullptr, nullptr};
 	size_t m_pinnedListCap[2] = {0, 0};
 	bool ensureListReadout();
+	void waitEvent(hipEvent_t);
+	void waitStream(hipStream_t);
 	void growPinnedList(int half, size_t need, size_t keep);
 	unsigned long long* m_pinnedCounters = nullptr; /* [2][m_blockEvents.n]: the event counters at the
 	                                                   start and at the end of a run */
