@@ -246,6 +246,33 @@ def test_run_matches_step():
     assert np.array_equal(a.fixed_weights(), b.fixed_weights())
 
 
+def test_run_readout_longer_than_expected_matches_step():
+    """A run's fired list is compacted on the device and copied with the
+    counts up to an expected length; a network that fires far more (4000
+    Poisson sources at 0.5 per step, added in a shuffled index order, so the
+    index map is not linear) takes the rest-of-list copy and grows the host
+    buffer, and still equals the lists of single steps."""
+    idx = np.random.default_rng(3).permutation(4000).astype(np.uint32)
+
+    def make():
+        net = nemo_amd.Network()
+        t = net.add_neuron_type("PoissonSource")
+        net.add_neuron(t, idx, np.full(idx.size, 0.5, np.float32))
+        return nemo_amd.Simulation(net, nemo_amd.Configuration())
+
+    a, b = make(), make()
+    ca, fa, _ = a.run(130)
+    ca2, fa2, _ = a.run(70)
+    fb, cb = [], []
+    for _ in range(200):
+        f = b.step()
+        cb.append(len(f))
+        fb.extend(f)
+    assert np.array_equal(np.concatenate([ca, ca2]), cb)
+    assert np.array_equal(np.concatenate([fa, fa2]), fb)
+    assert ca.sum() > 64 * 1024  # more than the host buffer first held for a run of 64 steps
+
+
 def _conf(api, stdp):
     c = api.Configuration()
     c.set_stdp_function(*stdp)
