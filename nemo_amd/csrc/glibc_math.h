@@ -269,6 +269,23 @@ NEMO_HD float fmodf_exact(float x, float y)
 	return asfloat(ux | sign);
 }
 
+/* An argument of expf divided by a constant c > 0 (HH's rate exponents
+ * divide by 20, 80 and 18, or by 1), with inv = 1/c rounded to float: the
+ * product x inv corrected once by its exact FMA residual x - q c.  For every
+ * float x of magnitude at least 2^-122 this is the correctly rounded x / c,
+ * bit for bit; below that both quotients are under 2^-126 in magnitude, so
+ * expf of either is 1.  Infinite x keep the product (the residual would be
+ * NaN).  The tests check both statements over every float for the four
+ * divisors.  It replaces a correctly rounded division, the longest link in
+ * HH's substep chain. */
+NEMO_HD float div_by_const(float x, float c, float inv)
+{
+	const float q0 = x * inv;
+	const float r = __builtin_fmaf(-q0, c, x);
+	const float q1 = __builtin_fmaf(r, inv, q0);
+	return __builtin_isfinite(q0) ? q1 : q0;
+}
+
 } // namespace nemo_math
 
 #endif

@@ -148,9 +148,11 @@ hh_input(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
  * The substeps are one dependent chain, so their latency is the launch's
  * length: every lane runs the same instructions (the six rate formulas are
  * one division of selected operands: a division by 1 is exact, so
- * "fac * e / 1" and "(-x / 1)" are fac * e and -x bit for bit), expf's
- * special cases are selected rather than branched, and its table is read
- * from LDS. */
+ * "fac * e / 1" and "(-x / 1)" are fac * e and -x bit for bit), the
+ * exponent's division by its constant is a reciprocal product with one FMA
+ * correction (nemo_math::div_by_const: the same bits, or for tiny arguments
+ * the same expf), expf's special cases are selected rather than branched,
+ * and its table is read from LDS. */
 __global__ void __launch_bounds__(SUB_BLOCK)
 hh_substeps(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 {
@@ -182,6 +184,7 @@ hh_substeps(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 	const bool linear = r == 0 || r == 1 || r == 5;
 	const float lin0 = r == 0 ? 1.0f : r == 1 ? 2.5f : 3.0f;
 	const float adiv = linear ? 1.0f : r == 2 ? 20.0f : r == 3 ? 80.0f : 18.0f;
+	const float ainv = 1.0f / adiv;
 	const float fac = r == 2 ? 0.07f : r == 3 ? 0.125f : 4.0f;
 	const float c0 = r == 0 ? 0.1f : 2.5f;
 	const float c1 = r == 0 ? 0.01f : 0.1f;
@@ -191,7 +194,8 @@ hh_substeps(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 	for(int it = 0; it < inc_max; ++it) {
 		const float x = v + 65.0f;
 		const float anum = linear ? lin0 - 0.1f * x : -x;
-		const float e = nemo_math::expf_exact_tab(anum / adiv, s_exp);
+		/* anum / adiv, the same bits or (tiny anum) the same expf */
+		const float e = nemo_math::expf_exact_tab(nemo_math::div_by_const(anum, adiv, ainv), s_exp);
 		const float poly = c0 - c1 * x;
 		const float fe = fac * e;
 		const float rnum = r <= 1 ? poly : r == 5 ? 1.0f : fe;

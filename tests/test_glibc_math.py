@@ -8,6 +8,8 @@ host C library, over every float the callers can produce:
   expf(x),     every float (HH.cpp:92-98 calls it on the membrane
                potential's functions; the special cases are selected, not
                branched, so they are checked too)
+  x / c        HH's exponent divisions by 20, 80, 18 and 1 as a reciprocal
+               product with one FMA correction (div_by_const), every float
 
 The same header is compiled for the host here; the GPU tests run it on the
 device against the same host results.
@@ -73,7 +75,26 @@ int main() {
             if (bits(a) != bits(b) && !(a != a && b != b)) bad_fmod++;
         }
     }
-    printf("%ld %ld %ld %ld %ld %ld %ld %ld\n", bad_log, n_log, bad_sin, n_sin, bad_exp, n_exp, bad_fmod, n_fmod);
+    /* div_by_const against the correctly rounded division, every float x,
+     * for HH's divisors: quotients must agree wherever |x| >= 2^-122, and
+     * expf must agree on the two quotients everywhere */
+    long div_far = 0, div_exp = 0, div_near = 0;
+    const float cs[] = {1.0f, 20.0f, 80.0f, 18.0f};
+    for (float c : cs) {
+        const float inv = 1.0f / c;
+        #pragma omp parallel for reduction(+:div_far, div_exp, div_near) schedule(static, 1 << 20)
+        for (int64_t u = 0; u < (int64_t(1) << 32); u += 1) {
+            uint32_t w = (uint32_t)u;
+            float x; memcpy(&x, &w, 4);
+            if (x != x) continue;
+            const float a = x / c, b = div_by_const(x, c, inv);
+            if (bits(a) == bits(b)) continue;
+            if (fabsf(x) >= 0x1p-122f) div_far++; else div_near++;
+            if (bits(expf(a)) != bits(expf(b))) div_exp++;
+        }
+    }
+    printf("%ld %ld %ld %ld %ld %ld %ld %ld %ld %ld %ld\n", bad_log, n_log, bad_sin, n_sin, bad_exp, n_exp, bad_fmod, n_fmod,
+           div_far, div_exp, div_near);
     return 0;
 }
 """
@@ -107,6 +128,17 @@ def test_expf_matches_host_over_every_float(results):
     bad, n = results[4], results[5]
     assert n == 1 << 32
     assert bad == 0
+
+
+def test_division_by_hh_constants_matches_for_expf(results):
+    """HH's exponent divisions (x / 20, / 80, / 18, / 1) by reciprocal and one
+    FMA correction: the correctly rounded quotient wherever |x| >= 2^-122,
+    and the same expf everywhere.  Below 2^-122 the quotients do differ
+    (div_near > 0), so the comparison is not vacuous."""
+    div_far, div_exp, div_near = results[8], results[9], results[10]
+    assert div_far == 0
+    assert div_exp == 0
+    assert div_near > 0
 
 
 def test_fmodf_matches_host(results):
