@@ -449,8 +449,6 @@ void Configuration::setHipBackend(int dev)
 	description = std::string("HIP backend (") + prop.name + ", " + prop.gcnArchName + ")";
 }
 
-/* LDS an update kernel declares itself, on top of what the propagate blocks
- * it may carry need */
 /* static LDS of an update kernel that carries propagate blocks, beside the
  * dynamic buffer both kinds of block share (block scans' wave totals) */
 static const size_t COLAUNCH_LDS_MARGIN = 1024;
@@ -1185,10 +1183,11 @@ static Partition partitionPairs(const std::vector<uint64_t>& pairWeight, const s
 
 /* Default shape: bin capacity 4096 events (halved while the LDS does not
  * allow it), enough delivery blocks for about 1% of the synapses per step,
- * and, with delays above 1, a delay-1 set of about 1/D of them, so that the
- * update launches carry the next step's delivery and push its delay-1
- * events (overlap 3); STDP accumulated in batches beside the steps, one
- * bitmap word per 64 threads of an STDP block.  NEMO_HIP_DELIVER_BLOCKS,
+ * and, with delays above 1 and few enough blocks to be resident together, a
+ * delay-1 set of about 1/D of them, so that the update launches carry the
+ * next step's delivery and push its delay-1 events (overlap 3); STDP
+ * accumulated in batches beside the steps, one bitmap word per 64 threads
+ * of an STDP block.  NEMO_HIP_DELIVER_BLOCKS,
  * NEMO_HIP_LATE_BLOCKS, NEMO_HIP_BIN_CAPACITY, NEMO_HIP_STDP_MODE,
  * NEMO_HIP_STDP_WORDS and NEMO_HIP_OVERLAP override it (tuning). */
 LaunchShape Simulation::defaultShape() const
@@ -1204,11 +1203,18 @@ LaunchShape Simulation::defaultShape() const
 		if(ldsNeeded(s) <= m_ldsBudget) break;
 	}
 	if(m_D >= 2) {
+		/* one launch per step pays when the update launch's tiles and the
+		 * delivery blocks it carries are all resident at once (a 100k-neuron
+		 * network: 391 + 245 blocks, 3 per CU); with many more (a batch of
+		 * networks, 4050 tiles) its blocks wait for each other's CU slots */
 		LaunchShape c = s;
 		c.lateBlocks = std::max(1u, unsigned(std::lround(double(s.deliverBlocks) / m_D)));
 		c.deliverBlocks = std::min(s.deliverBlocks, NEMO_MAX_BIN_BLOCKS - c.lateBlocks);
 		c.overlap = 3;
-		if(ldsNeeded(c) <= m_ldsBudget) s = c;
+		const size_t lds = ldsNeeded(c);
+		const size_t perCu = std::min<size_t>(m_ldsPerCu / std::max<size_t>(1, lds + COLAUNCH_LDS_MARGIN),
+				2048 / NEMO_COLAUNCH_BLOCK);
+		if(lds <= m_ldsBudget && size_t(m_tiles) + c.deliverBlocks <= m_cuCount * perCu) s = c;
 	}
 	s.stdpMode = 4;
 	s.stdpWords = NEMO_STDP_MAX_WORDS; /* few, wide batch blocks disturb the steps least */
@@ -1403,6 +1409,11 @@ void Simulation::allocateStepState()
 	int maxLds = 65536;
 	(void) hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, m_device);
 	m_ldsBudget = size_t(maxLds) - std::min<size_t>(size_t(maxLds), nemo_propagate_static_lds());
+	int cus = 256, ldsPerCu = 163840;
+	(void) hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, m_device);
+	(void) hipDeviceGetAttribute(&ldsPerCu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, m_device);
+	m_cuCount = unsigned(std::max(1, cus));
+	m_ldsPerCu = size_t(std::max(65536, ldsPerCu));
 	m_clock.allocate(1);
 	nemo_hip_clock c0;
 	c0.base_step = 0;

@@ -370,6 +370,8 @@ private:
 	LaunchShape m_shape;
 	unsigned m_stdpBlocks = 0;
 	size_t m_ldsBudget = 0;
+	unsigned m_cuCount = 256;         /* compute units of the device */
+	size_t m_ldsPerCu = 163840;       /* LDS bytes per compute unit */
 
 	/* host side of synapse queries (network order = source ascending, then insertion) */
 	std::map<uint64_t, std::pair<size_t, size_t> > m_synRange; /* network << 32 | gsrc -> [first,last) */
