@@ -2173,13 +2173,13 @@ void Simulation::run(unsigned nsteps, unsigned stdpPeriod, float reward, bool re
 			if(readRaster) {
 				uint32_t* listDev = devList ? m_listDev.ptr + size_t(half) * CHUNK * m_npad : nullptr;
 				uint32_t* countsDev = devList ? m_listCountsDev.ptr + size_t(half) * CHUNK : nullptr;
-				if(devList) {
-					HIP_CHECK(nemo_launch_fired_list(m_stream, m_firedBits.ptr, m_firedWords, t0, k, m_l2gDev.ptr,
-								countsDev, listDev));
-				}
 				HIP_CHECK(hipEventRecord(ready[half], m_stream));
 				HIP_CHECK(hipStreamWaitEvent(m_copyStream, ready[half], 0));
 				if(devList) {
+					/* compacted on the copy stream too, beside the next chunk (or
+					 * the run's closing STDP batch) */
+					HIP_CHECK(nemo_launch_fired_list(m_copyStream, m_firedBits.ptr, m_firedWords, t0, k,
+								m_l2gDev.ptr, countsDev, listDev));
 					/* the counts and as much of the list as recent rates suggest,
 					 * at least 1/64 of the neurons per step (half the bytes of
 					 * the bitmaps) */
