@@ -1611,7 +1611,7 @@ void Simulation::launchDelivery(const nemo_propagate_args& set, uint64_t offset,
  * snapshot ends the run (and every 64 steps).  With ev (four events per
  * step), the kernels record the start and end of the step's delivery on this
  * stream and of its neuron updates. */
-void Simulation::enqueueSteps(uint64_t first, unsigned k, bool ext, bool fstim, hipEvent_t* ev)
+void Simulation::enqueueSteps(uint64_t first, unsigned k, bool ext, bool fstim, hipEvent_t* ev, bool forkPending)
 {
 	const bool split = m_late.deliver_blocks && m_early.deliver_blocks;
 	const bool overlap = split && m_shape.overlap == 1 && k > 1;
@@ -1629,13 +1629,16 @@ void Simulation::enqueueSteps(uint64_t first, unsigned k, bool ext, bool fstim, 
 	/* the pending STDP batch runs on its own branch beside the steps; it
 	 * reads only the history snapshots, and is joined before the next
 	 * snapshot replaces the older of them */
+	bool forked = false;
 	auto forkBatch = [&]() {
 		HIP_CHECK(hipEventRecord(m_stdpFork, m_stream));
 		HIP_CHECK(hipStreamWaitEvent(m_stdpStream, m_stdpFork, 0));
 		HIP_CHECK(nemo_launch_stdp_batch(m_stdpStream, &m_propagate, m_stdpBlocks));
 		HIP_CHECK(hipEventRecord(m_stdpDone, m_stdpStream));
+		forked = true;
 	};
-	if(batched && k) forkBatch();
+	/* without forkPending the caller knows that no batch is pending */
+	if(batched && k && forkPending) forkBatch();
 	bool pending[2] = {false, false};
 	auto forkEarly = [&](unsigned i) {
 		HIP_CHECK(hipEventRecord(m_earlyFork, m_stream));
@@ -1759,7 +1762,7 @@ void Simulation::enqueueSteps(uint64_t first, unsigned k, bool ext, bool fstim, 
 		if(batched && (i + 1 == k || (i + 1) % 64 == 0)) {
 			/* snapshot after this step: the steps since the last one are
 			 * pending for the next batch */
-			HIP_CHECK(hipStreamWaitEvent(m_stream, m_stdpDone, 0));
+			if(forked) HIP_CHECK(hipStreamWaitEvent(m_stream, m_stdpDone, 0));
 			HIP_CHECK(nemo_launch_stdp_snapshot(m_stream, &m_propagate, first + i));
 			if(i + 1 < k) forkBatch();
 		}
@@ -1999,16 +2002,17 @@ const std::vector<unsigned>& Simulation::step(const unsigned* fstim, size_t nf,
 	return m_firedOut;
 }
 
-hipGraphExec_t Simulation::graphFor(unsigned k)
+hipGraphExec_t Simulation::graphFor(unsigned k, bool forkPending)
 {
-	auto it = m_graphs.find(k);
+	const unsigned key = 2 * k + (forkPending ? 1u : 0u);
+	auto it = m_graphs.find(key);
 	if(it != m_graphs.end()) return it->second;
 	hipGraph_t graph;
 	HIP_CHECK(hipStreamBeginCapture(m_stream, hipStreamCaptureModeThreadLocal));
 	const bool debug = m_debug;
 	m_debug = false;
 	try {
-		enqueueSteps(0, k, false, false);
+		enqueueSteps(0, k, false, false, nullptr, forkPending);
 	} catch(...) {
 		m_debug = debug;
 		m_stdpPending = false;
@@ -2022,7 +2026,7 @@ hipGraphExec_t Simulation::graphFor(unsigned k)
 	HIP_CHECK(hipGraphDestroy(graph));
 	/* its launch packets go to the device now, not in its first (timed) launch */
 	HIP_CHECK(hipGraphUpload(exec, m_stream));
-	m_graphs[k] = exec;
+	m_graphs[key] = exec;
 	return exec;
 }
 
@@ -2090,11 +2094,17 @@ void Simulation::run(unsigned nsteps, unsigned stdpPeriod, float reward, bool re
 	waitStream(m_stream);
 	/* the event counters before and after the run's work, copied on its
 	 * stream (no blocking copy at either end) */
+	/* The counters are copied even when the caller does not ask for the
+	 * events: a run whose graph was the first work queued on the stream
+	 * after the synchronisation above started it about 5 ms late on the
+	 * MI355X in every 20-step run measured (with the copy queued first: 22-23
+	 * us per step; without: 250 us). */
 	const size_t nCounters = m_blockEvents.n;
-	if(events) {
-		HIP_CHECK(hipMemcpyAsync(m_pinnedCounters, m_blockEvents.ptr, m_blockEvents.bytes(), hipMemcpyDeviceToHost,
-					m_stream));
-	}
+	HIP_CHECK(hipMemcpyAsync(m_pinnedCounters, m_blockEvents.ptr, m_blockEvents.bytes(), hipMemcpyDeviceToHost,
+				m_stream));
+	/* a run begins with no STDP batch pending when the last thing before it
+	 * was a run (which ends with its flush) */
+	bool batchPending = m_stdpBlocks && m_hostStep > 0 && m_flushed != m_hostStep - 1;
 	uint64_t stepsSinceApply = 0;
 	const unsigned CHUNK = NEMO_STEP_RING / 2;
 	/* room for the fired list at the rate of earlier runs, so that it is not
@@ -2164,7 +2174,8 @@ void Simulation::run(unsigned nsteps, unsigned stdpPeriod, float reward, bool re
 			unsigned k = std::min(CHUNK, nsteps - done);
 			if(stdpPeriod) k = unsigned(std::min<uint64_t>(k, stdpPeriod - stepsSinceApply));
 			const uint64_t t0 = m_hostStep;
-			hipGraphExec_t g = graphFor(k);
+			hipGraphExec_t g = graphFor(k, batchPending);
+			batchPending = true; /* each chunk leaves its steps pending */
 			setBase(t0);
 			/* the ring slots this chunk overwrites were copied out */
 			if(inFlight[half]) HIP_CHECK(hipStreamWaitEvent(m_stream, copied[half], 0));
@@ -2300,7 +2311,8 @@ void Simulation::prepare(unsigned nsteps, unsigned stdpPeriod)
 	while(done < nsteps) {
 		unsigned k = std::min(64u, nsteps - done);
 		if(stdpPeriod) k = unsigned(std::min<uint64_t>(k, stdpPeriod - since));
-		graphFor(k);
+		graphFor(k, true);
+		if(done == 0) graphFor(k, false); /* a run's first chunk with nothing pending */
 		done += k;
 		since += k;
 		if(stdpPeriod && since == stdpPeriod) since = 0;
@@ -2379,7 +2391,7 @@ double Simulation::timeSteps(unsigned nsteps)
 	unsigned done = 0;
 	while(done < nsteps) {
 		const unsigned k = std::min(64u, nsteps - done);
-		hipGraphExec_t g = graphFor(k);
+		hipGraphExec_t g = graphFor(k, true);
 		setBase(m_hostStep);
 		HIP_CHECK(hipGraphLaunch(g, m_stream));
 		advanceSteps(k);

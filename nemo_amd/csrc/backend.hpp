@@ -431,7 +431,8 @@ private:
 	nemo_hip_step_buffers stepBuffers(uint64_t offset, bool ext, bool fstim) const;
 	void copyScalar(uint64_t* dst, uint64_t value);
 	void setBase(uint64_t base);
-	void enqueueSteps(uint64_t first, unsigned k, bool ext, bool fstim, hipEvent_t* ev = nullptr);
+	void enqueueSteps(uint64_t first, unsigned k, bool ext, bool fstim, hipEvent_t* ev = nullptr,
+			bool forkPending = true);
 	void launchDelivery(const nemo_propagate_args& set, uint64_t offset, hipStream_t st, bool withStdp,
 			hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 	void flushStdp();
@@ -441,7 +442,8 @@ private:
 	template<typename Out> void decodeFiringBatch(const uint64_t* bits, Out& out, std::vector<unsigned>& counts) const;
 	void readBitmaps(uint64_t t0, unsigned k, uint64_t* dst = nullptr, hipStream_t st = nullptr);
 	uint64_t sumCounters(const DeviceArray<unsigned long long>& a);
-	hipGraphExec_t graphFor(unsigned k);
+	/* the graph of k steps (forkPending: it runs the pending STDP batch first) */
+	hipGraphExec_t graphFor(unsigned k, bool forkPending = true);
 	void clearGraphs();
 	struct Snapshot;
 	void takeSnapshot(Snapshot& s);
