@@ -60,7 +60,8 @@ extern "C" {
 #define NEMO_STEP_RING 128
 /* Threads per block of an update launch that carries propagate blocks
  * (nemo_hip_colaunch): the propagate blocks use all of them, a tile's block
- * its first NEMO_TILE (the others end at once). */
+ * its first NEMO_TILE (one per neuron; the others end at once, or first help
+ * to gather the tile's input). */
 #define NEMO_COLAUNCH_BLOCK 512
 /* Most delivery blocks (bin regions) a step's input can come from. */
 #define NEMO_MAX_BIN_BLOCKS 1024

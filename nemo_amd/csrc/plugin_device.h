@@ -6,6 +6,9 @@
  *   UpdateLds& L = update_lds<CO>();            (CO: the launch carries propagate blocks)
  *   h = load_history(b, ng, valid);
  *   gather_input(b, g, L);                      (block barriers inside)
+ * or, in a launch of NEMO_COLAUNCH_BLOCK-thread blocks whose tile blocks keep
+ * their other waves (colaunched_block<true>), those waves help to gather:
+ *   if(!gather_input<NEMO_COLAUNCH_BLOCK>(b, g, L)) return;
  *   take_current(b, L.input, ng, E, I);         (valid threads)
  *   record_firing(b, L, ng, valid, fired, h);   (every thread; pushes the delay-1 events when asked)
  *   stdp_tile(b, L, ng, valid, h);              (every thread; barriers inside)
@@ -104,11 +107,12 @@ struct TileInput {
  * is found from buckets of 2^GATHER_SHIFT events, marked per column when the
  * tile has at most GATHER_MARKED events. */
 constexpr int GATHER_ILP = 8;
-constexpr unsigned GATHER_BATCH = GATHER_ILP * NEMO_TILE;
 constexpr unsigned GATHER_SHIFT = 3;
 constexpr unsigned GATHER_MARKED = 4096;
-constexpr unsigned GATHER_BUCKETS = (GATHER_MARKED > GATHER_BATCH ? GATHER_MARKED : GATHER_BATCH) / (1u << GATHER_SHIFT)
-	+ 2;
+/* events of the largest batch: GATHER_ILP per thread of a carrying block */
+constexpr unsigned GATHER_BUCKETS_MAX_EVENTS = GATHER_MARKED > GATHER_ILP * NEMO_COLAUNCH_BLOCK ? GATHER_MARKED
+	: GATHER_ILP * NEMO_COLAUNCH_BLOCK;
+constexpr unsigned GATHER_BUCKETS = GATHER_BUCKETS_MAX_EVENTS / (1u << GATHER_SHIFT) + 2;
 
 /* The LDS an update block uses: its tile's input sums, the gather's column
  * prefix, bases and buckets, and the per-tile STDP lists.  A kernel takes it
@@ -159,15 +163,24 @@ __device__ inline uint2 push_row(const nemo_hip_step_buffers& b, unsigned ng)
 
 /* prefetch() is called once the tile's table row and accumulators are
  * requested: the loads it issues (the model's state, say) queue behind them,
- * so that waiting for the table does not wait for them too. */
-template<typename Prefetch = NoPrefetch>
-__device__ inline void gather_input(const nemo_hip_step_buffers& b, const nemo_hip_neuron_group& g, UpdateLds& L,
+ * so that waiting for the table does not wait for them too.
+ *
+ * THREADS threads of the block take part: the tile's NEMO_TILE (one per
+ * neuron, which alone read the table row, the accumulators and, through
+ * prefetch, their neuron's state) and, in a block of NEMO_COLAUNCH_BLOCK
+ * threads, helpers that only load and sum events, so that a tile of up to
+ * GATHER_ILP * THREADS events takes one round of loads.  Returns false for
+ * the helpers, which take no further part in the step. */
+template<unsigned THREADS = NEMO_TILE, typename Prefetch = NoPrefetch>
+__device__ inline bool gather_input(const nemo_hip_step_buffers& b, const nemo_hip_neuron_group& g, UpdateLds& L,
 		Prefetch prefetch = Prefetch())
 {
+	static_assert(THREADS % 64 == 0 && THREADS >= NEMO_TILE, "whole waves beyond the tile");
 	TileInput& in = L.input;
 	uint32_t* s_pre = L.pre;
 	uint32_t* s_base = L.base;
 	const unsigned tid = threadIdx.x;
+	const bool lead = THREADS == NEMO_TILE || tid < NEMO_TILE; /* uniform per wave */
 	const unsigned tile = g.start / NEMO_TILE + blockIdx.x;
 	const size_t parity = current_step(b) & 1;
 	int64_t* acc = b.acc + parity * 2 * b.acc_pitch;
@@ -190,24 +203,27 @@ __device__ inline void gather_input(const nemo_hip_step_buffers& b, const nemo_h
 	constexpr unsigned PER = NEMO_MAX_BIN_BLOCKS / NEMO_TILE;
 	uint32_t ent[PER];
 	unsigned col[PER];
+	/* a helper's i is at least NEMO_TILE * PER = NEMO_MAX_BIN_BLOCKS >= B */
 #pragma unroll
 	for(unsigned k = 0; k < PER; ++k) {
 		const unsigned i = tid * PER + k;
 		col[k] = i < n0 ? first0 + i : first1 + (i - n0);
 		ent[k] = i < B ? row[col[k]] : 0u;
 	}
-	const int64_t oe = acc[ng];
-	const int64_t oi = acc[b.acc_pitch + ng];
-	const uint2 prow = push_row(b, ng);
-	prefetch();
-	in.E[tid] = oe;
-	in.I[tid] = oi;
-	L.push_rows[tid] = prow;
-	if(oe) acc[ng] = 0;
-	if(oi) acc[b.acc_pitch + ng] = 0;
+	if(lead) {
+		const int64_t oe = acc[ng];
+		const int64_t oi = acc[b.acc_pitch + ng];
+		const uint2 prow = push_row(b, ng);
+		prefetch();
+		in.E[tid] = oe;
+		in.I[tid] = oi;
+		L.push_rows[tid] = prow;
+		if(oe) acc[ng] = 0;
+		if(oi) acc[b.acc_pitch + ng] = 0;
+	}
 	if(B == 0) {
 		__syncthreads();
-		return;
+		return lead;
 	}
 	uint32_t mine = 0;
 #pragma unroll
@@ -221,12 +237,13 @@ __device__ inline void gather_input(const nemo_hip_step_buffers& b, const nemo_h
 	 * between the columns of the bucket bounds around it (every 8th event),
 	 * which each column marks when the tile has at most MARKED events, or
 	 * which are searched per batch; it is refined between them. */
-	constexpr unsigned BATCH = GATHER_BATCH;
+	constexpr unsigned BATCH = GATHER_ILP * THREADS;
 	constexpr unsigned SHIFT = GATHER_SHIFT;
 	constexpr unsigned MARKED = GATHER_MARKED;
+	static_assert(BATCH <= GATHER_BUCKETS_MAX_EVENTS, "the bucket array holds a batch");
 	uint32_t* s_bucket = L.bucket;
 	uint32_t total;
-	uint32_t pre = block_exclusive_scan(mine, &total);
+	uint32_t pre = block_exclusive_scan<THREADS>(mine, &total);
 	const bool marked = total <= MARKED;
 #pragma unroll
 	for(unsigned k = 0; k < PER; ++k) {
@@ -248,7 +265,7 @@ __device__ inline void gather_input(const nemo_hip_step_buffers& b, const nemo_h
 	for(uint32_t base = 0; base < total; base += BATCH) {
 		const uint32_t b0 = marked ? 0u : base; /* first event of the buckets */
 		if(!marked) {
-			for(unsigned j = tid; j <= (BATCH >> SHIFT); j += NEMO_TILE) {
+			for(unsigned j = tid; j <= (BATCH >> SHIFT); j += THREADS) {
 				s_bucket[j] = last_not_above<NEMO_MAX_BIN_BLOCKS>(s_pre, B, min(base + (j << SHIFT), total - 1));
 			}
 			__syncthreads();
@@ -259,7 +276,7 @@ __device__ inline void gather_input(const nemo_hip_step_buffers& b, const nemo_h
 		for(int u = 0; u < GATHER_ILP; ++u) {
 			/* buckets past the last event may be unset: only the events of
 			 * the tile read theirs */
-			jj[u] = base + u * NEMO_TILE + tid;
+			jj[u] = base + u * THREADS + tid;
 			const unsigned j = (jj[u] - b0) >> SHIFT;
 			lo[u] = jj[u] < total ? s_bucket[j] : 0u;
 			hi[u] = jj[u] < total ? s_bucket[j + 1] : 0u;
@@ -292,6 +309,7 @@ __device__ inline void gather_input(const nemo_hip_step_buffers& b, const nemo_h
 		if(!marked) __syncthreads(); /* the buckets are rewritten by the next batch */
 	}
 	__syncthreads();
+	return lead;
 }
 
 /* Saturate a 64-bit fixed-point sum to the int32 range and convert to float,

@@ -29,7 +29,7 @@ template<bool CO>
 __global__ void __launch_bounds__(CO ? NEMO_COLAUNCH_BLOCK : NEMO_UPDATE_BLOCK)
 izhikevich_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 {
-	if(CO && nemo_dev::colaunched_block(b, g)) return;
+	if(CO && nemo_dev::colaunched_block<true>(b, g)) return;
 	nemo_dev::UpdateLds& L = nemo_dev::update_lds<CO>();
 	const unsigned nl = blockIdx.x * NEMO_UPDATE_BLOCK + threadIdx.x;
 	const bool valid = nl < g.count;
@@ -41,7 +41,8 @@ izhikevich_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 	float sigma = 0.0f, a = 0.0f, bb = 0.0f, c = 0.0f, d = 0.0f, u = 0.0f, v = 0.0f;
 	uint4 s = make_uint4(0, 0, 0, 0);
 	uint64_t h = 0;
-	nemo_dev::gather_input(b, g, L, [&]() {
+	/* in a carrying launch the block's other waves help with the gather */
+	if(!nemo_dev::gather_input<CO ? NEMO_COLAUNCH_BLOCK : NEMO_TILE>(b, g, L, [&]() {
 		if(valid) {
 			sigma = g.param[PARAM_SIGMA * P + nl];
 			a = g.param[PARAM_A * P + nl];
@@ -53,7 +54,8 @@ izhikevich_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 			if(g.rng) s = reinterpret_cast<const uint4*>(g.rng)[nl];
 			h = nemo_dev::load_history(b, ng, true);
 		}
-	});
+	}))
+		return;
 	bool fired = false;
 
 	if(valid) {

@@ -20,7 +20,7 @@ template<bool CO>
 __global__ void __launch_bounds__(CO ? NEMO_COLAUNCH_BLOCK : NEMO_UPDATE_BLOCK)
 qif_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 {
-	if(CO && nemo_dev::colaunched_block(b, g)) return;
+	if(CO && nemo_dev::colaunched_block<true>(b, g)) return;
 	nemo_dev::UpdateLds& L = nemo_dev::update_lds<CO>();
 	const unsigned nl = blockIdx.x * NEMO_UPDATE_BLOCK + threadIdx.x;
 	const bool valid = nl < g.count;
@@ -28,11 +28,12 @@ qif_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 	/* state, parameter and history load while the input is gathered */
 	uint64_t h = 0;
 	float v = 0.0f, a = 0.0f;
-	nemo_dev::gather_input(b, g, L, [&]() {
+	if(!nemo_dev::gather_input<CO ? NEMO_COLAUNCH_BLOCK : NEMO_TILE>(b, g, L, [&]() {
 		h = nemo_dev::load_history(b, ng, valid);
 		v = valid ? g.state[nl] : 0.0f;
 		a = valid ? g.param[nl] : 0.0f;
-	});
+	}))
+		return; /* helpers end here */
 	bool fired = false;
 
 	if(valid) {

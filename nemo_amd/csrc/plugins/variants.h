@@ -181,9 +181,9 @@ template<bool CO>
 __global__ void __launch_bounds__(CO ? NEMO_COLAUNCH_BLOCK : NEMO_UPDATE_BLOCK)
 variant_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 {
-	if(CO && nemo_dev::colaunched_block(b, g)) return;
+	if(CO && nemo_dev::colaunched_block<true>(b, g)) return;
 	nemo_dev::UpdateLds& L = nemo_dev::update_lds<CO>();
-	nemo_dev::gather_input(b, g, L);
+	if(!nemo_dev::gather_input<CO ? NEMO_COLAUNCH_BLOCK : NEMO_TILE>(b, g, L)) return; /* helpers end here */
 	const unsigned nl = blockIdx.x * NEMO_UPDATE_BLOCK + threadIdx.x;
 	const bool valid = nl < g.count;
 	const unsigned ng = g.start + nl;

@@ -598,15 +598,18 @@ __device__ inline void stdp_step(const nemo_propagate_args& a, uint64_t t, unsig
 /* In an update launch of NEMO_COLAUNCH_BLOCK threads per block: run this
  * block as one of the propagate blocks the launch carries after the group's
  * tiles (nemo_hip_colaunch) and return true, or, in a block of the group's
- * own tiles, return false for its first NEMO_TILE threads (the tile's) and
- * true for the others, whose waves then end (a block barrier waits only for
- * the waves still running).  Every update kernel of such a launch calls it
- * first and returns when it returns true. */
+ * own tiles, return false for its first NEMO_TILE threads (the tile's) and,
+ * for the others, true (their waves then end; a block barrier waits only for
+ * the waves still running) or, with GATHER_HELPERS, false: the kernel then
+ * passes them to gather_input<NEMO_COLAUNCH_BLOCK>, which releases them.
+ * Every update kernel of such a launch calls it first and returns when it
+ * returns true. */
+template<bool GATHER_HELPERS = false>
 __device__ inline bool colaunched_block(const nemo_hip_step_buffers& b, const nemo_hip_neuron_group& g)
 {
 	static_assert(NEMO_COLAUNCH_BLOCK % 64 == 0 && NEMO_COLAUNCH_BLOCK >= NEMO_TILE, "whole waves beyond the tile");
 	const unsigned tiles = (g.count + NEMO_TILE - 1) / NEMO_TILE;
-	if(blockIdx.x < tiles) return threadIdx.x >= NEMO_TILE;
+	if(blockIdx.x < tiles) return !GATHER_HELPERS && threadIdx.x >= NEMO_TILE;
 	extern __shared__ uint64_t colaunch_lds[];
 	unsigned char* lds = reinterpret_cast<unsigned char*>(colaunch_lds);
 	const unsigned k = blockIdx.x - tiles;
