@@ -63,6 +63,11 @@ extern "C" {
  * its first NEMO_TILE (one per neuron; the others end at once, or first help
  * to gather the tile's input). */
 #define NEMO_COLAUNCH_BLOCK 512
+/* Waves per SIMD an update kernel compiled for such launches allows
+ * (__launch_bounds__(NEMO_COLAUNCH_BLOCK, NEMO_COLAUNCH_WAVES): at most 72
+ * VGPRs), so that a CU holds three of its blocks and, beside them, one wave
+ * per SIMD of an STDP batch running beside the steps. */
+#define NEMO_COLAUNCH_WAVES 7
 /* Most delivery blocks (bin regions) a step's input can come from. */
 #define NEMO_MAX_BIN_BLOCKS 1024
 /* Firing history: two planes of hist_pitch words, plane s % 2 after step s. */

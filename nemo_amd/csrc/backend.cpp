@@ -1217,7 +1217,7 @@ LaunchShape Simulation::defaultShape() const
 		if(lds <= m_ldsBudget && size_t(m_tiles) + c.deliverBlocks <= m_cuCount * perCu) s = c;
 	}
 	s.stdpMode = 4;
-	s.stdpWords = NEMO_STDP_MAX_WORDS; /* few, wide batch blocks disturb the steps least */
+	s.stdpWords = NEMO_STDP_BATCH_WORDS; /* about one batch block per CU disturbs the steps least */
 	auto env = [](const char* name, int lo, int hi, unsigned& field) {
 		if(const char* e = std::getenv(name)) field = unsigned(std::max(lo, std::min(std::atoi(e), hi)));
 	};
@@ -2502,7 +2502,7 @@ LaunchShape Simulation::autotune(unsigned stepsPerTrial)
 			c.stdpMode = m;
 			consider(c);
 		}
-		for(unsigned w : {1u, 2u, 4u, 8u, 16u, 32u, 64u}) {
+		for(unsigned w : {1u, 2u, 4u, 6u, 8u, 16u, 32u, 64u}) {
 			if(w > (best.stdpMode == 4 ? NEMO_STDP_BATCH_MAX_WORDS : NEMO_STDP_MAX_WORDS)) continue;
 			if(w == best.stdpWords) continue;
 			LaunchShape c = best;

@@ -30,6 +30,15 @@ extern "C" {
  * per thread, or, in STDP batches, groups of that many in turn. */
 #define NEMO_STDP_MAX_WORDS (NEMO_PROPAGATE_BLOCK / 64)
 #define NEMO_STDP_BATCH_MAX_WORDS 64
+/* Threads per STDP batch block, and the waves per SIMD its registers allow
+ * (80 VGPRs): one wave per SIMD, which fits beside three blocks of an update
+ * launch that carries delivery (NEMO_COLAUNCH_WAVES), so that a batch running
+ * beside the steps does not take their blocks' places. */
+#define NEMO_STDP_BATCH_BLOCK 256
+#define NEMO_STDP_BATCH_WAVES 6
+/* Default bitmap words per batch block: 261 blocks for 100k neurons, about
+ * one per CU (8: 22.8 us per step beside the batch; 6: 22.5; 4: 23.2) */
+#define NEMO_STDP_BATCH_WORDS 6
 
 
 /* dynamic LDS bytes a delivery block needs */

@@ -84,11 +84,11 @@ stdp_kernel(nemo_propagate_args a)
 
 /* The pending STDP accumulation, one block per group of stdp_words bitmap
  * words (stdp_batch_block) */
-__global__ void __launch_bounds__(PBLOCK)
+__global__ void __launch_bounds__(NEMO_STDP_BATCH_BLOCK, NEMO_STDP_BATCH_WAVES)
 stdp_batch_kernel(nemo_propagate_args a)
 {
 	extern __shared__ uint64_t lds_dyn[];
-	nemo_dev::stdp_batch_block<PBLOCK>(a, blockIdx.x, reinterpret_cast<unsigned char*>(lds_dyn));
+	nemo_dev::stdp_batch_block<NEMO_STDP_BATCH_BLOCK>(a, blockIdx.x, reinterpret_cast<unsigned char*>(lds_dyn));
 }
 
 /* A history snapshot after step base + offset into the free slot (the one
@@ -295,8 +295,8 @@ hipError_t nemo_launch_stdp_batch(hipStream_t st, const nemo_propagate_args* a, 
 			|| size_t(stdp_blocks) * a->stdp_words < a->fired_words || !a->stdp.rcm_ptr) {
 		return hipErrorInvalidValue;
 	}
-	hipLaunchKernelGGL(stdp_batch_kernel, dim3(stdp_blocks), dim3(PBLOCK),
-			nemo_dev::PropagateLds<PBLOCK>::stdp_bytes(), st, *a);
+	hipLaunchKernelGGL(stdp_batch_kernel, dim3(stdp_blocks), dim3(NEMO_STDP_BATCH_BLOCK),
+			nemo_dev::PropagateLds<NEMO_STDP_BATCH_BLOCK>::stdp_bytes(), st, *a);
 	return hipGetLastError();
 }
 
@@ -316,7 +316,8 @@ hipError_t nemo_launch_stdp_snapshot(hipStream_t st, const nemo_propagate_args* 
 size_t nemo_colaunch_lds_bytes(unsigned max_window, unsigned bin_capacity, int delivery)
 {
 	/* the update blocks of the launch take their LDS from the same buffer */
-	return std::max(nemo_dev::PropagateLds<NEMO_COLAUNCH_BLOCK>::bytes(delivery != 0, bin_capacity, max_window),
+	return std::max(nemo_dev::PropagateLds<NEMO_COLAUNCH_BLOCK, nemo_dev::CARRIED_EVENT_ILP>::bytes(delivery != 0,
+					bin_capacity, max_window),
 			sizeof(nemo_dev::UpdateLds));
 }
 

@@ -23,6 +23,10 @@
 #include "glibc_math.h"
 
 #define NEMO_UPDATE_BLOCK NEMO_TILE
+/* Launch bounds of an update kernel, compiled with CO true for launches that
+ * carry propagate blocks and false otherwise. */
+#define NEMO_UPDATE_BOUNDS(CO) \
+	__launch_bounds__((CO) ? NEMO_COLAUNCH_BLOCK : NEMO_UPDATE_BLOCK, (CO) ? NEMO_COLAUNCH_WAVES : 1)
 
 namespace nemo_dev {
 

@@ -71,7 +71,7 @@ __device__ inline void hh_substep(float& v, float& n, float& m, float& h, float&
 /* CO: the launch may carry propagate blocks after the tiles (compiled
  * out otherwise, which leaves the update far fewer registers) */
 template<bool CO>
-__global__ void __launch_bounds__(CO ? NEMO_COLAUNCH_BLOCK : NEMO_UPDATE_BLOCK)
+__global__ void NEMO_UPDATE_BOUNDS(CO)
 hh_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 {
 	if(CO && nemo_dev::colaunched_block<true>(b, g)) return;
@@ -120,7 +120,7 @@ hh_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 /* CO: the launch may carry propagate blocks after the tiles (compiled
  * out otherwise, which leaves the update far fewer registers) */
 template<bool CO>
-__global__ void __launch_bounds__(CO ? NEMO_COLAUNCH_BLOCK : NEMO_UPDATE_BLOCK)
+__global__ void NEMO_UPDATE_BOUNDS(CO)
 hh_input(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 {
 	if(CO && nemo_dev::colaunched_block<true>(b, g)) return;

@@ -25,7 +25,7 @@ enum { S_V = 0, S_IE = 1, S_II = 2, S_LASTFIRED = 3 };
 /* CO: the launch may carry propagate blocks after the tiles (compiled
  * out otherwise, which leaves the update far fewer registers) */
 template<bool CO>
-__global__ void __launch_bounds__(CO ? NEMO_COLAUNCH_BLOCK : NEMO_UPDATE_BLOCK)
+__global__ void NEMO_UPDATE_BOUNDS(CO)
 if_curr_exp_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 {
 	if(CO && nemo_dev::colaunched_block<true>(b, g)) return;

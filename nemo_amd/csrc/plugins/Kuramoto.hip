@@ -102,7 +102,7 @@ __device__ inline uint32_t cycle_of(const nemo_hip_step_buffers& b)
 /* One launch per step: each thread one oscillator of its tile, then the
  * tile's (empty) firing record. */
 template<bool CO>
-__global__ void __launch_bounds__(CO ? NEMO_COLAUNCH_BLOCK : NEMO_UPDATE_BLOCK)
+__global__ void NEMO_UPDATE_BOUNDS(CO)
 kuramoto_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 {
 	if(CO && nemo_dev::colaunched_block(b, g)) return;
@@ -146,7 +146,7 @@ kuramoto_pass(nemo_hip_neuron_group g, nemo_hip_step_buffers b, uint32_t first, 
 }
 
 template<bool CO>
-__global__ void __launch_bounds__(CO ? NEMO_COLAUNCH_BLOCK : NEMO_UPDATE_BLOCK)
+__global__ void NEMO_UPDATE_BOUNDS(CO)
 kuramoto_record(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 {
 	if(CO && nemo_dev::colaunched_block(b, g)) return;

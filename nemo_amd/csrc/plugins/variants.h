@@ -178,7 +178,7 @@ __device__ inline bool variant_step(const nemo_hip_neuron_group& g, const nemo_h
 }
 
 template<bool CO>
-__global__ void __launch_bounds__(CO ? NEMO_COLAUNCH_BLOCK : NEMO_UPDATE_BLOCK)
+__global__ void NEMO_UPDATE_BOUNDS(CO)
 variant_update(nemo_hip_neuron_group g, nemo_hip_step_buffers b)
 {
 	if(CO && nemo_dev::colaunched_block<true>(b, g)) return;

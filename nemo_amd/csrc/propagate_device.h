@@ -27,6 +27,16 @@ constexpr int ILP = 8;
  * delivery blocks at once instead of two, which pays when there are more
  * blocks than CUs (batches) and costs nothing with one block per CU */
 constexpr int EVENT_ILP = 8;
+/* The delivery blocks an update launch carries (overlap 3) keep 6: that
+ * launch is held to 72 VGPRs (NEMO_COLAUNCH_WAVES), so that beside three of
+ * its blocks a CU's SIMDs still hold one wave each of an STDP batch block
+ * (NEMO_STDP_BATCH_BLOCK threads, 80 VGPRs); with 8 the launch needs 78, and
+ * a batch block beside it leaves a CU room for one of its blocks instead of
+ * three, so that the blocks of the steps beside a batch wait for each other */
+constexpr int CARRIED_EVENT_ILP = 6;
+/* independent global loads per thread in the synapse loop of the STDP batch,
+ * held to 80 VGPRs (STDP_BATCH_WAVES waves per SIMD) */
+constexpr int STDP_BATCH_ILP = 5;
 /* staged events per thread the delivery sort keeps in registers */
 constexpr unsigned SORT_REGS = 8;
 
@@ -35,7 +45,7 @@ constexpr unsigned SORT_REGS = 8;
  * bitmap words of a batch with their prefix, and the fired rows of a chunk
  * with their event prefix.  An STDP block uses the start of the same buffer
  * for its weight-change table, row bases and prefix. */
-template<unsigned THREADS>
+template<unsigned THREADS, int EV_ILP = EVENT_ILP>
 struct PropagateLds {
 	/* fired rows a delivery block lists at a time */
 	static constexpr unsigned ROWS = THREADS < 512 ? 512 : THREADS;
@@ -50,7 +60,7 @@ struct PropagateLds {
 	/* events of a batch of the event loop, events per row bucket, and the
 	 * most events of a chunk whose buckets the rows mark (buckets hold the
 	 * larger of a batch and that) */
-	static constexpr unsigned BATCH = EVENT_ILP * THREADS;
+	static constexpr unsigned BATCH = EV_ILP * THREADS;
 	static constexpr unsigned BUCKET_SHIFT = 5;
 	static constexpr unsigned MARKED = 16384;
 	static constexpr unsigned BUCKETS = (BATCH > MARKED ? BATCH : MARKED) >> BUCKET_SHIFT;
@@ -91,12 +101,13 @@ __device__ inline unsigned kth_set_bit(uint64_t w, unsigned k)
  * over the block (four loads in flight per thread), staging them in LDS;
  * finally counting-sort the staged events by target tile (relative to the
  * block's window) into the block's bins. */
-template<unsigned THREADS>
+template<unsigned THREADS, int EV_ILP = EVENT_ILP>
 __device__ inline void deliver_block(const nemo_propagate_args& a, uint64_t t, unsigned b, unsigned char* lds)
 {
-	constexpr unsigned ROWS = PropagateLds<THREADS>::ROWS;
+	using Lds = PropagateLds<THREADS, EV_ILP>;
+	constexpr unsigned ROWS = Lds::ROWS;
 	constexpr unsigned ROWS_PER_THREAD = ROWS / THREADS;
-	const PropagateLds<THREADS> L(lds, a.bin_capacity, a.max_window);
+	const Lds L(lds, a.bin_capacity, a.max_window);
 	int2* stage = L.stage;
 	uint32_t* cnt = L.cnt;
 	uint32_t* cur = L.cur;
@@ -204,13 +215,13 @@ __device__ inline void deliver_block(const nemo_propagate_args& a, uint64_t t, u
 				}
 			};
 			/* the events of these rows evenly over the block, in batches of
-			 * EVENT_ILP per thread.  An event's row lies between the rows of
+			 * EV_ILP per thread.  An event's row lies between the rows of
 			 * the bucket bounds around it (every 32nd event), which each row
 			 * marks when the chunk has at most MARKED events, or which are
 			 * searched per batch; it is refined between them. */
-			constexpr unsigned BATCH = PropagateLds<THREADS>::BATCH;
-			constexpr unsigned SHIFT = PropagateLds<THREADS>::BUCKET_SHIFT;
-			const bool marked = M <= PropagateLds<THREADS>::MARKED;
+			constexpr unsigned BATCH = Lds::BATCH;
+			constexpr unsigned SHIFT = Lds::BUCKET_SHIFT;
+			const bool marked = M <= Lds::MARKED;
 			if(marked) {
 #pragma unroll
 				for(unsigned k = 0; k < ROWS_PER_THREAD; ++k) {
@@ -231,10 +242,10 @@ __device__ inline void deliver_block(const nemo_propagate_args& a, uint64_t t, u
 					}
 					__syncthreads();
 				}
-				uint32_t ee[EVENT_ILP];
-				unsigned lo[EVENT_ILP], hi[EVENT_ILP];
+				uint32_t ee[EV_ILP];
+				unsigned lo[EV_ILP], hi[EV_ILP];
 #pragma unroll
-				for(int u = 0; u < EVENT_ILP; ++u) {
+				for(int u = 0; u < EV_ILP; ++u) {
 					/* buckets past the last event may be unset: only the
 					 * events of the chunk read theirs */
 					ee[u] = base + u * THREADS + tid;
@@ -246,7 +257,7 @@ __device__ inline void deliver_block(const nemo_propagate_args& a, uint64_t t, u
 				while(open) {
 					open = false;
 #pragma unroll
-					for(int u = 0; u < EVENT_ILP; ++u) {
+					for(int u = 0; u < EV_ILP; ++u) {
 						if(lo[u] < hi[u]) {
 							const unsigned mid = lo[u] + ((hi[u] - lo[u] + 1) >> 1);
 							if(s_rpre[mid] <= ee[u]) lo[u] = mid; else hi[u] = mid - 1;
@@ -254,14 +265,14 @@ __device__ inline void deliver_block(const nemo_propagate_args& a, uint64_t t, u
 						}
 					}
 				}
-				int2 ev[EVENT_ILP];
+				int2 ev[EV_ILP];
 #pragma unroll
-				for(int u = 0; u < EVENT_ILP; ++u) {
+				for(int u = 0; u < EV_ILP; ++u) {
 					ev[u] = make_int2(0, 0);
 					if(ee[u] < M) ev[u] = a.fcm[s_rb[lo[u]] + (ee[u] - s_rpre[lo[u]])];
 				}
 #pragma unroll
-				for(int u = 0; u < EVENT_ILP; ++u) {
+				for(int u = 0; u < EV_ILP; ++u) {
 					if(ee[u] < M) stage_event(ev[u], ee[u]);
 				}
 				if(!marked) __syncthreads(); /* the buckets are rewritten by the next batch */
@@ -527,17 +538,17 @@ __device__ inline void stdp_batch_block(const nemo_propagate_args& a, unsigned s
 		__syncthreads();
 		if(total == 0) continue;
 
-		for(uint32_t e0 = tid; e0 < total; e0 += ILP * THREADS) {
-			uint32_t k[ILP];
-			uint32_t sd[ILP];
-			int32_t acc[ILP];
-			uint64_t pm[ILP];
+		for(uint32_t e0 = tid; e0 < total; e0 += STDP_BATCH_ILP * THREADS) {
+			uint32_t k[STDP_BATCH_ILP];
+			uint32_t sd[STDP_BATCH_ILP];
+			int32_t acc[STDP_BATCH_ILP];
+			uint64_t pm[STDP_BATCH_ILP];
 	#pragma unroll
-			for(int u = 0; u < ILP; ++u) k[u] = e0 + u * THREADS;
-			unsigned lo[ILP];
+			for(int u = 0; u < STDP_BATCH_ILP; ++u) k[u] = e0 + u * THREADS;
+			unsigned lo[STDP_BATCH_ILP];
 			last_not_above_k<THREADS>(s_pos, nfired, k, lo);
 	#pragma unroll
-			for(int u = 0; u < ILP; ++u) {
+			for(int u = 0; u < STDP_BATCH_ILP; ++u) {
 				const uint32_t e = k[u];
 				k[u] = 0;
 				sd[u] = 0;
@@ -550,9 +561,9 @@ __device__ inline void stdp_batch_block(const nemo_propagate_args& a, unsigned s
 					acc[u] = st.rcm_acc[k[u]];
 				}
 			}
-			uint64_t hc[ILP], hs[ILP];
+			uint64_t hc[STDP_BATCH_ILP], hs[STDP_BATCH_ILP];
 	#pragma unroll
-			for(int u = 0; u < ILP; ++u) {
+			for(int u = 0; u < STDP_BATCH_ILP; ++u) {
 				hc[u] = 0;
 				hs[u] = 0;
 				if(e0 + u * THREADS < total) {
@@ -560,7 +571,7 @@ __device__ inline void stdp_batch_block(const nemo_propagate_args& a, unsigned s
 				}
 			}
 	#pragma unroll
-			for(int u = 0; u < ILP; ++u) {
+			for(int u = 0; u < STDP_BATCH_ILP; ++u) {
 				if(e0 + u * THREADS < total) {
 					const unsigned d = (sd[u] >> 26) + 1;
 					int32_t dw = 0;
@@ -616,7 +627,7 @@ __device__ inline bool colaunched_block(const nemo_hip_step_buffers& b, const ne
 	const nemo_propagate_args& a = b.co.args;
 	const uint64_t t = a.clock->base_step + a.step_offset;
 	if(k < b.co.deliver_blocks) {
-		deliver_block<NEMO_COLAUNCH_BLOCK>(a, t, k, lds);
+		deliver_block<NEMO_COLAUNCH_BLOCK, CARRIED_EVENT_ILP>(a, t, k, lds);
 	} else {
 		stdp_step<NEMO_COLAUNCH_BLOCK>(a, t - 1, k - b.co.deliver_blocks, lds);
 	}
