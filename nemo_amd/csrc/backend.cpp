@@ -1571,6 +1571,7 @@ void Simulation::waitStream(hipStream_t s)
 void Simulation::setBase(uint64_t base)
 {
 	copyScalar(&m_clock.ptr->base_step, base);
+	m_base = base;
 }
 
 void Simulation::launchDelivery(const nemo_propagate_args& set, uint64_t offset, hipStream_t st, bool withStdp,
@@ -1803,13 +1804,12 @@ bool Simulation::enqueueFlush()
 	if(m_shape.stdpMode == 4) {
 		/* the pending batch: every run of steps ends with a snapshot.  Here
 		 * nothing runs beside it, so it takes narrower blocks (more of them)
-		 * than the batches beside the steps */
+		 * than the batches beside the steps; a kernel after it marks it done
+		 * (the flushed step relative to the device's base, modulo 2^64) */
 		nemo_propagate_args a = m_propagate;
 		a.stdp_words = std::min(a.stdp_words, 4u);
 		const unsigned blocks = unsigned((m_firedWords + a.stdp_words - 1) / a.stdp_words);
-		HIP_CHECK(nemo_launch_stdp_batch(m_stream, &a, blocks));
-		copyScalar(&m_clock.ptr->stdp_count, 0);
-		copyScalar(&m_clock.ptr->stdp_flushed, m_hostStep - 1);
+		HIP_CHECK(nemo_launch_stdp_flush(m_stream, &a, blocks, m_hostStep - 1 - m_base));
 		m_flushed = m_hostStep - 1;
 		return true;
 	}
@@ -2325,6 +2325,7 @@ struct Simulation::Snapshot {
 	std::vector<DeviceArray<unsigned char> > copies;
 	uint64_t hostStep = 0;
 	uint64_t flushed = 0;
+	uint64_t base = 0; /* the host's copy of the clock's base_step */
 	uint64_t lastCycle = 0;
 	bool updated = false;
 };
@@ -2360,6 +2361,7 @@ void Simulation::takeSnapshot(Snapshot& s)
 	HIP_CHECK(hipStreamSynchronize(m_stream));
 	s.hostStep = m_hostStep;
 	s.flushed = m_flushed;
+	s.base = m_base;
 	s.lastCycle = m_lastCycle;
 	s.updated = m_updated;
 }
@@ -2375,6 +2377,7 @@ void Simulation::restoreSnapshot(const Snapshot& s)
 	HIP_CHECK(hipStreamSynchronize(m_stream));
 	m_hostStep = s.hostStep;
 	m_flushed = s.flushed;
+	m_base = s.base;
 	m_lastCycle = s.lastCycle;
 	m_updated = s.updated;
 }

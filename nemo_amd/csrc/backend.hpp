@@ -362,6 +362,7 @@ private:
 	bool m_stdp = false;
 	int32_t m_minE = 0, m_maxE = 0, m_minI = 0, m_maxI = 0;
 	uint64_t m_flushed = ~uint64_t(0); /* step whose STDP accumulation ran on its own */
+	uint64_t m_base = 0;               /* the step base last set on the device (setBase) */
 
 	/* propagate-kernel arguments and launch shape */
 	nemo_propagate_args m_propagate;     /* common delivery arguments */

@@ -47,6 +47,10 @@ size_t nemo_propagate_lds_bytes(unsigned max_window, unsigned bin_capacity);
  * of stdp_blocks blocks, from the two history snapshots.  It reads neither
  * the live history nor the bitmaps, so it may run beside later steps. */
 hipError_t nemo_launch_stdp_batch(hipStream_t, const nemo_propagate_args*, unsigned stdp_blocks);
+/* The same as a flush: afterwards nothing is pending and clock->stdp_flushed
+ * is step base + offset (device-side, so that a captured graph can end with
+ * it whatever its base) */
+hipError_t nemo_launch_stdp_flush(hipStream_t, const nemo_propagate_args*, unsigned stdp_blocks, uint64_t offset);
 /* A snapshot of the history after step base + offset (at most 64 steps after
  * the last one); the steps since the last snapshot become the pending
  * accumulation.  The pending batch must have run. */

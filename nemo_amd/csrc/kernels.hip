@@ -114,6 +114,14 @@ __global__ void stdp_snapshot_done_kernel(nemo_hip_clock* clock, uint64_t offset
 	clock->stdp_snaps += 1;
 }
 
+/* ... and after a flush of the pending batch: nothing is pending, and the
+ * accumulation of step base + offset has run */
+__global__ void stdp_flush_done_kernel(nemo_hip_clock* clock, uint64_t offset)
+{
+	clock->stdp_count = 0;
+	clock->stdp_flushed = clock->base_step + offset;
+}
+
 /* applyStdp in forward order: every synapse of the FCM that is plastic
  * (rcm_of[j] its accumulator) takes its weight change; reading the weights in
  * order and the accumulators scattered costs one random read per synapse,
@@ -297,6 +305,14 @@ hipError_t nemo_launch_stdp_batch(hipStream_t st, const nemo_propagate_args* a, 
 	}
 	hipLaunchKernelGGL(stdp_batch_kernel, dim3(stdp_blocks), dim3(NEMO_STDP_BATCH_BLOCK),
 			nemo_dev::PropagateLds<NEMO_STDP_BATCH_BLOCK>::stdp_bytes(), st, *a);
+	return hipGetLastError();
+}
+
+hipError_t nemo_launch_stdp_flush(hipStream_t st, const nemo_propagate_args* a, unsigned stdp_blocks, uint64_t offset)
+{
+	hipError_t e = nemo_launch_stdp_batch(st, a, stdp_blocks);
+	if(e != hipSuccess) return e;
+	hipLaunchKernelGGL(stdp_flush_done_kernel, dim3(1), dim3(1), 0, st, const_cast<nemo_hip_clock*>(a->clock), offset);
 	return hipGetLastError();
 }
 
