@@ -510,6 +510,10 @@ void Simulation::construct(const std::vector<const Network*>& nets, const Config
 	HIP_CHECK(hipEventCreateWithFlags(&m_earlyFork, hipEventDisableTiming));
 	HIP_CHECK(hipEventCreateWithFlags(&m_earlyDone[0], hipEventDisableTiming));
 	HIP_CHECK(hipEventCreateWithFlags(&m_earlyDone[1], hipEventDisableTiming));
+	for(int i = 0; i < 2; ++i) {
+		HIP_CHECK(hipEventCreateWithFlags(&m_readoutReady[i], hipEventDisableTiming));
+		HIP_CHECK(hipEventCreateWithFlags(&m_readoutCopied[i], hipEventDisableTiming));
+	}
 	HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&m_pinnedScalars), SCALAR_SLOTS * sizeof(uint64_t),
 				hipHostMallocDefault));
 
@@ -575,6 +579,9 @@ Simulation::~Simulation()
 		if(e) (void) hipEventDestroy(e);
 	}
 	if(m_groupFork) (void) hipEventDestroy(m_groupFork);
+	for(hipEvent_t e : {m_readoutReady[0], m_readoutReady[1], m_readoutCopied[0], m_readoutCopied[1]}) {
+		if(e) (void) hipEventDestroy(e);
+	}
 	if(m_earlyStream) {
 		(void) hipStreamDestroy(m_earlyStream);
 		(void) hipEventDestroy(m_earlyFork);
@@ -2122,18 +2129,9 @@ void Simulation::run(unsigned nsteps, unsigned stdpPeriod, float reward, bool re
 	 * its half's device list) are intact until chunk i+2, which waits for
 	 * that copy (the copy of chunk i is the last use of its half's event
 	 * before chunk i+2 records it again). */
-	hipEvent_t copied[2], ready[2];
+	hipEvent_t* copied = m_readoutCopied;
+	hipEvent_t* ready = m_readoutReady;
 	bool inFlight[2] = {false, false};
-	HIP_CHECK(hipEventCreateWithFlags(&copied[0], hipEventDisableTiming));
-	HIP_CHECK(hipEventCreateWithFlags(&copied[1], hipEventDisableTiming));
-	HIP_CHECK(hipEventCreateWithFlags(&ready[0], hipEventDisableTiming));
-	HIP_CHECK(hipEventCreateWithFlags(&ready[1], hipEventDisableTiming));
-	auto destroyEvents = [&]() {
-		for(int i = 0; i < 2; ++i) {
-			(void) hipEventDestroy(copied[i]);
-			(void) hipEventDestroy(ready[i]);
-		}
-	};
 	unsigned pendingSteps = 0; /* steps of the chunk waiting to be decoded */
 	int pendingHalf = -1;
 	const bool devList = readRaster && ensureListReadout();
@@ -2230,11 +2228,9 @@ void Simulation::run(unsigned nsteps, unsigned stdpPeriod, float reward, bool re
 	} catch(...) {
 		(void) hipStreamSynchronize(m_stream);
 		(void) hipStreamSynchronize(m_copyStream);
-		destroyEvents();
 		throw;
 	}
 	waitStream(m_copyStream);
-	destroyEvents();
 	waitStream(m_stream);
 	if(readRaster && fired && nsteps) m_firedPerStep = double(fired->size() - firedBefore) / nsteps;
 	if(events) {

@@ -297,6 +297,10 @@ private:
 	bool m_decodeInOrder = false;         /* batch whose device order lists each network in turn, in order */
 	bool m_binRanged = false;             /* updates read only their tiles' column ranges */
 	hipEvent_t m_earlyFork = nullptr, m_earlyDone[2] = {nullptr, nullptr};
+	/* run(): a chunk's steps are done (ready) and its firing is copied out
+	 * (copied), per half of the readout buffers; created once, as creating
+	 * them per run delays a short run's first work */
+	hipEvent_t m_readoutReady[2] = {nullptr, nullptr}, m_readoutCopied[2] = {nullptr, nullptr};
 	bool m_stdpPending = false;          /* the STDP branch has not been joined yet */
 	unsigned m_nets = 1;   /* networks in the batch */
 	size_t m_ncount = 0;   /* neurons */
