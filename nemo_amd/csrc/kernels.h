@@ -37,7 +37,8 @@ extern "C" {
 #define NEMO_STDP_BATCH_BLOCK 256
 #define NEMO_STDP_BATCH_WAVES 6
 /* Default bitmap words per batch block: 261 blocks for 100k neurons, about
- * one per CU (8: 22.8 us per step beside the batch; 6: 22.5; 4: 23.2) */
+ * one per CU (100k x 1000 with STDP, us per step: 8 words 22.8, 6 words
+ * 22.5, 4 words 23.2) */
 #define NEMO_STDP_BATCH_WORDS 6
 
 
@@ -48,8 +49,8 @@ size_t nemo_propagate_lds_bytes(unsigned max_window, unsigned bin_capacity);
  * the live history nor the bitmaps, so it may run beside later steps. */
 hipError_t nemo_launch_stdp_batch(hipStream_t, const nemo_propagate_args*, unsigned stdp_blocks);
 /* The same as a flush: afterwards nothing is pending and clock->stdp_flushed
- * is step base + offset (device-side, so that a captured graph can end with
- * it whatever its base) */
+ * is step base + offset (set by a kernel after the batch, so that no copy
+ * from the host follows it) */
 hipError_t nemo_launch_stdp_flush(hipStream_t, const nemo_propagate_args*, unsigned stdp_blocks, uint64_t offset);
 /* A snapshot of the history after step base + offset (at most 64 steps after
  * the last one); the steps since the last snapshot become the pending
