@@ -1810,9 +1810,10 @@ bool Simulation::enqueueFlush()
 	if(!m_stdpBlocks || m_hostStep == 0 || m_flushed == m_hostStep - 1) return false;
 	if(m_shape.stdpMode == 4) {
 		/* the pending batch: every run of steps ends with a snapshot.  Here
-		 * nothing runs beside it, so it takes narrower blocks (more of them)
-		 * than the batches beside the steps; a kernel after it marks it done
-		 * (the flushed step relative to the device's base, modulo 2^64) */
+		 * nothing runs beside it, so it takes its own kernel (512-thread
+		 * blocks, 8 loads in flight per thread, 4 words per block); a kernel
+		 * after it marks it done (the flushed step relative to the device's
+		 * base, modulo 2^64) */
 		nemo_propagate_args a = m_propagate;
 		a.stdp_words = std::min(a.stdp_words, 4u);
 		const unsigned blocks = unsigned((m_firedWords + a.stdp_words - 1) / a.stdp_words);

@@ -91,6 +91,16 @@ stdp_batch_kernel(nemo_propagate_args a)
 	nemo_dev::stdp_batch_block<NEMO_STDP_BATCH_BLOCK>(a, blockIdx.x, reinterpret_cast<unsigned char*>(lds_dyn));
 }
 
+/* The same as a flush, which nothing runs beside: wider blocks with more
+ * loads in flight per thread */
+__global__ void __launch_bounds__(PBLOCK)
+stdp_flush_kernel(nemo_propagate_args a)
+{
+	extern __shared__ uint64_t lds_dyn[];
+	nemo_dev::stdp_batch_block<PBLOCK, nemo_dev::STDP_FLUSH_ILP>(a, blockIdx.x,
+			reinterpret_cast<unsigned char*>(lds_dyn));
+}
+
 /* A history snapshot after step base + offset into the free slot (the one
  * the last snapshot is not in) */
 __global__ void __launch_bounds__(NEMO_BLOCK)
@@ -296,13 +306,16 @@ size_t nemo_propagate_lds_bytes(unsigned max_window, unsigned bin_capacity)
 	return nemo_dev::PropagateLds<PBLOCK>::delivery_bytes(bin_capacity, max_window);
 }
 
+static bool stdp_batch_args_valid(const nemo_propagate_args* a, unsigned stdp_blocks)
+{
+	return a->stdp_snap && a->stdp.post_window <= 63 && a->stdp_words > 0 && a->stdp_words <= NEMO_STDP_BATCH_MAX_WORDS
+		&& size_t(stdp_blocks) * a->stdp_words >= a->fired_words && a->stdp.rcm_ptr;
+}
+
 hipError_t nemo_launch_stdp_batch(hipStream_t st, const nemo_propagate_args* a, unsigned stdp_blocks)
 {
 	if(stdp_blocks == 0) return hipSuccess;
-	if(!a->stdp_snap || a->stdp.post_window > 63 || a->stdp_words == 0 || a->stdp_words > NEMO_STDP_BATCH_MAX_WORDS
-			|| size_t(stdp_blocks) * a->stdp_words < a->fired_words || !a->stdp.rcm_ptr) {
-		return hipErrorInvalidValue;
-	}
+	if(!stdp_batch_args_valid(a, stdp_blocks)) return hipErrorInvalidValue;
 	hipLaunchKernelGGL(stdp_batch_kernel, dim3(stdp_blocks), dim3(NEMO_STDP_BATCH_BLOCK),
 			nemo_dev::PropagateLds<NEMO_STDP_BATCH_BLOCK>::stdp_bytes(), st, *a);
 	return hipGetLastError();
@@ -310,7 +323,11 @@ hipError_t nemo_launch_stdp_batch(hipStream_t st, const nemo_propagate_args* a, 
 
 hipError_t nemo_launch_stdp_flush(hipStream_t st, const nemo_propagate_args* a, unsigned stdp_blocks, uint64_t offset)
 {
-	hipError_t e = nemo_launch_stdp_batch(st, a, stdp_blocks);
+	if(stdp_blocks == 0) return hipSuccess;
+	if(!stdp_batch_args_valid(a, stdp_blocks)) return hipErrorInvalidValue;
+	hipLaunchKernelGGL(stdp_flush_kernel, dim3(stdp_blocks), dim3(PBLOCK), nemo_dev::PropagateLds<PBLOCK>::stdp_bytes(),
+			st, *a);
+	hipError_t e = hipGetLastError();
 	if(e != hipSuccess) return e;
 	hipLaunchKernelGGL(stdp_flush_done_kernel, dim3(1), dim3(1), 0, st, const_cast<nemo_hip_clock*>(a->clock), offset);
 	return hipGetLastError();

@@ -34,9 +34,11 @@ constexpr int EVENT_ILP = 8;
  * a batch block beside it leaves a CU room for one of its blocks instead of
  * three, so that the blocks of the steps beside a batch wait for each other */
 constexpr int CARRIED_EVENT_ILP = 6;
-/* independent global loads per thread in the synapse loop of the STDP batch,
- * held to 80 VGPRs (STDP_BATCH_WAVES waves per SIMD) */
+/* independent global loads per thread in the synapse loop of the STDP batch
+ * beside the steps, held to 80 VGPRs (NEMO_STDP_BATCH_WAVES waves per SIMD),
+ * and of a flush's batch, which nothing runs beside */
 constexpr int STDP_BATCH_ILP = 5;
+constexpr int STDP_FLUSH_ILP = 8;
 /* staged events per thread the delivery sort keeps in registers */
 constexpr unsigned SORT_REGS = 8;
 
@@ -466,7 +468,7 @@ __device__ inline uint64_t history_window(uint64_t c, uint64_t s, unsigned n, un
  * does not depend on their order, so the result equals n single-step
  * accumulations.  Nothing here reads the live history, so the batch may run
  * beside the steps that follow. */
-template<unsigned THREADS>
+template<unsigned THREADS, int SYN_ILP = STDP_BATCH_ILP>
 __device__ inline void stdp_batch_block(const nemo_propagate_args& a, unsigned sb, unsigned char* lds)
 {
 	const uint64_t r0 = a.clock->stdp_first;
@@ -538,17 +540,17 @@ __device__ inline void stdp_batch_block(const nemo_propagate_args& a, unsigned s
 		__syncthreads();
 		if(total == 0) continue;
 
-		for(uint32_t e0 = tid; e0 < total; e0 += STDP_BATCH_ILP * THREADS) {
-			uint32_t k[STDP_BATCH_ILP];
-			uint32_t sd[STDP_BATCH_ILP];
-			int32_t acc[STDP_BATCH_ILP];
-			uint64_t pm[STDP_BATCH_ILP];
+		for(uint32_t e0 = tid; e0 < total; e0 += SYN_ILP * THREADS) {
+			uint32_t k[SYN_ILP];
+			uint32_t sd[SYN_ILP];
+			int32_t acc[SYN_ILP];
+			uint64_t pm[SYN_ILP];
 	#pragma unroll
-			for(int u = 0; u < STDP_BATCH_ILP; ++u) k[u] = e0 + u * THREADS;
-			unsigned lo[STDP_BATCH_ILP];
+			for(int u = 0; u < SYN_ILP; ++u) k[u] = e0 + u * THREADS;
+			unsigned lo[SYN_ILP];
 			last_not_above_k<THREADS>(s_pos, nfired, k, lo);
 	#pragma unroll
-			for(int u = 0; u < STDP_BATCH_ILP; ++u) {
+			for(int u = 0; u < SYN_ILP; ++u) {
 				const uint32_t e = k[u];
 				k[u] = 0;
 				sd[u] = 0;
@@ -561,9 +563,9 @@ __device__ inline void stdp_batch_block(const nemo_propagate_args& a, unsigned s
 					acc[u] = st.rcm_acc[k[u]];
 				}
 			}
-			uint64_t hc[STDP_BATCH_ILP], hs[STDP_BATCH_ILP];
+			uint64_t hc[SYN_ILP], hs[SYN_ILP];
 	#pragma unroll
-			for(int u = 0; u < STDP_BATCH_ILP; ++u) {
+			for(int u = 0; u < SYN_ILP; ++u) {
 				hc[u] = 0;
 				hs[u] = 0;
 				if(e0 + u * THREADS < total) {
@@ -571,7 +573,7 @@ __device__ inline void stdp_batch_block(const nemo_propagate_args& a, unsigned s
 				}
 			}
 	#pragma unroll
-			for(int u = 0; u < STDP_BATCH_ILP; ++u) {
+			for(int u = 0; u < SYN_ILP; ++u) {
 				if(e0 + u * THREADS < total) {
 					const unsigned d = (sd[u] >> 26) + 1;
 					int32_t dw = 0;
