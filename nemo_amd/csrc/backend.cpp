@@ -1619,7 +1619,8 @@ void Simulation::launchDelivery(const nemo_propagate_args& set, uint64_t offset,
  * snapshot ends the run (and every 64 steps).  With ev (four events per
  * step), the kernels record the start and end of the step's delivery on this
  * stream and of its neuron updates. */
-void Simulation::enqueueSteps(uint64_t first, unsigned k, bool ext, bool fstim, hipEvent_t* ev, bool forkPending)
+void Simulation::enqueueSteps(uint64_t first, unsigned k, bool ext, bool fstim, hipEvent_t* ev, bool forkPending,
+		bool advanceBase)
 {
 	const bool split = m_late.deliver_blocks && m_early.deliver_blocks;
 	const bool overlap = split && m_shape.overlap == 1 && k > 1;
@@ -1771,7 +1772,10 @@ void Simulation::enqueueSteps(uint64_t first, unsigned k, bool ext, bool fstim, 
 			/* snapshot after this step: the steps since the last one are
 			 * pending for the next batch */
 			if(forked) HIP_CHECK(hipStreamWaitEvent(m_stream, m_stdpDone, 0));
-			HIP_CHECK(nemo_launch_stdp_snapshot(m_stream, &m_propagate, first + i));
+			/* the last step's snapshot is the last work of the steps: with
+			 * advanceBase it moves the step base past them */
+			const uint64_t advance = advanceBase && i + 1 == k ? first + k : 0;
+			HIP_CHECK(nemo_launch_stdp_snapshot(m_stream, &m_propagate, first + i, advance));
 			if(i + 1 < k) forkBatch();
 		}
 	}
@@ -2010,6 +2014,13 @@ const std::vector<unsigned>& Simulation::step(const unsigned* fstim, size_t nf,
 	return m_firedOut;
 }
 
+/* With batched STDP every graph of steps ends with a snapshot, which then
+ * moves the step base on (the other placements set it from the host). */
+bool Simulation::graphsAdvanceBase() const
+{
+	return m_shape.stdpMode == 4 && m_stdpBlocks;
+}
+
 hipGraphExec_t Simulation::graphFor(unsigned k, bool forkPending)
 {
 	const unsigned key = 2 * k + (forkPending ? 1u : 0u);
@@ -2020,7 +2031,7 @@ hipGraphExec_t Simulation::graphFor(unsigned k, bool forkPending)
 	const bool debug = m_debug;
 	m_debug = false;
 	try {
-		enqueueSteps(0, k, false, false, nullptr, forkPending);
+		enqueueSteps(0, k, false, false, nullptr, forkPending, graphsAdvanceBase());
 	} catch(...) {
 		m_debug = debug;
 		m_stdpPending = false;
@@ -2175,10 +2186,12 @@ void Simulation::run(unsigned nsteps, unsigned stdpPeriod, float reward, bool re
 			const uint64_t t0 = m_hostStep;
 			hipGraphExec_t g = graphFor(k, batchPending);
 			batchPending = true; /* each chunk leaves its steps pending */
-			setBase(t0);
+			const bool advances = graphsAdvanceBase();
+			if(!advances || m_base != t0) setBase(t0);
 			/* the ring slots this chunk overwrites were copied out */
 			if(inFlight[half]) HIP_CHECK(hipStreamWaitEvent(m_stream, copied[half], 0));
 			HIP_CHECK(hipGraphLaunch(g, m_stream));
+			if(advances) m_base = t0 + k;
 			advanceSteps(k);
 			if(readRaster) {
 				uint32_t* listDev = devList ? m_listDev.ptr + size_t(half) * CHUNK * m_npad : nullptr;
@@ -2392,8 +2405,10 @@ double Simulation::timeSteps(unsigned nsteps)
 	while(done < nsteps) {
 		const unsigned k = std::min(64u, nsteps - done);
 		hipGraphExec_t g = graphFor(k, true);
-		setBase(m_hostStep);
+		const bool advances = graphsAdvanceBase();
+		if(!advances || m_base != m_hostStep) setBase(m_hostStep);
 		HIP_CHECK(hipGraphLaunch(g, m_stream));
+		if(advances) m_base = m_hostStep + k;
 		advanceSteps(k);
 		done += k;
 	}

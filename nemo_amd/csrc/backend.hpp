@@ -437,7 +437,9 @@ private:
 	void copyScalar(uint64_t* dst, uint64_t value);
 	void setBase(uint64_t base);
 	void enqueueSteps(uint64_t first, unsigned k, bool ext, bool fstim, hipEvent_t* ev = nullptr,
-			bool forkPending = true);
+			bool forkPending = true, bool advanceBase = false);
+	/* whether the graphs of steps move the device's step base on themselves */
+	bool graphsAdvanceBase() const;
 	void launchDelivery(const nemo_propagate_args& set, uint64_t offset, hipStream_t st, bool withStdp,
 			hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 	void flushStdp();

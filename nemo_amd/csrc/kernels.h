@@ -54,8 +54,9 @@ hipError_t nemo_launch_stdp_batch(hipStream_t, const nemo_propagate_args*, unsig
 hipError_t nemo_launch_stdp_flush(hipStream_t, const nemo_propagate_args*, unsigned stdp_blocks, uint64_t offset);
 /* A snapshot of the history after step base + offset (at most 64 steps after
  * the last one); the steps since the last snapshot become the pending
- * accumulation.  The pending batch must have run. */
-hipError_t nemo_launch_stdp_snapshot(hipStream_t, const nemo_propagate_args*, uint64_t offset);
+ * accumulation.  The pending batch must have run.  Then the clock's step
+ * base moves on by advance (0: it stays). */
+hipError_t nemo_launch_stdp_snapshot(hipStream_t, const nemo_propagate_args*, uint64_t offset, uint64_t advance);
 /* dynamic LDS an update launch carrying propagate blocks needs (delivery
  * blocks when delivery is set, else STDP blocks only; the launch's update
  * blocks share the buffer) */

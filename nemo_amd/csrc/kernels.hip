@@ -114,14 +114,18 @@ stdp_snapshot_kernel(nemo_propagate_args a, uint64_t offset)
 	}
 }
 
-/* ... after which the steps since the last snapshot are pending */
-__global__ void stdp_snapshot_done_kernel(nemo_hip_clock* clock, uint64_t offset)
+/* ... after which the steps since the last snapshot are pending; and the
+ * step base moves on by advance (the last node of a graph of steps, so that
+ * the next graph needs no copy of its base from the host) */
+__global__ void stdp_snapshot_done_kernel(nemo_hip_clock* clock, uint64_t offset, uint64_t advance)
 {
-	const uint64_t s = clock->base_step + offset;
+	const uint64_t base = clock->base_step;
+	const uint64_t s = base + offset;
 	clock->stdp_first = clock->stdp_last + 1;
 	clock->stdp_count = s - clock->stdp_last;
 	clock->stdp_last = s;
 	clock->stdp_snaps += 1;
+	if(advance) clock->base_step = base + advance;
 }
 
 /* ... and after a flush of the pending batch: nothing is pending, and the
@@ -333,7 +337,7 @@ hipError_t nemo_launch_stdp_flush(hipStream_t st, const nemo_propagate_args* a, 
 	return hipGetLastError();
 }
 
-hipError_t nemo_launch_stdp_snapshot(hipStream_t st, const nemo_propagate_args* a, uint64_t offset)
+hipError_t nemo_launch_stdp_snapshot(hipStream_t st, const nemo_propagate_args* a, uint64_t offset, uint64_t advance)
 {
 	if(!a->stdp_snap || !a->hist) return hipErrorInvalidValue;
 	const size_t n = a->fired_words * 64;
@@ -342,7 +346,7 @@ hipError_t nemo_launch_stdp_snapshot(hipStream_t st, const nemo_propagate_args* 
 	hipError_t e = hipGetLastError();
 	if(e != hipSuccess) return e;
 	hipLaunchKernelGGL(stdp_snapshot_done_kernel, dim3(1), dim3(1), 0, st, const_cast<nemo_hip_clock*>(a->clock),
-			offset);
+			offset, advance);
 	return hipGetLastError();
 }
 
