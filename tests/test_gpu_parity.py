@@ -246,6 +246,47 @@ def test_run_matches_step():
     assert np.array_equal(a.fixed_weights(), b.fixed_weights())
 
 
+def test_interleaved_runs_steps_and_apply_match_oracle():
+    """Runs of several graphs, single steps, applyStdp between them, a raster-
+    less run and a state change: the step base that consecutive graphs move
+    on themselves, the host's copy of it (used by the flush after each run)
+    and the STDP batches must stay right whatever comes between the runs.
+    Raster, weights and accumulators bit-exact with the oracle doing the same."""
+    spec = gen.random_izhikevich(3000, 120, dmax=12, seed=21, stdp=True)
+    ref, hip = both(spec, stdp_fn())
+    nsyn = spec.synapse_count
+
+    def same():
+        assert np.array_equal(ref.fixed_weights(nsyn), hip.fixed_weights())
+        assert np.array_equal(ref.stdp_accumulators(nsyn), hip.stdp_accumulators())
+
+    def run(n, period=0, raster=True):
+        cr, fr, _ = ref.run(n, stdp_period=period, reward=1.0, raster=raster)
+        ch, fh, _ = hip.run(n, stdp_period=period, reward=1.0, raster=raster)
+        if raster:
+            assert np.array_equal(cr, ch) and np.array_equal(fr, fh)
+
+    def steps(n):
+        for _ in range(n):
+            assert np.array_equal(ref.step(), hip.step())
+
+    run(150)
+    steps(3)
+    run(70, period=35)
+    same()
+    for s_ in (ref, hip):
+        s_.apply_stdp(0.5)
+    run(1)
+    run(130, raster=False)
+    steps(2)
+    for s_ in (ref, hip):
+        s_.set_neuron_state(5, 1, -70.0)  # v of neuron 5
+    run(64)
+    run(65)
+    same()
+    steps(1)
+
+
 def test_run_readout_longer_than_expected_matches_step():
     """A run's fired list is compacted on the device and copied with the
     counts up to an expected length; a network that fires far more (4000
