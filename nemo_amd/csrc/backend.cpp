@@ -2525,6 +2525,22 @@ LaunchShape Simulation::autotune(unsigned stepsPerTrial)
 			consider(c);
 		}
 	}
+	/* the descent keeps whatever won one trial; a shape other than the
+	 * starting one is timed against it again, twice each in turn, and kept
+	 * only if it is faster in total (single trials vary by a few percent) */
+	auto same = [](const LaunchShape& x, const LaunchShape& y) {
+		return x.deliverBlocks == y.deliverBlocks && x.lateBlocks == y.lateBlocks && x.binCapacity == y.binCapacity
+			&& x.stdpMode == y.stdpMode && x.stdpWords == y.stdpWords && x.overlap == y.overlap;
+	};
+	if(!same(best, start)) {
+		double tb = 0, ts = 0;
+		for(int r = 0; r < 2; ++r) {
+			tb += trial(best);
+			ts += trial(start);
+		}
+		if(ts <= tb) best = start;
+		tBest = std::min(tb, ts) / 2;
+	}
 	restoreSnapshot(snap);
 	applyShape(best);
 	if(m_logging) {

@@ -107,15 +107,19 @@ struct TileInput {
 	long long I[NEMO_TILE];
 };
 
-/* Gather batches: GATHER_ILP events per thread in flight; an event's column
+/* Gather batches: GATHER_ILP events per thread in flight (6 when the tile's
+ * block has NEMO_COLAUNCH_BLOCK threads, all of which gather: 3072 events per
+ * batch, and 0.25 us per step faster than 8 at 100k x 1000); an event's column
  * is found from buckets of 2^GATHER_SHIFT events, marked per column when the
  * tile has at most GATHER_MARKED events. */
 constexpr int GATHER_ILP = 8;
+constexpr int GATHER_ILP_WIDE = 6;
 constexpr unsigned GATHER_SHIFT = 3;
 constexpr unsigned GATHER_MARKED = 4096;
-/* events of the largest batch: GATHER_ILP per thread of a carrying block */
-constexpr unsigned GATHER_BUCKETS_MAX_EVENTS = GATHER_MARKED > GATHER_ILP * NEMO_COLAUNCH_BLOCK ? GATHER_MARKED
-	: GATHER_ILP * NEMO_COLAUNCH_BLOCK;
+/* events of the largest batch or marked gather */
+constexpr unsigned GATHER_BUCKETS_MAX_EVENTS = GATHER_MARKED > GATHER_ILP_WIDE * NEMO_COLAUNCH_BLOCK
+	? (GATHER_MARKED > GATHER_ILP * NEMO_TILE ? GATHER_MARKED : GATHER_ILP * NEMO_TILE)
+	: GATHER_ILP_WIDE * NEMO_COLAUNCH_BLOCK;
 constexpr unsigned GATHER_BUCKETS = GATHER_BUCKETS_MAX_EVENTS / (1u << GATHER_SHIFT) + 2;
 
 /* The LDS an update block uses: its tile's input sums, the gather's column
@@ -173,7 +177,7 @@ __device__ inline uint2 push_row(const nemo_hip_step_buffers& b, unsigned ng)
  * neuron, which alone read the table row, the accumulators and, through
  * prefetch, their neuron's state) and, in a block of NEMO_COLAUNCH_BLOCK
  * threads, helpers that only load and sum events, so that a tile of up to
- * GATHER_ILP * THREADS events takes one round of loads.  Returns false for
+ * 3072 events (GATHER_ILP_WIDE per thread) takes one round of loads.  Returns false for
  * the helpers, which take no further part in the step. */
 template<unsigned THREADS = NEMO_TILE, typename Prefetch = NoPrefetch>
 __device__ inline bool gather_input(const nemo_hip_step_buffers& b, const nemo_hip_neuron_group& g, UpdateLds& L,
@@ -236,12 +240,13 @@ __device__ inline bool gather_input(const nemo_hip_step_buffers& b, const nemo_h
 		if(i < B) s_base[i] = uint32_t(col[k] * cap + (ent[k] & 0xffffu));
 		mine += ent[k] >> 16;
 	}
-	/* batches of GATHER_ILP events per thread, GATHER_ILP independent loads in
-	 * flight per thread before the LDS atomics.  An event's column lies
+	/* batches of ILP events per thread, ILP independent loads in flight per
+	 * thread before the LDS atomics.  An event's column lies
 	 * between the columns of the bucket bounds around it (every 8th event),
 	 * which each column marks when the tile has at most MARKED events, or
 	 * which are searched per batch; it is refined between them. */
-	constexpr unsigned BATCH = GATHER_ILP * THREADS;
+	constexpr int ILP = THREADS > NEMO_TILE ? GATHER_ILP_WIDE : GATHER_ILP;
+	constexpr unsigned BATCH = ILP * THREADS;
 	constexpr unsigned SHIFT = GATHER_SHIFT;
 	constexpr unsigned MARKED = GATHER_MARKED;
 	static_assert(BATCH <= GATHER_BUCKETS_MAX_EVENTS, "the bucket array holds a batch");
@@ -274,10 +279,10 @@ __device__ inline bool gather_input(const nemo_hip_step_buffers& b, const nemo_h
 			}
 			__syncthreads();
 		}
-		uint32_t jj[GATHER_ILP];
-		unsigned lo[GATHER_ILP], hi[GATHER_ILP];
+		uint32_t jj[ILP];
+		unsigned lo[ILP], hi[ILP];
 #pragma unroll
-		for(int u = 0; u < GATHER_ILP; ++u) {
+		for(int u = 0; u < ILP; ++u) {
 			/* buckets past the last event may be unset: only the events of
 			 * the tile read theirs */
 			jj[u] = base + u * THREADS + tid;
@@ -289,7 +294,7 @@ __device__ inline bool gather_input(const nemo_hip_step_buffers& b, const nemo_h
 		while(open) {
 			open = false;
 #pragma unroll
-			for(int u = 0; u < GATHER_ILP; ++u) {
+			for(int u = 0; u < ILP; ++u) {
 				if(lo[u] < hi[u]) {
 					const unsigned mid = lo[u] + ((hi[u] - lo[u] + 1) >> 1);
 					if(s_pre[mid] <= jj[u]) lo[u] = mid; else hi[u] = mid - 1;
@@ -297,14 +302,14 @@ __device__ inline bool gather_input(const nemo_hip_step_buffers& b, const nemo_h
 				}
 			}
 		}
-		int2 ev[GATHER_ILP];
+		int2 ev[ILP];
 #pragma unroll
-		for(int u = 0; u < GATHER_ILP; ++u) {
+		for(int u = 0; u < ILP; ++u) {
 			ev[u] = make_int2(0, 0);
 			if(jj[u] < total) ev[u] = bins[s_base[lo[u]] + (jj[u] - s_pre[lo[u]])];
 		}
 #pragma unroll
-		for(int u = 0; u < GATHER_ILP; ++u) {
+		for(int u = 0; u < ILP; ++u) {
 			if(jj[u] < total) {
 				long long* dst = ev[u].y >= 0 ? &in.E[ev[u].x] : &in.I[ev[u].x];
 				atomicAdd(reinterpret_cast<unsigned long long*>(dst), (unsigned long long)(long long)ev[u].y);
