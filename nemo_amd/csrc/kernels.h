@@ -37,8 +37,8 @@ extern "C" {
 #define NEMO_STDP_BATCH_BLOCK 256
 #define NEMO_STDP_BATCH_WAVES 6
 /* Default bitmap words per batch block: 261 blocks for 100k neurons, about
- * one per CU (100k x 1000 with STDP, us per step: 8 words 22.8, 6 words
- * 22.5, 4 words 23.2) */
+ * one per CU (100k x 1000 with STDP, us per step in one box: 4 words 23.2,
+ * 5 words 22.4, 6 words 22.1, 7 words 22.2, 8 words 22.6) */
 #define NEMO_STDP_BATCH_WORDS 6
 
 
